@@ -1,0 +1,152 @@
+/*
+ * lnt.h -- C-ABI of the MI355X (gfx950) sparse multi-index-array library.
+ *
+ * The drop-in boundary for LibNT/LibMIA's sparse hot path: relinearisation,
+ * radix sort and change_sort_order of (int64 linear index, double value)
+ * pairs, sparse merge (+, -, Hadamard) and the sparse lattice product.  Every
+ * entry point replaces one reference routine, cited below (paths relative to
+ * the reference's src/).  The reference's own FFI for this path is the set of
+ * MATLAB mex functions in src/MATLAB/mex/; the in-process C++ call sites are in
+ * src/multi_index_array/.
+ *
+ * Conventions
+ *   - Keys are int64 linear indices, always >= 0; values are double.  A key is
+ *     computed in a mode order "seq" (0-based mode numbers, seq[0] = fastest
+ *     varying mode): key = sum_t sub[seq[t]] * prod_{u<t} dims[seq[u]].
+ *   - dims, seq and other small descriptors are host pointers.  Key/value
+ *     arrays are host or device pointers according to the `mem` argument
+ *     (LNT_MEM_HOST / LNT_MEM_DEVICE).  Device pointers must be on the
+ *     context's device.
+ *   - Every function returns an lnt_status.  On error nothing is written to the
+ *     caller's arrays and lnt_last_error() describes the failure.  No function
+ *     throws or calls exit/longjmp.  The error string is per thread.
+ *   - Work is issued on the context's stream.  Functions that hand a count back
+ *     to the host (merge, lattice product) synchronise that stream before
+ *     returning; the in-place sorts do not (synchronise with
+ *     lnt_context_synchronize or the stream when results are needed on the
+ *     host).  Use one context per host thread.
+ */
+#ifndef LNT_H
+#define LNT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LNT_ABI_VERSION 1
+#define LNT_MAX_ORDER 16
+
+typedef enum lnt_status {
+  LNT_OK = 0,
+  LNT_ERR_INVALID_ARGUMENT = 1, /* malformed sizes, seq not a permutation, ... */
+  LNT_ERR_OUT_OF_MEMORY = 2,
+  LNT_ERR_DEVICE = 3,           /* a HIP runtime call failed */
+  LNT_ERR_NOT_SORTED = 4,       /* an input declared sorted is not */
+  LNT_ERR_KEY_RANGE = 5,        /* a key is negative or >= the dimensionality */
+  LNT_ERR_TOO_LARGE = 6,        /* size beyond the documented limits */
+  LNT_ERR_DIMENSION_MISMATCH = 7,
+  LNT_ERR_NO_DEVICE = 8
+} lnt_status;
+
+typedef enum lnt_memory { LNT_MEM_HOST = 0, LNT_MEM_DEVICE = 1 } lnt_memory;
+
+typedef enum lnt_merge_op {
+  LNT_OP_ADD = 0,     /* std::plus  */
+  LNT_OP_SUB = 1,     /* std::minus */
+  LNT_OP_HADAMARD = 2 /* element-wise product over the common support */
+} lnt_merge_op;
+
+typedef struct lnt_context_s *lnt_context;
+
+/* ------------------------------------------------------------- context ---- */
+/* Creates a context on HIP device `device`, issuing work on `stream` (a
+ * hipStream_t; NULL = the device's null stream).  The context owns scratch
+ * memory, tuning parameters and the learned-dispatch weights.  No reference
+ * counterpart (the reference is single-threaded CPU code). */
+int lnt_context_create(lnt_context *ctx, int device, void *stream);
+int lnt_context_destroy(lnt_context ctx);
+int lnt_context_set_stream(lnt_context ctx, void *stream);
+int lnt_context_synchronize(lnt_context ctx);
+/* Releases cached scratch memory. */
+int lnt_context_trim(lnt_context ctx);
+/* Input validation level.  0: none.  1 (default): sorts check every key
+ * against the dimensionality inside their first read of the keys, before any
+ * output is written (one stream synchronisation).  2: additionally, merges
+ * check that both inputs are strictly ascending (one extra read of the keys). */
+int lnt_context_set_validation(lnt_context ctx, int level);
+const char *lnt_last_error(void);
+int lnt_abi_version(void);
+/* Kernel timing with HIP events recorded on the context's stream around each
+ * launch.  Turning profiling on or off clears the records.  lnt_context_profile
+ * sums, for one kernel ("histogram", "onesweep", "partition", "merge",
+ * "relin"), the launches recorded, their total duration and the algorithmic
+ * bytes they move (sort pass: 32 B per pair; histogram: 8 B per key; merge:
+ * 16 B per input and per output element; relinearisation: 16 B per key).  No
+ * reference counterpart (the reference has only ad-hoc std::chrono prints,
+ * multi_index_array/SparseLatticeBase.h:1885,1981-1982). */
+int lnt_context_set_profiling(lnt_context ctx, int on);
+int lnt_context_profile(lnt_context ctx, const char *kernel, int64_t *launches,
+                        double *total_ms, double *bytes);
+
+/* -------------------------------------------------- relinearisation (a2) -- */
+/* In place: every key is recomputed from mode order cur_seq to new_seq.  Does
+ * not sort.  Replaces SparseMIABase::change_linIdx_sequence
+ * (multi_index_array/SparseMIABase.h:768-814, per-key
+ * IndexUtil.h:567-589 reShuffleLinearIndex) and SparseChangeLinIdxMex
+ * (MATLAB/mex/sparse_change_linIdx.cpp:17-71; that entry takes 1-based uint8
+ * sequences, this one 0-based int32). */
+int lnt_change_linidx(lnt_context ctx, int64_t *keys, int64_t n, int order,
+                      const int64_t *dims, const int32_t *cur_seq,
+                      const int32_t *new_seq, int mem);
+
+/* ------------------------------------------------------------ sort (a3) -- */
+/* In place: sorts keys ascending and permutes vals alongside.  Keys must be
+ * unique and in [0, max_size).  Replaces SparseMIABase::sort()
+ * (multi_index_array/SparseMIABase.h:701-734 ->
+ * LibMIARadix.h:444-484 RadixSortInPlace_PowerOf2Radix_Unsigned) and
+ * SparseSortMex (MATLAB/mex/sparse_sort.cpp:14-68).  n < 2^32. */
+int lnt_sort_pairs(lnt_context ctx, int64_t *keys, double *vals, int64_t n,
+                   uint64_t max_size, int mem);
+
+/* ------------------------------------------------ change_sort_order (a4) -- */
+/* In place: relinearises every key from cur_seq to new_seq and sorts (keys,
+ * vals) ascending in the new order.  `is_sorted` states whether the input is
+ * sorted in cur_seq (a hint; the result does not depend on it).  Replaces
+ * SparseMIABase::sort(seq) (multi_index_array/SparseMIABase.h:590-645, which
+ * runs change_linIdx_sequence + RadixShuffle::permute, LibMIARadix.h:811-900)
+ * and SparsePermuteMex (MATLAB/mex/sparse_permute.cpp:15-116, which expects the
+ * keys already relinearised).  n < 2^32. */
+int lnt_change_sort_order(lnt_context ctx, int64_t *keys, double *vals,
+                          int64_t n, int order, const int64_t *dims,
+                          const int32_t *cur_seq, const int32_t *new_seq,
+                          int is_sorted, int mem);
+
+/* ------------------------------------------------------ merge (a5, a7) -- */
+/* A and B sorted ascending (same mode order, unique keys).  op ADD/SUB: sorted
+ * union, equal keys -> a op b, A only -> a, B only -> 0 op b (exact IEEE).  op
+ * HADAMARD: sorted intersection with a*b.  C is sorted.  `capacity` must be at
+ * least na+nb (ADD/SUB) or min(na,nb) (HADAMARD); *nc receives nnz(C).
+ * Replaces outside_merge_sparse_storage_containers
+ * (multi_index_array/FunctionUtil.h:373-419, reached from
+ * SparseMIABase::outside_scanMerge, SparseMIABase.h:1404-1455), the pure
+ * Hadamard case of perform_noLatticeMult (SparseMIABase.h:1717-1826) and
+ * SparseMergeMex (MATLAB/mex/sparse_merge.cpp:15-162).  na + nb < 2^32. */
+int lnt_merge_into(lnt_context ctx, const int64_t *a_keys,
+                   const double *a_vals, int64_t na, const int64_t *b_keys,
+                   const double *b_vals, int64_t nb, int op, int64_t *c_keys,
+                   double *c_vals, int64_t capacity, int64_t *nc, int mem);
+/* As lnt_merge_into, but the library allocates C (device memory for
+ * LNT_MEM_DEVICE, host memory otherwise); release with lnt_free. */
+int lnt_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
+              int64_t na, const int64_t *b_keys, const double *b_vals,
+              int64_t nb, int op, int64_t **c_keys, double **c_vals,
+              int64_t *nc, int mem);
+int lnt_free(lnt_context ctx, void *ptr, int mem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LNT_H */

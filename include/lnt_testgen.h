@@ -1,0 +1,31 @@
+/*
+ * lnt_testgen.h -- synthetic-input generators that run on the GPU.
+ *
+ * No reference counterpart: the reference draws its test data from
+ * boost::mt19937 seeded by time(0) (multi_index_array/LibMIAUtil.h:43-46).
+ * These counter-based generators are bit-identical to the CPU oracle's
+ * (oracle/lnt_oracle.c lnto_permute_index / lnto_uniform_pm1), so a benchmark
+ * can build inputs of hundreds of millions of entries directly in HBM and a
+ * test can rebuild the same inputs on the CPU.
+ */
+#ifndef LNT_TESTGEN_H
+#define LNT_TESTGEN_H
+
+#include "lnt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* keys[t] = permute_index(offset + t, domain, seed): distinct values in
+ * [0, domain) for distinct t < domain.  Device pointer. */
+int lnt_testgen_distinct_keys(lnt_context ctx, int64_t *keys, int64_t n,
+                              uint64_t domain, uint64_t seed, uint64_t offset);
+/* vals[t] = uniform_pm1(seed, offset + t) in [-1, 1).  Device pointer. */
+int lnt_testgen_uniform(lnt_context ctx, double *vals, int64_t n, uint64_t seed,
+                        uint64_t offset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,253 @@
+"""libnt_amd -- MI355X (gfx950) sparse multi-index-array hot path.
+
+Python host binding of the C-ABI in ``include/lnt.h`` (``liblnt.so``, built by
+``libnt_amd/build.py``).  Arrays are torch tensors on the GPU (device memory,
+``LNT_MEM_DEVICE``) or numpy arrays (host memory, ``LNT_MEM_HOST``); torch is
+only the allocator and stream provider.  Every call runs the HIP kernels; if
+the shared library is missing, importing ``lib()`` raises -- there is no CPU
+fallback.
+
+``libnt_amd.sparse`` mirrors the reference's ``SparseMIA`` front end on top of
+this binding.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblnt.so")
+
+LNT_OK = 0
+STATUS_NAMES = {
+    0: "LNT_OK", 1: "LNT_ERR_INVALID_ARGUMENT", 2: "LNT_ERR_OUT_OF_MEMORY", 3: "LNT_ERR_DEVICE",
+    4: "LNT_ERR_NOT_SORTED", 5: "LNT_ERR_KEY_RANGE", 6: "LNT_ERR_TOO_LARGE",
+    7: "LNT_ERR_DIMENSION_MISMATCH", 8: "LNT_ERR_NO_DEVICE",
+}
+MEM_HOST, MEM_DEVICE = 0, 1
+OP_ADD, OP_SUB, OP_HADAMARD = 0, 1, 2
+
+_lib = None
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+_int = ctypes.c_int
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+SIGNATURES = {
+    "lnt_abi_version": [],
+    "lnt_last_error": [],
+    "lnt_context_create": [ctypes.POINTER(_vp), _int, _vp],
+    "lnt_context_destroy": [_vp],
+    "lnt_context_set_stream": [_vp, _vp],
+    "lnt_context_synchronize": [_vp],
+    "lnt_context_trim": [_vp],
+    "lnt_context_set_validation": [_vp, _int],
+    "lnt_context_set_profiling": [_vp, _int],
+    "lnt_context_profile": [_vp, ctypes.c_char_p, _i64p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+    "lnt_change_linidx": [_vp, _vp, _i64, _int, _i64p, _i32p, _i32p, _int],
+    "lnt_sort_pairs": [_vp, _vp, _vp, _i64, _u64, _int],
+    "lnt_change_sort_order": [_vp, _vp, _vp, _i64, _int, _i64p, _i32p, _i32p, _int, _int],
+    "lnt_merge_into": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _vp, _i64, _i64p, _int],
+    "lnt_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
+    "lnt_free": [_vp, _vp, _int],
+    "lnt_testgen_distinct_keys": [_vp, _vp, _i64, _u64, _u64, _u64],
+    "lnt_testgen_uniform": [_vp, _vp, _i64, _u64, _u64],
+}
+_RESTYPES = {"lnt_last_error": ctypes.c_char_p}
+
+
+class LntError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+def lib():
+    """Loads liblnt.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libnt_amd/liblnt.so is not built; run `python -m libnt_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != LNT_OK:
+        raise LntError(status, lib().lnt_last_error().decode(errors="replace"))
+
+
+def _arr_i64(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a, a.ctypes.data_as(_i64p)
+
+
+def _arr_i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_i32p)
+
+
+def _ptr_mem(x):
+    """(pointer, mem kind, length) of a torch GPU tensor or a numpy array."""
+    if isinstance(x, np.ndarray):
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("numpy arrays must be contiguous")
+        return x.ctypes.data, MEM_HOST, x.shape[0]
+    import torch
+
+    if isinstance(x, torch.Tensor):
+        if not x.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return x.data_ptr(), (MEM_DEVICE if x.is_cuda else MEM_HOST), x.shape[0]
+    raise TypeError("expected a numpy array or a torch tensor, got %r" % type(x))
+
+
+class Context:
+    """Owns an ``lnt_context`` (device, stream, scratch memory)."""
+
+    def __init__(self, device: int = 0, stream=None, validation: int = 1):
+        self._h = _vp()
+        check(lib().lnt_context_create(ctypes.byref(self._h), device, stream))
+        self.device = device
+        if validation != 1:
+            self.set_validation(validation)
+
+    def close(self):
+        if self._h:
+            lib().lnt_context_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream):
+        check(lib().lnt_context_set_stream(self._h, stream))
+
+    def synchronize(self):
+        check(lib().lnt_context_synchronize(self._h))
+
+    def trim(self):
+        check(lib().lnt_context_trim(self._h))
+
+    def set_validation(self, level: int):
+        check(lib().lnt_context_set_validation(self._h, level))
+
+    def set_profiling(self, on: bool):
+        check(lib().lnt_context_set_profiling(self._h, int(bool(on))))
+
+    def profile(self, kernel: str):
+        """(launches, total_ms, algorithmic_bytes) recorded for one kernel."""
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        b = ctypes.c_double()
+        check(lib().lnt_context_profile(self._h, kernel.encode(), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
+        return n.value, ms.value, b.value
+
+    # ------------------------------------------------------------ hot path
+    def change_linidx(self, keys, dims, cur_seq, new_seq):
+        """In-place relinearisation (SparseMIABase.h:768-814)."""
+        kp, mem, n = _ptr_mem(keys)
+        d, dp = _arr_i64(dims)
+        c, cp = _arr_i32(cur_seq)
+        s, sp = _arr_i32(new_seq)
+        check(lib().lnt_change_linidx(self._h, kp, n, len(d), dp, cp, sp, mem))
+
+    def sort_pairs(self, keys, vals, max_size):
+        """In-place ascending sort of unique keys with values alongside."""
+        kp, mem, n = _ptr_mem(keys)
+        vp, mem2, n2 = _ptr_mem(vals)
+        if mem != mem2 or n != n2:
+            raise ValueError("keys and values must match in length and memory kind")
+        check(lib().lnt_sort_pairs(self._h, kp, vp, n, int(max_size), mem))
+
+    def change_sort_order(self, keys, vals, dims, cur_seq, new_seq, is_sorted=True):
+        """In-place relinearise + sort (SparseMIABase.h:590-645)."""
+        kp, mem, n = _ptr_mem(keys)
+        vp, mem2, n2 = _ptr_mem(vals)
+        if mem != mem2 or n != n2:
+            raise ValueError("keys and values must match in length and memory kind")
+        d, dp = _arr_i64(dims)
+        c, cp = _arr_i32(cur_seq)
+        s, sp = _arr_i32(new_seq)
+        check(lib().lnt_change_sort_order(self._h, kp, vp, n, len(d), dp, cp, sp, int(bool(is_sorted)), mem))
+
+    def merge_into(self, ak, av, bk, bv, op, ck, cv) -> int:
+        """Sorted union (op ADD/SUB) or intersection (HADAMARD) into ck/cv."""
+        akp, mem, na = _ptr_mem(ak)
+        avp, _, _ = _ptr_mem(av)
+        bkp, memb, nb = _ptr_mem(bk)
+        bvp, _, _ = _ptr_mem(bv)
+        ckp, memc, cap = _ptr_mem(ck)
+        cvp, _, capv = _ptr_mem(cv)
+        if not (mem == memb == memc):
+            raise ValueError("all arrays must live in the same memory kind")
+        nc = ctypes.c_int64(0)
+        check(lib().lnt_merge_into(self._h, akp, avp, na, bkp, bvp, nb, op, ckp, cvp,
+                                   min(cap, capv), ctypes.byref(nc), mem))
+        return nc.value
+
+    def merge(self, ak, av, bk, bv, op):
+        """Returns new (keys, values) of the merge, of the operands' kind."""
+        n_a, n_b = ak.shape[0], bk.shape[0]
+        cap = min(n_a, n_b) if op == OP_HADAMARD else n_a + n_b
+        if isinstance(ak, np.ndarray):
+            ck = np.empty(max(cap, 1), dtype=np.int64)
+            cv = np.empty(max(cap, 1), dtype=np.float64)
+        else:
+            import torch
+
+            ck = torch.empty(max(cap, 1), dtype=torch.int64, device=ak.device)
+            cv = torch.empty(max(cap, 1), dtype=torch.float64, device=ak.device)
+        n = self.merge_into(ak, av, bk, bv, op, ck, cv)
+        return ck[:n], cv[:n]
+
+    # ---------------------------------------------------- synthetic inputs
+    def testgen_distinct_keys(self, keys, domain, seed, offset=0):
+        kp, mem, n = _ptr_mem(keys)
+        if mem != MEM_DEVICE:
+            raise ValueError("testgen writes device memory")
+        check(lib().lnt_testgen_distinct_keys(self._h, kp, n, domain, seed, offset))
+
+    def testgen_uniform(self, vals, seed, offset=0):
+        vp, mem, n = _ptr_mem(vals)
+        if mem != MEM_DEVICE:
+            raise ValueError("testgen writes device memory")
+        check(lib().lnt_testgen_uniform(self._h, vp, n, seed, offset))
+
+
+def exported_symbols_declared_in_headers(include_dir=None):
+    """Names of the functions declared in include/*.h (used by the ABI test)."""
+    import glob
+    import re
+
+    include_dir = include_dir or os.path.join(os.path.dirname(HERE), "include")
+    names = set()
+    for h in glob.glob(os.path.join(include_dir, "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(lnt_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)

@@ -1,0 +1,486 @@
+// api.hip -- the extern "C" boundary of include/lnt.h: argument validation,
+// device/host staging, context and scratch management, error reporting.
+// Every compute path below runs the HIP kernels of sort.hip / merge.hip; there
+// is no CPU fallback.
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+
+#include "lnt_internal.h"
+
+namespace lnt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int status, const std::string &msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int ensure(DeviceBuffer &b, size_t bytes) {
+  if (b.bytes >= bytes) return LNT_OK;
+  if (b.ptr) {
+    hipError_t e = hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return fail(LNT_ERR_DEVICE, std::string("hipFree: ") + hipGetErrorString(e));
+  }
+  if (bytes == 0) return LNT_OK;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(LNT_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  b.ptr = p;
+  b.bytes = bytes;
+  return LNT_OK;
+}
+
+void release(DeviceBuffer &b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+int make_relin(int order, const int64_t *dims, const int32_t *cur_seq,
+               const int32_t *new_seq, Relin *out, uint64_t *dimensionality) {
+  if (order < 1 || order > LNT_MAX_ORDER)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "order must be in [1, " + std::to_string(LNT_MAX_ORDER) + "]");
+  if (!dims || !cur_seq || !new_seq) return fail(LNT_ERR_INVALID_ARGUMENT, "null dims or sequence");
+  unsigned __int128 total = 1;
+  for (int m = 0; m < order; ++m) {
+    if (dims[m] < 1) return fail(LNT_ERR_INVALID_ARGUMENT, "every dimension must be >= 1");
+    total *= (uint64_t)dims[m];
+    if (total > ((unsigned __int128)1 << 63))
+      return fail(LNT_ERR_TOO_LARGE, "dimensionality exceeds 2^63 (int64 linear indices)");
+  }
+  uint32_t seen_c = 0, seen_n = 0;
+  for (int t = 0; t < order; ++t) {
+    if (cur_seq[t] < 0 || cur_seq[t] >= order || new_seq[t] < 0 || new_seq[t] >= order)
+      return fail(LNT_ERR_INVALID_ARGUMENT, "sequence entry out of range");
+    seen_c |= 1u << cur_seq[t];
+    seen_n |= 1u << new_seq[t];
+  }
+  uint32_t all = order == 32 ? ~0u : ((1u << order) - 1);
+  if (seen_c != all || seen_n != all)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "sequence is not a permutation of the modes");
+  Relin r{};
+  r.order = order;
+  r.active = 0;
+  for (int t = 0; t < order; ++t) r.active |= cur_seq[t] != new_seq[t];
+  uint64_t stride_of_mode[LNT_MAX_ORDER];
+  uint64_t acc = 1;
+  for (int t = 0; t < order; ++t) {
+    stride_of_mode[new_seq[t]] = acc;
+    acc *= (uint64_t)dims[new_seq[t]];
+  }
+  acc = 1;
+  for (int t = 0; t < order; ++t) {
+    r.place[t] = make_divider(acc);
+    r.weight[t] = stride_of_mode[cur_seq[t]];
+    acc *= (uint64_t)dims[cur_seq[t]];
+  }
+  *out = r;
+  *dimensionality = (uint64_t)total;
+  return LNT_OK;
+}
+
+static const char *kProfNames[kProfKinds] = {"histogram", "onesweep", "partition", "merge", "relin"};
+
+static hipEvent_t take_event(lnt_context ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int prof_begin(lnt_context ctx, int kind, double bytes) {
+  if (!ctx->profiling) return -1;
+  ProfRecord r{kind, take_event(ctx), take_event(ctx), bytes};
+  if (!r.start || !r.stop || hipEventRecord(r.start, ctx->stream) != hipSuccess) return -1;
+  ctx->prof.push_back(r);
+  return (int)ctx->prof.size() - 1;
+}
+
+void prof_end(lnt_context ctx, int slot) {
+  if (slot < 0) return;
+  (void)hipEventRecord(ctx->prof[slot].stop, ctx->stream);
+}
+
+void prof_set_bytes(lnt_context ctx, int slot, double bytes) {
+  if (slot >= 0) ctx->prof[slot].bytes = bytes;
+}
+
+static void prof_clear(lnt_context ctx) {
+  for (auto &r : ctx->prof) {
+    ctx->event_pool.push_back(r.start);
+    ctx->event_pool.push_back(r.stop);
+  }
+  ctx->prof.clear();
+}
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = true;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+#define LNT_CTX(ctx)                                                       \
+  if (!(ctx)) return fail(LNT_ERR_INVALID_ARGUMENT, "null context");      \
+  DeviceGuard _guard((ctx)->device);                                       \
+  if (!_guard.ok) return fail(LNT_ERR_DEVICE, "hipSetDevice failed")
+
+int check_mem(int mem) {
+  if (mem != LNT_MEM_HOST && mem != LNT_MEM_DEVICE)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "mem must be LNT_MEM_HOST or LNT_MEM_DEVICE");
+  return LNT_OK;
+}
+
+// Per-call device staging for host-memory arguments.
+struct Staging {
+  DeviceBuffer b[6];
+  ~Staging() {
+    for (auto &x : b) release(x);
+  }
+};
+
+__global__ void check_sorted_kernel(const uint64_t *keys, uint32_t n, uint32_t *err) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    bad |= (int64_t)k < 0;
+    if (i + 1 < n) bad |= !(k < keys[i + 1]);
+  }
+  if (bad) atomicOr(err, 1u);
+}
+
+int check_sorted(lnt_context ctx, const uint64_t *a, uint32_t na, const uint64_t *b, uint32_t nb) {
+  LNT_TRY(ensure(ctx->small, kSmallWords * sizeof(uint32_t)));
+  uint32_t *err = (uint32_t *)ctx->small.ptr + kSmallFlags + 2;
+  LNT_TRY(launch_fill_u32(ctx, err, 0, 1));
+  if (na) check_sorted_kernel<<<std::min<uint32_t>((na + 255) / 256, 4096), 256, 0, ctx->stream>>>(a, na, err);
+  if (nb) check_sorted_kernel<<<std::min<uint32_t>((nb + 255) / 256, 4096), 256, 0, ctx->stream>>>(b, nb, err);
+  LNT_HIP(hipGetLastError());
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 2, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->pinned[2]) return fail(LNT_ERR_NOT_SORTED, "merge input is not strictly ascending");
+  return LNT_OK;
+}
+
+}  // namespace
+}  // namespace lnt
+
+using namespace lnt;
+
+extern "C" {
+
+int lnt_abi_version(void) { return LNT_ABI_VERSION; }
+
+const char *lnt_last_error(void) { return g_last_error.c_str(); }
+
+int lnt_context_create(lnt_context *out, int device, void *stream) {
+  if (!out) return fail(LNT_ERR_INVALID_ARGUMENT, "null output pointer");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return fail(LNT_ERR_NO_DEVICE, "no HIP device is visible");
+  }
+  if (device < 0 || device >= count) return fail(LNT_ERR_INVALID_ARGUMENT, "device index out of range");
+  DeviceGuard guard(device);
+  if (!guard.ok) return fail(LNT_ERR_DEVICE, "hipSetDevice failed");
+  lnt_context ctx = new (std::nothrow) lnt_context_s();
+  if (!ctx) return fail(LNT_ERR_OUT_OF_MEMORY, "context allocation failed");
+  ctx->device = device;
+  ctx->stream = (hipStream_t)stream;
+  if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    delete ctx;
+    return fail(LNT_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+  }
+  memset(ctx->pinned, 0, 64 * sizeof(uint32_t));
+  int rc = ensure(ctx->small, kSmallWords * sizeof(uint32_t));
+  if (rc != LNT_OK) {
+    (void)hipHostFree(ctx->pinned);
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return LNT_OK;
+}
+
+int lnt_context_destroy(lnt_context ctx) {
+  if (!ctx) return LNT_OK;
+  DeviceGuard guard(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
+                          &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
+                          &ctx->host_tmp_b, &ctx->host_tmp_c})
+    release(*b);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  prof_clear(ctx);
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  delete ctx;
+  return LNT_OK;
+}
+
+int lnt_context_set_profiling(lnt_context ctx, int on) {
+  LNT_CTX(ctx);
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  prof_clear(ctx);
+  ctx->profiling = on ? 1 : 0;
+  return LNT_OK;
+}
+
+int lnt_context_profile(lnt_context ctx, const char *kernel, int64_t *launches,
+                        double *total_ms, double *bytes) {
+  LNT_CTX(ctx);
+  if (!kernel || !launches || !total_ms || !bytes) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  int kind = -1;
+  for (int k = 0; k < kProfKinds; ++k)
+    if (strcmp(kernel, kProfNames[k]) == 0) kind = k;
+  if (kind < 0) return fail(LNT_ERR_INVALID_ARGUMENT, std::string("unknown kernel name ") + kernel);
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  int64_t n = 0;
+  double ms = 0, b = 0;
+  for (auto &r : ctx->prof) {
+    if (r.kind != kind) continue;
+    float t = 0;
+    LNT_HIP(hipEventElapsedTime(&t, r.start, r.stop));
+    ++n;
+    ms += t;
+    b += r.bytes;
+  }
+  *launches = n;
+  *total_ms = ms;
+  *bytes = b;
+  return LNT_OK;
+}
+
+int lnt_context_set_stream(lnt_context ctx, void *stream) {
+  if (!ctx) return fail(LNT_ERR_INVALID_ARGUMENT, "null context");
+  ctx->stream = (hipStream_t)stream;
+  return LNT_OK;
+}
+
+int lnt_context_synchronize(lnt_context ctx) {
+  LNT_CTX(ctx);
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  return LNT_OK;
+}
+
+int lnt_context_trim(lnt_context ctx) {
+  LNT_CTX(ctx);
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
+                          &ctx->aux, &ctx->host_tmp_a, &ctx->host_tmp_b, &ctx->host_tmp_c})
+    release(*b);
+  return LNT_OK;
+}
+
+int lnt_context_set_validation(lnt_context ctx, int level) {
+  if (!ctx) return fail(LNT_ERR_INVALID_ARGUMENT, "null context");
+  if (level < 0 || level > 2) return fail(LNT_ERR_INVALID_ARGUMENT, "validation level must be 0, 1 or 2");
+  ctx->validate = level;
+  return LNT_OK;
+}
+
+int lnt_change_linidx(lnt_context ctx, int64_t *keys, int64_t n, int order,
+                      const int64_t *dims, const int32_t *cur_seq,
+                      const int32_t *new_seq, int mem) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  if (n < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  if (n >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "n must be < 2^32");
+  Relin r;
+  uint64_t dimensionality;
+  LNT_TRY(make_relin(order, dims, cur_seq, new_seq, &r, &dimensionality));
+  if (n == 0 || !r.active) return LNT_OK;
+  if (!keys) return fail(LNT_ERR_INVALID_ARGUMENT, "null keys");
+  if (mem == LNT_MEM_DEVICE) return relin_device(ctx, (uint64_t *)keys, (uint32_t)n, r);
+  Staging st;
+  LNT_TRY(ensure(st.b[0], (size_t)n * 8));
+  LNT_HIP(hipMemcpyAsync(st.b[0].ptr, keys, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+  LNT_TRY(relin_device(ctx, (uint64_t *)st.b[0].ptr, (uint32_t)n, r));
+  LNT_HIP(hipMemcpyAsync(keys, st.b[0].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  return LNT_OK;
+}
+
+static int sort_common(lnt_context ctx, int64_t *keys, double *vals, int64_t n,
+                       uint64_t bound, const Relin *relin, int mem) {
+  if (n < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  if (n >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "n must be < 2^32");
+  if (n == 0) return LNT_OK;
+  if (!keys || !vals) return fail(LNT_ERR_INVALID_ARGUMENT, "null keys or values");
+  if (mem == LNT_MEM_DEVICE)
+    return sort_device(ctx, (uint64_t *)keys, vals, (uint32_t)n, bound, relin);
+  Staging st;
+  LNT_TRY(ensure(st.b[0], (size_t)n * 8));
+  LNT_TRY(ensure(st.b[1], (size_t)n * 8));
+  LNT_HIP(hipMemcpyAsync(st.b[0].ptr, keys, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+  LNT_HIP(hipMemcpyAsync(st.b[1].ptr, vals, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+  int rc = sort_device(ctx, (uint64_t *)st.b[0].ptr, (double *)st.b[1].ptr, (uint32_t)n, bound, relin);
+  if (rc != LNT_OK) return rc;
+  LNT_HIP(hipMemcpyAsync(keys, st.b[0].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipMemcpyAsync(vals, st.b[1].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  return LNT_OK;
+}
+
+int lnt_sort_pairs(lnt_context ctx, int64_t *keys, double *vals, int64_t n,
+                   uint64_t max_size, int mem) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  if (max_size == 0 || max_size > (uint64_t(1) << 63))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "max_size must be in [1, 2^63]");
+  return sort_common(ctx, keys, vals, n, max_size, nullptr, mem);
+}
+
+int lnt_change_sort_order(lnt_context ctx, int64_t *keys, double *vals,
+                          int64_t n, int order, const int64_t *dims,
+                          const int32_t *cur_seq, const int32_t *new_seq,
+                          int is_sorted, int mem) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  Relin r;
+  uint64_t dimensionality;
+  LNT_TRY(make_relin(order, dims, cur_seq, new_seq, &r, &dimensionality));
+  // SparseMIABase.h:590-645: a sorted tensor whose order does not change is
+  // left alone; everything else is relinearised and sorted.
+  if (is_sorted && !r.active) return LNT_OK;
+  return sort_common(ctx, keys, vals, n, dimensionality, &r, mem);
+}
+
+int lnt_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
+                   int64_t na, const int64_t *b_keys, const double *b_vals,
+                   int64_t nb, int op, int64_t *c_keys, double *c_vals,
+                   int64_t capacity, int64_t *nc, int mem) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  if (!nc) return fail(LNT_ERR_INVALID_ARGUMENT, "null nc");
+  if (op != LNT_OP_ADD && op != LNT_OP_SUB && op != LNT_OP_HADAMARD)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "op must be LNT_OP_ADD, LNT_OP_SUB or LNT_OP_HADAMARD");
+  if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  if (na + nb >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "na + nb must be < 2^32");
+  int64_t need = op == LNT_OP_HADAMARD ? std::min(na, nb) : na + nb;
+  if (capacity < need)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) +
+                                              " is below the bound " + std::to_string(need));
+  if ((na && (!a_keys || !a_vals)) || (nb && (!b_keys || !b_vals)) || (need && (!c_keys || !c_vals)))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "null array");
+  uint32_t count = 0;
+  if (mem == LNT_MEM_DEVICE) {
+    if (ctx->validate >= 2)
+      LNT_TRY(check_sorted(ctx, (const uint64_t *)a_keys, (uint32_t)na, (const uint64_t *)b_keys, (uint32_t)nb));
+    LNT_TRY(merge_device(ctx, (const uint64_t *)a_keys, a_vals, (uint32_t)na, (const uint64_t *)b_keys,
+                         b_vals, (uint32_t)nb, op, (uint64_t *)c_keys, c_vals, (uint64_t)capacity, &count));
+    *nc = count;
+    return LNT_OK;
+  }
+  Staging st;
+  size_t sa = (size_t)na * 8, sb = (size_t)nb * 8, sc = (size_t)need * 8;
+  LNT_TRY(ensure(st.b[0], sa ? sa : 8));
+  LNT_TRY(ensure(st.b[1], sa ? sa : 8));
+  LNT_TRY(ensure(st.b[2], sb ? sb : 8));
+  LNT_TRY(ensure(st.b[3], sb ? sb : 8));
+  LNT_TRY(ensure(st.b[4], sc ? sc : 8));
+  LNT_TRY(ensure(st.b[5], sc ? sc : 8));
+  if (sa) {
+    LNT_HIP(hipMemcpyAsync(st.b[0].ptr, a_keys, sa, hipMemcpyHostToDevice, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(st.b[1].ptr, a_vals, sa, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (sb) {
+    LNT_HIP(hipMemcpyAsync(st.b[2].ptr, b_keys, sb, hipMemcpyHostToDevice, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(st.b[3].ptr, b_vals, sb, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (ctx->validate >= 2)
+    LNT_TRY(check_sorted(ctx, (const uint64_t *)st.b[0].ptr, (uint32_t)na, (const uint64_t *)st.b[2].ptr, (uint32_t)nb));
+  LNT_TRY(merge_device(ctx, (const uint64_t *)st.b[0].ptr, (const double *)st.b[1].ptr, (uint32_t)na,
+                       (const uint64_t *)st.b[2].ptr, (const double *)st.b[3].ptr, (uint32_t)nb, op,
+                       (uint64_t *)st.b[4].ptr, (double *)st.b[5].ptr, (uint64_t)need, &count));
+  if (count) {
+    LNT_HIP(hipMemcpyAsync(c_keys, st.b[4].ptr, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(c_vals, st.b[5].ptr, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  *nc = count;
+  return LNT_OK;
+}
+
+int lnt_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
+              int64_t na, const int64_t *b_keys, const double *b_vals,
+              int64_t nb, int op, int64_t **c_keys, double **c_vals,
+              int64_t *nc, int mem) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  if (!c_keys || !c_vals || !nc) return fail(LNT_ERR_INVALID_ARGUMENT, "null output pointer");
+  if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  int64_t cap = op == LNT_OP_HADAMARD ? std::min(na, nb) : na + nb;
+  size_t bytes = (size_t)(cap > 0 ? cap : 1) * 8;
+  void *k = nullptr, *v = nullptr;
+  if (mem == LNT_MEM_DEVICE) {
+    if (hipMalloc(&k, bytes) != hipSuccess || hipMalloc(&v, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      if (k) (void)hipFree(k);
+      return fail(LNT_ERR_OUT_OF_MEMORY, "hipMalloc of the merge result failed");
+    }
+  } else {
+    k = malloc(bytes);
+    v = malloc(bytes);
+    if (!k || !v) {
+      free(k);
+      free(v);
+      return fail(LNT_ERR_OUT_OF_MEMORY, "malloc of the merge result failed");
+    }
+  }
+  int rc = lnt_merge_into(ctx, a_keys, a_vals, na, b_keys, b_vals, nb, op, (int64_t *)k, (double *)v, cap, nc, mem);
+  if (rc != LNT_OK) {
+    if (mem == LNT_MEM_DEVICE) {
+      (void)hipFree(k);
+      (void)hipFree(v);
+    } else {
+      free(k);
+      free(v);
+    }
+    return rc;
+  }
+  *c_keys = (int64_t *)k;
+  *c_vals = (double *)v;
+  return LNT_OK;
+}
+
+int lnt_free(lnt_context ctx, void *ptr, int mem) {
+  if (!ptr) return LNT_OK;
+  if (mem == LNT_MEM_DEVICE) {
+    if (ctx) {
+      DeviceGuard guard(ctx->device);
+      LNT_HIP(hipFree(ptr));
+    } else {
+      LNT_HIP(hipFree(ptr));
+    }
+    return LNT_OK;
+  }
+  if (mem != LNT_MEM_HOST) return fail(LNT_ERR_INVALID_ARGUMENT, "mem must be LNT_MEM_HOST or LNT_MEM_DEVICE");
+  free(ptr);
+  return LNT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,259 @@
+// lnt_internal.h -- shared internals of the gfx950 sparse multi-index-array
+// library: exact 64-bit division by invariant divisors, the relinearisation
+// descriptor, decoupled look-back status words, the context, and error
+// reporting.  Device code is written for CDNA4 (wave64) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lnt.h"
+
+namespace lnt {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- errors ---
+void set_error(const std::string &msg);
+int fail(int status, const std::string &msg);
+
+#define LNT_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t _e = (call);                                                  \
+    if (_e != hipSuccess)                                                    \
+      return ::lnt::fail(LNT_ERR_DEVICE, std::string(#call) + ": " +        \
+                                             hipGetErrorString(_e));         \
+  } while (0)
+
+#define LNT_TRY(expr)          \
+  do {                         \
+    int _s = (expr);           \
+    if (_s != LNT_OK) return _s; \
+  } while (0)
+
+// ------------------------------------------------- exact unsigned division ---
+// q = n / d for a divisor fixed per launch, by the multiply-high method of
+// Granlund & Montgomery ("Division by invariant integers using
+// multiplication", PLDI 1994, Fig. 4.1): with l = ceil(log2 d) and
+// m = floor(2^64 (2^l - d) / d) + 1,
+//   t = mulhi(m, n);  q = (t + ((n - t) >> s1)) >> s2,  s1 = min(l,1), s2 = max(l-1,0)
+// is exact for every 64-bit n and every 1 <= d < 2^64.
+struct Divider {
+  uint64_t d;
+  uint64_t magic;
+  uint32_t s1, s2;
+};
+
+inline Divider make_divider(uint64_t d) {
+  Divider v;
+  v.d = d;
+  uint32_t l = 0;
+  while (l < 64 && (uint64_t(1) << l) < d) ++l;
+  unsigned __int128 two_l = (unsigned __int128)1 << l;
+  unsigned __int128 m = (((unsigned __int128)1 << 64) * (two_l - d)) / d + 1;
+  v.magic = (uint64_t)m;
+  v.s1 = l > 0 ? 1 : 0;
+  v.s2 = l > 0 ? l - 1 : 0;
+  return v;
+}
+
+__host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+__host__ __device__ inline uint64_t divide(uint64_t n, const Divider &v) {
+  uint64_t t = mulhi64(v.magic, n);
+  return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
+// ------------------------------------------------------ relinearisation ---
+// New key from old key: peel the old-order digits from the most significant
+// down (division by the old place values) and re-weight each by the place
+// value of the same mode in the new order.  Closed form of
+// sub2ind_reorder(ind2sub_reorder(key, cur), new).
+struct Relin {
+  int32_t order;
+  int32_t active;  // 0: identity
+  Divider place[LNT_MAX_ORDER];    // old place value of old position t
+  uint64_t weight[LNT_MAX_ORDER];  // new place value of the mode at old position t
+};
+
+__host__ __device__ inline uint64_t relin_key(const Relin &r, uint64_t idx) {
+  uint64_t out = 0;
+  for (int t = r.order - 1; t >= 1; --t) {
+    uint64_t q = divide(idx, r.place[t]);
+    out += q * r.weight[t];
+    idx -= q * r.place[t].d;
+  }
+  return out + idx * r.weight[0];
+}
+
+// Validates (order, dims, seqs) and fills a Relin.  *dimensionality receives
+// prod(dims).  Returns an lnt_status.
+int make_relin(int order, const int64_t *dims, const int32_t *cur_seq,
+               const int32_t *new_seq, Relin *out, uint64_t *dimensionality);
+
+// ----------------------------------------------- decoupled look-back word ---
+// 64-bit status word per (tile, digit): [63:62] flag, [61:32] epoch, [31:0]
+// value.  A word whose epoch differs from the current pass's is "not yet
+// published", so the status array never needs clearing between passes.
+constexpr uint64_t kFlagAggregate = 1;
+constexpr uint64_t kFlagInclusive = 2;
+constexpr uint32_t kEpochMask = 0x3FFFFFFFu;
+
+__device__ inline uint64_t status_word(uint64_t flag, uint32_t epoch,
+                                       uint32_t value) {
+  return (flag << 62) | ((uint64_t)(epoch & kEpochMask) << 32) | value;
+}
+
+__device__ inline void status_store(uint64_t *p, uint64_t w) {
+  __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint64_t status_load(const uint64_t *p) {
+  return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of this tile's value among all earlier tiles, by walking
+// back until an inclusive word is found.  Tile 0 never calls this.
+__device__ inline uint32_t lookback(const uint64_t *status, uint32_t tile,
+                                    uint32_t stride, uint32_t slot,
+                                    uint32_t epoch) {
+  uint32_t excl = 0;
+  int64_t t = (int64_t)tile - 1;
+  while (t >= 0) {
+    uint64_t w = status_load(status + (uint64_t)t * stride + slot);
+    uint64_t flag = w >> 62;
+    if (((uint32_t)(w >> 32) & kEpochMask) != (epoch & kEpochMask) || flag == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += (uint32_t)w;
+    if (flag == kFlagInclusive) break;
+    --t;
+  }
+  return excl;
+}
+
+// ------------------------------------------------------- wave primitives ---
+__device__ inline uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// popcount(mask & lanes below this one)
+__device__ inline uint32_t mask_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ inline uint32_t wave_inclusive_sum(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t y = __shfl_up(v, off, kWave);
+    if (lane >= (uint32_t)off) v += y;
+  }
+  return v;
+}
+
+// Block-wide exclusive sum for a 256-thread block; `scratch` holds 4 words.
+// Returns the exclusive prefix; *total receives the block sum.
+__device__ inline uint32_t block256_exclusive_sum(uint32_t v, uint32_t *scratch,
+                                                  uint32_t *total) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t incl = wave_inclusive_sum(v);
+  if (lane == 63) scratch[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    uint32_t s = scratch[w];
+    before += (uint32_t)w < wave ? s : 0;
+    all += s;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+// ---------------------------------------------------------------- context ---
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Grows `b` to at least `bytes` (contents are not preserved).
+int ensure(DeviceBuffer &b, size_t bytes);
+void release(DeviceBuffer &b);
+
+struct SortTuning {
+  int items_per_thread = 16;  // onesweep tile = 256 * items
+  int hist_blocks = 1024;
+  int merge_items = 8;  // merge tile = 256 * items
+};
+
+// Kernel kinds recorded by the profiler (names in api.hip).
+enum ProfKind { kProfHistogram = 0, kProfOnesweep, kProfPartition, kProfMerge, kProfRelin, kProfKinds };
+
+struct ProfRecord {
+  int kind;
+  hipEvent_t start, stop;
+  double bytes;  // algorithmic bytes moved by this launch
+};
+
+}  // namespace lnt
+
+struct lnt_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int validate = 1;
+  uint32_t epoch = 0;
+  lnt::SortTuning tuning;
+  lnt::DeviceBuffer alt_keys, alt_vals, alt2_keys, alt2_vals;
+  lnt::DeviceBuffer status;     // look-back words
+  lnt::DeviceBuffer small;      // histograms, counters, flags
+  lnt::DeviceBuffer aux;        // merge-path splits and other per-call scratch
+  lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
+  uint32_t *pinned = nullptr;   // small pinned host buffer for counts/flags
+  int profiling = 0;
+  std::vector<lnt::ProfRecord> prof;   // recorded launches (events owned)
+  std::vector<hipEvent_t> event_pool;  // recycled events
+};
+
+namespace lnt {
+
+// Layout of ctx->small (uint32 words).
+constexpr size_t kSmallHist = 0;                  // [8][256] digit histograms
+constexpr size_t kSmallCounters = 8 * 256;        // [16] tile counters
+constexpr size_t kSmallFlags = kSmallCounters + 16;  // [16] error flags / totals
+constexpr size_t kSmallWords = kSmallFlags + 16;
+
+// Profiling: prof_begin records a start event (returns a slot, or -1 when
+// profiling is off); prof_end records the stop event of that slot.
+int prof_begin(lnt_context ctx, int kind, double bytes);
+void prof_end(lnt_context ctx, int slot);
+void prof_set_bytes(lnt_context ctx, int slot, double bytes);
+
+// Next epoch for a look-back pass; clears the status array on wrap-around.
+int next_epoch(lnt_context ctx, uint32_t *epoch);
+int ensure_status(lnt_context ctx, size_t words);
+
+// Kernel launchers (sort.hip, merge.hip).  All pointers are device pointers.
+int launch_fill_u32(lnt_context ctx, uint32_t *p, uint32_t value, size_t n);
+int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
+                uint64_t bound, const Relin *relin);
+int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin);
+int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
+                 uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
+                 int op, uint64_t *ck, double *cv, uint64_t capacity,
+                 uint32_t *nc_host);
+
+}  // namespace lnt

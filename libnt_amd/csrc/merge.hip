@@ -1,0 +1,254 @@
+// merge.hip -- merge-path sparse union (+, -) and intersection (Hadamard) of
+// two sorted (uint64 key, double value) arrays for gfx950.
+//
+// Replaces the serial two-pointer merge of the reference
+// (multi_index_array/FunctionUtil.h:373-419 outside_merge_sparse_storage_containers)
+// and the pure Hadamard case of SparseMIABase::perform_noLatticeMult
+// (SparseMIABase.h:1717-1826).  Value rules, exactly as the reference:
+//   union:        equal keys -> op(a, b); A only -> a; B only -> op(0, b)
+//   intersection: equal keys -> a * b
+// Each value is a single IEEE operation, so results are bit-identical.
+//
+// One kernel, one pass over the inputs.  The merge path of A and B (ties: A
+// first) is cut into tiles of 256*ITEMS steps; a tile finds its end points by
+// binary search on its two diagonals, loads its slices into LDS, and each
+// thread merges ITEMS consecutive steps.  Because ties put A first, a key
+// present in both inputs appears as A[i] immediately followed by B[j]: the A
+// step emits the combined value (peeking at the next B key, which may belong to
+// the next tile) and the B step is dropped (peeking at the previous A key,
+// which may belong to the previous tile).  Output counts are block-scanned and
+// the tile's output offset comes from a decoupled look-back; outputs are staged
+// in LDS and written contiguously.
+#include "lnt_internal.h"
+
+namespace lnt {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Number of A elements among the first `diag` steps of the merge path.
+__device__ inline uint32_t path_search(const uint64_t *a, uint32_t na,
+                                       const uint64_t *b, uint32_t nb,
+                                       uint32_t diag) {
+  uint32_t lo = diag > nb ? diag - nb : 0;
+  uint32_t hi = diag < na ? diag : na;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= b[diag - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Merge-path split of every tile boundary, all searched in parallel; also
+// resets the tile counter of the merge kernel that follows.
+__global__ void partition_kernel(const uint64_t *__restrict__ a, uint32_t na,
+                                 const uint64_t *__restrict__ b, uint32_t nb,
+                                 uint32_t tile, uint32_t num_tiles,
+                                 uint32_t *__restrict__ splits,
+                                 uint32_t *__restrict__ tile_counter) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *tile_counter = 0;
+  if (t > num_tiles) return;
+  uint64_t total = (uint64_t)na + nb;
+  uint32_t diag = (uint32_t)min<uint64_t>((uint64_t)t * tile, total);
+  splits[t] = path_search(a, na, b, nb, diag);
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(kBlock) void merge_kernel(
+    const uint64_t *__restrict__ ak, const double *__restrict__ av, uint32_t na,
+    const uint64_t *__restrict__ bk, const double *__restrict__ bv, uint32_t nb,
+    int op, uint64_t *__restrict__ ck, double *__restrict__ cv,
+    const uint32_t *__restrict__ splits, uint64_t *__restrict__ status,
+    uint32_t epoch, uint32_t *__restrict__ tile_counter,
+    uint32_t *__restrict__ total_out, uint32_t num_tiles, uint64_t capacity) {
+  constexpr int kTile = kBlock * ITEMS;
+  __shared__ uint64_t s_key[kTile + 1];
+  __shared__ double s_val[kTile + 1];
+  __shared__ uint32_t s_tile, s_i0, s_i1;
+  __shared__ uint64_t s_prev_a;
+  __shared__ uint32_t s_has_prev;
+  __shared__ uint32_t s_scan[4];
+  __shared__ uint32_t s_excl;
+
+  const uint32_t tid = threadIdx.x;
+  const uint64_t total = (uint64_t)na + nb;
+  if (tid == 0)
+    s_tile = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t d0 = (uint32_t)min<uint64_t>((uint64_t)tile * kTile, total);
+  const uint32_t d1 = (uint32_t)min<uint64_t>((uint64_t)d0 + kTile, total);
+  if (tid == 0) s_i0 = splits[tile];
+  if (tid == 1) s_i1 = splits[tile + 1];
+  __syncthreads();
+  const uint32_t i0 = s_i0, i1 = s_i1;
+  const uint32_t j0 = d0 - i0, j1 = d1 - i1;
+  const uint32_t la = i1 - i0, lb = j1 - j0;
+  // A slice, B slice, then the B element after the slice (if any).
+  for (uint32_t x = tid; x < la; x += kBlock) {
+    s_key[x] = ak[i0 + x];
+    s_val[x] = av[i0 + x];
+  }
+  for (uint32_t x = tid; x < lb; x += kBlock) {
+    s_key[la + x] = bk[j0 + x];
+    s_val[la + x] = bv[j0 + x];
+  }
+  const bool has_next_b = j1 < nb;
+  if (tid == 0) {
+    if (has_next_b) {
+      s_key[la + lb] = bk[j1];
+      s_val[la + lb] = bv[j1];
+    }
+    s_has_prev = i0 > 0;
+    s_prev_a = i0 > 0 ? ak[i0 - 1] : 0;
+  }
+  __syncthreads();
+  const uint64_t *sa = s_key, *sb = s_key + la;
+  const double *sav = s_val, *sbv = s_val + la;
+  const uint32_t nlb = lb + (has_next_b ? 1 : 0);  // B keys visible to peeks
+
+  // Per-thread sub-path.
+  const uint32_t t0 = min<uint32_t>(tid * ITEMS, la + lb);
+  const uint32_t t1 = min<uint32_t>(t0 + ITEMS, la + lb);
+  uint32_t ia, jb;
+  {
+    uint32_t lo = t0 > lb ? t0 - lb : 0, hi = t0 < la ? t0 : la;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (sa[mid] <= sb[t0 - 1 - mid]) lo = mid + 1;
+      else hi = mid;
+    }
+    ia = lo;
+    jb = t0 - lo;
+  }
+  // Step s of this thread fills slot s; emit marks the slots that produce
+  // an output (register arrays are only indexed by unrolled constants).
+  uint64_t ok[ITEMS];
+  double ov[ITEMS];
+  uint32_t emit = 0;
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) {
+    ok[s] = 0;
+    ov[s] = 0.0;
+    if (t0 + s < t1) {
+      bool take_a = ia < la && (jb >= lb || sa[ia] <= sb[jb]);
+      if (take_a) {
+        uint64_t key = sa[ia];
+        double a = sav[ia];
+        bool match = jb < nlb && sb[jb] == key;
+        double b = match ? sbv[jb] : 0.0;
+        ok[s] = key;
+        if (op == LNT_OP_HADAMARD) {
+          ov[s] = a * b;
+          emit |= (match ? 1u : 0u) << s;
+        } else {
+          ov[s] = match ? (op == LNT_OP_SUB ? a - b : a + b) : a;
+          emit |= 1u << s;
+        }
+        ++ia;
+      } else {
+        uint64_t key = sb[jb];
+        bool absorbed = ia > 0 ? sa[ia - 1] == key : (s_has_prev && s_prev_a == key);
+        ok[s] = key;
+        ov[s] = op == LNT_OP_SUB ? 0.0 - sbv[jb] : 0.0 + sbv[jb];
+        emit |= ((!absorbed && op != LNT_OP_HADAMARD) ? 1u : 0u) << s;
+        ++jb;
+      }
+    }
+  }
+  const uint32_t cnt = (uint32_t)__popc(emit);
+  uint32_t tile_count;
+  uint32_t my_off = block256_exclusive_sum(cnt, s_scan, &tile_count);
+
+  if (tid == 0) {
+    uint32_t excl = 0;
+    uint64_t *my_status = status + tile;
+    if (tile == 0) {
+      status_store(my_status, status_word(kFlagInclusive, epoch, tile_count));
+    } else {
+      status_store(my_status, status_word(kFlagAggregate, epoch, tile_count));
+      excl = lookback(status, tile, 1, 0, epoch);
+      status_store(my_status, status_word(kFlagInclusive, epoch, excl + tile_count));
+    }
+    if (tile == num_tiles - 1) *total_out = excl + tile_count;
+    s_excl = excl;
+  }
+  __syncthreads();  // everyone is done reading the input slices
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s)
+    if ((emit >> s) & 1u) {
+      uint32_t idx = my_off + (uint32_t)__popc(emit & ((1u << s) - 1u));
+      s_key[idx] = ok[s];
+      s_val[idx] = ov[s];
+    }
+  __syncthreads();
+  const uint64_t base = s_excl;
+  // Sorted unique inputs never exceed the capacity; malformed ones (duplicate
+  // keys) could, so writes are bounded and the host reports the overflow.
+  for (uint32_t x = tid; x < tile_count && base + x < capacity; x += kBlock) {
+    ck[base + x] = s_key[x];
+    cv[base + x] = s_val[x];
+  }
+}
+
+template <int ITEMS>
+int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
+                 uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
+                 int op, uint64_t *ck, double *cv, uint64_t capacity,
+                 uint32_t *counter, uint32_t *total, int *slot) {
+  constexpr uint64_t tile = kBlock * ITEMS;
+  uint64_t steps = (uint64_t)na + nb;
+  uint32_t tiles = (uint32_t)((steps + tile - 1) / tile);
+  LNT_TRY(ensure_status(ctx, tiles));
+  LNT_TRY(ensure(ctx->aux, ((size_t)tiles + 1) * sizeof(uint32_t)));
+  uint32_t *splits = (uint32_t *)ctx->aux.ptr;
+  uint32_t epoch;
+  LNT_TRY(next_epoch(ctx, &epoch));
+  int pslot = prof_begin(ctx, kProfPartition, 0.0);
+  partition_kernel<<<(tiles + 1 + 255) / 256, 256, 0, ctx->stream>>>(
+      ak, na, bk, nb, (uint32_t)tile, tiles, splits, counter);
+  prof_end(ctx, pslot);
+  LNT_HIP(hipGetLastError());
+  *slot = prof_begin(ctx, kProfMerge, 16.0 * ((double)na + nb));
+  merge_kernel<ITEMS><<<tiles, kBlock, 0, ctx->stream>>>(
+      ak, av, na, bk, bv, nb, op, ck, cv, splits, (uint64_t *)ctx->status.ptr,
+      epoch, counter, total, tiles, capacity);
+  prof_end(ctx, *slot);
+  LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+}  // namespace
+
+int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
+                 uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
+                 int op, uint64_t *ck, double *cv, uint64_t capacity,
+                 uint32_t *nc_host) {
+  if ((uint64_t)na + nb == 0) {
+    *nc_host = 0;
+    return LNT_OK;
+  }
+  LNT_TRY(ensure(ctx->small, kSmallWords * sizeof(uint32_t)));
+  uint32_t *small = (uint32_t *)ctx->small.ptr;
+  uint32_t *counter = small + kSmallCounters + 15;
+  uint32_t *total = small + kSmallFlags + 1;
+  int rc, slot = -1;
+  if (ctx->tuning.merge_items == 16)
+    rc = launch_merge<16>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot);
+  else
+    rc = launch_merge<8>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot);
+  if (rc != LNT_OK) return rc;
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 1, total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  *nc_host = ctx->pinned[1];
+  prof_set_bytes(ctx, slot, 16.0 * ((double)na + nb + *nc_host));
+  if (*nc_host > capacity)
+    return fail(LNT_ERR_NOT_SORTED, "merge inputs are not sorted with unique keys (output overflow)");
+  return LNT_OK;
+}
+
+}  // namespace lnt

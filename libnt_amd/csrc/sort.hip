@@ -1,0 +1,391 @@
+// sort.hip -- LSD onesweep radix sort of (uint64 key, double value) pairs for
+// gfx950, with the relinearisation of change_sort_order fused into the first
+// read of the keys.
+//
+// Replaces the reference's serial in-place MSD radix
+// (multi_index_array/LibMIARadix.h:119-180, 444-484) and RadixShuffle
+// (LibMIARadix.h:705-927).  Keys are unique, so the sorted order and the
+// permuted values are fully determined and the result is bit-identical to the
+// reference whatever the algorithm.
+//
+// Plan for b = ceil(log2(bound)) key bits: P = ceil(b/8) passes of 8-bit digits.
+//   1. histogram kernel: one read of the keys (relinearised on the fly),
+//      P digit histograms at once, key-range check;
+//   2. P onesweep passes: each 256-thread block takes the next tile (dynamic
+//      tile id) of 256*ITEMS pairs, ranks digits inside each wave with ballot
+//      matching (stable), publishes its digit counts, obtains the global digit
+//      offsets by decoupled look-back, stages the tile in LDS in digit order and
+//      writes it out so that consecutive lanes store consecutive addresses.
+//   Even P ping-pongs caller buffer -> scratch -> caller; odd P goes through two
+//   scratch buffers so the result still lands in the caller's arrays.
+#include "lnt_internal.h"
+
+namespace lnt {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = 8;
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t value, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = value;
+}
+
+__global__ void fill_u64_kernel(uint64_t *p, uint64_t value, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = value;
+}
+
+// ------------------------------------------------------------ histogram ---
+struct DigitPlan {
+  int passes;
+  uint32_t shift[kMaxPasses];
+  uint32_t bits[kMaxPasses];
+};
+
+template <bool RELIN>
+__global__ __launch_bounds__(kBlock) void histogram_kernel(
+    const uint64_t *__restrict__ keys, uint32_t n, DigitPlan plan, uint64_t bound,
+    Relin relin, uint32_t *__restrict__ hist, uint32_t *__restrict__ counters,
+    uint32_t *__restrict__ err) {
+  const int passes = plan.passes;
+  __shared__ uint32_t h[kMaxPasses * kRadix];
+  const uint32_t tid = threadIdx.x;
+  for (int i = tid; i < passes * kRadix; i += kBlock) h[i] = 0;
+  if (blockIdx.x == 0 && tid < 16) counters[tid] = 0;
+  __syncthreads();
+
+  constexpr int kUnroll = 8;
+  bool bad = false;
+  const uint64_t chunk = (uint64_t)kBlock * kUnroll;
+  for (uint64_t base = (uint64_t)blockIdx.x * chunk; base < n;
+       base += (uint64_t)gridDim.x * chunk) {
+    uint64_t k[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      uint64_t i = base + (uint64_t)u * kBlock + tid;
+      k[u] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      uint64_t i = base + (uint64_t)u * kBlock + tid;
+      if (i < n) {
+        uint64_t key = k[u];
+        bad |= key >= bound;
+        if (RELIN) key = relin_key(relin, key);
+        for (int p = 0; p < passes; ++p)
+          atomicAdd(&h[p * kRadix + ((key >> plan.shift[p]) & ((1u << plan.bits[p]) - 1))], 1u);
+      }
+    }
+  }
+  if (bad) atomicOr(err, 1u);
+  __syncthreads();
+  for (int i = tid; i < passes * kRadix; i += kBlock)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// -------------------------------------------------------------- onesweep ---
+template <int ITEMS, bool RELIN>
+__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+    const uint64_t *__restrict__ kin, const double *__restrict__ vin,
+    uint64_t *__restrict__ kout, double *__restrict__ vout, uint32_t n,
+    uint32_t shift, uint32_t dbits, const uint32_t *__restrict__ pass_hist,
+    uint64_t *__restrict__ status, uint32_t epoch,
+    uint32_t *__restrict__ tile_counter, Relin relin) {
+  constexpr int kTile = kBlock * ITEMS;
+  constexpr int kWaves = kBlock / kWave;
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wcount[kWaves][kRadix];
+  __shared__ uint32_t s_lstart[kRadix];
+  __shared__ uint32_t s_gofs[kRadix];
+  __shared__ uint32_t s_scan[4];
+  __shared__ uint64_t s_stage[kTile];
+
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0)
+    s_tile = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) s_wcount[w][tid] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t tile_base = (uint64_t)tile * kTile;
+  const uint32_t tile_n = (uint32_t)min<uint64_t>(kTile, n - tile_base);
+  const uint64_t wave_base = tile_base + (uint64_t)wave * (kWave * ITEMS);
+  const uint32_t dmask = (1u << dbits) - 1;
+
+  uint64_t k[ITEMS];
+  double v[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint64_t pos = wave_base + (uint64_t)i * kWave + lane;
+    bool valid = pos < n;
+    k[i] = valid ? kin[pos] : 0;
+    v[i] = valid ? vin[pos] : 0.0;
+  }
+  if (RELIN) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) k[i] = relin_key(relin, k[i]);
+  }
+
+  // Stable in-wave ranking: lanes holding the same digit are found with one
+  // ballot per digit bit; the highest such lane bumps the wave's counter.
+  uint32_t local_rank[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint64_t pos = wave_base + (uint64_t)i * kWave + lane;
+    bool valid = pos < n;
+    uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
+    uint64_t peers = __ballot(valid);
+    for (uint32_t b = 0; b < dbits; ++b) {
+      uint64_t bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t r = 0;
+    if (valid) {
+      uint32_t cnt = s_wcount[wave][d];
+      r = cnt + mask_rank(peers);
+      uint32_t leader = 63u - (uint32_t)__builtin_clzll(peers);
+      if (lane == leader) s_wcount[wave][d] = cnt + (uint32_t)__popcll(peers);
+    }
+    local_rank[i] = r;
+  }
+  __syncthreads();
+
+  // Per digit (one thread each): prefix over waves, then over digits.
+  uint32_t count = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    uint32_t c = s_wcount[w][tid];
+    s_wcount[w][tid] = count;
+    count += c;
+  }
+  uint32_t total;
+  uint32_t lstart = block256_exclusive_sum(count, s_scan, &total);
+  s_lstart[tid] = lstart;
+
+  uint32_t excl;
+  uint64_t *my_status = status + (uint64_t)tile * kRadix + tid;
+  if (tile == 0) {
+    uint32_t all;
+    excl = block256_exclusive_sum(pass_hist[tid], s_scan, &all);
+    status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
+  } else {
+    status_store(my_status, status_word(kFlagAggregate, epoch, count));
+    excl = lookback(status, tile, kRadix, tid, epoch);
+    status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
+  }
+  s_gofs[tid] = excl - lstart;
+  __syncthreads();
+
+  // Keys: stage in digit order, then store.
+  uint32_t lpos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint64_t pos = wave_base + (uint64_t)i * kWave + lane;
+    uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
+    lpos[i] = 0xFFFFFFFFu;
+    if (pos < n) {
+      lpos[i] = s_lstart[d] + s_wcount[wave][d] + local_rank[i];
+      s_stage[lpos[i]] = k[i];
+    }
+  }
+  __syncthreads();
+  uint32_t dst[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint32_t p = (uint32_t)i * kBlock + tid;
+    dst[i] = 0;
+    if (p < tile_n) {
+      uint64_t key = s_stage[p];
+      uint32_t d = (uint32_t)(key >> shift) & dmask;
+      dst[i] = s_gofs[d] + p;
+      kout[dst[i]] = key;
+    }
+  }
+  __syncthreads();
+  // Values: same permutation.
+  double *s_vstage = reinterpret_cast<double *>(s_stage);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (lpos[i] != 0xFFFFFFFFu) s_vstage[lpos[i]] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    uint32_t p = (uint32_t)i * kBlock + tid;
+    if (p < tile_n) vout[dst[i]] = s_vstage[p];
+  }
+}
+
+__global__ void relin_kernel(uint64_t *keys, uint32_t n, Relin relin) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) keys[i] = relin_key(relin, keys[i]);
+}
+
+template <int ITEMS>
+int launch_pass(lnt_context ctx, bool relin, const uint64_t *kin,
+                const double *vin, uint64_t *kout, double *vout, uint32_t n,
+                uint32_t shift, uint32_t dbits, const uint32_t *pass_hist,
+                uint32_t *counter, const Relin &r) {
+  constexpr uint32_t tile = kBlock * ITEMS;
+  uint32_t tiles = (n + tile - 1) / tile;
+  LNT_TRY(ensure_status(ctx, (size_t)tiles * kRadix));
+  uint32_t epoch;
+  LNT_TRY(next_epoch(ctx, &epoch));
+  uint64_t *status = (uint64_t *)ctx->status.ptr;
+  int slot = prof_begin(ctx, kProfOnesweep, 32.0 * n);
+  if (relin)
+    onesweep_kernel<ITEMS, true><<<tiles, kBlock, 0, ctx->stream>>>(
+        kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
+  else
+    onesweep_kernel<ITEMS, false><<<tiles, kBlock, 0, ctx->stream>>>(
+        kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
+  prof_end(ctx, slot);
+  LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+}  // namespace
+
+int launch_fill_u32(lnt_context ctx, uint32_t *p, uint32_t value, size_t n) {
+  if (n == 0) return LNT_OK;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  fill_u32_kernel<<<(uint32_t)blocks, 256, 0, ctx->stream>>>(p, value, n);
+  LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+int ensure_status(lnt_context ctx, size_t words) {
+  size_t bytes = words * sizeof(uint64_t);
+  if (ctx->status.bytes >= bytes) return LNT_OK;
+  // Over-allocate so that slightly larger sorts do not reallocate.
+  LNT_TRY(ensure(ctx->status, bytes + bytes / 4));
+  size_t all = ctx->status.bytes / sizeof(uint64_t);
+  fill_u64_kernel<<<4096, 256, 0, ctx->stream>>>((uint64_t *)ctx->status.ptr, 0, all);
+  LNT_HIP(hipGetLastError());
+  ctx->epoch = 0;  // fresh words carry epoch 0, which is never current
+  return LNT_OK;
+}
+
+int next_epoch(lnt_context ctx, uint32_t *epoch) {
+  ctx->epoch = (ctx->epoch + 1) & kEpochMask;
+  if (ctx->epoch == 0) {
+    size_t all = ctx->status.bytes / sizeof(uint64_t);
+    fill_u64_kernel<<<4096, 256, 0, ctx->stream>>>((uint64_t *)ctx->status.ptr, 0, all);
+    LNT_HIP(hipGetLastError());
+    ctx->epoch = 1;
+  }
+  *epoch = ctx->epoch;
+  return LNT_OK;
+}
+
+int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin) {
+  if (n == 0 || !relin.active) return LNT_OK;
+  uint32_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  int slot = prof_begin(ctx, kProfRelin, 16.0 * n);
+  relin_kernel<<<blocks, 256, 0, ctx->stream>>>(keys, n, relin);
+  prof_end(ctx, slot);
+  LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+// Sorts (keys, vals) in place.  bound: keys must lie in [0, bound) before the
+// optional relinearisation (which maps [0,bound) onto itself).
+int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
+                uint64_t bound, const Relin *relin) {
+  if (n == 0) return LNT_OK;
+  const bool do_relin = relin && relin->active;
+  Relin r{};
+  if (do_relin) r = *relin;
+  uint32_t bits = 0;
+  while (bits < 64 && (uint64_t(1) << bits) < bound) ++bits;
+  // Digit plan: ceil(bits/8) passes (at least two so that no pass runs in
+  // place), digit widths balanced, least significant digit first.
+  DigitPlan plan{};
+  int passes = (int)((bits + kRadixBits - 1) / kRadixBits);
+  if (passes == 1) passes = 2;
+  if (n == 1 || bits == 0) passes = 0;
+  plan.passes = passes;
+  uint32_t shift = 0;
+  for (int p = 0; p < passes; ++p) {
+    uint32_t w = bits / passes + ((uint32_t)p < bits % passes ? 1 : 0);
+    plan.shift[p] = shift;
+    plan.bits[p] = w;
+    shift += w;
+  }
+  LNT_TRY(ensure(ctx->small, kSmallWords * sizeof(uint32_t)));
+  uint32_t *small = (uint32_t *)ctx->small.ptr;
+  uint32_t *hist = small + kSmallHist;
+  uint32_t *counters = small + kSmallCounters;
+  uint32_t *err = small + kSmallFlags;
+  LNT_TRY(launch_fill_u32(ctx, hist, 0, kSmallCounters - kSmallHist));
+  LNT_TRY(launch_fill_u32(ctx, err, 0, 1));
+
+  uint32_t hblocks = (uint32_t)ctx->tuning.hist_blocks;
+  uint32_t need = (n + kBlock * 8 - 1) / (kBlock * 8);
+  if (hblocks > need) hblocks = need;
+  int slot = prof_begin(ctx, kProfHistogram, 8.0 * n);
+  if (do_relin)
+    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, r, hist, counters, err);
+  else
+    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, r, hist, counters, err);
+  prof_end(ctx, slot);
+  LNT_HIP(hipGetLastError());
+
+  if (ctx->validate) {
+    LNT_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->pinned[0])
+      return fail(LNT_ERR_KEY_RANGE, "a key is negative or not below the dimensionality");
+  }
+  if (passes == 0) {
+    if (do_relin) LNT_TRY(relin_device(ctx, keys, n, r));
+    return LNT_OK;
+  }
+
+  LNT_TRY(ensure(ctx->alt_keys, (size_t)n * 8));
+  LNT_TRY(ensure(ctx->alt_vals, (size_t)n * 8));
+  uint64_t *bufk[3] = {keys, (uint64_t *)ctx->alt_keys.ptr, nullptr};
+  double *bufv[3] = {vals, (double *)ctx->alt_vals.ptr, nullptr};
+  if (passes & 1) {
+    LNT_TRY(ensure(ctx->alt2_keys, (size_t)n * 8));
+    LNT_TRY(ensure(ctx->alt2_vals, (size_t)n * 8));
+    bufk[2] = (uint64_t *)ctx->alt2_keys.ptr;
+    bufv[2] = (double *)ctx->alt2_vals.ptr;
+  }
+  // Buffers: even P: 0->1->0->...->0; odd P: 0->1->2->1->...->0.  No pass
+  // reads and writes the same buffer.
+  int src = 0;
+  for (int p = 0; p < passes; ++p) {
+    int dst;
+    if (p == passes - 1) dst = 0;
+    else if (passes & 1) dst = (src == 1) ? 2 : 1;
+    else dst = (src == 0) ? 1 : 0;
+    const bool rl = do_relin && p == 0;
+    int rc;
+    switch (ctx->tuning.items_per_thread) {
+      case 8:
+        rc = launch_pass<8>(ctx, rl, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
+                            plan.bits[p], hist + p * kRadix, counters + p, r);
+        break;
+      case 12:
+        rc = launch_pass<12>(ctx, rl, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
+                             plan.bits[p], hist + p * kRadix, counters + p, r);
+        break;
+      default:
+        rc = launch_pass<16>(ctx, rl, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
+                             plan.bits[p], hist + p * kRadix, counters + p, r);
+        break;
+    }
+    if (rc != LNT_OK) return rc;
+    src = dst;
+  }
+  return LNT_OK;
+}
+
+}  // namespace lnt

@@ -1,0 +1,695 @@
+/*
+ * lnt_oracle.c -- CPU restatement of LibMIA's sparse hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU
+ * baseline.  Nothing in the product library links or calls this file.
+ *
+ * Reference paths are relative to src/multi_index_array/ unless they start
+ * with src/.  Each function states what pins it (tests/test_oracle.py):
+ *   - generators: exact integer/IEEE arithmetic, pinned by committed vectors;
+ *   - lnto_change_linidx: closed form sub2ind(ind2sub) on exhaustive small dims;
+ *   - sorts: ascending order + (key,value) multiset, against numpy argsort;
+ *   - merges: dictionary union/intersection with the reference's op rules;
+ *   - lattice product / contraction: symbolic boolean product (index set) and
+ *     dense FP64 einsum (values within 1e-12, isEqualFuzzy semantics).
+ * Parity unpinned: the exact FP64 bits of contraction sums in the reference
+ * (they depend on its unstable per-column radix order; this file restates that
+ * order swap for swap, but without running the reference nothing pins it).
+ */
+#include "lnt_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ======================= synthetic-input generators ======================= */
+/* Not reference code: the reference seeds boost::mt19937 from time(0)
+ * (LibMIAUtil.h:43-46), which is neither reproducible nor portable.  These are
+ * counter-based and use only integer ops and exact IEEE ops, so the GPU
+ * generator in the bench reproduces them bit for bit. */
+
+uint64_t lnto_mix64(uint64_t z) {
+  /* SplitMix64 finaliser (Steele, Lea, Flood 2014). */
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint64_t lnto_rand_u64(uint64_t seed, uint64_t i) {
+  return lnto_mix64(seed + (i + 1) * 0x9E3779B97F4A7C15ull);
+}
+
+double lnto_uniform_pm1(uint64_t seed, uint64_t i) {
+  /* 53 random bits -> m * 2^-52 - 1 in [-1, 1 - 2^-52]; every step is exact. */
+  uint64_t m = lnto_rand_u64(seed, i) >> 11;
+  return (double)m * 0x1p-52 - 1.0;
+}
+
+double lnto_uniform_01(uint64_t seed, uint64_t i) {
+  return (double)(lnto_rand_u64(seed, i) >> 11) * 0x1p-53;
+}
+
+double lnto_normal(uint64_t seed, uint64_t i) {
+  /* Box-Muller on two independent 53-bit uniforms; u1 in (0,1]. */
+  double u1 = (double)((lnto_rand_u64(seed, 2 * i) >> 11) + 1) * 0x1p-53;
+  double u2 = (double)(lnto_rand_u64(seed, 2 * i + 1) >> 11) * 0x1p-53;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+uint64_t lnto_feistel(uint64_t x, int bits, uint64_t seed) {
+  /* Balanced 4-round Feistel network on [0, 2^bits), bits even: a bijection
+   * for any round function. */
+  int half = bits / 2;
+  uint64_t mask = half >= 32 ? 0xFFFFFFFFull : ((1ull << half) - 1);
+  uint64_t l = (x >> half) & mask, r = x & mask;
+  for (int round = 0; round < 4; ++round) {
+    uint64_t f = lnto_mix64(r ^ (seed + 0xA24BAED4963EE407ull * (uint64_t)(round + 1))) & mask;
+    uint64_t nl = r;
+    r = l ^ f;
+    l = nl;
+  }
+  return (l << half) | r;
+}
+
+uint64_t lnto_permute_index(uint64_t i, uint64_t domain, uint64_t seed) {
+  /* Bijection on [0, domain) by cycle walking a Feistel permutation of the
+   * next even power of two. */
+  if (domain <= 1) return 0;
+  int bits = 0;
+  while (bits < 64 && (1ull << bits) < domain) ++bits;
+  if (bits & 1) ++bits;
+  if (bits < 2) bits = 2;
+  uint64_t x = i;
+  do {
+    x = lnto_feistel(x, bits, seed);
+  } while (x >= domain);
+  return x;
+}
+
+void lnto_gen_distinct_keys(int64_t *keys, int64_t n, uint64_t domain,
+                            uint64_t seed, uint64_t offset) {
+  for (int64_t t = 0; t < n; ++t)
+    keys[t] = (int64_t)lnto_permute_index(offset + (uint64_t)t, domain, seed);
+}
+
+void lnto_gen_uniform_vals(double *vals, int64_t n, uint64_t seed,
+                           uint64_t offset) {
+  for (int64_t t = 0; t < n; ++t)
+    vals[t] = lnto_uniform_pm1(seed, offset + (uint64_t)t);
+}
+
+void lnto_random_perm(int32_t *perm, int n, uint64_t seed) {
+  /* Fisher-Yates with an unbiased-enough multiply-high draw (n is tiny). */
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int i = n - 1; i > 0; --i) {
+    uint64_t r = lnto_rand_u64(seed, (uint64_t)i);
+    int j = (int)(((unsigned __int128)r * (uint64_t)(i + 1)) >> 64);
+    int32_t t = perm[i];
+    perm[i] = perm[j];
+    perm[j] = t;
+  }
+}
+
+static uint64_t hash_slot(uint64_t key, uint64_t mask) {
+  return lnto_mix64(key ^ 0x5851F42D4C957F2Dull) & mask;
+}
+
+int lnto_gen_coords(int64_t *keys, double *vals, int64_t nnz, int order,
+                    const int64_t *dims, const int32_t *dist, double zipf_s,
+                    int value_kind, uint64_t seed) {
+  if (order <= 0 || order > 16 || nnz < 0) return -1;
+  double total = 1.0;
+  for (int m = 0; m < order; ++m) total *= (double)dims[m];
+  if ((double)nnz > total) return -1;
+  /* Zipf tables: P(x) proportional to (x+1)^-s over [0, dim). */
+  double *cdf[16] = {0};
+  for (int m = 0; m < order; ++m) {
+    if (!dist[m]) continue;
+    cdf[m] = (double *)malloc(sizeof(double) * (size_t)dims[m]);
+    double acc = 0.0;
+    for (int64_t x = 0; x < dims[m]; ++x) {
+      acc += pow((double)(x + 1), -zipf_s);
+      cdf[m][x] = acc;
+    }
+    for (int64_t x = 0; x < dims[m]; ++x) cdf[m][x] /= acc;
+  }
+  uint64_t cap = 1;
+  while (cap < (uint64_t)(2 * nnz + 2)) cap <<= 1;
+  uint64_t *table = (uint64_t *)malloc(sizeof(uint64_t) * cap);
+  for (uint64_t s = 0; s < cap; ++s) table[s] = ~0ull;
+  int64_t got = 0;
+  uint64_t draw = 0, max_draws = 1000ull * (uint64_t)(nnz + 1);
+  int rc = 0;
+  while (got < nnz) {
+    if (draw >= max_draws) { rc = -1; break; }
+    int64_t key = 0, stride = 1;
+    for (int m = 0; m < order; ++m) {
+      uint64_t r = lnto_rand_u64(seed, draw * (uint64_t)order + (uint64_t)m);
+      int64_t sub;
+      if (!dist[m]) {
+        sub = (int64_t)(((unsigned __int128)r * (uint64_t)dims[m]) >> 64);
+      } else {
+        double u = (double)(r >> 11) * 0x1p-53;
+        int64_t lo = 0, hi = dims[m] - 1; /* first x with cdf[x] > u */
+        while (lo < hi) {
+          int64_t mid = (lo + hi) / 2;
+          if (cdf[m][mid] > u) hi = mid; else lo = mid + 1;
+        }
+        sub = lo;
+      }
+      key += sub * stride;
+      stride *= dims[m];
+    }
+    ++draw;
+    uint64_t s = hash_slot((uint64_t)key, cap - 1);
+    int dup = 0;
+    while (table[s] != ~0ull) {
+      if (table[s] == (uint64_t)key) { dup = 1; break; }
+      s = (s + 1) & (cap - 1);
+    }
+    if (dup) continue;
+    table[s] = (uint64_t)key;
+    keys[got] = key;
+    vals[got] = value_kind == 1 ? lnto_normal(seed ^ 0xD1B54A32D192ED03ull, (uint64_t)got)
+                                : lnto_uniform_pm1(seed ^ 0xD1B54A32D192ED03ull, (uint64_t)got);
+    ++got;
+  }
+  free(table);
+  for (int m = 0; m < order; ++m) free(cdf[m]);
+  return rc;
+}
+
+/* ================================ index math =============================== */
+
+/* IndexUtil.h:866-879 sub2ind_reorder. */
+int64_t lnto_sub2ind(const int64_t *sub, const int32_t *seq,
+                     const int64_t *dims, int order) {
+  uint64_t key = 0, mult = 1;
+  for (int t = 0; t < order; ++t) {
+    key += (uint64_t)sub[seq[t]] * mult;
+    mult *= (uint64_t)dims[seq[t]];
+  }
+  return (int64_t)key;
+}
+
+/* IndexUtil.h:318-333 ind2sub_reorder. */
+void lnto_ind2sub(int64_t key, const int32_t *seq, const int64_t *dims,
+                  int order, int64_t *sub) {
+  uint64_t k = (uint64_t)key;
+  for (int t = 0; t < order; ++t) {
+    sub[seq[t]] = (int64_t)(k % (uint64_t)dims[seq[t]]);
+    k /= (uint64_t)dims[seq[t]];
+  }
+}
+
+/* SparseMIABase.h:768-814 change_linIdx_sequence with the relinearisation of
+ * IndexUtil.h:540-598 reShuffleLinearIndex: peel the old-order digits from the
+ * most significant down (dividing by the old-order place values, the
+ * dim_accumulator of create_shuffle_needs, IndexUtil.h:480-487) and re-weight
+ * each by that mode's place value in the new order.  Does not sort. */
+void lnto_change_linidx(int64_t *keys, int64_t n, int order,
+                        const int64_t *dims, const int32_t *cur_seq,
+                        const int32_t *new_seq) {
+  if (order <= 1) return;
+  int same = 1;
+  for (int t = 0; t < order; ++t) same &= cur_seq[t] == new_seq[t];
+  if (same) return;
+  uint64_t old_place[64], new_weight[64];
+  uint64_t acc = 1;
+  for (int t = 0; t < order; ++t) {
+    old_place[t] = acc;
+    acc *= (uint64_t)dims[cur_seq[t]];
+  }
+  uint64_t stride_of_mode[64];
+  acc = 1;
+  for (int t = 0; t < order; ++t) {
+    stride_of_mode[new_seq[t]] = acc;
+    acc *= (uint64_t)dims[new_seq[t]];
+  }
+  for (int t = 0; t < order; ++t) new_weight[t] = stride_of_mode[cur_seq[t]];
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t idx = (uint64_t)keys[i], out = 0;
+    for (int t = order - 1; t >= 1; --t) {
+      uint64_t q = idx / old_place[t];
+      out += q * new_weight[t];
+      idx -= q * old_place[t];
+    }
+    out += idx * new_weight[0];
+    keys[i] = (int64_t)out;
+  }
+}
+
+/* ================================= sorting ================================ */
+/* Introsort with a follower payload: LibMIAAlgorithm.h:153-174 (DualSwapper),
+ * :439-511 (Partition), :522-535 (HeapSort/SiftDown), :547-560 (MedianOfThree),
+ * :570-611 (IntrosortRec), :621-632 (IntrosortDepth).  Same comparisons and
+ * swaps in the same order, so equal keys end up where the reference puts them. */
+
+static inline void dual_swap(int64_t *k, double *v, int64_t i, int64_t j) {
+  double tv = v[i];
+  v[i] = v[j];
+  v[j] = tv;
+  int64_t tk = k[i];
+  k[i] = k[j];
+  k[j] = tk;
+}
+
+static void sift_down(int64_t *k, double *v, int64_t size, int64_t cur) {
+  int64_t root = cur;
+  while (root * 2 + 1 < size) {
+    int64_t child = root * 2 + 1;
+    if (child + 1 < size && k[child] < k[child + 1]) child += 1;
+    if (k[root] < k[child]) {
+      dual_swap(k, v, child, root);
+      root = child;
+    } else {
+      return;
+    }
+  }
+}
+
+static void heap_sort(int64_t *k, double *v, int64_t size) {
+  for (int64_t s = (size - 2) / 2; s >= 0; --s) sift_down(k, v, size, s);
+  for (int64_t e = size - 1; e > 0; --e) {
+    dual_swap(k, v, e, 0);
+    sift_down(k, v, e, 0);
+  }
+}
+
+static int64_t median_of_three(const int64_t *k, int64_t one, int64_t two,
+                               int64_t three) {
+  int c12 = k[one] < k[two], c13 = k[one] < k[three], c23 = k[two] < k[three];
+  if (c12 && c23) return two;
+  if (c12 && !c23 && c13) return three;
+  if (!c12 && c13) return one;
+  if (!c12 && !c13 && c23) return three;
+  if (c12 && !c13) return one;
+  return two;
+}
+
+static int64_t partition_pairs(int64_t *k, double *v, int64_t b, int64_t e) {
+  int64_t lhs = b + 1, rhs = e - 1;
+  for (;;) {
+    while (lhs < rhs && !(k[rhs] < k[b])) --rhs;
+    while (lhs < rhs && k[lhs] < k[b]) ++lhs;
+    if (lhs == rhs) break;
+    dual_swap(k, v, lhs, rhs);
+  }
+  if (!(k[lhs] < k[b])) return b;
+  dual_swap(k, v, b, lhs);
+  return lhs;
+}
+
+static uint64_t introsort_depth(int64_t n) {
+  uint64_t lg2 = 0;
+  for (uint64_t m = (uint64_t)n; m != 0; m >>= 1) ++lg2;
+  return lg2 * 2;
+}
+
+static void introsort_rec(int64_t *k, double *v, int64_t b, int64_t e,
+                          uint64_t depth) {
+  const int64_t block = 50;
+  for (;;) {
+    int64_t n = e - b;
+    if (n < block) return;
+    if (depth == 0) {
+      heap_sort(k + b, v + b, n);
+      return;
+    }
+    int64_t pivot = median_of_three(k, b, b + n / 2, e - 1);
+    dual_swap(k, v, pivot, b);
+    int64_t p = partition_pairs(k, v, b, e);
+    introsort_rec(k, v, b, p, depth - 1);
+    b = p + 1; /* second recursive call of the reference, as a loop */
+    depth -= 1;
+  }
+}
+
+/* LibMIAAlgorithm.h:673-706 InsertionSortImproved (stable shifting insert). */
+static void insertion_sort_improved(int64_t *k, double *v, int64_t n) {
+  for (int64_t i = 1; i < n; ++i) {
+    if (k[i] < k[i - 1]) {
+      int64_t ck = k[i];
+      double cv = v[i];
+      k[i] = k[i - 1];
+      v[i] = v[i - 1];
+      int64_t j = i - 1;
+      for (; j > 0 && ck < k[j - 1]; --j) {
+        k[j] = k[j - 1];
+        v[j] = v[j - 1];
+      }
+      k[j] = ck;
+      v[j] = cv;
+    }
+  }
+}
+
+/* LibMIARadix.h:119-180: in-place MSD radix, 2048 bins (11-bit digits),
+ * cycle-leader permutation with the follower swapped alongside; bins of at
+ * least 3000 recurse, bins of 50..2999 go to IntrosortRec, smaller bins are left
+ * for the final insertion sort.  The Threshold template argument of the
+ * reference's call sites is the intended 3000 (see the arity defect at
+ * LibMIARadix.h:470). */
+enum { RADIX = 2048, LOG_RADIX = 11, RADIX_THRESHOLD = 3000, INSERT_THRESHOLD = 50 };
+
+static void radix_l1(int64_t *k, double *v, int64_t len, unsigned shift) {
+  int64_t *count = (int64_t *)calloc(RADIX, sizeof(int64_t));
+  int64_t *start = (int64_t *)malloc(sizeof(int64_t) * (RADIX + 1));
+  int64_t *end = (int64_t *)malloc(sizeof(int64_t) * (RADIX + 1));
+  const uint64_t mask = RADIX - 1;
+  for (int64_t i = 0; i < len; ++i) count[((uint64_t)k[i] >> shift) & mask]++;
+  start[0] = end[0] = 0;
+  start[RADIX] = -1; /* sentinel */
+  for (int b = 1; b < RADIX; ++b) start[b] = end[b] = start[b - 1] + count[b - 1];
+  int64_t next_bin = 1;
+  for (int64_t cur = 0; cur < len;) {
+    uint64_t d;
+    while (end[d = (((uint64_t)k[cur] >> shift) & mask)] != cur) {
+      dual_swap(k, v, cur, end[d]);
+      end[d]++;
+    }
+    end[d]++;
+    while (end[next_bin - 1] == start[next_bin]) ++next_bin;
+    cur = end[next_bin - 1];
+  }
+  if (shift != 0) {
+    shift -= LOG_RADIX;
+    for (int b = 0; b < RADIX; ++b) {
+      int64_t nb = end[b] - start[b];
+      if (nb >= RADIX_THRESHOLD)
+        radix_l1(k + start[b], v + start[b], nb, shift);
+      else if (nb >= INSERT_THRESHOLD)
+        introsort_rec(k + start[b], v + start[b], 0, nb, introsort_depth(nb));
+    }
+  }
+  free(count);
+  free(start);
+  free(end);
+}
+
+/* LibMIARadix.h:444-484 RadixSortInPlace_PowerOf2Radix_Unsigned: the digit
+ * plan comes from ceil(log2(max_size)) bits, the top digit first. */
+void lnto_radix_sort_pairs(int64_t *keys, double *vals, int64_t n,
+                           uint64_t max_size) {
+  if (n <= 0) return;
+  unsigned long bits = max_size <= 1 ? 0 : (unsigned long)ceil(log2((double)max_size));
+  unsigned long shifts = (unsigned long)ceil((double)bits / LOG_RADIX);
+  unsigned long shift = bits >= LOG_RADIX ? (shifts - 1) * LOG_RADIX : 0;
+  if (n >= RADIX_THRESHOLD)
+    radix_l1(keys, vals, n, (unsigned)shift);
+  else
+    introsort_rec(keys, vals, 0, n, introsort_depth(n));
+  insertion_sort_improved(keys, vals, n);
+}
+
+/* LibMIAAlgorithm.h:752-764 Introsort with DualSwapper (keys are unique where
+ * the reference calls it, so its final insertion pass is interchangeable with
+ * the improved one). */
+void lnto_introsort_pairs(int64_t *keys, double *vals, int64_t n) {
+  if (n <= 0) return;
+  introsort_rec(keys, vals, 0, n, introsort_depth(n));
+  insertion_sort_improved(keys, vals, n);
+}
+
+static uint64_t dimensionality(const int64_t *dims, int order) {
+  uint64_t d = 1;
+  for (int m = 0; m < order; ++m) d *= (uint64_t)dims[m];
+  return d;
+}
+
+/* SparseMIABase.h:590-645 sort(seq) = change_sort_order.  Unsorted input:
+ * change_linIdx_sequence then the MSD radix (:596-599, :727-728).  Sorted input
+ * of fewer than 3000 entries: relinearise then Introsort (:615-618).  Sorted
+ * larger input: the reference runs RadixShuffle::permute (:622-634,
+ * LibMIARadix.h:811-900); with unique keys its output is identical to the
+ * relinearise + MSD radix path restated here, which is what this oracle times. */
+void lnto_sort_seq(int64_t *keys, double *vals, int64_t n, int order,
+                   const int64_t *dims, const int32_t *cur_seq,
+                   const int32_t *new_seq, int is_sorted) {
+  int same = 1;
+  for (int t = 0; t < order; ++t) same &= cur_seq[t] == new_seq[t];
+  if (!is_sorted) {
+    lnto_change_linidx(keys, n, order, dims, cur_seq, new_seq);
+    lnto_radix_sort_pairs(keys, vals, n, dimensionality(dims, order));
+    return;
+  }
+  if (order == 1 || same) return;
+  lnto_change_linidx(keys, n, order, dims, cur_seq, new_seq);
+  if (n < 3000)
+    lnto_introsort_pairs(keys, vals, n);
+  else
+    lnto_radix_sort_pairs(keys, vals, n, dimensionality(dims, order));
+}
+
+/* FunctionUtil.h:700-728 collect_duplicates_function with std::plus. */
+int64_t lnto_collect_duplicates(int64_t *keys, double *vals, int64_t n) {
+  if (n <= 0) return 0;
+  int64_t r = 0;
+  for (int64_t f = 1; f < n; ++f) {
+    if (keys[r] != keys[f]) {
+      ++r;
+      keys[r] = keys[f];
+      vals[r] = vals[f];
+    } else {
+      vals[r] = vals[r] + vals[f];
+    }
+  }
+  return r + 1;
+}
+
+/* ================================= merging ================================ */
+
+static inline double merge_op(int op, double a, double b) {
+  return op == LNTO_OP_SUB ? a - b : a + b;
+}
+
+/* FunctionUtil.h:373-419 outside_merge_sparse_storage_containers (op 0/1):
+ * sorted union; equal keys -> op(a,b); A only -> a; B only -> op(0,b).
+ * op 2: the pure Hadamard product of SparseMIABase.h:1717-1826 (no outer
+ * indices): sorted intersection with a*b. */
+int64_t lnto_merge(const int64_t *ak, const double *av, int64_t na,
+                   const int64_t *bk, const double *bv, int64_t nb, int op,
+                   int64_t *ck, double *cv) {
+  int64_t i = 0, j = 0, c = 0;
+  if (op == LNTO_OP_HADAMARD)
+    return lnto_outer_inter_product(ak, av, na, 1, bk, bv, nb, 1, ck, cv);
+  while (i < na && j < nb) {
+    if (ak[i] < bk[j]) {
+      ck[c] = ak[i]; cv[c++] = av[i++];
+    } else if (bk[j] < ak[i]) {
+      ck[c] = bk[j]; cv[c++] = merge_op(op, 0.0, bv[j++]);
+    } else {
+      ck[c] = ak[i]; cv[c++] = merge_op(op, av[i++], bv[j++]);
+    }
+  }
+  while (j < nb) { ck[c] = bk[j]; cv[c++] = merge_op(op, 0.0, bv[j++]); }
+  while (i < na) { ck[c] = ak[i]; cv[c++] = av[i++]; }
+  return c;
+}
+
+/* SparseMIABase.h:1717-1826 perform_noLatticeMult after both operands were
+ * sorted to [outer, inter] order (:1746-1747): for every inter index present in
+ * both, the outer product of the two groups, B outer loop, A inner loop, key
+ * l_outer + r_outer*L + inter*L*R (:1811-1820).  Scans instead of the
+ * reference's interpolation searches (find_start/end_idx, :2055-2132), which
+ * only change how fast the group bounds are found.  ck == NULL counts only. */
+int64_t lnto_outer_inter_product(const int64_t *ak, const double *av,
+                                 int64_t na, int64_t l_outer_size,
+                                 const int64_t *bk, const double *bv,
+                                 int64_t nb, int64_t r_outer_size,
+                                 int64_t *ck, double *cv) {
+  int64_t i = 0, j = 0, c = 0;
+  const uint64_t L = (uint64_t)l_outer_size, R = (uint64_t)r_outer_size;
+  while (i < na && j < nb) {
+    uint64_t ta = (uint64_t)ak[i] / L, tb = (uint64_t)bk[j] / R;
+    if (ta < tb) { ++i; continue; }
+    if (tb < ta) { ++j; continue; }
+    int64_t ie = i, je = j;
+    while (ie < na && (uint64_t)ak[ie] / L == ta) ++ie;
+    while (je < nb && (uint64_t)bk[je] / R == tb) ++je;
+    uint64_t base = ta * L * R;
+    for (int64_t q = j; q < je; ++q) {
+      uint64_t ro = (uint64_t)bk[q] % R;
+      for (int64_t p = i; p < ie; ++p) {
+        if (ck) {
+          ck[c] = (int64_t)((uint64_t)ak[p] % L + ro * L + base);
+          cv[c] = av[p] * bv[q];
+        }
+        ++c;
+      }
+    }
+    i = ie;
+    j = je;
+  }
+  return c;
+}
+
+int64_t lnto_outer_inter_count(const int64_t *ak, int64_t na,
+                               int64_t l_outer_size, const int64_t *bk,
+                               int64_t nb, int64_t r_outer_size) {
+  return lnto_outer_inter_product(ak, NULL, na, l_outer_size, bk, NULL, nb,
+                                  r_outer_size, NULL, NULL);
+}
+
+/* ============================ lattice product ============================= */
+/* Lattice key = row + h*col + h*w*tab (SparseLatticeBase.h:867-877); operands
+ * sorted ascending (column major).  nprod = sum over B entries (k,j,t) of
+ * nnz(A(:,k,t)). */
+
+static int64_t lower_bound_i64(const int64_t *a, int64_t n, int64_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+int64_t lnto_lattice_nprod(const int64_t *ak, int64_t na, int64_t ah,
+                           int64_t aw, const int64_t *bk, int64_t nb,
+                           int64_t bh, int64_t bw) {
+  int64_t total = 0;
+  for (int64_t q = 0; q < nb; ++q) {
+    int64_t t = bk[q] / (bh * bw);
+    int64_t krow = bk[q] % bh;
+    int64_t lo = lower_bound_i64(ak, na, t * ah * aw + krow * ah);
+    int64_t hi = lower_bound_i64(ak, na, t * ah * aw + (krow + 1) * ah);
+    total += hi - lo;
+  }
+  return total;
+}
+
+/* SparseLatticeBase.h:1854-1990 csc_no_accum<true>: for each tab present in
+ * both lattices and each column j of B (in key order), scatter beta*A(:,k) for
+ * every B entry (k,j) (mult_scatter, :1993-2031, value beta*a), then
+ * RadixSortInPlace the column's rows with max_size = height (:1966-1969),
+ * collect duplicates with plus (:1970-1972) and add tab*h*w + h*j (:1974-1978).
+ * Returns nnz(C), or -1 if cap is too small. */
+int64_t lnto_lattice_mult(const int64_t *ak, const double *av, int64_t na,
+                          int64_t ah, int64_t aw, int64_t ad,
+                          const int64_t *bk, const double *bv, int64_t nb,
+                          int64_t bh, int64_t bw, int64_t bd, int64_t *ck,
+                          double *cv, int64_t cap) {
+  (void)ad; (void)bd;
+  if (aw != bh) return -1;
+  const int64_t a_tab_size = ah * aw, b_tab_size = bh * bw;
+  int64_t c = 0, q = 0;
+  while (q < nb) {
+    int64_t tab = bk[q] / b_tab_size;
+    int64_t col = (bk[q] % b_tab_size) / bh;
+    int64_t col_end = tab * b_tab_size + (col + 1) * bh;
+    int64_t start = c;
+    for (; q < nb && bk[q] < col_end; ++q) {
+      int64_t krow = bk[q] % bh;
+      int64_t lo = lower_bound_i64(ak, na, tab * a_tab_size + krow * ah);
+      int64_t hi = lower_bound_i64(ak, na, tab * a_tab_size + (krow + 1) * ah);
+      if (c + (hi - lo) > cap) return -1;
+      for (int64_t p = lo; p < hi; ++p) {
+        ck[c] = ak[p] % ah;
+        cv[c] = bv[q] * av[p];
+        ++c;
+      }
+    }
+    if (c - start > 1) {
+      lnto_radix_sort_pairs(ck + start, cv + start, c - start, (uint64_t)ah);
+      c = start + lnto_collect_duplicates(ck + start, cv + start, c - start);
+    }
+    int64_t off = tab * ah * bw + ah * col;
+    for (int64_t p = start; p < c; ++p) ck[p] += off;
+  }
+  return c;
+}
+
+/* Index bookkeeping of the lattice product, MIA_Expr.h:449-561
+ * (solve_product_expr_helper + product_lattice_expr_helper): A's rows are its
+ * outer modes, columns its inner (contracted) modes, tabs its inter modes, each
+ * in A's mode order; B's rows are its inner modes in the order matched to A's,
+ * columns its outer modes in B's order, tabs its inter modes matched to A's.
+ * C's dims are [A-outer, B-outer, inter] (:493-501). */
+int lnto_contract_plan(int order_a, const int64_t *dims_a, const char *labels_a,
+                       int order_b, const int64_t *dims_b, const char *labels_b,
+                       int32_t *seq_a, int32_t *seq_b, int64_t *lat_a,
+                       int64_t *lat_b, int64_t *dims_c, int *order_c) {
+  int a_out[32], a_in[32], a_int[32], b_out[32], b_in[32], b_int[32];
+  int na_out = 0, na_in = 0, na_int = 0, nb_out = 0;
+  if (order_a > 32 || order_b > 32) return -1;
+  for (int m = 0; m < order_a; ++m) {
+    int match = -1;
+    for (int r = 0; r < order_b; ++r)
+      if (labels_b[r] == labels_a[m]) match = r;
+    for (int r = 0; r < m; ++r)
+      if (labels_a[r] == labels_a[m]) return -1; /* repeated label */
+    int inter = labels_a[m] >= 'A' && labels_a[m] <= 'Z';
+    if (match < 0) {
+      if (inter) return -1;
+      a_out[na_out++] = m;
+    } else if (inter) {
+      a_int[na_int] = m;
+      b_int[na_int++] = match;
+    } else {
+      a_in[na_in] = m;
+      b_in[na_in++] = match;
+    }
+    if (match >= 0 && dims_b[match] != dims_a[m]) return -1;
+  }
+  for (int r = 0; r < order_b; ++r) {
+    int match = -1;
+    for (int m = 0; m < order_a; ++m)
+      if (labels_a[m] == labels_b[r]) match = m;
+    for (int s = 0; s < r; ++s)
+      if (labels_b[s] == labels_b[r]) return -1;
+    if (match < 0) {
+      if (labels_b[r] >= 'A' && labels_b[r] <= 'Z') return -1;
+      b_out[nb_out++] = r;
+    }
+  }
+  int t = 0;
+  int64_t h = 1, w = 1, d = 1;
+  for (int x = 0; x < na_out; ++x) { seq_a[t++] = a_out[x]; h *= dims_a[a_out[x]]; }
+  for (int x = 0; x < na_in; ++x) { seq_a[t++] = a_in[x]; w *= dims_a[a_in[x]]; }
+  for (int x = 0; x < na_int; ++x) { seq_a[t++] = a_int[x]; d *= dims_a[a_int[x]]; }
+  lat_a[0] = h; lat_a[1] = w; lat_a[2] = d;
+  t = 0;
+  int64_t bwid = 1;
+  for (int x = 0; x < na_in; ++x) seq_b[t++] = b_in[x];
+  for (int x = 0; x < nb_out; ++x) { seq_b[t++] = b_out[x]; bwid *= dims_b[b_out[x]]; }
+  for (int x = 0; x < na_int; ++x) seq_b[t++] = b_int[x];
+  lat_b[0] = w; lat_b[1] = bwid; lat_b[2] = d;
+  t = 0;
+  for (int x = 0; x < na_out; ++x) dims_c[t++] = dims_a[a_out[x]];
+  for (int x = 0; x < nb_out; ++x) dims_c[t++] = dims_b[b_out[x]];
+  for (int x = 0; x < na_int; ++x) dims_c[t++] = dims_a[a_int[x]];
+  *order_c = t;
+  return 0;
+}
+
+/* MIA_Expr.h:864-925 lattice-product operator*: unfold both operands
+ * (lattice_permutation_delegator, :339-419 -> SparseMIABase::sort(seq) through
+ * toStraightLattice / toLatticeSort, SparseMIABase.h:1457-1586), multiply
+ * (SparseLatticeBase::operator* -> csc_no_accum) and refold
+ * (SparseMIA(dims, SparseLattice&&), SparseMIA.h:362-372: C keeps the lattice
+ * keys, which are C's default-seq keys). */
+int64_t lnto_contract(int64_t *ak, double *av, int64_t na, int order_a,
+                      const int64_t *dims_a, const int32_t *seq_a,
+                      int a_sorted, const char *labels_a, int64_t *bk,
+                      double *bv, int64_t nb, int order_b,
+                      const int64_t *dims_b, const int32_t *seq_b,
+                      int b_sorted, const char *labels_b, int64_t *ck,
+                      double *cv, int64_t cap, int64_t *dims_c, int *order_c) {
+  int32_t la[32], lb[32];
+  int64_t lat_a[3], lat_b[3];
+  if (lnto_contract_plan(order_a, dims_a, labels_a, order_b, dims_b, labels_b,
+                         la, lb, lat_a, lat_b, dims_c, order_c) != 0)
+    return -1;
+  lnto_sort_seq(ak, av, na, order_a, dims_a, seq_a, la, a_sorted);
+  lnto_sort_seq(bk, bv, nb, order_b, dims_b, seq_b, lb, b_sorted);
+  return lnto_lattice_mult(ak, av, na, lat_a[0], lat_a[1], lat_a[2], bk, bv, nb,
+                           lat_b[0], lat_b[1], lat_b[2], ck, cv, cap);
+}
+
+/* FunctionUtil.h:688-698 isEqualFuzzy. */
+int lnto_is_equal_fuzzy(double a, double b, double prec) {
+  if (fabs(a) < 1 || fabs(b) < 1) return fabs(a - b) <= prec;
+  double m = fabs(a) < fabs(b) ? fabs(a) : fabs(b);
+  return fabs(a - b) <= m * prec;
+}

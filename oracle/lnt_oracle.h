@@ -1,0 +1,112 @@
+/*
+ * lnt_oracle.h -- CPU restatement of LibMIA's sparse multi-index-array hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path and
+ * the timed CPU baseline; it is never linked into, loaded by, or called from the
+ * product library (libnt_amd/liblnt.so).  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load it.
+ *
+ * Conventions (all follow the reference, paths relative to src/multi_index_array/):
+ *   - a sparse tensor is two parallel arrays: int64 linear indices ("keys",
+ *     always >= 0) and double values (SparseMIA.h:61-65, 203-206);
+ *   - a key is computed in a mode order "seq" whose first entry is the fastest
+ *     varying mode: key = sum_t sub[seq[t]] * prod_{u<t} dims[seq[u]]
+ *     (IndexUtil.h:866-879 sub2ind_reorder, IndexUtil.h:318-333 ind2sub_reorder);
+ *   - "sorted" means ascending keys; keys of one tensor are unique.
+ *
+ * Parity status of each function is recorded in the header comment of its
+ * definition in lnt_oracle.c and pinned by tests/test_oracle.py.
+ */
+#ifndef LNT_ORACLE_H
+#define LNT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- bit-portable synthetic-input generators ---------------- */
+uint64_t lnto_mix64(uint64_t x);
+uint64_t lnto_rand_u64(uint64_t seed, uint64_t i);
+double lnto_uniform_pm1(uint64_t seed, uint64_t i);
+double lnto_uniform_01(uint64_t seed, uint64_t i);
+double lnto_normal(uint64_t seed, uint64_t i);
+uint64_t lnto_feistel(uint64_t x, int bits, uint64_t seed);
+uint64_t lnto_permute_index(uint64_t i, uint64_t domain, uint64_t seed);
+void lnto_gen_distinct_keys(int64_t *keys, int64_t n, uint64_t domain,
+                            uint64_t seed, uint64_t offset);
+void lnto_gen_uniform_vals(double *vals, int64_t n, uint64_t seed,
+                           uint64_t offset);
+void lnto_random_perm(int32_t *perm, int n, uint64_t seed);
+/* Draws distinct coordinates mode by mode (dist[m]: 0 uniform, 1 Zipf with
+ * exponent zipf_s over [0,dims[m])), rejecting duplicates, values N(0,1)
+ * (value_kind 1) or U[-1,1] (value_kind 0).  Keys in the default seq. */
+int lnto_gen_coords(int64_t *keys, double *vals, int64_t nnz, int order,
+                    const int64_t *dims, const int32_t *dist, double zipf_s,
+                    int value_kind, uint64_t seed);
+
+/* ------------------------------ index math ------------------------------ */
+int64_t lnto_sub2ind(const int64_t *sub, const int32_t *seq,
+                     const int64_t *dims, int order);
+void lnto_ind2sub(int64_t key, const int32_t *seq, const int64_t *dims,
+                  int order, int64_t *sub);
+void lnto_change_linidx(int64_t *keys, int64_t n, int order,
+                        const int64_t *dims, const int32_t *cur_seq,
+                        const int32_t *new_seq);
+
+/* -------------------------------- sorting ------------------------------- */
+void lnto_radix_sort_pairs(int64_t *keys, double *vals, int64_t n,
+                           uint64_t max_size);
+void lnto_introsort_pairs(int64_t *keys, double *vals, int64_t n);
+void lnto_sort_seq(int64_t *keys, double *vals, int64_t n, int order,
+                   const int64_t *dims, const int32_t *cur_seq,
+                   const int32_t *new_seq, int is_sorted);
+int64_t lnto_collect_duplicates(int64_t *keys, double *vals, int64_t n);
+
+/* -------------------------------- merging ------------------------------- */
+enum { LNTO_OP_ADD = 0, LNTO_OP_SUB = 1, LNTO_OP_HADAMARD = 2 };
+int64_t lnto_merge(const int64_t *ak, const double *av, int64_t na,
+                   const int64_t *bk, const double *bv, int64_t nb, int op,
+                   int64_t *ck, double *cv);
+int64_t lnto_outer_inter_count(const int64_t *ak, int64_t na,
+                               int64_t l_outer_size, const int64_t *bk,
+                               int64_t nb, int64_t r_outer_size);
+int64_t lnto_outer_inter_product(const int64_t *ak, const double *av,
+                                 int64_t na, int64_t l_outer_size,
+                                 const int64_t *bk, const double *bv,
+                                 int64_t nb, int64_t r_outer_size,
+                                 int64_t *ck, double *cv);
+
+/* --------------------------- lattice product ---------------------------- */
+int64_t lnto_lattice_nprod(const int64_t *ak, int64_t na, int64_t ah,
+                           int64_t aw, const int64_t *bk, int64_t nb,
+                           int64_t bh, int64_t bw);
+int64_t lnto_lattice_mult(const int64_t *ak, const double *av, int64_t na,
+                          int64_t ah, int64_t aw, int64_t ad,
+                          const int64_t *bk, const double *bv, int64_t nb,
+                          int64_t bh, int64_t bw, int64_t bd, int64_t *ck,
+                          double *cv, int64_t cap);
+/* Full MIA contraction A(labels_a) * B(labels_b) (lattice product path of
+ * MIA_Expr.h:864-925).  labels are ASCII mode labels; an upper-case label is an
+ * inter ("!") index.  Sorts A and B in place to their lattice orders.  Writes C
+ * in C's default seq over dims [A-outer, B-outer, inter]; returns nnz(C) or -1. */
+int64_t lnto_contract(int64_t *ak, double *av, int64_t na, int order_a,
+                      const int64_t *dims_a, const int32_t *seq_a,
+                      int a_sorted, const char *labels_a, int64_t *bk,
+                      double *bv, int64_t nb, int order_b,
+                      const int64_t *dims_b, const int32_t *seq_b,
+                      int b_sorted, const char *labels_b, int64_t *ck,
+                      double *cv, int64_t cap, int64_t *dims_c, int *order_c);
+int lnto_contract_plan(int order_a, const int64_t *dims_a, const char *labels_a,
+                       int order_b, const int64_t *dims_b, const char *labels_b,
+                       int32_t *seq_a, int32_t *seq_b, int64_t *lat_a,
+                       int64_t *lat_b, int64_t *dims_c, int *order_c);
+
+/* ------------------------------- comparison ----------------------------- */
+int lnto_is_equal_fuzzy(double a, double b, double prec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

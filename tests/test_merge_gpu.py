@@ -1,0 +1,160 @@
+"""GPU parity of the sparse merges (+, -, Hadamard) through the C-ABI against
+the CPU oracle: bit-exact keys and values."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+OPS = (0, 1, 2)
+
+
+def gpu(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def _bits_equal(a, b):
+    return a.shape == b.shape and np.array_equal(np.asarray(a).view(np.int64), np.asarray(b).view(np.int64))
+
+
+def _pair(orc, na, nb, overlap, domain, seed):
+    keys = orc.gen_distinct_keys(na + nb - overlap, domain, seed)
+    ak = np.sort(keys[:na])
+    bk = np.sort(np.concatenate([keys[:overlap], keys[na:]]))
+    av = orc.gen_uniform_vals(na, seed + 1)
+    bv = orc.gen_uniform_vals(len(bk), seed + 2)
+    return ak, av, bk, bv
+
+
+def _check(ctx, orc, ak, av, bk, bv):
+    for op in OPS:
+        ek, ev = orc.merge(ak, av, bk, bv, op)
+        ck, cv = ctx.merge(gpu(ak), gpu(av), gpu(bk), gpu(bv), op)
+        assert np.array_equal(host(ck), ek), op
+        assert _bits_equal(host(cv), ev), op
+
+
+@pytest.mark.parametrize(
+    "na,nb,ov",
+    [(0, 0, 0), (0, 7, 0), (7, 0, 0), (1, 1, 1), (1, 1, 0), (1024, 1024, 1024), (2048, 2048, 2048),
+     (3000, 5000, 1500), (5000, 3000, 3000), (4097, 1, 1), (1, 4097, 1), (100000, 100000, 50000),
+     (250000, 10, 5), (333333, 444444, 111111)],
+)
+def test_merge_matches_oracle(ctx, orc, na, nb, ov):
+    print("merge na=%d nb=%d overlap=%d" % (na, nb, ov), flush=True)
+    ak, av, bk, bv = _pair(orc, na, nb, ov, 1 << 40, na * 7 + nb)
+    _check(ctx, orc, ak, av, bk, bv)
+
+
+def test_merge_dense_interleaving(ctx, orc):
+    # Every key of a small domain present in A or B or both: equal keys fall on
+    # every possible tile and thread boundary.
+    rng = np.random.default_rng(3)
+    for trial in range(4):
+        dom = 20000 + trial * 777
+        where = rng.integers(0, 3, dom)  # 0: A only, 1: B only, 2: both
+        ak = np.flatnonzero(where != 1).astype(np.int64)
+        bk = np.flatnonzero(where != 0).astype(np.int64)
+        av = rng.uniform(-1, 1, len(ak))
+        bv = rng.uniform(-1, 1, len(bk))
+        _check(ctx, orc, ak, av, bk, bv)
+
+
+def test_merge_signed_zeros(ctx, orc):
+    ak = np.array([1, 4], dtype=np.int64)
+    av = np.array([-0.0, 0.0])
+    bk = np.array([2, 3, 4], dtype=np.int64)
+    bv = np.array([-0.0, 0.0, -0.0])
+    _check(ctx, orc, ak, av, bk, bv)
+
+
+def test_merge_c2_scaled(ctx, orc):
+    """c2 layout at 1/10 scale: 7.5M distinct keys over 2^32, A = first 5M,
+    B = first 2.5M + last 2.5M."""
+    import torch
+
+    n_shared, n_only = 2_500_000, 2_500_000
+    total = n_shared + 2 * n_only
+    ak = torch.empty(n_shared + n_only, dtype=torch.int64, device="cuda")
+    bk = torch.empty(n_shared + n_only, dtype=torch.int64, device="cuda")
+    ctx.testgen_distinct_keys(ak, 1 << 32, 1, offset=0)
+    ctx.testgen_distinct_keys(bk[:n_shared], 1 << 32, 1, offset=0)
+    ctx.testgen_distinct_keys(bk[n_shared:], 1 << 32, 1, offset=n_shared + n_only)
+    av = torch.empty(len(ak), dtype=torch.float64, device="cuda")
+    bv = torch.empty(len(bk), dtype=torch.float64, device="cuda")
+    ctx.testgen_uniform(av, 2)
+    ctx.testgen_uniform(bv, 3)
+    ctx.sort_pairs(ak, av, 1 << 32)
+    ctx.sort_pairs(bk, bv, 1 << 32)
+    hak, hav, hbk, hbv = host(ak), host(av), host(bk), host(bv)
+    for op in OPS:
+        ck, cv = ctx.merge(ak, av, bk, bv, op)
+        ek, ev = orc.merge(hak, hav, hbk, hbv, op)
+        assert len(ek) == (n_shared if op == 2 else total)
+        assert np.array_equal(host(ck), ek) and _bits_equal(host(cv), ev)
+
+
+def test_merge_library_allocates(ctx, orc):
+    import libnt_amd
+
+    L = libnt_amd.lib()
+    ak, av, bk, bv = _pair(orc, 3000, 4000, 1000, 1 << 30, 5)
+    ga, gav, gb, gbv = gpu(ak), gpu(av), gpu(bk), gpu(bv)
+    ck, cv = ctypes.c_void_p(), ctypes.c_void_p()
+    nc = ctypes.c_int64()
+    libnt_amd.check(L.lnt_merge(ctx.handle, ga.data_ptr(), gav.data_ptr(), len(ak), gb.data_ptr(),
+                                gbv.data_ptr(), len(bk), 0, ctypes.byref(ck), ctypes.byref(cv),
+                                ctypes.byref(nc), libnt_amd.MEM_DEVICE))
+    out_k = np.empty(nc.value, dtype=np.int64)
+    out_v = np.empty(nc.value, dtype=np.float64)
+    import torch
+
+    torch.cuda.synchronize()
+    tk = torch.empty(nc.value, dtype=torch.int64, device="cuda")
+    tv = torch.empty(nc.value, dtype=torch.float64, device="cuda")
+    # copy the library-owned buffers into tensors through a device-to-device memcpy
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipMemcpy(ctypes.c_void_p(tk.data_ptr()), ck, nc.value * 8, 3) == 0
+    assert hip.hipMemcpy(ctypes.c_void_p(tv.data_ptr()), cv, nc.value * 8, 3) == 0
+    out_k, out_v = host(tk), host(tv)
+    ek, ev = orc.merge(ak, av, bk, bv, 0)
+    assert np.array_equal(out_k, ek) and _bits_equal(out_v, ev)
+    libnt_amd.check(L.lnt_free(ctx.handle, ck, libnt_amd.MEM_DEVICE))
+    libnt_amd.check(L.lnt_free(ctx.handle, cv, libnt_amd.MEM_DEVICE))
+
+
+def test_merge_host_memory(ctx, orc):
+    ak, av, bk, bv = _pair(orc, 10000, 8000, 3000, 1 << 33, 8)
+    for op in OPS:
+        ck, cv = ctx.merge(ak, av, bk, bv, op)
+        ek, ev = orc.merge(ak, av, bk, bv, op)
+        assert np.array_equal(ck, ek) and _bits_equal(cv, ev)
+
+
+def test_merge_capacity_and_sortedness_errors(ctx, orc):
+    import libnt_amd
+    import torch
+
+    ak, av, bk, bv = _pair(orc, 100, 100, 10, 1 << 20, 9)
+    small = torch.empty(10, dtype=torch.int64, device="cuda")
+    smallv = torch.empty(10, dtype=torch.float64, device="cuda")
+    with pytest.raises(libnt_amd.LntError) as e:
+        ctx.merge_into(gpu(ak), gpu(av), gpu(bk), gpu(bv), 0, small, smallv)
+    assert e.value.status == 1
+    bad = ak.copy()
+    bad[[10, 11]] = bad[[11, 10]]
+    ctx.set_validation(2)
+    try:
+        with pytest.raises(libnt_amd.LntError) as e:
+            ctx.merge(gpu(bad), gpu(av), gpu(bk), gpu(bv), 0)
+        assert e.value.status == 4
+    finally:
+        ctx.set_validation(1)

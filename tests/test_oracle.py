@@ -1,0 +1,326 @@
+"""Pins the CPU oracle (oracle/lnt_oracle.c) against known answers, numpy
+library routines, dense brute force and invariants -- never against a retyped
+copy of its own formulas.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+
+# ------------------------------------------------------------- generators
+def test_splitmix64_known_answers(orc):
+    # Published first outputs of SplitMix64 with state 0 (Vigna's splitmix64.c).
+    assert orc.rand_u64(0, 0) == 0xE220A8397B1DCDAF
+    assert orc.rand_u64(0, 1) == 0x6E789E6AA1B965F4
+    assert orc.rand_u64(0, 2) == 0x06C45D188009454F
+
+
+def test_uniform_is_exact_53_bit_grid(orc):
+    for i in range(2000):
+        v = orc.uniform_pm1(12345, i)
+        assert -1.0 <= v < 1.0
+        m = (v + 1.0) * 2.0 ** 52
+        assert m == int(m) and int(m) == orc.rand_u64(12345, i) >> 11
+
+
+@pytest.mark.parametrize("bits", [2, 4, 6, 8, 10, 12, 14, 16])
+def test_feistel_is_a_bijection(orc, bits):
+    import ctypes
+
+    out = [orc.feistel(x, bits, 99) for x in range(1 << bits)]
+    assert sorted(out) == list(range(1 << bits))
+    assert out != list(range(1 << bits))
+
+
+@pytest.mark.parametrize("domain", [1, 2, 3, 5, 1000, 4097, 12345])
+def test_permute_index_is_a_bijection(orc, domain):
+    k = orc.gen_distinct_keys(domain, domain, 7)
+    assert sorted(k.tolist()) == list(range(domain))
+
+
+def test_distinct_keys_prefix(orc):
+    k = orc.gen_distinct_keys(10486, 1 << 20, 1)
+    assert len(np.unique(k)) == len(k) and k.min() >= 0 and k.max() < (1 << 20)
+    k2 = orc.gen_distinct_keys(100, 1 << 20, 1, offset=50)
+    assert np.array_equal(k[50:150], k2)
+
+
+def test_random_perm(orc):
+    for seed in range(50):
+        p = orc.random_perm(6, seed)
+        assert sorted(p.tolist()) == list(range(6))
+
+
+def test_gen_coords_distinct_and_skewed(orc):
+    dims = [64, 64, 32]
+    k, v = orc.gen_coords(5000, dims, [1, 0, 0], zipf_s=1.1, value_kind=1, seed=3)
+    assert len(np.unique(k)) == 5000 and k.min() >= 0 and k.max() < 64 * 64 * 32
+    i = k % 64
+    # Zipf(1.1) over 64 values puts far more mass on 0 than uniform (1/64).
+    assert (i == 0).mean() > 5.0 / 64
+    j = (k // 64) % 64
+    assert abs((j == 0).mean() - 1.0 / 64) < 0.02
+    assert abs(v.mean()) < 0.1 and 0.8 < v.std() < 1.2
+    k2, v2 = orc.gen_coords(5000, dims, [1, 0, 0], zipf_s=1.1, value_kind=1, seed=3)
+    assert np.array_equal(k, k2) and np.array_equal(v, v2)
+
+
+# ------------------------------------------------------------- index math
+def _np_relin(keys, dims, cur, new):
+    dims = np.asarray(dims)
+    subs_r = np.unravel_index(keys, [int(dims[m]) for m in cur], order="F")
+    sub = [None] * len(dims)
+    for t, m in enumerate(cur):
+        sub[m] = subs_r[t]
+    return np.ravel_multi_index([sub[m] for m in new], [int(dims[m]) for m in new], order="F")
+
+
+def test_change_linidx_exhaustive_small(orc):
+    dims = [2, 3, 4, 5]
+    keys = np.arange(120, dtype=np.int64)
+    perms = list(itertools.permutations(range(4)))
+    for cur in perms[::3]:
+        for new in perms:
+            got = orc.change_linidx(keys, dims, cur, new)
+            assert np.array_equal(got, _np_relin(keys, dims, cur, new)), (cur, new)
+
+
+def test_change_linidx_round_trip_large(orc):
+    rng = np.random.default_rng(0)
+    dims = [4096, 4096, 512, 64, 3]  # 2^42 * 3 cells
+    total = int(np.prod(dims, dtype=object))
+    keys = np.array([int(x) for x in rng.integers(0, total, 5000, dtype=np.int64)], dtype=np.int64)
+    cur, new = [0, 1, 2, 3, 4], [3, 0, 4, 2, 1]
+    fwd = orc.change_linidx(keys, dims, cur, new)
+    # Python big-int closed form.
+    for key, f in zip(keys[:200].tolist(), fwd[:200].tolist()):
+        sub = [0] * 5
+        k = key
+        for m in cur:
+            sub[m] = k % dims[m]
+            k //= dims[m]
+        e, mult = 0, 1
+        for m in new:
+            e += sub[m] * mult
+            mult *= dims[m]
+        assert e == f
+    back = orc.change_linidx(fwd, dims, new, cur)
+    assert np.array_equal(back, keys)
+
+
+def test_ind2sub_sub2ind(orc):
+    dims = [3, 5, 7]
+    seq = [2, 0, 1]
+    for key in range(105):
+        sub = orc.ind2sub(key, dims, seq)
+        assert orc.sub2ind(sub, dims, seq) == key
+        assert np.ravel_multi_index([sub[m] for m in seq], [dims[m] for m in seq], order="F") == key
+
+
+# ---------------------------------------------------------------- sorting
+@pytest.mark.parametrize("n", [0, 1, 2, 49, 50, 51, 2999, 3000, 3001, 20000, 200000])
+@pytest.mark.parametrize("bound", [1 << 20, (1 << 33) + 17, 1 << 62])
+def test_radix_sort_matches_argsort(orc, n, bound):
+    n = min(n, bound)
+    keys = orc.gen_distinct_keys(n, bound, n + 11)
+    vals = orc.gen_uniform_vals(n, 5)
+    k, v = orc.radix_sort_pairs(keys, vals, bound)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order])
+    assert np.array_equal(v.view(np.int64), vals[order].view(np.int64))
+
+
+def test_radix_sort_with_duplicates(orc):
+    rng = np.random.default_rng(1)
+    keys = rng.integers(0, 700, 50000).astype(np.int64)
+    vals = rng.standard_normal(50000)
+    k, v = orc.radix_sort_pairs(keys, vals, 700)
+    assert np.all(np.diff(k) >= 0)
+    # (key, value) multiset preserved
+    a = sorted(zip(keys.tolist(), vals.tolist()))
+    b = sorted(zip(k.tolist(), v.tolist()))
+    assert a == b
+
+
+def test_introsort(orc):
+    keys = orc.gen_distinct_keys(5000, 1 << 30, 4)
+    vals = orc.gen_uniform_vals(5000, 4)
+    k, v = orc.introsort_pairs(keys, vals)
+    o = np.argsort(keys)
+    assert np.array_equal(k, keys[o]) and np.array_equal(v, vals[o])
+
+
+@pytest.mark.parametrize("n", [100, 2999, 3000, 20000])
+@pytest.mark.parametrize("is_sorted", [False, True])
+def test_sort_seq_is_relinearise_then_sort(orc, n, is_sorted):
+    """sparse_mia_functions.cpp:100-202: sort(seq) == change_linIdx_sequence(seq)+sort()."""
+    dims = [32, 32, 32, 32]
+    keys = orc.gen_distinct_keys(n, 1 << 20, 9)
+    vals = orc.gen_uniform_vals(n, 9)
+    cur = [0, 1, 2, 3]
+    if is_sorted:
+        keys, vals = orc.radix_sort_pairs(keys, vals, 1 << 20)
+    for new in ([2, 0, 3, 1], [3, 2, 1, 0], [0, 1, 2, 3], [1, 0, 2, 3]):
+        k, v = orc.sort_seq(keys, vals, dims, cur, new, is_sorted)
+        rk = _np_relin(keys, dims, cur, new)
+        o = np.argsort(rk)
+        assert np.array_equal(k, rk[o]) and np.array_equal(v, vals[o])
+
+
+def test_collect_duplicates(orc):
+    keys = np.array([1, 1, 2, 5, 5, 5, 9], dtype=np.int64)
+    vals = np.array([1.0, 2.0, 3.0, 1.0, 1.0, 1.0, -4.0])
+    k, v = orc.collect_duplicates(keys, vals)
+    assert k.tolist() == [1, 2, 5, 9] and v.tolist() == [3.0, 3.0, 3.0, -4.0]
+
+
+# ---------------------------------------------------------------- merging
+def _dense(keys, vals, size):
+    d = np.zeros(size)
+    d[keys] = vals
+    return d
+
+
+def _mk_pair(orc, na, nb, overlap, domain, seed):
+    """Sorted A, B with `overlap` shared keys (distinct keys from one bijection)."""
+    keys = orc.gen_distinct_keys(na + nb - overlap, domain, seed)
+    ak = np.sort(keys[:na])
+    bk = np.sort(np.concatenate([keys[:overlap], keys[na:]]))
+    av = orc.gen_uniform_vals(na, seed + 1)
+    bv = orc.gen_uniform_vals(len(bk), seed + 2)
+    return ak, av, bk, bv
+
+
+@pytest.mark.parametrize("na,nb,ov", [(0, 0, 0), (0, 50, 0), (50, 0, 0), (300, 200, 100), (1000, 1000, 1000), (5000, 3000, 0)])
+def test_merge_union_and_hadamard_dense(orc, na, nb, ov):
+    domain = 1 << 16
+    ak, av, bk, bv = _mk_pair(orc, na, nb, ov, domain, 21)
+    A, B = _dense(ak, av, domain), _dense(bk, bv, domain)
+    supp_union = np.union1d(ak, bk)
+    for op, fn in ((orc.OP_ADD, np.add), (orc.OP_SUB, np.subtract)):
+        ck, cv = orc.merge(ak, av, bk, bv, op)
+        assert np.array_equal(ck, supp_union)
+        assert np.array_equal(cv, fn(A, B)[ck])
+    ck, cv = orc.merge(ak, av, bk, bv, orc.OP_HADAMARD)
+    assert np.array_equal(ck, np.intersect1d(ak, bk))
+    assert np.array_equal(cv, (A * B)[ck])
+
+
+def test_merge_signed_zero_rules(orc):
+    # B-only entries are op(0, b) (FunctionUtil.h:393,405): 0 + -0 = +0, 0 - +0 = +0,
+    # while A-only entries are copied unchanged.
+    ak = np.array([1], dtype=np.int64)
+    av = np.array([-0.0])
+    bk = np.array([2, 3], dtype=np.int64)
+    bv = np.array([-0.0, 0.0])
+    ck, cv = orc.merge(ak, av, bk, bv, orc.OP_ADD)
+    assert ck.tolist() == [1, 2, 3]
+    assert np.signbit(cv).tolist() == [True, False, False]
+    ck, cv = orc.merge(ak, av, bk, bv, orc.OP_SUB)
+    # 0 - (+0) is +0, where a negation (-b) would give -0.
+    assert np.signbit(cv).tolist() == [True, False, False]
+
+
+def test_outer_inter_product_dense(orc):
+    rng = np.random.default_rng(2)
+    L, R, T = 6, 5, 7
+    A = np.where(rng.random((L, T)) < 0.4, rng.standard_normal((L, T)), 0.0)
+    B = np.where(rng.random((R, T)) < 0.4, rng.standard_normal((R, T)), 0.0)
+    ak = np.flatnonzero(A.T.ravel() != 0).astype(np.int64)  # key = l + L*t
+    av = A.T.ravel()[ak]
+    bk = np.flatnonzero(B.T.ravel() != 0).astype(np.int64)
+    bv = B.T.ravel()[bk]
+    ck, cv = orc.outer_inter_product(ak, av, L, bk, bv, R)
+    C = np.einsum("it,jt->ijt", A, B)
+    Cf = C.transpose(2, 1, 0).ravel()  # key = i + L*j + L*R*t
+    assert np.array_equal(ck, np.flatnonzero(Cf != 0))
+    assert np.array_equal(cv, Cf[ck])
+    assert np.all(np.diff(ck) > 0)
+
+
+# -------------------------------------------------------- lattice product
+def _lattice_from_dense(D):
+    h, w, d = D.shape
+    flat = D.transpose(2, 1, 0).ravel()  # key = r + h*c + h*w*t
+    k = np.flatnonzero(flat != 0).astype(np.int64)
+    return k, flat[k]
+
+
+@pytest.mark.parametrize("shape", [(20, 30, 1), (7, 40, 3), (1, 1, 1), (33, 1, 2), (40, 25, 4)])
+@pytest.mark.parametrize("density", [0.05, 0.3])
+def test_lattice_mult_dense(orc, shape, density):
+    rng = np.random.default_rng(hash((shape, density)) & 0xFFFF)
+    h, w, d = shape
+    bw = 17
+    A = np.where(rng.random((h, w, d)) < density, rng.uniform(-1, 1, (h, w, d)), 0.0)
+    B = np.where(rng.random((w, bw, d)) < density, rng.uniform(-1, 1, (w, bw, d)), 0.0)
+    ak, av = _lattice_from_dense(A)
+    bk, bv = _lattice_from_dense(B)
+    ck, cv = orc.lattice_mult(ak, av, (h, w, d), bk, bv, (w, bw, d))
+    C = np.einsum("rkt,kct->rct", A, B)
+    S = np.einsum("rkt,kct->rct", (A != 0).astype(np.int64), (B != 0).astype(np.int64))
+    flatC = C.transpose(2, 1, 0).ravel()
+    flatS = S.transpose(2, 1, 0).ravel()
+    assert np.array_equal(ck, np.flatnonzero(flatS))  # structural nonzeros, sorted
+    assert orc.allclose_fuzzy(cv, flatC[ck], 1e-12)
+    nprod = int(S.sum())
+    assert orc.lattice_nprod(ak, (h, w, d), bk, (w, bw, d)) == nprod
+
+
+def _dense_mia(keys, vals, dims):
+    D = np.zeros(int(np.prod(dims)))
+    D[keys] = vals
+    return D.reshape(dims, order="F")
+
+
+@pytest.mark.parametrize(
+    "la,lb,dims_a,dims_b",
+    [
+        ("ijkl", "klmn", [5, 4, 6, 3], [6, 3, 4, 5]),  # the c1 pattern
+        ("ijk", "klm", [7, 6, 5], [5, 8, 3]),  # the c3 pattern
+        ("kij", "jlk", [4, 5, 6], [6, 7, 4]),  # non-consecutive
+        ("iJk", "kJm", [5, 3, 4], [4, 3, 6]),  # one inter index
+        ("Ij", "jI", [6, 5], [5, 6]),  # inter + inner
+        ("ij", "kl", [3, 4], [5, 2]),  # pure outer
+        ("IJ", "IJ", [4, 5], [4, 5]),  # pure Hadamard
+    ],
+)
+def test_contract_dense(orc, la, lb, dims_a, dims_b):
+    rng = np.random.default_rng(len(la) * 7 + len(lb))
+    na = max(1, int(0.3 * np.prod(dims_a)))
+    nb = max(1, int(0.3 * np.prod(dims_b)))
+    ak = orc.gen_distinct_keys(na, int(np.prod(dims_a)), 5)
+    av = orc.gen_uniform_vals(na, 6)
+    bk = orc.gen_distinct_keys(nb, int(np.prod(dims_b)), 7)
+    bv = orc.gen_uniform_vals(nb, 8)
+    A = _dense_mia(ak, av, dims_a)
+    B = _dense_mia(bk, bv, dims_b)
+    ck, cv, dims_c = orc.contract(ak, av, dims_a, list(range(len(dims_a))), False, la,
+                                  bk, bv, dims_b, list(range(len(dims_b))), False, lb)
+    # C's modes: [A outer, B outer, inter] (MIA_Expr.h:493-501)
+    a_out = [c for c in la if c not in lb]
+    b_out = [c for c in lb if c not in la]
+    inter = [c for c in la if c.isupper()]
+    out = "".join(a_out + b_out + inter).lower()
+    C = np.einsum(la.lower() + "," + lb.lower() + "->" + out, A, B)
+    S = np.einsum(la.lower() + "," + lb.lower() + "->" + out, (A != 0) * 1, (B != 0) * 1)
+    assert list(dims_c) == list(C.shape)
+    flatC, flatS = C.ravel(order="F"), S.ravel(order="F")
+    assert np.array_equal(ck, np.flatnonzero(flatS))
+    assert orc.allclose_fuzzy(cv, flatC[ck], 1e-12)
+
+
+def test_contract_plan_rejects_bad_labels(orc):
+    with pytest.raises(ValueError):
+        orc.contract_plan([2, 3], "ii", [3, 2], "ij")  # repeated label
+    with pytest.raises(ValueError):
+        orc.contract_plan([2, 3], "iJ", [3, 2], "ik")  # inter label on one side only
+    with pytest.raises(ValueError):
+        orc.contract_plan([2, 3], "ij", [4, 2], "jk")  # mismatched contracted dims
+
+
+def test_is_equal_fuzzy(orc):
+    assert orc.is_equal_fuzzy(0.5, 0.5 + 1e-13, 1e-12)
+    assert not orc.is_equal_fuzzy(0.5, 0.5 + 1e-11, 1e-12)
+    assert orc.is_equal_fuzzy(1e6, 1e6 * (1 + 5e-13), 1e-12)  # relative above 1
+    assert not orc.is_equal_fuzzy(1e6, 1e6 * (1 + 5e-12), 1e-12)

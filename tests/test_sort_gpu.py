@@ -1,0 +1,221 @@
+"""GPU parity of relinearisation, sort and change_sort_order (through the
+C-ABI) against the CPU oracle: bit-exact keys and values."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def _bits_equal(a, b):
+    return a.shape == b.shape and np.array_equal(np.asarray(a).view(np.int64), np.asarray(b).view(np.int64))
+
+
+@pytest.mark.parametrize("domain", [1 << 20, 1 << 32, 12345, (1 << 61) + 3])
+def test_testgen_matches_oracle(ctx, orc, domain):
+    import torch
+
+    n = min(100000, domain)
+    k = torch.empty(n, dtype=torch.int64, device="cuda")
+    v = torch.empty(n, dtype=torch.float64, device="cuda")
+    ctx.testgen_distinct_keys(k, domain, 77, offset=0)
+    ctx.testgen_uniform(v, 78, offset=5)
+    assert np.array_equal(host(k), orc.gen_distinct_keys(n, domain, 77))
+    assert _bits_equal(host(v), orc.gen_uniform_vals(n, 78, offset=5))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 1000, 4095, 4096, 4097, 10486, 65541, 1000003])
+@pytest.mark.parametrize("bound", [1 << 20, (1 << 33) + 17, 1 << 62])
+def test_sort_pairs_matches_oracle(ctx, orc, n, bound):
+    print("sort_pairs n=%d bound=%d" % (n, bound), flush=True)
+    keys = orc.gen_distinct_keys(n, bound, n ^ 0x55)
+    vals = orc.gen_uniform_vals(n, n + 1)
+    ek, ev = orc.radix_sort_pairs(keys, vals, bound)
+    k, v = gpu(keys), gpu(vals)
+    ctx.sort_pairs(k, v, bound)
+    assert np.array_equal(host(k), ek)
+    assert _bits_equal(host(v), ev)
+
+
+@pytest.mark.parametrize("bound", [2, 3, 255, 256, 257, 65536, 1 << 24])
+def test_sort_small_bounds(ctx, orc, bound):
+    n = min(bound, 5000)
+    keys = orc.gen_distinct_keys(n, bound, 3)
+    vals = orc.gen_uniform_vals(n, 4)
+    ek, ev = orc.radix_sort_pairs(keys, vals, bound)
+    k, v = gpu(keys), gpu(vals)
+    ctx.sort_pairs(k, v, bound)
+    assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
+
+
+def test_sort_empty(ctx):
+    import torch
+
+    k = torch.empty(0, dtype=torch.int64, device="cuda")
+    v = torch.empty(0, dtype=torch.float64, device="cuda")
+    ctx.sort_pairs(k, v, 100)
+    ctx.change_sort_order(k, v, [4, 5], [0, 1], [1, 0], True)
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_change_linidx_matches_oracle(ctx, orc, case):
+    rng = np.random.default_rng(case)
+    order = int(rng.integers(1, 7))
+    dims = [int(x) for x in rng.integers(1, 60, order)]
+    total = int(np.prod(dims))
+    n = min(total, 50000)
+    keys = orc.gen_distinct_keys(n, total, case)
+    cur = rng.permutation(order).astype(np.int32)
+    new = rng.permutation(order).astype(np.int32)
+    expect = orc.change_linidx(keys, dims, cur, new)
+    k = gpu(keys)
+    ctx.change_linidx(k, dims, cur, new)
+    assert np.array_equal(host(k), expect)
+
+
+def test_change_linidx_large_dims(ctx, orc):
+    dims = [4096, 4096, 512, 64, 3]
+    total = 4096 * 4096 * 512 * 64 * 3
+    keys = orc.gen_distinct_keys(200000, total, 5)
+    cur, new = [0, 1, 2, 3, 4], [3, 0, 4, 2, 1]
+    k = gpu(keys)
+    ctx.change_linidx(k, dims, cur, new)
+    assert np.array_equal(host(k), orc.change_linidx(keys, dims, cur, new))
+
+
+def _c1_operand(orc, seed):
+    # 32^4 cells, 1% density: 10,486 distinct uniform coordinates, U[-1,1).
+    keys = orc.gen_distinct_keys(10486, 1 << 20, seed)
+    vals = orc.gen_uniform_vals(10486, seed + 1000)
+    return keys, vals
+
+
+@pytest.mark.parametrize("is_sorted", [False, True])
+def test_change_sort_order_c1(ctx, orc, is_sorted):
+    dims = [32, 32, 32, 32]
+    ident = [0, 1, 2, 3]
+    for seed in (1, 2):
+        keys, vals = _c1_operand(orc, seed)
+        if is_sorted:
+            keys, vals = orc.radix_sort_pairs(keys, vals, 1 << 20)
+        perm = orc.random_perm(4, seed + 500)
+        ek, ev = orc.sort_seq(keys, vals, dims, ident, perm, is_sorted)
+        k, v = gpu(keys), gpu(vals)
+        ctx.change_sort_order(k, v, dims, ident, perm, is_sorted)
+        assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_change_sort_order_various(ctx, orc, case):
+    rng = np.random.default_rng(100 + case)
+    order = int(rng.integers(2, 7))
+    dims = [int(x) for x in rng.integers(2, 200, order)]
+    total = int(np.prod(dims, dtype=object))
+    while total >= (1 << 62):
+        dims[int(np.argmax(dims))] //= 2
+        total = int(np.prod(dims, dtype=object))
+    n = int(min(total // 2, rng.integers(1, 300000)))
+    keys = orc.gen_distinct_keys(n, total, case)
+    vals = orc.gen_uniform_vals(n, case)
+    cur = rng.permutation(order).astype(np.int32)
+    new = rng.permutation(order).astype(np.int32)
+    ek, ev = orc.sort_seq(keys, vals, dims, cur, new, False)
+    k, v = gpu(keys), gpu(vals)
+    ctx.change_sort_order(k, v, dims, cur, new, False)
+    assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
+
+
+def test_sort_repeat_is_bitwise_identical(ctx, orc):
+    """Look-back or ranking races would show up as run-to-run differences."""
+    import torch
+
+    n = 3_000_000
+    k0 = torch.empty(n, dtype=torch.int64, device="cuda")
+    v0 = torch.empty(n, dtype=torch.float64, device="cuda")
+    ctx.testgen_distinct_keys(k0, 1 << 40, 9)
+    ctx.testgen_uniform(v0, 10)
+    outs = []
+    for _ in range(3):
+        k, v = k0.clone(), v0.clone()
+        ctx.sort_pairs(k, v, 1 << 40)
+        outs.append((k, v))
+    torch.cuda.synchronize()
+    for k, v in outs[1:]:
+        assert torch.equal(k, outs[0][0]) and torch.equal(v.view(torch.int64), outs[0][1].view(torch.int64))
+    assert bool((outs[0][0][1:] > outs[0][0][:-1]).all())
+
+
+def test_key_range_error_leaves_input_untouched(ctx, orc):
+    import libnt_amd
+
+    keys = orc.gen_distinct_keys(5000, 1 << 20, 1)
+    keys[1234] = 1 << 20  # one key outside [0, 32^4)
+    vals = orc.gen_uniform_vals(5000, 2)
+    k, v = gpu(keys), gpu(vals)
+    with pytest.raises(libnt_amd.LntError) as e:
+        ctx.change_sort_order(k, v, [32, 32, 32, 32], [0, 1, 2, 3], [1, 0, 3, 2], False)
+    assert e.value.status == 5
+    assert np.array_equal(host(k), keys) and _bits_equal(host(v), vals)
+    keys[1234] = -7
+    k = gpu(keys)
+    with pytest.raises(libnt_amd.LntError):
+        ctx.sort_pairs(k, v, 1 << 20)
+    assert np.array_equal(host(k), keys)
+
+
+def test_invalid_arguments(ctx):
+    import libnt_amd
+    import torch
+
+    k = torch.zeros(4, dtype=torch.int64, device="cuda")
+    v = torch.zeros(4, dtype=torch.float64, device="cuda")
+    for dims, cur, new in (([2, 2], [0, 0], [0, 1]), ([2, 0], [0, 1], [1, 0]), ([2, 2], [0, 2], [1, 0])):
+        with pytest.raises(libnt_amd.LntError) as e:
+            ctx.change_sort_order(k, v, dims, cur, new, False)
+        assert e.value.status == 1
+    with pytest.raises(libnt_amd.LntError) as e:
+        ctx.change_sort_order(k, v, [1 << 32, 1 << 32], [0, 1], [1, 0], False)
+    assert e.value.status == 6
+
+
+def test_host_memory_mode(ctx, orc):
+    keys, vals = _c1_operand(orc, 7)
+    dims, ident, perm = [32, 32, 32, 32], [0, 1, 2, 3], [3, 1, 0, 2]
+    ek, ev = orc.sort_seq(keys, vals, dims, ident, perm, False)
+    k, v = keys.copy(), vals.copy()
+    ctx.change_sort_order(k, v, dims, ident, perm, False)
+    assert np.array_equal(k, ek) and _bits_equal(v, ev)
+    k2 = keys.copy()
+    ctx.change_linidx(k2, dims, ident, perm)
+    assert np.array_equal(k2, orc.change_linidx(keys, dims, ident, perm))
+
+
+def test_c2_scale_change_sort_order_against_torch_sort(ctx):
+    """50M keys over 256^4: the result equals torch.sort (an independent
+    library sort) of the relinearised keys, with values following."""
+    import torch
+
+    n = 50_000_000
+    dims, ident, perm = [256] * 4, [0, 1, 2, 3], [2, 0, 3, 1]
+    k = torch.empty(n, dtype=torch.int64, device="cuda")
+    v = torch.empty(n, dtype=torch.float64, device="cuda")
+    ctx.testgen_distinct_keys(k, 1 << 32, 11)
+    ctx.testgen_uniform(v, 12)
+    ctx.sort_pairs(k, v, 1 << 32)  # sorted in the default order
+    rk = k.clone()
+    ctx.change_linidx(rk, dims, ident, perm)
+    ref_k, idx = torch.sort(rk)
+    ref_v = v[idx]
+    ctx.change_sort_order(k, v, dims, ident, perm, True)
+    torch.cuda.synchronize()
+    assert torch.equal(k, ref_k)
+    assert torch.equal(v.view(torch.int64), ref_v.view(torch.int64))
