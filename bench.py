@@ -177,8 +177,8 @@ def main():
         "dtype": "int64 keys + f64 values",
         "data": "synthetic (counter-based generator, fixed seeds)",
         "config": {
-            "workload": "sort/merge at scale: order-4 dims 256^4, 50M nnz each, 50% overlap, "
-                        "change_sort_order to perm %s, then A+B (75M out) and A∘B (25M out)" % perm,
+            "workload": "sort/merge at scale: order-4 dims 256^4, 50M nnz each, 50%% overlap, "
+                        "change_sort_order to perm %s, then A+B (75M out) and A∘B (25M out)" % (perm,),
             "nnz_a": na, "nnz_b": nb, "union_nnz": counts["union"], "hadamard_nnz": counts["hadamard"],
             "parallelism": "single GPU",
         },

@@ -146,6 +146,44 @@ int lnt_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
               int64_t *nc, int mem);
 int lnt_free(lnt_context ctx, void *ptr, int mem);
 
+/* ------------------------------------------- lattice product (a8-a11) -- */
+typedef enum lnt_spgemm_strategy {
+  LNT_SPGEMM_AUTO = 0, /* library choice (heuristic or learned dispatch) */
+  LNT_SPGEMM_SORT = 1, /* expand, sort rows, sum runs (deterministic sums) */
+  LNT_SPGEMM_HASH = 2  /* LDS hash accumulation for short columns */
+} lnt_spgemm_strategy;
+
+/* A lattice is an order-3 view of a tensor: dims {h, w, d} (rows, columns,
+ * tabs), key = row + h*col + h*w*tab (multi_index_array/SparseLatticeBase.h:
+ * 867-877).  A is h x w x d, B is w x bw x d, both sorted ascending by key.
+ * *nprod receives the number of scalar products sum_{(k,j,t) in B}
+ * nnz(A(:,k,t)) -- the bound on nnz(C) and half the flop count. */
+int lnt_lattice_nprod(lnt_context ctx, const int64_t *a_keys, int64_t na,
+                      const int64_t a_dims[3], const int64_t *b_keys,
+                      int64_t nb, const int64_t b_dims[3], int64_t *nprod,
+                      int mem);
+/* C = A * B tab by tab (h x bw x d), sorted ascending; equal rows of a
+ * column are summed, exact zeros kept.  Replaces SparseLatticeBase::operator*
+ * -> csc_no_accum<true> (multi_index_array/SparseLatticeBase.h:1571-1576,
+ * 1854-2031) and SparseLatticeMultMex (MATLAB/mex/sparselattice_mul.cpp:
+ * 14-136).  Keys are bit-identical to the reference; values agree to 1e-12
+ * (only the summation order of duplicate rows differs).  capacity >= nnz(C)
+ * (lnt_lattice_nprod bounds it); at most 2^32 products per call. */
+int lnt_lattice_mult_into(lnt_context ctx, const int64_t *a_keys,
+                          const double *a_vals, int64_t na,
+                          const int64_t a_dims[3], const int64_t *b_keys,
+                          const double *b_vals, int64_t nb,
+                          const int64_t b_dims[3], int strategy,
+                          int64_t *c_keys, double *c_vals, int64_t capacity,
+                          int64_t *nc, int mem);
+/* As lnt_lattice_mult_into, with C allocated by the library at its exact size
+ * (release with lnt_free). */
+int lnt_lattice_mult(lnt_context ctx, const int64_t *a_keys,
+                     const double *a_vals, int64_t na, const int64_t a_dims[3],
+                     const int64_t *b_keys, const double *b_vals, int64_t nb,
+                     const int64_t b_dims[3], int strategy, int64_t **c_keys,
+                     double **c_vals, int64_t *nc, int mem);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,7 @@ STATUS_NAMES = {
 }
 MEM_HOST, MEM_DEVICE = 0, 1
 OP_ADD, OP_SUB, OP_HADAMARD = 0, 1, 2
+SPGEMM_AUTO, SPGEMM_SORT, SPGEMM_HASH = 0, 1, 2
 
 _lib = None
 _vp = ctypes.c_void_p
@@ -55,6 +56,9 @@ SIGNATURES = {
     "lnt_merge_into": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _vp, _i64, _i64p, _int],
     "lnt_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_free": [_vp, _vp, _int],
+    "lnt_lattice_nprod": [_vp, _vp, _i64, _i64p, _vp, _i64, _i64p, _i64p, _int],
+    "lnt_lattice_mult_into": [_vp, _vp, _vp, _i64, _i64p, _vp, _vp, _i64, _i64p, _int, _vp, _vp, _i64, _i64p, _int],
+    "lnt_lattice_mult": [_vp, _vp, _vp, _i64, _i64p, _vp, _vp, _i64, _i64p, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_testgen_distinct_keys": [_vp, _vp, _i64, _u64, _u64, _u64],
     "lnt_testgen_uniform": [_vp, _vp, _i64, _u64, _u64],
 }
@@ -222,6 +226,46 @@ class Context:
             ck = torch.empty(max(cap, 1), dtype=torch.int64, device=ak.device)
             cv = torch.empty(max(cap, 1), dtype=torch.float64, device=ak.device)
         n = self.merge_into(ak, av, bk, bv, op, ck, cv)
+        return ck[:n], cv[:n]
+
+    def lattice_nprod(self, ak, a_dims, bk, b_dims) -> int:
+        """Scalar products of A*B (bound on nnz(C); flops = 2*nprod)."""
+        akp, mem, na = _ptr_mem(ak)
+        bkp, memb, nb = _ptr_mem(bk)
+        ad, adp = _arr_i64(a_dims)
+        bd, bdp = _arr_i64(b_dims)
+        out = ctypes.c_int64()
+        check(lib().lnt_lattice_nprod(self._h, akp, na, adp, bkp, nb, bdp, ctypes.byref(out), mem))
+        return out.value
+
+    def lattice_mult_into(self, ak, av, a_dims, bk, bv, b_dims, ck, cv, strategy=SPGEMM_AUTO) -> int:
+        akp, mem, na = _ptr_mem(ak)
+        avp, _, _ = _ptr_mem(av)
+        bkp, memb, nb = _ptr_mem(bk)
+        bvp, _, _ = _ptr_mem(bv)
+        ckp, memc, cap = _ptr_mem(ck)
+        cvp, _, capv = _ptr_mem(cv)
+        if not (mem == memb == memc):
+            raise ValueError("all arrays must live in the same memory kind")
+        ad, adp = _arr_i64(a_dims)
+        bd, bdp = _arr_i64(b_dims)
+        nc = ctypes.c_int64()
+        check(lib().lnt_lattice_mult_into(self._h, akp, avp, na, adp, bkp, bvp, nb, bdp, strategy,
+                                          ckp, cvp, min(cap, capv), ctypes.byref(nc), mem))
+        return nc.value
+
+    def lattice_mult(self, ak, av, a_dims, bk, bv, b_dims, strategy=SPGEMM_AUTO):
+        """C = A*B per tab; returns (keys, values) of C's kind, sorted."""
+        cap = max(self.lattice_nprod(ak, a_dims, bk, b_dims), 1)
+        if isinstance(ak, np.ndarray):
+            ck = np.empty(cap, dtype=np.int64)
+            cv = np.empty(cap, dtype=np.float64)
+        else:
+            import torch
+
+            ck = torch.empty(cap, dtype=torch.int64, device=ak.device)
+            cv = torch.empty(cap, dtype=torch.float64, device=ak.device)
+        n = self.lattice_mult_into(ak, av, a_dims, bk, bv, b_dims, ck, cv, strategy)
         return ck[:n], cv[:n]
 
     # ---------------------------------------------------- synthetic inputs

@@ -89,7 +89,8 @@ int make_relin(int order, const int64_t *dims, const int32_t *cur_seq,
   return LNT_OK;
 }
 
-static const char *kProfNames[kProfKinds] = {"histogram", "onesweep", "partition", "merge", "relin"};
+static const char *kProfNames[kProfKinds] = {"histogram", "onesweep", "partition", "merge", "relin",
+                                             "lattice_mult"};
 
 static hipEvent_t take_event(lnt_context ctx) {
   if (!ctx->event_pool.empty()) {
@@ -232,7 +233,8 @@ int lnt_context_destroy(lnt_context ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
-                          &ctx->host_tmp_b, &ctx->host_tmp_c})
+                          &ctx->host_tmp_b, &ctx->host_tmp_c, &ctx->spg_ws, &ctx->spg_tmp,
+                          &ctx->spg_big})
     release(*b);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   prof_clear(ctx);
@@ -263,7 +265,10 @@ int lnt_context_profile(lnt_context ctx, const char *kernel, int64_t *launches,
   for (auto &r : ctx->prof) {
     if (r.kind != kind) continue;
     float t = 0;
-    LNT_HIP(hipEventElapsedTime(&t, r.start, r.stop));
+    if (hipEventElapsedTime(&t, r.start, r.stop) != hipSuccess) {
+      (void)hipGetLastError();  // a launch that failed before its stop event
+      continue;
+    }
     ++n;
     ms += t;
     b += r.bytes;
@@ -290,7 +295,8 @@ int lnt_context_trim(lnt_context ctx) {
   LNT_CTX(ctx);
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
-                          &ctx->aux, &ctx->host_tmp_a, &ctx->host_tmp_b, &ctx->host_tmp_c})
+                          &ctx->aux, &ctx->host_tmp_a, &ctx->host_tmp_b, &ctx->host_tmp_c,
+                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big})
     release(*b);
   return LNT_OK;
 }
@@ -465,6 +471,136 @@ int lnt_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
   *c_keys = (int64_t *)k;
   *c_vals = (double *)v;
   return LNT_OK;
+}
+
+static int lattice_common(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                          const int64_t a_dims[3], const int64_t *b_keys, const double *b_vals, int64_t nb,
+                          const int64_t b_dims[3], int strategy, int64_t *c_keys, double *c_vals,
+                          int64_t capacity, int64_t *nc, int64_t **c_keys_alloc, double **c_vals_alloc,
+                          int64_t *nprod_only, int mem) {
+  LNT_TRY(check_mem(mem));
+  if (!a_dims || !b_dims) return fail(LNT_ERR_INVALID_ARGUMENT, "null lattice dimensions");
+  if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  if (na >= (int64_t(1) << 32) || nb >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "operands must hold < 2^32 entries");
+  if (strategy < LNT_SPGEMM_AUTO || strategy > LNT_SPGEMM_HASH) return fail(LNT_ERR_INVALID_ARGUMENT, "unknown strategy");
+  if ((na && !a_keys) || (nb && !b_keys) || (!nprod_only && ((na && !a_vals) || (nb && !b_vals))))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "null array");
+  const uint64_t *ak = (const uint64_t *)a_keys, *bk = (const uint64_t *)b_keys;
+  const double *av = a_vals, *bv = b_vals;
+  Staging st;
+  if (mem == LNT_MEM_HOST) {
+    size_t sa = (size_t)na * 8, sb = (size_t)nb * 8;
+    LNT_TRY(ensure(st.b[0], sa ? sa : 8));
+    LNT_TRY(ensure(st.b[1], sa ? sa : 8));
+    LNT_TRY(ensure(st.b[2], sb ? sb : 8));
+    LNT_TRY(ensure(st.b[3], sb ? sb : 8));
+    if (sa) {
+      LNT_HIP(hipMemcpyAsync(st.b[0].ptr, a_keys, sa, hipMemcpyHostToDevice, ctx->stream));
+      if (a_vals) LNT_HIP(hipMemcpyAsync(st.b[1].ptr, a_vals, sa, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (sb) {
+      LNT_HIP(hipMemcpyAsync(st.b[2].ptr, b_keys, sb, hipMemcpyHostToDevice, ctx->stream));
+      if (b_vals) LNT_HIP(hipMemcpyAsync(st.b[3].ptr, b_vals, sb, hipMemcpyHostToDevice, ctx->stream));
+    }
+    ak = (const uint64_t *)st.b[0].ptr;
+    av = (const double *)st.b[1].ptr;
+    bk = (const uint64_t *)st.b[2].ptr;
+    bv = (const double *)st.b[3].ptr;
+  }
+  if (ctx->validate >= 2) LNT_TRY(check_sorted(ctx, ak, (uint32_t)na, bk, (uint32_t)nb));
+  if (nprod_only) {
+    uint64_t np = 0;
+    LNT_TRY(lattice_nprod_device(ctx, ak, (uint64_t)na, a_dims, bk, (uint64_t)nb, b_dims, &np));
+    *nprod_only = (int64_t)np;
+    return LNT_OK;
+  }
+  if (strategy == LNT_SPGEMM_AUTO) strategy = LNT_SPGEMM_SORT;
+  uint64_t count = 0;
+  if (mem == LNT_MEM_DEVICE) {
+    uint64_t *kalloc = nullptr;
+    double *valloc = nullptr;
+    LNT_TRY(lattice_mult_device(ctx, ak, av, (uint64_t)na, a_dims, bk, bv, (uint64_t)nb, b_dims, strategy,
+                                (uint64_t *)c_keys, c_vals, (uint64_t)(capacity < 0 ? 0 : capacity), &count,
+                                nullptr, c_keys_alloc ? &kalloc : nullptr, c_vals_alloc ? &valloc : nullptr));
+    if (c_keys_alloc) {
+      if (!kalloc) {  // empty product: hand back a valid 1-element allocation
+        if (hipMalloc((void **)&kalloc, 8) != hipSuccess || hipMalloc((void **)&valloc, 8) != hipSuccess)
+          return fail(LNT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+      }
+      *c_keys_alloc = (int64_t *)kalloc;
+      *c_vals_alloc = valloc;
+    }
+    *nc = (int64_t)count;
+    return LNT_OK;
+  }
+  uint64_t *dk = nullptr;
+  double *dv = nullptr;
+  LNT_TRY(lattice_mult_device(ctx, ak, av, (uint64_t)na, a_dims, bk, bv, (uint64_t)nb, b_dims, strategy, nullptr,
+                              nullptr, 0, &count, nullptr, &dk, &dv));
+  int rc = LNT_OK;
+  int64_t *hk = c_keys;
+  double *hv = c_vals;
+  if (c_keys_alloc) {
+    hk = (int64_t *)malloc((count ? count : 1) * 8);
+    hv = (double *)malloc((count ? count : 1) * 8);
+    if (!hk || !hv) {
+      free(hk);
+      free(hv);
+      rc = fail(LNT_ERR_OUT_OF_MEMORY, "malloc of the lattice product failed");
+    }
+  } else if ((int64_t)count > capacity) {
+    rc = fail(LNT_ERR_INVALID_ARGUMENT, "capacity is below nnz(C) = " + std::to_string(count));
+  }
+  if (rc == LNT_OK && count) {
+    if (hipMemcpyAsync(hk, dk, count * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(hv, dv, count * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = fail(LNT_ERR_DEVICE, "copy of the lattice product to the host failed");
+  }
+  if (dk) (void)hipFree(dk);
+  if (dv) (void)hipFree(dv);
+  if (rc != LNT_OK) {
+    if (c_keys_alloc) {
+      free(hk);
+      free(hv);
+    }
+    return rc;
+  }
+  if (c_keys_alloc) {
+    *c_keys_alloc = hk;
+    *c_vals_alloc = hv;
+  }
+  *nc = (int64_t)count;
+  return LNT_OK;
+}
+
+int lnt_lattice_nprod(lnt_context ctx, const int64_t *a_keys, int64_t na, const int64_t a_dims[3],
+                      const int64_t *b_keys, int64_t nb, const int64_t b_dims[3], int64_t *nprod, int mem) {
+  LNT_CTX(ctx);
+  if (!nprod) return fail(LNT_ERR_INVALID_ARGUMENT, "null nprod");
+  return lattice_common(ctx, a_keys, nullptr, na, a_dims, b_keys, nullptr, nb, b_dims, 0, nullptr, nullptr, 0,
+                        nullptr, nullptr, nullptr, nprod, mem);
+}
+
+int lnt_lattice_mult_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                          const int64_t a_dims[3], const int64_t *b_keys, const double *b_vals, int64_t nb,
+                          const int64_t b_dims[3], int strategy, int64_t *c_keys, double *c_vals,
+                          int64_t capacity, int64_t *nc, int mem) {
+  LNT_CTX(ctx);
+  if (!nc) return fail(LNT_ERR_INVALID_ARGUMENT, "null nc");
+  if (capacity > 0 && (!c_keys || !c_vals)) return fail(LNT_ERR_INVALID_ARGUMENT, "null output array");
+  return lattice_common(ctx, a_keys, a_vals, na, a_dims, b_keys, b_vals, nb, b_dims, strategy, c_keys, c_vals,
+                        capacity, nc, nullptr, nullptr, nullptr, mem);
+}
+
+int lnt_lattice_mult(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                     const int64_t a_dims[3], const int64_t *b_keys, const double *b_vals, int64_t nb,
+                     const int64_t b_dims[3], int strategy, int64_t **c_keys, double **c_vals, int64_t *nc,
+                     int mem) {
+  LNT_CTX(ctx);
+  if (!nc || !c_keys || !c_vals) return fail(LNT_ERR_INVALID_ARGUMENT, "null output pointer");
+  return lattice_common(ctx, a_keys, a_vals, na, a_dims, b_keys, b_vals, nb, b_dims, strategy, nullptr, nullptr, 0,
+                        nc, c_keys, c_vals, nullptr, mem);
 }
 
 int lnt_free(lnt_context ctx, void *ptr, int mem) {

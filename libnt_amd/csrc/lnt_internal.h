@@ -184,6 +184,35 @@ __device__ inline uint32_t block256_exclusive_sum(uint32_t v, uint32_t *scratch,
   return before + incl - v;
 }
 
+__device__ inline uint64_t wave_inclusive_sum64(uint64_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint64_t y = __shfl_up(v, off, kWave);
+    if (lane >= (uint32_t)off) v += y;
+  }
+  return v;
+}
+
+// 64-bit variant for a 256-thread block; `scratch` holds 4 words.
+__device__ inline uint64_t block_exclusive_sum64_lds(uint64_t v, uint64_t *scratch,
+                                                     uint64_t *total) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint64_t incl = wave_inclusive_sum64(v);
+  if (lane == 63) scratch[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    uint64_t s = scratch[w];
+    before += (uint32_t)w < wave ? s : 0;
+    all += s;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
 // ---------------------------------------------------------------- context ---
 struct DeviceBuffer {
   void *ptr = nullptr;
@@ -201,7 +230,8 @@ struct SortTuning {
 };
 
 // Kernel kinds recorded by the profiler (names in api.hip).
-enum ProfKind { kProfHistogram = 0, kProfOnesweep, kProfPartition, kProfMerge, kProfRelin, kProfKinds };
+enum ProfKind { kProfHistogram = 0, kProfOnesweep, kProfPartition, kProfMerge, kProfRelin,
+                kProfLatticeMult, kProfKinds };
 
 struct ProfRecord {
   int kind;
@@ -221,6 +251,7 @@ struct lnt_context_s {
   lnt::DeviceBuffer status;     // look-back words
   lnt::DeviceBuffer small;      // histograms, counters, flags
   lnt::DeviceBuffer aux;        // merge-path splits and other per-call scratch
+  lnt::DeviceBuffer spg_ws, spg_tmp, spg_big;  // lattice-product workspaces
   lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
   uint32_t *pinned = nullptr;   // small pinned host buffer for counts/flags
   int profiling = 0;
@@ -251,6 +282,14 @@ int launch_fill_u32(lnt_context ctx, uint32_t *p, uint32_t value, size_t n);
 int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
                 uint64_t bound, const Relin *relin);
 int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin);
+int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na,
+                        const int64_t a_dims[3], const uint64_t *bk, const double *bv,
+                        uint64_t nb, const int64_t b_dims[3], int strategy, uint64_t *ck,
+                        double *cv, uint64_t capacity, uint64_t *nc_host, uint64_t *nprod_host,
+                        uint64_t **ck_alloc, double **cv_alloc);
+int lattice_nprod_device(lnt_context ctx, const uint64_t *ak, uint64_t na, const int64_t a_dims[3],
+                         const uint64_t *bk, uint64_t nb, const int64_t b_dims[3],
+                         uint64_t *nprod_host);
 int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
                  uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
                  int op, uint64_t *ck, double *cv, uint64_t capacity,

@@ -1,0 +1,101 @@
+// scan.hip -- device-wide exclusive prefix sums (reduce-then-scan), used by
+// the lattice product for product offsets, segment ids and output offsets.
+// Chunk = 256 threads x 16 items; chunk sums are scanned recursively.
+#include "lnt_internal.h"
+
+namespace lnt {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr uint64_t kChunk = (uint64_t)kBlock * kItems;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void chunk_sum_kernel(const T *__restrict__ in, uint64_t n,
+                                                           uint64_t *__restrict__ sums) {
+  __shared__ uint64_t s[kBlock / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kChunk;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t p = base + (uint64_t)i * kBlock + threadIdx.x;
+    if (p < n) acc += (uint64_t)in[p];
+  }
+  uint64_t total;
+  block_exclusive_sum64_lds(acc, s, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Each thread owns kItems consecutive elements of the chunk.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void chunk_scan_kernel(const T *__restrict__ in, uint64_t n,
+                                                            const uint64_t *__restrict__ offsets,
+                                                            uint64_t *__restrict__ out,
+                                                            uint64_t *__restrict__ total_out) {
+  __shared__ uint64_t s[kBlock / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kChunk + (uint64_t)threadIdx.x * kItems;
+  uint64_t v[kItems];
+  uint64_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t p = base + i;
+    v[i] = p < n ? (uint64_t)in[p] : 0;
+    acc += v[i];
+  }
+  uint64_t total;
+  uint64_t run = block_exclusive_sum64_lds(acc, s, &total) + (offsets ? offsets[blockIdx.x] : 0);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    uint64_t p = base + i;
+    if (p < n) out[p] = run;
+    run += v[i];
+  }
+  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total_out = run;
+}
+
+}  // namespace
+
+// out[i] = sum_{j<i} in[j] for i < n; *total_dev (optional, device) = sum of all.
+// `in` and `out` may alias only when T is uint64_t.
+template <typename T>
+int exclusive_scan(lnt_context ctx, const T *in, uint64_t *out, uint64_t n, uint64_t *total_dev,
+                   DeviceBuffer &tmp, size_t tmp_offset) {
+  if (n == 0) {
+    if (total_dev) LNT_TRY(launch_fill_u32(ctx, (uint32_t *)total_dev, 0, 2));
+    return LNT_OK;
+  }
+  uint64_t chunks = (n + kChunk - 1) / kChunk;
+  if (chunks == 1) {
+    chunk_scan_kernel<T><<<1, kBlock, 0, ctx->stream>>>(in, n, nullptr, out, total_dev);
+    LNT_HIP(hipGetLastError());
+    return LNT_OK;
+  }
+  // chunk sums at tmp[tmp_offset ...], then their scan in place (recursively).
+  size_t need = (tmp_offset + chunks + 1) * sizeof(uint64_t);
+  if (tmp.bytes < need) return fail(LNT_ERR_DEVICE, "scan workspace too small");
+  uint64_t *sums = (uint64_t *)tmp.ptr + tmp_offset;
+  chunk_sum_kernel<T><<<(uint32_t)chunks, kBlock, 0, ctx->stream>>>(in, n, sums);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint64_t>(ctx, sums, sums, chunks, nullptr, tmp, tmp_offset + chunks + 1));
+  chunk_scan_kernel<T><<<(uint32_t)chunks, kBlock, 0, ctx->stream>>>(in, n, sums, out, total_dev);
+  LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+// Words of scan workspace needed for n elements.
+size_t scan_workspace_words(uint64_t n) {
+  size_t words = 0;
+  while (n > kChunk) {
+    n = (n + kChunk - 1) / kChunk;
+    words += n + 1;
+  }
+  return words + 2;
+}
+
+template int exclusive_scan<uint32_t>(lnt_context, const uint32_t *, uint64_t *, uint64_t, uint64_t *,
+                                      DeviceBuffer &, size_t);
+template int exclusive_scan<uint64_t>(lnt_context, const uint64_t *, uint64_t *, uint64_t, uint64_t *,
+                                      DeviceBuffer &, size_t);
+
+}  // namespace lnt

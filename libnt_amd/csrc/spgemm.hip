@@ -1,0 +1,602 @@
+// spgemm.hip -- sparse lattice product C = A * B per tab for gfx950.
+//
+// Replaces SparseLatticeBase::csc_no_accum<true> (multi_index_array/
+// SparseLatticeBase.h:1854-1990, scatter :1993-2031): for every tab t and
+// every column j of B, C(:,j,t) = sum_k B(k,j,t) * A(:,k,t), computed by
+// expanding the products, ordering them by row, and summing equal rows.
+// Lattice keys are column major, key = row + h*col + h*w*tab
+// (SparseLatticeBase.h:867-877); A (h x w x d) and B (w x bw x d) arrive sorted
+// ascending, C (h x bw x d) leaves sorted ascending.  Products are b*a as in
+// mult_scatter; equal rows are summed in ascending (B row, A row) order.
+//
+// Pipeline (one output column of B = one "segment"):
+//   1. per B entry: A column range by binary search, column id, head flag;
+//   2. scans: product offsets per entry, segment ids;
+//   3. per segment: classify by product count --
+//        one B entry     -> C(:,j) is that A column scaled: no sort at all;
+//        <= kSmall prods -> one workgroup expands into LDS, bitonic-sorts
+//                           (row, position) and sums runs ("sort" strategy), or
+//                           accumulates in an LDS hash table and sorts only the
+//                           unique rows ("hash" strategy);
+//        larger          -> all such segments together: products expanded to
+//                           HBM with composite key (segment rank, row), one
+//                           onesweep radix sort, run sums;
+//   4. scan of per-segment output counts, then one coalesced write of C with
+//      the lattice keys (row + h * (t*bw + j)).
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "lnt_internal.h"
+
+namespace lnt {
+
+template <typename T>
+int exclusive_scan(lnt_context ctx, const T *in, uint64_t *out, uint64_t n, uint64_t *total_dev,
+                   DeviceBuffer &tmp, size_t tmp_offset);
+size_t scan_workspace_words(uint64_t n);
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSmall = 2048;  // largest segment handled inside one workgroup's LDS
+
+__device__ inline uint64_t lower_bound_u64(const uint64_t *a, uint64_t n, uint64_t x) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Largest i in [lo, hi) with off[i] <= x (off ascending, off[lo] <= x).
+__device__ inline uint64_t upper_index(const uint64_t *off, uint64_t lo, uint64_t hi, uint64_t x) {
+  while (hi - lo > 1) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+struct LatticeDims {
+  uint64_t ah, aw;          // A: h x w
+  uint64_t bw;              // B: w x bw
+  Divider b_tab;            // bh*bw
+  Divider b_h;              // bh (= aw)
+};
+
+// 1. one thread per B entry.
+__global__ void entry_kernel(const uint64_t *__restrict__ ak, uint64_t na,
+                             const uint64_t *__restrict__ bk, uint64_t nb, LatticeDims L,
+                             uint64_t *__restrict__ a_lo, uint32_t *__restrict__ a_len,
+                             uint64_t *__restrict__ a_base, uint64_t *__restrict__ colid,
+                             uint32_t *__restrict__ head) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < nb; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t key = bk[i];
+    uint64_t t = divide(key, L.b_tab);
+    uint64_t rem = key - t * L.b_tab.d;
+    uint64_t col = divide(rem, L.b_h);
+    uint64_t k = rem - col * L.b_h.d;
+    uint64_t start = t * L.ah * L.aw + k * L.ah;
+    uint64_t lo = lower_bound_u64(ak, na, start);
+    uint64_t hi = lower_bound_u64(ak, na, start + L.ah);
+    a_lo[i] = lo;
+    a_len[i] = (uint32_t)(hi - lo);
+    a_base[i] = start;
+    uint64_t cid = t * L.bw + col;
+    colid[i] = cid;
+    uint32_t h = 1;
+    if (i > 0) {
+      uint64_t pk = bk[i - 1];
+      uint64_t pt = divide(pk, L.b_tab);
+      uint64_t pc = divide(pk - pt * L.b_tab.d, L.b_h);
+      h = (pt * L.bw + pc) != cid;
+    }
+    head[i] = h;
+  }
+}
+
+// 2b. segment starts: seg_first[seg_idx[i]] = i for head entries.
+__global__ void seg_first_kernel(const uint32_t *__restrict__ head, const uint64_t *__restrict__ seg_idx,
+                                 uint64_t nb, uint64_t *__restrict__ seg_first) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < nb; i += (uint64_t)gridDim.x * blockDim.x)
+    if (head[i]) seg_first[seg_idx[i]] = i;
+}
+
+// 3. classification.  kind: 0 empty, 1 single B entry, 2 small, 3 large.
+__global__ void classify_kernel(const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb,
+                                const uint64_t *__restrict__ prod_off, uint32_t small_max,
+                                uint64_t *__restrict__ seg_u, uint32_t *__restrict__ list2,
+                                uint32_t *__restrict__ list3, uint32_t *__restrict__ counts) {
+  uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; s < nseg; s += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t first = seg_first[s];
+    uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+    uint64_t np = prod_off[last] - prod_off[first];
+    if (np == 0 || last - first == 1) {
+      seg_u[s] = np;
+    } else if (np <= small_max) {
+      seg_u[s] = 0;
+      list2[atomicAdd(&counts[0], 1u)] = (uint32_t)s;
+    } else {
+      seg_u[s] = 0;
+      list3[atomicAdd(&counts[1], 1u)] = (uint32_t)s;
+    }
+  }
+}
+
+// Maps product p (0-based inside a segment) to its B entry e and A index.
+__device__ inline void locate(const uint64_t *prod_off, uint64_t first, uint64_t last, uint64_t p,
+                              const uint64_t *a_lo, uint64_t *e_out, uint64_t *a_idx) {
+  uint64_t base = prod_off[first];
+  uint64_t e = upper_index(prod_off, first, last, base + p);
+  *e_out = e;
+  *a_idx = a_lo[e] + (base + p - prod_off[e]);
+}
+
+// 3a. small segments, "sort" strategy: expand into LDS, bitonic sort of
+// (row, position), sum runs in position order.
+__global__ __launch_bounds__(kBlock) void small_sort_kernel(
+    const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
+    const uint64_t *__restrict__ a_lo, const uint64_t *__restrict__ a_base,
+    const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg,
+    uint64_t nb, const uint32_t *__restrict__ list2, uint64_t *__restrict__ seg_u,
+    uint64_t *__restrict__ tmp_row, double *__restrict__ tmp_val) {
+  __shared__ uint64_t s_row[kSmall];
+  __shared__ uint16_t s_pos[kSmall];
+  __shared__ double s_val[kSmall];
+  __shared__ uint64_t s_scan[kBlock / kWave];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t s = list2[blockIdx.x];
+  const uint64_t first = seg_first[s];
+  const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+  const uint64_t base = prod_off[first];
+  const uint32_t np = (uint32_t)(prod_off[last] - base);
+  uint32_t P = 1;
+  while (P < np) P <<= 1;
+  for (uint32_t p = tid; p < P; p += kBlock) {
+    if (p < np) {
+      uint64_t e, ai;
+      locate(prod_off, first, last, p, a_lo, &e, &ai);
+      s_row[p] = ak[ai] - a_base[e];
+      s_val[p] = bv[e] * av[ai];
+    } else {
+      s_row[p] = ~0ull;
+    }
+    s_pos[p] = (uint16_t)p;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = tid; i < P; i += kBlock) {
+        uint32_t l = i ^ j;
+        if (l > i) {
+          uint64_t ri = s_row[i], rl = s_row[l];
+          uint16_t pi = s_pos[i], pl = s_pos[l];
+          bool gt = ri > rl || (ri == rl && pi > pl);
+          bool up = (i & k) == 0;
+          if (gt == up) {
+            s_row[i] = rl; s_row[l] = ri;
+            s_pos[i] = pl; s_pos[l] = pi;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // Run heads, ranked with a block scan over kSmall/kBlock items per thread.
+  constexpr int kPer = kSmall / kBlock;
+  uint32_t flags = 0, cnt = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    uint32_t p = tid * kPer + q;
+    bool h = p < np && (p == 0 || s_row[p] != s_row[p - 1]);
+    flags |= (h ? 1u : 0u) << q;
+    cnt += h;
+  }
+  uint64_t total;
+  uint64_t rank = block_exclusive_sum64_lds(cnt, s_scan, &total);
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    if ((flags >> q) & 1u) {
+      uint32_t p = tid * kPer + q;
+      uint64_t row = s_row[p];
+      double sum = s_val[s_pos[p]];
+      for (uint32_t r = p + 1; r < np && s_row[r] == row; ++r) sum += s_val[s_pos[r]];
+      tmp_row[base + rank] = row;
+      tmp_val[base + rank] = sum;
+      ++rank;
+    }
+  }
+  if (tid == 0) seg_u[s] = total;
+}
+
+// 3a'. small segments, "hash" strategy: accumulate products of equal rows in
+// an LDS open-addressing table (64-bit CAS on the row, FP64 LDS atomic add on
+// the value), then bitonic-sort the table so that the unique rows come first.
+// The summation order follows the atomics, so the last bits of a sum may vary
+// from run to run (within the 1e-12 tolerance); indices do not.
+constexpr int kHashSlots = 2 * kSmall;
+
+__global__ __launch_bounds__(kBlock) void small_hash_kernel(
+    const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
+    const uint64_t *__restrict__ a_lo, const uint64_t *__restrict__ a_base,
+    const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg,
+    uint64_t nb, const uint32_t *__restrict__ list2, uint64_t *__restrict__ seg_u,
+    uint64_t *__restrict__ tmp_row, double *__restrict__ tmp_val) {
+  __shared__ unsigned long long s_key[kHashSlots];
+  __shared__ double s_val[kHashSlots];
+  __shared__ uint64_t s_scan[kBlock / kWave];
+  constexpr unsigned long long kEmpty = ~0ull;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t s = list2[blockIdx.x];
+  const uint64_t first = seg_first[s];
+  const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+  const uint64_t base = prod_off[first];
+  const uint32_t np = (uint32_t)(prod_off[last] - base);
+  uint32_t T = 2;
+  while (T < 2 * np) T <<= 1;
+  for (uint32_t i = tid; i < T; i += kBlock) {
+    s_key[i] = kEmpty;
+    s_val[i] = 0.0;
+  }
+  __syncthreads();
+  for (uint32_t p = tid; p < np; p += kBlock) {
+    uint64_t e, ai;
+    locate(prod_off, first, last, p, a_lo, &e, &ai);
+    unsigned long long row = ak[ai] - a_base[e];
+    double val = bv[e] * av[ai];
+    uint32_t slot = (uint32_t)((row * 0x9E3779B97F4A7C15ull) >> 40) & (T - 1);
+    for (;;) {
+      unsigned long long prev = atomicCAS(&s_key[slot], kEmpty, row);
+      if (prev == kEmpty || prev == row) {
+        atomicAdd(&s_val[slot], val);
+        break;
+      }
+      slot = (slot + 1) & (T - 1);
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= T; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = tid; i < T; i += kBlock) {
+        uint32_t l = i ^ j;
+        if (l > i) {
+          unsigned long long ki = s_key[i], kl = s_key[l];
+          bool up = (i & k) == 0;
+          if ((ki > kl) == up && ki != kl) {
+            double vi = s_val[i];
+            s_key[i] = kl; s_key[l] = ki;
+            s_val[i] = s_val[l]; s_val[l] = vi;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  uint32_t cnt = 0;
+  for (uint32_t i = tid; i < T; i += kBlock) cnt += s_key[i] != kEmpty;
+  uint64_t total;
+  block_exclusive_sum64_lds(cnt, s_scan, &total);
+  for (uint32_t i = tid; i < total; i += kBlock) {
+    tmp_row[base + i] = s_key[i];
+    tmp_val[base + i] = s_val[i];
+  }
+  if (tid == 0) seg_u[s] = total;
+}
+
+// 3b. large segments.  np3[r] = products of list3[r].
+__global__ void large_sizes_kernel(const uint32_t *__restrict__ list3, uint32_t n3,
+                                   const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb,
+                                   const uint64_t *__restrict__ prod_off, uint64_t *__restrict__ np3) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n3) return;
+  uint64_t s = list3[r];
+  uint64_t first = seg_first[s];
+  uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+  np3[r] = prod_off[last] - prod_off[first];
+}
+
+// Products of the large segments with composite key r * h + row.
+__global__ void large_expand_kernel(
+    const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
+    const uint64_t *__restrict__ a_lo, const uint64_t *__restrict__ a_base,
+    const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg,
+    uint64_t nb, const uint32_t *__restrict__ list3, uint32_t n3, const uint64_t *__restrict__ ex_off,
+    uint64_t total3, uint64_t ah, uint64_t *__restrict__ key3, double *__restrict__ val3) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; q < total3; q += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t r = upper_index(ex_off, 0, n3, q);
+    uint64_t s = list3[r];
+    uint64_t first = seg_first[s];
+    uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+    uint64_t e, ai;
+    locate(prod_off, first, last, q - ex_off[r], a_lo, &e, &ai);
+    key3[q] = r * ah + (ak[ai] - a_base[e]);
+    val3[q] = bv[e] * av[ai];
+  }
+}
+
+__global__ void run_heads_kernel(const uint64_t *__restrict__ key3, uint64_t total3,
+                                 uint32_t *__restrict__ heads) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; q < total3; q += (uint64_t)gridDim.x * blockDim.x)
+    heads[q] = q == 0 || key3[q] != key3[q - 1];
+}
+
+// Sums every run (in sorted = expansion order) and stores it in its segment's
+// region of tmp_row/tmp_val.
+__global__ void large_reduce_kernel(const uint64_t *__restrict__ key3, const double *__restrict__ val3,
+                                    const uint32_t *__restrict__ heads, const uint64_t *__restrict__ grank,
+                                    uint64_t total3, Divider ah, const uint32_t *__restrict__ list3,
+                                    const uint64_t *__restrict__ ex_off, const uint64_t *__restrict__ seg_first,
+                                    const uint64_t *__restrict__ prod_off, uint64_t *__restrict__ tmp_row,
+                                    double *__restrict__ tmp_val) {
+  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; q < total3; q += (uint64_t)gridDim.x * blockDim.x) {
+    if (!heads[q]) continue;
+    uint64_t key = key3[q];
+    uint64_t r = divide(key, ah);
+    uint64_t row = key - r * ah.d;
+    double sum = val3[q];
+    for (uint64_t x = q + 1; x < total3 && key3[x] == key; ++x) sum += val3[x];
+    uint64_t local = grank[q] - grank[ex_off[r]];
+    uint64_t dst = prod_off[seg_first[list3[r]]] + local;
+    tmp_row[dst] = row;
+    tmp_val[dst] = sum;
+  }
+}
+
+__global__ void large_counts_kernel(const uint32_t *__restrict__ list3, uint32_t n3,
+                                    const uint64_t *__restrict__ ex_off, const uint64_t *__restrict__ grank,
+                                    uint64_t total_heads, uint64_t total3, uint64_t *__restrict__ seg_u) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n3) return;
+  uint64_t lo = grank[ex_off[r]];
+  uint64_t hi = ex_off[r + 1] < total3 ? grank[ex_off[r + 1]] : total_heads;
+  seg_u[list3[r]] = hi - lo;
+}
+
+// 4. output: one thread per element of C.
+__global__ void write_kernel(const uint64_t *__restrict__ ak, const double *__restrict__ av,
+                             const double *__restrict__ bv, const uint64_t *__restrict__ a_lo,
+                             const uint64_t *__restrict__ a_base, const uint64_t *__restrict__ colid,
+                             const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first,
+                             uint64_t nseg, uint64_t nb, const uint64_t *__restrict__ out_off, uint64_t nc,
+                             const uint64_t *__restrict__ tmp_row, const double *__restrict__ tmp_val,
+                             uint64_t ah, uint64_t *__restrict__ ck, double *__restrict__ cv) {
+  uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; o < nc; o += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t s = upper_index(out_off, 0, nseg, o);
+    uint64_t q = o - out_off[s];
+    uint64_t first = seg_first[s];
+    uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+    uint64_t row;
+    double val;
+    if (last - first == 1) {
+      uint64_t ai = a_lo[first] + q;
+      row = ak[ai] - a_base[first];
+      val = bv[first] * av[ai];
+    } else {
+      row = tmp_row[prod_off[first] + q];
+      val = tmp_val[prod_off[first] + q];
+    }
+    ck[o] = row + ah * colid[first];
+    cv[o] = val;
+  }
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t cap = 65536) {
+  uint64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  return (uint32_t)(b > cap ? cap : b);
+}
+
+struct Carve {
+  char *p;
+  template <typename T>
+  T *take(uint64_t count) {
+    T *r = (T *)p;
+    p += ((count * sizeof(T) + 255) / 256) * 256;
+    return r;
+  }
+};
+
+int read_u64(lnt_context ctx, const uint64_t *dev, uint64_t *out, int count) {
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 8, dev, sizeof(uint64_t) * count, hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  memcpy(out, ctx->pinned + 8, sizeof(uint64_t) * count);
+  return LNT_OK;
+}
+
+}  // namespace
+
+static int check_lattice(const int64_t a_dims[3], const int64_t b_dims[3]) {
+  for (int i = 0; i < 3; ++i)
+    if (a_dims[i] < 1 || b_dims[i] < 1) return fail(LNT_ERR_INVALID_ARGUMENT, "lattice dimensions must be >= 1");
+  if (a_dims[1] != b_dims[0] || a_dims[2] != b_dims[2])
+    return fail(LNT_ERR_DIMENSION_MISMATCH, "A's width and depth must equal B's height and depth");
+  unsigned __int128 a = (unsigned __int128)a_dims[0] * a_dims[1] * a_dims[2];
+  unsigned __int128 b = (unsigned __int128)b_dims[0] * b_dims[1] * b_dims[2];
+  unsigned __int128 c = (unsigned __int128)a_dims[0] * b_dims[1] * b_dims[2];
+  const unsigned __int128 lim = (unsigned __int128)1 << 63;
+  if (a > lim || b > lim || c > lim) return fail(LNT_ERR_TOO_LARGE, "lattice dimensionality exceeds 2^63");
+  return LNT_OK;
+}
+
+struct EntryArrays {
+  uint64_t *a_lo, *a_base, *colid, *prod_off, *seg_idx, *seg_first, *seg_u, *out_off, *np3, *ex_off;
+  uint32_t *a_len, *head, *list2, *list3, *counts;
+  size_t scan_words;
+};
+
+static int entry_phase(lnt_context ctx, const uint64_t *ak, uint64_t na, const int64_t a_dims[3],
+                       const uint64_t *bk, uint64_t nb, const int64_t b_dims[3], EntryArrays *E) {
+  size_t scan_words = scan_workspace_words(nb + 1);
+  size_t bytes = (nb + 1) * (8 * 10 + 4 * 4) + 16 * 256 + 64 + scan_words * 8 + 8192;
+  LNT_TRY(ensure(ctx->spg_ws, bytes));
+  Carve c{(char *)ctx->spg_ws.ptr};
+  E->a_lo = c.take<uint64_t>(nb + 1);
+  E->a_base = c.take<uint64_t>(nb + 1);
+  E->colid = c.take<uint64_t>(nb + 1);
+  E->prod_off = c.take<uint64_t>(nb + 1);
+  E->seg_idx = c.take<uint64_t>(nb + 1);
+  E->seg_first = c.take<uint64_t>(nb + 1);
+  E->seg_u = c.take<uint64_t>(nb + 1);
+  E->out_off = c.take<uint64_t>(nb + 1);
+  E->np3 = c.take<uint64_t>(nb + 1);
+  E->ex_off = c.take<uint64_t>(nb + 1);
+  E->a_len = c.take<uint32_t>(nb + 1);
+  E->head = c.take<uint32_t>(nb + 1);
+  E->list2 = c.take<uint32_t>(nb + 1);
+  E->list3 = c.take<uint32_t>(nb + 1);
+  E->counts = c.take<uint32_t>(16);
+  E->scan_words = scan_words;
+  LatticeDims L;
+  L.ah = (uint64_t)a_dims[0];
+  L.aw = (uint64_t)a_dims[1];
+  L.bw = (uint64_t)b_dims[1];
+  L.b_tab = make_divider((uint64_t)b_dims[0] * (uint64_t)b_dims[1]);
+  L.b_h = make_divider((uint64_t)b_dims[0]);
+  entry_kernel<<<grid_for(nb), 256, 0, ctx->stream>>>(ak, na, bk, nb, L, E->a_lo, E->a_len, E->a_base,
+                                                        E->colid, E->head);
+  LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+int lattice_nprod_device(lnt_context ctx, const uint64_t *ak, uint64_t na, const int64_t a_dims[3],
+                         const uint64_t *bk, uint64_t nb, const int64_t b_dims[3], uint64_t *nprod_host) {
+  LNT_TRY(check_lattice(a_dims, b_dims));
+  *nprod_host = 0;
+  if (na == 0 || nb == 0) return LNT_OK;
+  EntryArrays E;
+  LNT_TRY(entry_phase(ctx, ak, na, a_dims, bk, nb, b_dims, &E));
+  // The scan workspace lives after out_off's region in the same carve: reuse
+  // ctx->aux for it.
+  LNT_TRY(ensure(ctx->aux, E.scan_words * 8 + 64));
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, E.a_len, E.prod_off, nb, E.prod_off + nb, ctx->aux, 0));
+  return read_u64(ctx, E.prod_off + nb, nprod_host, 1);
+}
+
+int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na,
+                        const int64_t a_dims[3], const uint64_t *bk, const double *bv, uint64_t nb,
+                        const int64_t b_dims[3], int strategy, uint64_t *ck, double *cv,
+                        uint64_t capacity, uint64_t *nc_host, uint64_t *nprod_host,
+                        uint64_t **ck_alloc, double **cv_alloc) {
+  LNT_TRY(check_lattice(a_dims, b_dims));
+  *nc_host = 0;
+  if (nprod_host) *nprod_host = 0;
+  if (na == 0 || nb == 0) return LNT_OK;
+  const uint64_t ah = (uint64_t)a_dims[0];
+  int slot = prof_begin(ctx, kProfLatticeMult, 16.0 * ((double)na + nb));
+  EntryArrays E;
+  LNT_TRY(entry_phase(ctx, ak, na, a_dims, bk, nb, b_dims, &E));
+  LNT_TRY(ensure(ctx->aux, E.scan_words * 8 + 64));
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, E.a_len, E.prod_off, nb, E.prod_off + nb, ctx->aux, 0));
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, E.head, E.seg_idx, nb, E.seg_idx + nb, ctx->aux, 0));
+  uint64_t hv[2];
+  LNT_TRY(read_u64(ctx, E.prod_off + nb, &hv[0], 1));
+  LNT_TRY(read_u64(ctx, E.seg_idx + nb, &hv[1], 1));
+  const uint64_t nprod = hv[0], nseg = hv[1];
+  if (nprod_host) *nprod_host = nprod;
+  if (nprod >= (uint64_t(1) << 32)) {
+    prof_end(ctx, slot);
+    return fail(LNT_ERR_TOO_LARGE, "more than 2^32 scalar products in one lattice product");
+  }
+  if (nprod == 0) {
+    prof_end(ctx, slot);
+    return LNT_OK;
+  }
+  seg_first_kernel<<<grid_for(nb), 256, 0, ctx->stream>>>(E.head, E.seg_idx, nb, E.seg_first);
+  LNT_TRY(launch_fill_u32(ctx, E.counts, 0, 4));
+  classify_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.seg_first, nseg, nb, E.prod_off, kSmall,
+                                                            E.seg_u, E.list2, E.list3, E.counts);
+  LNT_HIP(hipGetLastError());
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 4, E.counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  const uint32_t n2 = ctx->pinned[4], n3 = ctx->pinned[5];
+  if (n2 || n3) LNT_TRY(ensure(ctx->spg_tmp, nprod * 16));
+  uint64_t *tmp_row = (uint64_t *)ctx->spg_tmp.ptr;
+  double *tmp_val = tmp_row ? (double *)(tmp_row + nprod) : nullptr;
+  if (n2) {
+    if (strategy == LNT_SPGEMM_HASH)
+      small_hash_kernel<<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first,
+                                                          nseg, nb, E.list2, E.seg_u, tmp_row, tmp_val);
+    else
+      small_sort_kernel<<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first,
+                                                          nseg, nb, E.list2, E.seg_u, tmp_row, tmp_val);
+    LNT_HIP(hipGetLastError());
+  }
+  if (n3) {
+    // Composite keys r*h + row must stay below 2^63; otherwise the large
+    // segments go through in groups that satisfy it.
+    uint32_t group = n3;
+    while (group > 1 && (unsigned __int128)group * ah >= ((unsigned __int128)1 << 63)) group = (group + 1) / 2;
+    for (uint32_t g0 = 0; g0 < n3; g0 += group) {
+      uint32_t gn = std::min(group, n3 - g0);
+      const uint32_t *l3 = E.list3 + g0;
+      large_sizes_kernel<<<(gn + 255) / 256, 256, 0, ctx->stream>>>(l3, gn, E.seg_first, nseg, nb, E.prod_off, E.np3);
+      LNT_HIP(hipGetLastError());
+      LNT_TRY(exclusive_scan<uint64_t>(ctx, E.np3, E.ex_off, gn, E.ex_off + gn, ctx->aux, 0));
+      uint64_t total3;
+      LNT_TRY(read_u64(ctx, E.ex_off + gn, &total3, 1));
+      size_t scan3 = scan_workspace_words(total3);
+      LNT_TRY(ensure(ctx->spg_big, total3 * (8 + 8 + 4 + 8) + 1024));
+      Carve c{(char *)ctx->spg_big.ptr};
+      uint64_t *key3 = c.take<uint64_t>(total3);
+      double *val3 = c.take<double>(total3);
+      uint32_t *heads = c.take<uint32_t>(total3);
+      uint64_t *grank = c.take<uint64_t>(total3 + 1);
+      LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan3) * 8 + 64));
+      large_expand_kernel<<<grid_for(total3), 256, 0, ctx->stream>>>(
+          ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first, nseg, nb, l3, gn, E.ex_off, total3, ah, key3, val3);
+      LNT_HIP(hipGetLastError());
+      LNT_TRY(sort_device(ctx, key3, val3, (uint32_t)total3, (uint64_t)gn * ah, nullptr));
+      run_heads_kernel<<<grid_for(total3), 256, 0, ctx->stream>>>(key3, total3, heads);
+      LNT_HIP(hipGetLastError());
+      LNT_TRY(exclusive_scan<uint32_t>(ctx, heads, grank, total3, grank + total3, ctx->aux, 0));
+      uint64_t total_heads;
+      LNT_TRY(read_u64(ctx, grank + total3, &total_heads, 1));
+      large_reduce_kernel<<<grid_for(total3), 256, 0, ctx->stream>>>(key3, val3, heads, grank, total3, make_divider(ah),
+                                                                      l3, E.ex_off, E.seg_first, E.prod_off, tmp_row, tmp_val);
+      large_counts_kernel<<<(gn + 255) / 256, 256, 0, ctx->stream>>>(l3, gn, E.ex_off, grank, total_heads, total3, E.seg_u);
+      LNT_HIP(hipGetLastError());
+    }
+  }
+  LNT_TRY(exclusive_scan<uint64_t>(ctx, E.seg_u, E.out_off, nseg, E.out_off + nseg, ctx->aux, 0));
+  uint64_t nc;
+  LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
+  if (ck_alloc) {
+    void *k = nullptr, *v = nullptr;
+    size_t bytes = (nc ? nc : 1) * 8;
+    if (hipMalloc(&k, bytes) != hipSuccess || hipMalloc(&v, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      if (k) (void)hipFree(k);
+      prof_end(ctx, slot);
+      return fail(LNT_ERR_OUT_OF_MEMORY, "hipMalloc of the lattice product failed");
+    }
+    ck = (uint64_t *)k;
+    cv = (double *)v;
+    *ck_alloc = ck;
+    *cv_alloc = cv;
+  } else if (nc > capacity) {
+    prof_end(ctx, slot);
+    return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) + " is below nnz(C) = " +
+                                              std::to_string(nc));
+  }
+  write_kernel<<<grid_for(nc), 256, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.colid, E.prod_off, E.seg_first,
+                                                       nseg, nb, E.out_off, nc, tmp_row, tmp_val, ah, ck, cv);
+  LNT_HIP(hipGetLastError());
+  prof_end(ctx, slot);
+  prof_set_bytes(ctx, slot, 16.0 * ((double)na + nb + nc));
+  *nc_host = nc;
+  return LNT_OK;
+}
+
+}  // namespace lnt

@@ -1,0 +1,234 @@
+"""Host-side mirror of LibMIA's SparseMIA index-expression front end.
+
+``SparseMIA`` mirrors ``SparseMIA<double,N>`` (multi_index_array/SparseMIA.h:
+167-773): dims, int64 linear indices and double values (here GPU tensors), the
+mode order of the indices (``linIdxSequence``, SparseMIABase.h:98-153) and a
+sorted flag.  ``A("ijkl")`` mirrors ``A(i,j,k,l)`` (MIA.h:222-229) and returns
+an ``Atom`` on which ``+``, ``-`` and ``*`` follow the reference's expression
+rules (MIA_Expr.h); an upper-case label is an inter index (the reference's
+``!i``, Index.h:54).  Every operation runs on the GPU through the C-ABI
+(include/lnt.h): change_sort_order, merge, lattice product.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import OP_ADD, OP_HADAMARD, OP_SUB, SPGEMM_AUTO, Context
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _prod(xs):
+    p = 1
+    for x in xs:
+        p *= int(x)
+    return p
+
+
+class SparseMIA:
+    """Sparse order-N tensor: keys (int64) and values (float64) on the GPU."""
+
+    def __init__(self, dims, keys=None, vals=None, seq=None, is_sorted=False, ctx=None):
+        import torch
+
+        self.dims = tuple(int(d) for d in dims)
+        self.ctx = ctx or default_context()
+        if keys is None:
+            keys = torch.empty(0, dtype=torch.int64, device="cuda")
+            vals = torch.empty(0, dtype=torch.float64, device="cuda")
+        if isinstance(keys, np.ndarray):
+            keys = torch.from_numpy(np.ascontiguousarray(keys, dtype=np.int64)).cuda()
+        if isinstance(vals, np.ndarray):
+            vals = torch.from_numpy(np.ascontiguousarray(vals, dtype=np.float64)).cuda()
+        if keys.shape != vals.shape:
+            raise ValueError("keys and values differ in length")
+        self.keys = keys.contiguous()
+        self.vals = vals.contiguous()
+        self.seq = list(range(len(self.dims))) if seq is None else [int(x) for x in seq]
+        self.is_sorted = bool(is_sorted)
+
+    # ------------------------------------------------------------ properties
+    @property
+    def order(self) -> int:
+        return len(self.dims)
+
+    @property
+    def dimensionality(self) -> int:
+        return _prod(self.dims)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.keys.shape[0])
+
+    def linIdxSequence(self):
+        return list(self.seq)
+
+    # ----------------------------------------------------------------- sorts
+    def sort(self, seq=None):
+        """SparseMIABase::sort() / sort(seq) (SparseMIABase.h:590-645, 701-734)."""
+        new = self.seq if seq is None else [int(x) for x in seq]
+        if self.nnz:
+            self.ctx.change_sort_order(self.keys, self.vals, self.dims, self.seq, new, self.is_sorted)
+        self.seq = list(new)
+        self.is_sorted = True
+        return self
+
+    def change_linIdx_sequence(self, seq):
+        """Relinearise without sorting (SparseMIABase.h:768-814)."""
+        seq = [int(x) for x in seq]
+        if seq == self.seq or self.order == 1:
+            return self
+        if self.nnz:
+            self.ctx.change_linidx(self.keys, self.dims, self.seq, seq)
+        self.seq = seq
+        self.is_sorted = False
+        return self
+
+    def copy(self):
+        return SparseMIA(self.dims, self.keys.clone(), self.vals.clone(), self.seq, self.is_sorted, self.ctx)
+
+    # -------------------------------------------------------------- helpers
+    def __call__(self, labels: str) -> "Atom":
+        if len(labels) != self.order:
+            raise ValueError("expected %d index labels, got %r" % (self.order, labels))
+        return Atom(self, labels)
+
+    def to_dense(self):
+        """Dense numpy copy in the default (first mode fastest) layout."""
+        import torch
+
+        k = self.keys.clone()
+        if self.seq != list(range(self.order)) and self.nnz:
+            self.ctx.change_linidx(k, self.dims, self.seq, list(range(self.order)))
+        d = np.zeros(self.dimensionality)
+        d[k.cpu().numpy()] = self.vals.cpu().numpy()
+        return d.reshape(self.dims, order="F")
+
+    def default_order_arrays(self):
+        """(keys, values) as numpy, sorted in the default mode order."""
+        c = self.copy().sort(list(range(self.order)))
+        return c.keys.cpu().numpy(), c.vals.cpu().numpy()
+
+
+class Atom:
+    """A(i,j,...) -- a tensor with index labels (MIA_Atom, MIA_Expr.h:663-689)."""
+
+    def __init__(self, mia: SparseMIA, labels: str):
+        self.mia = mia
+        self.labels = labels
+
+    # ------------------------------------------------------------- merges
+    def _merge(self, other: "Atom", op: int) -> SparseMIA:
+        """outside_merge -> outside_scanMerge (SparseMIABase.h:1357-1455)."""
+        a, b = self.mia, other.mia
+        if sorted(self.labels) != sorted(other.labels) or any(c.isupper() for c in self.labels):
+            raise ValueError("merge operands must carry the same plain labels")
+        if b is a:
+            b = b.copy()  # SparseMIABase.h:1367-1371
+        index_order = [other.labels.index(c) for c in self.labels]  # A mode -> B mode
+        for m, r in enumerate(index_order):
+            if a.dims[m] != b.dims[r]:
+                raise ValueError("merge dimensions mismatch")
+        b_in_a_order = [index_order[m] for m in a.seq]
+        lhs_order = [self.labels.index(c) for c in other.labels]  # B mode -> A mode
+        if b.is_sorted and a.is_sorted:
+            if b.nnz < a.nnz:
+                b.sort(b_in_a_order)
+            else:
+                a.sort([lhs_order[r] for r in b.seq])
+        elif b.is_sorted:
+            a.sort([lhs_order[r] for r in b.seq])
+        elif a.is_sorted:
+            b.sort(b_in_a_order)
+        else:
+            # Neither sorted: the reference concatenates and sorts (SparseMIA::
+            # sortMerge, SparseMIA.h:833-887); here both are sorted in A's
+            # order and scan-merged.
+            a.sort(a.seq)
+            b.sort(b_in_a_order)
+        ck, cv = a.ctx.merge(a.keys, a.vals, b.keys, b.vals, op)
+        return SparseMIA(a.dims, ck.clone(), cv.clone(), a.seq, True, a.ctx)
+
+    def __add__(self, other):
+        return self._merge(other, OP_ADD)
+
+    def __sub__(self, other):
+        return self._merge(other, OP_SUB)
+
+    # ----------------------------------------------------------- products
+    def plan(self, other: "Atom"):
+        """Lattice bookkeeping of MIA_Expr.h:449-561: A rows = outer, columns =
+        inner (contracted), tabs = inter, each in A's mode order; B rows = inner
+        matched to A's order, columns = B's outer modes, tabs = inter matched.
+        C's modes are [A outer, B outer, inter]."""
+        la, lb = self.labels, other.labels
+        a, b = self.mia, other.mia
+        if len(set(la)) != len(la) or len(set(lb)) != len(lb):
+            raise ValueError("an index label repeats within one operand")
+        a_out, a_in, a_int, b_in, b_int = [], [], [], [], []
+        for m, c in enumerate(la):
+            if c in lb:
+                r = lb.index(c)
+                if a.dims[m] != b.dims[r]:
+                    raise ValueError("matched indices differ in size")
+                if c.isupper():
+                    a_int.append(m)
+                    b_int.append(r)
+                else:
+                    a_in.append(m)
+                    b_in.append(r)
+            else:
+                if c.isupper():
+                    raise ValueError("inter index %r appears in one operand only" % c)
+                a_out.append(m)
+        b_out = [r for r, c in enumerate(lb) if c not in la]
+        if any(lb[r].isupper() for r in b_out):
+            raise ValueError("inter index appears in one operand only")
+        seq_a = a_out + a_in + a_int
+        seq_b = b_in + b_out + b_int
+        lat_a = (_prod(a.dims[m] for m in a_out), _prod(a.dims[m] for m in a_in), _prod(a.dims[m] for m in a_int))
+        lat_b = (lat_a[1], _prod(b.dims[r] for r in b_out), lat_a[2])
+        dims_c = [a.dims[m] for m in a_out] + [b.dims[r] for r in b_out] + [a.dims[m] for m in a_int]
+        labels_c = "".join([la[m] for m in a_out] + [lb[r] for r in b_out] + [la[m] for m in a_int])
+        return seq_a, seq_b, lat_a, lat_b, dims_c, labels_c, (a_in, a_out, b_out)
+
+    def __mul__(self, other: "Atom") -> "Atom":
+        a, b = self.mia, other.mia
+        if b is a:
+            b = b.copy()
+        seq_a, seq_b, lat_a, lat_b, dims_c, labels_c, (a_in, a_out, b_out) = self.plan(Atom(b, other.labels))
+        if not dims_c:
+            raise ValueError("a full contraction to a scalar is not a sparse tensor")
+        a.sort(seq_a)
+        b.sort(seq_b)
+        if not a_in and not a_out and not b_out:
+            # pure Hadamard (SparseMIABase.h:1717-1826 with no outer indices)
+            ck, cv = a.ctx.merge(a.keys, a.vals, b.keys, b.vals, OP_HADAMARD)
+        else:
+            ck, cv = a.ctx.lattice_mult(a.keys, a.vals, lat_a, b.keys, b.vals, lat_b, SPGEMM_AUTO)
+        # toStraightLattice/toLatticeSort leave the operands flagged unsorted
+        # (SparseMIABase.h:1507, SparseMIA.h:1063-1122).
+        a.is_sorted = False
+        b.is_sorted = False
+        c = SparseMIA(dims_c, ck.clone(), cv.clone(), None, True, a.ctx)
+        return Atom(c, labels_c)
+
+    # ---------------------------------------------------------- assignment
+    def assign(self, labels: str) -> SparseMIA:
+        """C(labels) = expr: relabels C's mode order without moving data
+        (SparseMIA::assign, SparseMIA.h:891-928)."""
+        if sorted(labels) != sorted(self.labels):
+            raise ValueError("assignment labels differ from the expression's")
+        perm = [self.labels.index(c) for c in labels]  # new mode m <- old mode perm[m]
+        old = self.mia
+        dims = [old.dims[p] for p in perm]
+        inv = {p: m for m, p in enumerate(perm)}
+        seq = [inv[s] for s in old.seq]
+        return SparseMIA(dims, old.keys, old.vals, seq, old.is_sorted, old.ctx)

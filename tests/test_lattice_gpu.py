@@ -1,0 +1,227 @@
+"""GPU parity of the sparse lattice product and of full MIA contractions
+(through the C-ABI and the SparseMIA mirror) against the CPU oracle: keys
+bit-exact, values within 1e-12 under isEqualFuzzy (FunctionUtil.h:688-698),
+because only the summation order of duplicate rows differs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def gpu(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def _random_lattice(orc, h, w, d, n, seed):
+    n = min(n, h * w * d)
+    k = np.sort(orc.gen_distinct_keys(n, h * w * d, seed))
+    v = orc.gen_uniform_vals(n, seed + 1)
+    return k, v
+
+
+CASES = [
+    # (ah, aw, d, bw, nA, nB)
+    (1024, 1024, 1, 1024, 10486, 10486),  # c1 shape
+    (50, 40, 3, 30, 1500, 900),  # several tabs, dense-ish
+    (1 << 20, 64, 1, 4096, 200000, 3000),  # tall A columns: large segments
+    (7, 5, 2, 9, 30, 40),
+    (1, 1, 1, 1, 1, 1),
+    (3000, 1, 4, 5000, 3000, 4000),  # no inner index (outer/inter product)
+    (100, 200, 1, 1, 2000, 150),  # single B column
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("strategy", [1, 2])
+def test_lattice_mult_matches_oracle(ctx, orc, case, strategy):
+    ah, aw, d, bw, n_a, n_b = case
+    print("lattice", case, "strategy", strategy, flush=True)
+    ak, av = _random_lattice(orc, ah, aw, d, n_a, 3)
+    bk, bv = _random_lattice(orc, aw, bw, d, n_b, 5)
+    ek, ev = orc.lattice_mult(ak, av, (ah, aw, d), bk, bv, (aw, bw, d))
+    nprod = orc.lattice_nprod(ak, (ah, aw, d), bk, (aw, bw, d))
+    assert ctx.lattice_nprod(gpu(ak), (ah, aw, d), gpu(bk), (aw, bw, d)) == nprod
+    ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, d), gpu(bk), gpu(bv), (aw, bw, d), strategy)
+    assert np.array_equal(host(ck), ek)
+    assert orc.allclose_fuzzy(host(cv), ev, TOL)
+
+
+def test_lattice_mult_skewed_columns(ctx, orc):
+    """B columns with many entries over long A columns exercise every segment
+    kind (single entry, LDS, global sort)."""
+    rng = np.random.default_rng(4)
+    ah, aw, bw = 1 << 16, 256, 512
+    a_cols = rng.integers(0, aw, 60000)
+    a_rows = rng.integers(0, ah, 60000)
+    ak = np.unique(a_rows + ah * a_cols).astype(np.int64)
+    av = rng.uniform(-1, 1, len(ak))
+    # B: column 0 holds every k, columns 1..40 a handful, the rest one entry
+    ks, cs = [], []
+    ks += list(range(aw)); cs += [0] * aw
+    for c in range(1, 41):
+        sel = rng.choice(aw, 6, replace=False)
+        ks += list(sel); cs += [c] * 6
+    for c in range(41, bw):
+        ks.append(int(rng.integers(0, aw))); cs.append(c)
+    bk = np.unique(np.array(ks) + aw * np.array(cs)).astype(np.int64)
+    bv = rng.uniform(-1, 1, len(bk))
+    ek, ev = orc.lattice_mult(ak, av, (ah, aw, 1), bk, bv, (aw, bw, 1))
+    for strategy in (1, 2):
+        ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1), strategy)
+        assert np.array_equal(host(ck), ek)
+        assert orc.allclose_fuzzy(host(cv), ev, TOL)
+
+
+def test_lattice_mult_sort_strategy_is_deterministic(ctx, orc):
+    ak, av = _random_lattice(orc, 4096, 512, 1, 300000, 8)
+    bk, bv = _random_lattice(orc, 512, 2048, 1, 20000, 9)
+    outs = [ctx.lattice_mult(gpu(ak), gpu(av), (4096, 512, 1), gpu(bk), gpu(bv), (512, 2048, 1), 1) for _ in range(3)]
+    for ck, cv in outs[1:]:
+        assert np.array_equal(host(ck), host(outs[0][0]))
+        assert np.array_equal(host(cv).view(np.int64), host(outs[0][1]).view(np.int64))
+
+
+def test_lattice_mult_empty_and_errors(ctx, orc):
+    import libnt_amd
+    import torch
+
+    e = torch.empty(0, dtype=torch.int64, device="cuda")
+    ev = torch.empty(0, dtype=torch.float64, device="cuda")
+    ak, av = _random_lattice(orc, 10, 10, 1, 20, 1)
+    ck, cv = ctx.lattice_mult(e, ev, (10, 10, 1), gpu(ak), gpu(av), (10, 10, 1))
+    assert ck.shape[0] == 0
+    with pytest.raises(libnt_amd.LntError) as err:
+        ctx.lattice_mult(gpu(ak), gpu(av), (10, 10, 1), gpu(ak), gpu(av), (9, 10, 1))
+    assert err.value.status == 7
+
+
+def test_lattice_mult_host_memory(ctx, orc):
+    ak, av = _random_lattice(orc, 300, 200, 2, 5000, 11)
+    bk, bv = _random_lattice(orc, 200, 100, 2, 3000, 12)
+    ek, ev = orc.lattice_mult(ak, av, (300, 200, 2), bk, bv, (200, 100, 2))
+    ck, cv = ctx.lattice_mult(ak, av, (300, 200, 2), bk, bv, (200, 100, 2))
+    assert np.array_equal(ck, ek) and orc.allclose_fuzzy(cv, ev, TOL)
+
+
+# ------------------------------------------------ full MIA contractions
+def _c1_operand(orc, seed):
+    keys = orc.gen_distinct_keys(10486, 1 << 20, seed)
+    vals = orc.gen_uniform_vals(10486, seed + 1000)
+    return keys, vals
+
+
+def test_c1_contraction(orc):
+    """C(i,j,m,n) = A(i,j,k,l) * B(k,l,m,n), dims 32, 1% density, unsorted
+    operands in the default order."""
+    from libnt_amd.sparse import SparseMIA
+
+    ak, av = _c1_operand(orc, 1)
+    bk, bv = _c1_operand(orc, 2)
+    dims = [32, 32, 32, 32]
+    ek, ev, edims = orc.contract(ak, av, dims, [0, 1, 2, 3], False, "ijkl", bk, bv, dims, [0, 1, 2, 3], False, "klmn")
+    A = SparseMIA(dims, ak, av)
+    B = SparseMIA(dims, bk, bv)
+    C = (A("ijkl") * B("klmn")).mia
+    assert list(C.dims) == list(edims)
+    assert np.array_equal(host(C.keys), ek)
+    assert orc.allclose_fuzzy(host(C.vals), ev, TOL)
+
+
+@pytest.mark.parametrize(
+    "la,lb,dims_a,dims_b",
+    [
+        ("kij", "jlk", [40, 50, 60], [60, 70, 40]),
+        ("iJk", "kJm", [50, 30, 40], [40, 30, 60]),
+        ("ij", "kl", [30, 40], [50, 20]),
+        ("IJK", "KIJ", [30, 20, 10], [10, 30, 20]),
+        ("aIbc", "cbdI", [9, 8, 7, 6], [6, 7, 5, 8]),
+    ],
+)
+def test_contractions_match_oracle(orc, la, lb, dims_a, dims_b):
+    from libnt_amd.sparse import SparseMIA
+
+    na = max(1, int(0.05 * np.prod(dims_a)))
+    nb = max(1, int(0.05 * np.prod(dims_b)))
+    ak = orc.gen_distinct_keys(na, int(np.prod(dims_a)), 5)
+    av = orc.gen_uniform_vals(na, 6)
+    bk = orc.gen_distinct_keys(nb, int(np.prod(dims_b)), 7)
+    bv = orc.gen_uniform_vals(nb, 8)
+    ident_a, ident_b = list(range(len(dims_a))), list(range(len(dims_b)))
+    ek, ev, edims = orc.contract(ak, av, dims_a, ident_a, False, la, bk, bv, dims_b, ident_b, False, lb)
+    C = (SparseMIA(dims_a, ak, av)(la) * SparseMIA(dims_b, bk, bv)(lb)).mia
+    assert list(C.dims) == list(edims)
+    assert np.array_equal(host(C.keys), ek)
+    assert orc.allclose_fuzzy(host(C.vals), ev, TOL)
+
+
+def test_front_end_merge_rules(orc):
+    """C(i,j,k,l) = A(i,j,k,l) + B(k,l,i,j) against dense arithmetic, for every
+    combination of sorted flags (SparseMIABase.h:1404-1455)."""
+    from libnt_amd.sparse import SparseMIA
+
+    dims_a = [6, 5, 4, 3]
+    dims_b = [4, 3, 6, 5]
+    ak = orc.gen_distinct_keys(150, 360, 1)
+    av = orc.gen_uniform_vals(150, 2)
+    bk = orc.gen_distinct_keys(200, 360, 3)
+    bv = orc.gen_uniform_vals(200, 4)
+    for a_sorted in (False, True):
+        for b_sorted in (False, True):
+            A = SparseMIA(dims_a, ak, av)
+            B = SparseMIA(dims_b, bk, bv)
+            if a_sorted:
+                A.sort([2, 0, 1, 3])
+            if b_sorted:
+                B.sort([3, 1, 0, 2])
+            Ad, Bd = A.to_dense(), B.to_dense()
+            Bt = Bd.transpose(2, 3, 0, 1)  # B(k,l,i,j) as (i,j,k,l)
+            for op, fn in (("+", np.add), ("-", np.subtract)):
+                C = A("ijkl") + B("klij") if op == "+" else A("ijkl") - B("klij")
+                assert C.dims == tuple(dims_a)
+                D = C.to_dense()
+                assert np.array_equal(D, fn(Ad, Bt))
+                assert C.nnz == int(np.count_nonzero((Ad != 0) | (Bt != 0)))
+
+
+def test_front_end_hadamard(orc):
+    from libnt_amd.sparse import SparseMIA
+
+    dims = [16, 16, 16]
+    ak = orc.gen_distinct_keys(1500, 4096, 1)
+    av = orc.gen_uniform_vals(1500, 2)
+    bk = orc.gen_distinct_keys(1500, 4096, 3)
+    bv = orc.gen_uniform_vals(1500, 4)
+    A, B = SparseMIA(dims, ak, av), SparseMIA(dims, bk, bv)
+    Ad, Bd = A.to_dense(), B.to_dense()
+    C = (A("IJK") * B("KIJ")).mia
+    D = C.to_dense()
+    expect = Ad * Bd.transpose(1, 2, 0)
+    assert np.array_equal(D, expect)
+
+
+@pytest.mark.parametrize("strategy", [1, 2])
+def test_c3_skewed_contraction_full_size(orc, strategy):
+    """A(i,j,k) * B(k,l,m): A 4096x4096x512, B 512x4096x64, density 1e-4, i and
+    l Zipf(1.1), values N(0,1) -- the full-size skewed configuration."""
+    import torch
+
+    from libnt_amd.sparse import SparseMIA
+
+    dims_a, dims_b = [4096, 4096, 512], [512, 4096, 64]
+    ak, av = orc.gen_coords(858993, dims_a, [1, 0, 0], 1.1, 1, 31)
+    bk, bv = orc.gen_coords(13422, dims_b, [0, 1, 0], 1.1, 1, 32)
+    ek, ev, edims = orc.contract(ak, av, dims_a, [0, 1, 2], False, "ijk", bk, bv, dims_b, [0, 1, 2], False, "klm")
+    A, B = SparseMIA(dims_a, ak, av), SparseMIA(dims_b, bk, bv)
+    A.sort([0, 1, 2])
+    B.sort([0, 1, 2])
+    ck, cv = A.ctx.lattice_mult(A.keys, A.vals, (4096 * 4096, 512, 1), B.keys, B.vals, (512, 4096 * 64, 1), strategy)
+    assert np.array_equal(host(ck), ek)
+    assert orc.allclose_fuzzy(host(cv), ev, TOL)
