@@ -1,0 +1,427 @@
+// SparseMIA.hpp -- header-only C++17 front end over the C-ABI of lnt.h.
+//
+// Mirrors the reference's sparse multi-index-array API so that code written
+// against it keeps compiling:
+//   NTINDEX i; NTINDEX j; ...                         (Index.h:27-30)
+//   lnt::SparseMIA<double,4> A(32,32,32,32);          (SparseMIA.h:167-773)
+//   C(i,j,m,n) = A(i,j,k,l) * B(k,l,m,n);              (MIA_Expr.h:864-925)
+//   C(i,j,k,l) = A(i,j,k,l) + B(k,l,i,j);              (MIA_Expr.h:1185-1250)
+//   C(i,j) = A(!i,!j) * B(!i,!j);                      Hadamard (MIA_Expr.h:1012-1059)
+//   A.sort(seq); A.change_linIdx_sequence(seq);        (SparseMIABase.h:590-814)
+// Storage is the reference's: a std::vector of int64 linear indices and one of
+// values, computed in the mode order linIdxSequence().  Every operation runs
+// on the GPU through lnt.h (host arrays are staged to HBM and back); only
+// double values are supported.  Errors become exceptions, as in the reference
+// (LibMIAException.h:25-92).
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../lnt.h"
+
+namespace lnt {
+
+// ------------------------------------------------------------- exceptions --
+struct MIAException : std::runtime_error {
+  int status;
+  MIAException(int s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+struct DimensionMismatchException : MIAException {
+  using MIAException::MIAException;
+};
+struct MIAParameterException : MIAException {
+  using MIAException::MIAException;
+};
+
+inline void check(int status) {
+  if (status == LNT_OK) return;
+  std::string msg = lnt_last_error();
+  if (status == LNT_ERR_DIMENSION_MISMATCH) throw DimensionMismatchException(status, msg);
+  if (status == LNT_ERR_INVALID_ARGUMENT) throw MIAParameterException(status, msg);
+  throw MIAException(status, msg);
+}
+
+// ---------------------------------------------------------------- context --
+class Context {
+ public:
+  explicit Context(int device = 0, void *stream = nullptr) { check(lnt_context_create(&ctx_, device, stream)); }
+  ~Context() { lnt_context_destroy(ctx_); }
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+  lnt_context get() const { return ctx_; }
+
+ private:
+  lnt_context ctx_ = nullptr;
+};
+
+// One context per host thread (the C-ABI contexts are not shared).
+inline Context &default_context() {
+  thread_local std::unique_ptr<Context> ctx;
+  if (!ctx) ctx.reset(new Context(0));
+  return *ctx;
+}
+
+// ------------------------------------------------------------------ index --
+// An index label.  `!i` marks an inter-product (element-wise) index.
+template <size_t ID, int ElemWise = 0>
+struct ProdInd {
+  static constexpr size_t id = ID;
+  static constexpr int elemval = ElemWise;
+  ProdInd<ID, ElemWise + 1> operator!() const { return {}; }
+};
+struct Label {
+  size_t id;
+  bool inter;
+  bool operator==(const Label &o) const { return id == o.id && inter == o.inter; }
+};
+
+template <class T, size_t N>
+class SparseMIA;
+
+namespace detail {
+
+inline int64_t product(const std::vector<int64_t> &v) {
+  int64_t p = 1;
+  for (int64_t x : v) p *= x;
+  return p;
+}
+
+// Type-erased tensor used by expressions (any order).
+struct Tensor {
+  std::vector<int64_t> dims;
+  std::vector<int32_t> seq;
+  std::vector<int64_t> idx;
+  std::vector<double> data;
+  bool sorted = false;
+
+  void sort_to(const std::vector<int32_t> &new_seq) {
+    if (!idx.empty())
+      check(lnt_change_sort_order(default_context().get(), idx.data(), data.data(), (int64_t)idx.size(),
+                                  (int)dims.size(), dims.data(), seq.data(), new_seq.data(), sorted ? 1 : 0,
+                                  LNT_MEM_HOST));
+    seq = new_seq;
+    sorted = true;
+  }
+};
+
+struct Expr {
+  std::shared_ptr<Tensor> t;  // owning result or a view of a SparseMIA's storage
+  std::vector<Label> labels;
+};
+
+inline std::vector<int32_t> identity(size_t n) {
+  std::vector<int32_t> s(n);
+  for (size_t i = 0; i < n; ++i) s[i] = (int32_t)i;
+  return s;
+}
+
+inline int find_label(const std::vector<Label> &ls, size_t id) {
+  for (size_t i = 0; i < ls.size(); ++i)
+    if (ls[i].id == id) return (int)i;
+  return -1;
+}
+
+// outside_merge -> outside_scanMerge (SparseMIABase.h:1357-1455).
+inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const std::vector<Label> &lb, int op) {
+  if (la.size() != lb.size()) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "merge orders differ");
+  Tensor b_copy;
+  Tensor *bp = &b_in;
+  if (&a == &b_in) {
+    b_copy = b_in;
+    bp = &b_copy;
+  }
+  Tensor &b = *bp;
+  const size_t n = la.size();
+  std::vector<int32_t> index_order(n), lhs_order(n);
+  for (size_t m = 0; m < n; ++m) {
+    int r = find_label(lb, la[m].id);
+    if (r < 0 || la[m].inter || lb[r].inter)
+      throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "merge operands need the same plain indices");
+    if (a.dims[m] != b.dims[r]) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "merge dims differ");
+    index_order[m] = r;
+    lhs_order[r] = (int32_t)m;
+  }
+  std::vector<int32_t> b_in_a(n), a_in_b(n);
+  for (size_t t = 0; t < n; ++t) b_in_a[t] = index_order[a.seq[t]];
+  for (size_t t = 0; t < n; ++t) a_in_b[t] = lhs_order[b.seq[t]];
+  if (a.sorted && b.sorted) {
+    if (b.idx.size() < a.idx.size()) b.sort_to(b_in_a);
+    else a.sort_to(a_in_b);
+  } else if (b.sorted) {
+    a.sort_to(a_in_b);
+  } else if (a.sorted) {
+    b.sort_to(b_in_a);
+  } else {
+    a.sort_to(a.seq);
+    for (size_t t = 0; t < n; ++t) b_in_a[t] = index_order[a.seq[t]];
+    b.sort_to(b_in_a);
+  }
+  auto c = std::make_shared<Tensor>();
+  c->dims = a.dims;
+  c->seq = a.seq;
+  c->sorted = true;
+  int64_t *ck = nullptr;
+  double *cv = nullptr;
+  int64_t nc = 0;
+  check(lnt_merge(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), b.idx.data(),
+                  b.data.data(), (int64_t)b.idx.size(), op, &ck, &cv, &nc, LNT_MEM_HOST));
+  c->idx.assign(ck, ck + nc);
+  c->data.assign(cv, cv + nc);
+  lnt_free(default_context().get(), ck, LNT_MEM_HOST);
+  lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+  return {c, la};
+}
+
+// Lattice product / Hadamard (MIA_Expr.h:449-561, 864-925, 1012-1059).
+inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const std::vector<Label> &lb) {
+  Tensor b_copy;
+  Tensor *bp = &b_in;
+  if (&a == &b_in) {
+    b_copy = b_in;
+    bp = &b_copy;
+  }
+  Tensor &b = *bp;
+  std::vector<int32_t> a_out, a_in, a_int, b_in_m, b_int, b_out;
+  for (size_t m = 0; m < la.size(); ++m) {
+    int r = find_label(lb, la[m].id);
+    if (r < 0) {
+      if (la[m].inter) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "inter index in one operand only");
+      a_out.push_back((int32_t)m);
+      continue;
+    }
+    if (la[m].inter != lb[r].inter)
+      throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "an index is inter on one side only");
+    if (a.dims[m] != b.dims[r]) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "matched dims differ");
+    if (la[m].inter) {
+      a_int.push_back((int32_t)m);
+      b_int.push_back(r);
+    } else {
+      a_in.push_back((int32_t)m);
+      b_in_m.push_back(r);
+    }
+  }
+  for (size_t r = 0; r < lb.size(); ++r)
+    if (find_label(la, lb[r].id) < 0) {
+      if (lb[r].inter) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "inter index in one operand only");
+      b_out.push_back((int32_t)r);
+    }
+  std::vector<int32_t> seq_a, seq_b;
+  for (auto v : {&a_out, &a_in, &a_int}) seq_a.insert(seq_a.end(), v->begin(), v->end());
+  for (auto v : {&b_in_m, &b_out, &b_int}) seq_b.insert(seq_b.end(), v->begin(), v->end());
+  auto dprod = [](const Tensor &t, const std::vector<int32_t> &ms) {
+    int64_t p = 1;
+    for (int32_t m : ms) p *= t.dims[m];
+    return p;
+  };
+  const int64_t lat_a[3] = {dprod(a, a_out), dprod(a, a_in), dprod(a, a_int)};
+  const int64_t lat_b[3] = {lat_a[1], dprod(b, b_out), lat_a[2]};
+  auto c = std::make_shared<Tensor>();
+  std::vector<Label> lc;
+  for (int32_t m : a_out) c->dims.push_back(a.dims[m]), lc.push_back(la[m]);
+  for (int32_t r : b_out) c->dims.push_back(b.dims[r]), lc.push_back(lb[r]);
+  for (int32_t m : a_int) c->dims.push_back(a.dims[m]), lc.push_back(la[m]);
+  if (c->dims.empty()) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "full contraction to a scalar");
+  c->seq = identity(c->dims.size());
+  c->sorted = true;
+  a.sort_to(seq_a);
+  b.sort_to(seq_b);
+  int64_t *ck = nullptr;
+  double *cv = nullptr;
+  int64_t nc = 0;
+  if (a_in.empty() && a_out.empty() && b_out.empty())
+    check(lnt_merge(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), b.idx.data(),
+                    b.data.data(), (int64_t)b.idx.size(), LNT_OP_HADAMARD, &ck, &cv, &nc, LNT_MEM_HOST));
+  else
+    check(lnt_lattice_mult(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), lat_a,
+                           b.idx.data(), b.data.data(), (int64_t)b.idx.size(), lat_b, LNT_SPGEMM_AUTO, &ck, &cv,
+                           &nc, LNT_MEM_HOST));
+  c->idx.assign(ck, ck + nc);
+  c->data.assign(cv, cv + nc);
+  lnt_free(default_context().get(), ck, LNT_MEM_HOST);
+  lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+  a.sorted = false;  // lattice unfolding leaves operands flagged unsorted
+  b.sorted = false;
+  return {c, lc};
+}
+
+}  // namespace detail
+
+// Expression node: A(i,j,...) or the result of an operation on atoms.
+class Atom {
+ public:
+  Atom(detail::Expr e) : e_(std::move(e)) {}
+  Atom operator+(const Atom &o) const { return detail::merge(*e_.t, e_.labels, *o.e_.t, o.e_.labels, LNT_OP_ADD); }
+  Atom operator-(const Atom &o) const { return detail::merge(*e_.t, e_.labels, *o.e_.t, o.e_.labels, LNT_OP_SUB); }
+  Atom operator*(const Atom &o) const { return detail::multiply(*e_.t, e_.labels, *o.e_.t, o.e_.labels); }
+  const detail::Expr &expr() const { return e_; }
+
+ private:
+  detail::Expr e_;
+};
+
+// Left-hand side of an assignment: C(i,j,...) = expr (SparseMIA::assign,
+// SparseMIA.h:891-928 -- relabels the mode order, moves no index data).
+template <class MIA>
+class AssignTarget {
+ public:
+  AssignTarget(MIA &m, std::vector<Label> labels) : m_(m), labels_(std::move(labels)) {}
+  operator Atom() const { return Atom(detail::Expr{m_.tensor_ptr(), labels_}); }
+  Atom operator+(const Atom &o) const { return Atom(*this) + o; }
+  Atom operator-(const Atom &o) const { return Atom(*this) - o; }
+  Atom operator*(const Atom &o) const { return Atom(*this) * o; }
+  Atom operator+(const AssignTarget &o) const { return Atom(*this) + Atom(o); }
+  Atom operator-(const AssignTarget &o) const { return Atom(*this) - Atom(o); }
+  Atom operator*(const AssignTarget &o) const { return Atom(*this) * Atom(o); }
+  AssignTarget &operator=(const Atom &rhs) {
+    const detail::Expr &e = rhs.expr();
+    if (e.labels.size() != labels_.size())
+      throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "assignment order differs");
+    const size_t n = labels_.size();
+    std::vector<int32_t> perm(n), inv(n);  // C mode m <- result mode perm[m]
+    for (size_t m = 0; m < n; ++m) {
+      int p = detail::find_label(e.labels, labels_[m].id);
+      if (p < 0) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "assignment indices differ");
+      perm[m] = p;
+      inv[p] = (int32_t)m;
+    }
+    detail::Tensor &t = *m_.tensor_ptr();
+    detail::Tensor src = *e.t;
+    std::vector<int64_t> dims(n);
+    for (size_t m = 0; m < n; ++m) dims[m] = src.dims[perm[m]];
+    if (dims != t.dims) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "assignment dims differ");
+    std::vector<int32_t> seq(n);
+    for (size_t q = 0; q < n; ++q) seq[q] = inv[src.seq[q]];
+    t.idx = std::move(src.idx);
+    t.data = std::move(src.data);
+    t.seq = seq;
+    t.sorted = src.sorted;
+    return *this;
+  }
+  AssignTarget &operator=(const AssignTarget &rhs) { return *this = Atom(rhs); }
+
+ private:
+  MIA &m_;
+  std::vector<Label> labels_;
+};
+
+// ------------------------------------------------------------- SparseMIA --
+template <class T, size_t N>
+class SparseMIA {
+  static_assert(std::is_same<T, double>::value, "the GPU path computes in double");
+
+ public:
+  using index_type = int64_t;
+  using data_type = T;
+
+  SparseMIA() : t_(std::make_shared<detail::Tensor>()) {
+    t_->dims.assign(N, 0);
+    t_->seq = detail::identity(N);
+  }
+  template <class... D, class = typename std::enable_if<sizeof...(D) == N>::type>
+  explicit SparseMIA(D... dims) : SparseMIA() {
+    t_->dims = {static_cast<int64_t>(dims)...};
+  }
+  explicit SparseMIA(const std::array<int64_t, N> &dims) : SparseMIA() { t_->dims.assign(dims.begin(), dims.end()); }
+  SparseMIA(const SparseMIA &o) : t_(std::make_shared<detail::Tensor>(*o.t_)) {}
+  SparseMIA &operator=(const SparseMIA &o) {
+    *t_ = *o.t_;
+    return *this;
+  }
+
+  // ---- storage (SparseMIA.h:61-65, 203-206)
+  size_t size() const { return t_->idx.size(); }
+  std::array<int64_t, N> dims() const {
+    std::array<int64_t, N> d;
+    std::copy(t_->dims.begin(), t_->dims.end(), d.begin());
+    return d;
+  }
+  int64_t dim(size_t i) const { return t_->dims[i]; }
+  int64_t dimensionality() const { return detail::product(t_->dims); }
+  std::array<size_t, N> linIdxSequence() const {
+    std::array<size_t, N> s;
+    for (size_t i = 0; i < N; ++i) s[i] = (size_t)t_->seq[i];
+    return s;
+  }
+  bool is_sorted() const { return t_->sorted; }
+  void setSorted(bool s) { t_->sorted = s; }
+  void push_back(T value, int64_t index) {
+    t_->data.push_back(value);
+    t_->idx.push_back(index);
+    t_->sorted = false;
+  }
+  void reserve(size_t n) {
+    t_->idx.reserve(n);
+    t_->data.reserve(n);
+  }
+  void resize(size_t n) {
+    t_->idx.resize(n);
+    t_->data.resize(n);
+  }
+  int64_t *index_begin() { return t_->idx.data(); }
+  int64_t *index_end() { return t_->idx.data() + t_->idx.size(); }
+  T *data_begin() { return t_->data.data(); }
+  T *data_end() { return t_->data.data() + t_->data.size(); }
+  const std::vector<int64_t> &indices() const { return t_->idx; }
+  const std::vector<T> &data() const { return t_->data; }
+
+  // ---- linear-index math (IndexUtil.h:318-333, 866-879)
+  int64_t sub2ind(const std::array<int64_t, N> &sub) const {
+    int64_t k = 0, mult = 1;
+    for (size_t t = 0; t < N; ++t) {
+      k += sub[t_->seq[t]] * mult;
+      mult *= t_->dims[t_->seq[t]];
+    }
+    return k;
+  }
+  std::array<int64_t, N> ind2sub(int64_t k) const {
+    std::array<int64_t, N> sub{};
+    for (size_t t = 0; t < N; ++t) {
+      sub[t_->seq[t]] = k % t_->dims[t_->seq[t]];
+      k /= t_->dims[t_->seq[t]];
+    }
+    return sub;
+  }
+
+  // ---- sorts (SparseMIABase.h:590-645, 701-734, 768-814)
+  void sort() { sort(linIdxSequence()); }
+  template <class S>
+  void sort(const std::array<S, N> &seq) {
+    std::vector<int32_t> s(seq.begin(), seq.end());
+    t_->sort_to(s);
+  }
+  template <class S>
+  void change_linIdx_sequence(const std::array<S, N> &seq) {
+    std::vector<int32_t> s(seq.begin(), seq.end());
+    if (s == t_->seq || N == 1) return;
+    if (!t_->idx.empty())
+      check(lnt_change_linidx(default_context().get(), t_->idx.data(), (int64_t)t_->idx.size(), (int)N,
+                              t_->dims.data(), t_->seq.data(), s.data(), LNT_MEM_HOST));
+    t_->seq = s;
+    t_->sorted = false;
+  }
+
+  // ---- index expressions (MIA.h:222-229)
+  template <class... Ts>
+  AssignTarget<SparseMIA> operator()(Ts... ts) {
+    static_assert(sizeof...(Ts) == N, "one index per mode");
+    return AssignTarget<SparseMIA>(*this, std::vector<Label>{Label{Ts::id, Ts::elemval > 0}...});
+  }
+
+  std::shared_ptr<detail::Tensor> tensor_ptr() { return t_; }
+
+ private:
+  std::shared_ptr<detail::Tensor> t_;
+};
+
+}  // namespace lnt
+
+#define LNT_UNIQUE_ID __COUNTER__
+#define NTINDEX ::lnt::ProdInd<LNT_UNIQUE_ID>
+#define MIAINDEX ::lnt::ProdInd<LNT_UNIQUE_ID>

@@ -1,0 +1,129 @@
+// C++ front-end test in the style of the reference's own tests
+// (src/tests/SparseMIA/sparse_mia_functions.cpp:100-202,
+// sparse_mia_add_sub_tests.cpp, sparse_mia_mult_test.cpp): every sparse result
+// is compared with dense brute force computed here on the CPU.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <random>
+#include <vector>
+
+#include "lnt/SparseMIA.hpp"
+
+static int failures = 0;
+#define CHECK(cond, msg)                                  \
+  do {                                                    \
+    if (!(cond)) {                                        \
+      std::printf("FAIL: %s (line %d)\n", msg, __LINE__); \
+      ++failures;                                         \
+    }                                                     \
+  } while (0)
+
+template <size_t N>
+static void fill_random(lnt::SparseMIA<double, N> &m, size_t nnz, unsigned seed) {
+  std::mt19937_64 rng(seed);
+  std::map<int64_t, double> entries;
+  std::uniform_int_distribution<int64_t> key(0, m.dimensionality() - 1);
+  std::uniform_real_distribution<double> val(-1.0, 1.0);
+  while (entries.size() < nnz) entries[key(rng)] = val(rng);
+  std::vector<std::pair<int64_t, double>> v(entries.begin(), entries.end());
+  std::shuffle(v.begin(), v.end(), rng);
+  for (auto &e : v) m.push_back(e.second, e.first);
+}
+
+// Dense value at full subscript `sub` (default first-mode-fastest layout).
+template <size_t N>
+static std::vector<double> dense(const lnt::SparseMIA<double, N> &m) {
+  std::vector<double> d(m.dimensionality(), 0.0);
+  for (size_t q = 0; q < m.size(); ++q) {
+    auto sub = m.ind2sub(m.indices()[q]);
+    int64_t k = 0, mult = 1;
+    for (size_t t = 0; t < N; ++t) {
+      k += sub[t] * mult;
+      mult *= m.dim(t);
+    }
+    d[k] = m.data()[q];
+  }
+  return d;
+}
+
+int main() {
+  NTINDEX i; NTINDEX j; NTINDEX k; NTINDEX l; NTINDEX m; NTINDEX n;
+  // ---- sort(seq) == change_linIdx_sequence(seq) + sort()  (sparse_mia_functions.cpp:100-202)
+  {
+    lnt::SparseMIA<double, 4> a(7, 9, 5, 9), b(7, 9, 5, 9);
+    fill_random(a, 1200, 1);
+    b = a;
+    std::array<size_t, 4> seq{{2, 0, 3, 1}};
+    a.sort(seq);
+    b.change_linIdx_sequence(seq);
+    b.sort();
+    CHECK(a.indices() == b.indices() && a.data() == b.data(), "sort(seq) differs from relinearise+sort");
+    for (size_t q = 1; q < a.size(); ++q) CHECK(a.indices()[q - 1] < a.indices()[q], "not ascending");
+  }
+  // ---- C(i,j,k,l) = A(i,j,k,l) +/- B(k,l,i,j)  (sparse_mia_add_sub_tests.cpp)
+  {
+    lnt::SparseMIA<double, 4> a(6, 5, 4, 3), b(4, 3, 6, 5), c(6, 5, 4, 3), c2(6, 5, 4, 3);
+    fill_random(a, 150, 2);
+    fill_random(b, 200, 3);
+    auto da = dense(a), db = dense(b);
+    c(i, j, k, l) = a(i, j, k, l) + b(k, l, i, j);
+    c2(i, j, k, l) = a(i, j, k, l) - b(k, l, i, j);
+    auto dc = dense(c), dc2 = dense(c2);
+    for (int64_t q = 0; q < c.dimensionality(); ++q) {
+      int64_t ii = q % 6, jj = (q / 6) % 5, kk = (q / 30) % 4, ll = q / 120;
+      double bv = db[kk + 4 * ll + 12 * ii + 72 * jj];
+      CHECK(dc[q] == da[q] + bv, "A+B mismatch");
+      CHECK(dc2[q] == da[q] - bv, "A-B mismatch");
+    }
+  }
+  // ---- C(i,j,m,n) = A(i,j,k,l) * B(k,l,m,n)  (sparse_mia_mult_test.cpp)
+  {
+    lnt::SparseMIA<double, 4> a(5, 4, 6, 3), b(6, 3, 4, 5), c(5, 4, 4, 5);
+    fill_random(a, 120, 4);
+    fill_random(b, 110, 5);
+    auto da = dense(a), db = dense(b);
+    c(i, j, m, n) = a(i, j, k, l) * b(k, l, m, n);
+    auto dc = dense(c);
+    double max_err = 0;
+    for (int ii = 0; ii < 5; ++ii)
+      for (int jj = 0; jj < 4; ++jj)
+        for (int mm = 0; mm < 4; ++mm)
+          for (int nn = 0; nn < 5; ++nn) {
+            double s = 0;
+            for (int kk = 0; kk < 6; ++kk)
+              for (int ll = 0; ll < 3; ++ll) s += da[ii + 5 * jj + 20 * kk + 120 * ll] * db[kk + 6 * ll + 18 * mm + 72 * nn];
+            max_err = std::max(max_err, std::fabs(s - dc[ii + 5 * jj + 20 * mm + 80 * nn]));
+          }
+    CHECK(max_err < 1e-12, "contraction mismatch");
+  }
+  // ---- Hadamard C(i,j) = A(!i,!j) * B(!j,!i)
+  {
+    lnt::SparseMIA<double, 2> a(30, 20), b(20, 30), c(30, 20);
+    fill_random(a, 300, 6);
+    fill_random(b, 300, 7);
+    auto da = dense(a), db = dense(b);
+    c(i, j) = a(!i, !j) * b(!j, !i);
+    auto dc = dense(c);
+    for (int ii = 0; ii < 30; ++ii)
+      for (int jj = 0; jj < 20; ++jj)
+        CHECK(dc[ii + 30 * jj] == da[ii + 30 * jj] * db[jj + 20 * ii], "Hadamard mismatch");
+  }
+  // ---- errors become exceptions (LibMIAException.h)
+  {
+    lnt::SparseMIA<double, 2> a(3, 4), b(5, 4), c(3, 4);
+    fill_random(a, 5, 8);
+    fill_random(b, 5, 9);
+    bool thrown = false;
+    try {
+      c(i, j) = a(i, j) + b(i, j);
+    } catch (const lnt::DimensionMismatchException &) {
+      thrown = true;
+    }
+    CHECK(thrown, "dimension mismatch not reported");
+  }
+  if (failures) return 1;
+  std::printf("C++ front end: all checks passed\n");
+  return 0;
+}
