@@ -87,6 +87,15 @@ int lnt_abi_version(void);
  * 16 B per input and per output element; relinearisation: 16 B per key).  No
  * reference counterpart (the reference has only ad-hoc std::chrono prints,
  * multi_index_array/SparseLatticeBase.h:1885,1981-1982). */
+/* Tuning parameters (the autotuner's knobs; results never depend on them):
+ *   "sort_items"   onesweep items per thread: 8, 12 or 16 (tile = 256*items)
+ *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096
+ *   "merge_items"  merge-path steps per thread: 8 or 16 (tile = 256*items)
+ *   "spgemm_small" longest output column (products) kept in LDS, 2..2048
+ * No reference counterpart (the reference hard-codes radix 2048 and the
+ * thresholds 3000/50, multi_index_array/LibMIARadix.h:454-457). */
+int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value);
+int lnt_context_get_param(lnt_context ctx, const char *name, int64_t *value);
 int lnt_context_set_profiling(lnt_context ctx, int on);
 int lnt_context_profile(lnt_context ctx, const char *kernel, int64_t *launches,
                         double *total_ms, double *bytes);

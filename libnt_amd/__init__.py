@@ -48,6 +48,8 @@ SIGNATURES = {
     "lnt_context_synchronize": [_vp],
     "lnt_context_trim": [_vp],
     "lnt_context_set_validation": [_vp, _int],
+    "lnt_context_set_param": [_vp, ctypes.c_char_p, _i64],
+    "lnt_context_get_param": [_vp, ctypes.c_char_p, _i64p],
     "lnt_context_set_profiling": [_vp, _int],
     "lnt_context_profile": [_vp, ctypes.c_char_p, _i64p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     "lnt_change_linidx": [_vp, _vp, _i64, _int, _i64p, _i32p, _i32p, _int],
@@ -158,6 +160,14 @@ class Context:
 
     def set_validation(self, level: int):
         check(lib().lnt_context_set_validation(self._h, level))
+
+    def set_param(self, name: str, value: int):
+        check(lib().lnt_context_set_param(self._h, name.encode(), int(value)))
+
+    def get_param(self, name: str) -> int:
+        v = ctypes.c_int64()
+        check(lib().lnt_context_get_param(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
 
     def set_profiling(self, on: bool):
         check(lib().lnt_context_set_profiling(self._h, int(bool(on))))

@@ -79,10 +79,25 @@ int make_relin(int order, const int64_t *dims, const int32_t *cur_seq,
     acc *= (uint64_t)dims[new_seq[t]];
   }
   acc = 1;
+  r.pow2 = 1;
   for (int t = 0; t < order; ++t) {
     r.place[t] = make_divider(acc);
     r.weight[t] = stride_of_mode[cur_seq[t]];
     acc *= (uint64_t)dims[cur_seq[t]];
+    r.pow2 &= (dims[t] & (dims[t] - 1)) == 0;
+  }
+  if (r.pow2) {
+    uint32_t sh = 0;
+    for (int t = 0; t < order; ++t) {
+      uint32_t b = 0;
+      while ((uint64_t(1) << b) < (uint64_t)dims[cur_seq[t]]) ++b;
+      uint32_t ns = 0;
+      while ((uint64_t(1) << ns) < r.weight[t]) ++ns;
+      r.old_shift[t] = (uint8_t)sh;
+      r.bits[t] = (uint8_t)b;
+      r.new_shift[t] = (uint8_t)ns;
+      sh += b;
+    }
   }
   *out = r;
   *dimensionality = (uint64_t)total;
@@ -234,12 +249,35 @@ int lnt_context_destroy(lnt_context ctx) {
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
                           &ctx->host_tmp_b, &ctx->host_tmp_c, &ctx->spg_ws, &ctx->spg_tmp,
-                          &ctx->spg_big})
+                          &ctx->spg_big, &ctx->hist_part})
     release(*b);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   prof_clear(ctx);
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   delete ctx;
+  return LNT_OK;
+}
+
+int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
+  if (!ctx || !name) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  auto in = [&](int64_t lo, int64_t hi) { return value >= lo && value <= hi; };
+  std::string n(name);
+  if (n == "sort_items" && (value == 8 || value == 12 || value == 16)) ctx->tuning.items_per_thread = (int)value;
+  else if (n == "hist_blocks" && in(1, kMaxHistBlocks)) ctx->tuning.hist_blocks = (int)value;
+  else if (n == "merge_items" && (value == 8 || value == 16)) ctx->tuning.merge_items = (int)value;
+  else if (n == "spgemm_small" && in(2, 2048)) ctx->tuning.spgemm_small = (int)value;
+  else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter or value out of range: " + n);
+  return LNT_OK;
+}
+
+int lnt_context_get_param(lnt_context ctx, const char *name, int64_t *value) {
+  if (!ctx || !name || !value) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  std::string n(name);
+  if (n == "sort_items") *value = ctx->tuning.items_per_thread;
+  else if (n == "hist_blocks") *value = ctx->tuning.hist_blocks;
+  else if (n == "merge_items") *value = ctx->tuning.merge_items;
+  else if (n == "spgemm_small") *value = ctx->tuning.spgemm_small;
+  else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter: " + n);
   return LNT_OK;
 }
 

@@ -81,11 +81,22 @@ __host__ __device__ inline uint64_t divide(uint64_t n, const Divider &v) {
 struct Relin {
   int32_t order;
   int32_t active;  // 0: identity
+  int32_t pow2;    // every dimension is a power of two: shifts and masks only
+  int32_t pad;
   Divider place[LNT_MAX_ORDER];    // old place value of old position t
   uint64_t weight[LNT_MAX_ORDER];  // new place value of the mode at old position t
+  uint8_t old_shift[LNT_MAX_ORDER], bits[LNT_MAX_ORDER], new_shift[LNT_MAX_ORDER];
 };
 
 __host__ __device__ inline uint64_t relin_key(const Relin &r, uint64_t idx) {
+  if (r.pow2) {
+    uint64_t out = 0;
+    for (int t = 0; t < r.order; ++t) {
+      uint64_t mask = r.bits[t] >= 64 ? ~0ull : ((uint64_t(1) << r.bits[t]) - 1);
+      out |= ((idx >> r.old_shift[t]) & mask) << r.new_shift[t];
+    }
+    return out;
+  }
   uint64_t out = 0;
   for (int t = r.order - 1; t >= 1; --t) {
     uint64_t q = divide(idx, r.place[t]);
@@ -227,6 +238,7 @@ struct SortTuning {
   int items_per_thread = 16;  // onesweep tile = 256 * items
   int hist_blocks = 1024;
   int merge_items = 8;  // merge tile = 256 * items
+  int spgemm_small = 2048;  // largest output column handled in LDS
 };
 
 // Kernel kinds recorded by the profiler (names in api.hip).
@@ -251,6 +263,7 @@ struct lnt_context_s {
   lnt::DeviceBuffer status;     // look-back words
   lnt::DeviceBuffer small;      // histograms, counters, flags
   lnt::DeviceBuffer aux;        // merge-path splits and other per-call scratch
+  lnt::DeviceBuffer hist_part;  // per-block digit histograms
   lnt::DeviceBuffer spg_ws, spg_tmp, spg_big;  // lattice-product workspaces
   lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
   uint32_t *pinned = nullptr;   // small pinned host buffer for counts/flags
@@ -266,6 +279,7 @@ constexpr size_t kSmallHist = 0;                  // [8][256] digit histograms
 constexpr size_t kSmallCounters = 8 * 256;        // [16] tile counters
 constexpr size_t kSmallFlags = kSmallCounters + 16;  // [16] error flags / totals
 constexpr size_t kSmallWords = kSmallFlags + 16;
+constexpr int kMaxHistBlocks = 4096;
 
 // Profiling: prof_begin records a start event (returns a slot, or -1 when
 // profiling is off); prof_end records the stop event of that slot.

@@ -88,14 +88,33 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(
   const uint32_t i0 = s_i0, i1 = s_i1;
   const uint32_t j0 = d0 - i0, j1 = d1 - i1;
   const uint32_t la = i1 - i0, lb = j1 - j0;
-  // A slice, B slice, then the B element after the slice (if any).
-  for (uint32_t x = tid; x < la; x += kBlock) {
-    s_key[x] = ak[i0 + x];
-    s_val[x] = av[i0 + x];
-  }
-  for (uint32_t x = tid; x < lb; x += kBlock) {
-    s_key[la + x] = bk[j0 + x];
-    s_val[la + x] = bv[j0 + x];
+  // A slice, B slice, then the B element after the slice (if any).  All of a
+  // thread's loads are issued before any is used, so a tile waits for one
+  // memory round trip, not ITEMS of them.
+  {
+    uint64_t lk[ITEMS];
+    double lv[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      uint32_t x = (uint32_t)u * kBlock + tid;
+      lk[u] = 0;
+      lv[u] = 0.0;
+      if (x < la + lb) {
+        const bool from_a = x < la;
+        const uint64_t *kp = from_a ? ak + i0 + x : bk + j0 + (x - la);
+        const double *vp = from_a ? av + i0 + x : bv + j0 + (x - la);
+        lk[u] = *kp;
+        lv[u] = *vp;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      uint32_t x = (uint32_t)u * kBlock + tid;
+      if (x < la + lb) {
+        s_key[x] = lk[u];
+        s_val[x] = lv[u];
+      }
+    }
   }
   const bool has_next_b = j1 < nb;
   if (tid == 0) {

@@ -49,7 +49,7 @@ struct DigitPlan {
 template <bool RELIN>
 __global__ __launch_bounds__(kBlock) void histogram_kernel(
     const uint64_t *__restrict__ keys, uint32_t n, DigitPlan plan, uint64_t bound,
-    Relin relin, uint32_t *__restrict__ hist, uint32_t *__restrict__ counters,
+    Relin relin, uint32_t *__restrict__ part, uint32_t *__restrict__ counters,
     uint32_t *__restrict__ err) {
   const int passes = plan.passes;
   __shared__ uint32_t h[kMaxPasses * kRadix];
@@ -83,8 +83,20 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
   }
   if (bad) atomicOr(err, 1u);
   __syncthreads();
-  for (int i = tid; i < passes * kRadix; i += kBlock)
-    if (h[i]) atomicAdd(&hist[i], h[i]);
+  // Per-block partial histograms (no contended device-wide atomics: the
+  // counters of all blocks would otherwise serialise across the XCDs' L2s).
+  uint32_t *mine = part + (size_t)blockIdx.x * passes * kRadix;
+  for (int i = tid; i < passes * kRadix; i += kBlock) mine[i] = h[i];
+}
+
+// hist[i] = sum over blocks of part[b][i], one thread per (pass, digit).
+__global__ void histogram_reduce_kernel(const uint32_t *__restrict__ part, uint32_t blocks,
+                                        uint32_t width, uint32_t *__restrict__ hist) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= width) return;
+  uint32_t acc = 0;
+  for (uint32_t b = 0; b < blocks; ++b) acc += part[(size_t)b * width + i];
+  hist[i] = acc;
 }
 
 // -------------------------------------------------------------- onesweep ---
@@ -323,19 +335,25 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   uint32_t *hist = small + kSmallHist;
   uint32_t *counters = small + kSmallCounters;
   uint32_t *err = small + kSmallFlags;
-  LNT_TRY(launch_fill_u32(ctx, hist, 0, kSmallCounters - kSmallHist));
   LNT_TRY(launch_fill_u32(ctx, err, 0, 1));
 
   uint32_t hblocks = (uint32_t)ctx->tuning.hist_blocks;
   uint32_t need = (n + kBlock * 8 - 1) / (kBlock * 8);
   if (hblocks > need) hblocks = need;
+  const uint32_t width = (uint32_t)(passes > 0 ? passes : 1) * kRadix;
+  LNT_TRY(ensure(ctx->hist_part, (size_t)kMaxHistBlocks * kMaxPasses * kRadix * sizeof(uint32_t)));
+  uint32_t *part = (uint32_t *)ctx->hist_part.ptr;
   int slot = prof_begin(ctx, kProfHistogram, 8.0 * n);
   if (do_relin)
-    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, r, hist, counters, err);
+    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, r, part, counters, err);
   else
-    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, r, hist, counters, err);
+    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, r, part, counters, err);
   prof_end(ctx, slot);
   LNT_HIP(hipGetLastError());
+  if (passes > 0) {
+    histogram_reduce_kernel<<<(width + 255) / 256, 256, 0, ctx->stream>>>(part, hblocks, width, hist);
+    LNT_HIP(hipGetLastError());
+  }
 
   if (ctx->validate) {
     LNT_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));

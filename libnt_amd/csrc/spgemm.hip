@@ -515,7 +515,8 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
   }
   seg_first_kernel<<<grid_for(nb), 256, 0, ctx->stream>>>(E.head, E.seg_idx, nb, E.seg_first);
   LNT_TRY(launch_fill_u32(ctx, E.counts, 0, 4));
-  classify_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.seg_first, nseg, nb, E.prod_off, kSmall,
+  classify_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.seg_first, nseg, nb, E.prod_off,
+                                                            (uint32_t)std::min(ctx->tuning.spgemm_small, kSmall),
                                                             E.seg_u, E.list2, E.list3, E.counts);
   LNT_HIP(hipGetLastError());
   LNT_HIP(hipMemcpyAsync(ctx->pinned + 4, E.counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
