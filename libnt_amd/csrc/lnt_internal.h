@@ -154,6 +154,38 @@ __device__ inline uint32_t lookback(const uint64_t *status, uint32_t tile,
   return excl;
 }
 
+// Same walk with K status words of one slot loaded per round trip (the loads
+// of one batch are independent, so a walk over many tiles that have only
+// published their aggregates costs 1/K of the round trips).
+template <int K>
+__device__ inline uint32_t lookback_batched(const uint64_t *status, uint32_t tile,
+                                            uint32_t stride, uint32_t slot,
+                                            uint32_t epoch) {
+  uint32_t excl = 0;
+  int64_t t = (int64_t)tile - 1;
+  while (t >= 0) {
+    uint64_t w[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      w[j] = t - j >= 0 ? status_load(status + (uint64_t)(t - j) * stride + slot) : 0;
+    int taken = 0;
+    bool done = false;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      if (done || taken != j || t - j < 0) continue;
+      uint64_t flag = w[j] >> 62;
+      if (((uint32_t)(w[j] >> 32) & kEpochMask) != (epoch & kEpochMask) || flag == 0) continue;
+      excl += (uint32_t)w[j];
+      ++taken;
+      done = flag == kFlagInclusive;
+    }
+    if (done) break;
+    if (taken == 0) __builtin_amdgcn_s_sleep(1);
+    t -= taken;
+  }
+  return excl;
+}
+
 // ------------------------------------------------------- wave primitives ---
 __device__ inline uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -173,6 +205,41 @@ __device__ inline uint32_t wave_inclusive_sum(uint32_t v) {
     if (lane >= (uint32_t)off) v += y;
   }
   return v;
+}
+
+// Look-back over a single-slot status array by one whole wave: lane l reads
+// tile (base - l), so one round trip inspects 64 predecessors.  Every lane
+// returns the same exclusive prefix.
+__device__ inline uint32_t wave_lookback(const uint64_t *status, uint32_t tile, uint32_t epoch) {
+  const uint32_t lane = lane_id();
+  uint32_t excl = 0;
+  int64_t base = (int64_t)tile - 1;
+  while (base >= 0) {
+    const int64_t t = base - (int64_t)lane;
+    bool ready = true, incl = true;
+    uint32_t val = 0;
+    if (t >= 0) {
+      uint64_t w = status_load(status + t);
+      uint64_t flag = w >> 62;
+      ready = ((uint32_t)(w >> 32) & kEpochMask) == (epoch & kEpochMask) && flag != 0;
+      incl = ready && flag == kFlagInclusive;
+      val = (uint32_t)w;
+    }
+    const uint64_t incl_mask = __ballot(incl);
+    const uint64_t not_ready = __ballot(!ready);
+    const uint32_t stop = incl_mask ? (uint32_t)__builtin_ctzll(incl_mask) : 63u;
+    const uint64_t needed = stop == 63u ? ~0ull : ((2ull << stop) - 1ull);
+    if (not_ready & needed) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint32_t part = lane <= stop ? val : 0u;
+    part = wave_inclusive_sum(part);
+    excl += __shfl(part, 63, kWave);
+    if (incl_mask) break;
+    base -= kWave;
+  }
+  return excl;
 }
 
 // Block-wide exclusive sum for a 256-thread block; `scratch` holds 4 words.
@@ -237,7 +304,7 @@ void release(DeviceBuffer &b);
 struct SortTuning {
   int items_per_thread = 16;  // onesweep tile = 256 * items
   int hist_blocks = 1024;
-  int merge_items = 8;  // merge tile = 256 * items
+  int merge_items = 16;  // merge tile = 256 * items
   int spgemm_small = 2048;  // largest output column handled in LDS
 };
 

@@ -183,18 +183,20 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(
   uint32_t tile_count;
   uint32_t my_off = block256_exclusive_sum(cnt, s_scan, &tile_count);
 
-  if (tid == 0) {
+  if (tid < kWave) {  // wave 0 publishes and looks back
     uint32_t excl = 0;
     uint64_t *my_status = status + tile;
     if (tile == 0) {
-      status_store(my_status, status_word(kFlagInclusive, epoch, tile_count));
+      if (tid == 0) status_store(my_status, status_word(kFlagInclusive, epoch, tile_count));
     } else {
-      status_store(my_status, status_word(kFlagAggregate, epoch, tile_count));
-      excl = lookback(status, tile, 1, 0, epoch);
-      status_store(my_status, status_word(kFlagInclusive, epoch, excl + tile_count));
+      if (tid == 0) status_store(my_status, status_word(kFlagAggregate, epoch, tile_count));
+      excl = wave_lookback(status, tile, epoch);
+      if (tid == 0) status_store(my_status, status_word(kFlagInclusive, epoch, excl + tile_count));
     }
-    if (tile == num_tiles - 1) *total_out = excl + tile_count;
-    s_excl = excl;
+    if (tid == 0) {
+      if (tile == num_tiles - 1) *total_out = excl + tile_count;
+      s_excl = excl;
+    }
   }
   __syncthreads();  // everyone is done reading the input slices
 #pragma unroll

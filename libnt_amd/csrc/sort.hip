@@ -69,15 +69,30 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
       uint64_t i = base + (uint64_t)u * kBlock + tid;
       k[u] = i < n ? keys[i] : 0;
     }
+    // Consecutive lanes hold consecutive keys; inputs are often sorted or
+    // nearly so, so equal digits come in runs across lanes.  Only the first
+    // lane of each run adds the run length: no same-address LDS atomics
+    // serialise inside a wave.
+    const uint32_t lane = tid & 63;
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       uint64_t i = base + (uint64_t)u * kBlock + tid;
-      if (i < n) {
-        uint64_t key = k[u];
-        bad |= key >= bound;
-        if (RELIN) key = relin_key(relin, key);
-        for (int p = 0; p < passes; ++p)
-          atomicAdd(&h[p * kRadix + ((key >> plan.shift[p]) & ((1u << plan.bits[p]) - 1))], 1u);
+      const bool valid = i < n;
+      uint64_t key = k[u];
+      bad |= valid && key >= bound;
+      if (RELIN) key = relin_key(relin, key);
+      const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+      for (int p = 0; p < passes; ++p) {
+        uint32_t d = (uint32_t)(key >> plan.shift[p]) & ((1u << plan.bits[p]) - 1);
+        uint32_t prev = __shfl_up(d, 1, kWave);
+        bool head = valid && (lane == 0 || prev != d);
+        uint64_t heads = __ballot(head);
+        if (head) {
+          uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+          uint32_t next = later ? (uint32_t)__builtin_ctzll(later) : 64u;
+          if (next > nvalid) next = nvalid;
+          atomicAdd(&h[p * kRadix + d], next - lane);
+        }
       }
     }
   }
@@ -187,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
   } else {
     status_store(my_status, status_word(kFlagAggregate, epoch, count));
-    excl = lookback(status, tile, kRadix, tid, epoch);
+    excl = lookback_batched<4>(status, tile, kRadix, tid, epoch);
     status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
   }
   s_gofs[tid] = excl - lstart;
