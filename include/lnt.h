@@ -92,6 +92,8 @@ int lnt_abi_version(void);
  *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096
  *   "merge_items"  merge-path steps per thread: 8 or 16 (tile = 256*items)
  *   "spgemm_small" longest output column (products) kept in LDS, 2..2048
+ *   "narrow_keys"  1: keys of at most 32 bits move as 32-bit words between
+ *                  radix passes (default); 0: always 64-bit
  * No reference counterpart (the reference hard-codes radix 2048 and the
  * thresholds 3000/50, multi_index_array/LibMIARadix.h:454-457). */
 int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value);

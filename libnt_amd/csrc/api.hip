@@ -266,6 +266,7 @@ int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
   else if (n == "hist_blocks" && in(1, kMaxHistBlocks)) ctx->tuning.hist_blocks = (int)value;
   else if (n == "merge_items" && (value == 8 || value == 16)) ctx->tuning.merge_items = (int)value;
   else if (n == "spgemm_small" && in(2, 2048)) ctx->tuning.spgemm_small = (int)value;
+  else if (n == "narrow_keys" && in(0, 1)) ctx->tuning.narrow_keys = (int)value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter or value out of range: " + n);
   return LNT_OK;
 }
@@ -277,6 +278,7 @@ int lnt_context_get_param(lnt_context ctx, const char *name, int64_t *value) {
   else if (n == "hist_blocks") *value = ctx->tuning.hist_blocks;
   else if (n == "merge_items") *value = ctx->tuning.merge_items;
   else if (n == "spgemm_small") *value = ctx->tuning.spgemm_small;
+  else if (n == "narrow_keys") *value = ctx->tuning.narrow_keys;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter: " + n);
   return LNT_OK;
 }

@@ -88,22 +88,58 @@ struct Relin {
   uint8_t old_shift[LNT_MAX_ORDER], bits[LNT_MAX_ORDER], new_shift[LNT_MAX_ORDER];
 };
 
-__host__ __device__ inline uint64_t relin_key(const Relin &r, uint64_t idx) {
-  if (r.pow2) {
-    uint64_t out = 0;
-    for (int t = 0; t < r.order; ++t) {
-      uint64_t mask = r.bits[t] >= 64 ? ~0ull : ((uint64_t(1) << r.bits[t]) - 1);
-      out |= ((idx >> r.old_shift[t]) & mask) << r.new_shift[t];
+// Device copy of a Relin kept in LDS: a block stores it once, then every
+// lane reads it with uniform addresses inside a loop over the runtime order.
+// (Indexing the by-value kernel argument with a runtime index makes every
+// lane re-load it from the argument segment.)
+struct RelinLds {
+  uint64_t magic[LNT_MAX_ORDER], d[LNT_MAX_ORDER], weight[LNT_MAX_ORDER];
+  uint32_t s1[LNT_MAX_ORDER], s2[LNT_MAX_ORDER];
+  uint32_t old_shift[LNT_MAX_ORDER], bits[LNT_MAX_ORDER], new_shift[LNT_MAX_ORDER];
+  int32_t order, pow2;
+};
+
+// Threads 0..LNT_MAX_ORDER-1 each copy one mode; follow with __syncthreads().
+__device__ inline void relin_to_lds(const Relin &r, RelinLds *s) {
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < LNT_MAX_ORDER; ++m) {
+    if (t == (uint32_t)m) {
+      s->magic[m] = r.place[m].magic;
+      s->d[m] = r.place[m].d;
+      s->s1[m] = r.place[m].s1;
+      s->s2[m] = r.place[m].s2;
+      s->weight[m] = r.weight[m];
+      s->old_shift[m] = r.old_shift[m];
+      s->bits[m] = r.bits[m];
+      s->new_shift[m] = r.new_shift[m];
+    }
+  }
+  if (t == 0) {
+    s->order = r.order;
+    s->pow2 = r.pow2;
+  }
+}
+
+__device__ inline uint64_t relin_key(const RelinLds *s, uint64_t idx) {
+  const int order = s->order;
+  uint64_t out = 0;
+  if (s->pow2) {
+    for (int t = 0; t < order; ++t) {
+      const uint32_t b = s->bits[t];
+      const uint64_t mask = b >= 64 ? ~0ull : ((uint64_t(1) << b) - 1);
+      out |= ((idx >> s->old_shift[t]) & mask) << s->new_shift[t];
     }
     return out;
   }
-  uint64_t out = 0;
-  for (int t = r.order - 1; t >= 1; --t) {
-    uint64_t q = divide(idx, r.place[t]);
-    out += q * r.weight[t];
-    idx -= q * r.place[t].d;
+  for (int t = order - 1; t >= 1; --t) {
+    const uint64_t m = s->magic[t];
+    const uint64_t hi = mulhi64(m, idx);
+    const uint64_t q = (hi + ((idx - hi) >> s->s1[t])) >> s->s2[t];
+    out += q * s->weight[t];
+    idx -= q * s->d[t];
   }
-  return out + idx * r.weight[0];
+  return out + idx * s->weight[0];
 }
 
 // Validates (order, dims, seqs) and fills a Relin.  *dimensionality receives
@@ -306,6 +342,7 @@ struct SortTuning {
   int hist_blocks = 1024;
   int merge_items = 16;  // merge tile = 256 * items
   int spgemm_small = 2048;  // largest output column handled in LDS
+  int narrow_keys = 1;  // 32-bit keys between passes when they fit
 };
 
 // Kernel kinds recorded by the profiler (names in api.hip).

@@ -53,8 +53,10 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
     uint32_t *__restrict__ err) {
   const int passes = plan.passes;
   __shared__ uint32_t h[kMaxPasses * kRadix];
+  __shared__ RelinLds s_relin;
   const uint32_t tid = threadIdx.x;
   for (int i = tid; i < passes * kRadix; i += kBlock) h[i] = 0;
+  if (RELIN) relin_to_lds(relin, &s_relin);
   if (blockIdx.x == 0 && tid < 16) counters[tid] = 0;
   __syncthreads();
 
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
       const bool valid = i < n;
       uint64_t key = k[u];
       bad |= valid && key >= bound;
-      if (RELIN) key = relin_key(relin, key);
+      if (RELIN) key = relin_key(&s_relin, key);
       const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
       for (int p = 0; p < passes; ++p) {
         uint32_t d = (uint32_t)(key >> plan.shift[p]) & ((1u << plan.bits[p]) - 1);
@@ -115,10 +117,13 @@ __global__ void histogram_reduce_kernel(const uint32_t *__restrict__ part, uint3
 }
 
 // -------------------------------------------------------------- onesweep ---
-template <int ITEMS, bool RELIN>
+// KIn/KOut: key word read/written by this pass.  When the key space fits in
+// 32 bits the passes between the first and the last move 32-bit keys (24 B per
+// pair instead of 32 B); the first pass narrows, the last widens.
+template <int ITEMS, bool RELIN, typename KIn, typename KOut>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
-    const uint64_t *__restrict__ kin, const double *__restrict__ vin,
-    uint64_t *__restrict__ kout, double *__restrict__ vout, uint32_t n,
+    const KIn *__restrict__ kin, const double *__restrict__ vin,
+    KOut *__restrict__ kout, double *__restrict__ vout, uint32_t n,
     uint32_t shift, uint32_t dbits, const uint32_t *__restrict__ pass_hist,
     uint64_t *__restrict__ status, uint32_t epoch,
     uint32_t *__restrict__ tile_counter, Relin relin) {
@@ -130,8 +135,10 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   __shared__ uint32_t s_gofs[kRadix];
   __shared__ uint32_t s_scan[4];
   __shared__ uint64_t s_stage[kTile];
+  __shared__ RelinLds s_relin;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (RELIN) relin_to_lds(relin, &s_relin);
   if (tid == 0)
     s_tile = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
@@ -150,27 +157,35 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   for (int i = 0; i < ITEMS; ++i) {
     uint64_t pos = wave_base + (uint64_t)i * kWave + lane;
     bool valid = pos < n;
-    k[i] = valid ? kin[pos] : 0;
+    k[i] = valid ? (uint64_t)kin[pos] : 0;
     v[i] = valid ? vin[pos] : 0.0;
   }
   if (RELIN) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) k[i] = relin_key(relin, k[i]);
+    for (int i = 0; i < ITEMS; ++i) k[i] = relin_key(&s_relin, k[i]);
   }
 
-  // Stable in-wave ranking: lanes holding the same digit are found with one
-  // ballot per digit bit; the highest such lane bumps the wave's counter.
+  // Stable in-wave ranking: the lanes holding the same digit are the lanes
+  // that agree with this one on every digit bit -- one ballot per bit, folded
+  // in with peers &= ~(ballot ^ bitmask).  Bits above dbits are zero in every
+  // lane, so a fixed 8-bit loop serves every digit width.  The highest lane of
+  // a group bumps the wave's counter for that digit.
   uint32_t local_rank[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     uint64_t pos = wave_base + (uint64_t)i * kWave + lane;
     bool valid = pos < n;
     uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
-    uint64_t peers = __ballot(valid);
-    for (uint32_t b = 0; b < dbits; ++b) {
-      uint64_t bb = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    uint64_t vb = __ballot(valid);
+    uint32_t p_lo = (uint32_t)vb, p_hi = (uint32_t)(vb >> 32);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // 0 or ~0
+      const uint64_t bb = __ballot(m != 0);
+      p_lo &= ~((uint32_t)bb ^ m);
+      p_hi &= ~((uint32_t)(bb >> 32) ^ m);
     }
+    const uint64_t peers = ((uint64_t)p_hi << 32) | p_lo;
     uint32_t r = 0;
     if (valid) {
       uint32_t cnt = s_wcount[wave][d];
@@ -230,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
       uint64_t key = s_stage[p];
       uint32_t d = (uint32_t)(key >> shift) & dmask;
       dst[i] = s_gofs[d] + p;
-      kout[dst[i]] = key;
+      kout[dst[i]] = (KOut)key;
     }
   }
   __syncthreads();
@@ -248,13 +263,26 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
 }
 
 __global__ void relin_kernel(uint64_t *keys, uint32_t n, Relin relin) {
+  __shared__ RelinLds s_relin;
+  relin_to_lds(relin, &s_relin);
+  __syncthreads();
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) keys[i] = relin_key(relin, keys[i]);
+  for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) keys[i] = relin_key(&s_relin, keys[i]);
+}
+
+enum KeyWidth { kW64 = 0, kW32 = 1 };
+
+template <int ITEMS, bool RELIN, typename KIn, typename KOut>
+void launch_one(lnt_context ctx, uint32_t tiles, const void *kin, const double *vin, void *kout, double *vout,
+                uint32_t n, uint32_t shift, uint32_t dbits, const uint32_t *pass_hist, uint64_t *status,
+                uint32_t epoch, uint32_t *counter, const Relin &r) {
+  onesweep_kernel<ITEMS, RELIN, KIn, KOut><<<tiles, kBlock, 0, ctx->stream>>>(
+      (const KIn *)kin, vin, (KOut *)kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
 }
 
 template <int ITEMS>
-int launch_pass(lnt_context ctx, bool relin, const uint64_t *kin,
-                const double *vin, uint64_t *kout, double *vout, uint32_t n,
+int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
+                const double *vin, void *kout, double *vout, uint32_t n,
                 uint32_t shift, uint32_t dbits, const uint32_t *pass_hist,
                 uint32_t *counter, const Relin &r) {
   constexpr uint32_t tile = kBlock * ITEMS;
@@ -263,13 +291,19 @@ int launch_pass(lnt_context ctx, bool relin, const uint64_t *kin,
   uint32_t epoch;
   LNT_TRY(next_epoch(ctx, &epoch));
   uint64_t *status = (uint64_t *)ctx->status.ptr;
+  // Roofline basis: the fixed model of 32 B per pair per pass (8 B key + 8 B
+  // value, read and written), whatever key width this pass actually moves.
   int slot = prof_begin(ctx, kProfOnesweep, 32.0 * n);
-  if (relin)
-    onesweep_kernel<ITEMS, true><<<tiles, kBlock, 0, ctx->stream>>>(
-        kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
-  else
-    onesweep_kernel<ITEMS, false><<<tiles, kBlock, 0, ctx->stream>>>(
-        kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
+  const int sel = (relin ? 4 : 0) + win * 2 + wout;
+  switch (sel) {
+    case 0: launch_one<ITEMS, false, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 1: launch_one<ITEMS, false, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 2: launch_one<ITEMS, false, uint32_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 3: launch_one<ITEMS, false, uint32_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 4: launch_one<ITEMS, true, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 5: launch_one<ITEMS, true, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    default: return fail(LNT_ERR_INVALID_ARGUMENT, "internal: unsupported key widths");
+  }
   prof_end(ctx, slot);
   LNT_HIP(hipGetLastError());
   return LNT_OK;
@@ -392,7 +426,9 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
     bufv[2] = (double *)ctx->alt2_vals.ptr;
   }
   // Buffers: even P: 0->1->0->...->0; odd P: 0->1->2->1->...->0.  No pass
-  // reads and writes the same buffer.
+  // reads and writes the same buffer.  Keys of at most 32 bits travel as
+  // 32-bit words between the first and the last pass.
+  const bool narrow = bits <= 32 && ctx->tuning.narrow_keys;
   int src = 0;
   for (int p = 0; p < passes; ++p) {
     int dst;
@@ -400,18 +436,20 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
     else if (passes & 1) dst = (src == 1) ? 2 : 1;
     else dst = (src == 0) ? 1 : 0;
     const bool rl = do_relin && p == 0;
+    const int win = (narrow && p > 0) ? kW32 : kW64;
+    const int wout = (narrow && p < passes - 1) ? kW32 : kW64;
     int rc;
     switch (ctx->tuning.items_per_thread) {
       case 8:
-        rc = launch_pass<8>(ctx, rl, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
+        rc = launch_pass<8>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
                             plan.bits[p], hist + p * kRadix, counters + p, r);
         break;
       case 12:
-        rc = launch_pass<12>(ctx, rl, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
+        rc = launch_pass<12>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
                              plan.bits[p], hist + p * kRadix, counters + p, r);
         break;
       default:
-        rc = launch_pass<16>(ctx, rl, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
+        rc = launch_pass<16>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
                              plan.bits[p], hist + p * kRadix, counters + p, r);
         break;
     }
