@@ -94,18 +94,16 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(
   {
     uint64_t lk[ITEMS];
     double lv[ITEMS];
+    const uint32_t last = la + lb - 1;  // a tile holds at least one step
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
       uint32_t x = (uint32_t)u * kBlock + tid;
-      lk[u] = 0;
-      lv[u] = 0.0;
-      if (x < la + lb) {
-        const bool from_a = x < la;
-        const uint64_t *kp = from_a ? ak + i0 + x : bk + j0 + (x - la);
-        const double *vp = from_a ? av + i0 + x : bv + j0 + (x - la);
-        lk[u] = *kp;
-        lv[u] = *vp;
-      }
+      x = x < la + lb ? x : last;  // clamped, unconditional loads
+      const bool from_a = x < la;
+      const uint64_t *kp = from_a ? ak + i0 + x : bk + j0 + (x - la);
+      const double *vp = from_a ? av + i0 + x : bv + j0 + (x - la);
+      lk[u] = *kp;
+      lv[u] = *vp;
     }
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {

@@ -20,7 +20,8 @@ __global__ __launch_bounds__(kBlock) void chunk_sum_kernel(const T *__restrict__
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     uint64_t p = base + (uint64_t)i * kBlock + threadIdx.x;
-    if (p < n) acc += (uint64_t)in[p];
+    uint64_t x = (uint64_t)in[p < n ? p : n - 1];  // unconditional load
+    acc += p < n ? x : 0;
   }
   uint64_t total;
   block_exclusive_sum64_lds(acc, s, &total);
@@ -40,7 +41,8 @@ __global__ __launch_bounds__(kBlock) void chunk_scan_kernel(const T *__restrict_
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     uint64_t p = base + i;
-    v[i] = p < n ? (uint64_t)in[p] : 0;
+    uint64_t x = (uint64_t)in[p < n ? p : n - 1];  // unconditional load
+    v[i] = p < n ? x : 0;
     acc += v[i];
   }
   uint64_t total;

@@ -66,10 +66,12 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
   for (uint64_t base = (uint64_t)blockIdx.x * chunk; base < n;
        base += (uint64_t)gridDim.x * chunk) {
     uint64_t k[kUnroll];
+    // Unconditional loads of a clamped index: a guarded load compiles to a
+    // branch that waits for it, which would serialise the kUnroll loads.
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       uint64_t i = base + (uint64_t)u * kBlock + tid;
-      k[u] = i < n ? keys[i] : 0;
+      k[u] = keys[i < n ? i : n - 1];
     }
     // Consecutive lanes hold consecutive keys; inputs are often sorted or
     // nearly so, so equal digits come in runs across lanes.  Only the first
@@ -153,12 +155,14 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
 
   uint64_t k[ITEMS];
   double v[ITEMS];
+  // Unconditional loads of a clamped index, so all 2*ITEMS loads are in
+  // flight together (a guarded load becomes a branch that waits for it).
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     uint64_t pos = wave_base + (uint64_t)i * kWave + lane;
-    bool valid = pos < n;
-    k[i] = valid ? (uint64_t)kin[pos] : 0;
-    v[i] = valid ? vin[pos] : 0.0;
+    uint64_t at = pos < n ? pos : n - 1;
+    k[i] = (uint64_t)kin[at];
+    v[i] = vin[at];
   }
   if (RELIN) {
 #pragma unroll
