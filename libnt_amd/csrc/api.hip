@@ -249,7 +249,7 @@ int lnt_context_destroy(lnt_context ctx) {
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
                           &ctx->host_tmp_b, &ctx->host_tmp_c, &ctx->spg_ws, &ctx->spg_tmp,
-                          &ctx->spg_big, &ctx->hist_part})
+                          &ctx->spg_big, &ctx->hist_part, &ctx->relin_dev})
     release(*b);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   prof_clear(ctx);

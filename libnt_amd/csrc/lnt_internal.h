@@ -100,7 +100,11 @@ struct RelinLds {
 };
 
 // Threads 0..LNT_MAX_ORDER-1 each copy one mode; follow with __syncthreads().
-__device__ inline void relin_to_lds(const Relin &r, RelinLds *s) {
+// `r` points to device memory (the table is stored by a one-thread kernel):
+// taking the address of a by-value kernel argument would copy it to private
+// memory in every lane.
+__device__ __forceinline__ void relin_to_lds(const Relin *__restrict__ rp, RelinLds *s) {
+  const Relin &r = *rp;
   const uint32_t t = threadIdx.x;
 #pragma unroll
   for (int m = 0; m < LNT_MAX_ORDER; ++m) {
@@ -119,6 +123,44 @@ __device__ inline void relin_to_lds(const Relin &r, RelinLds *s) {
     s->order = r.order;
     s->pow2 = r.pow2;
   }
+}
+
+// Relinearises N keys at once: the loop over modes is outermost, so each
+// mode's parameters are read from LDS once into scalar registers and applied
+// to all N keys.
+template <int N>
+__device__ __forceinline__ void relin_keys(const RelinLds *s, uint64_t (&k)[N]) {
+  const int order = __builtin_amdgcn_readfirstlane(s->order);
+  uint64_t out[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) out[i] = 0;
+  if (__builtin_amdgcn_readfirstlane(s->pow2)) {
+    for (int t = 0; t < order; ++t) {
+      const uint32_t os = __builtin_amdgcn_readfirstlane(s->old_shift[t]);
+      const uint32_t b = __builtin_amdgcn_readfirstlane(s->bits[t]);
+      const uint32_t ns = __builtin_amdgcn_readfirstlane(s->new_shift[t]);
+      const uint64_t mask = b >= 64 ? ~0ull : ((uint64_t(1) << b) - 1);
+#pragma unroll
+      for (int i = 0; i < N; ++i) out[i] |= ((k[i] >> os) & mask) << ns;
+    }
+  } else {
+    for (int t = order - 1; t >= 1; --t) {
+      const uint64_t m = s->magic[t], d = s->d[t], w = s->weight[t];
+      const uint32_t s1 = s->s1[t], s2 = s->s2[t];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const uint64_t hi = mulhi64(m, k[i]);
+        const uint64_t q = (hi + ((k[i] - hi) >> s1)) >> s2;
+        out[i] += q * w;
+        k[i] -= q * d;
+      }
+    }
+    const uint64_t w0 = s->weight[0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] += k[i] * w0;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) k[i] = out[i];
 }
 
 __device__ inline uint64_t relin_key(const RelinLds *s, uint64_t idx) {
@@ -368,6 +410,7 @@ struct lnt_context_s {
   lnt::DeviceBuffer small;      // histograms, counters, flags
   lnt::DeviceBuffer aux;        // merge-path splits and other per-call scratch
   lnt::DeviceBuffer hist_part;  // per-block digit histograms
+  lnt::DeviceBuffer relin_dev;  // device copy of the current relinearisation table
   lnt::DeviceBuffer spg_ws, spg_tmp, spg_big;  // lattice-product workspaces
   lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
   uint32_t *pinned = nullptr;   // small pinned host buffer for counts/flags

@@ -49,7 +49,7 @@ struct DigitPlan {
 template <bool RELIN>
 __global__ __launch_bounds__(kBlock) void histogram_kernel(
     const uint64_t *__restrict__ keys, uint32_t n, DigitPlan plan, uint64_t bound,
-    Relin relin, uint32_t *__restrict__ part, uint32_t *__restrict__ counters,
+    const Relin *__restrict__ relin, uint32_t *__restrict__ part, uint32_t *__restrict__ counters,
     uint32_t *__restrict__ err) {
   const int passes = plan.passes;
   __shared__ uint32_t h[kMaxPasses * kRadix];
@@ -77,25 +77,27 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
     // nearly so, so equal digits come in runs across lanes.  Only the first
     // lane of each run adds the run length: no same-address LDS atomics
     // serialise inside a wave.
-    const uint32_t lane = tid & 63;
+    // A wave whose lanes all hold the same digit adds once (sorted inputs
+    // make such waves common); otherwise every lane adds its own key.
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      uint64_t i = base + (uint64_t)u * kBlock + tid;
+      bad |= i < n && k[u] >= bound;
+    }
+    if (RELIN) relin_keys<kUnroll>(&s_relin, k);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       uint64_t i = base + (uint64_t)u * kBlock + tid;
       const bool valid = i < n;
-      uint64_t key = k[u];
-      bad |= valid && key >= bound;
-      if (RELIN) key = relin_key(&s_relin, key);
-      const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));
+      const uint64_t key = k[u];
+      const uint64_t vmask = __ballot(valid);
       for (int p = 0; p < passes; ++p) {
-        uint32_t d = (uint32_t)(key >> plan.shift[p]) & ((1u << plan.bits[p]) - 1);
-        uint32_t prev = __shfl_up(d, 1, kWave);
-        bool head = valid && (lane == 0 || prev != d);
-        uint64_t heads = __ballot(head);
-        if (head) {
-          uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
-          uint32_t next = later ? (uint32_t)__builtin_ctzll(later) : 64u;
-          if (next > nvalid) next = nvalid;
-          atomicAdd(&h[p * kRadix + d], next - lane);
+        const uint32_t d = (uint32_t)(key >> plan.shift[p]) & ((1u << plan.bits[p]) - 1);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if (__ballot(valid && d == d0) == vmask) {
+          if ((tid & 63) == 0) atomicAdd(&h[p * kRadix + d0], (uint32_t)__popcll(vmask));
+        } else if (valid) {
+          atomicAdd(&h[p * kRadix + d], 1u);
         }
       }
     }
@@ -123,12 +125,12 @@ __global__ void histogram_reduce_kernel(const uint32_t *__restrict__ part, uint3
 // 32 bits the passes between the first and the last move 32-bit keys (24 B per
 // pair instead of 32 B); the first pass narrows, the last widens.
 template <int ITEMS, bool RELIN, typename KIn, typename KOut>
-__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void onesweep_kernel(
     const KIn *__restrict__ kin, const double *__restrict__ vin,
     KOut *__restrict__ kout, double *__restrict__ vout, uint32_t n,
     uint32_t shift, uint32_t dbits, const uint32_t *__restrict__ pass_hist,
     uint64_t *__restrict__ status, uint32_t epoch,
-    uint32_t *__restrict__ tile_counter, Relin relin) {
+    uint32_t *__restrict__ tile_counter, const Relin *__restrict__ relin) {
   constexpr int kTile = kBlock * ITEMS;
   constexpr int kWaves = kBlock / kWave;
   __shared__ uint32_t s_tile;
@@ -164,10 +166,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     k[i] = (uint64_t)kin[at];
     v[i] = vin[at];
   }
-  if (RELIN) {
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) k[i] = relin_key(&s_relin, k[i]);
-  }
+  if (RELIN) relin_keys<ITEMS>(&s_relin, k);
 
   // Stable in-wave ranking: the lanes holding the same digit are the lanes
   // that agree with this one on every digit bit -- one ballot per bit, folded
@@ -266,7 +265,11 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
 }
 
-__global__ void relin_kernel(uint64_t *keys, uint32_t n, Relin relin) {
+__global__ void store_relin_kernel(Relin r, Relin *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = r;
+}
+
+__global__ void relin_kernel(uint64_t *keys, uint32_t n, const Relin *__restrict__ relin) {
   __shared__ RelinLds s_relin;
   relin_to_lds(relin, &s_relin);
   __syncthreads();
@@ -279,7 +282,7 @@ enum KeyWidth { kW64 = 0, kW32 = 1 };
 template <int ITEMS, bool RELIN, typename KIn, typename KOut>
 void launch_one(lnt_context ctx, uint32_t tiles, const void *kin, const double *vin, void *kout, double *vout,
                 uint32_t n, uint32_t shift, uint32_t dbits, const uint32_t *pass_hist, uint64_t *status,
-                uint32_t epoch, uint32_t *counter, const Relin &r) {
+                uint32_t epoch, uint32_t *counter, const Relin *r) {
   onesweep_kernel<ITEMS, RELIN, KIn, KOut><<<tiles, kBlock, 0, ctx->stream>>>(
       (const KIn *)kin, vin, (KOut *)kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
 }
@@ -288,7 +291,7 @@ template <int ITEMS>
 int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
                 const double *vin, void *kout, double *vout, uint32_t n,
                 uint32_t shift, uint32_t dbits, const uint32_t *pass_hist,
-                uint32_t *counter, const Relin &r) {
+                uint32_t *counter, const Relin *r) {
   constexpr uint32_t tile = kBlock * ITEMS;
   uint32_t tiles = (n + tile - 1) / tile;
   LNT_TRY(ensure_status(ctx, (size_t)tiles * kRadix));
@@ -348,12 +351,23 @@ int next_epoch(lnt_context ctx, uint32_t *epoch) {
   return LNT_OK;
 }
 
+// Stores `r` in the context's device table (stream ordered) and returns it.
+static int upload_relin(lnt_context ctx, const Relin &r, const Relin **dev) {
+  LNT_TRY(ensure(ctx->relin_dev, sizeof(Relin)));
+  store_relin_kernel<<<1, 64, 0, ctx->stream>>>(r, (Relin *)ctx->relin_dev.ptr);
+  LNT_HIP(hipGetLastError());
+  *dev = (const Relin *)ctx->relin_dev.ptr;
+  return LNT_OK;
+}
+
 int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin) {
   if (n == 0 || !relin.active) return LNT_OK;
+  const Relin *rdev = nullptr;
+  LNT_TRY(upload_relin(ctx, relin, &rdev));
   uint32_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   int slot = prof_begin(ctx, kProfRelin, 16.0 * n);
-  relin_kernel<<<blocks, 256, 0, ctx->stream>>>(keys, n, relin);
+  relin_kernel<<<blocks, 256, 0, ctx->stream>>>(keys, n, rdev);
   prof_end(ctx, slot);
   LNT_HIP(hipGetLastError());
   return LNT_OK;
@@ -367,6 +381,8 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   const bool do_relin = relin && relin->active;
   Relin r{};
   if (do_relin) r = *relin;
+  const Relin *rdev = nullptr;
+  if (do_relin) LNT_TRY(upload_relin(ctx, r, &rdev));
   uint32_t bits = 0;
   while (bits < 64 && (uint64_t(1) << bits) < bound) ++bits;
   // Digit plan: ceil(bits/8) passes (at least two so that no pass runs in
@@ -398,9 +414,9 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   uint32_t *part = (uint32_t *)ctx->hist_part.ptr;
   int slot = prof_begin(ctx, kProfHistogram, 8.0 * n);
   if (do_relin)
-    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, r, part, counters, err);
+    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, rdev, part, counters, err);
   else
-    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, r, part, counters, err);
+    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, rdev, part, counters, err);
   prof_end(ctx, slot);
   LNT_HIP(hipGetLastError());
   if (passes > 0) {
@@ -446,15 +462,15 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
     switch (ctx->tuning.items_per_thread) {
       case 8:
         rc = launch_pass<8>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
-                            plan.bits[p], hist + p * kRadix, counters + p, r);
+                            plan.bits[p], hist + p * kRadix, counters + p, rdev);
         break;
       case 12:
         rc = launch_pass<12>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
-                             plan.bits[p], hist + p * kRadix, counters + p, r);
+                             plan.bits[p], hist + p * kRadix, counters + p, rdev);
         break;
       default:
         rc = launch_pass<16>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
-                             plan.bits[p], hist + p * kRadix, counters + p, r);
+                             plan.bits[p], hist + p * kRadix, counters + p, rdev);
         break;
     }
     if (rc != LNT_OK) return rc;
