@@ -239,16 +239,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
   }
   __syncthreads();
+  // Destinations stay below n whenever the digit counts of the histogram
+  // pass agree with the keys seen here; the bound keeps a malformed input
+  // (validation off) from writing outside the output.
   uint32_t dst[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     uint32_t p = (uint32_t)i * kBlock + tid;
-    dst[i] = 0;
+    dst[i] = 0xFFFFFFFFu;
     if (p < tile_n) {
       uint64_t key = s_stage[p];
       uint32_t d = (uint32_t)(key >> shift) & dmask;
       dst[i] = s_gofs[d] + p;
-      kout[dst[i]] = (KOut)key;
+      if (dst[i] < n) kout[dst[i]] = (KOut)key;
     }
   }
   __syncthreads();
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     uint32_t p = (uint32_t)i * kBlock + tid;
-    if (p < tile_n) vout[dst[i]] = s_vstage[p];
+    if (p < tile_n && dst[i] < n) vout[dst[i]] = s_vstage[p];
   }
 }
 
