@@ -88,57 +88,45 @@ struct Relin {
   uint8_t old_shift[LNT_MAX_ORDER], bits[LNT_MAX_ORDER], new_shift[LNT_MAX_ORDER];
 };
 
-// Device copy of a Relin kept in LDS: a block stores it once, then every
-// lane reads it with uniform addresses inside a loop over the runtime order.
-// (Indexing the by-value kernel argument with a runtime index makes every
-// lane re-load it from the argument segment.)
-struct RelinLds {
+// Device-resident form of a Relin: word-sized fields, read by every lane
+// with uniform addresses (scalar loads) inside a loop over the runtime order.
+// (A by-value kernel argument indexed at run time is copied to private
+// memory in every lane.)
+struct RelinDev {
   uint64_t magic[LNT_MAX_ORDER], d[LNT_MAX_ORDER], weight[LNT_MAX_ORDER];
   uint32_t s1[LNT_MAX_ORDER], s2[LNT_MAX_ORDER];
   uint32_t old_shift[LNT_MAX_ORDER], bits[LNT_MAX_ORDER], new_shift[LNT_MAX_ORDER];
   int32_t order, pow2;
 };
 
-// Threads 0..LNT_MAX_ORDER-1 each copy one mode; follow with __syncthreads().
-// `r` points to device memory (the table is stored by a one-thread kernel):
-// taking the address of a by-value kernel argument would copy it to private
-// memory in every lane.
-__device__ __forceinline__ void relin_to_lds(const Relin *__restrict__ rp, RelinLds *s) {
-  const Relin &r = *rp;
-  const uint32_t t = threadIdx.x;
-#pragma unroll
-  for (int m = 0; m < LNT_MAX_ORDER; ++m) {
-    if (t == (uint32_t)m) {
-      s->magic[m] = r.place[m].magic;
-      s->d[m] = r.place[m].d;
-      s->s1[m] = r.place[m].s1;
-      s->s2[m] = r.place[m].s2;
-      s->weight[m] = r.weight[m];
-      s->old_shift[m] = r.old_shift[m];
-      s->bits[m] = r.bits[m];
-      s->new_shift[m] = r.new_shift[m];
-    }
+inline RelinDev relin_for_device(const Relin &r) {
+  RelinDev d{};
+  for (int t = 0; t < LNT_MAX_ORDER; ++t) {
+    d.magic[t] = r.place[t].magic;
+    d.d[t] = r.place[t].d;
+    d.s1[t] = r.place[t].s1;
+    d.s2[t] = r.place[t].s2;
+    d.weight[t] = r.weight[t];
+    d.old_shift[t] = r.old_shift[t];
+    d.bits[t] = r.bits[t];
+    d.new_shift[t] = r.new_shift[t];
   }
-  if (t == 0) {
-    s->order = r.order;
-    s->pow2 = r.pow2;
-  }
+  d.order = r.order;
+  d.pow2 = r.pow2;
+  return d;
 }
 
-// Relinearises N keys at once: the loop over modes is outermost, so each
-// mode's parameters are read from LDS once into scalar registers and applied
-// to all N keys.
+// Relinearises N keys at once: modes outermost, so each mode's parameters
+// are loaded once (uniform loads) and applied to all N keys.
 template <int N>
-__device__ __forceinline__ void relin_keys(const RelinLds *s, uint64_t (&k)[N]) {
-  const int order = __builtin_amdgcn_readfirstlane(s->order);
+__device__ __forceinline__ void relin_keys(const RelinDev *__restrict__ s, uint64_t (&k)[N]) {
+  const int order = s->order;
   uint64_t out[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) out[i] = 0;
-  if (__builtin_amdgcn_readfirstlane(s->pow2)) {
+  if (s->pow2) {
     for (int t = 0; t < order; ++t) {
-      const uint32_t os = __builtin_amdgcn_readfirstlane(s->old_shift[t]);
-      const uint32_t b = __builtin_amdgcn_readfirstlane(s->bits[t]);
-      const uint32_t ns = __builtin_amdgcn_readfirstlane(s->new_shift[t]);
+      const uint32_t os = s->old_shift[t], b = s->bits[t], ns = s->new_shift[t];
       const uint64_t mask = b >= 64 ? ~0ull : ((uint64_t(1) << b) - 1);
 #pragma unroll
       for (int i = 0; i < N; ++i) out[i] |= ((k[i] >> os) & mask) << ns;
@@ -161,27 +149,6 @@ __device__ __forceinline__ void relin_keys(const RelinLds *s, uint64_t (&k)[N]) 
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) k[i] = out[i];
-}
-
-__device__ inline uint64_t relin_key(const RelinLds *s, uint64_t idx) {
-  const int order = s->order;
-  uint64_t out = 0;
-  if (s->pow2) {
-    for (int t = 0; t < order; ++t) {
-      const uint32_t b = s->bits[t];
-      const uint64_t mask = b >= 64 ? ~0ull : ((uint64_t(1) << b) - 1);
-      out |= ((idx >> s->old_shift[t]) & mask) << s->new_shift[t];
-    }
-    return out;
-  }
-  for (int t = order - 1; t >= 1; --t) {
-    const uint64_t m = s->magic[t];
-    const uint64_t hi = mulhi64(m, idx);
-    const uint64_t q = (hi + ((idx - hi) >> s->s1[t])) >> s->s2[t];
-    out += q * s->weight[t];
-    idx -= q * s->d[t];
-  }
-  return out + idx * s->weight[0];
 }
 
 // Validates (order, dims, seqs) and fills a Relin.  *dimensionality receives

@@ -49,14 +49,12 @@ struct DigitPlan {
 template <bool RELIN>
 __global__ __launch_bounds__(kBlock) void histogram_kernel(
     const uint64_t *__restrict__ keys, uint32_t n, DigitPlan plan, uint64_t bound,
-    const Relin *__restrict__ relin, uint32_t *__restrict__ part, uint32_t *__restrict__ counters,
+    const RelinDev *__restrict__ relin, uint32_t *__restrict__ part, uint32_t *__restrict__ counters,
     uint32_t *__restrict__ err) {
   const int passes = plan.passes;
   __shared__ uint32_t h[kMaxPasses * kRadix];
-  __shared__ RelinLds s_relin;
   const uint32_t tid = threadIdx.x;
   for (int i = tid; i < passes * kRadix; i += kBlock) h[i] = 0;
-  if (RELIN) relin_to_lds(relin, &s_relin);
   if (blockIdx.x == 0 && tid < 16) counters[tid] = 0;
   __syncthreads();
 
@@ -84,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
       uint64_t i = base + (uint64_t)u * kBlock + tid;
       bad |= i < n && k[u] >= bound;
     }
-    if (RELIN) relin_keys<kUnroll>(&s_relin, k);
+    if (RELIN) relin_keys<kUnroll>(relin, k);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       uint64_t i = base + (uint64_t)u * kBlock + tid;
@@ -130,7 +128,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     KOut *__restrict__ kout, double *__restrict__ vout, uint32_t n,
     uint32_t shift, uint32_t dbits, const uint32_t *__restrict__ pass_hist,
     uint64_t *__restrict__ status, uint32_t epoch,
-    uint32_t *__restrict__ tile_counter, const Relin *__restrict__ relin) {
+    uint32_t *__restrict__ tile_counter, const RelinDev *__restrict__ relin) {
   constexpr int kTile = kBlock * ITEMS;
   constexpr int kWaves = kBlock / kWave;
   __shared__ uint32_t s_tile;
@@ -139,10 +137,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   __shared__ uint32_t s_gofs[kRadix];
   __shared__ uint32_t s_scan[4];
   __shared__ uint64_t s_stage[kTile];
-  __shared__ RelinLds s_relin;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (RELIN) relin_to_lds(relin, &s_relin);
   if (tid == 0)
     s_tile = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
@@ -166,7 +162,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     k[i] = (uint64_t)kin[at];
     v[i] = vin[at];
   }
-  if (RELIN) relin_keys<ITEMS>(&s_relin, k);
+  if (RELIN) relin_keys<ITEMS>(relin, k);
 
   // Stable in-wave ranking: the lanes holding the same digit are the lanes
   // that agree with this one on every digit bit -- one ballot per bit, folded
@@ -268,16 +264,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   }
 }
 
-__global__ void store_relin_kernel(Relin r, Relin *out) {
+__global__ void store_relin_kernel(RelinDev r, RelinDev *out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *out = r;
 }
 
-__global__ void relin_kernel(uint64_t *keys, uint32_t n, const Relin *__restrict__ relin) {
-  __shared__ RelinLds s_relin;
-  relin_to_lds(relin, &s_relin);
-  __syncthreads();
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) keys[i] = relin_key(&s_relin, keys[i]);
+__global__ void relin_kernel(uint64_t *keys, uint32_t n, const RelinDev *__restrict__ relin) {
+  constexpr int kPer = 4;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; base < n; base += stride * kPer) {
+    uint64_t k[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      uint64_t i = base + (uint64_t)u * stride;
+      k[u] = keys[i < n ? i : n - 1];
+    }
+    relin_keys<kPer>(relin, k);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      uint64_t i = base + (uint64_t)u * stride;
+      if (i < n) keys[i] = k[u];
+    }
+  }
 }
 
 enum KeyWidth { kW64 = 0, kW32 = 1 };
@@ -285,7 +292,7 @@ enum KeyWidth { kW64 = 0, kW32 = 1 };
 template <int ITEMS, bool RELIN, typename KIn, typename KOut>
 void launch_one(lnt_context ctx, uint32_t tiles, const void *kin, const double *vin, void *kout, double *vout,
                 uint32_t n, uint32_t shift, uint32_t dbits, const uint32_t *pass_hist, uint64_t *status,
-                uint32_t epoch, uint32_t *counter, const Relin *r) {
+                uint32_t epoch, uint32_t *counter, const RelinDev *r) {
   onesweep_kernel<ITEMS, RELIN, KIn, KOut><<<tiles, kBlock, 0, ctx->stream>>>(
       (const KIn *)kin, vin, (KOut *)kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
 }
@@ -294,7 +301,7 @@ template <int ITEMS>
 int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
                 const double *vin, void *kout, double *vout, uint32_t n,
                 uint32_t shift, uint32_t dbits, const uint32_t *pass_hist,
-                uint32_t *counter, const Relin *r) {
+                uint32_t *counter, const RelinDev *r) {
   constexpr uint32_t tile = kBlock * ITEMS;
   uint32_t tiles = (n + tile - 1) / tile;
   LNT_TRY(ensure_status(ctx, (size_t)tiles * kRadix));
@@ -355,17 +362,17 @@ int next_epoch(lnt_context ctx, uint32_t *epoch) {
 }
 
 // Stores `r` in the context's device table (stream ordered) and returns it.
-static int upload_relin(lnt_context ctx, const Relin &r, const Relin **dev) {
-  LNT_TRY(ensure(ctx->relin_dev, sizeof(Relin)));
-  store_relin_kernel<<<1, 64, 0, ctx->stream>>>(r, (Relin *)ctx->relin_dev.ptr);
+static int upload_relin(lnt_context ctx, const Relin &r, const RelinDev **dev) {
+  LNT_TRY(ensure(ctx->relin_dev, sizeof(RelinDev)));
+  store_relin_kernel<<<1, 64, 0, ctx->stream>>>(relin_for_device(r), (RelinDev *)ctx->relin_dev.ptr);
   LNT_HIP(hipGetLastError());
-  *dev = (const Relin *)ctx->relin_dev.ptr;
+  *dev = (const RelinDev *)ctx->relin_dev.ptr;
   return LNT_OK;
 }
 
 int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin) {
   if (n == 0 || !relin.active) return LNT_OK;
-  const Relin *rdev = nullptr;
+  const RelinDev *rdev = nullptr;
   LNT_TRY(upload_relin(ctx, relin, &rdev));
   uint32_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
@@ -384,7 +391,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   const bool do_relin = relin && relin->active;
   Relin r{};
   if (do_relin) r = *relin;
-  const Relin *rdev = nullptr;
+  const RelinDev *rdev = nullptr;
   if (do_relin) LNT_TRY(upload_relin(ctx, r, &rdev));
   uint32_t bits = 0;
   while (bits < 64 && (uint64_t(1) << bits) < bound) ++bits;

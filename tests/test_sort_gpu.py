@@ -219,3 +219,44 @@ def test_c2_scale_change_sort_order_against_torch_sort(ctx):
     torch.cuda.synchronize()
     assert torch.equal(k, ref_k)
     assert torch.equal(v.view(torch.int64), ref_v.view(torch.int64))
+
+
+def _relin_bytes_np(keys, perm):
+    """Closed form for dims 256^4: byte t of the new key is byte perm[t] of the old."""
+    out = np.zeros_like(keys)
+    for t, m in enumerate(perm):
+        out |= ((keys >> (8 * m)) & 255) << (8 * t)
+    return out
+
+
+@pytest.mark.parametrize("n", [3_000_000, 20_000_000])
+def test_change_linidx_large_n_against_closed_form(ctx, n):
+    """Relinearisation over millions of keys (every block of the grid and every
+    grid-stride round) against a numpy closed form, repeated to catch
+    intermittent errors."""
+    import torch
+
+    perm = [2, 0, 3, 1]
+    k = torch.empty(n, dtype=torch.int64, device="cuda")
+    ctx.testgen_distinct_keys(k, 1 << 32, 11)
+    expect = _relin_bytes_np(host(k), perm)
+    for _ in range(3):
+        rk = k.clone()
+        ctx.change_linidx(rk, [256] * 4, [0, 1, 2, 3], perm)
+        assert np.array_equal(host(rk), expect)
+
+
+def test_change_sort_order_large_n_against_closed_form(ctx):
+    import torch
+
+    n, perm = 20_000_000, [2, 0, 3, 1]
+    k = torch.empty(n, dtype=torch.int64, device="cuda")
+    v = torch.empty(n, dtype=torch.float64, device="cuda")
+    ctx.testgen_distinct_keys(k, 1 << 32, 13)
+    ctx.testgen_uniform(v, 14)
+    nk = _relin_bytes_np(host(k), perm)
+    order = np.argsort(nk, kind="stable")
+    ek, ev = nk[order], host(v)[order]
+    ctx.change_sort_order(k, v, [256] * 4, [0, 1, 2, 3], perm, False)
+    assert np.array_equal(host(k), ek)
+    assert _bits_equal(host(v), ev)
