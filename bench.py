@@ -99,6 +99,20 @@ def cpu_baseline(scale=8, repeats=3):
         os.sched_setaffinity(0, old)
 
 
+def _profiled_traffic():
+    """HBM bytes per onesweep launch measured by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, scaled by the calibration kernels), from the committed profile
+    summary; None if absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "onesweep_sort_merge_at_scale_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +120,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-scale", type=int, default=8)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after the timed steps, run a plain 1.6 GB write (fill) and read (sum) so that "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE can be scaled by kernels of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -195,6 +212,15 @@ def main():
         },
         "phases": phases,
     }
+    traffic = _profiled_traffic()
+    if traffic:
+        out["roofline"]["traffic"] = traffic
+    if args.calibrate:
+        cal = torch.empty(200_000_000, dtype=torch.float64, device="cuda")
+        cal.fill_(1.0)
+        float(cal.sum())
+        torch.cuda.synchronize()
+        del cal
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scale=args.cpu_scale)
     print(json.dumps(out), flush=True)
