@@ -108,14 +108,30 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
   for (int i = tid; i < passes * kRadix; i += kBlock) mine[i] = h[i];
 }
 
-// hist[i] = sum over blocks of part[b][i], one thread per (pass, digit).
-__global__ void histogram_reduce_kernel(const uint32_t *__restrict__ part, uint32_t blocks,
-                                        uint32_t width, uint32_t *__restrict__ hist) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= width) return;
+// hist[i] = sum over blocks of part[b][i].  A block covers 64 consecutive
+// (pass, digit) slots; its four waves each sum every fourth partial row with
+// coalesced, unrolled loads, then add up in LDS.
+__global__ __launch_bounds__(256) void histogram_reduce_kernel(const uint32_t *__restrict__ part,
+                                                               uint32_t blocks, uint32_t width,
+                                                               uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_sum[4][64];
+  const uint32_t lane = threadIdx.x & 63, group = threadIdx.x >> 6;
+  const uint32_t slot = blockIdx.x * 64 + lane;
   uint32_t acc = 0;
-  for (uint32_t b = 0; b < blocks; ++b) acc += part[(size_t)b * width + i];
-  hist[i] = acc;
+  if (slot < width) {
+    uint32_t b = group;
+    for (; b + 12 < blocks; b += 16) {
+      uint32_t x0 = part[(size_t)b * width + slot];
+      uint32_t x1 = part[(size_t)(b + 4) * width + slot];
+      uint32_t x2 = part[(size_t)(b + 8) * width + slot];
+      uint32_t x3 = part[(size_t)(b + 12) * width + slot];
+      acc += x0 + x1 + x2 + x3;
+    }
+    for (; b < blocks; b += 4) acc += part[(size_t)b * width + slot];
+  }
+  s_sum[group][lane] = acc;
+  __syncthreads();
+  if (group == 0 && slot < width) hist[slot] = s_sum[0][lane] + s_sum[1][lane] + s_sum[2][lane] + s_sum[3][lane];
 }
 
 // -------------------------------------------------------------- onesweep ---
@@ -430,7 +446,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   prof_end(ctx, slot);
   LNT_HIP(hipGetLastError());
   if (passes > 0) {
-    histogram_reduce_kernel<<<(width + 255) / 256, 256, 0, ctx->stream>>>(part, hblocks, width, hist);
+    histogram_reduce_kernel<<<(width + 63) / 64, 256, 0, ctx->stream>>>(part, hblocks, width, hist);
     LNT_HIP(hipGetLastError());
   }
 
