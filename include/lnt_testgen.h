@@ -25,6 +25,15 @@ int lnt_testgen_distinct_keys(lnt_context ctx, int64_t *keys, int64_t n,
 int lnt_testgen_uniform(lnt_context ctx, double *vals, int64_t n, uint64_t seed,
                         uint64_t offset);
 
+/* A batch of instances: instance i (0 <= i < count) owns entries
+ * [offsets[i], offsets[i+1]) (host array, offsets[0] = 0) and gets
+ * keys permute_index(t, domains[i], key_seed + i) and values
+ * uniform_pm1(val_seed + i, t), t = 0, 1, ... (each instance as one
+ * lnt_testgen_distinct_keys / lnt_testgen_uniform call with offset 0).
+ * keys or vals may be NULL.  Device pointers; synchronises. */
+int lnt_testgen_batch(lnt_context ctx, int64_t *keys, double *vals, const int64_t *offsets,
+                      const uint64_t *domains, int64_t count, uint64_t key_seed, uint64_t val_seed);
+
 #ifdef __cplusplus
 }
 #endif

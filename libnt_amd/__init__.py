@@ -63,6 +63,18 @@ SIGNATURES = {
     "lnt_lattice_mult": [_vp, _vp, _vp, _i64, _i64p, _vp, _vp, _i64, _i64p, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_testgen_distinct_keys": [_vp, _vp, _i64, _u64, _u64, _u64],
     "lnt_testgen_uniform": [_vp, _vp, _i64, _u64, _u64],
+    "lnt_testgen_batch": [_vp, _vp, _vp, _i64p, ctypes.POINTER(_u64), _i64, _u64, _u64],
+    "lnt_batch_create": [_vp, _vp, _i64, ctypes.POINTER(_vp)],
+    "lnt_batch_destroy": [_vp],
+    "lnt_batch_contract": [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p],
+    "lnt_batch_symbolic": [_vp, _vp, _vp, _vp, _vp, _i64p],
+    "lnt_batch_numeric": [_vp, _vp, _vp, _i64, _i64p],
+    "lnt_batch_capture": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64p],
+    "lnt_batch_replay": [_vp, _i64p],
+    "lnt_batch_launch": [_vp],
+    "lnt_batch_set_param": [_vp, ctypes.c_char_p, _i64],
+    "lnt_batch_get_param": [_vp, ctypes.c_char_p, _i64p],
+    "lnt_batch_nprod": [_vp, _vp, _vp, _i64p],
 }
 _RESTYPES = {"lnt_last_error": ctypes.c_char_p}
 
@@ -290,6 +302,15 @@ class Context:
         if mem != MEM_DEVICE:
             raise ValueError("testgen writes device memory")
         check(lib().lnt_testgen_uniform(self._h, vp, n, seed, offset))
+
+    def testgen_batch(self, keys, vals, offsets, domains, key_seed, val_seed):
+        """Per-instance keys/values (instance i: seeds key_seed+i, val_seed+i)."""
+        kp = _ptr_mem(keys)[0] if keys is not None else None
+        vp = _ptr_mem(vals)[0] if vals is not None else None
+        o, op = _arr_i64(offsets)
+        d = np.ascontiguousarray(domains, dtype=np.uint64)
+        check(lib().lnt_testgen_batch(self._h, kp, vp, op, d.ctypes.data_as(ctypes.POINTER(_u64)),
+                                      len(d), key_seed, val_seed))
 
 
 def exported_symbols_declared_in_headers(include_dir=None):

@@ -20,8 +20,15 @@ int fail(int status, const std::string &msg) {
   return status;
 }
 
+static thread_local bool g_alloc_forbidden = false;
+
+void set_alloc_forbidden(bool forbidden) { g_alloc_forbidden = forbidden; }
+
 int ensure(DeviceBuffer &b, size_t bytes) {
   if (b.bytes >= bytes) return LNT_OK;
+  if (g_alloc_forbidden)
+    return fail(LNT_ERR_DEVICE, "internal: workspace of " + std::to_string(bytes) +
+                                    " bytes needed while recording a graph (size it before capture)");
   if (b.ptr) {
     hipError_t e = hipFree(b.ptr);
     b.ptr = nullptr;
@@ -119,7 +126,7 @@ static hipEvent_t take_event(lnt_context ctx) {
 }
 
 int prof_begin(lnt_context ctx, int kind, double bytes) {
-  if (!ctx->profiling) return -1;
+  if (!ctx->profiling || ctx->capturing) return -1;
   ProfRecord r{kind, take_event(ctx), take_event(ctx), bytes};
   if (!r.start || !r.stop || hipEventRecord(r.start, ctx->stream) != hipSuccess) return -1;
   ctx->prof.push_back(r);
@@ -267,6 +274,7 @@ int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
   else if (n == "merge_items" && (value == 8 || value == 16)) ctx->tuning.merge_items = (int)value;
   else if (n == "spgemm_small" && in(2, 2048)) ctx->tuning.spgemm_small = (int)value;
   else if (n == "narrow_keys" && in(0, 1)) ctx->tuning.narrow_keys = (int)value;
+  else if (n == "digit_bits" && in(4, 8)) ctx->tuning.digit_bits = (int)value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter or value out of range: " + n);
   return LNT_OK;
 }
@@ -279,6 +287,7 @@ int lnt_context_get_param(lnt_context ctx, const char *name, int64_t *value) {
   else if (n == "merge_items") *value = ctx->tuning.merge_items;
   else if (n == "spgemm_small") *value = ctx->tuning.spgemm_small;
   else if (n == "narrow_keys") *value = ctx->tuning.narrow_keys;
+  else if (n == "digit_bits") *value = ctx->tuning.digit_bits;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter: " + n);
   return LNT_OK;
 }

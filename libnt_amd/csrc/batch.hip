@@ -1,0 +1,1168 @@
+// batch.hip -- many independent sparse contractions sharing every launch,
+// for gfx950.
+//
+// Every instance is the lattice product of spgemm.hip, C(:,j,t) =
+// sum_k B(k,j,t) * A(:,k,t) (SparseLatticeBase.h:1854-1990), after both operands
+// are re-sorted into lattice order (SparseMIABase.h:590-645 sort(seq)).  The
+// reference runs one expression at a time; here one pipeline serves a whole
+// batch and never waits for the host between its first and last launch, so it
+// can be recorded once into a hipGraph and replayed.
+//
+//   1. relinearise: every key into its instance's lattice order plus the
+//      instance's base, so that one radix sort of all instances at once leaves
+//      each instance's entries, sorted, at the offsets they came in at;
+//   2. columns: per B entry the A column range (binary search inside its
+//      instance), the output column and segment heads; scans give product
+//      offsets and segment ids (segment = one output column of one instance);
+//   3. classify: single-entry segments (a scaled copy of one A column), small
+//      segments (<= small_max products: grouped into work items of about
+//      item_products products, sorted in LDS together) and large segments
+//      (processed alone, in row windows of about `window` products);
+//   4. symbolic pass: unique rows per segment, then a scan gives every segment
+//      its output offset;
+//   5. numeric pass: the same work again, writing C straight to its offsets
+//      (no intermediate buffer: HBM holds only the inputs, the column tables
+//      and C).
+// Steps 4 and 5 run on a persistent grid that takes work items from atomic
+// tickets (a work queue).  Equal rows are summed in ascending B-row order --
+// the order of the sort strategy of spgemm.hip -- except for one row met by
+// more than `window` products of one column, which is summed by a tree.
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/lnt_batch.h"
+#include "lnt_internal.h"
+
+namespace lnt {
+
+template <typename T>
+int exclusive_scan(lnt_context ctx, const T *in, uint64_t *out, uint64_t n, uint64_t *total_dev,
+                   DeviceBuffer &tmp, size_t tmp_offset);
+size_t scan_workspace_words(uint64_t n);
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kCap = 2048;      // LDS elements of one sorted group
+constexpr int kPer = kCap / kBlock;  // sorted elements per thread
+constexpr uint32_t kMaxSegsPerItem = 1024 + 1;
+constexpr int kRowBits = 40;         // small items sort (segment, row) in one word
+constexpr uint64_t kRowMask = (uint64_t(1) << kRowBits) - 1;
+
+// Counters and totals (uint64 words of the batch's control block).
+enum {
+  kCtrNprod = 0,    // total products (scan total)
+  kCtrNseg,         // segments
+  kCtrNsmall,       // small segments
+  kCtrSmallProd,    // products of small segments
+  kCtrNc,           // nnz of all C
+  kCtrN3,           // large segments
+  kCtrTicketSmall,
+  kCtrTicketLarge,
+  kCtrKeyErr,       // an input key outside its instance's dimensionality
+  kCtrOverflow,     // C larger than the output capacity
+  kCtrWords = 16
+};
+
+struct InstDev {
+  uint64_t a_off, na, b_off, nb;  // packed entry ranges
+  uint64_t a_base, b_base;        // composite key bases
+  uint64_t a_dim, b_dim;          // key bounds
+  uint64_t h, w, bw;              // A h x w x d, B w x bw x d
+  Divider div_w, div_wbw;         // B lattice key -> (k, j, t)
+};
+
+__device__ inline uint64_t lower_bound_u64(const uint64_t *a, uint64_t lo, uint64_t hi, uint64_t x) {
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Largest i in [lo, hi) with off[i] <= x (off ascending, off[lo] <= x).
+template <typename T>
+__device__ inline uint64_t upper_index(const T *off, uint64_t lo, uint64_t hi, uint64_t x) {
+  while (hi - lo > 1) {
+    uint64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)off[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t *scratch) {
+  uint64_t total;
+  block_exclusive_sum64_lds(v, scratch, &total);
+  return total;
+}
+
+__device__ inline uint64_t block_min_u64(uint64_t v, uint64_t *scratch) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t y = __shfl_xor(v, off, kWave);
+    v = y < v ? y : v;
+  }
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  uint64_t m = scratch[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / kWave; ++w) m = scratch[w] < m ? scratch[w] : m;
+  __syncthreads();
+  return m;
+}
+
+// Tree sum of one double per thread (deterministic for a fixed assignment).
+__device__ inline double block_sum_f64(double v, double *scratch) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double s = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
+  __syncthreads();
+  return s;
+}
+
+// Bitonic sort of s_key[0, P) with s_pos as tie-break; P a power of two in
+// [2, kCap].  Each step, every thread handles P/2/kBlock compare-exchanges.
+__device__ inline void bitonic_key_pos(uint64_t *s_key, uint16_t *s_pos, uint32_t P) {
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t q = threadIdx.x; q < (P >> 1); q += kBlock) {
+        const uint32_t lo = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+        const uint32_t hi = lo + j;
+        const uint64_t kl = s_key[lo], kh = s_key[hi];
+        const uint16_t pl = s_pos[lo], ph = s_pos[hi];
+        const bool gt = kl > kh || (kl == kh && pl > ph);
+        const bool up = (lo & k) == 0;
+        if (gt == up) {
+          s_key[lo] = kh;
+          s_key[hi] = kl;
+          s_pos[lo] = ph;
+          s_pos[hi] = pl;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Instance of element p, given that this block's elements lie in instances
+// [lo, hi] (found once per block).
+__device__ inline uint32_t instance_of(const uint64_t *off, uint32_t lo, uint32_t hi, uint64_t p) {
+  return lo == hi ? lo : (uint32_t)upper_index(off, lo, (uint64_t)hi + 1, p);
+}
+
+// ------------------------------------------------------------------ 1. keys
+// Composite key = instance base + key relinearised into lattice order; values
+// copied alongside (the caller's operands stay untouched).
+__global__ __launch_bounds__(kBlock) void batch_relin_kernel(
+    const int64_t *__restrict__ keys, const double *__restrict__ vals, uint64_t n,
+    const uint64_t *__restrict__ off, uint32_t n_inst, const InstDev *__restrict__ inst, int which,
+    const RelinDev *__restrict__ relin, uint64_t *__restrict__ out_keys, double *__restrict__ out_vals,
+    uint64_t *__restrict__ ctr) {
+  __shared__ uint32_t s_lo, s_hi;
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < n; base += (uint64_t)gridDim.x * kBlock) {
+    if (threadIdx.x == 0) {
+      uint64_t last = (base + kBlock < n ? base + kBlock : n) - 1;
+      s_lo = (uint32_t)upper_index(off, 0, n_inst, base);
+      s_hi = (uint32_t)upper_index(off, 0, n_inst, last);
+    }
+    __syncthreads();
+    const uint32_t lo = s_lo, hi = s_hi;
+    __syncthreads();
+    const uint64_t p = base + threadIdx.x;
+    if (p >= n) continue;
+    const uint32_t i = instance_of(off, lo, hi, p);
+    const int64_t raw = keys[p];
+    const uint64_t bound = which ? inst[i].b_dim : inst[i].a_dim;
+    uint64_t k[1] = {raw < 0 || (uint64_t)raw >= bound ? 0 : (uint64_t)raw};
+    if (raw < 0 || (uint64_t)raw >= bound) atomicOr((unsigned long long *)&ctr[kCtrKeyErr], 1ull);
+    relin_keys<1>(relin + i, k);
+    out_keys[p] = (which ? inst[i].b_base : inst[i].a_base) + k[0];
+    if (vals) out_vals[p] = vals[p];
+  }
+}
+
+// --------------------------------------------------------------- 2. columns
+__global__ __launch_bounds__(kBlock) void batch_entry_kernel(
+    const uint64_t *__restrict__ ak, const uint64_t *__restrict__ bk, uint64_t nb,
+    const uint64_t *__restrict__ b_off, uint32_t n_inst, const InstDev *__restrict__ inst,
+    uint64_t *__restrict__ a_lo, uint32_t *__restrict__ a_len, uint64_t *__restrict__ a_base,
+    uint64_t *__restrict__ colkey, uint32_t *__restrict__ einst, uint32_t *__restrict__ head) {
+  __shared__ uint32_t s_lo, s_hi;
+  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < nb; base += (uint64_t)gridDim.x * kBlock) {
+    if (threadIdx.x == 0) {
+      uint64_t last = (base + kBlock < nb ? base + kBlock : nb) - 1;
+      s_lo = (uint32_t)upper_index(b_off, 0, n_inst, base);
+      s_hi = (uint32_t)upper_index(b_off, 0, n_inst, last);
+    }
+    __syncthreads();
+    const uint32_t lo = s_lo, hi = s_hi;
+    __syncthreads();
+    const uint64_t p = base + threadIdx.x;
+    if (p >= nb) continue;
+    const uint32_t i = instance_of(b_off, lo, hi, p);
+    const InstDev &I = inst[i];
+    const uint64_t key = bk[p] - I.b_base;
+    const uint64_t t = divide(key, I.div_wbw);
+    const uint64_t rem = key - t * I.div_wbw.d;
+    const uint64_t j = divide(rem, I.div_w);
+    const uint64_t k = rem - j * I.w;
+    const uint64_t start = I.a_base + (t * I.w + k) * I.h;
+    const uint64_t a_end = I.a_off + I.na;
+    const uint64_t l = lower_bound_u64(ak, I.a_off, a_end, start);
+    const uint64_t r = lower_bound_u64(ak, l, a_end, start + I.h);
+    const uint64_t col = t * I.bw + j;
+    a_lo[p] = l;
+    a_len[p] = (uint32_t)(r - l);
+    a_base[p] = start;
+    colkey[p] = I.h * col;
+    einst[p] = i;
+    uint32_t hd = 1;
+    if (p > I.b_off) {
+      const uint64_t pk = bk[p - 1] - I.b_base;
+      const uint64_t pt = divide(pk, I.div_wbw);
+      const uint64_t pj = divide(pk - pt * I.div_wbw.d, I.div_w);
+      hd = (pt * I.bw + pj) != col;
+    }
+    head[p] = hd;
+  }
+}
+
+__global__ void batch_seg_first_kernel(const uint32_t *__restrict__ head, const uint64_t *__restrict__ seg_idx,
+                                       uint64_t nb, uint64_t *__restrict__ seg_first) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < nb; i += (uint64_t)gridDim.x * blockDim.x)
+    if (head[i]) seg_first[seg_idx[i]] = i;
+}
+
+__device__ inline uint64_t seg_last(const uint64_t *seg_first, uint64_t s, uint64_t nseg, uint64_t nb) {
+  return s + 1 < nseg ? seg_first[s + 1] : nb;
+}
+
+// -------------------------------------------------------------- 3. classify
+// kind: 0 no products, 1 single B entry, 2 small, 3 large.  Arrays are
+// written for all nb slots (zero past the last segment) so that scans over
+// nb elements see only real segments.
+__global__ void batch_classify_kernel(const uint64_t *__restrict__ seg_first, const uint64_t *__restrict__ prod_off,
+                                      const uint32_t *__restrict__ einst, const InstDev *__restrict__ inst,
+                                      uint64_t nb, uint32_t small_max, uint64_t *__restrict__ ctr,
+                                      uint32_t *__restrict__ kind, uint32_t *__restrict__ is_small,
+                                      uint64_t *__restrict__ seg_u, uint32_t *__restrict__ list3) {
+  const uint64_t nseg = ctr[kCtrNseg];
+  uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; s < nb; s += (uint64_t)gridDim.x * blockDim.x) {
+    if (s >= nseg) {
+      kind[s] = 0;
+      is_small[s] = 0;
+      seg_u[s] = 0;
+      continue;
+    }
+    const uint64_t first = seg_first[s], last = seg_last(seg_first, s, nseg, nb);
+    const uint64_t np = prod_off[last] - prod_off[first];
+    uint32_t k;
+    if (np == 0) k = 0;
+    else if (last - first == 1) k = 1;
+    else if (np <= small_max && inst[einst[first]].h <= kRowMask) k = 2;
+    else k = 3;
+    kind[s] = k;
+    is_small[s] = k == 2;
+    seg_u[s] = k == 1 ? np : 0;
+    if (k == 3) list3[atomicAdd((unsigned long long *)&ctr[kCtrN3], 1ull)] = (uint32_t)s;
+  }
+}
+
+// Ordered list of small segments and their product counts (np_list must be
+// zero beyond the last small segment).
+__global__ void batch_small_list_kernel(const uint32_t *__restrict__ is_small, const uint64_t *__restrict__ small_rank,
+                                        const uint64_t *__restrict__ seg_first, const uint64_t *__restrict__ prod_off,
+                                        uint64_t nb, const uint64_t *__restrict__ ctr,
+                                        uint32_t *__restrict__ small_list, uint32_t *__restrict__ np_list) {
+  const uint64_t nseg = ctr[kCtrNseg];
+  uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; s < nseg; s += (uint64_t)gridDim.x * blockDim.x) {
+    if (!is_small[s]) continue;
+    const uint64_t r = small_rank[s];
+    const uint64_t first = seg_first[s], last = seg_last(seg_first, s, nseg, nb);
+    small_list[r] = (uint32_t)s;
+    np_list[r] = (uint32_t)(prod_off[last] - prod_off[first]);
+  }
+}
+
+// Product p (0-based inside the segment [first, last)) -> B entry e and A index.
+__device__ inline void locate(const uint64_t *prod_off, uint64_t first, uint64_t last, uint64_t p,
+                              const uint64_t *a_lo, uint64_t *e_out, uint64_t *a_idx) {
+  const uint64_t base = prod_off[first];
+  const uint64_t e = upper_index(prod_off, first, last, base + p);
+  *e_out = e;
+  *a_idx = a_lo[e] + (base + p - prod_off[e]);
+}
+
+struct ColumnArgs {
+  const uint64_t *ak;
+  const double *av, *bv;
+  const uint64_t *a_lo, *a_base, *colkey, *prod_off, *seg_first, *out_off;
+  const uint32_t *a_len, *einst;
+  const InstDev *inst;
+  uint64_t nb;
+  uint64_t *seg_u;  // symbolic output
+  uint64_t *ck;     // numeric output
+  double *cv;
+  uint64_t capacity;
+};
+
+// ------------------------------------------------ 4/5. small segments, items
+// Work item t = the small segments whose compact product offset lies in
+// [t*G, (t+1)*G): at most G segments and G + small_max - 1 products.  The
+// products of all of them are sorted together by (segment, row, position).
+template <bool NUMERIC>
+__global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const uint32_t *__restrict__ small_list,
+                                                             const uint64_t *__restrict__ sp_off, uint32_t G,
+                                                             uint64_t *__restrict__ ctr) {
+  __shared__ uint64_t s_key[kCap];
+  __shared__ uint16_t s_pos[kCap];
+  __shared__ double s_val[NUMERIC ? kCap : 1];
+  __shared__ uint32_t s_seg[kMaxSegsPerItem];
+  __shared__ uint32_t s_spo[kMaxSegsPerItem + 1];
+  __shared__ uint16_t s_rstart[kMaxSegsPerItem];
+  __shared__ uint32_t s_scan[kBlock / kWave];
+  __shared__ uint64_t s_item, s_rb, s_re;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t nseg = ctr[kCtrNseg];
+  const uint64_t n_small = ctr[kCtrNsmall];
+  const uint64_t total = ctr[kCtrSmallProd];
+  const uint64_t n_items = (total + G - 1) / G;
+  for (;;) {
+    if (tid == 0) {
+      uint64_t item = atomicAdd((unsigned long long *)&ctr[kCtrTicketSmall], 1ull);
+      s_item = item;
+      if (item < n_items) {
+        s_rb = lower_bound_u64(sp_off, 0, n_small, item * G);
+        s_re = lower_bound_u64(sp_off, s_rb, n_small, (item + 1) * G);
+      }
+    }
+    __syncthreads();
+    const uint64_t item = s_item;
+    if (item >= n_items) break;
+    const uint64_t rb = s_rb, re = s_re;
+    const uint32_t nr = (uint32_t)(re - rb);
+    const uint64_t pbase = rb < n_small ? sp_off[rb] : total;
+    for (uint32_t q = tid; q <= nr; q += kBlock) {
+      if (q < nr) s_seg[q] = small_list[rb + q];
+      s_spo[q] = (uint32_t)((rb + q < n_small ? sp_off[rb + q] : total) - pbase);
+    }
+    __syncthreads();
+    const uint32_t np = s_spo[nr];
+    uint32_t P = 2;
+    while (P < np) P <<= 1;
+    for (uint32_t p = tid; p < P; p += kBlock) {
+      uint64_t key = ~0ull;
+      if (p < np) {
+        const uint32_t q = (uint32_t)upper_index(s_spo, 0, nr, p);
+        const uint64_t s = s_seg[q];
+        const uint64_t first = A.seg_first[s], last = seg_last(A.seg_first, s, nseg, A.nb);
+        uint64_t e, ai;
+        locate(A.prod_off, first, last, p - s_spo[q], A.a_lo, &e, &ai);
+        key = ((uint64_t)q << kRowBits) | (A.ak[ai] - A.a_base[e]);
+        if (NUMERIC) s_val[p] = A.bv[e] * A.av[ai];
+      }
+      s_key[p] = key;
+      s_pos[p] = (uint16_t)p;
+    }
+    __syncthreads();
+    bitonic_key_pos(s_key, s_pos, P);
+    // Heads of equal-key runs, ranked; the first element of every segment
+    // records its rank so that ranks can be made segment-relative.
+    uint32_t flags = 0, cnt = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const uint32_t p = tid * kPer + u;
+      const bool h = p < np && (p == 0 || s_key[p] != s_key[p - 1]);
+      flags |= (h ? 1u : 0u) << u;
+      cnt += h;
+    }
+    uint32_t heads;
+    uint32_t rank = block256_exclusive_sum(cnt, s_scan, &heads);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const uint32_t p = tid * kPer + u;
+      if ((flags >> u) & 1u) {
+        if (p == 0 || (s_key[p] >> kRowBits) != (s_key[p - 1] >> kRowBits))
+          s_rstart[s_key[p] >> kRowBits] = (uint16_t)rank;
+        ++rank;
+      }
+    }
+    __syncthreads();
+    if (!NUMERIC) {
+      for (uint32_t q = tid; q < nr; q += kBlock) {
+        const uint32_t end = q + 1 < nr ? s_rstart[q + 1] : heads;
+        A.seg_u[s_seg[q]] = end - s_rstart[q];
+      }
+    } else {
+      rank -= cnt;
+      for (int u = 0; u < kPer; ++u) {
+        if (!((flags >> u) & 1u)) continue;
+        const uint32_t p = tid * kPer + u;
+        const uint64_t key = s_key[p];
+        const uint32_t q = (uint32_t)(key >> kRowBits);
+        double sum = s_val[s_pos[p]];
+        for (uint32_t r = p + 1; r < np && s_key[r] == key; ++r) sum += s_val[s_pos[r]];
+        const uint64_t s = s_seg[q];
+        const uint64_t dst = A.out_off[s] + (rank - s_rstart[q]);
+        if (dst < A.capacity) {
+          A.ck[dst] = (key & kRowMask) + A.colkey[A.seg_first[s]];
+          A.cv[dst] = sum;
+        }
+        ++rank;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------- 4/5. large segments, windows
+// One segment per work item.  Rows are taken in windows [r0, r1) holding at
+// most kCap products (about `window` aimed for), found by binary searches of
+// every entry's A column from its cursor; empty stretches are skipped by
+// jumping to the smallest next row.  Each window is sorted by (row, position)
+// and its runs summed, so rows are summed in ascending B-row order.
+template <bool NUMERIC>
+__global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const uint32_t *__restrict__ list3,
+                                                             uint64_t *__restrict__ cur, uint32_t *__restrict__ wcnt,
+                                                             uint32_t *__restrict__ woff, uint32_t window,
+                                                             uint64_t *__restrict__ ctr) {
+  __shared__ uint64_t s_key[kCap];
+  __shared__ uint16_t s_pos[kCap];
+  __shared__ double s_val[NUMERIC ? kCap : 1];
+  __shared__ uint64_t s_red[kBlock / kWave];
+  __shared__ double s_fred[kBlock / kWave];
+  __shared__ uint32_t s_scan[kBlock / kWave];
+  __shared__ uint64_t s_item;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t nseg = ctr[kCtrNseg];
+  const uint64_t n3 = ctr[kCtrN3];
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd((unsigned long long *)&ctr[kCtrTicketLarge], 1ull);
+    __syncthreads();
+    const uint64_t item = s_item;
+    __syncthreads();
+    if (item >= n3) break;
+    const uint64_t s = list3[item];
+    const uint64_t first = A.seg_first[s], last = seg_last(A.seg_first, s, nseg, A.nb);
+    const uint64_t np = A.prod_off[last] - A.prod_off[first];
+    const uint64_t h = A.inst[A.einst[first]].h;
+    const uint64_t ckey = A.colkey[first];
+    const uint64_t obase = NUMERIC ? A.out_off[s] : 0;
+    for (uint64_t e = first + tid; e < last; e += kBlock) cur[e] = A.a_lo[e];
+    __syncthreads();
+    uint64_t done = 0, written = 0, r0 = 0;
+    while (done < np) {
+      const uint64_t left = np - done, span = h - r0;
+      uint64_t S = span;
+      if (left > window) {
+        double est = (double)span * (double)window / (double)left;
+        S = est < 1.0 ? 1 : (est >= (double)span ? span : (uint64_t)est);
+      }
+      uint64_t cnt, r1;
+      for (;;) {
+        r1 = S >= span ? h : r0 + S;
+        uint64_t local = 0;
+        for (uint64_t e = first + tid; e < last; e += kBlock) {
+          const uint64_t c = cur[e], end = A.a_lo[e] + A.a_len[e];
+          const uint64_t x = lower_bound_u64(A.ak, c, end, A.a_base[e] + r1);
+          wcnt[e] = (uint32_t)(x - c);
+          local += x - c;
+        }
+        cnt = block_sum_u64(local, s_red);
+        if (cnt <= kCap || S == 1) break;
+        uint64_t S2 = (uint64_t)((double)S * (double)window / (double)cnt);
+        S = S2 < 1 ? 1 : (S2 < S ? S2 : S / 2);
+      }
+      if (cnt == 0) {
+        uint64_t local = ~0ull;
+        for (uint64_t e = first + tid; e < last; e += kBlock) {
+          const uint64_t c = cur[e], end = A.a_lo[e] + A.a_len[e];
+          if (c < end) {
+            const uint64_t row = A.ak[c] - A.a_base[e];
+            local = row < local ? row : local;
+          }
+        }
+        r0 = block_min_u64(local, s_red);
+        if (r0 >= h) break;  // unreachable while the cursors agree with np
+        continue;
+      }
+      if (cnt > kCap) {
+        // One row (r1 = r0 + 1) met by more than kCap products, each from a
+        // different A column: a tree sum over the entries.
+        double part = 0.0;
+        for (uint64_t e = first + tid; e < last; e += kBlock)
+          if (wcnt[e]) part += A.bv[e] * A.av[cur[e]];
+        const double sum = block_sum_f64(part, s_fred);
+        if (NUMERIC && tid == 0 && obase + written < A.capacity) {
+          A.ck[obase + written] = r0 + ckey;
+          A.cv[obase + written] = sum;
+        }
+        for (uint64_t e = first + tid; e < last; e += kBlock) cur[e] += wcnt[e];
+        __syncthreads();
+        done += cnt;
+        written += 1;
+        r0 = r1;
+        continue;
+      }
+      // Window offsets of every entry (chunked block scan over the entries).
+      uint32_t running = 0;
+      for (uint64_t c0 = first; c0 < last; c0 += kBlock) {
+        const uint64_t e = c0 + tid;
+        const uint32_t v = e < last ? wcnt[e] : 0;
+        uint32_t tot;
+        const uint32_t ex = block256_exclusive_sum(v, s_scan, &tot);
+        if (e < last) woff[e] = running + ex;
+        running += tot;
+      }
+      __syncthreads();
+      const uint32_t n = (uint32_t)cnt;
+      uint32_t P = 2;
+      while (P < n) P <<= 1;
+      for (uint32_t p = tid; p < P; p += kBlock) {
+        uint64_t key = ~0ull;
+        if (p < n) {
+          const uint64_t e = upper_index(woff, first, last, p);
+          const uint64_t x = cur[e] + (p - woff[e]);
+          key = A.ak[x] - A.a_base[e];
+          if (NUMERIC) s_val[p] = A.bv[e] * A.av[x];
+        }
+        s_key[p] = key;
+        s_pos[p] = (uint16_t)p;
+      }
+      __syncthreads();
+      bitonic_key_pos(s_key, s_pos, P);
+      uint32_t flags = 0, hc = 0;
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const uint32_t p = tid * kPer + u;
+        const bool hd = p < n && (p == 0 || s_key[p] != s_key[p - 1]);
+        flags |= (hd ? 1u : 0u) << u;
+        hc += hd;
+      }
+      uint32_t heads;
+      uint32_t rank = block256_exclusive_sum(hc, s_scan, &heads);
+      if (NUMERIC) {
+        for (int u = 0; u < kPer; ++u) {
+          if (!((flags >> u) & 1u)) continue;
+          const uint32_t p = tid * kPer + u;
+          const uint64_t key = s_key[p];
+          double sum = s_val[s_pos[p]];
+          for (uint32_t r = p + 1; r < n && s_key[r] == key; ++r) sum += s_val[s_pos[r]];
+          const uint64_t dst = obase + written + rank;
+          if (dst < A.capacity) {
+            A.ck[dst] = key + ckey;
+            A.cv[dst] = sum;
+          }
+          ++rank;
+        }
+      }
+      for (uint64_t e = first + tid; e < last; e += kBlock) cur[e] += wcnt[e];
+      __syncthreads();
+      done += cnt;
+      written += heads;
+      r0 = r1;
+    }
+    if (!NUMERIC && tid == 0) A.seg_u[s] = written;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------- 5. single-entry segments: copy
+// One wave per segment (grid-stride over segments): C(:,j) = b * A(:,k).
+__global__ __launch_bounds__(kBlock) void batch_single_kernel(ColumnArgs A, const uint32_t *__restrict__ kind,
+                                                              const uint64_t *__restrict__ ctr) {
+  const uint64_t nseg = ctr[kCtrNseg];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6); s < nseg; s += waves) {
+    if (kind[s] != 1) continue;
+    const uint64_t e = A.seg_first[s];
+    const uint64_t lo = A.a_lo[e], n = A.a_len[e], ab = A.a_base[e], ck0 = A.colkey[e];
+    const double b = A.bv[e];
+    const uint64_t o = A.out_off[s];
+    for (uint64_t q = lane; q < n; q += kWave) {
+      if (o + q >= A.capacity) break;
+      A.ck[o + q] = A.ak[lo + q] - ab + ck0;
+      A.cv[o + q] = b * A.av[lo + q];
+    }
+  }
+}
+
+// Per-instance output offsets and products; overflow check.
+__global__ void batch_offsets_kernel(const uint64_t *__restrict__ b_off, uint32_t n_inst, uint64_t nb,
+                                     const uint64_t *__restrict__ seg_idx, const uint64_t *__restrict__ out_off,
+                                     const uint64_t *__restrict__ prod_off, uint64_t capacity,
+                                     uint64_t *__restrict__ ctr, uint64_t *__restrict__ c_off,
+                                     uint64_t *__restrict__ nprod) {
+  const uint64_t nc = ctr[kCtrNc];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n_inst;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t p = b_off[i];
+    c_off[i] = (i < n_inst && p < nb) ? out_off[seg_idx[p]] : nc;
+    if (i < n_inst) nprod[i] = prod_off[b_off[i + 1]] - prod_off[p];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && nc > capacity) ctr[kCtrOverflow] = 1;
+}
+
+__global__ void batch_clear_kernel(uint64_t *__restrict__ ctr) {
+  if (threadIdx.x < kCtrWords) ctr[threadIdx.x] = 0;
+}
+
+__global__ void batch_zero_u32_kernel(uint32_t *__restrict__ p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t cap = 16384) {
+  uint64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  return (uint32_t)(b > cap ? cap : b);
+}
+
+struct Carve {
+  char *p;
+  template <typename T>
+  T *take(uint64_t count) {
+    T *r = (T *)p;
+    p += ((count * sizeof(T) + 255) / 256) * 256;
+    return r;
+  }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------- host
+struct BatchTuning {
+  uint32_t item_products = 1024;
+  uint32_t small_max = 1024;
+  uint32_t window = 1024;
+  uint32_t blocks_per_cu = 3;
+};
+
+}  // namespace lnt
+
+struct lnt_batch_s {
+  lnt_context ctx = nullptr;
+  uint32_t count = 0;
+  uint64_t na = 0, nb = 0;
+  uint64_t a_bound = 0, b_bound = 0;
+  int cus = 256;
+  lnt::BatchTuning tuning;
+  lnt::DeviceBuffer meta;  // InstDev[count], RelinDev[2*count], a_off/b_off[count+1]
+  lnt::DeviceBuffer ws;    // per-entry tables
+  lnt::DeviceBuffer scan_tmp;
+  lnt::DeviceBuffer ctrl;  // counters, c_off[count+1], nprod[count]
+  uint64_t *pinned = nullptr;  // host mirror of counters + c_off + nprod
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t own_stream = nullptr;
+  int64_t graph_capacity = 0;
+  bool symbolic_ready = false;  // tables of the last symbolic phase are live
+};
+
+namespace lnt {
+namespace {
+
+struct BatchPtrs {
+  InstDev *inst;
+  RelinDev *relin_a, *relin_b;
+  uint64_t *a_off, *b_off;
+  uint64_t *ka, *kb;
+  double *va, *vb;
+  uint64_t *a_lo, *a_base, *colkey, *prod_off, *seg_idx, *seg_first, *seg_u, *out_off, *small_rank, *sp_off, *cur;
+  uint32_t *a_len, *einst, *head, *kind, *is_small, *small_list, *np_list, *list3, *wcnt, *woff;
+  uint64_t *ctr, *c_off, *nprod;
+};
+
+size_t ws_bytes(uint64_t na, uint64_t nb) {
+  const uint64_t n1 = nb + 1;
+  return 256 * 40 + 16 * na + n1 * (8 + 8 + 8 * 11 + 4 * 10);
+}
+
+void carve(lnt_batch b, BatchPtrs *P) {
+  Carve m{(char *)b->meta.ptr};
+  P->inst = m.take<InstDev>(b->count);
+  P->relin_a = m.take<RelinDev>(b->count);
+  P->relin_b = m.take<RelinDev>(b->count);
+  P->a_off = m.take<uint64_t>(b->count + 1);
+  P->b_off = m.take<uint64_t>(b->count + 1);
+  Carve c{(char *)b->ws.ptr};
+  const uint64_t n1 = b->nb + 1;
+  P->ka = c.take<uint64_t>(b->na);
+  P->va = c.take<double>(b->na);
+  P->kb = c.take<uint64_t>(b->nb);
+  P->vb = c.take<double>(b->nb);
+  P->a_lo = c.take<uint64_t>(n1);
+  P->a_base = c.take<uint64_t>(n1);
+  P->colkey = c.take<uint64_t>(n1);
+  P->prod_off = c.take<uint64_t>(n1);
+  P->seg_idx = c.take<uint64_t>(n1);
+  P->seg_first = c.take<uint64_t>(n1);
+  P->seg_u = c.take<uint64_t>(n1);
+  P->out_off = c.take<uint64_t>(n1);
+  P->small_rank = c.take<uint64_t>(n1);
+  P->sp_off = c.take<uint64_t>(n1);
+  P->cur = c.take<uint64_t>(n1);
+  P->a_len = c.take<uint32_t>(n1);
+  P->einst = c.take<uint32_t>(n1);
+  P->head = c.take<uint32_t>(n1);
+  P->kind = c.take<uint32_t>(n1);
+  P->is_small = c.take<uint32_t>(n1);
+  P->small_list = c.take<uint32_t>(n1);
+  P->np_list = c.take<uint32_t>(n1);
+  P->list3 = c.take<uint32_t>(n1);
+  P->wcnt = c.take<uint32_t>(n1);
+  P->woff = c.take<uint32_t>(n1);
+  Carve k{(char *)b->ctrl.ptr};
+  P->ctr = k.take<uint64_t>(kCtrWords);
+  P->c_off = k.take<uint64_t>(b->count + 1);
+  P->nprod = k.take<uint64_t>(b->count);
+}
+
+// Everything up to and including the column tables and the symbolic pass
+// (when `symbolic`); no host synchronisation.
+int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const double *a_vals,
+                  const int64_t *b_keys, const double *b_vals, bool symbolic) {
+  lnt_context ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const uint64_t nb = b->nb;
+  batch_clear_kernel<<<1, 64, 0, st>>>(P.ctr);
+  LNT_HIP(hipGetLastError());
+  if (b->na)
+    batch_relin_kernel<<<grid_for(b->na), kBlock, 0, st>>>(a_keys, a_vals, b->na, P.a_off, b->count, P.inst, 0,
+                                                           P.relin_a, P.ka, P.va, P.ctr);
+  if (nb)
+    batch_relin_kernel<<<grid_for(nb), kBlock, 0, st>>>(b_keys, b_vals, nb, P.b_off, b->count, P.inst, 1,
+                                                        P.relin_b, P.kb, P.vb, P.ctr);
+  LNT_HIP(hipGetLastError());
+  if (b->na > 1) LNT_TRY(sort_device(ctx, P.ka, P.va, (uint32_t)b->na, b->a_bound, nullptr));
+  if (nb > 1) LNT_TRY(sort_device(ctx, P.kb, P.vb, (uint32_t)nb, b->b_bound, nullptr));
+  if (nb == 0) return LNT_OK;
+  batch_entry_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.ka, P.kb, nb, P.b_off, b->count, P.inst, P.a_lo, P.a_len,
+                                                      P.a_base, P.colkey, P.einst, P.head);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, P.a_len, P.prod_off, nb, P.prod_off + nb, b->scan_tmp, 0));
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, P.head, P.seg_idx, nb, P.ctr + kCtrNseg, b->scan_tmp, 0));
+  if (!symbolic) return LNT_OK;
+  batch_seg_first_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.head, P.seg_idx, nb, P.seg_first);
+  const uint32_t small_max = std::min<uint32_t>(b->tuning.small_max, kCap / 2);
+  batch_classify_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.seg_first, P.prod_off, P.einst, P.inst, nb, small_max,
+                                                         P.ctr, P.kind, P.is_small, P.seg_u, P.list3);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, P.is_small, P.small_rank, nb, P.ctr + kCtrNsmall, b->scan_tmp, 0));
+  batch_zero_u32_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.np_list, nb);
+  batch_small_list_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.is_small, P.small_rank, P.seg_first, P.prod_off, nb,
+                                                           P.ctr, P.small_list, P.np_list);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, P.np_list, P.sp_off, nb, P.ctr + kCtrSmallProd, b->scan_tmp, 0));
+  ColumnArgs A{P.ka, P.va, P.vb, P.a_lo, P.a_base, P.colkey, P.prod_off, P.seg_first, P.out_off,
+               P.a_len, P.einst, P.inst, nb, P.seg_u, nullptr, nullptr, 0};
+  const uint32_t grid = (uint32_t)b->cus * b->tuning.blocks_per_cu;
+  const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
+  const uint32_t window = std::min<uint32_t>(b->tuning.window, kCap / 2);
+  batch_small_kernel<false><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, P.ctr);
+  batch_large_kernel<false><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint64_t>(ctx, P.seg_u, P.out_off, nb, P.ctr + kCtrNc, b->scan_tmp, 0));
+  return LNT_OK;
+}
+
+int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, uint64_t capacity) {
+  hipStream_t st = b->ctx->stream;
+  const uint64_t nb = b->nb;
+  if (nb) {
+    ColumnArgs A{P.ka, P.va, P.vb, P.a_lo, P.a_base, P.colkey, P.prod_off, P.seg_first, P.out_off,
+                 P.a_len, P.einst, P.inst, nb, P.seg_u, ck, cv, capacity};
+    const uint32_t grid = (uint32_t)b->cus * b->tuning.blocks_per_cu;
+    const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
+    const uint32_t window = std::min<uint32_t>(b->tuning.window, kCap / 2);
+    batch_single_kernel<<<std::min<uint32_t>(grid_for(nb * 64), 65536), kBlock, 0, st>>>(A, P.kind, P.ctr);
+    batch_small_kernel<true><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, P.ctr);
+    batch_large_kernel<true><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
+    LNT_HIP(hipGetLastError());
+  }
+  batch_offsets_kernel<<<grid_for(b->count + 1), kBlock, 0, st>>>(P.b_off, b->count, nb, P.seg_idx, P.out_off,
+                                                                  P.prod_off, capacity, P.ctr, P.c_off, P.nprod);
+  LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+// Tickets of the numeric pass start from zero again.
+__global__ void batch_reset_tickets_kernel(uint64_t *__restrict__ ctr) {
+  if (threadIdx.x == 0) {
+    ctr[kCtrTicketSmall] = 0;
+    ctr[kCtrTicketLarge] = 0;
+  }
+}
+
+int enqueue_all(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const double *a_vals, const int64_t *b_keys,
+                const double *b_vals, uint64_t *ck, double *cv, uint64_t capacity) {
+  LNT_TRY(enqueue_front(b, P, a_keys, a_vals, b_keys, b_vals, true));
+  batch_reset_tickets_kernel<<<1, 64, 0, b->ctx->stream>>>(P.ctr);
+  LNT_HIP(hipGetLastError());
+  return enqueue_numeric(b, P, ck, cv, capacity);
+}
+
+// Copies counters, offsets and products to the host mirror and waits.
+int read_back(lnt_batch b) {
+  LNT_HIP(hipMemcpyAsync(b->pinned, b->ctrl.ptr, b->ctrl.bytes, hipMemcpyDeviceToHost, b->ctx->stream));
+  LNT_HIP(hipStreamSynchronize(b->ctx->stream));
+  return LNT_OK;
+}
+
+// c_off sits right after the counters in the control block (256-byte carve).
+const uint64_t *host_c_off(lnt_batch b) { return b->pinned + ((kCtrWords * 8 + 255) / 256) * 256 / 8; }
+const uint64_t *host_nprod(lnt_batch b) {
+  return host_c_off(b) + (((b->count + 1) * 8 + 255) / 256) * 256 / 8;
+}
+
+int check_key_error(lnt_batch b) {
+  if (b->pinned[kCtrKeyErr])
+    return fail(LNT_ERR_KEY_RANGE, "a key is negative or not below its instance's dimensionality");
+  return LNT_OK;
+}
+
+int prepare(lnt_batch b) {
+  LNT_TRY(ensure(b->ws, ws_bytes(b->na, b->nb)));
+  LNT_TRY(ensure(b->scan_tmp, (scan_workspace_words(b->nb + 1) + 64) * 8));
+  return LNT_OK;
+}
+
+struct StreamSwap {
+  lnt_context ctx;
+  hipStream_t saved;
+  StreamSwap(lnt_context c, hipStream_t s) : ctx(c), saved(c->stream) { ctx->stream = s; }
+  ~StreamSwap() { ctx->stream = saved; }
+};
+
+}  // namespace
+}  // namespace lnt
+
+using namespace lnt;
+
+extern "C" {
+
+int lnt_batch_create(lnt_context ctx, const lnt_contraction *items, int64_t count, lnt_batch *out) {
+  if (!ctx || !out || (count > 0 && !items)) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (count < 0 || count >= ((int64_t)1 << 31)) return fail(LNT_ERR_INVALID_ARGUMENT, "count out of range");
+  LNT_HIP(hipSetDevice(ctx->device));
+  std::vector<InstDev> inst(count);
+  std::vector<RelinDev> ra(count), rb(count);
+  std::vector<uint64_t> a_off(count + 1), b_off(count + 1);
+  unsigned __int128 abound = 0, bbound = 0;
+  uint64_t na = 0, nb = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const lnt_contraction &c = items[i];
+    const std::string at = "instance " + std::to_string(i) + ": ";
+    Relin r1{}, r2{};
+    uint64_t da = 0, db = 0;
+    int rc = make_relin(c.order_a, c.dims_a, c.cur_seq_a, c.new_seq_a, &r1, &da);
+    if (rc != LNT_OK) return fail(rc, at + lnt_last_error());
+    rc = make_relin(c.order_b, c.dims_b, c.cur_seq_b, c.new_seq_b, &r2, &db);
+    if (rc != LNT_OK) return fail(rc, at + lnt_last_error());
+    for (int q = 0; q < 3; ++q)
+      if (c.lat_a[q] < 1 || c.lat_b[q] < 1) return fail(LNT_ERR_INVALID_ARGUMENT, at + "lattice dims must be >= 1");
+    if (c.lat_a[1] != c.lat_b[0] || c.lat_a[2] != c.lat_b[2])
+      return fail(LNT_ERR_DIMENSION_MISMATCH, at + "A's width and depth must equal B's height and depth");
+    unsigned __int128 pa = (unsigned __int128)c.lat_a[0] * c.lat_a[1] * c.lat_a[2];
+    unsigned __int128 pb = (unsigned __int128)c.lat_b[0] * c.lat_b[1] * c.lat_b[2];
+    unsigned __int128 pc = (unsigned __int128)c.lat_a[0] * c.lat_b[1] * c.lat_b[2];
+    if (pa != da || pb != db)
+      return fail(LNT_ERR_DIMENSION_MISMATCH, at + "lattice shape does not match the operand's dimensionality");
+    if (pc > ((unsigned __int128)1 << 63)) return fail(LNT_ERR_TOO_LARGE, at + "C's dimensionality exceeds 2^63");
+    if (c.nnz_a < 0 || c.nnz_b < 0 || (uint64_t)c.nnz_a > da || (uint64_t)c.nnz_b > db)
+      return fail(LNT_ERR_INVALID_ARGUMENT, at + "nnz must lie in [0, dimensionality]");
+    InstDev &I = inst[i];
+    I.a_off = na;
+    I.na = (uint64_t)c.nnz_a;
+    I.b_off = nb;
+    I.nb = (uint64_t)c.nnz_b;
+    I.a_base = (uint64_t)abound;
+    I.b_base = (uint64_t)bbound;
+    I.a_dim = da;
+    I.b_dim = db;
+    I.h = (uint64_t)c.lat_a[0];
+    I.w = (uint64_t)c.lat_a[1];
+    I.bw = (uint64_t)c.lat_b[1];
+    I.div_w = make_divider(I.w);
+    I.div_wbw = make_divider(I.w * I.bw);
+    a_off[i] = na;
+    b_off[i] = nb;
+    na += I.na;
+    nb += I.nb;
+    abound += da;
+    bbound += db;
+    if (abound >= ((unsigned __int128)1 << 63) || bbound >= ((unsigned __int128)1 << 63))
+      return fail(LNT_ERR_TOO_LARGE, "the instances' dimensionalities sum past 2^63");
+    if (na >= (uint64_t(1) << 32) || nb >= (uint64_t(1) << 32))
+      return fail(LNT_ERR_TOO_LARGE, "a batch holds fewer than 2^32 entries per operand");
+    ra[i] = r1.active ? relin_for_device(r1) : RelinDev{};
+    rb[i] = r2.active ? relin_for_device(r2) : RelinDev{};
+    if (!r1.active) ra[i].weight[0] = 1;  // order 0: the key times 1
+    if (!r2.active) rb[i].weight[0] = 1;
+  }
+  a_off[count] = na;
+  b_off[count] = nb;
+
+  lnt_batch b = new lnt_batch_s();
+  b->ctx = ctx;
+  b->count = (uint32_t)count;
+  b->na = na;
+  b->nb = nb;
+  b->a_bound = std::max<uint64_t>((uint64_t)abound, 1);
+  b->b_bound = std::max<uint64_t>((uint64_t)bbound, 1);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0)
+    b->cus = cus;
+  auto bail = [&](int rc) {
+    lnt_batch_destroy(b);
+    return rc;
+  };
+  const size_t meta = (((count * sizeof(InstDev) + 255) / 256) + 2 * ((count * sizeof(RelinDev) + 255) / 256) +
+                       2 * (((count + 1) * 8 + 255) / 256)) * 256 + 256;
+  int rc = ensure(b->meta, meta);
+  if (rc != LNT_OK) return bail(rc);
+  const size_t ctrl = (((kCtrWords * 8 + 255) / 256) + (((count + 1) * 8 + 255) / 256) + ((count * 8 + 255) / 256)) * 256 + 256;
+  rc = ensure(b->ctrl, ctrl);
+  if (rc != LNT_OK) return bail(rc);
+  if (hipHostMalloc((void **)&b->pinned, ctrl) != hipSuccess) {
+    (void)hipGetLastError();
+    b->pinned = nullptr;
+    return bail(fail(LNT_ERR_OUT_OF_MEMORY, "hipHostMalloc of the batch's control block failed"));
+  }
+  BatchPtrs P{};
+  carve(b, &P);
+  auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+    if (!bytes) return LNT_OK;
+    LNT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return LNT_OK;
+  };
+  if ((rc = up(P.inst, inst.data(), count * sizeof(InstDev))) != LNT_OK) return bail(rc);
+  if ((rc = up(P.relin_a, ra.data(), count * sizeof(RelinDev))) != LNT_OK) return bail(rc);
+  if ((rc = up(P.relin_b, rb.data(), count * sizeof(RelinDev))) != LNT_OK) return bail(rc);
+  if ((rc = up(P.a_off, a_off.data(), (count + 1) * 8)) != LNT_OK) return bail(rc);
+  if ((rc = up(P.b_off, b_off.data(), (count + 1) * 8)) != LNT_OK) return bail(rc);
+  *out = b;
+  return LNT_OK;
+}
+
+int lnt_batch_destroy(lnt_batch b) {
+  if (!b) return LNT_OK;
+  if (b->ctx) (void)hipSetDevice(b->ctx->device);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+  release(b->meta);
+  release(b->ws);
+  release(b->scan_tmp);
+  release(b->ctrl);
+  if (b->pinned) (void)hipHostFree(b->pinned);
+  delete b;
+  return LNT_OK;
+}
+
+int lnt_batch_symbolic(lnt_batch b, const int64_t *a_keys, const double *a_vals, const int64_t *b_keys,
+                       const double *b_vals, int64_t *nnz_c) {
+  if (!b || !nnz_c) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  if ((b->na && (!a_keys || !a_vals)) || (b->nb && (!b_keys || !b_vals)))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "null operand");
+  b->symbolic_ready = false;
+  LNT_HIP(hipSetDevice(b->ctx->device));
+  LNT_TRY(prepare(b));
+  BatchPtrs P{};
+  carve(b, &P);
+  LNT_TRY(enqueue_front(b, P, a_keys, a_vals, b_keys, b_vals, true));
+  LNT_TRY(read_back(b));
+  LNT_TRY(check_key_error(b));
+  *nnz_c = (int64_t)b->pinned[kCtrNc];
+  b->symbolic_ready = true;
+  return LNT_OK;
+}
+
+int lnt_batch_numeric(lnt_batch b, int64_t *c_keys, double *c_vals, int64_t capacity, int64_t *c_offsets) {
+  if (!b || capacity < 0 || (capacity > 0 && (!c_keys || !c_vals)))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "null argument or negative capacity");
+  if (!b->symbolic_ready) return fail(LNT_ERR_INVALID_ARGUMENT, "lnt_batch_numeric needs lnt_batch_symbolic first");
+  const uint64_t nc = b->pinned[kCtrNc];
+  if ((uint64_t)capacity < nc)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) + " is below nnz(C) = " +
+                                              std::to_string(nc));
+  b->symbolic_ready = false;
+  LNT_HIP(hipSetDevice(b->ctx->device));
+  BatchPtrs P{};
+  carve(b, &P);
+  batch_reset_tickets_kernel<<<1, 64, 0, b->ctx->stream>>>(P.ctr);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(enqueue_numeric(b, P, (uint64_t *)c_keys, c_vals, (uint64_t)capacity));
+  LNT_TRY(read_back(b));
+  if (c_offsets) {
+    const uint64_t *co = host_c_off(b);
+    for (uint32_t i = 0; i <= b->count; ++i) c_offsets[i] = (int64_t)co[i];
+  }
+  return LNT_OK;
+}
+
+int lnt_batch_contract(lnt_batch b, const int64_t *a_keys, const double *a_vals, const int64_t *b_keys,
+                       const double *b_vals, int64_t **c_keys, double **c_vals, int64_t *c_offsets) {
+  if (!b || !c_keys || !c_vals || !c_offsets) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  *c_keys = nullptr;
+  *c_vals = nullptr;
+  int64_t nc = 0;
+  LNT_TRY(lnt_batch_symbolic(b, a_keys, a_vals, b_keys, b_vals, &nc));
+  void *k = nullptr, *v = nullptr;
+  const size_t bytes = (nc ? (size_t)nc : 1) * 8;
+  if (hipMalloc(&k, bytes) != hipSuccess || hipMalloc(&v, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (k) (void)hipFree(k);
+    b->symbolic_ready = false;
+    return fail(LNT_ERR_OUT_OF_MEMORY, "hipMalloc of the batch result (" + std::to_string(nc) + " entries) failed");
+  }
+  int rc = lnt_batch_numeric(b, (int64_t *)k, (double *)v, nc, c_offsets);
+  if (rc != LNT_OK) {
+    (void)hipFree(k);
+    (void)hipFree(v);
+    return rc;
+  }
+  *c_keys = (int64_t *)k;
+  *c_vals = (double *)v;
+  return LNT_OK;
+}
+
+int lnt_batch_nprod(lnt_batch b, const int64_t *a_keys, const int64_t *b_keys, int64_t *products) {
+  if (!b || !products) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  if ((b->na && !a_keys) || (b->nb && !b_keys)) return fail(LNT_ERR_INVALID_ARGUMENT, "null operand");
+  b->symbolic_ready = false;
+  LNT_HIP(hipSetDevice(b->ctx->device));
+  if (b->nb == 0) {
+    for (uint32_t i = 0; i < b->count; ++i) products[i] = 0;
+    return LNT_OK;
+  }
+  LNT_TRY(prepare(b));
+  BatchPtrs P{};
+  carve(b, &P);
+  LNT_TRY(enqueue_front(b, P, a_keys, nullptr, b_keys, nullptr, false));
+  batch_offsets_kernel<<<grid_for(b->count + 1), kBlock, 0, b->ctx->stream>>>(
+      P.b_off, b->count, b->nb, P.seg_idx, P.prod_off, P.prod_off, ~0ull, P.ctr, P.c_off, P.nprod);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(read_back(b));
+  LNT_TRY(check_key_error(b));
+  const uint64_t *np = host_nprod(b);
+  for (uint32_t i = 0; i < b->count; ++i) products[i] = (int64_t)np[i];
+  return LNT_OK;
+}
+
+int lnt_batch_capture(lnt_batch b, const int64_t *a_keys, const double *a_vals, const int64_t *b_keys,
+                      const double *b_vals, int64_t *c_keys, double *c_vals, int64_t capacity, int64_t *nnz_c) {
+  if (!b || (capacity > 0 && (!c_keys || !c_vals)) || capacity < 0)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "null argument or negative capacity");
+  if ((b->na && (!a_keys || !a_vals)) || (b->nb && (!b_keys || !b_vals)))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "null operand");
+  lnt_context ctx = b->ctx;
+  b->symbolic_ready = false;
+  LNT_HIP(hipSetDevice(ctx->device));
+  LNT_TRY(prepare(b));
+  BatchPtrs P{};
+  carve(b, &P);
+  // One unrecorded run sizes every workspace (sort scratch, status words)
+  // and checks the operands.
+  LNT_TRY(enqueue_all(b, P, a_keys, a_vals, b_keys, b_vals, (uint64_t *)c_keys, c_vals, (uint64_t)capacity));
+  LNT_TRY(read_back(b));
+  LNT_TRY(check_key_error(b));
+  if (nnz_c) *nnz_c = (int64_t)b->pinned[kCtrNc];
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  // The legacy null stream cannot be captured: record on a stream of our own
+  // (blocking, default priority) when the context has none.
+  hipStream_t st = ctx->stream;
+  if (!st) {
+    if (!b->own_stream) LNT_HIP(hipStreamCreate(&b->own_stream));
+    st = b->own_stream;
+  }
+  int rc = LNT_OK;
+  {
+    StreamSwap swap(ctx, st);
+    LNT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    ctx->capturing = 1;
+    set_alloc_forbidden(true);
+    rc = enqueue_all(b, P, a_keys, a_vals, b_keys, b_vals, (uint64_t *)c_keys, c_vals, (uint64_t)capacity);
+    set_alloc_forbidden(false);
+    ctx->capturing = 0;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc == LNT_OK && e != hipSuccess) rc = fail(LNT_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (rc == LNT_OK) {
+      e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = fail(LNT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    if (g && rc == LNT_OK) b->graph = g;
+    else if (g) (void)hipGraphDestroy(g);
+  }
+  b->graph_capacity = capacity;
+  return rc;
+}
+
+int lnt_batch_launch(lnt_batch b) {
+  if (!b) return fail(LNT_ERR_INVALID_ARGUMENT, "null batch");
+  b->symbolic_ready = false;
+  if (!b->exec) return fail(LNT_ERR_INVALID_ARGUMENT, "no recorded graph: call lnt_batch_capture first");
+  LNT_HIP(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream ? b->ctx->stream : b->own_stream;
+  LNT_HIP(hipGraphLaunch(b->exec, st));
+  return LNT_OK;
+}
+
+int lnt_batch_replay(lnt_batch b, int64_t *c_offsets) {
+  LNT_TRY(lnt_batch_launch(b));
+  hipStream_t st = b->ctx->stream ? b->ctx->stream : b->own_stream;
+  LNT_HIP(hipMemcpyAsync(b->pinned, b->ctrl.ptr, b->ctrl.bytes, hipMemcpyDeviceToHost, st));
+  LNT_HIP(hipStreamSynchronize(st));
+  LNT_TRY(check_key_error(b));
+  if (b->pinned[kCtrOverflow])
+    return fail(LNT_ERR_TOO_LARGE, "the batch's result (" + std::to_string(b->pinned[kCtrNc]) +
+                                       " entries) exceeds the recorded capacity " + std::to_string(b->graph_capacity));
+  if (c_offsets) {
+    const uint64_t *co = host_c_off(b);
+    for (uint32_t i = 0; i <= b->count; ++i) c_offsets[i] = (int64_t)co[i];
+  }
+  return LNT_OK;
+}
+
+int lnt_batch_set_param(lnt_batch b, const char *name, int64_t value) {
+  if (!b || !name) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  std::string n(name);
+  auto in = [&](int64_t lo, int64_t hi) { return value >= lo && value <= hi; };
+  if (n == "item_products" && in(64, kCap / 2)) b->tuning.item_products = (uint32_t)value;
+  else if (n == "small_max" && in(2, kCap / 2)) b->tuning.small_max = (uint32_t)value;
+  else if (n == "window" && in(64, kCap / 2)) b->tuning.window = (uint32_t)value;
+  else if (n == "blocks_per_cu" && in(1, 8)) b->tuning.blocks_per_cu = (uint32_t)value;
+  else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown batch parameter or value out of range: " + n);
+  return LNT_OK;
+}
+
+int lnt_batch_get_param(lnt_batch b, const char *name, int64_t *value) {
+  if (!b || !name || !value) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  std::string n(name);
+  if (n == "item_products") *value = b->tuning.item_products;
+  else if (n == "small_max") *value = b->tuning.small_max;
+  else if (n == "window") *value = b->tuning.window;
+  else if (n == "blocks_per_cu") *value = b->tuning.blocks_per_cu;
+  else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown batch parameter: " + n);
+  return LNT_OK;
+}
+
+}  // extern "C"

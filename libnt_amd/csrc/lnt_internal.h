@@ -355,8 +355,11 @@ struct DeviceBuffer {
   size_t bytes = 0;
 };
 
-// Grows `b` to at least `bytes` (contents are not preserved).
+// Grows `b` to at least `bytes` (contents are not preserved).  While a
+// stream capture is being recorded on this thread, growing fails instead
+// (allocations are not stream work; workspaces are sized before capture).
 int ensure(DeviceBuffer &b, size_t bytes);
+void set_alloc_forbidden(bool forbidden);
 void release(DeviceBuffer &b);
 
 struct SortTuning {
@@ -365,6 +368,7 @@ struct SortTuning {
   int merge_items = 16;  // merge tile = 256 * items
   int spgemm_small = 2048;  // largest output column handled in LDS
   int narrow_keys = 1;  // 32-bit keys between passes when they fit
+  int digit_bits = 8;  // largest radix digit per onesweep pass (4..8)
 };
 
 // Kernel kinds recorded by the profiler (names in api.hip).
@@ -383,6 +387,7 @@ struct lnt_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
   int validate = 1;
+  int capturing = 0;  // launches are being recorded into a graph: no host syncs
   uint32_t epoch = 0;
   lnt::SortTuning tuning;
   lnt::DeviceBuffer alt_keys, alt_vals, alt2_keys, alt2_vals;

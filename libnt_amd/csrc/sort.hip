@@ -18,6 +18,8 @@
 //      writes it out so that consecutive lanes store consecutive addresses.
 //   Even P ping-pongs caller buffer -> scratch -> caller; odd P goes through two
 //   scratch buffers so the result still lands in the caller's arrays.
+#include <algorithm>
+
 #include "lnt_internal.h"
 
 namespace lnt {
@@ -324,6 +326,13 @@ int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
   uint32_t epoch;
   LNT_TRY(next_epoch(ctx, &epoch));
   uint64_t *status = (uint64_t *)ctx->status.ptr;
+  if (ctx->capturing) {
+    // A recorded graph replays this launch with the same epoch: clear the
+    // words the pass uses so that a replay never sees its predecessor's.
+    size_t words = (size_t)tiles * kRadix;
+    fill_u64_kernel<<<(uint32_t)std::min<size_t>((words + 255) / 256, 4096), 256, 0, ctx->stream>>>(status, 0, words);
+    LNT_HIP(hipGetLastError());
+  }
   // Roofline basis: the fixed model of 32 B per pair per pass (8 B key + 8 B
   // value, read and written), whatever key width this pass actually moves.
   int slot = prof_begin(ctx, kProfOnesweep, 32.0 * n);
@@ -411,10 +420,12 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   if (do_relin) LNT_TRY(upload_relin(ctx, r, &rdev));
   uint32_t bits = 0;
   while (bits < 64 && (uint64_t(1) << bits) < bound) ++bits;
-  // Digit plan: ceil(bits/8) passes (at least two so that no pass runs in
-  // place), digit widths balanced, least significant digit first.
+  // Digit plan: ceil(bits/digit_bits) passes (at least two so that no pass
+  // runs in place), digit widths balanced, least significant digit first.
   DigitPlan plan{};
-  int passes = (int)((bits + kRadixBits - 1) / kRadixBits);
+  const uint32_t dmax = (uint32_t)std::min(std::max(ctx->tuning.digit_bits, 4), kRadixBits);
+  int passes = (int)((bits + dmax - 1) / dmax);
+  if (passes > kMaxPasses) passes = kMaxPasses;  // 64-bit keys with narrow digits: widen
   if (passes == 1) passes = 2;
   if (n == 1 || bits == 0) passes = 0;
   plan.passes = passes;
@@ -450,7 +461,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
     LNT_HIP(hipGetLastError());
   }
 
-  if (ctx->validate) {
+  if (ctx->validate && !ctx->capturing) {
     LNT_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     LNT_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->pinned[0])

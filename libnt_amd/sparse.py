@@ -164,40 +164,13 @@ class Atom:
 
     # ----------------------------------------------------------- products
     def plan(self, other: "Atom"):
-        """Lattice bookkeeping of MIA_Expr.h:449-561: A rows = outer, columns =
-        inner (contracted), tabs = inter, each in A's mode order; B rows = inner
-        matched to A's order, columns = B's outer modes, tabs = inter matched.
-        C's modes are [A outer, B outer, inter]."""
-        la, lb = self.labels, other.labels
-        a, b = self.mia, other.mia
-        if len(set(la)) != len(la) or len(set(lb)) != len(lb):
-            raise ValueError("an index label repeats within one operand")
-        a_out, a_in, a_int, b_in, b_int = [], [], [], [], []
-        for m, c in enumerate(la):
-            if c in lb:
-                r = lb.index(c)
-                if a.dims[m] != b.dims[r]:
-                    raise ValueError("matched indices differ in size")
-                if c.isupper():
-                    a_int.append(m)
-                    b_int.append(r)
-                else:
-                    a_in.append(m)
-                    b_in.append(r)
-            else:
-                if c.isupper():
-                    raise ValueError("inter index %r appears in one operand only" % c)
-                a_out.append(m)
-        b_out = [r for r, c in enumerate(lb) if c not in la]
-        if any(lb[r].isupper() for r in b_out):
-            raise ValueError("inter index appears in one operand only")
-        seq_a = a_out + a_in + a_int
-        seq_b = b_in + b_out + b_int
-        lat_a = (_prod(a.dims[m] for m in a_out), _prod(a.dims[m] for m in a_in), _prod(a.dims[m] for m in a_int))
-        lat_b = (lat_a[1], _prod(b.dims[r] for r in b_out), lat_a[2])
-        dims_c = [a.dims[m] for m in a_out] + [b.dims[r] for r in b_out] + [a.dims[m] for m in a_int]
-        labels_c = "".join([la[m] for m in a_out] + [lb[r] for r in b_out] + [la[m] for m in a_int])
-        return seq_a, seq_b, lat_a, lat_b, dims_c, labels_c, (a_in, a_out, b_out)
+        """Lattice bookkeeping of MIA_Expr.h:449-561 (see ``batch.lattice_plan``):
+        A rows = outer, columns = inner (contracted), tabs = inter, each in A's
+        mode order; B rows = inner matched to A's order, columns = B's outer
+        modes, tabs = inter matched.  C's modes are [A outer, B outer, inter]."""
+        from .batch import lattice_plan
+
+        return lattice_plan(self.mia.dims, self.labels, other.mia.dims, other.labels)
 
     def __mul__(self, other: "Atom") -> "Atom":
         a, b = self.mia, other.mia

@@ -1,0 +1,354 @@
+"""GPU parity of batched contractions (include/lnt_batch.h) through the C-ABI:
+every instance against the CPU oracle's full contraction (keys bit-exact,
+values within 1e-12 under isEqualFuzzy, FunctionUtil.h:688-698) and against
+the single-contraction GPU path (bit-identical: both sum equal rows in
+ascending B-row order); the recorded hipGraph against the eager run; and the
+full-size mix through size-independent properties (per-instance sortedness,
+nnz bounds, the linearity checksum sum(C) = sum_k colsum_A(k) * rowsum_B(k))."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def _oracle_operands(orc, insts, i):
+    from libnt_amd import workloads as W
+
+    x = insts[i]
+    ak = orc.gen_distinct_keys(x.nnz_a, x.dimensionality_a, W.KEY_SEED_A + i)
+    av = orc.gen_uniform_vals(x.nnz_a, W.VAL_SEED_A + i)
+    bk = orc.gen_distinct_keys(x.nnz_b, x.dimensionality_b, W.KEY_SEED_B + i)
+    bv = orc.gen_uniform_vals(x.nnz_b, W.VAL_SEED_B + i)
+    return ak, av, bk, bv
+
+
+def _small_mix(count=120, seed=9):
+    from libnt_amd import workloads as W
+
+    return W.batched_contractions(count, seed=seed, nnz_lo=10, nnz_hi=3000, dim_lo=3, dim_hi=24)
+
+
+def _run(ctx, insts):
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch
+
+    ops = W.generate_operands(ctx, insts)
+    b = Batch(ctx, insts)
+    ck, cv, off = b.contract(*ops)
+    return b, ops, ck, cv, off
+
+
+def test_generated_operands_match_oracle(ctx, orc):
+    insts = _small_mix(40, seed=3)
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import packed_offsets
+
+    ak, av, bk, bv = W.generate_operands(ctx, insts)
+    a_off = packed_offsets([x.nnz_a for x in insts])
+    b_off = packed_offsets([x.nnz_b for x in insts])
+    for i in range(len(insts)):
+        eak, eav, ebk, ebv = _oracle_operands(orc, insts, i)
+        assert np.array_equal(host(ak[a_off[i]:a_off[i + 1]]), eak)
+        assert np.array_equal(host(av[a_off[i]:a_off[i + 1]]).view(np.int64), eav.view(np.int64))
+        assert np.array_equal(host(bk[b_off[i]:b_off[i + 1]]), ebk)
+        assert np.array_equal(host(bv[b_off[i]:b_off[i + 1]]).view(np.int64), ebv.view(np.int64))
+
+
+@pytest.mark.parametrize("seed", [9, 10])
+def test_batch_matches_oracle(ctx, orc, seed):
+    insts = _small_mix(120, seed=seed)
+    print("batch vs oracle, seed", seed, flush=True)
+    b, ops, ck, cv, off = _run(ctx, insts)
+    ck, cv = host(ck), host(cv)
+    for i, x in enumerate(insts):
+        ak, av, bk, bv = _oracle_operands(orc, insts, i)
+        ident_a, ident_b = list(range(len(x.dims_a))), list(range(len(x.dims_b)))
+        ek, ev, edims = orc.contract(ak, av, x.dims_a, ident_a, False, x.labels_a,
+                                     bk, bv, x.dims_b, ident_b, False, x.labels_b)
+        assert list(edims) == x.dims_c
+        got_k, got_v = ck[off[i]:off[i + 1]], cv[off[i]:off[i + 1]]
+        assert np.array_equal(got_k, ek), "instance %d keys" % i
+        assert orc.allclose_fuzzy(got_v, ev, TOL), "instance %d values" % i
+
+
+def test_batch_nprod_matches_oracle(ctx, orc):
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch
+
+    insts = _small_mix(60, seed=12)
+    ak, av, bk, bv = W.generate_operands(ctx, insts)
+    got = Batch(ctx, insts).nprod(ak, bk)
+    for i, x in enumerate(insts):
+        a, _, bb, _ = _oracle_operands(orc, insts, i)
+        ident_a, ident_b = list(range(len(x.dims_a))), list(range(len(x.dims_b)))
+        a2, _ = orc.sort_seq(a, np.zeros(len(a)), x.dims_a, ident_a, x.seq_a, False)
+        b2, _ = orc.sort_seq(bb, np.zeros(len(bb)), x.dims_b, ident_b, x.seq_b, False)
+        assert got[i] == orc.lattice_nprod(a2, x.lat_a, b2, x.lat_b), i
+
+
+def _single_path(ctx, x, ak, av, bk, bv):
+    """One instance through the single-contraction path (SparseMIA mirror)."""
+    from libnt_amd.sparse import SparseMIA
+
+    A = SparseMIA(x.dims_a, ak.clone(), av.clone(), ctx=ctx)
+    B = SparseMIA(x.dims_b, bk.clone(), bv.clone(), ctx=ctx)
+    C = (A(x.labels_a) * B(x.labels_b)).mia
+    assert list(C.dims) == x.dims_c
+    return C.keys, C.vals
+
+
+def test_batch_matches_single_contraction_path(ctx):
+    """A slice of the full-size mix (every size class of columns) equals the
+    single-contraction path bit for bit."""
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import packed_offsets
+
+    insts = W.batched_contractions(300, seed=4)
+    b, (ak, av, bk, bv), ck, cv, off = _run(ctx, insts)
+    a_off = packed_offsets([x.nnz_a for x in insts])
+    b_off = packed_offsets([x.nnz_b for x in insts])
+    for i, x in enumerate(insts):
+        sk, sv = _single_path(ctx, x, ak[a_off[i]:a_off[i + 1]], av[a_off[i]:a_off[i + 1]],
+                              bk[b_off[i]:b_off[i + 1]], bv[b_off[i]:b_off[i + 1]])
+        import torch
+
+        assert torch.equal(ck[off[i]:off[i + 1]], sk), "instance %d keys" % i
+        assert torch.equal(cv[off[i]:off[i + 1]].view(torch.int64), sv.view(torch.int64)), "instance %d values" % i
+
+
+def test_batch_tuning_does_not_change_results(ctx):
+    import torch
+
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch
+
+    insts = W.batched_contractions(120, seed=21)
+    ops = W.generate_operands(ctx, insts)
+    b = Batch(ctx, insts)
+    ref = b.contract(*ops)
+    for params in ({"small_max": 2, "item_products": 64, "window": 64, "blocks_per_cu": 1},
+                   {"small_max": 1024, "item_products": 1024, "window": 1024, "blocks_per_cu": 8},
+                   {"small_max": 300, "item_products": 200, "window": 100, "blocks_per_cu": 2}):
+        for k, v in params.items():
+            b.set_param(k, v)
+        ck, cv, off = b.contract(*ops)
+        assert np.array_equal(off, ref[2])
+        assert torch.equal(ck, ref[0]), params
+        # A window change moves sums of one row between the sort order and
+        # the tree order only for rows met by > 2048 products: none here.
+        assert torch.equal(cv.view(torch.int64), ref[1].view(torch.int64)), params
+
+
+def test_batch_graph_replay_matches_eager(ctx):
+    import torch
+
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch
+
+    insts = W.batched_contractions(200, seed=5)
+    ak, av, bk, bv = W.generate_operands(ctx, insts)
+    b = Batch(ctx, insts)
+    ek, ev, eoff = b.contract(ak, av, bk, bv)
+    n = b.capture(ak, av, bk, bv)
+    assert n == ek.shape[0]
+    for rep in range(3):
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        gk, gv, goff = b.replay()
+        assert np.array_equal(goff, eoff)
+        assert torch.equal(gk, ek) and torch.equal(gv.view(torch.int64), ev.view(torch.int64)), rep
+        del junk
+    # New values in the same buffers: the graph reads them on the next replay.
+    av.mul_(-2.0)
+    bv.add_(0.5)
+    ek2, ev2, _ = b.contract(ak, av, bk, bv)
+    gk, gv, _ = b.replay()
+    assert torch.equal(gk, ek2) and torch.equal(gv.view(torch.int64), ev2.view(torch.int64))
+
+
+def test_batch_graph_capacity_overflow(ctx):
+    import libnt_amd
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch
+
+    insts = W.batched_contractions(50, seed=6)
+    ak, av, bk, bv = W.generate_operands(ctx, insts)
+    b = Batch(ctx, insts)
+    n = b.capture(ak, av, bk, bv)
+    assert n > 10
+    b.capture(ak, av, bk, bv, capacity=n - 7)
+    with pytest.raises(libnt_amd.LntError) as e:
+        b.replay()
+    assert e.value.status == 6
+
+
+def test_batch_edge_cases(ctx, orc):
+    import torch
+
+    import libnt_amd
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch, Instance
+
+    insts = [
+        Instance([4, 5, 6], "ikj", [6, 7], "jl", 0, 10),      # empty A
+        Instance([4, 5, 6], "ikj", [6, 7], "jl", 12, 0),      # empty B
+        Instance([1, 3], "ij", [3, 1], "jk", 2, 2),           # h = 1, bw = 1
+        Instance([5, 4, 3], "iJk", [3, 4, 6], "kJm", 30, 40),  # inter index: 4 tabs
+        Instance([9, 8, 7], "abc", [7, 8, 9], "cbd", 200, 150),  # contracted modes in opposite orders
+        Instance([6, 6, 6, 6], "abcd", [6, 6, 6, 6], "cdef", 500, 600),
+    ]
+    b, ops, ck, cv, off = _run(ctx, insts)
+    ck, cv = host(ck), host(cv)
+    for i, x in enumerate(insts):
+        ak, av, bk, bv = _oracle_operands(orc, insts, i)
+        ident_a, ident_b = list(range(len(x.dims_a))), list(range(len(x.dims_b)))
+        ek, ev, _ = orc.contract(ak, av, x.dims_a, ident_a, False, x.labels_a,
+                                 bk, bv, x.dims_b, ident_b, False, x.labels_b)
+        assert np.array_equal(ck[off[i]:off[i + 1]], ek), i
+        assert orc.allclose_fuzzy(cv[off[i]:off[i + 1]], ev, TOL), i
+    # empty batch
+    e = torch.empty(0, dtype=torch.int64, device="cuda")
+    ev = torch.empty(0, dtype=torch.float64, device="cuda")
+    k, v, o = Batch(ctx, []).contract(e, ev, e, ev)
+    assert k.shape[0] == 0 and list(o) == [0]
+    # a key outside its instance's dimensionality
+    ak, av, bk, bv = ops
+    bad = ak.clone()
+    bad[3] = insts[0].dimensionality_a * 1000
+    with pytest.raises(libnt_amd.LntError) as err:
+        b.contract(bad, av, bk, bv)
+    assert err.value.status == 5
+
+
+def test_batch_invalid_descriptions(ctx):
+    import libnt_amd
+    from libnt_amd.batch import Batch, Instance
+
+    x = Instance([4, 5], "ij", [5, 6], "jk", 3, 3)
+    x.lat_a = (4, 5, 2)  # inconsistent lattice shape
+    with pytest.raises(libnt_amd.LntError) as e:
+        Batch(ctx, [x])
+    assert e.value.status == 7
+    y = Instance([4, 5], "ij", [5, 6], "jk", 30, 3)  # nnz above the dimensionality
+    with pytest.raises(libnt_amd.LntError) as e:
+        Batch(ctx, [y])
+    assert e.value.status == 1
+
+
+def _linearity_checksums(insts, ak, av, bk, bv, a_off, b_off):
+    """Per instance: sum_k colsum_A(k) * rowsum_B(k) and the bound
+    sum_k colabs_A(k) * rowabs_B(k), computed from the operands alone."""
+    import torch
+
+    want = np.zeros(len(insts))
+    bound = np.zeros(len(insts))
+    for i, x in enumerate(insts):
+        w = x.lat_a[1] * x.lat_a[2]
+
+        def index_of(keys, dims, labels, contracted):
+            # (contracted index, tab) of every key in default order, as one integer
+            idx = torch.zeros_like(keys)
+            stride, place = 1, {}
+            for m, d in enumerate(dims):
+                place[labels[m]] = (stride, d)
+                stride *= d
+            mult = 1
+            for c in contracted:
+                s, d = place[c]
+                idx += ((keys // s) % d) * mult
+                mult *= d
+            return idx
+
+        inner = [c for c in x.labels_a if c in x.labels_b and c.islower()]
+        inter = [c for c in x.labels_a if c in x.labels_b and c.isupper()]
+        ka = ak[a_off[i]:a_off[i + 1]]
+        kb = bk[b_off[i]:b_off[i + 1]]
+        ia = index_of(ka, x.dims_a, x.labels_a, inner + inter)
+        ib = index_of(kb, x.dims_b, x.labels_b, inner + inter)
+        ca = torch.zeros(w, dtype=torch.float64, device="cuda").index_add_(0, ia, av[a_off[i]:a_off[i + 1]])
+        cb = torch.zeros(w, dtype=torch.float64, device="cuda").index_add_(0, ib, bv[b_off[i]:b_off[i + 1]])
+        aa = torch.zeros(w, dtype=torch.float64, device="cuda").index_add_(0, ia, av[a_off[i]:a_off[i + 1]].abs())
+        ab = torch.zeros(w, dtype=torch.float64, device="cuda").index_add_(0, ib, bv[b_off[i]:b_off[i + 1]].abs())
+        want[i] = float((ca * cb).sum())
+        bound[i] = float((aa * ab).sum())
+    return want, bound
+
+
+def test_c4_full_size_batch(ctx):
+    """The full batched mix: 10,000 instances, order 3-4, 1-2 contracted
+    modes, dims log-U[8,128], nnz log-U[100,50k].  Checked by the recorded
+    graph against the eager run, per-instance sortedness and bounds, and the
+    linearity checksum of every instance."""
+    import torch
+
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch, packed_offsets
+
+    insts = W.batched_contractions(10000, seed=4)
+    ak, av, bk, bv = W.generate_operands(ctx, insts)
+    a_off = packed_offsets([x.nnz_a for x in insts])
+    b_off = packed_offsets([x.nnz_b for x in insts])
+    b = Batch(ctx, insts)
+    nprod = b.nprod(ak, bk)
+    print("c4: %d instances, %d + %d entries, %.3g products" % (len(insts), a_off[-1], b_off[-1], nprod.sum()),
+          flush=True)
+    ck, cv, off = b.contract(ak, av, bk, bv)
+    nnz = np.diff(off)
+    print("c4: nnz(C) = %d" % off[-1], flush=True)
+    assert (nnz <= nprod).all()
+    assert ((nnz > 0) == (nprod > 0)).all()
+    # keys strictly ascending inside every instance (a step down is allowed
+    # only where a new instance starts), first key >= 0, last key below C's
+    # dimensionality
+    starts = off[1:-1][(off[1:-1] > 0) & (off[1:-1] < off[-1])]
+    down = []
+    chunk = 1 << 28
+    for lo in range(0, max(ck.shape[0] - 1, 0), chunk):
+        seg = ck[lo:lo + chunk + 1]
+        down.append(torch.nonzero(seg[1:] <= seg[:-1]).flatten().cpu().numpy() + lo + 1)
+    down = np.concatenate(down) if down else np.zeros(0, dtype=np.int64)
+    assert set(down.tolist()) <= set(starts.tolist())
+    nz = np.nonzero(nnz)[0]
+    first = ck[torch.from_numpy(off[nz]).cuda()].cpu().numpy()
+    last = ck[torch.from_numpy(off[nz + 1] - 1).cuda()].cpu().numpy()
+    dimc = np.array([math.prod(insts[i].dims_c) for i in nz], dtype=object)
+    assert (first >= 0).all() and all(int(l) < int(d) for l, d in zip(last, dimc))
+    # per-instance sums against the linearity checksum
+    fp_eager = _fingerprints(ck, cv, off)
+    got = fp_eager[2]
+    want, bound = _linearity_checksums(insts, ak, av, bk, bv, a_off, b_off)
+    assert np.all(np.abs(got - want) <= 1e-10 * bound + 1e-300)
+    # the recorded graph reproduces the eager result bit for bit (compared by
+    # per-instance fingerprints: the eager result is freed first)
+    del ck, cv
+    torch.cuda.empty_cache()
+    n = b.capture(ak, av, bk, bv)
+    assert n == off[-1]
+    gk, gv, goff = b.replay()
+    assert np.array_equal(goff, off)
+    fp_graph = _fingerprints(gk, gv, goff)
+    assert np.array_equal(fp_eager[0], fp_graph[0])  # key words
+    assert np.array_equal(fp_eager[1], fp_graph[1])  # value bit patterns
+
+
+def _fingerprints(ck, cv, off):
+    """Per instance: wrapping sums of the key words and of the value bit
+    patterns, and the floating-point sum of the values."""
+    import torch
+
+    ks, vs, fs = [], [], []
+    vb = cv.view(torch.int64)
+    for i in range(len(off) - 1):
+        a, b = int(off[i]), int(off[i + 1])
+        ks.append(ck[a:b].sum())
+        vs.append(vb[a:b].sum())
+        fs.append(cv[a:b].sum())
+    return (torch.stack(ks).cpu().numpy(), torch.stack(vs).cpu().numpy(), torch.stack(fs).cpu().numpy())
