@@ -117,6 +117,24 @@ __device__ inline uint64_t block_min_u64(uint64_t v, uint64_t *scratch) {
   return m;
 }
 
+__device__ inline uint32_t bit_width(uint64_t x) { return x ? 64u - (uint32_t)__clzll(x) : 0u; }
+
+__device__ inline uint64_t block_max_u64(uint64_t v, uint64_t *scratch) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t y = __shfl_xor(v, off, kWave);
+    v = y > v ? y : v;
+  }
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  uint64_t m = scratch[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / kWave; ++w) m = scratch[w] > m ? scratch[w] : m;
+  __syncthreads();
+  return m;
+}
+
 // Tree sum of one double per thread (deterministic for a fixed assignment).
 __device__ inline double block_sum_f64(double v, double *scratch) {
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -151,6 +169,76 @@ __device__ inline void bitonic_key_pos(uint64_t *s_key, uint16_t *s_pos, uint32_
       __syncthreads();
     }
   }
+}
+
+// Stable LSD radix sort (8-bit digits) of (key, pos) pairs in LDS by bits
+// [0, lo_bits) and [hi_shift, hi_shift + hi_bits) of the key (all other bits
+// zero); n <= kCap; the result is left in (k0, p0) when the pass count is
+// even, else in (k1, p1) (returned: 0 or 1).  Element arrangement:
+// wave w owns elements [w*512, w*512+512), lane l of slot u holds
+// w*512 + u*64 + l, so ranking slot by slot inside each wave and waves in
+// order keeps equal keys in element order.  Peers with the same digit are
+// found by ballot matching; per-wave digit counters live in s_cnt[4][256].
+__device__ inline int lds_radix_sort(uint64_t *k0, uint16_t *p0, uint64_t *k1, uint16_t *p1, uint32_t n,
+                                     uint32_t lo_bits, uint32_t hi_shift, uint32_t hi_bits, uint32_t *s_cnt,
+                                     uint32_t *s_scan) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t first_idx = (int32_t)(wave * 512);
+  const int slots = n > (uint32_t)first_idx ? (int)min(8u, (n - first_idx + 63) / 64) : 0;
+  int buf = 0;
+  const uint32_t lo_passes = (lo_bits + 7) / 8, passes = lo_passes + (hi_bits + 7) / 8;
+  for (uint32_t pass = 0; pass < passes; ++pass) {
+    const uint32_t shift = pass < lo_passes ? 8 * pass : hi_shift + 8 * (pass - lo_passes);
+    uint64_t *kin = buf ? k1 : k0, *kout = buf ? k0 : k1;
+    uint16_t *pin = buf ? p1 : p0, *pout = buf ? p0 : p1;
+    for (uint32_t i = tid; i < 4 * 256; i += kBlock) s_cnt[i] = 0;
+    __syncthreads();
+    uint64_t key[8];
+    uint16_t pos[8];
+    uint32_t rank[8], dig[8];
+    const uint64_t lt = (1ull << lane) - 1;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (u >= slots) break;
+      const uint32_t idx = first_idx + u * 64 + lane;
+      key[u] = idx < n ? kin[idx] : ~0ull;
+      pos[u] = idx < n ? pin[idx] : 0;
+      const uint32_t d = (uint32_t)(key[u] >> shift) & 255u;
+      uint64_t match = ~0ull;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const uint64_t bal = __ballot((d >> b) & 1u);
+        match &= ((d >> b) & 1u) ? bal : ~bal;
+      }
+      const uint32_t r = (uint32_t)__popcll(match & lt);
+      const uint32_t base = s_cnt[wave * 256 + d];
+      rank[u] = base + r;
+      dig[u] = d;
+      if (r == 0) s_cnt[wave * 256 + d] = base + (uint32_t)__popcll(match);
+    }
+    __syncthreads();
+    {
+      const uint32_t d = tid;  // 256 threads <-> 256 digits
+      const uint32_t c0 = s_cnt[d], c1 = s_cnt[256 + d], c2 = s_cnt[512 + d], c3 = s_cnt[768 + d];
+      uint32_t tot;
+      const uint32_t base = block256_exclusive_sum(c0 + c1 + c2 + c3, s_scan, &tot);
+      s_cnt[d] = base;
+      s_cnt[256 + d] = base + c0;
+      s_cnt[512 + d] = base + c0 + c1;
+      s_cnt[768 + d] = base + c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (u >= slots) break;
+      const uint32_t dst = s_cnt[wave * 256 + dig[u]] + rank[u];
+      kout[dst] = key[u];
+      pout[dst] = pos[u];
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+  return buf;
 }
 
 // Instance of element p, given that this block's elements lie in instances
@@ -318,20 +406,58 @@ struct ColumnArgs {
   uint64_t capacity;
 };
 
+// LDS hash set of 64-bit keys: the symbolic pass counts distinct keys
+// without sorting (at most kCap keys in a table of 2*kCap slots).
+constexpr uint32_t kHashSlots = 2 * kCap;
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+__device__ inline bool hash_insert(unsigned long long *table, uint64_t key) {
+  uint32_t slot = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 52) & (kHashSlots - 1);
+  for (;;) {
+    const unsigned long long prev = atomicCAS(&table[slot], kEmptyKey, (unsigned long long)key);
+    if (prev == kEmptyKey) return true;
+    if (prev == key) return false;
+    slot = (slot + 1) & (kHashSlots - 1);
+  }
+}
+
+// Position inside one B entry's run of products: entry e, the A index of
+// product g of the flat product numbering is alo + g.
+struct EntryCursor {
+  uint64_t e, end, alo, abase;
+  double b;
+};
+
+template <bool NUMERIC>
+__device__ inline void load_entry(const ColumnArgs &A, uint64_t e, EntryCursor &c) {
+  c.e = e;
+  c.end = A.prod_off[e + 1];
+  c.alo = A.a_lo[e] - A.prod_off[e];
+  c.abase = A.a_base[e];
+  c.b = NUMERIC ? A.bv[e] : 0.0;
+}
+
 // ------------------------------------------------ 4/5. small segments, items
 // Work item t = the small segments whose compact product offset lies in
-// [t*G, (t+1)*G): at most G segments and G + small_max - 1 products.  The
-// products of all of them are sorted together by (segment, row, position).
+// [t*G, (t+1)*G): at most G segments and G + small_max - 1 products.  Each
+// thread expands kPer consecutive products (one search for its first entry,
+// then a walk).  Symbolic: distinct (segment, row) keys counted in an LDS
+// hash set.  Numeric: the products are sorted together by (segment, row,
+// position) and equal runs summed in position order.
 template <bool NUMERIC>
 __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const uint32_t *__restrict__ small_list,
                                                              const uint64_t *__restrict__ sp_off, uint32_t G,
                                                              uint64_t *__restrict__ ctr) {
-  __shared__ uint64_t s_key[kCap];
-  __shared__ uint16_t s_pos[kCap];
+  __shared__ unsigned long long s_key[NUMERIC ? kCap : kHashSlots];
+  __shared__ uint16_t s_pos[NUMERIC ? kCap : 1];
+  __shared__ uint64_t s_key2[NUMERIC ? kCap : 1];
+  __shared__ uint16_t s_pos2[NUMERIC ? kCap : 1];
+  __shared__ uint32_t s_rcnt[NUMERIC ? 4 * 256 : 1];
+  __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ double s_val[NUMERIC ? kCap : 1];
   __shared__ uint32_t s_seg[kMaxSegsPerItem];
   __shared__ uint32_t s_spo[kMaxSegsPerItem + 1];
-  __shared__ uint16_t s_rstart[kMaxSegsPerItem];
+  __shared__ uint32_t s_cnt[kMaxSegsPerItem];  // symbolic: distinct rows; numeric: rank base
   __shared__ uint32_t s_scan[kBlock / kWave];
   __shared__ uint64_t s_item, s_rb, s_re;
   const uint32_t tid = threadIdx.x;
@@ -355,36 +481,74 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
     const uint32_t nr = (uint32_t)(re - rb);
     const uint64_t pbase = rb < n_small ? sp_off[rb] : total;
     for (uint32_t q = tid; q <= nr; q += kBlock) {
-      if (q < nr) s_seg[q] = small_list[rb + q];
+      if (q < nr) {
+        s_seg[q] = small_list[rb + q];
+        s_cnt[q] = 0;
+      }
       s_spo[q] = (uint32_t)((rb + q < n_small ? sp_off[rb + q] : total) - pbase);
     }
+    if (!NUMERIC)
+      for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_key[i] = kEmptyKey;
     __syncthreads();
     const uint32_t np = s_spo[nr];
-    uint32_t P = 2;
-    while (P < np) P <<= 1;
-    for (uint32_t p = tid; p < P; p += kBlock) {
-      uint64_t key = ~0ull;
-      if (p < np) {
-        const uint32_t q = (uint32_t)upper_index(s_spo, 0, nr, p);
-        const uint64_t s = s_seg[q];
-        const uint64_t first = A.seg_first[s], last = seg_last(A.seg_first, s, nseg, A.nb);
-        uint64_t e, ai;
-        locate(A.prod_off, first, last, p - s_spo[q], A.a_lo, &e, &ai);
-        key = ((uint64_t)q << kRowBits) | (A.ak[ai] - A.a_base[e]);
-        if (NUMERIC) s_val[p] = A.bv[e] * A.av[ai];
+    const uint32_t p0 = tid * kPer;
+    uint64_t maxrow = 0;
+    if (p0 < np) {
+      uint32_t q = (uint32_t)upper_index(s_spo, 0, nr, p0);
+      uint64_t s = s_seg[q];
+      uint64_t first = A.seg_first[s], last = seg_last(A.seg_first, s, nseg, A.nb);
+      uint64_t sbase = A.prod_off[first];
+      uint32_t seg_end = s_spo[q + 1];
+      uint64_t g = sbase + (p0 - s_spo[q]);
+      EntryCursor c;
+      load_entry<NUMERIC>(A, upper_index(A.prod_off, first, last, g), c);
+      for (int u = 0; u < kPer; ++u) {
+        const uint32_t p = p0 + u;
+        if (p >= np) break;
+        if (p >= seg_end) {  // next small segment of the item (each has >= 1 product)
+          ++q;
+          s = s_seg[q];
+          first = A.seg_first[s];
+          last = seg_last(A.seg_first, s, nseg, A.nb);
+          sbase = A.prod_off[first];
+          seg_end = s_spo[q + 1];
+          g = sbase + (p - s_spo[q]);
+          load_entry<NUMERIC>(A, upper_index(A.prod_off, first, last, g), c);
+        } else if (g >= c.end) {  // next entry with products
+          load_entry<NUMERIC>(A, upper_index(A.prod_off, c.e + 1, last, g), c);
+        }
+        const uint64_t ai = c.alo + g;
+        const uint64_t row = A.ak[ai] - c.abase;
+        const uint64_t key = ((uint64_t)q << kRowBits) | row;
+        maxrow = row > maxrow ? row : maxrow;
+        if (NUMERIC) {
+          s_key[p] = key;
+          s_pos[p] = (uint16_t)p;
+          s_val[p] = c.b * A.av[ai];
+        } else if (hash_insert(s_key, key)) {
+          atomicAdd(&s_cnt[q], 1u);
+        }
+        ++g;
       }
-      s_key[p] = key;
-      s_pos[p] = (uint16_t)p;
     }
-    __syncthreads();
-    bitonic_key_pos(s_key, s_pos, P);
+    if (!NUMERIC) {
+      __syncthreads();
+      for (uint32_t q = tid; q < nr; q += kBlock) A.seg_u[s_seg[q]] = s_cnt[q];
+      __syncthreads();
+      continue;
+    }
+    const uint32_t rowbits = bit_width(block_max_u64(maxrow, s_red));
+    const int which = lds_radix_sort((uint64_t *)s_key, s_pos, s_key2, s_pos2, np, rowbits, kRowBits,
+                                     nr > 1 ? bit_width(nr - 1) : 0, s_rcnt, s_scan);
+    const uint64_t *kk = which ? s_key2 : (const uint64_t *)s_key;
+    const uint16_t *pp = which ? s_pos2 : s_pos;
     // Heads of equal-key runs, ranked; the first element of every segment
     // records its rank so that ranks can be made segment-relative.
     uint32_t flags = 0, cnt = 0;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const uint32_t p = tid * kPer + u;
-      const bool h = p < np && (p == 0 || s_key[p] != s_key[p - 1]);
+      const bool h = p < np && (p == 0 || kk[p] != kk[p - 1]);
       flags |= (h ? 1u : 0u) << u;
       cnt += h;
     }
@@ -394,52 +558,62 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
     for (int u = 0; u < kPer; ++u) {
       const uint32_t p = tid * kPer + u;
       if ((flags >> u) & 1u) {
-        if (p == 0 || (s_key[p] >> kRowBits) != (s_key[p - 1] >> kRowBits))
-          s_rstart[s_key[p] >> kRowBits] = (uint16_t)rank;
+        if (p == 0 || (kk[p] >> kRowBits) != (kk[p - 1] >> kRowBits))
+          s_cnt[kk[p] >> kRowBits] = rank;
         ++rank;
       }
     }
     __syncthreads();
-    if (!NUMERIC) {
-      for (uint32_t q = tid; q < nr; q += kBlock) {
-        const uint32_t end = q + 1 < nr ? s_rstart[q + 1] : heads;
-        A.seg_u[s_seg[q]] = end - s_rstart[q];
+    rank -= cnt;
+    for (int u = 0; u < kPer; ++u) {
+      if (!((flags >> u) & 1u)) continue;
+      const uint32_t p = tid * kPer + u;
+      const uint64_t key = kk[p];
+      const uint32_t q = (uint32_t)(key >> kRowBits);
+      double sum = s_val[pp[p]];
+      for (uint32_t r = p + 1; r < np && kk[r] == key; ++r) sum += s_val[pp[r]];
+      const uint64_t s = s_seg[q];
+      const uint64_t dst = A.out_off[s] + (rank - s_cnt[q]);
+      if (dst < A.capacity) {
+        A.ck[dst] = (key & kRowMask) + A.colkey[A.seg_first[s]];
+        A.cv[dst] = sum;
       }
-    } else {
-      rank -= cnt;
-      for (int u = 0; u < kPer; ++u) {
-        if (!((flags >> u) & 1u)) continue;
-        const uint32_t p = tid * kPer + u;
-        const uint64_t key = s_key[p];
-        const uint32_t q = (uint32_t)(key >> kRowBits);
-        double sum = s_val[s_pos[p]];
-        for (uint32_t r = p + 1; r < np && s_key[r] == key; ++r) sum += s_val[s_pos[r]];
-        const uint64_t s = s_seg[q];
-        const uint64_t dst = A.out_off[s] + (rank - s_rstart[q]);
-        if (dst < A.capacity) {
-          A.ck[dst] = (key & kRowMask) + A.colkey[A.seg_first[s]];
-          A.cv[dst] = sum;
-        }
-        ++rank;
-      }
+      ++rank;
     }
     __syncthreads();
   }
 }
 
+// First index x in [c, end) with ak[x] >= target (ak ascending), probing
+// c, c+1, c+3, c+7, ... before a binary search: the answer is usually a few
+// entries past the cursor.
+__device__ inline uint64_t gallop(const uint64_t *ak, uint64_t c, uint64_t end, uint64_t target) {
+  uint64_t lo = c, hi = c, step = 1;
+  while (hi < end && ak[hi] < target) {
+    lo = hi + 1;
+    hi = hi + step < end ? hi + step : end;
+    step <<= 1;
+  }
+  return lower_bound_u64(ak, lo, hi, target);
+}
+
 // ----------------------------------------------- 4/5. large segments, windows
 // One segment per work item.  Rows are taken in windows [r0, r1) holding at
-// most kCap products (about `window` aimed for), found by binary searches of
-// every entry's A column from its cursor; empty stretches are skipped by
-// jumping to the smallest next row.  Each window is sorted by (row, position)
-// and its runs summed, so rows are summed in ascending B-row order.
+// most kCap products (about `window` aimed for); every entry's share of a
+// window is found by galloping from its cursor; empty stretches are skipped
+// by jumping to the smallest next row.  Symbolic: distinct rows counted in an
+// LDS hash set.  Numeric: each window is sorted by (row, position) and its
+// runs summed, so rows are summed in ascending B-row order.
 template <bool NUMERIC>
 __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const uint32_t *__restrict__ list3,
                                                              uint64_t *__restrict__ cur, uint32_t *__restrict__ wcnt,
                                                              uint32_t *__restrict__ woff, uint32_t window,
                                                              uint64_t *__restrict__ ctr) {
-  __shared__ uint64_t s_key[kCap];
-  __shared__ uint16_t s_pos[kCap];
+  __shared__ unsigned long long s_key[NUMERIC ? kCap : kHashSlots];
+  __shared__ uint16_t s_pos[NUMERIC ? kCap : 1];
+  __shared__ uint64_t s_key2[NUMERIC ? kCap : 1];
+  __shared__ uint16_t s_pos2[NUMERIC ? kCap : 1];
+  __shared__ uint32_t s_rcnt[NUMERIC ? 4 * 256 : 1];
   __shared__ double s_val[NUMERIC ? kCap : 1];
   __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ double s_fred[kBlock / kWave];
@@ -476,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
         uint64_t local = 0;
         for (uint64_t e = first + tid; e < last; e += kBlock) {
           const uint64_t c = cur[e], end = A.a_lo[e] + A.a_len[e];
-          const uint64_t x = lower_bound_u64(A.ak, c, end, A.a_base[e] + r1);
+          const uint64_t x = gallop(A.ak, c, end, A.a_base[e] + r1);
           wcnt[e] = (uint32_t)(x - c);
           local += x - c;
         }
@@ -502,8 +676,9 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
         // One row (r1 = r0 + 1) met by more than kCap products, each from a
         // different A column: a tree sum over the entries.
         double part = 0.0;
-        for (uint64_t e = first + tid; e < last; e += kBlock)
-          if (wcnt[e]) part += A.bv[e] * A.av[cur[e]];
+        if (NUMERIC)
+          for (uint64_t e = first + tid; e < last; e += kBlock)
+            if (wcnt[e]) part += A.bv[e] * A.av[cur[e]];
         const double sum = block_sum_f64(part, s_fred);
         if (NUMERIC && tid == 0 && obase + written < A.capacity) {
           A.ck[obase + written] = r0 + ckey;
@@ -526,43 +701,67 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
         if (e < last) woff[e] = running + ex;
         running += tot;
       }
+      if (!NUMERIC)
+        for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_key[i] = kEmptyKey;
       __syncthreads();
       const uint32_t n = (uint32_t)cnt;
-      uint32_t P = 2;
-      while (P < n) P <<= 1;
-      for (uint32_t p = tid; p < P; p += kBlock) {
-        uint64_t key = ~0ull;
-        if (p < n) {
-          const uint64_t e = upper_index(woff, first, last, p);
-          const uint64_t x = cur[e] + (p - woff[e]);
-          key = A.ak[x] - A.a_base[e];
-          if (NUMERIC) s_val[p] = A.bv[e] * A.av[x];
+      uint32_t fresh = 0;
+      uint64_t maxrel = 0;
+      const uint32_t p0 = tid * kPer;
+      if (p0 < n) {
+        uint64_t e = upper_index(woff, first, last, p0);
+        uint64_t wend = e + 1 < last ? woff[e + 1] : n;
+        uint64_t c = cur[e] - woff[e], ab = A.a_base[e];
+        double bval = NUMERIC ? A.bv[e] : 0.0;
+        for (int u = 0; u < kPer; ++u) {
+          const uint32_t p = p0 + u;
+          if (p >= n) break;
+          if (p >= wend) {
+            e = upper_index(woff, e + 1, last, p);
+            wend = e + 1 < last ? woff[e + 1] : n;
+            c = cur[e] - woff[e];
+            ab = A.a_base[e];
+            if (NUMERIC) bval = A.bv[e];
+          }
+          const uint64_t x = c + p;
+          const uint64_t row = A.ak[x] - ab;
+          if (NUMERIC) {
+            const uint64_t rel = row - r0;  // rows of the window lie in [r0, r1)
+            maxrel = rel > maxrel ? rel : maxrel;
+            s_key[p] = rel;
+            s_pos[p] = (uint16_t)p;
+            s_val[p] = bval * A.av[x];
+          } else {
+            fresh += hash_insert(s_key, row);
+          }
         }
-        s_key[p] = key;
-        s_pos[p] = (uint16_t)p;
-      }
-      __syncthreads();
-      bitonic_key_pos(s_key, s_pos, P);
-      uint32_t flags = 0, hc = 0;
-#pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const uint32_t p = tid * kPer + u;
-        const bool hd = p < n && (p == 0 || s_key[p] != s_key[p - 1]);
-        flags |= (hd ? 1u : 0u) << u;
-        hc += hd;
       }
       uint32_t heads;
-      uint32_t rank = block256_exclusive_sum(hc, s_scan, &heads);
-      if (NUMERIC) {
+      if (!NUMERIC) {
+        block256_exclusive_sum(fresh, s_scan, &heads);
+      } else {
+        const uint32_t relbits = bit_width(block_max_u64(maxrel, s_red));
+        const int which = lds_radix_sort((uint64_t *)s_key, s_pos, s_key2, s_pos2, n, relbits, 0, 0, s_rcnt, s_scan);
+        const uint64_t *kk = which ? s_key2 : (const uint64_t *)s_key;
+        const uint16_t *pp = which ? s_pos2 : s_pos;
+        uint32_t flags = 0, hc = 0;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+          const uint32_t p = tid * kPer + u;
+          const bool hd = p < n && (p == 0 || kk[p] != kk[p - 1]);
+          flags |= (hd ? 1u : 0u) << u;
+          hc += hd;
+        }
+        uint32_t rank = block256_exclusive_sum(hc, s_scan, &heads);
         for (int u = 0; u < kPer; ++u) {
           if (!((flags >> u) & 1u)) continue;
           const uint32_t p = tid * kPer + u;
-          const uint64_t key = s_key[p];
-          double sum = s_val[s_pos[p]];
-          for (uint32_t r = p + 1; r < n && s_key[r] == key; ++r) sum += s_val[s_pos[r]];
+          const uint64_t key = kk[p];
+          double sum = s_val[pp[p]];
+          for (uint32_t r = p + 1; r < n && kk[r] == key; ++r) sum += s_val[pp[r]];
           const uint64_t dst = obase + written + rank;
           if (dst < A.capacity) {
-            A.ck[dst] = key + ckey;
+            A.ck[dst] = r0 + key + ckey;
             A.cv[dst] = sum;
           }
           ++rank;
@@ -647,7 +846,7 @@ struct Carve {
 struct BatchTuning {
   uint32_t item_products = 1024;
   uint32_t small_max = 1024;
-  uint32_t window = 1024;
+  uint32_t window = 1280;
   uint32_t blocks_per_cu = 3;
 };
 
@@ -771,7 +970,7 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
                P.a_len, P.einst, P.inst, nb, P.seg_u, nullptr, nullptr, 0};
   const uint32_t grid = (uint32_t)b->cus * b->tuning.blocks_per_cu;
   const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
-  const uint32_t window = std::min<uint32_t>(b->tuning.window, kCap / 2);
+  const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
   batch_small_kernel<false><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, P.ctr);
   batch_large_kernel<false><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
   LNT_HIP(hipGetLastError());
@@ -787,7 +986,7 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
                  P.a_len, P.einst, P.inst, nb, P.seg_u, ck, cv, capacity};
     const uint32_t grid = (uint32_t)b->cus * b->tuning.blocks_per_cu;
     const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
-    const uint32_t window = std::min<uint32_t>(b->tuning.window, kCap / 2);
+    const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
     batch_single_kernel<<<std::min<uint32_t>(grid_for(nb * 64), 65536), kBlock, 0, st>>>(A, P.kind, P.ctr);
     batch_small_kernel<true><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, P.ctr);
     batch_large_kernel<true><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
@@ -1148,7 +1347,7 @@ int lnt_batch_set_param(lnt_batch b, const char *name, int64_t value) {
   auto in = [&](int64_t lo, int64_t hi) { return value >= lo && value <= hi; };
   if (n == "item_products" && in(64, kCap / 2)) b->tuning.item_products = (uint32_t)value;
   else if (n == "small_max" && in(2, kCap / 2)) b->tuning.small_max = (uint32_t)value;
-  else if (n == "window" && in(64, kCap / 2)) b->tuning.window = (uint32_t)value;
+  else if (n == "window" && in(64, 3 * kCap / 4)) b->tuning.window = (uint32_t)value;
   else if (n == "blocks_per_cu" && in(1, 8)) b->tuning.blocks_per_cu = (uint32_t)value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown batch parameter or value out of range: " + n);
   return LNT_OK;
