@@ -102,7 +102,7 @@ int lnt_context_set_profiling(lnt_context ctx, int on);
 int lnt_context_profile(lnt_context ctx, const char *kernel, int64_t *launches,
                         double *total_ms, double *bytes);
 
-/* -------------------------------------------------- relinearisation (a2) -- */
+/* ------------------------------------------------------- relinearisation -- */
 /* In place: every key is recomputed from mode order cur_seq to new_seq.  Does
  * not sort.  Replaces SparseMIABase::change_linIdx_sequence
  * (multi_index_array/SparseMIABase.h:768-814, per-key
@@ -113,7 +113,7 @@ int lnt_change_linidx(lnt_context ctx, int64_t *keys, int64_t n, int order,
                       const int64_t *dims, const int32_t *cur_seq,
                       const int32_t *new_seq, int mem);
 
-/* ------------------------------------------------------------ sort (a3) -- */
+/* ------------------------------------------------------------------ sort -- */
 /* In place: sorts keys ascending and permutes vals alongside.  Keys must be
  * unique and in [0, max_size).  Replaces SparseMIABase::sort()
  * (multi_index_array/SparseMIABase.h:701-734 ->
@@ -122,7 +122,7 @@ int lnt_change_linidx(lnt_context ctx, int64_t *keys, int64_t n, int order,
 int lnt_sort_pairs(lnt_context ctx, int64_t *keys, double *vals, int64_t n,
                    uint64_t max_size, int mem);
 
-/* ------------------------------------------------ change_sort_order (a4) -- */
+/* ----------------------------------------------------- change_sort_order -- */
 /* In place: relinearises every key from cur_seq to new_seq and sorts (keys,
  * vals) ascending in the new order.  `is_sorted` states whether the input is
  * sorted in cur_seq (a hint; the result does not depend on it).  Replaces
@@ -135,7 +135,7 @@ int lnt_change_sort_order(lnt_context ctx, int64_t *keys, double *vals,
                           const int32_t *cur_seq, const int32_t *new_seq,
                           int is_sorted, int mem);
 
-/* ------------------------------------------------------ merge (a5, a7) -- */
+/* --------------------------------------------------- merge and Hadamard -- */
 /* A and B sorted ascending (same mode order, unique keys).  op ADD/SUB: sorted
  * union, equal keys -> a op b, A only -> a, B only -> 0 op b (exact IEEE).  op
  * HADAMARD: sorted intersection with a*b.  C is sorted.  `capacity` must be at
@@ -156,8 +156,13 @@ int lnt_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
               int64_t nb, int op, int64_t **c_keys, double **c_vals,
               int64_t *nc, int mem);
 int lnt_free(lnt_context ctx, void *ptr, int mem);
+/* Copies `bytes` bytes from src to dst (device memory, either side may be an
+ * array the library allocated) on the context's stream and waits: how a
+ * caller moves a library-allocated result into its own buffers before
+ * lnt_free.  No counterpart in the reference (mex outputs are mxArrays). */
+int lnt_copy(lnt_context ctx, void *dst, const void *src, int64_t bytes);
 
-/* ------------------------------------------- lattice product (a8-a11) -- */
+/* ------------------------------------------------------- lattice product -- */
 typedef enum lnt_spgemm_strategy {
   LNT_SPGEMM_AUTO = 0, /* library choice (heuristic or learned dispatch) */
   LNT_SPGEMM_SORT = 1, /* expand, sort rows, sum runs (deterministic sums) */

@@ -58,6 +58,7 @@ SIGNATURES = {
     "lnt_merge_into": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _vp, _i64, _i64p, _int],
     "lnt_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_free": [_vp, _vp, _int],
+    "lnt_copy": [_vp, _vp, _vp, _i64],
     "lnt_lattice_nprod": [_vp, _vp, _i64, _i64p, _vp, _i64, _i64p, _i64p, _int],
     "lnt_lattice_mult_into": [_vp, _vp, _vp, _i64, _i64p, _vp, _vp, _i64, _i64p, _int, _vp, _vp, _i64, _i64p, _int],
     "lnt_lattice_mult": [_vp, _vp, _vp, _i64, _i64p, _vp, _vp, _i64, _i64p, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
@@ -75,8 +76,23 @@ SIGNATURES = {
     "lnt_batch_set_param": [_vp, ctypes.c_char_p, _i64],
     "lnt_batch_get_param": [_vp, ctypes.c_char_p, _i64p],
     "lnt_batch_nprod": [_vp, _vp, _vp, _i64p],
+    "lnt_contract": [_vp, _vp, _vp, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _vp, ctypes.POINTER(_int)],
+    "lnt_contract_features": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    "lnt_contract_heuristic": [_vp, _vp, ctypes.POINTER(_int)],
+    "lnt_contract_time": [_vp, _vp, _vp, _int, _int, ctypes.POINTER(ctypes.c_double)],
+    "lnt_model_create": [_vp, _u64, ctypes.POINTER(_vp)],
+    "lnt_model_destroy": [_vp],
+    "lnt_model_param_count": [],
+    "lnt_model_get_params": [_vp, ctypes.POINTER(ctypes.c_float)],
+    "lnt_model_set_params": [_vp, ctypes.POINTER(ctypes.c_float)],
+    "lnt_model_train": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _i64, _int, _int,
+                        ctypes.c_float, _u64, ctypes.POINTER(ctypes.c_float)],
+    "lnt_model_predict": [_vp, ctypes.POINTER(ctypes.c_float), _i64, ctypes.POINTER(ctypes.c_float)],
+    "lnt_model_gradient": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _i64,
+                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    "lnt_context_set_model": [_vp, _vp],
 }
-_RESTYPES = {"lnt_last_error": ctypes.c_char_p}
+_RESTYPES = {"lnt_last_error": ctypes.c_char_p, "lnt_model_param_count": ctypes.c_int64}
 
 
 class LntError(RuntimeError):

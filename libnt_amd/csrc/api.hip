@@ -256,7 +256,8 @@ int lnt_context_destroy(lnt_context ctx) {
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
                           &ctx->host_tmp_b, &ctx->host_tmp_c, &ctx->spg_ws, &ctx->spg_tmp,
-                          &ctx->spg_big, &ctx->hist_part, &ctx->relin_dev})
+                          &ctx->spg_big, &ctx->hist_part, &ctx->relin_dev, &ctx->exec_buf[0],
+                          &ctx->exec_buf[1], &ctx->exec_buf[2], &ctx->exec_buf[3]})
     release(*b);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   prof_clear(ctx);
@@ -345,7 +346,8 @@ int lnt_context_trim(lnt_context ctx) {
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->aux, &ctx->host_tmp_a, &ctx->host_tmp_b, &ctx->host_tmp_c,
-                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big})
+                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big, &ctx->exec_buf[0], &ctx->exec_buf[1],
+                          &ctx->exec_buf[2], &ctx->exec_buf[3]})
     release(*b);
   return LNT_OK;
 }
@@ -665,6 +667,15 @@ int lnt_free(lnt_context ctx, void *ptr, int mem) {
   }
   if (mem != LNT_MEM_HOST) return fail(LNT_ERR_INVALID_ARGUMENT, "mem must be LNT_MEM_HOST or LNT_MEM_DEVICE");
   free(ptr);
+  return LNT_OK;
+}
+
+int lnt_copy(lnt_context ctx, void *dst, const void *src, int64_t bytes) {
+  LNT_CTX(ctx);
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return fail(LNT_ERR_INVALID_ARGUMENT, "bad copy arguments");
+  if (bytes == 0) return LNT_OK;
+  LNT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
   return LNT_OK;
 }
 

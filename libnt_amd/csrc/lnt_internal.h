@@ -398,6 +398,8 @@ struct lnt_context_s {
   lnt::DeviceBuffer relin_dev;  // device copy of the current relinearisation table
   lnt::DeviceBuffer spg_ws, spg_tmp, spg_big;  // lattice-product workspaces
   lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
+  lnt::DeviceBuffer exec_buf[4];  // operand copies of a whole contraction (keys, values of both)
+  void *model = nullptr;          // lnt_model used by LNT_STRAT_AUTO (not owned)
   uint32_t *pinned = nullptr;   // small pinned host buffer for counts/flags
   int profiling = 0;
   std::vector<lnt::ProfRecord> prof;   // recorded launches (events owned)

@@ -44,6 +44,10 @@ def lib():
         u64, i64, f64, i32 = ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_int
         sig = {
             "lnto_mix64": (u64, [u64]),
+            "lnto_mlp_param_count": (i64, []),
+            "lnto_mlp_weight_count": (i64, []),
+            "lnto_mlp_forward": (None, [_f64p, _f64p, _f64p]),
+            "lnto_mlp_loss_grad": (f64, [_f64p, _f64p, _f64p, i64, _f64p]),
             "lnto_rand_u64": (u64, [u64, u64]),
             "lnto_uniform_pm1": (f64, [u64, u64]),
             "lnto_uniform_01": (f64, [u64, u64]),
@@ -276,3 +280,32 @@ def allclose_fuzzy(a, b, prec=1e-12) -> bool:
     small = (np.abs(a) < 1) | (np.abs(b) < 1)
     ok = np.where(small, absd <= prec, absd <= np.minimum(np.abs(a), np.abs(b)) * prec)
     return bool(np.all(ok))
+
+
+# ------------------------------------------------------------------ cost model
+def mlp_param_count() -> int:
+    return int(lib().lnto_mlp_param_count())
+
+
+def mlp_weight_count() -> int:
+    return int(lib().lnto_mlp_weight_count())
+
+
+def mlp_forward(params, X) -> np.ndarray:
+    """Predictions (n x 8) of the dispatch model for features X (n x 18)."""
+    P = _f64(params)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    out = np.empty((X.shape[0], 8))
+    for i in range(X.shape[0]):
+        lib().lnto_mlp_forward(_p(P, _f64p), _p(X[i], _f64p), _p(out[i], _f64p))
+    return out
+
+
+def mlp_loss_grad(params, X, T):
+    """(loss, gradient over the trainable parameters) in FP64."""
+    P = _f64(params)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    g = np.zeros(mlp_weight_count())
+    loss = lib().lnto_mlp_loss_grad(_p(P, _f64p), _p(X, _f64p), _p(T, _f64p), X.shape[0], _p(g, _f64p))
+    return float(loss), g

@@ -693,3 +693,87 @@ int lnto_is_equal_fuzzy(double a, double b, double prec) {
   double m = fabs(a) < fabs(b) ? fabs(a) : fabs(b);
   return fabs(a - b) <= m * prec;
 }
+
+/* ------------------------------ cost model ------------------------------ */
+enum {
+  MF = LNTO_MLP_F, MH = LNTO_MLP_H, MS = LNTO_MLP_S,
+  M_W1 = 0, M_B1 = M_W1 + MH * MF, M_W2 = M_B1 + MH, M_B2 = M_W2 + MH * MH,
+  M_W3 = M_B2 + MH, M_B3 = M_W3 + MS * MH, M_NW = M_B3 + MS, M_MEAN = M_NW,
+  M_ISTD = M_NW + MF, M_NP = M_NW + 2 * MF
+};
+
+int64_t lnto_mlp_param_count(void) { return M_NP; }
+int64_t lnto_mlp_weight_count(void) { return M_NW; }
+
+/* Activations of one sample: xs (standardised input), z1, z2, y. */
+static void mlp_sample(const double *P, const double *x, double *xs, double *z1, double *z2, double *y) {
+  double h1[MH], h2[MH];
+  for (int k = 0; k < MF; ++k) xs[k] = (x[k] - P[M_MEAN + k]) * P[M_ISTD + k];
+  for (int j = 0; j < MH; ++j) {
+    double z = P[M_B1 + j];
+    for (int k = 0; k < MF; ++k) z += P[M_W1 + j * MF + k] * xs[k];
+    z1[j] = z;
+    h1[j] = z > 0 ? z : 0;
+  }
+  for (int j = 0; j < MH; ++j) {
+    double z = P[M_B2 + j];
+    for (int i = 0; i < MH; ++i) z += P[M_W2 + j * MH + i] * h1[i];
+    z2[j] = z;
+    h2[j] = z > 0 ? z : 0;
+  }
+  for (int o = 0; o < MS; ++o) {
+    double v = P[M_B3 + o];
+    for (int j = 0; j < MH; ++j) v += P[M_W3 + o * MH + j] * h2[j];
+    y[o] = v;
+  }
+}
+
+void lnto_mlp_forward(const double *params, const double *x, double *y) {
+  double xs[MF], z1[MH], z2[MH];
+  mlp_sample(params, x, xs, z1, z2, y);
+}
+
+double lnto_mlp_loss_grad(const double *P, const double *X, const double *T, int64_t n, double *grad) {
+  int64_t m = 0;
+  for (int64_t i = 0; i < n * MS; ++i) m += T[i] == T[i];
+  for (int p = 0; p < M_NW; ++p) grad[p] = 0;
+  if (m == 0) return 0;
+  const double scale = 1.0 / (double)m;
+  double loss = 0;
+  for (int64_t s = 0; s < n; ++s) {
+    double xs[MF], z1[MH], z2[MH], y[MS], d3[MS], dz2[MH], dz1[MH];
+    mlp_sample(P, X + s * MF, xs, z1, z2, y);
+    for (int o = 0; o < MS; ++o) {
+      const double t = T[s * MS + o];
+      d3[o] = 0;
+      if (t == t) {
+        const double e = y[o] - t;
+        loss += scale * e * e;
+        d3[o] = 2 * scale * e;
+      }
+    }
+    for (int j = 0; j < MH; ++j) {
+      double g = 0;
+      for (int o = 0; o < MS; ++o) g += P[M_W3 + o * MH + j] * d3[o];
+      dz2[j] = z2[j] > 0 ? g : 0;
+    }
+    for (int i = 0; i < MH; ++i) {
+      double g = 0;
+      for (int j = 0; j < MH; ++j) g += P[M_W2 + j * MH + i] * dz2[j];
+      dz1[i] = z1[i] > 0 ? g : 0;
+    }
+    for (int o = 0; o < MS; ++o) {
+      for (int j = 0; j < MH; ++j) grad[M_W3 + o * MH + j] += d3[o] * (z2[j] > 0 ? z2[j] : 0);
+      grad[M_B3 + o] += d3[o];
+    }
+    for (int j = 0; j < MH; ++j) {
+      for (int i = 0; i < MH; ++i) grad[M_W2 + j * MH + i] += dz2[j] * (z1[i] > 0 ? z1[i] : 0);
+      grad[M_B2 + j] += dz2[j];
+    }
+    for (int j = 0; j < MH; ++j) {
+      for (int k = 0; k < MF; ++k) grad[M_W1 + j * MF + k] += dz1[j] * xs[k];
+      grad[M_B1 + j] += dz1[j];
+    }
+  }
+  return loss;
+}

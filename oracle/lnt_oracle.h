@@ -103,6 +103,22 @@ int lnto_contract_plan(int order_a, const int64_t *dims_a, const char *labels_a,
                        int32_t *seq_a, int32_t *seq_b, int64_t *lat_a,
                        int64_t *lat_b, int64_t *dims_c, int *order_c);
 
+/* ------------------------------ cost model ------------------------------ *
+ * FP64 restatement of the dispatch model (no reference counterpart): a
+ * perceptron F=18 -> 64 -> 64 -> S=8 with ReLU (derivative 0 at 0); params
+ * W1[64][18] b1[64] W2[64][64] b2[64] W3[8][64] b3[8] then mean[18] and
+ * inv_std[18]; input x is standardised as (x - mean) * inv_std.  Loss =
+ * (1/m) sum over measured targets (not NaN) of (y - t)^2, m = their count. */
+#define LNTO_MLP_F 18
+#define LNTO_MLP_H 64
+#define LNTO_MLP_S 8
+int64_t lnto_mlp_param_count(void);   /* trainable + standardisation */
+int64_t lnto_mlp_weight_count(void);  /* trainable only */
+void lnto_mlp_forward(const double *params, const double *x, double *y);
+/* Returns the loss; grad receives d loss / d (trainable params). */
+double lnto_mlp_loss_grad(const double *params, const double *X, const double *T, int64_t n,
+                          double *grad);
+
 /* ------------------------------- comparison ----------------------------- */
 int lnto_is_equal_fuzzy(double a, double b, double prec);
 
