@@ -15,6 +15,17 @@ radix pass) -- algorithmic bytes (32 B per key per pass) over its measured
 average launch duration (HIP events on the library's stream during the timed
 steps).  cpu_baseline: the CPU restatement of the reference algorithms
 (oracle/), one core, on a bounded 1/8-scale sample of the same workload.
+
+Further objects on the same line (skip with --no-extras):
+  contraction: the large skewed contraction A(i,j,k)*B(k,l,m), A 4096x4096x512
+    (858,993 nnz), B 512x4096x64 (13,422 nnz), i and l Zipf(1.1): whole
+    contraction (unfold sorts + lattice product + refold) per accumulator,
+    flops/s = 2 * scalar products / time;
+  batched: 10,000 independent contractions (order 3-4, 1-2 contracted modes,
+    dims log-U[8,128], nnz log-U[100,50k]) as one recorded hipGraph with
+    autotuned parameters, against one lnt_contract call per instance with the
+    restated polyalgorithm's strategy and with the learned model's strategy;
+  learned: the shipped cost model on the committed held-out timing set.
 """
 from __future__ import annotations
 
@@ -113,6 +124,143 @@ def _profiled_traffic():
     return None
 
 
+def _log(msg):
+    print("[bench] %s" % msg, file=sys.stderr, flush=True)
+
+
+def bench_contraction(ctx, torch):
+    """Large skewed contraction, both accumulators, timed whole."""
+    from libnt_amd import dispatch
+
+    sp = dispatch.Spec("skewed", [4096, 4096, 512], "ijk", [512, 4096, 64], "klm", 858993, 13422,
+                       "default", "default", (0,), (1,))
+    A, B = dispatch.materialize(ctx, sp, 31)
+    # scalar products: sum over k of nnzA(:,k) * nnzB(k,:)
+    ka = A.keys.cpu().numpy()
+    kb = B.keys.cpu().numpy()
+    ca = np.bincount(ka // (4096 * 4096), minlength=512)
+    cb = np.bincount(kb % 512, minlength=512)
+    nprod = int((ca.astype(np.int64) * cb).sum())
+    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, 0b0111, 3)
+    res = {"workload": "A(i,j,k)*B(k,l,m), A 4096x4096x512 nnz 858993, B 512x4096x64 nnz 13422, "
+                       "i and l Zipf(1.1); unfold sorts + lattice product + refold",
+           "products": nprod, "unit": "GFLOP/s (2 flops per scalar product)"}
+    for s in (0, 1, 2):
+        res[dispatch.strategy_name(s)] = {"ms": round(ms[s], 3), "gflops": round(2 * nprod / (ms[s] * 1e-3) / 1e9, 2)}
+    best = min((0, 1, 2), key=lambda s: ms[s])
+    res["best"] = dispatch.strategy_name(best)
+    _log("contraction: %s" % res)
+    return res
+
+
+def bench_batched(ctx, torch):
+    """The batched mix: autotuned recorded graph vs one call per instance."""
+    import ctypes
+
+    import libnt_amd
+    from libnt_amd import autotune, dispatch, workloads
+    from libnt_amd.batch import Batch, packed_offsets
+    from libnt_amd.sparse import SparseMIA
+
+    insts = workloads.batched_contractions(10000, seed=4)
+    ops = workloads.generate_operands(ctx, insts)
+    tuned = autotune.tune(ctx, insts, ops, stride=10, passes=2, reps=2)
+    _log("autotune: %s" % tuned)
+    b = Batch(ctx, insts)
+    autotune.apply_batch(ctx, b, tuned["config"])
+    nprod = b.nprod(ops[0], ops[2])
+    nnz = b.capture(*ops)
+    b.replay()
+    times = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.replay()
+        times.append(time.perf_counter() - t0)
+    t_batched = float(np.median(times))
+    b.close()
+    del b
+    torch.cuda.empty_cache()
+    # one lnt_contract call per instance (library-allocated result, freed)
+    a_off = packed_offsets([x.nnz_a for x in insts])
+    b_off = packed_offsets([x.nnz_b for x in insts])
+    ak, av, bk, bv = ops
+    pairs = []
+    for i, x in enumerate(insts):
+        A = SparseMIA(x.dims_a, ak[a_off[i]:a_off[i + 1]], av[a_off[i]:a_off[i + 1]], ctx=ctx)
+        B = SparseMIA(x.dims_b, bk[b_off[i]:b_off[i + 1]], bv[b_off[i]:b_off[i + 1]], ctx=ctx)
+        pairs.append((dispatch.operand(A, x.labels_a), dispatch.operand(B, x.labels_b)))
+    L = libnt_amd.lib()
+
+    def per_instance(strategy):
+        ck, cv = ctypes.c_void_p(), ctypes.c_void_p()
+        n = ctypes.c_int64()
+        shape = dispatch.ResultShape()
+        used = ctypes.c_int()
+        hist = np.zeros(dispatch.NUM_STRATEGIES, dtype=np.int64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for oa, ob in pairs:
+            libnt_amd.check(L.lnt_contract(ctx.handle, ctypes.byref(oa), ctypes.byref(ob), strategy, ctypes.byref(ck),
+                                           ctypes.byref(cv), ctypes.byref(n), ctypes.byref(shape), ctypes.byref(used)))
+            L.lnt_free(ctx.handle, ck, libnt_amd.MEM_DEVICE)
+            L.lnt_free(ctx.handle, cv, libnt_amd.MEM_DEVICE)
+            hist[used.value] += 1
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, hist
+
+    t_plain, h_plain = per_instance(dispatch.STRAT_AUTO)  # no model: the restated polyalgorithm
+    res = {
+        "workload": "10,000 contractions: order 3-4, 1-2 contracted modes, dims log-U[8,128], nnz log-U[100,50k]",
+        "products": int(nprod.sum()), "nnz_c": int(nnz),
+        "batched_graph_s": round(t_batched, 4),
+        "batched_gflops": round(2 * nprod.sum() / t_batched / 1e9, 2),
+        "instances_per_s": round(len(insts) / t_batched, 1),
+        "autotuned": tuned["config"], "autotune_subsample_ms": {"default": round(tuned["default_ms"], 2),
+                                                                 "tuned": round(tuned["ms"], 2)},
+        "plain_heuristic_per_instance_s": round(t_plain, 4),
+        "plain_strategy_histogram": h_plain.tolist(),
+        "speedup_batched_vs_plain_heuristic": round(t_plain / t_batched, 2),
+    }
+    model_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnt_amd", "data", "dispatch_model.npy")
+    if os.path.exists(model_path):
+        m = dispatch.Model(ctx, 1)
+        m.load(model_path)
+        m.attach()
+        t_learned, h_learned = per_instance(dispatch.STRAT_AUTO)
+        libnt_amd.check(L.lnt_context_set_model(ctx.handle, None))
+        m.close()
+        res["learned_per_instance_s"] = round(t_learned, 4)
+        res["learned_strategy_histogram"] = h_learned.tolist()
+        res["speedup_learned_vs_plain_heuristic"] = round(t_plain / t_learned, 3)
+    _log("batched: %s" % res)
+    return res
+
+
+def bench_learned(ctx):
+    """Held-out regret of the shipped model (committed MI355X timings)."""
+    from libnt_amd import dispatch
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnt_amd", "data")
+    tpath, mpath = os.path.join(here, "dispatch_timings.npz"), os.path.join(here, "dispatch_model.npy")
+    if not (os.path.exists(tpath) and os.path.exists(mpath)):
+        return None
+    d = np.load(tpath, allow_pickle=False)
+    m = dispatch.Model(ctx, 1)
+    m.load(mpath)
+    ms = np.exp2(d["T_test"].astype(np.float64))
+    learned = dispatch.choose(m.predict(d["X_test"]))
+    idx = np.arange(len(ms))
+    res = {"held_out": int(len(ms)), "train": int(len(d["X_train"]))}
+    for name, ch in (("learned", learned), ("heuristic", d["H_test"]), ("fixed_default", np.zeros(len(ms), int))):
+        r = dispatch.regret(ms, ch)
+        res[name] = {"total_ms": round(float(ms[idx, ch].sum()), 2), "mean_regret": round(float(r.mean()), 4),
+                     "median_regret": round(float(np.median(r)), 4)}
+    res["best_total_ms"] = round(float(ms.min(axis=1).sum()), 2)
+    m.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +268,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-scale", type=int, default=8)
+    ap.add_argument("--no-extras", action="store_true", help="only the sort/merge workload")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, run a plain 1.6 GB write (fill) and read (sum) so that "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE can be scaled by kernels of known traffic")
@@ -201,7 +350,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "onesweep radix pass (8-bit digit, 256x16 tile)",
+            "kernel": "onesweep radix pass (8-bit digit, %dx%d tile)" % (ctx.get_param("sort_block"),
+                                                                      ctx.get_param("sort_items")),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -223,6 +373,12 @@ def main():
         del cal
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scale=args.cpu_scale)
+    if not args.no_extras:
+        del ak, av, bk, bv, cu_k, cu_v, ch_k, ch_v
+        torch.cuda.empty_cache()
+        out["contraction"] = bench_contraction(ctx, torch)
+        out["batched"] = bench_batched(ctx, torch)
+        out["learned"] = bench_learned(ctx)
     print(json.dumps(out), flush=True)
 
 

@@ -88,7 +88,9 @@ int lnt_abi_version(void);
  * reference counterpart (the reference has only ad-hoc std::chrono prints,
  * multi_index_array/SparseLatticeBase.h:1885,1981-1982). */
 /* Tuning parameters (the autotuner's knobs; results never depend on them):
- *   "sort_items"   onesweep items per thread: 8, 12 or 16 (tile = 256*items)
+ *   "sort_items"   onesweep items per thread: 8, 12 or 16 (tile = sort_block*items)
+ *   "sort_block"   onesweep threads per workgroup: 256 or 512 (default 512)
+ *   "digit_bits"   widest radix digit per onesweep pass, 4..8 (default 8)
  *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096
  *   "merge_items"  merge-path steps per thread: 8 or 16 (tile = 256*items)
  *   "spgemm_small" longest output column (products) kept in LDS, 2..2048
@@ -155,6 +157,23 @@ int lnt_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
               int64_t na, const int64_t *b_keys, const double *b_vals,
               int64_t nb, int op, int64_t **c_keys, double **c_vals,
               int64_t *nc, int mem);
+/* Union of two operands that are NOT sorted (SparseMIA::sortMerge,
+ * SparseMIA.h:833-887, the path of C = A + B / A += B when neither operand is
+ * sorted, SparseMIABase.h:1383-1398, SparseMIA.h:1030-1050).  Both key arrays
+ * must already use one linearisation (the caller relinearises the smaller
+ * operand, as sortMerge does) with keys in [0, bound).  A then B (B negated
+ * for LNT_OP_SUB) are sorted together and equal keys collapsed by
+ * collect_duplicates (SparseMIA.h:628-665): a key's first entry opens the key
+ * only if its value is non-zero (the smallest key always opens), later
+ * entries are added to it -- so keys whose entries are all zero vanish.  C is
+ * sorted; capacity >= na + nb. */
+int lnt_sort_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                        const int64_t *b_keys, const double *b_vals, int64_t nb, int op, uint64_t bound,
+                        int64_t *c_keys, double *c_vals, int64_t capacity, int64_t *nc, int mem);
+/* As lnt_sort_merge_into with C allocated by the library (lnt_free). */
+int lnt_sort_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                   const int64_t *b_keys, const double *b_vals, int64_t nb, int op, uint64_t bound,
+                   int64_t **c_keys, double **c_vals, int64_t *nc, int mem);
 int lnt_free(lnt_context ctx, void *ptr, int mem);
 /* Copies `bytes` bytes from src to dst (device memory, either side may be an
  * array the library allocated) on the context's stream and waits: how a

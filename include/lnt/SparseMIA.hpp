@@ -101,6 +101,16 @@ struct Tensor {
   std::vector<double> data;
   bool sorted = false;
 
+  // change_linIdx_sequence (SparseMIABase.h:768-814): relinearise, unsorted.
+  void relinearise(const std::vector<int32_t> &new_seq) {
+    if (new_seq == seq) return;
+    if (!idx.empty())
+      check(lnt_change_linidx(default_context().get(), idx.data(), (int64_t)idx.size(), (int)dims.size(), dims.data(),
+                              seq.data(), new_seq.data(), LNT_MEM_HOST));
+    seq = new_seq;
+    sorted = false;
+  }
+
   void sort_to(const std::vector<int32_t> &new_seq) {
     if (!idx.empty())
       check(lnt_change_sort_order(default_context().get(), idx.data(), data.data(), (int64_t)idx.size(),
@@ -128,7 +138,8 @@ inline int find_label(const std::vector<Label> &ls, size_t id) {
   return -1;
 }
 
-// outside_merge -> outside_scanMerge (SparseMIABase.h:1357-1455).
+// outside_merge (SparseMIABase.h:1357-1455): outside_scanMerge when an operand
+// is sorted, sortMerge when neither is.
 inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const std::vector<Label> &lb, int op) {
   if (la.size() != lb.size()) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "merge orders differ");
   Tensor b_copy;
@@ -159,9 +170,33 @@ inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const s
   } else if (a.sorted) {
     b.sort_to(b_in_a);
   } else {
-    a.sort_to(a.seq);
-    for (size_t t = 0; t < n; ++t) b_in_a[t] = index_order[a.seq[t]];
-    b.sort_to(b_in_a);
+    // Neither sorted: C = copy of A, C.sortMerge(B) (SparseMIABase.h:1393-1398,
+    // SparseMIA.h:833-887): the smaller operand is relinearised into the
+    // other's sequence, then concatenation, sort and collect_duplicates.
+    auto c = std::make_shared<Tensor>();
+    c->dims = a.dims;
+    c->sorted = true;
+    std::vector<int64_t> a_idx = a.idx;
+    if (a.idx.size() > b.idx.size()) {
+      b.relinearise(b_in_a);
+      c->seq = a.seq;
+    } else {
+      if (!a_idx.empty() && a.seq != a_in_b)
+        check(lnt_change_linidx(default_context().get(), a_idx.data(), (int64_t)a_idx.size(), (int)n, a.dims.data(),
+                                a.seq.data(), a_in_b.data(), LNT_MEM_HOST));
+      c->seq = a_in_b;
+    }
+    int64_t *ck = nullptr;
+    double *cv = nullptr;
+    int64_t nc = 0;
+    check(lnt_sort_merge(default_context().get(), a_idx.data(), a.data.data(), (int64_t)a_idx.size(), b.idx.data(),
+                         b.data.data(), (int64_t)b.idx.size(), op, (uint64_t)product(a.dims), &ck, &cv, &nc,
+                         LNT_MEM_HOST));
+    c->idx.assign(ck, ck + nc);
+    c->data.assign(cv, cv + nc);
+    lnt_free(default_context().get(), ck, LNT_MEM_HOST);
+    lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+    return {c, la};
   }
   auto c = std::make_shared<Tensor>();
   c->dims = a.dims;

@@ -85,9 +85,10 @@ int lnt_contract_features(const lnt_operand *a, const lnt_operand *b, float *fea
 int lnt_contract_heuristic(const lnt_operand *a, const lnt_operand *b, int *strategy);
 
 /* Wall time (ms, host clock around the call and a final synchronisation) of
- * each strategy in `mask` (bit s = strategy s), the minimum over `reps` runs;
- * one untimed warm-up run of the first strategy precedes them.  Unmeasured
- * entries of ms[LNT_NUM_STRATEGIES] get -1. */
+ * each strategy in `mask` (bit s = strategy s), the minimum over `reps` runs
+ * after one untimed warm-up run of that strategy.  The result goes to one
+ * reusable buffer (sized by a first, untimed run), so no allocation is timed.
+ * Unmeasured entries of ms[LNT_NUM_STRATEGIES] get -1. */
 int lnt_contract_time(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, int mask, int reps,
                       double *ms);
 

@@ -59,6 +59,9 @@ SIGNATURES = {
     "lnt_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_free": [_vp, _vp, _int],
     "lnt_copy": [_vp, _vp, _vp, _i64],
+    "lnt_sort_merge_into": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _u64, _vp, _vp, _i64, _i64p, _int],
+    "lnt_sort_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _u64, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                       _i64p, _int],
     "lnt_lattice_nprod": [_vp, _vp, _i64, _i64p, _vp, _i64, _i64p, _i64p, _int],
     "lnt_lattice_mult_into": [_vp, _vp, _vp, _i64, _i64p, _vp, _vp, _i64, _i64p, _int, _vp, _vp, _i64, _i64p, _int],
     "lnt_lattice_mult": [_vp, _vp, _vp, _i64, _i64p, _vp, _vp, _i64, _i64p, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
@@ -264,6 +267,35 @@ class Context:
             ck = torch.empty(max(cap, 1), dtype=torch.int64, device=ak.device)
             cv = torch.empty(max(cap, 1), dtype=torch.float64, device=ak.device)
         n = self.merge_into(ak, av, bk, bv, op, ck, cv)
+        return ck[:n], cv[:n]
+
+    def sort_merge_into(self, ak, av, bk, bv, op, bound, ck, cv) -> int:
+        """Union of two unsorted operands in one linearisation (SparseMIA::
+        sortMerge, with collect_duplicates' zero-entry rule) into ck/cv."""
+        akp, mem, na = _ptr_mem(ak)
+        avp, _, _ = _ptr_mem(av)
+        bkp, memb, nb = _ptr_mem(bk)
+        bvp, _, _ = _ptr_mem(bv)
+        ckp, memc, cap = _ptr_mem(ck)
+        cvp, _, capv = _ptr_mem(cv)
+        if not (mem == memb == memc):
+            raise ValueError("all arrays must live in the same memory kind")
+        nc = ctypes.c_int64(0)
+        check(lib().lnt_sort_merge_into(self._h, akp, avp, na, bkp, bvp, nb, op, int(bound), ckp, cvp,
+                                        min(cap, capv), ctypes.byref(nc), mem))
+        return nc.value
+
+    def sort_merge(self, ak, av, bk, bv, op, bound):
+        cap = max(ak.shape[0] + bk.shape[0], 1)
+        if isinstance(ak, np.ndarray):
+            ck = np.empty(cap, dtype=np.int64)
+            cv = np.empty(cap, dtype=np.float64)
+        else:
+            import torch
+
+            ck = torch.empty(cap, dtype=torch.int64, device=ak.device)
+            cv = torch.empty(cap, dtype=torch.float64, device=ak.device)
+        n = self.sort_merge_into(ak, av, bk, bv, op, bound, ck, cv)
         return ck[:n], cv[:n]
 
     def lattice_nprod(self, ak, a_dims, bk, b_dims) -> int:

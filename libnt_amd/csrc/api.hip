@@ -257,7 +257,8 @@ int lnt_context_destroy(lnt_context ctx) {
                           &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
                           &ctx->host_tmp_b, &ctx->host_tmp_c, &ctx->spg_ws, &ctx->spg_tmp,
                           &ctx->spg_big, &ctx->hist_part, &ctx->relin_dev, &ctx->exec_buf[0],
-                          &ctx->exec_buf[1], &ctx->exec_buf[2], &ctx->exec_buf[3]})
+                          &ctx->exec_buf[1], &ctx->exec_buf[2], &ctx->exec_buf[3], &ctx->exec_buf[4],
+                          &ctx->exec_buf[5]})
     release(*b);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   prof_clear(ctx);
@@ -276,6 +277,7 @@ int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
   else if (n == "spgemm_small" && in(2, 2048)) ctx->tuning.spgemm_small = (int)value;
   else if (n == "narrow_keys" && in(0, 1)) ctx->tuning.narrow_keys = (int)value;
   else if (n == "digit_bits" && in(4, 8)) ctx->tuning.digit_bits = (int)value;
+  else if (n == "sort_block" && (value == 256 || value == 512)) ctx->tuning.sort_block = (int)value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter or value out of range: " + n);
   return LNT_OK;
 }
@@ -289,6 +291,7 @@ int lnt_context_get_param(lnt_context ctx, const char *name, int64_t *value) {
   else if (n == "spgemm_small") *value = ctx->tuning.spgemm_small;
   else if (n == "narrow_keys") *value = ctx->tuning.narrow_keys;
   else if (n == "digit_bits") *value = ctx->tuning.digit_bits;
+  else if (n == "sort_block") *value = ctx->tuning.sort_block;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter: " + n);
   return LNT_OK;
 }
@@ -347,7 +350,7 @@ int lnt_context_trim(lnt_context ctx) {
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->aux, &ctx->host_tmp_a, &ctx->host_tmp_b, &ctx->host_tmp_c,
                           &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big, &ctx->exec_buf[0], &ctx->exec_buf[1],
-                          &ctx->exec_buf[2], &ctx->exec_buf[3]})
+                          &ctx->exec_buf[2], &ctx->exec_buf[3], &ctx->exec_buf[4], &ctx->exec_buf[5]})
     release(*b);
   return LNT_OK;
 }
@@ -509,6 +512,95 @@ int lnt_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
     }
   }
   int rc = lnt_merge_into(ctx, a_keys, a_vals, na, b_keys, b_vals, nb, op, (int64_t *)k, (double *)v, cap, nc, mem);
+  if (rc != LNT_OK) {
+    if (mem == LNT_MEM_DEVICE) {
+      (void)hipFree(k);
+      (void)hipFree(v);
+    } else {
+      free(k);
+      free(v);
+    }
+    return rc;
+  }
+  *c_keys = (int64_t *)k;
+  *c_vals = (double *)v;
+  return LNT_OK;
+}
+
+int lnt_sort_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                        const int64_t *b_keys, const double *b_vals, int64_t nb, int op, uint64_t bound,
+                        int64_t *c_keys, double *c_vals, int64_t capacity, int64_t *nc, int mem) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  if (!nc) return fail(LNT_ERR_INVALID_ARGUMENT, "null nc");
+  if (op != LNT_OP_ADD && op != LNT_OP_SUB) return fail(LNT_ERR_INVALID_ARGUMENT, "op must be LNT_OP_ADD or LNT_OP_SUB");
+  if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  if (na + nb >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "na + nb must be < 2^32");
+  if (capacity < na + nb)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) + " is below na + nb");
+  if ((na && (!a_keys || !a_vals)) || (nb && (!b_keys || !b_vals)) || (na + nb && (!c_keys || !c_vals)))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "null array");
+  if (bound == 0 || bound > (uint64_t(1) << 63)) return fail(LNT_ERR_INVALID_ARGUMENT, "bound must be in [1, 2^63]");
+  *nc = 0;
+  uint64_t count = 0;
+  if (mem == LNT_MEM_DEVICE) {
+    LNT_TRY(sort_merge_device(ctx, (const uint64_t *)a_keys, a_vals, (uint32_t)na, (const uint64_t *)b_keys, b_vals,
+                              (uint32_t)nb, op, bound, (uint64_t *)c_keys, c_vals, (uint64_t)capacity, &count));
+    *nc = (int64_t)count;
+    return LNT_OK;
+  }
+  Staging st;
+  const size_t sa = (size_t)na * 8, sb = (size_t)nb * 8, sc = (size_t)(na + nb) * 8;
+  for (int i = 0; i < 2; ++i) LNT_TRY(ensure(st.b[i], sa ? sa : 8));
+  for (int i = 2; i < 4; ++i) LNT_TRY(ensure(st.b[i], sb ? sb : 8));
+  for (int i = 4; i < 6; ++i) LNT_TRY(ensure(st.b[i], sc ? sc : 8));
+  if (sa) {
+    LNT_HIP(hipMemcpyAsync(st.b[0].ptr, a_keys, sa, hipMemcpyHostToDevice, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(st.b[1].ptr, a_vals, sa, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (sb) {
+    LNT_HIP(hipMemcpyAsync(st.b[2].ptr, b_keys, sb, hipMemcpyHostToDevice, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(st.b[3].ptr, b_vals, sb, hipMemcpyHostToDevice, ctx->stream));
+  }
+  LNT_TRY(sort_merge_device(ctx, (const uint64_t *)st.b[0].ptr, (const double *)st.b[1].ptr, (uint32_t)na,
+                            (const uint64_t *)st.b[2].ptr, (const double *)st.b[3].ptr, (uint32_t)nb, op, bound,
+                            (uint64_t *)st.b[4].ptr, (double *)st.b[5].ptr, (uint64_t)(na + nb), &count));
+  if (count) {
+    LNT_HIP(hipMemcpyAsync(c_keys, st.b[4].ptr, count * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(c_vals, st.b[5].ptr, count * 8, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  *nc = (int64_t)count;
+  return LNT_OK;
+}
+
+int lnt_sort_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                   const int64_t *b_keys, const double *b_vals, int64_t nb, int op, uint64_t bound,
+                   int64_t **c_keys, double **c_vals, int64_t *nc, int mem) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  if (!c_keys || !c_vals || !nc) return fail(LNT_ERR_INVALID_ARGUMENT, "null output pointer");
+  if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  const int64_t cap = na + nb;
+  const size_t bytes = (size_t)(cap > 0 ? cap : 1) * 8;
+  void *k = nullptr, *v = nullptr;
+  if (mem == LNT_MEM_DEVICE) {
+    if (hipMalloc(&k, bytes) != hipSuccess || hipMalloc(&v, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      if (k) (void)hipFree(k);
+      return fail(LNT_ERR_OUT_OF_MEMORY, "hipMalloc of the merge result failed");
+    }
+  } else {
+    k = malloc(bytes);
+    v = malloc(bytes);
+    if (!k || !v) {
+      free(k);
+      free(v);
+      return fail(LNT_ERR_OUT_OF_MEMORY, "malloc of the merge result failed");
+    }
+  }
+  int rc = lnt_sort_merge_into(ctx, a_keys, a_vals, na, b_keys, b_vals, nb, op, bound, (int64_t *)k, (double *)v,
+                               cap, nc, mem);
   if (rc != LNT_OK) {
     if (mem == LNT_MEM_DEVICE) {
       (void)hipFree(k);

@@ -13,6 +13,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <string>
 
@@ -154,12 +155,15 @@ struct DigitScope {
   ~DigitScope() { ctx->tuning.digit_bits = saved; }
 };
 
-// Runs strategy s; the result is allocated with hipMalloc.
-int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, int s, uint64_t **ck, double **cv,
-                 uint64_t *nc, lnt_result_shape *shape) {
+// Runs strategy s.  With out_k/out_v (capacity entries) the result is
+// written there; otherwise it is allocated with hipMalloc.
+int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, int s, uint64_t *out_k,
+                 double *out_v, uint64_t capacity, uint64_t **ck, double **cv, uint64_t *nc,
+                 lnt_result_shape *shape) {
   *ck = nullptr;
   *cv = nullptr;
   *nc = 0;
+  const bool fixed = out_k != nullptr;
   const bool swap = s & LNT_STRAT_SWAP;
   const lnt_operand *f = swap ? b : a, *g = swap ? a : b;
   Plan P, P0;
@@ -171,15 +175,23 @@ int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
   LNT_TRY(unfold(ctx, g, P.seq_b, ctx->exec_buf[2], ctx->exec_buf[3]));
   const uint64_t *fk = (const uint64_t *)ctx->exec_buf[0].ptr, *gk = (const uint64_t *)ctx->exec_buf[2].ptr;
   const double *fv = (const double *)ctx->exec_buf[1].ptr, *gv = (const double *)ctx->exec_buf[3].ptr;
-  uint64_t *k = nullptr;
-  double *v = nullptr;
+  uint64_t *k = out_k;
+  double *v = out_v;
   uint64_t n = 0;
+  auto release_out = [&]() {
+    if (!fixed) {
+      if (k) (void)hipFree(k);
+      if (v) (void)hipFree(v);
+    }
+  };
   if (P.n_out_a == 0 && P.n_in == 0 && P.n_out_b == 0) {
     // Pure Hadamard (SparseMIABase.h:1717-1826): both operands now share one
     // linearisation; sorted intersection with products.
     const uint64_t cap = (uint64_t)std::min(f->nnz, g->nnz);
-    if (hipMalloc((void **)&k, (cap ? cap : 1) * 8) != hipSuccess ||
-        hipMalloc((void **)&v, (cap ? cap : 1) * 8) != hipSuccess) {
+    if (fixed) {
+      if (capacity < cap) return fail(LNT_ERR_INVALID_ARGUMENT, "output capacity below min(nnz(A), nnz(B))");
+    } else if (hipMalloc((void **)&k, (cap ? cap : 1) * 8) != hipSuccess ||
+               hipMalloc((void **)&v, (cap ? cap : 1) * 8) != hipSuccess) {
       (void)hipGetLastError();
       if (k) (void)hipFree(k);
       return fail(LNT_ERR_OUT_OF_MEMORY, "hipMalloc of the Hadamard product failed");
@@ -188,17 +200,18 @@ int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
     int rc = cap ? merge_device(ctx, fk, fv, (uint32_t)f->nnz, gk, gv, (uint32_t)g->nnz, LNT_OP_HADAMARD, k, v, cap, &c)
                  : LNT_OK;
     if (rc != LNT_OK) {
-      (void)hipFree(k);
-      (void)hipFree(v);
+      release_out();
       return rc;
     }
     n = c;
   } else {
-    int rc = lattice_mult_device(ctx, fk, fv, (uint64_t)f->nnz, P.lat_a, gk, gv, (uint64_t)g->nnz, P.lat_b,
-                                 (s & LNT_STRAT_HASH) ? LNT_SPGEMM_HASH : LNT_SPGEMM_SORT, nullptr, nullptr, 0, &n,
-                                 nullptr, &k, &v);
+    const int acc = (s & LNT_STRAT_HASH) ? LNT_SPGEMM_HASH : LNT_SPGEMM_SORT;
+    int rc = fixed ? lattice_mult_device(ctx, fk, fv, (uint64_t)f->nnz, P.lat_a, gk, gv, (uint64_t)g->nnz, P.lat_b,
+                                         acc, k, v, capacity, &n, nullptr, nullptr, nullptr)
+                   : lattice_mult_device(ctx, fk, fv, (uint64_t)f->nnz, P.lat_a, gk, gv, (uint64_t)g->nnz, P.lat_b,
+                                         acc, nullptr, nullptr, 0, &n, nullptr, &k, &v);
     if (rc != LNT_OK) return rc;
-    if (!k) {  // empty product: nothing allocated
+    if (!fixed && !k) {  // empty product: nothing allocated
       if (hipMalloc((void **)&k, 8) != hipSuccess || hipMalloc((void **)&v, 8) != hipSuccess) {
         (void)hipGetLastError();
         if (k) (void)hipFree(k);
@@ -221,8 +234,7 @@ int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
     int rc = make_relin(P.shape.order, P.shape.dims, cur, nxt, &r, &dim);
     if (rc == LNT_OK && r.active) rc = sort_device(ctx, k, v, (uint32_t)n, dim, &r);
     if (rc != LNT_OK) {
-      (void)hipFree(k);
-      (void)hipFree(v);
+      release_out();
       return rc;
     }
   }
@@ -388,7 +400,7 @@ int lnt_contract(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
   uint64_t *k;
   double *v;
   uint64_t n;
-  LNT_TRY(run_strategy(ctx, a, b, strategy, &k, &v, &n, shape));
+  LNT_TRY(run_strategy(ctx, a, b, strategy, nullptr, nullptr, 0, &k, &v, &n, shape));
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   *c_keys = (int64_t *)k;
   *c_vals = v;
@@ -404,27 +416,35 @@ int lnt_contract_time(lnt_context ctx, const lnt_operand *a, const lnt_operand *
   LNT_TRY(check_operand(b, "B"));
   LNT_HIP(hipSetDevice(ctx->device));
   for (int s = 0; s < LNT_NUM_STRATEGIES; ++s) ms[s] = -1.0;
-  bool warm = false;
+  if (!(mask & ((1 << LNT_NUM_STRATEGIES) - 1))) return LNT_OK;
+  // One allocating run gives nnz(C); every timed run then writes into one
+  // reusable buffer, so no allocation falls inside a timed region.
+  uint64_t *k;
+  double *v;
+  uint64_t n;
+  lnt_result_shape shape;
+  LNT_TRY(run_strategy(ctx, a, b, 0, nullptr, nullptr, 0, &k, &v, &n, &shape));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  (void)hipFree(k);
+  (void)hipFree(v);
+  const uint64_t cap = std::max<uint64_t>({n, (uint64_t)std::min(a->nnz, b->nnz), 1});
+  LNT_TRY(ensure(ctx->exec_buf[4], cap * 8));
+  LNT_TRY(ensure(ctx->exec_buf[5], cap * 8));
+  uint64_t *ok = (uint64_t *)ctx->exec_buf[4].ptr;
+  double *ov = (double *)ctx->exec_buf[5].ptr;
   for (int s = 0; s < LNT_NUM_STRATEGIES; ++s) {
     if (!((mask >> s) & 1)) continue;
     double best = 1e300;
-    for (int rep = warm ? 1 : 0; rep <= reps; ++rep) {  // rep 0 warms up (first strategy only)
+    for (int rep = 0; rep <= reps; ++rep) {  // rep 0 warms this strategy's workspaces up
       LNT_HIP(hipStreamSynchronize(ctx->stream));
       const auto t0 = std::chrono::steady_clock::now();
-      uint64_t *k;
-      double *v;
-      uint64_t n;
-      lnt_result_shape shape;
-      LNT_TRY(run_strategy(ctx, a, b, s, &k, &v, &n, &shape));
+      LNT_TRY(run_strategy(ctx, a, b, s, ok, ov, cap, &k, &v, &n, &shape));
       LNT_HIP(hipStreamSynchronize(ctx->stream));
       const auto t1 = std::chrono::steady_clock::now();
-      (void)hipFree(k);
-      (void)hipFree(v);
       const double t = std::chrono::duration<double, std::milli>(t1 - t0).count();
       if (rep > 0 && t < best) best = t;
     }
     ms[s] = best;
-    warm = true;
   }
   return LNT_OK;
 }

@@ -369,6 +369,7 @@ struct SortTuning {
   int spgemm_small = 2048;  // largest output column handled in LDS
   int narrow_keys = 1;  // 32-bit keys between passes when they fit
   int digit_bits = 8;  // largest radix digit per onesweep pass (4..8)
+  int sort_block = 512;  // onesweep threads per block (256 or 512): tile = sort_block * items_per_thread
 };
 
 // Kernel kinds recorded by the profiler (names in api.hip).
@@ -398,7 +399,7 @@ struct lnt_context_s {
   lnt::DeviceBuffer relin_dev;  // device copy of the current relinearisation table
   lnt::DeviceBuffer spg_ws, spg_tmp, spg_big;  // lattice-product workspaces
   lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
-  lnt::DeviceBuffer exec_buf[4];  // operand copies of a whole contraction (keys, values of both)
+  lnt::DeviceBuffer exec_buf[6];  // operand copies of a whole contraction (keys, values of both), timing output
   void *model = nullptr;          // lnt_model used by LNT_STRAT_AUTO (not owned)
   uint32_t *pinned = nullptr;   // small pinned host buffer for counts/flags
   int profiling = 0;
@@ -442,5 +443,8 @@ int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
                  uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
                  int op, uint64_t *ck, double *cv, uint64_t capacity,
                  uint32_t *nc_host);
+int sort_merge_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
+                      const double *bv, uint32_t nb, int op, uint64_t bound, uint64_t *ck, double *cv,
+                      uint64_t capacity, uint64_t *nc_host);
 
 }  // namespace lnt

@@ -19,6 +19,10 @@
 // which may belong to the previous tile).  Output counts are block-scanned and
 // the tile's output offset comes from a decoupled look-back; outputs are staged
 // in LDS and written contiguously.
+#include <string.h>
+
+#include <algorithm>
+
 #include "lnt_internal.h"
 
 namespace lnt {
@@ -241,6 +245,68 @@ int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
   return LNT_OK;
 }
 
+// ------------------------------------------------------------ sortMerge
+// SparseMIA::sortMerge (SparseMIA.h:833-887): A then B concatenated (B
+// negated, -1*b, for subtraction), sorted, then collect_duplicates
+// (SparseMIA.h:628-665).
+__global__ void concat_kernel(const uint64_t *__restrict__ ak, const double *__restrict__ av, uint32_t na,
+                              const uint64_t *__restrict__ bk, const double *__restrict__ bv, uint32_t nb,
+                              int negate_b, uint64_t *__restrict__ k, double *__restrict__ v) {
+  const uint64_t n = (uint64_t)na + nb;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < na) {
+      k[i] = ak[i];
+      v[i] = av[i];
+    } else {
+      k[i] = bk[i - na];
+      v[i] = negate_b ? -1.0 * bv[i - na] : bv[i - na];
+    }
+  }
+}
+
+// collect_duplicates on sorted keys: one thread per run of equal keys (run
+// heads).  The run's first entry is kept only if its value is non-zero (the
+// reference tests `if(this->data_at(first))` before appending a new key) --
+// except the first entry of the whole array, which is always kept; a zero
+// head is skipped and the next entry of the same key is tried as a new key;
+// entries after the kept one are added in sorted order.  keep[p] / v[p]
+// receive the run's outcome at its head; the sum overwrites v[p] in place
+// (runs are disjoint).
+__global__ void collect_kernel(const uint64_t *__restrict__ k, double *__restrict__ v, uint32_t n,
+                               uint32_t *__restrict__ keep) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = k[p];
+    if (p > 0 && k[p - 1] == key) {
+      keep[p] = 0;
+      continue;
+    }
+    uint64_t q = p;
+    if (p > 0)
+      while (q < n && k[q] == key && v[q] == 0.0) ++q;
+    if (q == n || k[q] != key) {  // every entry of the key is zero: dropped
+      keep[p] = 0;
+      continue;
+    }
+    double sum = v[q];
+    for (uint64_t r = q + 1; r < n && k[r] == key; ++r) sum = sum + v[r];
+    v[p] = sum;
+    keep[p] = 1;
+  }
+}
+
+__global__ void compact_kernel(const uint64_t *__restrict__ k, const double *__restrict__ v,
+                               const uint32_t *__restrict__ keep, const uint64_t *__restrict__ pos, uint32_t n,
+                               uint64_t *__restrict__ ck, double *__restrict__ cv, uint64_t capacity) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (uint64_t)gridDim.x * blockDim.x) {
+    if (!keep[p]) continue;
+    const uint64_t d = pos[p];
+    if (d < capacity) {
+      ck[d] = k[p];
+      cv[d] = v[p];
+    }
+  }
+}
+
 }  // namespace
 
 int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
@@ -267,6 +333,48 @@ int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
   prof_set_bytes(ctx, slot, 16.0 * ((double)na + nb + *nc_host));
   if (*nc_host > capacity)
     return fail(LNT_ERR_NOT_SORTED, "merge inputs are not sorted with unique keys (output overflow)");
+  return LNT_OK;
+}
+
+template <typename T>
+int exclusive_scan(lnt_context ctx, const T *in, uint64_t *out, uint64_t n, uint64_t *total_dev,
+                   DeviceBuffer &tmp, size_t tmp_offset);
+size_t scan_workspace_words(uint64_t n);
+
+// SparseMIA::sortMerge of two operands in one linearisation, keys < bound.
+// Writes at most capacity (>= na + nb) entries; *nc_host = nnz(C).
+int sort_merge_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
+                      const double *bv, uint32_t nb, int op, uint64_t bound, uint64_t *ck, double *cv,
+                      uint64_t capacity, uint64_t *nc_host) {
+  const uint64_t n = (uint64_t)na + nb;
+  *nc_host = 0;
+  if (n == 0) return LNT_OK;
+  if (n >= (uint64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "na + nb must be < 2^32");
+  // Work arrays: the concatenation (keys, values), keep flags, positions.
+  const size_t words = scan_workspace_words(n) + 64;
+  LNT_TRY(ensure(ctx->exec_buf[0], n * 8));
+  LNT_TRY(ensure(ctx->exec_buf[1], n * 8));
+  LNT_TRY(ensure(ctx->exec_buf[2], n * 4 + 256));
+  LNT_TRY(ensure(ctx->exec_buf[3], (n + 1) * 8 + 256));
+  LNT_TRY(ensure(ctx->aux, words * 8));
+  uint64_t *k = (uint64_t *)ctx->exec_buf[0].ptr;
+  double *v = (double *)ctx->exec_buf[1].ptr;
+  uint32_t *keep = (uint32_t *)ctx->exec_buf[2].ptr;
+  uint64_t *pos = (uint64_t *)ctx->exec_buf[3].ptr;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
+  concat_kernel<<<grid, 256, 0, ctx->stream>>>(ak, av, na, bk, bv, nb, op == LNT_OP_SUB, k, v);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(sort_device(ctx, k, v, (uint32_t)n, bound, nullptr));
+  collect_kernel<<<grid, 256, 0, ctx->stream>>>(k, v, (uint32_t)n, keep);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, keep, pos, n, pos + n, ctx->aux, 0));
+  compact_kernel<<<grid, 256, 0, ctx->stream>>>(k, v, keep, pos, (uint32_t)n, ck, cv, capacity);
+  LNT_HIP(hipGetLastError());
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 8, pos + n, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  uint64_t nc;
+  memcpy(&nc, ctx->pinned + 8, sizeof(nc));
+  *nc_host = nc;
   return LNT_OK;
 }
 

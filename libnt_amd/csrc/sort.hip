@@ -140,28 +140,49 @@ __global__ __launch_bounds__(256) void histogram_reduce_kernel(const uint32_t *_
 // KIn/KOut: key word read/written by this pass.  When the key space fits in
 // 32 bits the passes between the first and the last move 32-bit keys (24 B per
 // pair instead of 32 B); the first pass narrows, the last widens.
-template <int ITEMS, bool RELIN, typename KIn, typename KOut>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void onesweep_kernel(
+// Exclusive sum over a block of BT threads (BT/64 waves); scratch holds BT/64
+// words.  Every thread of the block must call it.
+template <int BT>
+__device__ inline uint32_t block_exclusive_sum_bt(uint32_t v, uint32_t *scratch, uint32_t *total) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_inclusive_sum(v);
+  if (lane == 63) scratch[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < BT / kWave; ++w) {
+    const uint32_t x = scratch[w];
+    before += (uint32_t)w < wave ? x : 0;
+    all += x;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+template <int BT, int ITEMS, bool RELIN, typename KIn, typename KOut>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void onesweep_kernel(
     const KIn *__restrict__ kin, const double *__restrict__ vin,
     KOut *__restrict__ kout, double *__restrict__ vout, uint32_t n,
     uint32_t shift, uint32_t dbits, const uint32_t *__restrict__ pass_hist,
     uint64_t *__restrict__ status, uint32_t epoch,
     uint32_t *__restrict__ tile_counter, const RelinDev *__restrict__ relin) {
-  constexpr int kTile = kBlock * ITEMS;
-  constexpr int kWaves = kBlock / kWave;
+  static_assert(BT >= kRadix && BT % kWave == 0, "one thread per digit at least");
+  constexpr int kTile = BT * ITEMS;
+  constexpr int kWaves = BT / kWave;
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wcount[kWaves][kRadix];
   __shared__ uint32_t s_lstart[kRadix];
   __shared__ uint32_t s_gofs[kRadix];
-  __shared__ uint32_t s_scan[4];
+  __shared__ uint32_t s_scan[kWaves];
   __shared__ uint64_t s_stage[kTile];
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool digit_thread = tid < kRadix;  // one thread per digit for the per-digit steps
   if (tid == 0)
     s_tile = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-  for (int w = 0; w < kWaves; ++w) s_wcount[w][tid] = 0;
+  for (uint32_t i = tid; i < (uint32_t)(kWaves * kRadix); i += BT) (&s_wcount[0][0])[i] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint64_t tile_base = (uint64_t)tile * kTile;
@@ -216,28 +237,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 
   // Per digit (one thread each): prefix over waves, then over digits.
   uint32_t count = 0;
+  if (digit_thread) {
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) {
-    uint32_t c = s_wcount[w][tid];
-    s_wcount[w][tid] = count;
-    count += c;
+    for (int w = 0; w < kWaves; ++w) {
+      uint32_t c = s_wcount[w][tid];
+      s_wcount[w][tid] = count;
+      count += c;
+    }
   }
   uint32_t total;
-  uint32_t lstart = block256_exclusive_sum(count, s_scan, &total);
-  s_lstart[tid] = lstart;
-
-  uint32_t excl;
-  uint64_t *my_status = status + (uint64_t)tile * kRadix + tid;
-  if (tile == 0) {
-    uint32_t all;
-    excl = block256_exclusive_sum(pass_hist[tid], s_scan, &all);
-    status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
-  } else {
-    status_store(my_status, status_word(kFlagAggregate, epoch, count));
-    excl = lookback_batched<4>(status, tile, kRadix, tid, epoch);
-    status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
+  const uint32_t lstart = block_exclusive_sum_bt<BT>(count, s_scan, &total);
+  uint32_t hist0 = 0;
+  if (tile == 0 && digit_thread) hist0 = pass_hist[tid];
+  uint32_t all;
+  const uint32_t excl0 = tile == 0 ? block_exclusive_sum_bt<BT>(hist0, s_scan, &all) : 0;
+  if (digit_thread) {
+    s_lstart[tid] = lstart;
+    uint32_t excl;
+    uint64_t *my_status = status + (uint64_t)tile * kRadix + tid;
+    if (tile == 0) {
+      excl = excl0;
+      status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
+    } else {
+      status_store(my_status, status_word(kFlagAggregate, epoch, count));
+      excl = lookback_batched<4>(status, tile, kRadix, tid, epoch);
+      status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
+    }
+    s_gofs[tid] = excl - lstart;
   }
-  s_gofs[tid] = excl - lstart;
   __syncthreads();
 
   // Keys: stage in digit order, then store.
@@ -259,7 +286,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   uint32_t dst[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    uint32_t p = (uint32_t)i * kBlock + tid;
+    uint32_t p = (uint32_t)i * BT + tid;
     dst[i] = 0xFFFFFFFFu;
     if (p < tile_n) {
       uint64_t key = s_stage[p];
@@ -277,7 +304,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    uint32_t p = (uint32_t)i * kBlock + tid;
+    uint32_t p = (uint32_t)i * BT + tid;
     if (p < tile_n && dst[i] < n) vout[dst[i]] = s_vstage[p];
   }
 }
@@ -307,20 +334,20 @@ __global__ void relin_kernel(uint64_t *keys, uint32_t n, const RelinDev *__restr
 
 enum KeyWidth { kW64 = 0, kW32 = 1 };
 
-template <int ITEMS, bool RELIN, typename KIn, typename KOut>
+template <int BT, int ITEMS, bool RELIN, typename KIn, typename KOut>
 void launch_one(lnt_context ctx, uint32_t tiles, const void *kin, const double *vin, void *kout, double *vout,
                 uint32_t n, uint32_t shift, uint32_t dbits, const uint32_t *pass_hist, uint64_t *status,
                 uint32_t epoch, uint32_t *counter, const RelinDev *r) {
-  onesweep_kernel<ITEMS, RELIN, KIn, KOut><<<tiles, kBlock, 0, ctx->stream>>>(
+  onesweep_kernel<BT, ITEMS, RELIN, KIn, KOut><<<tiles, BT, 0, ctx->stream>>>(
       (const KIn *)kin, vin, (KOut *)kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
 }
 
-template <int ITEMS>
+template <int BT, int ITEMS>
 int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
                 const double *vin, void *kout, double *vout, uint32_t n,
                 uint32_t shift, uint32_t dbits, const uint32_t *pass_hist,
                 uint32_t *counter, const RelinDev *r) {
-  constexpr uint32_t tile = kBlock * ITEMS;
+  constexpr uint32_t tile = BT * ITEMS;
   uint32_t tiles = (n + tile - 1) / tile;
   LNT_TRY(ensure_status(ctx, (size_t)tiles * kRadix));
   uint32_t epoch;
@@ -338,12 +365,12 @@ int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
   int slot = prof_begin(ctx, kProfOnesweep, 32.0 * n);
   const int sel = (relin ? 4 : 0) + win * 2 + wout;
   switch (sel) {
-    case 0: launch_one<ITEMS, false, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 1: launch_one<ITEMS, false, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 2: launch_one<ITEMS, false, uint32_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 3: launch_one<ITEMS, false, uint32_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 4: launch_one<ITEMS, true, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 5: launch_one<ITEMS, true, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 0: launch_one<BT, ITEMS, false, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 1: launch_one<BT, ITEMS, false, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 2: launch_one<BT, ITEMS, false, uint32_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 3: launch_one<BT, ITEMS, false, uint32_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 4: launch_one<BT, ITEMS, true, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 5: launch_one<BT, ITEMS, true, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
     default: return fail(LNT_ERR_INVALID_ARGUMENT, "internal: unsupported key widths");
   }
   prof_end(ctx, slot);
@@ -496,20 +523,19 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
     const int win = (narrow && p > 0) ? kW32 : kW64;
     const int wout = (narrow && p < passes - 1) ? kW32 : kW64;
     int rc;
-    switch (ctx->tuning.items_per_thread) {
-      case 8:
-        rc = launch_pass<8>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
-                            plan.bits[p], hist + p * kRadix, counters + p, rdev);
-        break;
-      case 12:
-        rc = launch_pass<12>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
-                             plan.bits[p], hist + p * kRadix, counters + p, rdev);
-        break;
-      default:
-        rc = launch_pass<16>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p],
-                             plan.bits[p], hist + p * kRadix, counters + p, rdev);
-        break;
+    const int cfg = ctx->tuning.sort_block == 512 ? 100 + ctx->tuning.items_per_thread : ctx->tuning.items_per_thread;
+#define LNT_PASS(BT, IT)                                                                                     \
+  launch_pass<BT, IT>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p], \
+                      plan.bits[p], hist + p * kRadix, counters + p, rdev)
+    switch (cfg) {
+      case 8: rc = LNT_PASS(256, 8); break;
+      case 12: rc = LNT_PASS(256, 12); break;
+      case 108: rc = LNT_PASS(512, 8); break;
+      case 112: rc = LNT_PASS(512, 12); break;
+      case 116: rc = LNT_PASS(512, 16); break;
+      default: rc = LNT_PASS(256, 16); break;
     }
+#undef LNT_PASS
     if (rc != LNT_OK) return rc;
     src = dst;
   }

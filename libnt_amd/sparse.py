@@ -126,7 +126,8 @@ class Atom:
 
     # ------------------------------------------------------------- merges
     def _merge(self, other: "Atom", op: int) -> SparseMIA:
-        """outside_merge -> outside_scanMerge (SparseMIABase.h:1357-1455)."""
+        """outside_merge (SparseMIABase.h:1357-1455): outside_scanMerge when
+        either operand is sorted, sortMerge when neither is."""
         a, b = self.mia, other.mia
         if sorted(self.labels) != sorted(other.labels) or any(c.isupper() for c in self.labels):
             raise ValueError("merge operands must carry the same plain labels")
@@ -148,11 +149,20 @@ class Atom:
         elif a.is_sorted:
             b.sort(b_in_a_order)
         else:
-            # Neither sorted: the reference concatenates and sorts (SparseMIA::
-            # sortMerge, SparseMIA.h:833-887); here both are sorted in A's
-            # order and scan-merged.
-            a.sort(a.seq)
-            b.sort(b_in_a_order)
+            # Neither sorted: C = copy of A, C.sortMerge(B) (SparseMIABase.h:
+            # 1393-1398, SparseMIA.h:833-887): the smaller operand is
+            # relinearised into the other's sequence, then concatenation, sort
+            # and collect_duplicates (keys whose entries are all zero vanish).
+            if a.nnz > b.nnz:
+                b.change_linIdx_sequence(b_in_a_order)
+                ak, seq = a.keys, a.seq
+            else:
+                seq = [lhs_order[r] for r in b.seq]
+                c = a.copy()
+                c.change_linIdx_sequence(seq)
+                ak = c.keys
+            ck, cv = a.ctx.sort_merge(ak, a.vals, b.keys, b.vals, op, a.dimensionality)
+            return SparseMIA(a.dims, ck.clone(), cv.clone(), seq, True, a.ctx)
         ck, cv = a.ctx.merge(a.keys, a.vals, b.keys, b.vals, op)
         return SparseMIA(a.dims, ck.clone(), cv.clone(), a.seq, True, a.ctx)
 

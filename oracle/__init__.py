@@ -44,6 +44,8 @@ def lib():
         u64, i64, f64, i32 = ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_int
         sig = {
             "lnto_mix64": (u64, [u64]),
+            "lnto_collect_duplicates_member": (i64, [_i64p, _f64p, i64]),
+            "lnto_sort_merge": (i64, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, u64, _i64p, _f64p]),
             "lnto_mlp_param_count": (i64, []),
             "lnto_mlp_weight_count": (i64, []),
             "lnto_mlp_forward": (None, [_f64p, _f64p, _f64p]),
@@ -309,3 +311,20 @@ def mlp_loss_grad(params, X, T):
     g = np.zeros(mlp_weight_count())
     loss = lib().lnto_mlp_loss_grad(_p(P, _f64p), _p(X, _f64p), _p(T, _f64p), X.shape[0], _p(g, _f64p))
     return float(loss), g
+
+
+def collect_duplicates_member(keys, vals):
+    """SparseMIA::collect_duplicates with plus on sorted keys (zero first entries dropped)."""
+    k, v = _i64(keys).copy(), _f64(vals).copy()
+    n = lib().lnto_collect_duplicates_member(_p(k, _i64p), _p(v, _f64p), len(k))
+    return k[:n], v[:n]
+
+
+def sort_merge(ak, av, bk, bv, op, bound):
+    """SparseMIA::sortMerge of operands in one linearisation; returns (keys, values)."""
+    ak, av, bk, bv = _i64(ak), _f64(av), _i64(bk), _f64(bv)
+    ck = np.empty(len(ak) + len(bk), dtype=np.int64)
+    cv = np.empty(len(ak) + len(bk))
+    n = lib().lnto_sort_merge(_p(ak, _i64p), _p(av, _f64p), len(ak), _p(bk, _i64p), _p(bv, _f64p), len(bk), op,
+                              int(bound), _p(ck, _i64p), _p(cv, _f64p))
+    return ck[:n], cv[:n]

@@ -457,6 +457,38 @@ int64_t lnto_collect_duplicates(int64_t *keys, double *vals, int64_t n) {
   return r + 1;
 }
 
+int64_t lnto_collect_duplicates_member(int64_t *keys, double *vals, int64_t n) {
+  if (n <= 0) return 0;
+  int64_t r = 0;
+  for (int64_t f = 1; f < n; ++f) {
+    if (keys[r] != keys[f]) {
+      if (vals[f] != 0.0) {  /* `if(this->data_at(first))` */
+        ++r;
+        keys[r] = keys[f];
+        vals[r] = vals[f];
+      }
+    } else {
+      vals[r] = vals[r] + vals[f];
+    }
+  }
+  return r + 1;
+}
+
+int64_t lnto_sort_merge(const int64_t *ak, const double *av, int64_t na, const int64_t *bk,
+                        const double *bv, int64_t nb, int op, uint64_t bound, int64_t *ck,
+                        double *cv) {
+  for (int64_t i = 0; i < na; ++i) {
+    ck[i] = ak[i];
+    cv[i] = av[i];
+  }
+  for (int64_t i = 0; i < nb; ++i) {
+    ck[na + i] = bk[i];
+    cv[na + i] = op == LNTO_OP_SUB ? -1 * bv[i] : bv[i];
+  }
+  lnto_radix_sort_pairs(ck, cv, na + nb, bound);
+  return lnto_collect_duplicates_member(ck, cv, na + nb);
+}
+
 /* ================================= merging ================================ */
 
 static inline double merge_op(int op, double a, double b) {

@@ -63,6 +63,18 @@ void lnto_sort_seq(int64_t *keys, double *vals, int64_t n, int order,
                    const int64_t *dims, const int32_t *cur_seq,
                    const int32_t *new_seq, int is_sorted);
 int64_t lnto_collect_duplicates(int64_t *keys, double *vals, int64_t n);
+/* SparseMIA::collect_duplicates(Collector = plus) (SparseMIA.h:628-665) on
+ * sorted keys: a key's first entry is kept only if its value is non-zero
+ * (the array's first entry always), later entries of the kept key are added
+ * to it.  Returns the new length. */
+int64_t lnto_collect_duplicates_member(int64_t *keys, double *vals, int64_t n);
+/* SparseMIA::sortMerge (SparseMIA.h:833-887) on operands already in one
+ * linearisation: concatenate A then B (B negated, -1*b, for LNTO_OP_SUB),
+ * sort (the reference sort of sort()), collect_duplicates_member.  ck/cv hold
+ * na+nb entries; returns nnz(C). */
+int64_t lnto_sort_merge(const int64_t *ak, const double *av, int64_t na, const int64_t *bk,
+                        const double *bv, int64_t nb, int op, uint64_t bound, int64_t *ck,
+                        double *cv);
 
 /* -------------------------------- merging ------------------------------- */
 enum { LNTO_OP_ADD = 0, LNTO_OP_SUB = 1, LNTO_OP_HADAMARD = 2 };

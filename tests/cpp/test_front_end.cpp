@@ -78,6 +78,23 @@ int main() {
       CHECK(dc2[q] == da[q] - bv, "A-B mismatch");
     }
   }
+  // ---- neither operand sorted: sortMerge, whose collect_duplicates drops
+  // keys whose entries are all zero (SparseMIA.h:628-665, 833-887)
+  {
+    lnt::SparseMIA<double, 2> a(4, 4), b(4, 4), c(4, 4);
+    a.push_back(1.0, 5);
+    a.push_back(0.0, 9);
+    a.push_back(2.0, 3);
+    b.push_back(-2.0, 3);
+    b.push_back(0.0, 11);
+    b.push_back(4.0, 1);
+    c(i, j) = a(i, j) + b(i, j);
+    CHECK(c.size() == 3, "sortMerge result size");
+    if (c.size() == 3) {
+      CHECK(c.indices()[0] == 1 && c.indices()[1] == 3 && c.indices()[2] == 5, "sortMerge keys");
+      CHECK(c.data()[0] == 4.0 && c.data()[1] == 0.0 && c.data()[2] == 1.0, "sortMerge values");
+    }
+  }
   // ---- C(i,j,m,n) = A(i,j,k,l) * B(k,l,m,n)  (sparse_mia_mult_test.cpp)
   {
     lnt::SparseMIA<double, 4> a(5, 4, 6, 3), b(6, 3, 4, 5), c(5, 4, 4, 5);

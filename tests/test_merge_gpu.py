@@ -158,3 +158,72 @@ def test_merge_capacity_and_sortedness_errors(ctx, orc):
         assert e.value.status == 4
     finally:
         ctx.set_validation(1)
+
+
+# ------------------------------------------------ sortMerge (unsorted operands)
+def _unsorted_pair_with_zeros(orc, na, nb, overlap, domain, seed, zero_frac):
+    keys = orc.gen_distinct_keys(na + nb - overlap, domain, seed)
+    ak = keys[:na].copy()
+    bk = np.concatenate([keys[:overlap], keys[na:]])
+    rng = np.random.default_rng(seed)
+    bk = rng.permutation(bk)
+    av = orc.gen_uniform_vals(na, seed + 1)
+    bv = orc.gen_uniform_vals(len(bk), seed + 2)
+    av[rng.random(na) < zero_frac] = 0.0
+    bv[rng.random(len(bk)) < zero_frac] = -0.0
+    return ak, av, bk, bv
+
+
+@pytest.mark.parametrize("op", [0, 1])
+@pytest.mark.parametrize("na,nb,ov,zf", [(1, 1, 1, 1.0), (1000, 700, 300, 0.3), (50000, 80000, 20000, 0.05),
+                                         (3, 0, 0, 0.5), (0, 5, 0, 0.5), (200000, 200000, 200000, 0.5)])
+def test_sort_merge_matches_oracle(ctx, orc, op, na, nb, ov, zf):
+    """Unsorted operands with planted zeros (+0 and -0): keys whose entries are
+    all zero vanish, the smallest key always stays (SparseMIA.h:628-665)."""
+    ak, av, bk, bv = _unsorted_pair_with_zeros(orc, na, nb, ov, 1 << 30, na + 7 * nb, zf)
+    ek, ev = orc.sort_merge(ak, av, bk, bv, op, 1 << 30)
+    ck, cv = ctx.sort_merge(gpu(ak), gpu(av), gpu(bk), gpu(bv), op, 1 << 30)
+    assert np.array_equal(host(ck), ek)
+    assert _bits_equal(host(cv), ev)
+    k2, v2 = ctx.sort_merge(ak, av, bk, bv, op, 1 << 30)  # host memory
+    assert np.array_equal(k2, ek) and _bits_equal(v2, ev)
+
+
+def test_sort_merge_zero_rules(ctx, orc):
+    """Hand-made cases of collect_duplicates' rule: A={1:0, 3:1, 5:2, 7:0},
+    B={1:2, 4:0, 5:-2, 7:0}.  Key 1 is the smallest: kept (0 + 2); 4 (only a
+    zero) and 7 (zeros only) vanish; 5 sums to an explicit 0 and stays."""
+    ak = np.array([3, 1, 7, 5], dtype=np.int64)
+    av = np.array([1.0, 0.0, 0.0, 2.0])
+    bk = np.array([5, 1, 7, 4], dtype=np.int64)
+    bv = np.array([-2.0, 2.0, -0.0, 0.0])
+    ck, cv = ctx.sort_merge(gpu(ak), gpu(av), gpu(bk), gpu(bv), 0, 8)
+    assert host(ck).tolist() == [1, 3, 5] and host(cv).tolist() == [2.0, 1.0, 0.0]
+    # the smallest key stays even when all its entries are zero
+    ck, cv = ctx.sort_merge(gpu(np.array([0, 6])), gpu(np.array([0.0, 3.0])), gpu(np.array([0])),
+                            gpu(np.array([0.0])), 0, 8)
+    assert host(ck).tolist() == [0, 6] and host(cv).tolist() == [0.0, 3.0]
+    # subtraction negates B (-1*b): 3 - 3 = 0 kept, B-only 4 -> -5
+    ck, cv = ctx.sort_merge(gpu(np.array([2, 3])), gpu(np.array([1.0, 3.0])), gpu(np.array([4, 3])),
+                            gpu(np.array([5.0, 3.0])), 1, 8)
+    assert host(ck).tolist() == [2, 3, 4] and host(cv).tolist() == [1.0, 0.0, -5.0]
+
+
+def test_front_end_sort_merge_sequence(orc):
+    """C = A(i,j,k) + B(k,i,j) with neither sorted: the smaller operand is
+    relinearised into the other's sequence, C is sorted in that sequence
+    (SparseMIA.h:836-862), and C equals dense arithmetic."""
+    from libnt_amd.sparse import SparseMIA
+
+    dims_a, dims_b = [7, 6, 5], [5, 7, 6]
+    for na, nb in ((150, 60), (40, 170)):
+        ak = orc.gen_distinct_keys(na, 210, na)
+        bk = orc.gen_distinct_keys(nb, 210, nb)
+        A = SparseMIA(dims_a, ak, orc.gen_uniform_vals(na, 1), seq=[2, 0, 1])
+        B = SparseMIA(dims_b, bk, orc.gen_uniform_vals(nb, 2), seq=[1, 2, 0])
+        Ad, Bd = A.to_dense(), B.to_dense()
+        C = A("ijk") + B("kij")
+        # B's sequence [1,2,0] over (k,i,j) is (i,j,k): A's modes [0,1,2]
+        want_seq = [2, 0, 1] if na > nb else [0, 1, 2]
+        assert C.seq == want_seq and C.is_sorted
+        assert np.array_equal(C.to_dense(), Ad + Bd.transpose(1, 2, 0))

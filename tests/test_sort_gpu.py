@@ -260,3 +260,39 @@ def test_change_sort_order_large_n_against_closed_form(ctx):
     ctx.change_sort_order(k, v, [256] * 4, [0, 1, 2, 3], perm, False)
     assert np.array_equal(host(k), ek)
     assert _bits_equal(host(v), ev)
+
+
+@pytest.mark.parametrize("block,items", [(256, 8), (256, 12), (256, 16), (512, 8), (512, 12), (512, 16)])
+def test_sort_configurations_match_oracle(ctx, orc, block, items):
+    """Every onesweep configuration (threads per block x items per thread,
+    digit widths 8 and 6, narrow keys on and off) gives the oracle's result,
+    for sizes around one and several tiles."""
+    ctx.set_param("sort_block", block)
+    ctx.set_param("sort_items", items)
+    try:
+        tile = block * items
+        for n in (1, tile - 1, tile, tile + 1, 3 * tile + 17, 250000):
+            for bound in (1 << 20, (1 << 40) + 3):
+                for digits in (8, 6):
+                    for narrow in (1, 0):
+                        ctx.set_param("digit_bits", digits)
+                        ctx.set_param("narrow_keys", narrow)
+                        keys = orc.gen_distinct_keys(min(n, bound), bound, n + digits)
+                        vals = orc.gen_uniform_vals(len(keys), n)
+                        ek, ev = orc.radix_sort_pairs(keys, vals, bound)
+                        k, v = gpu(keys), gpu(vals)
+                        ctx.sort_pairs(k, v, bound)
+                        assert np.array_equal(host(k), ek), (n, bound, digits, narrow)
+                        assert _bits_equal(host(v), ev)
+        dims, ident, perm = [64, 32, 128, 16], [0, 1, 2, 3], [2, 0, 3, 1]
+        keys = orc.gen_distinct_keys(300000, 64 * 32 * 128 * 16, 5)
+        vals = orc.gen_uniform_vals(300000, 6)
+        ek, ev = orc.sort_seq(keys, vals, dims, ident, perm, False)
+        k, v = gpu(keys), gpu(vals)
+        ctx.change_sort_order(k, v, dims, ident, perm, False)
+        assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
+    finally:
+        ctx.set_param("sort_block", 512)
+        ctx.set_param("sort_items", 16)
+        ctx.set_param("digit_bits", 8)
+        ctx.set_param("narrow_keys", 1)
