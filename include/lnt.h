@@ -88,8 +88,9 @@ int lnt_abi_version(void);
  * reference counterpart (the reference has only ad-hoc std::chrono prints,
  * multi_index_array/SparseLatticeBase.h:1885,1981-1982). */
 /* Tuning parameters (the autotuner's knobs; results never depend on them):
- *   "sort_items"   onesweep items per thread: 8, 12 or 16 (tile = sort_block*items)
- *   "sort_block"   onesweep threads per workgroup: 256 or 512 (default 512)
+ *   "sort_items"   onesweep items per thread: 8, 12 (default) or 16 (tile = sort_block*items)
+ *   "sort_block"   onesweep threads per workgroup: 256, 512 or 1024 (default;
+ *                  1024 takes at most 12 items per thread)
  *   "digit_bits"   widest radix digit per onesweep pass, 4..8 (default 8)
  *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096
  *   "merge_items"  merge-path steps per thread: 8 or 16 (tile = 256*items)

@@ -19,8 +19,8 @@ from .batch import Batch, packed_offsets
 
 SPACE = {
     "digit_bits": [8, 7, 6],
-    "sort_block": [512, 256],
-    "sort_items": [16, 12, 8],
+    "sort_block": [1024, 512, 256],
+    "sort_items": [12, 16, 8],
     "small_max": [1024, 512, 256],
     "item_products": [1024, 512, 256],
     "window": [1280, 1024, 768, 1536],

@@ -277,7 +277,7 @@ int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
   else if (n == "spgemm_small" && in(2, 2048)) ctx->tuning.spgemm_small = (int)value;
   else if (n == "narrow_keys" && in(0, 1)) ctx->tuning.narrow_keys = (int)value;
   else if (n == "digit_bits" && in(4, 8)) ctx->tuning.digit_bits = (int)value;
-  else if (n == "sort_block" && (value == 256 || value == 512)) ctx->tuning.sort_block = (int)value;
+  else if (n == "sort_block" && (value == 256 || value == 512 || value == 1024)) ctx->tuning.sort_block = (int)value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter or value out of range: " + n);
   return LNT_OK;
 }

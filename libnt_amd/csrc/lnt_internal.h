@@ -363,13 +363,13 @@ void set_alloc_forbidden(bool forbidden);
 void release(DeviceBuffer &b);
 
 struct SortTuning {
-  int items_per_thread = 16;  // onesweep tile = 256 * items
+  int items_per_thread = 12;  // onesweep tile = sort_block * items
   int hist_blocks = 1024;
   int merge_items = 16;  // merge tile = 256 * items
   int spgemm_small = 2048;  // largest output column handled in LDS
   int narrow_keys = 1;  // 32-bit keys between passes when they fit
   int digit_bits = 8;  // largest radix digit per onesweep pass (4..8)
-  int sort_block = 512;  // onesweep threads per block (256 or 512): tile = sort_block * items_per_thread
+  int sort_block = 1024;  // onesweep threads per block (256, 512, 1024): tile = sort_block * items
 };
 
 // Kernel kinds recorded by the profiler (names in api.hip).

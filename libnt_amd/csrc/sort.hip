@@ -523,7 +523,9 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
     const int win = (narrow && p > 0) ? kW32 : kW64;
     const int wout = (narrow && p < passes - 1) ? kW32 : kW64;
     int rc;
-    const int cfg = ctx->tuning.sort_block == 512 ? 100 + ctx->tuning.items_per_thread : ctx->tuning.items_per_thread;
+    const int cfg = ctx->tuning.sort_block == 1024  ? 200 + ctx->tuning.items_per_thread
+                    : ctx->tuning.sort_block == 512 ? 100 + ctx->tuning.items_per_thread
+                                                    : ctx->tuning.items_per_thread;
 #define LNT_PASS(BT, IT)                                                                                     \
   launch_pass<BT, IT>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p], \
                       plan.bits[p], hist + p * kRadix, counters + p, rdev)
@@ -533,6 +535,9 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
       case 108: rc = LNT_PASS(512, 8); break;
       case 112: rc = LNT_PASS(512, 12); break;
       case 116: rc = LNT_PASS(512, 16); break;
+      case 208: rc = LNT_PASS(1024, 8); break;
+      case 212: rc = LNT_PASS(1024, 12); break;
+      case 216: rc = LNT_PASS(1024, 12); break;  // 1024x16 exceeds the LDS of one CU
       default: rc = LNT_PASS(256, 16); break;
     }
 #undef LNT_PASS

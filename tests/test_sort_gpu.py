@@ -262,7 +262,8 @@ def test_change_sort_order_large_n_against_closed_form(ctx):
     assert _bits_equal(host(v), ev)
 
 
-@pytest.mark.parametrize("block,items", [(256, 8), (256, 12), (256, 16), (512, 8), (512, 12), (512, 16)])
+@pytest.mark.parametrize("block,items", [(256, 8), (256, 12), (256, 16), (512, 8), (512, 12), (512, 16),
+                                         (1024, 8), (1024, 12)])
 def test_sort_configurations_match_oracle(ctx, orc, block, items):
     """Every onesweep configuration (threads per block x items per thread,
     digit widths 8 and 6, narrow keys on and off) gives the oracle's result,
@@ -292,7 +293,7 @@ def test_sort_configurations_match_oracle(ctx, orc, block, items):
         ctx.change_sort_order(k, v, dims, ident, perm, False)
         assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
     finally:
-        ctx.set_param("sort_block", 512)
-        ctx.set_param("sort_items", 16)
+        ctx.set_param("sort_block", 1024)
+        ctx.set_param("sort_items", 12)
         ctx.set_param("digit_bits", 8)
         ctx.set_param("narrow_keys", 1)
