@@ -93,7 +93,9 @@ int lnt_abi_version(void);
  *                  1024 takes at most 12 items per thread)
  *   "digit_bits"   widest radix digit per onesweep pass, 4..8 (default 8)
  *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096
- *   "merge_items"  merge-path steps per thread: 8 or 16 (tile = 256*items)
+ *   "merge_items"  merge-path steps per thread: 8, 12 (default) or 16
+ *                  (tile = merge_block*items)
+ *   "merge_block"  merge threads per workgroup: 256 or 512 (default)
  *   "spgemm_small" longest output column (products) kept in LDS, 2..2048
  *   "narrow_keys"  1: keys of at most 32 bits move as 32-bit words between
  *                  radix passes (default); 0: always 64-bit

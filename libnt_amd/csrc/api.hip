@@ -234,6 +234,10 @@ int lnt_context_create(lnt_context *out, int device, void *stream) {
   if (!ctx) return fail(LNT_ERR_OUT_OF_MEMORY, "context allocation failed");
   ctx->device = device;
   ctx->stream = (hipStream_t)stream;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    ctx->cus = cus;
+  (void)hipGetLastError();
   if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
     delete ctx;
     return fail(LNT_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
@@ -273,7 +277,8 @@ int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
   std::string n(name);
   if (n == "sort_items" && (value == 8 || value == 12 || value == 16)) ctx->tuning.items_per_thread = (int)value;
   else if (n == "hist_blocks" && in(1, kMaxHistBlocks)) ctx->tuning.hist_blocks = (int)value;
-  else if (n == "merge_items" && (value == 8 || value == 16)) ctx->tuning.merge_items = (int)value;
+  else if (n == "merge_items" && (value == 8 || value == 12 || value == 16)) ctx->tuning.merge_items = (int)value;
+  else if (n == "merge_block" && (value == 256 || value == 512)) ctx->tuning.merge_block = (int)value;
   else if (n == "spgemm_small" && in(2, 2048)) ctx->tuning.spgemm_small = (int)value;
   else if (n == "narrow_keys" && in(0, 1)) ctx->tuning.narrow_keys = (int)value;
   else if (n == "digit_bits" && in(4, 8)) ctx->tuning.digit_bits = (int)value;
@@ -288,6 +293,7 @@ int lnt_context_get_param(lnt_context ctx, const char *name, int64_t *value) {
   if (n == "sort_items") *value = ctx->tuning.items_per_thread;
   else if (n == "hist_blocks") *value = ctx->tuning.hist_blocks;
   else if (n == "merge_items") *value = ctx->tuning.merge_items;
+  else if (n == "merge_block") *value = ctx->tuning.merge_block;
   else if (n == "spgemm_small") *value = ctx->tuning.spgemm_small;
   else if (n == "narrow_keys") *value = ctx->tuning.narrow_keys;
   else if (n == "digit_bits") *value = ctx->tuning.digit_bits;

@@ -320,6 +320,26 @@ __device__ inline uint32_t block256_exclusive_sum(uint32_t v, uint32_t *scratch,
   return before + incl - v;
 }
 
+// Exclusive sum over a block of BT threads (BT/64 waves); scratch holds BT/64
+// words.  Every thread of the block must call it.
+template <int BT>
+__device__ inline uint32_t block_exclusive_sum_bt(uint32_t v, uint32_t *scratch, uint32_t *total) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_inclusive_sum(v);
+  if (lane == 63) scratch[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < BT / kWave; ++w) {
+    const uint32_t x = scratch[w];
+    before += (uint32_t)w < wave ? x : 0;
+    all += x;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
 __device__ inline uint64_t wave_inclusive_sum64(uint64_t v) {
   const uint32_t lane = lane_id();
 #pragma unroll
@@ -365,7 +385,8 @@ void release(DeviceBuffer &b);
 struct SortTuning {
   int items_per_thread = 12;  // onesweep tile = sort_block * items
   int hist_blocks = 1024;
-  int merge_items = 16;  // merge tile = 256 * items
+  int merge_items = 12;   // merge tile = merge_block * items
+  int merge_block = 512;  // merge threads per workgroup (256 or 512)
   int spgemm_small = 2048;  // largest output column handled in LDS
   int narrow_keys = 1;  // 32-bit keys between passes when they fit
   int digit_bits = 8;  // largest radix digit per onesweep pass (4..8)
@@ -389,6 +410,7 @@ struct lnt_context_s {
   hipStream_t stream = nullptr;
   int validate = 1;
   int capturing = 0;  // launches are being recorded into a graph: no host syncs
+  int cus = 256;      // compute units of the device (persistent grids)
   uint32_t epoch = 0;
   lnt::SortTuning tuning;
   lnt::DeviceBuffer alt_keys, alt_vals, alt2_keys, alt2_vals;

@@ -10,15 +10,22 @@
 // Each value is a single IEEE operation, so results are bit-identical.
 //
 // One kernel, one pass over the inputs.  The merge path of A and B (ties: A
-// first) is cut into tiles of 256*ITEMS steps; a tile finds its end points by
-// binary search on its two diagonals, loads its slices into LDS, and each
-// thread merges ITEMS consecutive steps.  Because ties put A first, a key
-// present in both inputs appears as A[i] immediately followed by B[j]: the A
-// step emits the combined value (peeking at the next B key, which may belong to
-// the next tile) and the B step is dropped (peeking at the previous A key,
-// which may belong to the previous tile).  Output counts are block-scanned and
-// the tile's output offset comes from a decoupled look-back; outputs are staged
-// in LDS and written contiguously.
+// first) is cut into tiles of BT*ITEMS steps; a tile finds its end points in
+// the splits precomputed by partition_kernel and loads only the KEYS of its two
+// slices into LDS.  Each thread merges ITEMS consecutive steps with the two
+// current keys held in registers (one LDS load per step) and records, per step,
+// a descriptor: which A entry and/or which B entry produces the output.
+// Because ties put A first, a key present in both inputs appears as A[i]
+// immediately followed by B[j]: the A step emits the combined entry (peeking
+// at the next B key, which may belong to the next tile) and the B step is
+// dropped (comparing with the last A key, which may belong to the previous
+// tile).  Output counts are block-scanned, the tile's output offset comes from
+// a decoupled look-back, and the emitted descriptors are compacted in LDS.
+// The write phase runs in output order: thread x of the block handles outputs
+// x, x+BT, ...; it reads the key from LDS and gathers the value(s) from HBM --
+// neighbouring lanes read neighbouring entries of A and B, so the value reads
+// coalesce and every value is read once (a Hadamard tile reads only the values
+// of matched keys).
 #include <string.h>
 
 #include <algorithm>
@@ -29,7 +36,6 @@ namespace lnt {
 
 namespace {
 
-constexpr int kBlock = 256;
 
 // Number of A elements among the first `diag` steps of the merge path.
 __device__ inline uint32_t path_search(const uint64_t *a, uint32_t na,
@@ -60,77 +66,82 @@ __global__ void partition_kernel(const uint64_t *__restrict__ a, uint32_t na,
   splits[t] = path_search(a, na, b, nb, diag);
 }
 
-template <int ITEMS>
-__global__ __launch_bounds__(kBlock) void merge_kernel(
+// Step descriptor: A entry index [14:0], B entry index [29:15], flags.
+constexpr uint32_t kDescA = 1u << 31;
+constexpr uint32_t kDescB = 1u << 30;
+constexpr uint32_t kDescIdx = 0x7FFFu;
+
+// Keys of one tile's A slice then B slice into registers: item u of thread
+// tid is merge-path step u*BT + tid of the tile.  All loads are issued before
+// any is used; a tile holds at least one step, so clamped loads stay inside it.
+template <int BT, int ITEMS>
+__device__ inline void load_tile_keys(const uint64_t *__restrict__ ak, const uint64_t *__restrict__ bk,
+                                      uint32_t i0, uint32_t la, uint32_t j0, uint32_t lb,
+                                      uint64_t (&lk)[ITEMS]) {
+  const uint32_t last = la + lb - 1;
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    uint32_t x = (uint32_t)u * BT + threadIdx.x;
+    x = x < la + lb ? x : last;
+    lk[u] = x < la ? ak[i0 + x] : bk[j0 + (x - la)];
+  }
+}
+
+template <int BT, int ITEMS>
+__global__ __launch_bounds__(BT) void merge_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, uint32_t na,
     const uint64_t *__restrict__ bk, const double *__restrict__ bv, uint32_t nb,
     int op, uint64_t *__restrict__ ck, double *__restrict__ cv,
     const uint32_t *__restrict__ splits, uint64_t *__restrict__ status,
     uint32_t epoch, uint32_t *__restrict__ tile_counter,
     uint32_t *__restrict__ total_out, uint32_t num_tiles, uint64_t capacity) {
-  constexpr int kTile = kBlock * ITEMS;
+  constexpr int kTile = BT * ITEMS;
+  static_assert(kTile <= (int)kDescIdx, "descriptor indices are 15 bits");
+  static_assert(ITEMS <= 32, "emit mask is 32 bits");
   __shared__ uint64_t s_key[kTile + 1];
-  __shared__ double s_val[kTile + 1];
+  __shared__ uint32_t s_desc[kTile];
   __shared__ uint32_t s_tile, s_i0, s_i1;
   __shared__ uint64_t s_prev_a;
   __shared__ uint32_t s_has_prev;
-  __shared__ uint32_t s_scan[4];
+  __shared__ uint32_t s_scan[BT / kWave];
   __shared__ uint32_t s_excl;
 
   const uint32_t tid = threadIdx.x;
   const uint64_t total = (uint64_t)na + nb;
-  if (tid == 0)
-    s_tile = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+  // Tiles are numbered in the order workgroups start (a global counter), so a
+  // tile's look-back only waits on tiles that are already running.
+  if (tid == 0) {
+    const uint32_t t = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_tile = t;
+    s_i0 = splits[t];
+    s_i1 = splits[t + 1];
+  }
   __syncthreads();
   const uint32_t tile = s_tile;
+  const uint32_t i0 = s_i0, i1 = s_i1;
   const uint32_t d0 = (uint32_t)min<uint64_t>((uint64_t)tile * kTile, total);
   const uint32_t d1 = (uint32_t)min<uint64_t>((uint64_t)d0 + kTile, total);
-  if (tid == 0) s_i0 = splits[tile];
-  if (tid == 1) s_i1 = splits[tile + 1];
-  __syncthreads();
-  const uint32_t i0 = s_i0, i1 = s_i1;
   const uint32_t j0 = d0 - i0, j1 = d1 - i1;
   const uint32_t la = i1 - i0, lb = j1 - j0;
-  // A slice, B slice, then the B element after the slice (if any).  All of a
-  // thread's loads are issued before any is used, so a tile waits for one
-  // memory round trip, not ITEMS of them.
   {
     uint64_t lk[ITEMS];
-    double lv[ITEMS];
-    const uint32_t last = la + lb - 1;  // a tile holds at least one step
+    load_tile_keys<BT, ITEMS>(ak, bk, i0, la, j0, lb, lk);
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
-      uint32_t x = (uint32_t)u * kBlock + tid;
-      x = x < la + lb ? x : last;  // clamped, unconditional loads
-      const bool from_a = x < la;
-      const uint64_t *kp = from_a ? ak + i0 + x : bk + j0 + (x - la);
-      const double *vp = from_a ? av + i0 + x : bv + j0 + (x - la);
-      lk[u] = *kp;
-      lv[u] = *vp;
-    }
-#pragma unroll
-    for (int u = 0; u < ITEMS; ++u) {
-      uint32_t x = (uint32_t)u * kBlock + tid;
-      if (x < la + lb) {
-        s_key[x] = lk[u];
-        s_val[x] = lv[u];
-      }
+      const uint32_t x = (uint32_t)u * BT + tid;
+      if (x < la + lb) s_key[x] = lk[u];
     }
   }
   const bool has_next_b = j1 < nb;
   if (tid == 0) {
-    if (has_next_b) {
-      s_key[la + lb] = bk[j1];
-      s_val[la + lb] = bv[j1];
-    }
+    if (has_next_b) s_key[la + lb] = bk[j1];  // the B key after the slice
     s_has_prev = i0 > 0;
     s_prev_a = i0 > 0 ? ak[i0 - 1] : 0;
   }
   __syncthreads();
   const uint64_t *sa = s_key, *sb = s_key + la;
-  const double *sav = s_val, *sbv = s_val + la;
   const uint32_t nlb = lb + (has_next_b ? 1 : 0);  // B keys visible to peeks
+  const bool hadamard = op == LNT_OP_HADAMARD;
 
   // Per-thread sub-path.
   const uint32_t t0 = min<uint32_t>(tid * ITEMS, la + lb);
@@ -139,51 +150,50 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(
   {
     uint32_t lo = t0 > lb ? t0 - lb : 0, hi = t0 < la ? t0 : la;
     while (lo < hi) {
-      uint32_t mid = (lo + hi) >> 1;
+      const uint32_t mid = (lo + hi) >> 1;
       if (sa[mid] <= sb[t0 - 1 - mid]) lo = mid + 1;
       else hi = mid;
     }
     ia = lo;
     jb = t0 - lo;
   }
-  // Step s of this thread fills slot s; emit marks the slots that produce
-  // an output (register arrays are only indexed by unrolled constants).
-  uint64_t ok[ITEMS];
-  double ov[ITEMS];
+  uint64_t ka = ia < la ? sa[ia] : 0;
+  uint64_t kb = jb < nlb ? sb[jb] : 0;
+  bool has_last = ia > 0 || s_has_prev;  // an A key precedes this sub-path
+  uint64_t last_a = ia > 0 ? sa[ia - 1] : s_prev_a;
+  // Step s fills slot s; emit marks the slots that produce an output.
+  uint32_t desc[ITEMS];
   uint32_t emit = 0;
 #pragma unroll
   for (int s = 0; s < ITEMS; ++s) {
-    ok[s] = 0;
-    ov[s] = 0.0;
+    desc[s] = 0;
     if (t0 + s < t1) {
-      bool take_a = ia < la && (jb >= lb || sa[ia] <= sb[jb]);
+      const bool take_a = ia < la && (jb >= lb || ka <= kb);
       if (take_a) {
-        uint64_t key = sa[ia];
-        double a = sav[ia];
-        bool match = jb < nlb && sb[jb] == key;
-        double b = match ? sbv[jb] : 0.0;
-        ok[s] = key;
-        if (op == LNT_OP_HADAMARD) {
-          ov[s] = a * b;
-          emit |= (match ? 1u : 0u) << s;
-        } else {
-          ov[s] = match ? (op == LNT_OP_SUB ? a - b : a + b) : a;
-          emit |= 1u << s;
-        }
+        const bool match = jb < nlb && kb == ka;
+        desc[s] = kDescA | ia | (match ? (kDescB | (jb << 15)) : 0u);
+        emit |= (uint32_t)(match || !hadamard) << s;
+        last_a = ka;
+        has_last = true;
         ++ia;
       } else {
-        uint64_t key = sb[jb];
-        bool absorbed = ia > 0 ? sa[ia - 1] == key : (s_has_prev && s_prev_a == key);
-        ok[s] = key;
-        ov[s] = op == LNT_OP_SUB ? 0.0 - sbv[jb] : 0.0 + sbv[jb];
-        emit |= ((!absorbed && op != LNT_OP_HADAMARD) ? 1u : 0u) << s;
+        const bool absorbed = has_last && last_a == kb;
+        desc[s] = kDescB | (jb << 15);
+        emit |= (uint32_t)(!absorbed && !hadamard) << s;
         ++jb;
       }
+      // One LDS read per step: the next key of the side just consumed.  The
+      // index never exceeds la + lb (a B step happens only while jb < lb), so
+      // it stays inside s_key; a key read past its slice is never used (the
+      // ia < la and jb < nlb tests guard every use).
+      const uint64_t nk = s_key[take_a ? ia : la + jb];
+      if (take_a) ka = nk;
+      else kb = nk;
     }
   }
   const uint32_t cnt = (uint32_t)__popc(emit);
   uint32_t tile_count;
-  uint32_t my_off = block256_exclusive_sum(cnt, s_scan, &tile_count);
+  const uint32_t my_off = block_exclusive_sum_bt<BT>(cnt, s_scan, &tile_count);
 
   if (tid < kWave) {  // wave 0 publishes and looks back
     uint32_t excl = 0;
@@ -200,30 +210,57 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(
       s_excl = excl;
     }
   }
-  __syncthreads();  // everyone is done reading the input slices
 #pragma unroll
   for (int s = 0; s < ITEMS; ++s)
-    if ((emit >> s) & 1u) {
-      uint32_t idx = my_off + (uint32_t)__popc(emit & ((1u << s) - 1u));
-      s_key[idx] = ok[s];
-      s_val[idx] = ov[s];
-    }
+    if ((emit >> s) & 1u) s_desc[my_off + (uint32_t)__popc(emit & ((1u << s) - 1u))] = desc[s];
   __syncthreads();
   const uint64_t base = s_excl;
-  // Sorted unique inputs never exceed the capacity; malformed ones (duplicate
-  // keys) could, so writes are bounded and the host reports the overflow.
-  for (uint32_t x = tid; x < tile_count && base + x < capacity; x += kBlock) {
-    ck[base + x] = s_key[x];
-    cv[base + x] = s_val[x];
+  // Write phase in output order.  Separate loops for descriptors, gathers and
+  // stores, so that every gather of the thread is in flight before the first
+  // store waits for one.  Sorted unique inputs never exceed the capacity;
+  // malformed ones (duplicate keys) could, so writes are bounded and the host
+  // reports the overflow.
+  const double *tav = av + i0, *tbv = bv + j0;
+  const uint32_t lim = (uint32_t)min<uint64_t>(tile_count, capacity > base ? capacity - base : 0);
+  uint32_t dd[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    const uint32_t x = (uint32_t)u * BT + tid;
+    dd[u] = x < lim ? s_desc[x] : 0u;
+  }
+  double va[ITEMS], vb[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    va[u] = (dd[u] & kDescA) ? tav[dd[u] & kDescIdx] : 0.0;
+    vb[u] = (dd[u] & kDescB) ? tbv[(dd[u] >> 15) & kDescIdx] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    const uint32_t x = (uint32_t)u * BT + tid;
+    if (x < lim) {
+      const uint32_t d = dd[u];
+      const bool has_a = (d & kDescA) != 0, has_b = (d & kDescB) != 0;
+      const uint64_t key = s_key[has_a ? (d & kDescIdx) : la + ((d >> 15) & kDescIdx)];
+      const double a = va[u], b = vb[u];
+      double v;
+      if (has_a && has_b)
+        v = hadamard ? a * b : (op == LNT_OP_SUB ? a - b : a + b);
+      else if (has_a)
+        v = a;
+      else
+        v = op == LNT_OP_SUB ? 0.0 - b : 0.0 + b;
+      ck[base + x] = key;
+      cv[base + x] = v;
+    }
   }
 }
 
-template <int ITEMS>
+template <int BT, int ITEMS>
 int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
                  uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
                  int op, uint64_t *ck, double *cv, uint64_t capacity,
                  uint32_t *counter, uint32_t *total, int *slot) {
-  constexpr uint64_t tile = kBlock * ITEMS;
+  constexpr uint64_t tile = BT * ITEMS;
   uint64_t steps = (uint64_t)na + nb;
   uint32_t tiles = (uint32_t)((steps + tile - 1) / tile);
   LNT_TRY(ensure_status(ctx, tiles));
@@ -237,7 +274,7 @@ int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
   prof_end(ctx, pslot);
   LNT_HIP(hipGetLastError());
   *slot = prof_begin(ctx, kProfMerge, 16.0 * ((double)na + nb));
-  merge_kernel<ITEMS><<<tiles, kBlock, 0, ctx->stream>>>(
+  merge_kernel<BT, ITEMS><<<tiles, BT, 0, ctx->stream>>>(
       ak, av, na, bk, bv, nb, op, ck, cv, splits, (uint64_t *)ctx->status.ptr,
       epoch, counter, total, tiles, capacity);
   prof_end(ctx, *slot);
@@ -322,10 +359,17 @@ int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
   uint32_t *counter = small + kSmallCounters + 15;
   uint32_t *total = small + kSmallFlags + 1;
   int rc, slot = -1;
-  if (ctx->tuning.merge_items == 16)
-    rc = launch_merge<16>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot);
-  else
-    rc = launch_merge<8>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot);
+#define LNT_MERGE(BT, IT) \
+  launch_merge<BT, IT>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot)
+  switch (ctx->tuning.merge_block * 100 + ctx->tuning.merge_items) {
+    case 25608: rc = LNT_MERGE(256, 8); break;
+    case 25612: rc = LNT_MERGE(256, 12); break;
+    case 25616: rc = LNT_MERGE(256, 16); break;
+    case 51208: rc = LNT_MERGE(512, 8); break;
+    case 51212: rc = LNT_MERGE(512, 12); break;
+    default: rc = LNT_MERGE(512, 16); break;
+  }
+#undef LNT_MERGE
   if (rc != LNT_OK) return rc;
   LNT_HIP(hipMemcpyAsync(ctx->pinned + 1, total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   LNT_HIP(hipStreamSynchronize(ctx->stream));

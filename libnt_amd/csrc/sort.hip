@@ -63,44 +63,66 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
   constexpr int kUnroll = 8;
   bool bad = false;
   const uint64_t chunk = (uint64_t)kBlock * kUnroll;
-  for (uint64_t base = (uint64_t)blockIdx.x * chunk; base < n;
-       base += (uint64_t)gridDim.x * chunk) {
-    uint64_t k[kUnroll];
-    // Unconditional loads of a clamped index: a guarded load compiles to a
-    // branch that waits for it, which would serialise the kUnroll loads.
+  const uint64_t stride = (uint64_t)gridDim.x * chunk;
+  // Unconditional loads of a clamped index: a guarded load compiles to a
+  // branch that waits for it, which would serialise the kUnroll loads.
+  auto load = [&](uint64_t (&k)[kUnroll], uint64_t base) {
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      uint64_t i = base + (uint64_t)u * kBlock + tid;
+      const uint64_t i = base + (uint64_t)u * kBlock + tid;
       k[u] = keys[i < n ? i : n - 1];
     }
-    // Consecutive lanes hold consecutive keys; inputs are often sorted or
-    // nearly so, so equal digits come in runs across lanes.  Only the first
-    // lane of each run adds the run length: no same-address LDS atomics
-    // serialise inside a wave.
-    // A wave whose lanes all hold the same digit adds once (sorted inputs
-    // make such waves common); otherwise every lane adds its own key.
+  };
+  auto count = [&](uint64_t (&k)[kUnroll], uint64_t base) {
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      uint64_t i = base + (uint64_t)u * kBlock + tid;
+      const uint64_t i = base + (uint64_t)u * kBlock + tid;
       bad |= i < n && k[u] >= bound;
     }
     if (RELIN) relin_keys<kUnroll>(relin, k);
+    // Consecutive lanes hold consecutive keys; inputs are often sorted or
+    // nearly so.  A wave whose lanes all hold the same digit adds once
+    // (sorted inputs make such waves common): no same-address LDS atomics
+    // serialise inside a wave; otherwise every lane adds its own key.
+    // The pass loop is unrolled to its maximum so each pass's shift and mask
+    // are read once per kernel into scalar registers, not once per key.
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      uint64_t i = base + (uint64_t)u * kBlock + tid;
+      const uint64_t i = base + (uint64_t)u * kBlock + tid;
       const bool valid = i < n;
       const uint64_t key = k[u];
       const uint64_t vmask = __ballot(valid);
-      for (int p = 0; p < passes; ++p) {
-        const uint32_t d = (uint32_t)(key >> plan.shift[p]) & ((1u << plan.bits[p]) - 1);
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        if (__ballot(valid && d == d0) == vmask) {
-          if ((tid & 63) == 0) atomicAdd(&h[p * kRadix + d0], (uint32_t)__popcll(vmask));
-        } else if (valid) {
-          atomicAdd(&h[p * kRadix + d], 1u);
+#pragma unroll
+      for (int p = 0; p < kMaxPasses; ++p) {
+        if (p < passes) {
+          const uint32_t d = (uint32_t)(key >> plan.shift[p]) & ((1u << plan.bits[p]) - 1);
+          const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+          if (__ballot(valid && d == d0) == vmask) {
+            if ((tid & 63) == 0) atomicAdd(&h[p * kRadix + d0], (uint32_t)__popcll(vmask));
+          } else if (valid) {
+            atomicAdd(&h[p * kRadix + d], 1u);
+          }
         }
       }
     }
+  };
+  // Two register buffers in turn: the next chunk's loads are issued before
+  // the current chunk is counted, so a workgroup always has a chunk in flight.
+  // The prefetches are unconditional (past the end they read key n-1): a
+  // load on only one path makes the compiler wait for the older buffer's
+  // loads as if the newer ones had not been issued.
+  uint64_t k0[kUnroll], k1[kUnroll];
+  uint64_t b0 = (uint64_t)blockIdx.x * chunk;
+  if (b0 >= n) b0 = n;  // no chunk: the loop below does not run
+  load(k0, b0 < n ? b0 : 0);
+  while (b0 < n) {
+    const uint64_t b1 = b0 + stride;
+    load(k1, b1);
+    count(k0, b0);
+    if (b1 >= n) break;
+    b0 = b1 + stride;
+    load(k0, b0);
+    count(k1, b1);
   }
   if (bad) atomicOr(err, 1u);
   __syncthreads();
@@ -140,26 +162,6 @@ __global__ __launch_bounds__(256) void histogram_reduce_kernel(const uint32_t *_
 // KIn/KOut: key word read/written by this pass.  When the key space fits in
 // 32 bits the passes between the first and the last move 32-bit keys (24 B per
 // pair instead of 32 B); the first pass narrows, the last widens.
-// Exclusive sum over a block of BT threads (BT/64 waves); scratch holds BT/64
-// words.  Every thread of the block must call it.
-template <int BT>
-__device__ inline uint32_t block_exclusive_sum_bt(uint32_t v, uint32_t *scratch, uint32_t *total) {
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t incl = wave_inclusive_sum(v);
-  if (lane == 63) scratch[wave] = incl;
-  __syncthreads();
-  uint32_t before = 0, all = 0;
-#pragma unroll
-  for (int w = 0; w < BT / kWave; ++w) {
-    const uint32_t x = scratch[w];
-    before += (uint32_t)w < wave ? x : 0;
-    all += x;
-  }
-  __syncthreads();
-  *total = all;
-  return before + incl - v;
-}
-
 template <int BT, int ITEMS, bool RELIN, typename KIn, typename KOut>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void onesweep_kernel(
     const KIn *__restrict__ kin, const double *__restrict__ vin,

@@ -67,6 +67,31 @@ def test_merge_dense_interleaving(ctx, orc):
         _check(ctx, orc, ak, av, bk, bv)
 
 
+@pytest.mark.parametrize("block,items", [(256, 8), (256, 12), (256, 16), (512, 8), (512, 12), (512, 16)])
+def test_merge_tile_shapes(ctx, orc, block, items):
+    """Every merge tile shape gives the oracle's result: dense interleaving (equal
+    keys on every tile and thread boundary), long runs of one operand, and a
+    tile-sized operand against a single key."""
+    print("merge tile %dx%d" % (block, items), flush=True)
+    ctx.set_param("merge_block", block)
+    ctx.set_param("merge_items", items)
+    try:
+        rng = np.random.default_rng(block + items)
+        where = rng.integers(0, 3, 50000)
+        ak = np.flatnonzero(where != 1).astype(np.int64)
+        bk = np.flatnonzero(where != 0).astype(np.int64)
+        _check(ctx, orc, ak, rng.uniform(-1, 1, len(ak)), bk, rng.uniform(-1, 1, len(bk)))
+        ak = np.arange(0, 3 * block * items, 3, dtype=np.int64)
+        bk = np.unique(np.concatenate([np.arange(1, 60000, 7), [ak[-1], 10 ** 6]])).astype(np.int64)
+        _check(ctx, orc, ak, rng.uniform(-1, 1, len(ak)), bk, rng.uniform(-1, 1, len(bk)))
+        ak, av, bk, bv = _pair(orc, block * items, 1, 1, 1 << 40, 11)
+        _check(ctx, orc, ak, av, bk, bv)
+        _check(ctx, orc, bk, bv, ak, av)
+    finally:
+        ctx.set_param("merge_block", 512)
+        ctx.set_param("merge_items", 12)
+
+
 def test_merge_signed_zeros(ctx, orc):
     ak = np.array([1, 4], dtype=np.int64)
     av = np.array([-0.0, 0.0])
