@@ -128,6 +128,33 @@ def _log(msg):
     print("[bench] %s" % msg, file=sys.stderr, flush=True)
 
 
+def bench_unsorted_sort(ctx, torch, n=50_000_000, reps=3):
+    """A full radix sort of n distinct keys in generator (unsorted) order over
+    2^32, values following: every digit pass runs (4 at 8 bits).  Median of
+    `reps` sorts, each of freshly generated input (generation untimed)."""
+    k = torch.empty(n, dtype=torch.int64, device="cuda")
+    v = torch.empty(n, dtype=torch.float64, device="cuda")
+    times, passes = [], 0
+    for r in range(reps + 1):
+        ctx.testgen_distinct_keys(k, 1 << 32, 31 + r)
+        ctx.testgen_uniform(v, 41 + r)
+        torch.cuda.synchronize()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        ctx.sort_pairs(k, v, 1 << 32)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        passes = ctx.profile("onesweep")[0]
+        ctx.set_profiling(False)
+        if r:
+            times.append(el)
+    ok = bool((k[1:] > k[:-1]).all())
+    t = sorted(times)[len(times) // 2]
+    del k, v
+    torch.cuda.empty_cache()
+    return {"keys": n, "radix_passes": passes, "ms": round(t * 1e3, 3), "keys_per_s": n / t, "sorted": ok}
+
+
 def bench_contraction(ctx, torch):
     """Large skewed contraction, both accumulators, timed whole."""
     from libnt_amd import dispatch
@@ -346,6 +373,7 @@ def main():
             "workload": "sort/merge at scale: order-4 dims 256^4, 50M nnz each, 50%% overlap, "
                         "change_sort_order to perm %s, then A+B (75M out) and A∘B (25M out)" % (perm,),
             "nnz_a": na, "nnz_b": nb, "union_nnz": counts["union"], "hadamard_nnz": counts["hadamard"],
+            "radix_passes_per_sort": launches / (2 * args.steps),
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -376,6 +404,8 @@ def main():
     if not args.no_extras:
         del ak, av, bk, bv, cu_k, cu_v, ch_k, ch_v
         torch.cuda.empty_cache()
+        out["unsorted_sort"] = bench_unsorted_sort(ctx, torch)
+        _log("unsorted sort: %s" % (out["unsorted_sort"],))
         out["contraction"] = bench_contraction(ctx, torch)
         out["batched"] = bench_batched(ctx, torch)
         out["learned"] = bench_learned(ctx)

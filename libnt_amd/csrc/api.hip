@@ -391,19 +391,19 @@ int lnt_change_linidx(lnt_context ctx, int64_t *keys, int64_t n, int order,
 }
 
 static int sort_common(lnt_context ctx, int64_t *keys, double *vals, int64_t n,
-                       uint64_t bound, const Relin *relin, int mem) {
+                       uint64_t bound, const Relin *relin, int mem, uint32_t sorted_low = 0) {
   if (n < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
   if (n >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "n must be < 2^32");
   if (n == 0) return LNT_OK;
   if (!keys || !vals) return fail(LNT_ERR_INVALID_ARGUMENT, "null keys or values");
   if (mem == LNT_MEM_DEVICE)
-    return sort_device(ctx, (uint64_t *)keys, vals, (uint32_t)n, bound, relin);
+    return sort_device(ctx, (uint64_t *)keys, vals, (uint32_t)n, bound, relin, sorted_low);
   Staging st;
   LNT_TRY(ensure(st.b[0], (size_t)n * 8));
   LNT_TRY(ensure(st.b[1], (size_t)n * 8));
   LNT_HIP(hipMemcpyAsync(st.b[0].ptr, keys, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
   LNT_HIP(hipMemcpyAsync(st.b[1].ptr, vals, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
-  int rc = sort_device(ctx, (uint64_t *)st.b[0].ptr, (double *)st.b[1].ptr, (uint32_t)n, bound, relin);
+  int rc = sort_device(ctx, (uint64_t *)st.b[0].ptr, (double *)st.b[1].ptr, (uint32_t)n, bound, relin, sorted_low);
   if (rc != LNT_OK) return rc;
   LNT_HIP(hipMemcpyAsync(keys, st.b[0].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
   LNT_HIP(hipMemcpyAsync(vals, st.b[1].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -431,8 +431,12 @@ int lnt_change_sort_order(lnt_context ctx, int64_t *keys, double *vals,
   LNT_TRY(make_relin(order, dims, cur_seq, new_seq, &r, &dimensionality));
   // SparseMIABase.h:590-645: a sorted tensor whose order does not change is
   // left alone; everything else is relinearised and sorted.
+  // A sorted tensor already orders the fastest modes of the new sequence
+  // when they keep their relative order (the structure RadixShuffle
+  // exploits, LibMIARadix.h:733-740): those low bits need no radix pass.
   if (is_sorted && !r.active) return LNT_OK;
-  return sort_common(ctx, keys, vals, n, dimensionality, &r, mem);
+  const uint32_t low = is_sorted ? sorted_low_bits(order, dims, cur_seq, new_seq) : 0;
+  return sort_common(ctx, keys, vals, n, dimensionality, &r, mem, low);
 }
 
 int lnt_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals,

@@ -145,7 +145,8 @@ int unfold(lnt_context ctx, const lnt_operand *x, const int32_t *lat_seq, Device
   uint64_t dim;
   LNT_TRY(make_relin(x->order, x->dims, x->seq, lat_seq, &r, &dim));
   if (x->sorted && !r.active) return LNT_OK;
-  return sort_device(ctx, (uint64_t *)kb.ptr, (double *)vb.ptr, (uint32_t)x->nnz, dim, &r);
+  const uint32_t low = x->sorted ? sorted_low_bits(x->order, x->dims, x->seq, lat_seq) : 0;
+  return sort_device(ctx, (uint64_t *)kb.ptr, (double *)vb.ptr, (uint32_t)x->nnz, dim, &r, low);
 }
 
 struct DigitScope {
@@ -232,7 +233,9 @@ int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
     Relin r;
     uint64_t dim;
     int rc = make_relin(P.shape.order, P.shape.dims, cur, nxt, &r, &dim);
-    if (rc == LNT_OK && r.active) rc = sort_device(ctx, k, v, (uint32_t)n, dim, &r);
+    // C' comes out sorted in its own order.
+    if (rc == LNT_OK && r.active)
+      rc = sort_device(ctx, k, v, (uint32_t)n, dim, &r, sorted_low_bits(P.shape.order, P.shape.dims, cur, nxt));
     if (rc != LNT_OK) {
       release_out();
       return rc;

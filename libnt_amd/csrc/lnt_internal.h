@@ -450,8 +450,20 @@ int ensure_status(lnt_context ctx, size_t words);
 
 // Kernel launchers (sort.hip, merge.hip).  All pointers are device pointers.
 int launch_fill_u32(lnt_context ctx, uint32_t *p, uint32_t value, size_t n);
+// sorted_low: number of low key bits (after the relinearisation) whose order
+// the input already carries -- see sorted_low_bits; the radix passes then
+// cover only the bits above them.
 int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
-                uint64_t bound, const Relin *relin);
+                uint64_t bound, const Relin *relin, uint32_t sorted_low = 0);
+
+// For keys sorted in cur_seq, relinearised into new_seq: let the first J
+// modes of new_seq (the fastest) appear in cur_seq in the same relative
+// order, and let their dims multiply to P = 2^b * odd.  Two relinearised keys
+// equal above bit b then agree on every other mode, so the old key orders
+// them by those J modes in the new significance, i.e. by the low b bits: the
+// input order is already right there and a stable LSD sort of the bits from
+// b up gives the full order.  Returns b for the largest such J.
+uint32_t sorted_low_bits(int order, const int64_t *dims, const int32_t *cur_seq, const int32_t *new_seq);
 int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin);
 int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na,
                         const int64_t a_dims[3], const uint64_t *bk, const double *bv,

@@ -439,8 +439,21 @@ int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin
 
 // Sorts (keys, vals) in place.  bound: keys must lie in [0, bound) before the
 // optional relinearisation (which maps [0,bound) onto itself).
+uint32_t sorted_low_bits(int order, const int64_t *dims, const int32_t *cur_seq, const int32_t *new_seq) {
+  int pos[LNT_MAX_ORDER];
+  for (int t = 0; t < order; ++t) pos[cur_seq[t]] = t;
+  uint64_t p = 1;
+  for (int j = 0, last = -1; j < order && pos[new_seq[j]] > last; ++j) {
+    last = pos[new_seq[j]];
+    p *= (uint64_t)dims[new_seq[j]];
+  }
+  uint32_t b = 0;
+  while (b < 63 && !((p >> b) & 1)) ++b;
+  return b;
+}
+
 int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
-                uint64_t bound, const Relin *relin) {
+                uint64_t bound, const Relin *relin, uint32_t sorted_low) {
   if (n == 0) return LNT_OK;
   const bool do_relin = relin && relin->active;
   Relin r{};
@@ -449,18 +462,20 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   if (do_relin) LNT_TRY(upload_relin(ctx, r, &rdev));
   uint32_t bits = 0;
   while (bits < 64 && (uint64_t(1) << bits) < bound) ++bits;
-  // Digit plan: ceil(bits/digit_bits) passes (at least two so that no pass
-  // runs in place), digit widths balanced, least significant digit first.
+  // Digit plan over the bits above the already ordered low bits:
+  // ceil(span/digit_bits) passes (at least two so that no pass runs in
+  // place), digit widths balanced, least significant digit first.
   DigitPlan plan{};
+  const uint32_t lo = std::min(sorted_low, bits), span = bits - lo;
   const uint32_t dmax = (uint32_t)std::min(std::max(ctx->tuning.digit_bits, 4), kRadixBits);
-  int passes = (int)((bits + dmax - 1) / dmax);
+  int passes = (int)((span + dmax - 1) / dmax);
   if (passes > kMaxPasses) passes = kMaxPasses;  // 64-bit keys with narrow digits: widen
   if (passes == 1) passes = 2;
-  if (n == 1 || bits == 0) passes = 0;
+  if (n == 1 || span == 0) passes = 0;
   plan.passes = passes;
-  uint32_t shift = 0;
+  uint32_t shift = lo;
   for (int p = 0; p < passes; ++p) {
-    uint32_t w = bits / passes + ((uint32_t)p < bits % passes ? 1 : 0);
+    uint32_t w = span / passes + ((uint32_t)p < span % passes ? 1 : 0);
     plan.shift[p] = shift;
     plan.bits[p] = w;
     shift += w;

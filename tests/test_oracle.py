@@ -167,6 +167,48 @@ def test_sort_seq_is_relinearise_then_sort(orc, n, is_sorted):
         assert np.array_equal(k, rk[o]) and np.array_equal(v, vals[o])
 
 
+def _v2(x):
+    b = 0
+    while x % 2 == 0 and b < 63:
+        x //= 2
+        b += 1
+    return b
+
+
+@pytest.mark.parametrize("dims,cur,new", [
+    ([256, 256, 256, 256], [0, 1, 2, 3], [0, 2, 1, 3]),
+    ([12, 5, 7, 9], [0, 1, 2, 3], [0, 2, 1, 3]),
+    ([6, 4, 10], [2, 0, 1], [2, 1, 0]),
+    ([4, 4, 4, 4, 4], [1, 0, 2, 3, 4], [1, 0, 4, 3, 2]),
+    ([24, 6, 5], [0, 1, 2], [0, 2, 1]),
+    ([8, 8, 8, 8], [0, 1, 2, 3], [1, 3, 0, 2]),
+])
+def test_sorted_low_modes_lemma(orc, dims, cur, new):
+    """Keys sorted in `cur`, relinearised into `new`: when the first J modes of
+    `new` appear in `cur` in the same relative order and their dims multiply
+    to P = 2^b * odd, a stable sort on (key >> b) alone already gives the full
+    order (the low b bits need no radix pass).  Plausible mistakes break it on
+    the same data: skipping P's bit length when P is not a power of two, and
+    counting the first mode out of order as in order."""
+    total = int(np.prod(dims))
+    keys = np.sort(orc.gen_distinct_keys(min(20000, total // 2), total, 3))
+    rk = orc.change_linidx(keys, dims, cur, new)
+    pos = {m: t for t, m in enumerate(cur)}
+    p, j = 1, 0
+    while j < len(dims) and (j == 0 or pos[new[j]] > pos[new[j - 1]]):
+        p *= dims[new[j]]
+        j += 1
+    b = _v2(p)
+    full = np.sort(rk)
+    assert np.array_equal(rk[np.argsort(rk >> b, kind="stable")], full)
+    if p & (p - 1):
+        wrong = int(p - 1).bit_length()
+        assert not np.array_equal(rk[np.argsort(rk >> wrong, kind="stable")], full)
+    wrong = _v2(p * dims[new[j]])
+    if wrong > b:
+        assert not np.array_equal(rk[np.argsort(rk >> wrong, kind="stable")], full)
+
+
 def test_collect_duplicates(orc):
     keys = np.array([1, 1, 2, 5, 5, 5, 9], dtype=np.int64)
     vals = np.array([1.0, 2.0, 3.0, 1.0, 1.0, 1.0, -4.0])

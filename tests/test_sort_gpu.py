@@ -134,6 +134,62 @@ def test_change_sort_order_various(ctx, orc, case):
     assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
 
 
+SHARED_PREFIX_CASES = [
+    # dims, cur, new: the fastest modes of `new` keep their order in `cur`
+    ([256, 256, 256, 256], [0, 1, 2, 3], [0, 2, 1, 3]),      # 2^16: two passes fewer
+    ([8, 8, 8, 8], [0, 1, 2, 3], [1, 3, 0, 2]),              # modes 1, 3 in order: 6 bits
+    ([12, 5, 7, 9], [0, 1, 2, 3], [0, 2, 1, 3]),              # 12 = 4 * 3: two low bits
+    ([8, 16, 3, 50], [0, 1, 3, 2], [0, 1, 2, 3]),             # 8 * 16 = 2^7
+    ([6, 4, 10], [2, 0, 1], [2, 1, 0]),                       # shared mode of size 10: one bit
+    ([1 << 20, 1 << 20, 16], [0, 1, 2], [0, 2, 1]),           # 20 ordered bits of 44
+    ([3, 5, 7], [0, 1, 2], [0, 2, 1]),                        # odd product: nothing skipped
+    ([1 << 30, 2, 2], [0, 1, 2], [0, 2, 1]),                  # two bits left to sort
+    ([4, 4, 4, 4, 4], [1, 0, 2, 3, 4], [1, 0, 4, 3, 2]),      # two shared modes, 2^4
+]
+
+
+@pytest.mark.parametrize("case", range(len(SHARED_PREFIX_CASES)))
+def test_change_sort_order_sorted_shared_prefix(ctx, orc, case):
+    """A sorted tensor whose new sequence starts with modes in their old
+    relative order: the radix passes skip the low bits whose order the input
+    carries, and the result equals the oracle's full sort."""
+    dims, cur, new = SHARED_PREFIX_CASES[case]
+    total = int(np.prod(dims, dtype=object))
+    for n in (1, 5000, 200000):
+        n = min(n, total)
+        keys = orc.gen_distinct_keys(n, total, case * 7 + n)
+        vals = orc.gen_uniform_vals(n, case + n)
+        order = np.argsort(keys, kind="stable")
+        keys, vals = keys[order], vals[order]  # sorted in the current sequence
+        ek, ev = orc.sort_seq(keys, vals, dims, cur, new, False)
+        k, v = gpu(keys), gpu(vals)
+        ctx.change_sort_order(k, v, dims, cur, new, True)
+        assert np.array_equal(host(k), ek), (case, n)
+        assert _bits_equal(host(v), ev), (case, n)
+
+
+def test_change_sort_order_sorted_shared_prefix_large(ctx):
+    """20M sorted keys over 256^4 into [0, 2, 1, 3] (two radix passes)
+    against a numpy closed form, then back into the identity sequence."""
+    import torch
+
+    n, perm = 20_000_000, [0, 2, 1, 3]
+    k = torch.empty(n, dtype=torch.int64, device="cuda")
+    v = torch.empty(n, dtype=torch.float64, device="cuda")
+    ctx.testgen_distinct_keys(k, 1 << 32, 17)
+    ctx.testgen_uniform(v, 18)
+    ctx.sort_pairs(k, v, 1 << 32)
+    k0, v0 = host(k), host(v)
+    nk = _relin_bytes_np(k0, perm)
+    order = np.argsort(nk, kind="stable")
+    ctx.change_sort_order(k, v, [256] * 4, [0, 1, 2, 3], perm, True)
+    assert np.array_equal(host(k), nk[order])
+    assert _bits_equal(host(v), v0[order])
+    ctx.change_sort_order(k, v, [256] * 4, perm, [0, 1, 2, 3], True)
+    assert np.array_equal(host(k), k0)
+    assert _bits_equal(host(v), v0)
+
+
 def test_sort_repeat_is_bitwise_identical(ctx, orc):
     """Look-back or ranking races would show up as run-to-run differences."""
     import torch
