@@ -206,32 +206,35 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void on
   if (RELIN) relin_keys<ITEMS>(relin, k);
 
   // Stable in-wave ranking: the lanes holding the same digit are the lanes
-  // that agree with this one on every digit bit -- one ballot per bit, folded
-  // in with peers &= ~(ballot ^ bitmask).  Bits above dbits are zero in every
-  // lane, so a fixed 8-bit loop serves every digit width.  The highest lane of
-  // a group bumps the wave's counter for that digit.
+  // that agree with this one on every digit bit.  Per bit: m = the bit
+  // sign-extended (0 or ~0), its ballot, and the lanes that differ,
+  // ballot ^ m, or-ed into a mismatch mask -- one v_bitop3 per half (truth
+  // table 0xF6 = src0 | (src1 ^ src2)).  Lanes past the end start as
+  // mismatches.  Bits above dbits are zero in every lane, so a fixed 8-bit
+  // loop serves every digit width.
   uint32_t local_rank[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     uint64_t pos = wave_base + (uint64_t)i * kWave + lane;
-    bool valid = pos < n;
+    const bool valid = pos < n;
     uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
-    uint64_t vb = __ballot(valid);
-    uint32_t p_lo = (uint32_t)vb, p_hi = (uint32_t)(vb >> 32);
+    const uint64_t vb = __ballot(valid);
+    uint32_t x_lo = ~(uint32_t)vb, x_hi = ~(uint32_t)(vb >> 32);
 #pragma unroll
     for (int b = 0; b < kRadixBits; ++b) {
-      const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // 0 or ~0
+      const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);  // 0 or ~0
       const uint64_t bb = __ballot(m != 0);
-      p_lo &= ~((uint32_t)bb ^ m);
-      p_hi &= ~((uint32_t)(bb >> 32) ^ m);
+      x_lo = __builtin_amdgcn_bitop3_b32(x_lo, (uint32_t)bb, m, 0xF6);
+      x_hi = __builtin_amdgcn_bitop3_b32(x_hi, (uint32_t)(bb >> 32), m, 0xF6);
     }
-    const uint64_t peers = ((uint64_t)p_hi << 32) | p_lo;
+    const uint64_t peers = ~(((uint64_t)x_hi << 32) | x_lo);
     uint32_t r = 0;
     if (valid) {
-      uint32_t cnt = s_wcount[wave][d];
+      // Every lane of a group reads the count before any writes it, and all
+      // of them store the same new count: no leader election, no branch.
+      const uint32_t cnt = s_wcount[wave][d];
       r = cnt + mask_rank(peers);
-      uint32_t leader = 63u - (uint32_t)__builtin_clzll(peers);
-      if (lane == leader) s_wcount[wave][d] = cnt + (uint32_t)__popcll(peers);
+      s_wcount[wave][d] = cnt + (uint32_t)__popcll(peers);
     }
     local_rank[i] = r;
   }
