@@ -195,38 +195,24 @@ __global__ __launch_bounds__(BT) void merge_kernel(
   uint32_t tile_count;
   const uint32_t my_off = block_exclusive_sum_bt<BT>(cnt, s_scan, &tile_count);
 
-  if (tid < kWave) {  // wave 0 publishes and looks back
-    uint32_t excl = 0;
-    uint64_t *my_status = status + tile;
-    if (tile == 0) {
-      if (tid == 0) status_store(my_status, status_word(kFlagInclusive, epoch, tile_count));
-    } else {
-      if (tid == 0) status_store(my_status, status_word(kFlagAggregate, epoch, tile_count));
-      excl = wave_lookback(status, tile, epoch);
-      if (tid == 0) status_store(my_status, status_word(kFlagInclusive, epoch, excl + tile_count));
-    }
-    if (tid == 0) {
-      if (tile == num_tiles - 1) *total_out = excl + tile_count;
-      s_excl = excl;
-    }
-  }
+  // The tile's count is published at once; the look-back for its output
+  // offset runs while the value gathers are in flight (only the stores need
+  // the offset).
+  uint64_t *my_status = status + tile;
+  if (tid == 0)
+    status_store(my_status, status_word(tile == 0 ? kFlagInclusive : kFlagAggregate, epoch, tile_count));
 #pragma unroll
   for (int s = 0; s < ITEMS; ++s)
     if ((emit >> s) & 1u) s_desc[my_off + (uint32_t)__popc(emit & ((1u << s) - 1u))] = desc[s];
   __syncthreads();
-  const uint64_t base = s_excl;
-  // Write phase in output order.  Separate loops for descriptors, gathers and
-  // stores, so that every gather of the thread is in flight before the first
-  // store waits for one.  Sorted unique inputs never exceed the capacity;
-  // malformed ones (duplicate keys) could, so writes are bounded and the host
-  // reports the overflow.
+  // Write phase in output order: descriptors, then every gather of the
+  // thread, then (after the offset is known) the stores.
   const double *tav = av + i0, *tbv = bv + j0;
-  const uint32_t lim = (uint32_t)min<uint64_t>(tile_count, capacity > base ? capacity - base : 0);
   uint32_t dd[ITEMS];
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     const uint32_t x = (uint32_t)u * BT + tid;
-    dd[u] = x < lim ? s_desc[x] : 0u;
+    dd[u] = x < tile_count ? s_desc[x] : 0u;
   }
   double va[ITEMS], vb[ITEMS];
 #pragma unroll
@@ -234,6 +220,19 @@ __global__ __launch_bounds__(BT) void merge_kernel(
     va[u] = (dd[u] & kDescA) ? tav[dd[u] & kDescIdx] : 0.0;
     vb[u] = (dd[u] & kDescB) ? tbv[(dd[u] >> 15) & kDescIdx] : 0.0;
   }
+  if (tid < kWave) {  // wave 0 looks back
+    const uint32_t excl = tile == 0 ? 0u : wave_lookback(status, tile, epoch);
+    if (tid == 0) {
+      if (tile > 0) status_store(my_status, status_word(kFlagInclusive, epoch, excl + tile_count));
+      if (tile == num_tiles - 1) *total_out = excl + tile_count;
+      s_excl = excl;
+    }
+  }
+  __syncthreads();
+  const uint64_t base = s_excl;
+  // Sorted unique inputs never exceed the capacity; malformed ones (duplicate
+  // keys) could, so writes are bounded and the host reports the overflow.
+  const uint32_t lim = (uint32_t)min<uint64_t>(tile_count, capacity > base ? capacity - base : 0);
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     const uint32_t x = (uint32_t)u * BT + tid;

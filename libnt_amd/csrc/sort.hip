@@ -256,19 +256,14 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void on
   if (tile == 0 && digit_thread) hist0 = pass_hist[tid];
   uint32_t all;
   const uint32_t excl0 = tile == 0 ? block_exclusive_sum_bt<BT>(hist0, s_scan, &all) : 0;
+  // The tile's digit counts are published at once; the look-back for its
+  // global offsets runs after this thread has staged its keys (staging needs
+  // only the tile-local offsets).
+  uint64_t *my_status = status + (uint64_t)tile * kRadix + tid;
   if (digit_thread) {
     s_lstart[tid] = lstart;
-    uint32_t excl;
-    uint64_t *my_status = status + (uint64_t)tile * kRadix + tid;
-    if (tile == 0) {
-      excl = excl0;
-      status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
-    } else {
-      status_store(my_status, status_word(kFlagAggregate, epoch, count));
-      excl = lookback_batched<4>(status, tile, kRadix, tid, epoch);
-      status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
-    }
-    s_gofs[tid] = excl - lstart;
+    status_store(my_status, tile == 0 ? status_word(kFlagInclusive, epoch, excl0 + count)
+                                      : status_word(kFlagAggregate, epoch, count));
   }
   __syncthreads();
 
@@ -283,6 +278,14 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void on
       lpos[i] = s_lstart[d] + s_wcount[wave][d] + local_rank[i];
       s_stage[lpos[i]] = k[i];
     }
+  }
+  if (digit_thread) {
+    uint32_t excl = excl0;
+    if (tile > 0) {
+      excl = lookback_batched<4>(status, tile, kRadix, tid, epoch);
+      status_store(my_status, status_word(kFlagInclusive, epoch, excl + count));
+    }
+    s_gofs[tid] = excl - lstart;
   }
   __syncthreads();
   // Destinations stay below n whenever the digit counts of the histogram
