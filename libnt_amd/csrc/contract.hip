@@ -132,19 +132,27 @@ bool same_seq(const lnt_operand *x, const int32_t *seq) {
   return true;
 }
 
-// Copies one operand to the context's scratch and sorts it into `lat_seq`
-// (change_sort_order, SparseMIABase.h:590-645).
-int unfold(lnt_context ctx, const lnt_operand *x, const int32_t *lat_seq, DeviceBuffer &kb, DeviceBuffer &vb) {
-  const size_t bytes = (size_t)(x->nnz ? x->nnz : 1) * 8;
-  LNT_TRY(ensure(kb, bytes));
-  LNT_TRY(ensure(vb, bytes));
-  if (x->nnz == 0) return LNT_OK;
-  LNT_HIP(hipMemcpyAsync(kb.ptr, x->keys, (size_t)x->nnz * 8, hipMemcpyDeviceToDevice, ctx->stream));
-  LNT_HIP(hipMemcpyAsync(vb.ptr, x->vals, (size_t)x->nnz * 8, hipMemcpyDeviceToDevice, ctx->stream));
+// The operand in the lattice sequence `lat_seq` (change_sort_order,
+// SparseMIABase.h:590-645): a sorted operand already in that sequence is used
+// where it lies; otherwise it is copied to the context's scratch and sorted
+// there.  *keys / *vals receive the arrays to read.
+int unfold(lnt_context ctx, const lnt_operand *x, const int32_t *lat_seq, DeviceBuffer &kb, DeviceBuffer &vb,
+           const uint64_t **keys, const double **vals) {
   Relin r;
   uint64_t dim;
   LNT_TRY(make_relin(x->order, x->dims, x->seq, lat_seq, &r, &dim));
-  if (x->sorted && !r.active) return LNT_OK;
+  if (x->nnz == 0 || (x->sorted && !r.active)) {
+    *keys = (const uint64_t *)x->keys;
+    *vals = x->vals;
+    return LNT_OK;
+  }
+  const size_t bytes = (size_t)x->nnz * 8;
+  LNT_TRY(ensure(kb, bytes));
+  LNT_TRY(ensure(vb, bytes));
+  LNT_HIP(hipMemcpyAsync(kb.ptr, x->keys, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  LNT_HIP(hipMemcpyAsync(vb.ptr, x->vals, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  *keys = (const uint64_t *)kb.ptr;
+  *vals = (const double *)vb.ptr;
   const uint32_t low = x->sorted ? sorted_low_bits(x->order, x->dims, x->seq, lat_seq) : 0;
   return sort_device(ctx, (uint64_t *)kb.ptr, (double *)vb.ptr, (uint32_t)x->nnz, dim, &r, low);
 }
@@ -172,10 +180,10 @@ int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
   LNT_TRY(make_plan(a, b, &P0));
   *shape = P0.shape;
   DigitScope digits(ctx, (s & LNT_STRAT_RADIX6) ? 6 : 8);
-  LNT_TRY(unfold(ctx, f, P.seq_a, ctx->exec_buf[0], ctx->exec_buf[1]));
-  LNT_TRY(unfold(ctx, g, P.seq_b, ctx->exec_buf[2], ctx->exec_buf[3]));
-  const uint64_t *fk = (const uint64_t *)ctx->exec_buf[0].ptr, *gk = (const uint64_t *)ctx->exec_buf[2].ptr;
-  const double *fv = (const double *)ctx->exec_buf[1].ptr, *gv = (const double *)ctx->exec_buf[3].ptr;
+  const uint64_t *fk = nullptr, *gk = nullptr;
+  const double *fv = nullptr, *gv = nullptr;
+  LNT_TRY(unfold(ctx, f, P.seq_a, ctx->exec_buf[0], ctx->exec_buf[1], &fk, &fv));
+  LNT_TRY(unfold(ctx, g, P.seq_b, ctx->exec_buf[2], ctx->exec_buf[3], &gk, &gv));
   uint64_t *k = out_k;
   double *v = out_v;
   uint64_t n = 0;

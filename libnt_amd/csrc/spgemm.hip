@@ -303,23 +303,77 @@ __global__ void large_sizes_kernel(const uint32_t *__restrict__ list3, uint32_t 
   np3[r] = prod_off[last] - prod_off[first];
 }
 
-// Products of the large segments with composite key r * h + row.
-__global__ void large_expand_kernel(
+// Outputs (or products) are handled in chunks of kChunk consecutive indices
+// per workgroup round.  The offsets of the segments that cover a chunk are
+// found with one global search per end and staged in LDS, where every index
+// of the chunk finds its segment; a chunk covering more segments than fit
+// (possible only with empty segments) searches the global offsets instead.
+constexpr uint32_t kChunk = 2048;
+
+struct ChunkSegs {
+  uint64_t s0, s1;  // segments [s0, s1) cover the chunk
+  bool staged;      // off[s0, s1) is in LDS
+};
+
+__device__ inline ChunkSegs stage_chunk(const uint64_t *__restrict__ off, uint64_t nseg, uint64_t q0, uint64_t q1,
+                                        uint64_t *s_off, uint64_t *s_seg) {
+  if (threadIdx.x == 0) {
+    const uint64_t s0 = upper_index(off, 0, nseg, q0);
+    s_seg[0] = s0;
+    s_seg[1] = upper_index(off, s0, nseg, q1 - 1) + 1;
+  }
+  __syncthreads();
+  ChunkSegs c{s_seg[0], s_seg[1], false};
+  c.staged = c.s1 - c.s0 <= kChunk + 1;
+  if (c.staged)
+    for (uint64_t x = threadIdx.x; x < c.s1 - c.s0; x += blockDim.x) s_off[x] = off[c.s0 + x];
+  __syncthreads();
+  return c;
+}
+
+// Segment of index q and that segment's offset.
+__device__ inline uint64_t chunk_segment(const ChunkSegs &c, const uint64_t *__restrict__ off, const uint64_t *s_off,
+                                         uint64_t q, uint64_t *seg_off) {
+  if (c.staged) {
+    uint32_t lo = 0, hi = (uint32_t)(c.s1 - c.s0);
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_off[mid] <= q) lo = mid;
+      else hi = mid;
+    }
+    *seg_off = s_off[lo];
+    return c.s0 + lo;
+  }
+  const uint64_t s = upper_index(off, c.s0, c.s1, q);
+  *seg_off = off[s];
+  return s;
+}
+
+// Products of the large segments with composite key r * h + row, kChunk per
+// workgroup round.
+__global__ __launch_bounds__(256) void large_expand_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
     const uint64_t *__restrict__ a_lo, const uint64_t *__restrict__ a_base,
     const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg,
     uint64_t nb, const uint32_t *__restrict__ list3, uint32_t n3, const uint64_t *__restrict__ ex_off,
     uint64_t total3, uint64_t ah, uint64_t *__restrict__ key3, double *__restrict__ val3) {
-  uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; q < total3; q += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t r = upper_index(ex_off, 0, n3, q);
-    uint64_t s = list3[r];
-    uint64_t first = seg_first[s];
-    uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
-    uint64_t e, ai;
-    locate(prod_off, first, last, q - ex_off[r], a_lo, &e, &ai);
-    key3[q] = r * ah + (ak[ai] - a_base[e]);
-    val3[q] = bv[e] * av[ai];
+  __shared__ uint64_t s_off[kChunk + 1];
+  __shared__ uint64_t s_seg[2];
+  for (uint64_t q0 = (uint64_t)blockIdx.x * kChunk; q0 < total3; q0 += (uint64_t)gridDim.x * kChunk) {
+    const uint64_t q1 = min<uint64_t>(q0 + kChunk, total3);
+    const ChunkSegs c = stage_chunk(ex_off, n3, q0, q1, s_off, s_seg);
+    for (uint64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+      uint64_t base;
+      const uint64_t r = chunk_segment(c, ex_off, s_off, q, &base);
+      const uint64_t s = list3[r];
+      const uint64_t first = seg_first[s];
+      const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+      uint64_t e, ai;
+      locate(prod_off, first, last, q - base, a_lo, &e, &ai);
+      key3[q] = r * ah + (ak[ai] - a_base[e]);
+      val3[q] = bv[e] * av[ai];
+    }
+    __syncthreads();  // s_off is restaged next round
   }
 }
 
@@ -363,33 +417,45 @@ __global__ void large_counts_kernel(const uint32_t *__restrict__ list3, uint32_t
   seg_u[list3[r]] = hi - lo;
 }
 
-// 4. output: one thread per element of C.
-__global__ void write_kernel(const uint64_t *__restrict__ ak, const double *__restrict__ av,
+// 4. output: the elements of C in order, kChunk per workgroup round.
+__global__ __launch_bounds__(256) void write_kernel(const uint64_t *__restrict__ ak, const double *__restrict__ av,
                              const double *__restrict__ bv, const uint64_t *__restrict__ a_lo,
                              const uint64_t *__restrict__ a_base, const uint64_t *__restrict__ colid,
                              const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first,
                              uint64_t nseg, uint64_t nb, const uint64_t *__restrict__ out_off, uint64_t nc,
                              const uint64_t *__restrict__ tmp_row, const double *__restrict__ tmp_val,
                              uint64_t ah, uint64_t *__restrict__ ck, double *__restrict__ cv) {
-  uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; o < nc; o += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t s = upper_index(out_off, 0, nseg, o);
-    uint64_t q = o - out_off[s];
-    uint64_t first = seg_first[s];
-    uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
-    uint64_t row;
-    double val;
-    if (last - first == 1) {
-      uint64_t ai = a_lo[first] + q;
-      row = ak[ai] - a_base[first];
-      val = bv[first] * av[ai];
-    } else {
-      row = tmp_row[prod_off[first] + q];
-      val = tmp_val[prod_off[first] + q];
+  __shared__ uint64_t s_off[kChunk + 1];
+  __shared__ uint64_t s_seg[2];
+  for (uint64_t o0 = (uint64_t)blockIdx.x * kChunk; o0 < nc; o0 += (uint64_t)gridDim.x * kChunk) {
+    const uint64_t o1 = min<uint64_t>(o0 + kChunk, nc);
+    const ChunkSegs c = stage_chunk(out_off, nseg, o0, o1, s_off, s_seg);
+    for (uint64_t o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
+      uint64_t base;
+      const uint64_t s = chunk_segment(c, out_off, s_off, o, &base);
+      const uint64_t q = o - base;
+      const uint64_t first = seg_first[s];
+      const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+      uint64_t row;
+      double val;
+      if (last - first == 1) {
+        const uint64_t ai = a_lo[first] + q;
+        row = ak[ai] - a_base[first];
+        val = bv[first] * av[ai];
+      } else {
+        row = tmp_row[prod_off[first] + q];
+        val = tmp_val[prod_off[first] + q];
+      }
+      ck[o] = row + ah * colid[first];
+      cv[o] = val;
     }
-    ck[o] = row + ah * colid[first];
-    cv[o] = val;
+    __syncthreads();  // s_off is restaged next round
   }
+}
+
+inline uint32_t chunk_grid(uint64_t n) {
+  const uint64_t b = (n + kChunk - 1) / kChunk;
+  return (uint32_t)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
 }
 
 inline uint32_t grid_for(uint64_t n, uint32_t cap = 65536) {
@@ -555,7 +621,7 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
       uint32_t *heads = c.take<uint32_t>(total3);
       uint64_t *grank = c.take<uint64_t>(total3 + 1);
       LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan3) * 8 + 64));
-      large_expand_kernel<<<grid_for(total3), 256, 0, ctx->stream>>>(
+      large_expand_kernel<<<chunk_grid(total3), 256, 0, ctx->stream>>>(
           ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first, nseg, nb, l3, gn, E.ex_off, total3, ah, key3, val3);
       LNT_HIP(hipGetLastError());
       LNT_TRY(sort_device(ctx, key3, val3, (uint32_t)total3, (uint64_t)gn * ah, nullptr));
@@ -591,7 +657,7 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) + " is below nnz(C) = " +
                                               std::to_string(nc));
   }
-  write_kernel<<<grid_for(nc), 256, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.colid, E.prod_off, E.seg_first,
+  write_kernel<<<chunk_grid(nc), 256, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.colid, E.prod_off, E.seg_first,
                                                        nseg, nb, E.out_off, nc, tmp_row, tmp_val, ah, ck, cv);
   LNT_HIP(hipGetLastError());
   prof_end(ctx, slot);
