@@ -234,10 +234,6 @@ int lnt_context_create(lnt_context *out, int device, void *stream) {
   if (!ctx) return fail(LNT_ERR_OUT_OF_MEMORY, "context allocation failed");
   ctx->device = device;
   ctx->stream = (hipStream_t)stream;
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
-    ctx->cus = cus;
-  (void)hipGetLastError();
   if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
     delete ctx;
     return fail(LNT_ERR_OUT_OF_MEMORY, "pinned host allocation failed");

@@ -410,7 +410,6 @@ struct lnt_context_s {
   hipStream_t stream = nullptr;
   int validate = 1;
   int capturing = 0;  // launches are being recorded into a graph: no host syncs
-  int cus = 256;      // compute units of the device (persistent grids)
   uint32_t epoch = 0;
   lnt::SortTuning tuning;
   lnt::DeviceBuffer alt_keys, alt_vals, alt2_keys, alt2_vals;
