@@ -177,6 +177,20 @@ int lnt_sort_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_
 int lnt_sort_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
                    const int64_t *b_keys, const double *b_vals, int64_t nb, int op, uint64_t bound,
                    int64_t **c_keys, double **c_vals, int64_t *nc, int mem);
+/* Equality of two sparse tensors in one linearisation, both sorted ascending
+ * with unique keys (SparseMIABase::compare_with_sparse,
+ * SparseMIABase.h:1243-1296, after its sort of both operands into one order,
+ * which the caller does).  Entries whose |value| <= 1.11e-16
+ * (SparseTolerance<double>, LibMIAUtil.h:137-150; below_tolerance,
+ * SparseMIABase.h:1170-1175) are skipped on either side; the other keys must
+ * be the same on both sides and every pair must satisfy the predicate:
+ * LNT_CMP_EXACT a == b (operator==, SparseMIABase.h:1616-1623), LNT_CMP_FUZZY
+ * isEqualFuzzy(a, b, precision) (fuzzy_equals, SparseMIABase.h:1627-1635;
+ * FunctionUtil.h:688-698).  *equal receives 1 or 0; nothing is written to the
+ * operands.  Validation level 2 also checks that both are sorted. */
+typedef enum lnt_compare_mode { LNT_CMP_EXACT = 0, LNT_CMP_FUZZY = 1 } lnt_compare_mode;
+int lnt_compare(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na, const int64_t *b_keys,
+                const double *b_vals, int64_t nb, int mode, double precision, int *equal, int mem);
 int lnt_free(lnt_context ctx, void *ptr, int mem);
 /* Copies `bytes` bytes from src to dst (device memory, either side may be an
  * array the library allocated) on the context's stream and waits: how a

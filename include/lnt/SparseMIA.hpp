@@ -442,9 +442,17 @@ class SparseMIA {
     t_->sorted = false;
   }
 
+  // ---- comparison (SparseMIABase.h:1243-1296, 1616-1635): dims must agree;
+  // this is sorted, the other sorted into this one's sequence (both in
+  // place), then compared on the GPU; entries with |value| <= 1.11e-16 count
+  // as absent.
+  bool operator==(SparseMIA &o) { return compare(o, LNT_CMP_EXACT, 0.0); }
+  bool operator!=(SparseMIA &o) { return !(*this == o); }
+  bool fuzzy_equals(SparseMIA &o, T precision) { return compare(o, LNT_CMP_FUZZY, precision); }
+
   // ---- index expressions (MIA.h:222-229)
   template <class... Ts>
-  AssignTarget<SparseMIA> operator()(Ts... ts) {
+  AssignTarget<SparseMIA> operator()(Ts...) {
     static_assert(sizeof...(Ts) == N, "one index per mode");
     return AssignTarget<SparseMIA>(*this, std::vector<Label>{Label{Ts::id, Ts::elemval > 0}...});
   }
@@ -452,6 +460,17 @@ class SparseMIA {
   std::shared_ptr<detail::Tensor> tensor_ptr() { return t_; }
 
  private:
+  bool compare(SparseMIA &o, int mode, double precision) {
+    if (t_->dims != o.t_->dims) return false;
+    sort();
+    o.t_->sort_to(t_->seq);
+    int eq = 0;
+    check(lnt_compare(default_context().get(), t_->idx.data(), t_->data.data(), (int64_t)t_->idx.size(),
+                      o.t_->idx.data(), o.t_->data.data(), (int64_t)o.t_->idx.size(), mode, precision, &eq,
+                      LNT_MEM_HOST));
+    return eq != 0;
+  }
+
   std::shared_ptr<detail::Tensor> t_;
 };
 

@@ -28,6 +28,7 @@ STATUS_NAMES = {
 }
 MEM_HOST, MEM_DEVICE = 0, 1
 OP_ADD, OP_SUB, OP_HADAMARD = 0, 1, 2
+CMP_EXACT, CMP_FUZZY = 0, 1
 SPGEMM_AUTO, SPGEMM_SORT, SPGEMM_HASH = 0, 1, 2
 
 _lib = None
@@ -56,6 +57,7 @@ SIGNATURES = {
     "lnt_sort_pairs": [_vp, _vp, _vp, _i64, _u64, _int],
     "lnt_change_sort_order": [_vp, _vp, _vp, _i64, _int, _i64p, _i32p, _i32p, _int, _int],
     "lnt_merge_into": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _vp, _i64, _i64p, _int],
+    "lnt_compare": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _int],
     "lnt_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_free": [_vp, _vp, _int],
     "lnt_copy": [_vp, _vp, _vp, _i64],
@@ -238,6 +240,22 @@ class Context:
         c, cp = _arr_i32(cur_seq)
         s, sp = _arr_i32(new_seq)
         check(lib().lnt_change_sort_order(self._h, kp, vp, n, len(d), dp, cp, sp, int(bool(is_sorted)), mem))
+
+    def compare(self, ak, av, bk, bv, precision=None) -> bool:
+        """compare_with_sparse (SparseMIABase.h:1243-1296) of two operands
+        sorted in one linearisation: exact (operator==) when precision is None,
+        else isEqualFuzzy(a, b, precision) (fuzzy_equals)."""
+        akp, mem, na = _ptr_mem(ak)
+        avp, _, _ = _ptr_mem(av)
+        bkp, memb, nb = _ptr_mem(bk)
+        bvp, _, _ = _ptr_mem(bv)
+        if mem != memb:
+            raise ValueError("both operands must live in the same memory kind")
+        eq = ctypes.c_int(0)
+        mode = CMP_EXACT if precision is None else CMP_FUZZY
+        check(lib().lnt_compare(self._h, akp, avp, na, bkp, bvp, nb, mode, float(precision or 0.0),
+                                ctypes.byref(eq), mem))
+        return bool(eq.value)
 
     def merge_into(self, ak, av, bk, bv, op, ck, cv) -> int:
         """Sorted union (op ADD/SUB) or intersection (HADAMARD) into ck/cv."""

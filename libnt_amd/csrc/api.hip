@@ -435,6 +435,41 @@ int lnt_change_sort_order(lnt_context ctx, int64_t *keys, double *vals,
   return sort_common(ctx, keys, vals, n, dimensionality, &r, mem, low);
 }
 
+int lnt_compare(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na, const int64_t *b_keys,
+                const double *b_vals, int64_t nb, int mode, double precision, int *equal, int mem) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  if (!equal) return fail(LNT_ERR_INVALID_ARGUMENT, "null equal");
+  if (mode != LNT_CMP_EXACT && mode != LNT_CMP_FUZZY)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "mode must be LNT_CMP_EXACT or LNT_CMP_FUZZY");
+  if (mode == LNT_CMP_FUZZY && !(precision > 0)) return fail(LNT_ERR_INVALID_ARGUMENT, "precision must be positive");
+  if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  if (na >= (int64_t(1) << 32) || nb >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "na, nb must be < 2^32");
+  if ((na && (!a_keys || !a_vals)) || (nb && (!b_keys || !b_vals))) return fail(LNT_ERR_INVALID_ARGUMENT, "null array");
+  if (mem == LNT_MEM_DEVICE) {
+    if (ctx->validate >= 2)
+      LNT_TRY(check_sorted(ctx, (const uint64_t *)a_keys, (uint32_t)na, (const uint64_t *)b_keys, (uint32_t)nb));
+    return compare_device(ctx, (const uint64_t *)a_keys, a_vals, (uint32_t)na, (const uint64_t *)b_keys, b_vals,
+                          (uint32_t)nb, mode, precision, equal);
+  }
+  Staging st;
+  const size_t sa = (size_t)na * 8, sb = (size_t)nb * 8;
+  for (int i = 0; i < 4; ++i) LNT_TRY(ensure(st.b[i], (i < 2 ? sa : sb) ? (i < 2 ? sa : sb) : 8));
+  if (na) {
+    LNT_HIP(hipMemcpyAsync(st.b[0].ptr, a_keys, sa, hipMemcpyHostToDevice, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(st.b[1].ptr, a_vals, sa, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (nb) {
+    LNT_HIP(hipMemcpyAsync(st.b[2].ptr, b_keys, sb, hipMemcpyHostToDevice, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(st.b[3].ptr, b_vals, sb, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (ctx->validate >= 2)
+    LNT_TRY(check_sorted(ctx, (const uint64_t *)st.b[0].ptr, (uint32_t)na, (const uint64_t *)st.b[2].ptr, (uint32_t)nb));
+  return compare_device(ctx, (const uint64_t *)st.b[0].ptr, (const double *)st.b[1].ptr, (uint32_t)na,
+                        (const uint64_t *)st.b[2].ptr, (const double *)st.b[3].ptr, (uint32_t)nb, mode, precision,
+                        equal);
+}
+
 int lnt_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
                    int64_t na, const int64_t *b_keys, const double *b_vals,
                    int64_t nb, int op, int64_t *c_keys, double *c_vals,

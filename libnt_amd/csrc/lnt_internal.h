@@ -463,6 +463,10 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
 // input order is already right there and a stable LSD sort of the bits from
 // b up gives the full order.  Returns b for the largest such J.
 uint32_t sorted_low_bits(int order, const int64_t *dims, const int32_t *cur_seq, const int32_t *new_seq);
+
+// Equality of two sorted operands in one linearisation (compare.hip).
+int compare_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
+                   const double *bv, uint32_t nb, int mode, double prec, int *equal);
 int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin);
 int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na,
                         const int64_t a_dims[3], const uint64_t *bk, const double *bv,

@@ -94,6 +94,29 @@ class SparseMIA:
     def copy(self):
         return SparseMIA(self.dims, self.keys.clone(), self.vals.clone(), self.seq, self.is_sorted, self.ctx)
 
+    # ------------------------------------------------------------ comparison
+    def _compare(self, other: "SparseMIA", precision):
+        # SparseMIABase.h:1243-1296: dims must agree; this is sorted, the other
+        # sorted into this one's sequence (both in place), then compared.
+        if tuple(self.dims) != tuple(other.dims):
+            return False
+        self.sort()
+        other.sort(self.seq)
+        return self.ctx.compare(self.keys, self.vals, other.keys, other.vals, precision)
+
+    def __eq__(self, other):
+        """operator== (SparseMIABase.h:1616-1623): exact values; entries with
+        |value| <= 1.11e-16 count as absent."""
+        if not isinstance(other, SparseMIA):
+            return NotImplemented
+        return self._compare(other, None)
+
+    __hash__ = None
+
+    def fuzzy_equals(self, other: "SparseMIA", precision: float) -> bool:
+        """fuzzy_equals (SparseMIABase.h:1627-1635): isEqualFuzzy per pair."""
+        return self._compare(other, precision)
+
     # -------------------------------------------------------------- helpers
     def __call__(self, labels: str) -> "Atom":
         if len(labels) != self.order:

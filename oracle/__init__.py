@@ -74,6 +74,7 @@ def lib():
             "lnto_lattice_mult": (i64, [_i64p, _f64p, i64, i64, i64, i64, _i64p, _f64p, i64, i64, i64, i64, _i64p, _f64p, i64]),
             "lnto_contract_plan": (i32, [i32, _i64p, ctypes.c_char_p, i32, _i64p, ctypes.c_char_p, _i32p, _i32p, _i64p, _i64p, _i64p, ctypes.POINTER(ctypes.c_int)]),
             "lnto_is_equal_fuzzy": (i32, [f64, f64, f64]),
+            "lnto_compare_sparse": (i32, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, f64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -273,6 +274,14 @@ def contract(ak, av, dims_a, seq_a_cur, a_sorted, labels_a, bk, bv, dims_b, seq_
 
 def is_equal_fuzzy(a: float, b: float, prec: float) -> bool:
     return bool(lib().lnto_is_equal_fuzzy(a, b, prec))
+
+
+def compare_sparse(ak, av, bk, bv, mode=0, prec=1e-6) -> bool:
+    """compare_with_sparse (SparseMIABase.h:1243-1296) of two operands sorted in
+    one linearisation; mode 0 exact, 1 isEqualFuzzy(prec)."""
+    ak, av, bk, bv = _i64(ak), _f64(av), _i64(bk), _f64(bv)
+    return bool(lib().lnto_compare_sparse(_p(ak, _i64p), _p(av, _f64p), len(ak), _p(bk, _i64p), _p(bv, _f64p), len(bk),
+                                          int(mode), float(prec)))
 
 
 def allclose_fuzzy(a, b, prec=1e-12) -> bool:

@@ -726,6 +726,33 @@ int lnto_is_equal_fuzzy(double a, double b, double prec) {
   return fabs(a - b) <= m * prec;
 }
 
+/* below_tolerance (SparseMIABase.h:1170-1175) with SparseTolerance<double>
+ * (LibMIAUtil.h:137-150). */
+static int below_tolerance(double v) { return fabs(v) <= 1.11e-16; }
+
+/* SparseMIABase.h:1245-1296: the two-pointer walk, restated step for step. */
+int lnto_compare_sparse(const int64_t *ak, const double *av, int64_t na, const int64_t *bk, const double *bv,
+                        int64_t nb, int mode, double prec) {
+  int64_t i = 0, j = 0;
+  while (i < na && j < nb) {
+    if (below_tolerance(av[i])) {
+      ++i;
+    } else if (below_tolerance(bv[j])) {
+      ++j;
+    } else {
+      if (ak[i] != bk[j]) return 0;
+      if (mode == 1 ? !lnto_is_equal_fuzzy(av[i], bv[j], prec) : !(av[i] == bv[j])) return 0;
+      ++i;
+      ++j;
+    }
+  }
+  while (i < na)
+    if (!below_tolerance(av[i++])) return 0;
+  while (j < nb)
+    if (!below_tolerance(bv[j++])) return 0;
+  return 1;
+}
+
 /* ------------------------------ cost model ------------------------------ */
 enum {
   MF = LNTO_MLP_F, MH = LNTO_MLP_H, MS = LNTO_MLP_S,

@@ -133,6 +133,11 @@ double lnto_mlp_loss_grad(const double *params, const double *X, const double *T
 
 /* ------------------------------- comparison ----------------------------- */
 int lnto_is_equal_fuzzy(double a, double b, double prec);
+/* compare_with_sparse (SparseMIABase.h:1243-1296) on two operands already
+ * sorted in one linearisation: mode 0 exact (operator==, :1616-1623), mode 1
+ * isEqualFuzzy with prec (fuzzy_equals, :1627-1635).  Returns 1 or 0. */
+int lnto_compare_sparse(const int64_t *ak, const double *av, int64_t na, const int64_t *bk, const double *bv,
+                        int64_t nb, int mode, double prec);
 
 #ifdef __cplusplus
 }

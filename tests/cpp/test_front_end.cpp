@@ -3,6 +3,7 @@
 // sparse_mia_add_sub_tests.cpp, sparse_mia_mult_test.cpp): every sparse result
 // is compared with dense brute force computed here on the CPU.
 #include <cmath>
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -139,6 +140,27 @@ int main() {
       thrown = true;
     }
     CHECK(thrown, "dimension mismatch not reported");
+  }
+  // ---- comparison (SparseMIABase.h:1243-1296, 1616-1635)
+  {
+    lnt::SparseMIA<double, 3> a(6, 7, 8), b(6, 7, 8);
+    fill_random(a, 100, 21);
+    b = a;
+    b.sort(std::array<size_t, 3>{{2, 0, 1}});  // same tensor, another order
+    CHECK(a == b, "a tensor differs from itself in another order");
+    int64_t absent = 0;  // a key that a does not hold
+    while (std::find(a.indices().begin(), a.indices().end(), absent) != a.indices().end()) ++absent;
+    b.change_linIdx_sequence(a.linIdxSequence());
+    b.push_back(1e-17, absent);  // below the sparse tolerance: counts as absent
+    CHECK(a == b, "an entry below the tolerance broke equality");
+    lnt::SparseMIA<double, 3> c(6, 7, 8);
+    c = a;
+    c.data_begin()[3] *= 1.0 + 1e-9;
+    c.setSorted(false);
+    CHECK(!(a == c), "a changed value compared equal");
+    CHECK(a.fuzzy_equals(c, 1e-6), "fuzzy_equals rejected a tiny change");
+    lnt::SparseMIA<double, 3> d(6, 7, 9);
+    CHECK(!(a == d), "different dims compared equal");
   }
   if (failures) return 1;
   std::printf("C++ front end: all checks passed\n");

@@ -209,6 +209,28 @@ def test_sorted_low_modes_lemma(orc, dims, cur, new):
         assert not np.array_equal(rk[np.argsort(rk >> wrong, kind="stable")], full)
 
 
+def test_compare_sparse_known_answers(orc):
+    """compare_with_sparse (SparseMIABase.h:1243-1296) on hand-built cases whose
+    answers follow from its definition: entries with |v| <= 1.11e-16 are absent,
+    the rest must pair up key for key with the predicate true."""
+    k = np.array([2, 5, 9, 11], dtype=np.int64)
+    v = np.array([1.5, -0.25, 3.0, 0.75])
+    c = orc.compare_sparse
+    assert c(k, v, k, v)
+    assert not c(k, v, k, v + np.array([0, 0, 1e-9, 0]))                    # exact
+    assert c(k, v, k, v + np.array([0, 0, 1e-9, 0]), 1, 1e-6)                # relative, |a|>=1
+    assert not c(k, v, k, v + np.array([0, 2e-6, 0, 0]), 1, 1e-6)            # absolute, |a|<1
+    assert c(k, v, k, v + np.array([0, 5e-7, 0, 0]), 1, 1e-6)
+    assert c(k, v, np.array([2, 5, 7, 9, 11]), np.array([1.5, -0.25, 1e-17, 3.0, 0.75]))   # absent extra
+    assert c(k, v, np.array([2, 5, 7, 9, 11]), np.array([1.5, -0.25, -1.11e-16, 3.0, 0.75]))
+    assert not c(k, v, np.array([2, 5, 7, 9, 11]), np.array([1.5, -0.25, 2e-16, 3.0, 0.75]))
+    assert not c(k, v, k[:3], v[:3])                                         # missing key
+    assert c(np.append(k, 20), np.append(v, 0.0), k, v)                      # trailing zero
+    assert not c(k, v, np.array([2, 5, 10, 11]), v)                          # key differs
+    assert not c(k, np.array([1.5, np.nan, 3.0, 0.75]), k, np.array([1.5, np.nan, 3.0, 0.75]))
+    assert c(np.zeros(0, np.int64), np.zeros(0), np.zeros(0, np.int64), np.zeros(0))
+
+
 def test_collect_duplicates(orc):
     keys = np.array([1, 1, 2, 5, 5, 5, 9], dtype=np.int64)
     vals = np.array([1.0, 2.0, 3.0, 1.0, 1.0, 1.0, -4.0])
