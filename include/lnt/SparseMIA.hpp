@@ -340,6 +340,14 @@ class AssignTarget {
     return *this;
   }
   AssignTarget &operator=(const AssignTarget &rhs) { return *this = Atom(rhs); }
+  // Destructive merge A(...) += B(...) / -= (SparseMIA::merge,
+  // SparseMIA.h:1030-1050: a scan merge when either operand is sorted, a sort
+  // merge otherwise) -- the merge of this target and the right-hand side,
+  // assigned back to this target.
+  AssignTarget &operator+=(const Atom &rhs) { return *this = (Atom(*this) + rhs); }
+  AssignTarget &operator-=(const Atom &rhs) { return *this = (Atom(*this) - rhs); }
+  AssignTarget &operator+=(const AssignTarget &rhs) { return *this += Atom(rhs); }
+  AssignTarget &operator-=(const AssignTarget &rhs) { return *this -= Atom(rhs); }
 
  private:
   MIA &m_;

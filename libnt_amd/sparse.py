@@ -195,6 +195,20 @@ class Atom:
     def __sub__(self, other):
         return self._merge(other, OP_SUB)
 
+    def _merge_into_self(self, other: "Atom", op: int) -> "Atom":
+        """Destructive A(...) += / -= B(...) (SparseMIA::merge,
+        SparseMIA.h:1030-1050): the merge, written back into this tensor."""
+        c = self._merge(other, op)
+        m = self.mia
+        m.keys, m.vals, m.seq, m.is_sorted = c.keys, c.vals, list(c.seq), c.is_sorted
+        return self
+
+    def __iadd__(self, other):
+        return self._merge_into_self(other, OP_ADD)
+
+    def __isub__(self, other):
+        return self._merge_into_self(other, OP_SUB)
+
     # ----------------------------------------------------------- products
     def plan(self, other: "Atom"):
         """Lattice bookkeeping of MIA_Expr.h:449-561 (see ``batch.lattice_plan``):

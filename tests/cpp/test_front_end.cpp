@@ -141,6 +141,32 @@ int main() {
     }
     CHECK(thrown, "dimension mismatch not reported");
   }
+  // ---- destructive merges A(i,j,k) += / -= B(...) (SparseMIA.h:1030-1050)
+  {
+    lnt::SparseMIA<double, 3> a(5, 6, 7), b(7, 5, 6), a0(5, 6, 7);
+    fill_random(a, 60, 31);
+    fill_random(b, 70, 32);
+    a0 = a;
+    std::vector<double> da = dense(a0), db = dense(b);
+    a(i, j, k) += b(k, i, j);
+    std::vector<double> dc = dense(a);
+    for (int64_t ii = 0; ii < 5; ++ii)
+      for (int64_t jj = 0; jj < 6; ++jj)
+        for (int64_t kk = 0; kk < 7; ++kk)
+          CHECK(dc[ii + 5 * (jj + 6 * kk)] == da[ii + 5 * (jj + 6 * kk)] + db[kk + 7 * (ii + 5 * jj)], "A += B mismatch");
+    a(i, j, k) -= b(k, i, j);  // per key: (a + b) - b, or a where b has no entry
+    std::vector<double> dd = dense(a);
+    for (int64_t ii = 0; ii < 5; ++ii)
+      for (int64_t jj = 0; jj < 6; ++jj)
+        for (int64_t kk = 0; kk < 7; ++kk) {
+          const int64_t q = ii + 5 * (jj + 6 * kk);
+          CHECK(dd[q] == dc[q] - db[kk + 7 * (ii + 5 * jj)], "A -= B mismatch");
+        }
+    lnt::SparseMIA<double, 3> z(5, 6, 7);
+    z = a0;
+    z(i, j, k) -= a0(i, j, k);
+    for (double v : dense(z)) CHECK(v == 0.0, "A -= A is not zero");
+  }
   // ---- comparison (SparseMIABase.h:1243-1296, 1616-1635)
   {
     lnt::SparseMIA<double, 3> a(6, 7, 8), b(6, 7, 8);

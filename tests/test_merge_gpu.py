@@ -252,3 +252,38 @@ def test_front_end_sort_merge_sequence(orc):
         want_seq = [2, 0, 1] if na > nb else [0, 1, 2]
         assert C.seq == want_seq and C.is_sorted
         assert np.array_equal(C.to_dense(), Ad + Bd.transpose(1, 2, 0))
+
+
+def test_front_end_destructive_merge(orc):
+    """A("ijk") += B("kij") and -= write the merge back into A
+    (SparseMIA::merge, SparseMIA.h:1030-1050), sorted or not."""
+    from libnt_amd.sparse import SparseMIA
+
+    for a_sorted, b_sorted in ((True, True), (False, False), (True, False)):
+        ka = orc.gen_distinct_keys(80, 5 * 6 * 7, 41)
+        kb = orc.gen_distinct_keys(90, 7 * 5 * 6, 42)
+        va, vb = orc.gen_uniform_vals(80, 43), orc.gen_uniform_vals(90, 44)
+        A = SparseMIA([5, 6, 7], ka, va)
+        B = SparseMIA([7, 5, 6], kb, vb)
+        if a_sorted:
+            A.sort()
+        if b_sorted:
+            B.sort()
+        da = np.zeros(5 * 6 * 7)
+        da[ka] = va
+        db = np.zeros(7 * 5 * 6)
+        db[kb] = vb
+        x = A("ijk")
+        x += B("kij")
+        A.sort([0, 1, 2])
+        got = np.zeros(5 * 6 * 7)
+        got[A.keys.cpu().numpy()] = A.vals.cpu().numpy()
+        want = np.zeros(5 * 6 * 7)
+        for i in range(5):
+            for j in range(6):
+                for k in range(7):
+                    want[i + 5 * (j + 6 * k)] = da[i + 5 * (j + 6 * k)] + db[k + 7 * (i + 5 * j)]
+        assert np.array_equal(got, want), (a_sorted, b_sorted)
+        y = A("ijk")
+        y -= A.copy()("ijk")
+        assert not np.any(A.vals.cpu().numpy())
