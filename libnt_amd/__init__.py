@@ -38,6 +38,8 @@ _u64 = ctypes.c_uint64
 _int = ctypes.c_int
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f64 = ctypes.c_double
 
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
@@ -96,6 +98,14 @@ SIGNATURES = {
     "lnt_model_gradient": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _i64,
                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "lnt_context_set_model": [_vp, _vp],
+    # include/lnt_mex.h (the alloc argument is a C function pointer; None = calloc)
+    "lnt_mex_sparse_sort": [_vp, _vp, _vp, _i64],
+    "lnt_mex_sparse_change_linidx": [_vp, _vp, _i64, _u8p, _u8p, _i64p, _int],
+    "lnt_mex_sparse_permute": [_vp, _vp, _vp, _i64, _u8p, _u8p, _i64p, _int],
+    "lnt_mex_sparse_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _f64, _vp, ctypes.POINTER(_vp),
+                             ctypes.POINTER(_vp), _i64p],
+    "lnt_mex_sparselattice_mult": [_vp, _vp, _vp, _i64, _i64p, _vp, _vp, _i64, _i64p, _f64, _vp,
+                                   ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p],
 }
 _RESTYPES = {"lnt_last_error": ctypes.c_char_p, "lnt_model_param_count": ctypes.c_int64}
 

@@ -152,38 +152,6 @@ static void prof_clear(lnt_context ctx) {
 
 namespace {
 
-struct DeviceGuard {
-  int prev = -1;
-  bool ok = true;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
-  }
-  ~DeviceGuard() {
-    int cur = -1;
-    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-  }
-};
-
-#define LNT_CTX(ctx)                                                       \
-  if (!(ctx)) return fail(LNT_ERR_INVALID_ARGUMENT, "null context");      \
-  DeviceGuard _guard((ctx)->device);                                       \
-  if (!_guard.ok) return fail(LNT_ERR_DEVICE, "hipSetDevice failed")
-
-int check_mem(int mem) {
-  if (mem != LNT_MEM_HOST && mem != LNT_MEM_DEVICE)
-    return fail(LNT_ERR_INVALID_ARGUMENT, "mem must be LNT_MEM_HOST or LNT_MEM_DEVICE");
-  return LNT_OK;
-}
-
-// Per-call device staging for host-memory arguments.
-struct Staging {
-  DeviceBuffer b[6];
-  ~Staging() {
-    for (auto &x : b) release(x);
-  }
-};
-
 __global__ void check_sorted_kernel(const uint64_t *keys, uint32_t n, uint32_t *err) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool bad = false;

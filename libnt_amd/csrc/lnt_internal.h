@@ -382,6 +382,40 @@ int ensure(DeviceBuffer &b, size_t bytes);
 void set_alloc_forbidden(bool forbidden);
 void release(DeviceBuffer &b);
 
+// Per-call device staging for host-memory arguments.
+struct Staging {
+  DeviceBuffer b[6];
+  ~Staging() {
+    for (auto &x : b) release(x);
+  }
+};
+
+// Makes `dev` current for the lifetime of the guard.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = true;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+// Entry-point prologue: rejects a null context and makes its device current.
+#define LNT_CTX(ctx)                                                       \
+  if (!(ctx)) return ::lnt::fail(LNT_ERR_INVALID_ARGUMENT, "null context"); \
+  ::lnt::DeviceGuard _guard((ctx)->device);                                \
+  if (!_guard.ok) return ::lnt::fail(LNT_ERR_DEVICE, "hipSetDevice failed")
+
+inline int check_mem(int mem) {
+  if (mem != LNT_MEM_HOST && mem != LNT_MEM_DEVICE)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "mem must be LNT_MEM_HOST or LNT_MEM_DEVICE");
+  return LNT_OK;
+}
+
 struct SortTuning {
   int items_per_thread = 12;  // onesweep tile = sort_block * items
   int hist_blocks = 1024;

@@ -28,9 +28,15 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 
 
+def _stale() -> bool:
+    t = os.path.getmtime(_LIB_PATH)
+    return any(os.path.getmtime(os.path.join(_HERE, f)) > t for f in ("lnt_oracle.c", "lnt_oracle.h", "Makefile"))
+
+
 def build(force: bool = False) -> str:
-    """Compile the oracle with gcc (no GPU, no reference sources needed)."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile the oracle with gcc (no GPU, no reference sources needed); make
+    rebuilds it when its sources are newer than the library."""
+    if force or not os.path.exists(_LIB_PATH) or _stale():
         subprocess.check_call(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE])
     return _LIB_PATH
 
@@ -68,6 +74,7 @@ def lib():
             "lnto_sort_seq": (None, [_i64p, _f64p, i64, i32, _i64p, _i32p, _i32p, i32]),
             "lnto_collect_duplicates": (i64, [_i64p, _f64p, i64]),
             "lnto_merge": (i64, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, _i64p, _f64p]),
+            "lnto_mex_merge": (i64, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, _i64p, _f64p]),
             "lnto_outer_inter_count": (i64, [_i64p, i64, i64, _i64p, i64, i64]),
             "lnto_outer_inter_product": (i64, [_i64p, _f64p, i64, i64, _i64p, _f64p, i64, i64, _i64p, _f64p]),
             "lnto_lattice_nprod": (i64, [_i64p, i64, i64, i64, _i64p, i64, i64, i64]),
@@ -212,6 +219,17 @@ def merge(ak, av, bk, bv, op):
     cv = np.empty(max(cap, 1), dtype=np.float64)
     n = lib().lnto_merge(_p(ak, _i64p), _p(av, _f64p), len(ak), _p(bk, _i64p), _p(bv, _f64p), len(bk),
                          op, _p(ck, _i64p), _p(cv, _f64p))
+    return ck[:n].copy(), cv[:n].copy()
+
+
+def mex_merge(ak, av, bk, bv, op):
+    """SparseMergeMex (op 0 add, 1 subtract), its empty-operand cases included."""
+    ak, av, bk, bv = _i64(ak), _f64(av), _i64(bk), _f64(bv)
+    cap = max(len(ak) + len(bk), 1)
+    ck = np.empty(cap, dtype=np.int64)
+    cv = np.empty(cap, dtype=np.float64)
+    n = lib().lnto_mex_merge(_p(ak, _i64p), _p(av, _f64p), len(ak), _p(bk, _i64p), _p(bv, _f64p), len(bk),
+                             op, _p(ck, _i64p), _p(cv, _f64p))
     return ck[:n].copy(), cv[:n].copy()
 
 

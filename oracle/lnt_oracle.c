@@ -519,6 +519,29 @@ int64_t lnto_merge(const int64_t *ak, const double *av, int64_t na,
   return c;
 }
 
+/* src/MATLAB/mex/sparse_merge.cpp:16-85 (SparseMergeMex): the general merge
+ * (latA + latB / latA - latB, :58-84) behind the mex's own shortcuts for an
+ * empty operand: both empty -> empty (:26-28), B empty -> a copy of A
+ * (:29-39), A empty -> B for op 0, -1*b for op 1 (:40-57). */
+int64_t lnto_mex_merge(const int64_t *ak, const double *av, int64_t na,
+                       const int64_t *bk, const double *bv, int64_t nb, int op,
+                       int64_t *ck, double *cv) {
+  if (na == 0 && nb == 0) return 0;
+  if (nb == 0) {
+    memcpy(ck, ak, (size_t)na * sizeof(*ck));
+    memcpy(cv, av, (size_t)na * sizeof(*cv));
+    return na;
+  }
+  if (na == 0) {
+    for (int64_t j = 0; j < nb; ++j) {
+      ck[j] = bk[j];
+      cv[j] = op == 0 ? bv[j] : -1 * bv[j];
+    }
+    return nb;
+  }
+  return lnto_merge(ak, av, na, bk, bv, nb, op, ck, cv);
+}
+
 /* SparseMIABase.h:1717-1826 perform_noLatticeMult after both operands were
  * sorted to [outer, inter] order (:1746-1747): for every inter index present in
  * both, the outer product of the two groups, B outer loop, A inner loop, key

@@ -81,6 +81,10 @@ enum { LNTO_OP_ADD = 0, LNTO_OP_SUB = 1, LNTO_OP_HADAMARD = 2 };
 int64_t lnto_merge(const int64_t *ak, const double *av, int64_t na,
                    const int64_t *bk, const double *bv, int64_t nb, int op,
                    int64_t *ck, double *cv);
+/* SparseMergeMex: lnto_merge (op 0/1) behind the mex's empty-operand cases. */
+int64_t lnto_mex_merge(const int64_t *ak, const double *av, int64_t na,
+                       const int64_t *bk, const double *bv, int64_t nb, int op,
+                       int64_t *ck, double *cv);
 int64_t lnto_outer_inter_count(const int64_t *ak, int64_t na,
                                int64_t l_outer_size, const int64_t *bk,
                                int64_t nb, int64_t r_outer_size);

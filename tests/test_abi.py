@@ -41,7 +41,7 @@ def test_abi_version_and_null_context_errors():
 
 def test_header_compiles_as_c_and_cxx(tmp_path):
     src = tmp_path / "t.c"
-    src.write_text('#include "lnt.h"\n#include "lnt_testgen.h"\nint main(void){return lnt_abi_version()==LNT_ABI_VERSION?0:1;}\n')
+    src.write_text('#include "lnt.h"\n#include "lnt_testgen.h"\n#include "lnt_mex.h"\nint main(void){return lnt_abi_version()==LNT_ABI_VERSION?0:1;}\n')
     for cc in ("gcc", "g++"):
         exe = tmp_path / ("t_" + cc)
         cmd = [cc, "-x", "c" if cc == "gcc" else "c++", str(src), "-I", os.path.join(ROOT, "include"),

@@ -231,6 +231,33 @@ def test_compare_sparse_known_answers(orc):
     assert c(np.zeros(0, np.int64), np.zeros(0), np.zeros(0, np.int64), np.zeros(0))
 
 
+def test_mex_merge_known_answers(orc):
+    """SparseMergeMex (src/MATLAB/mex/sparse_merge.cpp:16-85): with both
+    operands present it is the general merge; an empty operand takes the mex's
+    shortcuts, told apart from the general merge by signed zeros: a subtraction
+    from nothing is -1*b (so +0.0 -> -0.0), where the merge computes 0 - b
+    (+0.0 -> +0.0), and an empty B hands A back unchanged (-0.0 stays)."""
+    sb = lambda a: np.signbit(np.asarray(a)).tolist()
+    k = np.array([1, 4, 6], dtype=np.int64)
+    v = np.array([0.0, -0.0, 2.0])
+    e_k, e_v = np.zeros(0, np.int64), np.zeros(0)
+    ck, cv = orc.mex_merge(e_k, e_v, k, v, 1)
+    assert ck.tolist() == [1, 4, 6] and cv.tolist() == [0.0, 0.0, -2.0] and sb(cv) == [True, False, True]
+    gk, gv = orc.merge(e_k, e_v, k, v, 1)  # the general merge: 0 - b
+    assert gk.tolist() == [1, 4, 6] and sb(gv) == [False, False, True]
+    ck, cv = orc.mex_merge(e_k, e_v, k, v, 0)
+    assert sb(cv) == [False, True, False]
+    gk, gv = orc.merge(e_k, e_v, k, v, 0)  # 0 + b loses -0.0
+    assert sb(gv) == [False, False, False]
+    ck, cv = orc.mex_merge(k, v, e_k, e_v, 1)
+    assert ck.tolist() == [1, 4, 6] and sb(cv) == [False, True, False]
+    assert len(orc.mex_merge(e_k, e_v, e_k, e_v, 0)[0]) == 0
+    # both present: the general merge, by hand
+    ck, cv = orc.mex_merge(k, np.array([1.0, 2.0, 3.0]), np.array([0, 4, 9], np.int64),
+                           np.array([5.0, 7.0, 1.0]), 1)
+    assert ck.tolist() == [0, 1, 4, 6, 9] and cv.tolist() == [-5.0, 1.0, -5.0, 3.0, -1.0]
+
+
 def test_collect_duplicates(orc):
     keys = np.array([1, 1, 2, 5, 5, 5, 9], dtype=np.int64)
     vals = np.array([1.0, 2.0, 3.0, 1.0, 1.0, 1.0, -4.0])
