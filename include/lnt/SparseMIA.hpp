@@ -69,7 +69,9 @@ inline Context &default_context() {
 }
 
 // ------------------------------------------------------------------ index --
-// An index label.  `!i` marks an inter-product (element-wise) index.
+// An index label.  `!i` marks an inter-product (element-wise) index; each `!`
+// adds one to its element-wise count (Index.h:38-57), which `~` lowers again
+// after a product (MIA_Expr.h:257-301, 1376).
 template <size_t ID, int ElemWise = 0>
 struct ProdInd {
   static constexpr size_t id = ID;
@@ -78,8 +80,9 @@ struct ProdInd {
 };
 struct Label {
   size_t id;
-  bool inter;
-  bool operator==(const Label &o) const { return id == o.id && inter == o.inter; }
+  int bangs;  // element-wise count: 0 = plain index
+  bool inter() const { return bangs > 0; }
+  bool operator==(const Label &o) const { return id == o.id && bangs == o.bangs; }
 };
 
 template <class T, size_t N>
@@ -124,6 +127,9 @@ struct Tensor {
 struct Expr {
   std::shared_ptr<Tensor> t;  // owning result or a view of a SparseMIA's storage
   std::vector<Label> labels;
+  // Inter indices a product left at the back of `labels` (the reference's
+  // inter_product_number, MIA_Expr.h:903-920): the ones `~` decrements.
+  size_t inter_count = 0;
 };
 
 inline std::vector<int32_t> identity(size_t n) {
@@ -153,7 +159,7 @@ inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const s
   std::vector<int32_t> index_order(n), lhs_order(n);
   for (size_t m = 0; m < n; ++m) {
     int r = find_label(lb, la[m].id);
-    if (r < 0 || la[m].inter || lb[r].inter)
+    if (r < 0 || la[m].inter() || lb[r].inter())
       throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "merge operands need the same plain indices");
     if (a.dims[m] != b.dims[r]) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "merge dims differ");
     index_order[m] = r;
@@ -227,14 +233,16 @@ inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, cons
   for (size_t m = 0; m < la.size(); ++m) {
     int r = find_label(lb, la[m].id);
     if (r < 0) {
-      if (la[m].inter) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "inter index in one operand only");
+      if (la[m].inter()) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "inter index in one operand only");
       a_out.push_back((int32_t)m);
       continue;
     }
-    if (la[m].inter != lb[r].inter)
-      throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "an index is inter on one side only");
+    // One id with different element-wise counts may not match
+    // (product_rule_different_elem_wise, Index.h:95-101).
+    if (la[m].bangs != lb[r].bangs)
+      throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "an index has different element-wise counts on the two sides");
     if (a.dims[m] != b.dims[r]) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "matched dims differ");
-    if (la[m].inter) {
+    if (la[m].inter()) {
       a_int.push_back((int32_t)m);
       b_int.push_back(r);
     } else {
@@ -244,7 +252,7 @@ inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, cons
   }
   for (size_t r = 0; r < lb.size(); ++r)
     if (find_label(la, lb[r].id) < 0) {
-      if (lb[r].inter) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "inter index in one operand only");
+      if (lb[r].inter()) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "inter index in one operand only");
       b_out.push_back((int32_t)r);
     }
   std::vector<int32_t> seq_a, seq_b;
@@ -262,7 +270,8 @@ inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, cons
   for (int32_t m : a_out) c->dims.push_back(a.dims[m]), lc.push_back(la[m]);
   for (int32_t r : b_out) c->dims.push_back(b.dims[r]), lc.push_back(lb[r]);
   for (int32_t m : a_int) c->dims.push_back(a.dims[m]), lc.push_back(la[m]);
-  if (c->dims.empty()) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "full contraction to a scalar");
+  // Every index contracted: a 1 x 1 x 1 lattice product, the scalar of the
+  // reference's complete inner product (sparse_mia_mult_test.cpp:159-165).
   c->seq = identity(c->dims.size());
   c->sorted = true;
   a.sort_to(seq_a);
@@ -283,7 +292,7 @@ inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, cons
   lnt_free(default_context().get(), cv, LNT_MEM_HOST);
   a.sorted = false;  // lattice unfolding leaves operands flagged unsorted
   b.sorted = false;
-  return {c, lc};
+  return {c, lc, a_int.size()};
 }
 
 }  // namespace detail
@@ -295,6 +304,22 @@ class Atom {
   Atom operator+(const Atom &o) const { return detail::merge(*e_.t, e_.labels, *o.e_.t, o.e_.labels, LNT_OP_ADD); }
   Atom operator-(const Atom &o) const { return detail::merge(*e_.t, e_.labels, *o.e_.t, o.e_.labels, LNT_OP_SUB); }
   Atom operator*(const Atom &o) const { return detail::multiply(*e_.t, e_.labels, *o.e_.t, o.e_.labels); }
+  // ~(A*B): the product's inter indices lose one `!` (MIA_Expr.h:257-301,
+  // 1376), so `!j` becomes a plain index of the next product and `!!l` an
+  // inter index again.  No data moves.
+  Atom operator~() const {
+    detail::Expr e = e_;
+    for (size_t q = e.labels.size() - e.inter_count; q < e.labels.size(); ++q)
+      e.labels[q].bangs = std::max(e.labels[q].bangs - 1, 0);
+    e.inter_count = 0;
+    return Atom(e);
+  }
+  // The value of a complete inner product (every index contracted).
+  operator double() const {
+    if (!e_.labels.empty())
+      throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "only a complete inner product is a scalar");
+    return e_.t->data.empty() ? 0.0 : e_.t->data[0];
+  }
   const detail::Expr &expr() const { return e_; }
 
  private:
@@ -314,6 +339,8 @@ class AssignTarget {
   Atom operator+(const AssignTarget &o) const { return Atom(*this) + Atom(o); }
   Atom operator-(const AssignTarget &o) const { return Atom(*this) - Atom(o); }
   Atom operator*(const AssignTarget &o) const { return Atom(*this) * Atom(o); }
+  // Assignment matches indices by id, whatever their element-wise counts
+  // (MIA_Expr.h:112-135).
   AssignTarget &operator=(const Atom &rhs) {
     const detail::Expr &e = rhs.expr();
     if (e.labels.size() != labels_.size())
@@ -462,7 +489,7 @@ class SparseMIA {
   template <class... Ts>
   AssignTarget<SparseMIA> operator()(Ts...) {
     static_assert(sizeof...(Ts) == N, "one index per mode");
-    return AssignTarget<SparseMIA>(*this, std::vector<Label>{Label{Ts::id, Ts::elemval > 0}...});
+    return AssignTarget<SparseMIA>(*this, std::vector<Label>{Label{Ts::id, Ts::elemval}...});
   }
 
   std::shared_ptr<detail::Tensor> tensor_ptr() { return t_; }

@@ -4,10 +4,12 @@
 167-773): dims, int64 linear indices and double values (here GPU tensors), the
 mode order of the indices (``linIdxSequence``, SparseMIABase.h:98-153) and a
 sorted flag.  ``A("ijkl")`` mirrors ``A(i,j,k,l)`` (MIA.h:222-229) and returns
-an ``Atom`` on which ``+``, ``-`` and ``*`` follow the reference's expression
-rules (MIA_Expr.h); an upper-case label is an inter index (the reference's
-``!i``, Index.h:54).  Every operation runs on the GPU through the C-ABI
-(include/lnt.h): change_sort_order, merge, lattice product.
+an ``Atom`` on which ``+``, ``-``, ``*`` and ``~`` follow the reference's
+expression rules (MIA_Expr.h).  A label is a letter preceded by its
+element-wise marks: ``A("i!jk!!l")`` mirrors ``A(i,!j,k,!!l)`` (Index.h:38-57);
+an upper-case letter is one mark (``"iJ"`` = ``"i!j"``).  Every operation runs
+on the GPU through the C-ABI (include/lnt.h): change_sort_order, merge,
+lattice product.
 """
 from __future__ import annotations
 
@@ -30,6 +32,35 @@ def _prod(xs):
     for x in xs:
         p *= int(x)
     return p
+
+
+def parse_labels(labels):
+    """``"i!jk!!l"`` -> ``[("i", 0), ("j", 1), ("k", 0), ("l", 2)]``: one
+    (letter, element-wise count) per mode; an upper-case letter counts one
+    mark.  A list of such pairs passes through."""
+    if not isinstance(labels, str):
+        return [(str(c), int(b)) for c, b in labels]
+    out, bangs = [], 0
+    for ch in labels:
+        if ch == "!":
+            bangs += 1
+            continue
+        if not ch.isalpha():
+            raise ValueError("index labels are letters, optionally preceded by '!': %r" % labels)
+        out.append((ch.lower(), bangs + (1 if ch.isupper() else 0)))
+        bangs = 0
+    if bangs:
+        raise ValueError("a trailing '!' in %r" % labels)
+    letters = [c for c, _ in out]
+    if len(set(letters)) != len(letters):
+        raise ValueError("an index label repeats within one operand: %r" % labels)
+    return out
+
+
+def _plan_string(tokens) -> str:
+    """Plain labels lower case, element-wise ones upper case (the form the
+    lattice planner reads)."""
+    return "".join(c.upper() if b else c for c, b in tokens)
 
 
 class SparseMIA:
@@ -118,10 +149,11 @@ class SparseMIA:
         return self._compare(other, precision)
 
     # -------------------------------------------------------------- helpers
-    def __call__(self, labels: str) -> "Atom":
-        if len(labels) != self.order:
+    def __call__(self, labels) -> "Atom":
+        tokens = parse_labels(labels)
+        if len(tokens) != self.order:
             raise ValueError("expected %d index labels, got %r" % (self.order, labels))
-        return Atom(self, labels)
+        return Atom(self, tokens)
 
     def to_dense(self):
         """Dense numpy copy in the default (first mode fastest) layout."""
@@ -141,18 +173,34 @@ class SparseMIA:
 
 
 class Atom:
-    """A(i,j,...) -- a tensor with index labels (MIA_Atom, MIA_Expr.h:663-689)."""
+    """A(i,j,...) -- a tensor with index labels (MIA_Atom, MIA_Expr.h:663-689).
 
-    def __init__(self, mia: SparseMIA, labels: str):
+    ``tokens`` holds (letter, element-wise count) per mode; ``labels`` is the
+    same as a string, element-wise letters upper case; ``inter_count`` is the
+    number of inter indices a product left at the back, the ones ``~``
+    lowers (the reference's inter_product_number, MIA_Expr.h:903-920)."""
+
+    def __init__(self, mia: SparseMIA, labels, inter_count: int = 0):
         self.mia = mia
-        self.labels = labels
+        self.tokens = parse_labels(labels)
+        self.labels = _plan_string(self.tokens)
+        self.inter_count = int(inter_count)
+
+    def __invert__(self) -> "Atom":
+        """~(A*B): the product's inter indices lose one mark (MIA_Expr.h:
+        257-301, 1376): ``!j`` becomes a plain index of the next product,
+        ``!!l`` an inter index again.  No data moves."""
+        t = list(self.tokens)
+        for q in range(len(t) - self.inter_count, len(t)):
+            t[q] = (t[q][0], max(t[q][1] - 1, 0))
+        return Atom(self.mia, t, 0)
 
     # ------------------------------------------------------------- merges
     def _merge(self, other: "Atom", op: int) -> SparseMIA:
         """outside_merge (SparseMIABase.h:1357-1455): outside_scanMerge when
         either operand is sorted, sortMerge when neither is."""
         a, b = self.mia, other.mia
-        if sorted(self.labels) != sorted(other.labels) or any(c.isupper() for c in self.labels):
+        if sorted(self.labels) != sorted(other.labels) or any(bg for _, bg in self.tokens + other.tokens):
             raise ValueError("merge operands must carry the same plain labels")
         if b is a:
             b = b.copy()  # SparseMIABase.h:1367-1371
@@ -219,13 +267,18 @@ class Atom:
 
         return lattice_plan(self.mia.dims, self.labels, other.mia.dims, other.labels)
 
-    def __mul__(self, other: "Atom") -> "Atom":
+    def __mul__(self, other: "Atom"):
+        """The product; a complete inner product (every index contracted)
+        returns its value as a float (sparse_mia_mult_test.cpp:159-165)."""
         a, b = self.mia, other.mia
         if b is a:
             b = b.copy()
-        seq_a, seq_b, lat_a, lat_b, dims_c, labels_c, (a_in, a_out, b_out) = self.plan(Atom(b, other.labels))
-        if not dims_c:
-            raise ValueError("a full contraction to a scalar is not a sparse tensor")
+        counts_a, counts_b = dict(self.tokens), dict(other.tokens)
+        for c in counts_a.keys() & counts_b.keys():
+            # product_rule_different_elem_wise (Index.h:95-101)
+            if counts_a[c] != counts_b[c]:
+                raise ValueError("index %r has different element-wise counts on the two sides" % c)
+        seq_a, seq_b, lat_a, lat_b, dims_c, labels_c, (a_in, a_out, b_out) = self.plan(Atom(b, other.tokens))
         a.sort(seq_a)
         b.sort(seq_b)
         if not a_in and not a_out and not b_out:
@@ -237,16 +290,22 @@ class Atom:
         # (SparseMIABase.h:1507, SparseMIA.h:1063-1122).
         a.is_sorted = False
         b.is_sorted = False
+        if not dims_c:
+            return float(cv[0].item()) if cv.numel() else 0.0
+        tokens_c = [(ch.lower(), counts_a.get(ch.lower(), counts_b.get(ch.lower()))) for ch in labels_c]
         c = SparseMIA(dims_c, ck.clone(), cv.clone(), None, True, a.ctx)
-        return Atom(c, labels_c)
+        return Atom(c, tokens_c, len(dims_c) - len(a_out) - len(b_out))
 
     # ---------------------------------------------------------- assignment
-    def assign(self, labels: str) -> SparseMIA:
+    def assign(self, labels) -> SparseMIA:
         """C(labels) = expr: relabels C's mode order without moving data
-        (SparseMIA::assign, SparseMIA.h:891-928)."""
-        if sorted(labels) != sorted(self.labels):
+        (SparseMIA::assign, SparseMIA.h:891-928); indices match by letter,
+        whatever their element-wise counts (MIA_Expr.h:112-135)."""
+        mine = [c for c, _ in self.tokens]
+        theirs = [c for c, _ in parse_labels(labels)]
+        if sorted(theirs) != sorted(mine):
             raise ValueError("assignment labels differ from the expression's")
-        perm = [self.labels.index(c) for c in labels]  # new mode m <- old mode perm[m]
+        perm = [mine.index(c) for c in theirs]  # new mode m <- old mode perm[m]
         old = self.mia
         dims = [old.dims[p] for p in perm]
         inv = {p: m for m, p in enumerate(perm)}

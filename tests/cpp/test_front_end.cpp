@@ -49,6 +49,20 @@ static std::vector<double> dense(const lnt::SparseMIA<double, N> &m) {
   return d;
 }
 
+// Dense value at full subscript `sub` of a dense copy with dims `dims`.
+static double at(const std::vector<double> &d, std::initializer_list<int64_t> dims,
+                 std::initializer_list<int64_t> sub) {
+  int64_t k = 0, mult = 1;
+  auto di = dims.begin();
+  for (int64_t s : sub) {
+    k += s * mult;
+    mult *= *di++;
+  }
+  return d[k];
+}
+
+static bool near(double got, double want) { return std::fabs(got - want) <= 1e-12 * std::max(1.0, std::fabs(want)); }
+
 int main() {
   NTINDEX i; NTINDEX j; NTINDEX k; NTINDEX l; NTINDEX m; NTINDEX n;
   // ---- sort(seq) == change_linIdx_sequence(seq) + sort()  (sparse_mia_functions.cpp:100-202)
@@ -187,6 +201,85 @@ int main() {
     CHECK(a.fuzzy_equals(c, 1e-6), "fuzzy_equals rejected a tiny change");
     lnt::SparseMIA<double, 3> d(6, 7, 9);
     CHECK(!(a == d), "different dims compared equal");
+  }
+  // ---- chained products with ~ and repeated element-wise indices, complete
+  //      inner products (sparse_mia_mult_test.cpp:135-165)
+  {
+    const int64_t I = 3, J = 4, K = 5, L = 6, M = 3, N = 2;
+    lnt::SparseMIA<double, 4> a(I, J, K, L), b(J, L, M, N), c(I, J, K, L), ct(I, K, M, N);
+    lnt::SparseMIA<double, 2> b2(J, L), d2(J, L), d2t(L, J), sq(J, J);
+    lnt::SparseMIA<double, 1> d(L);
+    lnt::SparseMIA<double, 3> c3(J, J, J);
+    fill_random(a, 250, 31);
+    fill_random(b, 100, 32);
+    fill_random(b2, 15, 33);
+    fill_random(d2, 14, 34);
+    fill_random(d2t, 13, 35);
+    fill_random(d, 5, 36);
+    fill_random(sq, 9, 37);
+    auto da = dense(a), db = dense(b), db2 = dense(b2), dd = dense(d), dd2 = dense(d2), dd2t = dense(d2t);
+    auto dsq = dense(sq);
+    // Repeated Element-Wise Product 1: c(i,j,k,l) = ~(a(i,!j,k,!!l)*b2(!j,!!l))*d(!l)
+    c(i, j, k, l) = ~(a(i, !j, k, !!l) * b2(!j, !!l)) * d(!l);
+    auto dc = dense(c);
+    for (int64_t ii = 0; ii < I; ++ii)
+      for (int64_t jj = 0; jj < J; ++jj)
+        for (int64_t kk = 0; kk < K; ++kk)
+          for (int64_t ll = 0; ll < L; ++ll) {
+            double want = at(da, {I, J, K, L}, {ii, jj, kk, ll}) * at(db2, {J, L}, {jj, ll}) * dd[ll];
+            CHECK(near(at(dc, {I, J, K, L}, {ii, jj, kk, ll}), want), "repeated element-wise product");
+          }
+    // Ternary Inner Products 1 and 2: ~(a(i,!j,k,!l)*b(!j,!l,m,n))*d2(j,l) / d2(l,j)
+    for (int variant = 0; variant < 2; ++variant) {
+      if (variant == 0)
+        ct(i, k, m, n) = ~(a(i, !j, k, !l) * b(!j, !l, m, n)) * d2(j, l);
+      else
+        ct(i, k, m, n) = ~(a(i, !j, k, !l) * b(!j, !l, m, n)) * d2t(l, j);
+      auto dct = dense(ct);
+      for (int64_t ii = 0; ii < I; ++ii)
+        for (int64_t kk = 0; kk < K; ++kk)
+          for (int64_t mm = 0; mm < M; ++mm)
+            for (int64_t nn = 0; nn < N; ++nn) {
+              double want = 0.0;
+              for (int64_t jj = 0; jj < J; ++jj)
+                for (int64_t ll = 0; ll < L; ++ll)
+                  want += at(da, {I, J, K, L}, {ii, jj, kk, ll}) * at(db, {J, L, M, N}, {jj, ll, mm, nn}) *
+                          (variant == 0 ? at(dd2, {J, L}, {jj, ll}) : at(dd2t, {L, J}, {ll, jj}));
+              CHECK(near(at(dct, {I, K, M, N}, {ii, kk, mm, nn}), want), "ternary inner product");
+            }
+    }
+    // Outer/Element-Wise Product with self-multiplication: c3(i,j,l) = sq(i,!j)*sq(!j,l)
+    c3(i, j, l) = sq(i, !j) * sq(!j, l);
+    auto dc3 = dense(c3);
+    for (int64_t ii = 0; ii < J; ++ii)
+      for (int64_t jj = 0; jj < J; ++jj)
+        for (int64_t ll = 0; ll < J; ++ll)
+          CHECK(near(at(dc3, {J, J, J}, {ii, jj, ll}), at(dsq, {J, J}, {ii, jj}) * at(dsq, {J, J}, {jj, ll})),
+                "self element-wise product");
+    // Complete inner products: a(i,j,k,l)*e(i,l,j,k) and with itself
+    lnt::SparseMIA<double, 4> e(I, L, J, K), f(I, J, K, J);
+    fill_random(e, 200, 38);
+    fill_random(f, 150, 39);
+    auto de = dense(e), df = dense(f);
+    double s1 = a(i, j, k, l) * e(i, l, j, k), w1 = 0.0;
+    double s2 = f(i, l, k, j) * f(i, j, k, l), w2 = 0.0;
+    for (int64_t ii = 0; ii < I; ++ii)
+      for (int64_t jj = 0; jj < J; ++jj)
+        for (int64_t kk = 0; kk < K; ++kk) {
+          for (int64_t ll = 0; ll < L; ++ll)
+            w1 += at(da, {I, J, K, L}, {ii, jj, kk, ll}) * at(de, {I, L, J, K}, {ii, ll, jj, kk});
+          for (int64_t ll = 0; ll < J; ++ll)
+            w2 += at(df, {I, J, K, J}, {ii, ll, kk, jj}) * at(df, {I, J, K, J}, {ii, jj, kk, ll});
+        }
+    CHECK(near(s1, w1), "complete inner product");
+    CHECK(near(s2, w2), "complete inner product with itself");
+    bool thrown = false;
+    try {
+      c(i, j, k, l) = a(i, !j, k, l) * b2(j, l);  // one id, different element-wise counts
+    } catch (const std::exception &) {
+      thrown = true;
+    }
+    CHECK(thrown, "mismatched element-wise counts not reported");
   }
   if (failures) return 1;
   std::printf("C++ front end: all checks passed\n");
