@@ -93,12 +93,15 @@ int lnt_abi_version(void);
  *                  1024 takes at most 12 items per thread)
  *   "digit_bits"   widest radix digit per onesweep pass, 4..8 (default 8)
  *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096
- *   "merge_items"  merge-path steps per thread: 8, 12 (default) or 16
- *                  (tile = merge_block*items)
- *   "merge_block"  merge threads per workgroup: 256 or 512 (default)
+ *   "merge_items"  merge-path steps per thread: 8, 12 or 16 (tile =
+ *                  merge_block*items); 0 (default): 16 when both inputs'
+ *                  keys are below 2^32, else 12
+ *   "merge_block"  merge threads per workgroup: 256, 512 (default) or 1024
+ *                  (1024 takes at most 12 items per thread)
  *   "spgemm_small" longest output column (products) kept in LDS, 2..2048
  *   "narrow_keys"  1: keys of at most 32 bits move as 32-bit words between
- *                  radix passes (default); 0: always 64-bit
+ *                  radix passes and sit in LDS as 32-bit words during merges
+ *                  (default); 0: always 64-bit
  * No reference counterpart (the reference hard-codes radix 2048 and the
  * thresholds 3000/50, multi_index_array/LibMIARadix.h:454-457). */
 int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value);

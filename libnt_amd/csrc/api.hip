@@ -241,8 +241,8 @@ int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
   std::string n(name);
   if (n == "sort_items" && (value == 8 || value == 12 || value == 16)) ctx->tuning.items_per_thread = (int)value;
   else if (n == "hist_blocks" && in(1, kMaxHistBlocks)) ctx->tuning.hist_blocks = (int)value;
-  else if (n == "merge_items" && (value == 8 || value == 12 || value == 16)) ctx->tuning.merge_items = (int)value;
-  else if (n == "merge_block" && (value == 256 || value == 512)) ctx->tuning.merge_block = (int)value;
+  else if (n == "merge_items" && (value == 0 || value == 8 || value == 12 || value == 16)) ctx->tuning.merge_items = (int)value;
+  else if (n == "merge_block" && (value == 256 || value == 512 || value == 1024)) ctx->tuning.merge_block = (int)value;
   else if (n == "spgemm_small" && in(2, 2048)) ctx->tuning.spgemm_small = (int)value;
   else if (n == "narrow_keys" && in(0, 1)) ctx->tuning.narrow_keys = (int)value;
   else if (n == "digit_bits" && in(4, 8)) ctx->tuning.digit_bits = (int)value;

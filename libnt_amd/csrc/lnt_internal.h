@@ -419,7 +419,7 @@ inline int check_mem(int mem) {
 struct SortTuning {
   int items_per_thread = 12;  // onesweep tile = sort_block * items
   int hist_blocks = 1024;
-  int merge_items = 12;   // merge tile = merge_block * items
+  int merge_items = 0;    // merge tile = merge_block * items; 0: 16 with 32-bit LDS keys, else 12
   int merge_block = 512;  // merge threads per workgroup (256 or 512)
   int spgemm_small = 2048;  // largest output column handled in LDS
   int narrow_keys = 1;  // 32-bit keys between passes when they fit

@@ -87,7 +87,9 @@ __device__ inline void load_tile_keys(const uint64_t *__restrict__ ak, const uin
   }
 }
 
-template <int BT, int ITEMS>
+// LK: the key type held in LDS -- uint32_t when every key of both inputs is
+// below 2^32 (half the LDS per tile, so more tiles per CU), else uint64_t.
+template <int BT, int ITEMS, typename LK>
 __global__ __launch_bounds__(BT) void merge_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, uint32_t na,
     const uint64_t *__restrict__ bk, const double *__restrict__ bv, uint32_t nb,
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(BT) void merge_kernel(
   constexpr int kTile = BT * ITEMS;
   static_assert(kTile <= (int)kDescIdx, "descriptor indices are 15 bits");
   static_assert(ITEMS <= 32, "emit mask is 32 bits");
-  __shared__ uint64_t s_key[kTile + 1];
+  __shared__ LK s_key[kTile + 1];
   __shared__ uint32_t s_desc[kTile];
   __shared__ uint32_t s_tile, s_i0, s_i1;
   __shared__ uint64_t s_prev_a;
@@ -129,17 +131,17 @@ __global__ __launch_bounds__(BT) void merge_kernel(
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
       const uint32_t x = (uint32_t)u * BT + tid;
-      if (x < la + lb) s_key[x] = lk[u];
+      if (x < la + lb) s_key[x] = (LK)lk[u];
     }
   }
   const bool has_next_b = j1 < nb;
   if (tid == 0) {
-    if (has_next_b) s_key[la + lb] = bk[j1];  // the B key after the slice
+    if (has_next_b) s_key[la + lb] = (LK)bk[j1];  // the B key after the slice
     s_has_prev = i0 > 0;
     s_prev_a = i0 > 0 ? ak[i0 - 1] : 0;
   }
   __syncthreads();
-  const uint64_t *sa = s_key, *sb = s_key + la;
+  const LK *sa = s_key, *sb = s_key + la;
   const uint32_t nlb = lb + (has_next_b ? 1 : 0);  // B keys visible to peeks
   const bool hadamard = op == LNT_OP_HADAMARD;
 
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(BT) void merge_kernel(
     if (x < lim) {
       const uint32_t d = dd[u];
       const bool has_a = (d & kDescA) != 0, has_b = (d & kDescB) != 0;
-      const uint64_t key = s_key[has_a ? (d & kDescIdx) : la + ((d >> 15) & kDescIdx)];
+      const uint64_t key = (uint64_t)s_key[has_a ? (d & kDescIdx) : la + ((d >> 15) & kDescIdx)];
       const double a = va[u], b = vb[u];
       double v;
       if (has_a && has_b)
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(BT) void merge_kernel(
   }
 }
 
-template <int BT, int ITEMS>
+template <int BT, int ITEMS, typename LK>
 int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
                  uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
                  int op, uint64_t *ck, double *cv, uint64_t capacity,
@@ -273,7 +275,7 @@ int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
   prof_end(ctx, pslot);
   LNT_HIP(hipGetLastError());
   *slot = prof_begin(ctx, kProfMerge, 16.0 * ((double)na + nb));
-  merge_kernel<BT, ITEMS><<<tiles, BT, 0, ctx->stream>>>(
+  merge_kernel<BT, ITEMS, LK><<<tiles, BT, 0, ctx->stream>>>(
       ak, av, na, bk, bv, nb, op, ck, cv, splits, (uint64_t *)ctx->status.ptr,
       epoch, counter, total, tiles, capacity);
   prof_end(ctx, *slot);
@@ -357,16 +359,41 @@ int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
   uint32_t *small = (uint32_t *)ctx->small.ptr;
   uint32_t *counter = small + kSmallCounters + 15;
   uint32_t *total = small + kSmallFlags + 1;
+  // The largest keys are the last ones of the two sorted inputs: when both
+  // are below 2^32 the tile keeps 32-bit keys in LDS.
+  bool narrow = false;
+  if (ctx->tuning.narrow_keys) {
+    uint64_t *last = (uint64_t *)(ctx->pinned + 16);
+    last[0] = last[1] = 0;
+    if (na) LNT_HIP(hipMemcpyAsync(last, ak + na - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (nb) LNT_HIP(hipMemcpyAsync(last + 1, bk + nb - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipStreamSynchronize(ctx->stream));
+    narrow = std::max(last[0], last[1]) < (uint64_t(1) << 32);
+  }
   int rc, slot = -1;
-#define LNT_MERGE(BT, IT) \
-  launch_merge<BT, IT>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot)
-  switch (ctx->tuning.merge_block * 100 + ctx->tuning.merge_items) {
-    case 25608: rc = LNT_MERGE(256, 8); break;
-    case 25612: rc = LNT_MERGE(256, 12); break;
-    case 25616: rc = LNT_MERGE(256, 16); break;
-    case 51208: rc = LNT_MERGE(512, 8); break;
-    case 51212: rc = LNT_MERGE(512, 12); break;
-    default: rc = LNT_MERGE(512, 16); break;
+#define LNT_MERGE(BT, IT, LK) \
+  launch_merge<BT, IT, LK>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot)
+  // Measured on 2 x 50M-key merges: 512 x 16 with 32-bit LDS keys (two tiles
+  // per CU), 512 x 12 with 64-bit ones.
+  const int items = ctx->tuning.merge_items ? ctx->tuning.merge_items : (narrow ? 16 : 12);
+  switch (ctx->tuning.merge_block * 100 + items + (narrow ? 1000000 : 0)) {
+    case 25608: rc = LNT_MERGE(256, 8, uint64_t); break;
+    case 25612: rc = LNT_MERGE(256, 12, uint64_t); break;
+    case 25616: rc = LNT_MERGE(256, 16, uint64_t); break;
+    case 51208: rc = LNT_MERGE(512, 8, uint64_t); break;
+    case 51216: rc = LNT_MERGE(512, 16, uint64_t); break;
+    case 102408: rc = LNT_MERGE(1024, 8, uint64_t); break;
+    case 102412: rc = LNT_MERGE(1024, 12, uint64_t); break;
+    case 1025608: rc = LNT_MERGE(256, 8, uint32_t); break;
+    case 1025612: rc = LNT_MERGE(256, 12, uint32_t); break;
+    case 1025616: rc = LNT_MERGE(256, 16, uint32_t); break;
+    case 1051208: rc = LNT_MERGE(512, 8, uint32_t); break;
+    case 1051212: rc = LNT_MERGE(512, 12, uint32_t); break;
+    case 1051216: rc = LNT_MERGE(512, 16, uint32_t); break;
+    case 1102408: rc = LNT_MERGE(1024, 8, uint32_t); break;
+    case 1102412: rc = LNT_MERGE(1024, 12, uint32_t); break;
+    default:  // 512x12; also 1024x16, whose tile does not fit in LDS
+      rc = narrow ? LNT_MERGE(512, 12, uint32_t) : LNT_MERGE(512, 12, uint64_t);
   }
 #undef LNT_MERGE
   if (rc != LNT_OK) return rc;

@@ -67,15 +67,27 @@ def test_merge_dense_interleaving(ctx, orc):
         _check(ctx, orc, ak, av, bk, bv)
 
 
-@pytest.mark.parametrize("block,items", [(256, 8), (256, 12), (256, 16), (512, 8), (512, 12), (512, 16)])
-def test_merge_tile_shapes(ctx, orc, block, items):
-    """Every merge tile shape gives the oracle's result: dense interleaving (equal
-    keys on every tile and thread boundary), long runs of one operand, and a
-    tile-sized operand against a single key."""
-    print("merge tile %dx%d" % (block, items), flush=True)
+@pytest.mark.parametrize("narrow", [1, 0])
+@pytest.mark.parametrize("block,items", [(256, 8), (256, 12), (256, 16), (512, 8), (512, 12), (512, 16),
+                                         (1024, 8), (1024, 12)])
+def test_merge_tile_shapes(ctx, orc, block, items, narrow):
+    """Every merge tile shape, with 32-bit LDS keys allowed or not, gives the
+    oracle's result: dense interleaving (equal keys on every tile and thread
+    boundary), long runs of one operand, a tile-sized operand against a single
+    key, and keys on both sides of 2^32 (largest key 2^32 - 1: 32-bit LDS
+    keys; 2^32: 64-bit)."""
+    print("merge tile %dx%d narrow %d" % (block, items, narrow), flush=True)
     ctx.set_param("merge_block", block)
     ctx.set_param("merge_items", items)
+    ctx.set_param("narrow_keys", narrow)
     try:
+        for top in ((1 << 32) - 1, 1 << 32):
+            rng = np.random.default_rng(top % 97)
+            where = rng.integers(0, 3, 3 * block * items)
+            ak = top - np.flatnonzero(where != 1)[::-1].astype(np.int64) * 5
+            bk = top - np.flatnonzero(where != 0)[::-1].astype(np.int64) * 5
+            _check(ctx, orc, ak, rng.uniform(-1, 1, len(ak)), bk, rng.uniform(-1, 1, len(bk)))
+            _check(ctx, orc, ak[:-1], rng.uniform(-1, 1, len(ak) - 1), bk, rng.uniform(-1, 1, len(bk)))
         rng = np.random.default_rng(block + items)
         where = rng.integers(0, 3, 50000)
         ak = np.flatnonzero(where != 1).astype(np.int64)
@@ -89,7 +101,8 @@ def test_merge_tile_shapes(ctx, orc, block, items):
         _check(ctx, orc, bk, bv, ak, av)
     finally:
         ctx.set_param("merge_block", 512)
-        ctx.set_param("merge_items", 12)
+        ctx.set_param("merge_items", 0)
+        ctx.set_param("narrow_keys", 1)
 
 
 def test_merge_signed_zeros(ctx, orc):
