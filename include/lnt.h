@@ -239,6 +239,31 @@ int lnt_lattice_mult(lnt_context ctx, const int64_t *a_keys,
                      const int64_t b_dims[3], int strategy, int64_t **c_keys,
                      double **c_vals, int64_t *nc, int mem);
 
+/* ------------------------------------- unary contraction / attraction -- */
+typedef enum lnt_unary_kind {
+  LNT_UNARY_CONTRACT = 0, /* sum over the diagonal of the group's modes */
+  LNT_UNARY_ATTRACT = 1   /* keep the diagonal as one mode */
+} lnt_unary_kind;
+
+/* B = A with repeated indices resolved: A(i,j,i,k) (contraction) or
+ * A(!i,j,!i) (attraction).  Replaces SparseMIABase::contract_attract
+ * (multi_index_array/SparseMIABase.h:1913-2053), reached from an index
+ * expression with a repeated index (MIA_Expr.h:695-731).  A is (keys, vals,
+ * n) in mode order `seq`, any order of entries.  group[m] = 0: mode m stays;
+ * group[m] = g in 1..ngroups: mode m belongs to group g, whose modes must have
+ * one size; kind[g-1] says what group g does.  B's modes: the staying modes in
+ * increasing mode order, then the attraction groups in increasing group
+ * number; B's keys are in B's default order (first mode fastest), sorted
+ * ascending, library-allocated (lnt_free).  Each B entry is the sum, from 0,
+ * of the A entries on its diagonals in ascending order of their contraction
+ * values, the highest-numbered contraction group most significant (the
+ * reference's sort order when groups are numbered by the last occurrence of
+ * their index, contractions before attractions); sums equal to zero are
+ * dropped (:2044).  All modes contracted gives one entry at key 0 (or none). */
+int lnt_contract_attract(lnt_context ctx, const int64_t *keys, const double *vals, int64_t n, int order,
+                         const int64_t *dims, const int32_t *seq, const int32_t *group, const int32_t *kind,
+                         int ngroups, int64_t **c_keys, double **c_vals, int64_t *nc, int mem);
+
 #ifdef __cplusplus
 }
 #endif

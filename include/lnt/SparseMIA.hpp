@@ -220,6 +220,59 @@ inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const s
   return {c, la};
 }
 
+// Unary contraction / attraction of an operand indexed with a repeated index,
+// A(i,j,i,k) or A(!i,j,!i) (perform_unary, MIA_Expr.h:695-731 ->
+// SparseMIABase::contract_attract, SparseMIABase.h:1913-2053).  A repeated
+// plain index is contracted, a repeated `!` index attracted.  Groups are
+// numbered as the reference's expression code pulls them: contractions, then
+// attractions, each in the order of the last occurrence of its index
+// (ExprUtil.h:120-240) -- which fixes the order of the sums.  The result's
+// labels: the other indices in mode order, then the attracted ones.
+inline Expr unary(const std::shared_ptr<Tensor> &ap, const std::vector<Label> &la) {
+  Tensor &a = *ap;
+  const size_t n = la.size();
+  std::vector<size_t> con, att;  // positions of the last occurrences
+  for (size_t m = 0; m < n; ++m) {
+    size_t count = 0, last = m;
+    for (size_t q = 0; q < n; ++q)
+      if (la[q].id == la[m].id) {
+        ++count;
+        last = q;
+        if (la[q].bangs != la[m].bangs)
+          throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "a repeated index has different element-wise counts");
+      }
+    if (count > 1 && last == m) (la[m].inter() ? att : con).push_back(m);
+  }
+  if (con.empty() && att.empty()) return Expr{ap, la};
+  std::vector<int32_t> group(n, 0), kind;
+  std::vector<Label> lc;
+  auto c = std::make_shared<Tensor>();
+  int g = 0;
+  for (auto *list : {&con, &att})
+    for (size_t m : *list) {
+      ++g;
+      kind.push_back(list == &con ? LNT_UNARY_CONTRACT : LNT_UNARY_ATTRACT);
+      for (size_t q = 0; q < n; ++q)
+        if (la[q].id == la[m].id) group[q] = g;
+    }
+  for (size_t m = 0; m < n; ++m)
+    if (!group[m]) c->dims.push_back(a.dims[m]), lc.push_back(la[m]);
+  for (size_t m : att) c->dims.push_back(a.dims[m]), lc.push_back(la[m]);
+  c->seq = identity(c->dims.size());
+  c->sorted = true;
+  int64_t *ck = nullptr;
+  double *cv = nullptr;
+  int64_t nc = 0;
+  check(lnt_contract_attract(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), (int)n,
+                             a.dims.data(), a.seq.data(), group.data(), kind.data(), (int)kind.size(), &ck, &cv, &nc,
+                             LNT_MEM_HOST));
+  c->idx.assign(ck, ck + nc);
+  c->data.assign(cv, cv + nc);
+  lnt_free(default_context().get(), ck, LNT_MEM_HOST);
+  lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+  return {c, lc};
+}
+
 // Lattice product / Hadamard (MIA_Expr.h:449-561, 864-925, 1012-1059).
 inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const std::vector<Label> &lb) {
   Tensor b_copy;
@@ -332,7 +385,8 @@ template <class MIA>
 class AssignTarget {
  public:
   AssignTarget(MIA &m, std::vector<Label> labels) : m_(m), labels_(std::move(labels)) {}
-  operator Atom() const { return Atom(detail::Expr{m_.tensor_ptr(), labels_}); }
+  // A repeated index contracts or attracts at once (perform_unary).
+  operator Atom() const { return Atom(detail::unary(m_.tensor_ptr(), labels_)); }
   Atom operator+(const Atom &o) const { return Atom(*this) + o; }
   Atom operator-(const Atom &o) const { return Atom(*this) - o; }
   Atom operator*(const Atom &o) const { return Atom(*this) * o; }
@@ -342,6 +396,9 @@ class AssignTarget {
   // Assignment matches indices by id, whatever their element-wise counts
   // (MIA_Expr.h:112-135).
   AssignTarget &operator=(const Atom &rhs) {
+    for (size_t m = 0; m < labels_.size(); ++m)
+      if (detail::find_label(labels_, labels_[m].id) != (int)m)
+        throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "an assignment target repeats an index");
     const detail::Expr &e = rhs.expr();
     if (e.labels.size() != labels_.size())
       throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "assignment order differs");

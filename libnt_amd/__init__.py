@@ -30,6 +30,7 @@ MEM_HOST, MEM_DEVICE = 0, 1
 OP_ADD, OP_SUB, OP_HADAMARD = 0, 1, 2
 CMP_EXACT, CMP_FUZZY = 0, 1
 SPGEMM_AUTO, SPGEMM_SORT, SPGEMM_HASH = 0, 1, 2
+UNARY_CONTRACT, UNARY_ATTRACT = 0, 1
 
 _lib = None
 _vp = ctypes.c_void_p
@@ -98,6 +99,8 @@ SIGNATURES = {
     "lnt_model_gradient": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _i64,
                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "lnt_context_set_model": [_vp, _vp],
+    "lnt_contract_attract": [_vp, _vp, _vp, _i64, _int, _i64p, _i32p, _i32p, _i32p, _int, ctypes.POINTER(_vp),
+                             ctypes.POINTER(_vp), _i64p, _int],
     # include/lnt_mex.h (the alloc argument is a C function pointer; None = calloc)
     "lnt_mex_sparse_sort": [_vp, _vp, _vp, _i64],
     "lnt_mex_sparse_change_linidx": [_vp, _vp, _i64, _u8p, _u8p, _i64p, _int],
@@ -365,6 +368,39 @@ class Context:
             cv = torch.empty(cap, dtype=torch.float64, device=ak.device)
         n = self.lattice_mult_into(ak, av, a_dims, bk, bv, b_dims, ck, cv, strategy)
         return ck[:n], cv[:n]
+
+    def contract_attract(self, keys, vals, dims, seq, group, kind):
+        """Unary contraction/attraction (lnt_contract_attract): group[m] 0 keeps
+        mode m, g >= 1 puts it in group g; kind[g-1] UNARY_CONTRACT or
+        UNARY_ATTRACT.  Returns (keys, values) of the result, of the operand's
+        kind, sorted, in the result's default order (staying modes, then the
+        attraction groups)."""
+        kp, mem, n = _ptr_mem(keys)
+        vp, _, _ = _ptr_mem(vals)
+        d, dp = _arr_i64(dims)
+        sq, sp = _arr_i32(seq)
+        gr, gp = _arr_i32(group)
+        kd, kdp = _arr_i32(kind if len(kind) else [0])
+        ck, cv, nc = _vp(), _vp(), ctypes.c_int64()
+        check(lib().lnt_contract_attract(self._h, kp, vp, n, len(d), dp, sp, gp, kdp, len(kind), ctypes.byref(ck),
+                                         ctypes.byref(cv), ctypes.byref(nc), mem))
+        m = nc.value
+        try:
+            if mem == MEM_HOST:
+                ok = np.ctypeslib.as_array(ctypes.cast(ck, _i64p), (max(m, 1),))[:m].copy()
+                ov = np.ctypeslib.as_array(ctypes.cast(cv, ctypes.POINTER(ctypes.c_double)), (max(m, 1),))[:m].copy()
+                return ok, ov
+            import torch
+
+            ok = torch.empty(m, dtype=torch.int64, device=keys.device)
+            ov = torch.empty(m, dtype=torch.float64, device=keys.device)
+            if m:
+                check(lib().lnt_copy(self._h, ok.data_ptr(), ck, m * 8))
+                check(lib().lnt_copy(self._h, ov.data_ptr(), cv, m * 8))
+            return ok, ov
+        finally:
+            lib().lnt_free(self._h, ck, mem)
+            lib().lnt_free(self._h, cv, mem)
 
     # ---------------------------------------------------- synthetic inputs
     def testgen_distinct_keys(self, keys, domain, seed, offset=0):

@@ -34,10 +34,11 @@ def _prod(xs):
     return p
 
 
-def parse_labels(labels):
+def parse_labels(labels, allow_repeats: bool = False):
     """``"i!jk!!l"`` -> ``[("i", 0), ("j", 1), ("k", 0), ("l", 2)]``: one
     (letter, element-wise count) per mode; an upper-case letter counts one
-    mark.  A list of such pairs passes through."""
+    mark.  A list of such pairs passes through.  A letter may repeat only
+    where `allow_repeats` (indexing a tensor: unary contraction/attraction)."""
     if not isinstance(labels, str):
         return [(str(c), int(b)) for c, b in labels]
     out, bangs = [], 0
@@ -52,7 +53,7 @@ def parse_labels(labels):
     if bangs:
         raise ValueError("a trailing '!' in %r" % labels)
     letters = [c for c, _ in out]
-    if len(set(letters)) != len(letters):
+    if len(set(letters)) != len(letters) and not allow_repeats:
         raise ValueError("an index label repeats within one operand: %r" % labels)
     return out
 
@@ -149,11 +150,45 @@ class SparseMIA:
         return self._compare(other, precision)
 
     # -------------------------------------------------------------- helpers
-    def __call__(self, labels) -> "Atom":
-        tokens = parse_labels(labels)
+    def __call__(self, labels):
+        """A("ijk") -> Atom.  A repeated letter contracts (plain) or attracts
+        (``!``) at once (perform_unary, MIA_Expr.h:695-731); contracting every
+        mode gives a float."""
+        tokens = parse_labels(labels, allow_repeats=True)
         if len(tokens) != self.order:
             raise ValueError("expected %d index labels, got %r" % (self.order, labels))
-        return Atom(self, tokens)
+        if len({c for c, _ in tokens}) == len(tokens):
+            return Atom(self, tokens)
+        return self._unary(tokens)
+
+    def _unary(self, tokens):
+        """SparseMIABase::contract_attract (SparseMIABase.h:1913-2053).  Groups
+        are numbered as the reference's expression code pulls them:
+        contractions, then attractions, each in the order of the last
+        occurrence of its letter (ExprUtil.h:120-240), which fixes the order of
+        the sums.  The result's labels: the other letters in mode order, then
+        the attracted ones."""
+        from . import UNARY_ATTRACT, UNARY_CONTRACT
+
+        last = {c: p for p, (c, _) in enumerate(tokens)}
+        marks = {}
+        for c, b in tokens:
+            if marks.setdefault(c, b) != b:
+                raise ValueError("repeated index %r has different element-wise counts" % c)
+        letters = [c for c, _ in tokens]
+        rep = [c for c in sorted(last, key=last.get) if letters.count(c) > 1]
+        con = [c for c in rep if marks[c] == 0]
+        att = [c for c in rep if marks[c] > 0]
+        number = {c: g + 1 for g, c in enumerate(con + att)}
+        group = [number.get(c, 0) for c in letters]
+        kind = [UNARY_CONTRACT] * len(con) + [UNARY_ATTRACT] * len(att)
+        ck, cv = self.ctx.contract_attract(self.keys, self.vals, self.dims, self.seq, group, kind)
+        stays = [m for m in range(self.order) if group[m] == 0]
+        dims = [self.dims[m] for m in stays] + [self.dims[letters.index(c)] for c in att]
+        out = [tokens[m] for m in stays] + [(c, marks[c]) for c in att]
+        if not dims:
+            return float(cv[0].item()) if cv.numel() else 0.0
+        return Atom(SparseMIA(dims, ck, cv, None, True, self.ctx), out)
 
     def to_dense(self):
         """Dense numpy copy in the default (first mode fastest) layout."""

@@ -82,6 +82,7 @@ def lib():
             "lnto_contract_plan": (i32, [i32, _i64p, ctypes.c_char_p, i32, _i64p, ctypes.c_char_p, _i32p, _i32p, _i64p, _i64p, _i64p, ctypes.POINTER(ctypes.c_int)]),
             "lnto_is_equal_fuzzy": (i32, [f64, f64, f64]),
             "lnto_compare_sparse": (i32, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, f64]),
+            "lnto_contract_attract": (i64, [_i64p, _f64p, i64, i32, _i64p, _i32p, _i32p, _i32p, i32, _i64p, _f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -288,6 +289,19 @@ def contract(ak, av, dims_a, seq_a_cur, a_sorted, labels_a, bk, bv, dims_b, seq_
     bk2, bv2 = sort_seq(bk, bv, dims_b, seq_b_cur, seq_b, b_sorted)
     ck, cv = lattice_mult(ak2, av2, tuple(int(x) for x in lat_a), bk2, bv2, tuple(int(x) for x in lat_b))
     return ck, cv, dims_c
+
+
+def contract_attract(keys, vals, dims, seq, group, kind):
+    """SparseMIABase::contract_attract (SparseMIABase.h:1913-2053): group[m] 0
+    keeps mode m, g >= 1 puts it in group g; kind[g-1] 0 contracts, 1
+    attracts.  Returns (keys, values) of the result, sorted by key."""
+    k, v = _i64(keys).copy(), _f64(vals).copy()
+    dims, seq, group, kind = _i64(dims), _i32(seq), _i32(group), _i32(kind)
+    ck = np.empty(max(len(k), 1), dtype=np.int64)
+    cv = np.empty(max(len(k), 1), dtype=np.float64)
+    n = lib().lnto_contract_attract(_p(k, _i64p), _p(v, _f64p), len(k), len(dims), _p(dims, _i64p), _p(seq, _i32p),
+                                    _p(group, _i32p), _p(kind, _i32p), len(kind), _p(ck, _i64p), _p(cv, _f64p))
+    return ck[:n].copy(), cv[:n].copy()
 
 
 def is_equal_fuzzy(a: float, b: float, prec: float) -> bool:

@@ -753,6 +753,99 @@ int lnto_is_equal_fuzzy(double a, double b, double prec) {
  * (LibMIAUtil.h:137-150). */
 static int below_tolerance(double v) { return fabs(v) <= 1.11e-16; }
 
+/* ==================== unary contraction / attraction ===================== */
+/* SparseMIABase.h:1913-2053 contract_attract.  group[m] = 0: mode m stays;
+ * g in 1..ngroups: mode m is in group g; kind[g-1] 0 = contraction, 1 =
+ * attraction.  The reference's index arrays are built here as its expression
+ * code builds them: contraction positions grouped by group (increasing g),
+ * then attraction positions (increasing g), then the staying modes in mode
+ * order (:1920-1925), each group's positions ascending.  The tensor is sorted
+ * in that order (:1928), then walked group by group exactly as :1966-2047:
+ * the attraction check on the group's first entry, the running sum from 0 of
+ * the entries whose contraction indices agree, pushed when non-zero, at key
+ * other + attract_idx * other_dimensionality.  The pushed entries are
+ * returned sorted by key (the reference leaves them unsorted).  keys/vals are
+ * sorted in place as a side effect, as the reference sorts `this`. */
+static int cmp_pair_key(const void *x, const void *y) {
+  const int64_t a = ((const int64_t *)x)[0], b = ((const int64_t *)y)[0];
+  return a < b ? -1 : a > b;
+}
+
+int64_t lnto_contract_attract(int64_t *keys, double *vals, int64_t n, int order, const int64_t *dims,
+                              const int32_t *seq, const int32_t *group, const int32_t *kind, int ngroups,
+                              int64_t *ck, double *cv) {
+  int32_t sort_order[LNTO_MAX_ORDER], part_len[LNTO_MAX_ORDER], part_kind[LNTO_MAX_ORDER];
+  int np = 0, t = 0, ncon = 0, natt = 0, nparts_con = 0;
+  for (int pass = 0; pass < 2; ++pass)  /* contractions first, then attractions */
+    for (int g = 1; g <= ngroups; ++g) {
+      if (kind[g - 1] != pass) continue;
+      int len = 0;
+      for (int m = 0; m < order; ++m)
+        if (group[m] == g) { sort_order[t++] = m; ++len; }
+      part_len[np] = len;
+      part_kind[np++] = pass;
+      if (pass == 0) { ncon += len; ++nparts_con; } else natt += len;
+    }
+  const int nother = order - ncon - natt;
+  for (int m = 0; m < order; ++m)
+    if (group[m] == 0) sort_order[t++] = m;
+  /* dimensionalities (:1929-1944) and the other modes' (:1922-1924) */
+  uint64_t C = 1, A = 1, O = 1, att_range[LNTO_MAX_ORDER];
+  int idx = 0;
+  for (int p = 0; p < np; ++p) {
+    uint64_t d = (uint64_t)dims[sort_order[idx]], pw = 1;
+    for (int q = 0; q < part_len[p]; ++q) pw *= d;
+    if (part_kind[p] == 0) C *= pw;
+    else { att_range[p - nparts_con] = d; A *= pw; }
+    idx += part_len[p];
+  }
+  for (int q = ncon + natt; q < order; ++q) O *= (uint64_t)dims[sort_order[q]];
+  (void)nother;
+  lnto_sort_seq(keys, vals, n, order, dims, seq, sort_order, 0);
+  int64_t out = 0, cur = 0;
+  int64_t sub[LNTO_MAX_ORDER], ind[LNTO_MAX_ORDER];
+  while (cur < n) {
+    lnto_ind2sub(keys[cur], sort_order, dims, order, sub);
+    for (int q = 0; q < order; ++q) ind[q] = sub[sort_order[q]];
+    /* attraction check of the current entry (:1955-1973) */
+    int ok = 1, start = ncon;
+    uint64_t attract_idx = 0, mult = 1;
+    for (int p = nparts_con; p < np && ok; ++p) {
+      for (int q = start + 1; q < start + part_len[p]; ++q)
+        if (ind[q] != ind[start]) { ok = 0; break; }
+      attract_idx += (uint64_t)ind[start] * mult;
+      mult *= att_range[p - nparts_con];
+      start += part_len[p];
+    }
+    if (!ok) { ++cur; continue; }
+    const uint64_t head = (uint64_t)keys[cur];
+    double sum = 0;
+    while (cur < n && head / C == (uint64_t)keys[cur] / C) {
+      lnto_ind2sub(keys[cur], sort_order, dims, order, sub);
+      for (int q = 0; q < order; ++q) ind[q] = sub[sort_order[q]];
+      int add = 1, cs = 0;
+      for (int p = 0; p < nparts_con && add; ++p) {
+        for (int q = cs + 1; q < cs + part_len[p]; ++q)
+          if (ind[q] != ind[cs]) { add = 0; break; }
+        cs += part_len[p];
+      }
+      if (add) sum += vals[cur];
+      ++cur;
+    }
+    if (sum) {
+      ck[out] = (int64_t)(head / (C * A) + attract_idx * O);
+      cv[out++] = sum;
+    }
+  }
+  /* sort the (key, value) pairs by key */
+  int64_t *tmp = (int64_t *)malloc((size_t)(out ? out : 1) * 2 * sizeof(int64_t));
+  for (int64_t q = 0; q < out; ++q) { tmp[2 * q] = ck[q]; memcpy(&tmp[2 * q + 1], &cv[q], sizeof(double)); }
+  qsort(tmp, (size_t)out, 2 * sizeof(int64_t), cmp_pair_key);
+  for (int64_t q = 0; q < out; ++q) { ck[q] = tmp[2 * q]; memcpy(&cv[q], &tmp[2 * q + 1], sizeof(double)); }
+  free(tmp);
+  return out;
+}
+
 /* SparseMIABase.h:1245-1296: the two-pointer walk, restated step for step. */
 int lnto_compare_sparse(const int64_t *ak, const double *av, int64_t na, const int64_t *bk, const double *bv,
                         int64_t nb, int mode, double prec) {

@@ -20,6 +20,8 @@
 #ifndef LNT_ORACLE_H
 #define LNT_ORACLE_H
 
+#define LNTO_MAX_ORDER 16
+
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -134,6 +136,14 @@ void lnto_mlp_forward(const double *params, const double *x, double *y);
 /* Returns the loss; grad receives d loss / d (trainable params). */
 double lnto_mlp_loss_grad(const double *params, const double *X, const double *T, int64_t n,
                           double *grad);
+
+/* ------------------------ unary contraction ---------------------------- */
+/* SparseMIABase.h:1913-2053 contract_attract (see lnt_oracle.c).  ck/cv need
+ * room for n entries; returns the number written, sorted by key.  Sorts
+ * keys/vals in place. */
+int64_t lnto_contract_attract(int64_t *keys, double *vals, int64_t n, int order, const int64_t *dims,
+                              const int32_t *seq, const int32_t *group, const int32_t *kind, int ngroups,
+                              int64_t *ck, double *cv);
 
 /* ------------------------------- comparison ----------------------------- */
 int lnto_is_equal_fuzzy(double a, double b, double prec);

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <map>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "lnt/SparseMIA.hpp"
@@ -62,6 +63,30 @@ static double at(const std::vector<double> &d, std::initializer_list<int64_t> di
 }
 
 static bool near(double got, double want) { return std::fabs(got - want) <= 1e-12 * std::max(1.0, std::fabs(want)); }
+
+// Dense einsum-like reference: `in` labels one letter per mode of a dense
+// array with dims `dims` (first mode fastest); `out` lists the result's
+// letters.  Letters repeated in `in` must agree; the rest are summed.
+static std::vector<double> dense_unary(const std::vector<double> &d, const std::vector<int64_t> &dims,
+                                       const std::string &in, const std::string &out) {
+  std::vector<int64_t> odims;
+  for (char c : out) odims.push_back(dims[in.find(c)]);
+  int64_t osize = 1;
+  for (int64_t x : odims) osize *= x;
+  std::vector<double> r(osize, 0.0);
+  for (size_t q = 0; q < d.size(); ++q) {
+    std::vector<int64_t> sub(dims.size());
+    int64_t k = (int64_t)q;
+    for (size_t t = 0; t < dims.size(); ++t) sub[t] = k % dims[t], k /= dims[t];
+    bool ok = true;
+    for (size_t t = 0; t < in.size(); ++t) ok = ok && sub[t] == sub[in.find(in[t])];
+    if (!ok) continue;
+    int64_t o = 0, mult = 1;
+    for (size_t u = 0; u < out.size(); ++u) o += sub[in.find(out[u])] * mult, mult *= odims[u];
+    r[o] += d[q];
+  }
+  return r;
+}
 
 int main() {
   NTINDEX i; NTINDEX j; NTINDEX k; NTINDEX l; NTINDEX m; NTINDEX n;
@@ -280,6 +305,72 @@ int main() {
       thrown = true;
     }
     CHECK(thrown, "mismatched element-wise counts not reported");
+  }
+  // ---- unary contraction and attraction (tests/MIA/mia_unary_test.cpp:52-171)
+  {
+    lnt::SparseMIA<double, 6> a(5, 6, 5, 6, 5, 3), a2(5, 6, 5, 5, 6, 3);
+    lnt::SparseMIA<double, 3> c(5, 6, 5);
+    lnt::SparseMIA<double, 2> sq(5, 5);
+    fill_random(a, 1500, 41);
+    fill_random(a2, 1400, 42);
+    fill_random(c, 50, 43);
+    fill_random(sq, 10, 44);
+    a.sort(std::array<size_t, 6>{{3, 1, 5, 0, 2, 4}});  // any mode order
+    auto da = dense(a), da2 = dense(a2), dc = dense(c), dsq = dense(sq);
+    const std::vector<int64_t> A{5, 6, 5, 6, 5, 3}, A2{5, 6, 5, 5, 6, 3}, Cd{5, 6, 5}, S{5, 5};
+    auto same = [](const std::vector<double> &got, const std::vector<double> &want) {
+      if (got.size() != want.size()) return false;
+      for (size_t q = 0; q < got.size(); ++q)
+        if (!near(got[q], want[q])) return false;
+      return true;
+    };
+    lnt::SparseMIA<double, 4> b(6, 6, 5, 3);
+    b(j, k, l, m) = a(i, j, i, k, l, m);  // Simple Contraction 1
+    CHECK(same(dense(b), dense_unary(da, A, "ijiklm", "jklm")), "simple contraction");
+    lnt::SparseMIA<double, 1> e(6);
+    e(i) = c(j, i, j);  // Simple Contraction 2
+    CHECK(same(dense(e), dense_unary(dc, Cd, "jij", "i")), "simple contraction 2");
+    lnt::SparseMIA<double, 2> d(5, 3);
+    d(l, m) = a(i, j, i, j, l, m);  // Complex Contraction 1
+    CHECK(same(dense(d), dense_unary(da, A, "ijijlm", "lm")), "complex contraction");
+    lnt::SparseMIA<double, 1> e2(3);
+    e2(m) = a(i, j, i, j, i, m);  // Complex Contraction 2
+    CHECK(same(dense(e2), dense_unary(da, A, "ijijim", "m")), "complex contraction 2");
+    lnt::SparseMIA<double, 1> e3(5);
+    e3(i) = sq(!i, !i);  // Very Simple Attraction
+    CHECK(same(dense(e3), dense_unary(dsq, S, "ii", "i")), "simple attraction");
+    lnt::SparseMIA<double, 2> d2(5, 6);
+    d2(i, j) = c(!i, j, !i);  // Simple Attraction 1
+    CHECK(same(dense(d2), dense_unary(dc, Cd, "iji", "ij")), "simple attraction 1");
+    lnt::SparseMIA<double, 3> c3(5, 5, 3);
+    c3(i, k, m) = a(!i, j, !i, j, k, m);  // Combined Contraction/Attraction 1
+    CHECK(same(dense(c3), dense_unary(da, A, "ijijkm", "ikm")), "combined contraction/attraction");
+    lnt::SparseMIA<double, 2> d3(6, 3);
+    d3(j, m) = a2(i, !j, i, i, !j, m);  // Combined Contraction/Attraction 4
+    CHECK(same(dense(d3), dense_unary(da2, A2, "ijiijm", "jm")), "combined contraction/attraction 4");
+    // a contraction feeding a product: C(j,n) = A(i,j,i,k,l,m) * B(k,l,m,n)
+    lnt::SparseMIA<double, 4> bb(6, 5, 3, 4);
+    lnt::SparseMIA<double, 2> cj(6, 4);
+    fill_random(bb, 200, 45);
+    cj(j, n) = a(i, j, i, k, l, m) * bb(k, l, m, n);
+    auto tr = dense_unary(da, A, "ijiklm", "jklm");
+    auto dbb = dense(bb), dcj = dense(cj);
+    bool ok = true;
+    for (int64_t jj = 0; jj < 6; ++jj)
+      for (int64_t nn = 0; nn < 4; ++nn) {
+        double w = 0;
+        for (int64_t q = 0; q < 6 * 5 * 3; ++q) w += tr[jj + 6 * q] * dbb[q + 90 * nn];
+        ok = ok && near(dcj[jj + 6 * nn], w);
+      }
+    CHECK(ok, "contraction feeding a product");
+    bool thrown = false;
+    try {
+      lnt::SparseMIA<double, 2> z(5, 5);
+      z(i, i) = sq(i, j);
+    } catch (const std::exception &) {
+      thrown = true;
+    }
+    CHECK(thrown, "a repeated index in an assignment target not reported");
   }
   if (failures) return 1;
   std::printf("C++ front end: all checks passed\n");

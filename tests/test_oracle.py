@@ -415,3 +415,61 @@ def test_is_equal_fuzzy(orc):
     assert not orc.is_equal_fuzzy(0.5, 0.5 + 1e-11, 1e-12)
     assert orc.is_equal_fuzzy(1e6, 1e6 * (1 + 5e-13), 1e-12)  # relative above 1
     assert not orc.is_equal_fuzzy(1e6, 1e6 * (1 + 5e-12), 1e-12)
+
+
+# ------------------------------------------------- unary contraction/attraction
+from unary_cases import UNARY_CASES, unary_groups as _unary_groups  # noqa: E402
+
+
+@pytest.mark.parametrize("labels,dims,spec", UNARY_CASES)
+def test_contract_attract_against_dense(orc, labels, dims, spec):
+    """The reference's unary patterns (tests/MIA/mia_unary_test.cpp:52-171)
+    against dense numpy einsum, from an input in a shuffled mode order."""
+    group, kind = _unary_groups(labels)
+    dom = int(np.prod(dims))
+    n = dom // 3
+    k = orc.gen_distinct_keys(n, dom, 17)
+    v = orc.gen_uniform_vals(n, 18) * 5
+    seq = list(np.random.default_rng(len(labels)).permutation(len(dims)))
+    dense = np.zeros(dom)
+    dense[k] = v
+    dense = dense.reshape(dims, order="F")
+    # the same tensor, keyed in `seq`
+    subs = np.unravel_index(k, dims, order="F")
+    ks = np.zeros(n, dtype=np.int64)
+    place = 1
+    for m in seq:
+        ks += subs[m] * place
+        place *= dims[m]
+    ck, cv = orc.contract_attract(ks, v, dims, seq, group, kind)
+    want = np.einsum(spec, dense)
+    out_dims = want.shape
+    got = np.zeros(int(np.prod(out_dims)) if out_dims else 1)
+    got[ck] = cv
+    assert np.all(np.diff(ck) > 0)
+    assert np.all(cv != 0)  # zero sums are dropped
+    assert orc.allclose_fuzzy(got, np.ravel(want, order="F"), 1e-12)
+
+
+def test_contract_attract_summation_order(orc):
+    """The sums run in the reference's sort order: ascending diagonal value,
+    and with two contracted indices the one whose last occurrence is later is
+    the more significant.  The values make any other order give another sum
+    (1e16 + 1 rounds to 1e16), and a sum of exactly zero is dropped."""
+    # trace of a 3x3: diagonal 1e16, -1e16, 1 in that order -> 1; reversed -> 0
+    k = np.array([0, 4, 8], dtype=np.int64)
+    v = np.array([1e16, -1e16, 1.0])
+    ck, cv = orc.contract_attract(k, v, (3, 3), (0, 1), (1, 1), (0,))
+    assert ck.tolist() == [0] and cv.tolist() == [1.0]
+    ck, cv = orc.contract_attract(k, v[::-1].copy(), (3, 3), (0, 1), (1, 1), (0,))
+    assert len(ck) == 0  # 1 - 1e16 + 1e16 == 0: dropped
+    # a(i,j,i,j), dims 2x2x2x2: j (last at 3) is more significant than i (last at 2)
+    dims = (2, 2, 2, 2)
+    key = lambda i, j: i + 2 * j + 4 * i + 8 * j
+    k = np.array([key(0, 0), key(1, 0), key(0, 1)], dtype=np.int64)
+    v = np.array([1e16, -1e16, 1.0])
+    o = np.argsort(k)
+    group, kind = _unary_groups("ijij")
+    assert group == [1, 2, 1, 2]
+    ck, cv = orc.contract_attract(k[o], v[o], dims, (0, 1, 2, 3), group, kind)
+    assert cv.tolist() == [1.0]  # (i0,j0) + (i1,j0) + (i0,j1); i-major order would give 0
