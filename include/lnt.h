@@ -239,6 +239,24 @@ int lnt_lattice_mult(lnt_context ctx, const int64_t *a_keys,
                      const int64_t b_dims[3], int strategy, int64_t **c_keys,
                      double **c_vals, int64_t *nc, int mem);
 
+/* ------------------------------------------ sparse x dense, no contraction -- */
+/* C = A * B with B dense and no contracted index: C(l, r, t) = A(l, t) *
+ * B(r, t) over A's outer indices l, B's outer indices r and the shared
+ * element-wise indices t.  Replaces SparseMIABase::noLatticeMult with a
+ * DenseMIA operand (multi_index_array/SparseMIABase.h:1829-1910).  A: keys
+ * sorted ascending with key = l + l_size*t (its outer modes, then its inter
+ * modes).  B: order_b modes of sizes b_dims, first mode fastest, host or device
+ * memory like A; r_modes[nr] lists B's outer modes (r = their linear index,
+ * first listed fastest), i_modes[ni] its inter modes in the order of A's
+ * (t likewise); together they name each mode of B once.  For every A entry
+ * with a non-zero value and every r whose B value is non-zero, in order, C
+ * holds a*b at key r + l*R + t*l_size*R (R = prod of B's outer dims) -- sorted
+ * ascending when A is.  C is library-allocated (lnt_free).  B holds at most
+ * 2^40 elements. */
+int lnt_sparse_dense_mult(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na, int64_t l_size,
+                          int order_b, const int64_t *b_dims, const double *b, const int32_t *r_modes, int nr,
+                          const int32_t *i_modes, int ni, int64_t **c_keys, double **c_vals, int64_t *nc, int mem);
+
 /* ------------------------------------- unary contraction / attraction -- */
 typedef enum lnt_unary_kind {
   LNT_UNARY_CONTRACT = 0, /* sum over the diagonal of the group's modes */

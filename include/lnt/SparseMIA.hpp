@@ -348,7 +348,73 @@ inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, cons
   return {c, lc, a_int.size()};
 }
 
+// Dense operand of a product: dims and elements, first mode fastest.
+struct DenseView {
+  const std::vector<int64_t> *dims;
+  const std::vector<double> *data;
+};
+
+// Sparse x dense with no contracted index (SparseMIABase::noLatticeMult with
+// a DenseMIA, SparseMIABase.h:1829-1910): A sorted to [outer, inter] order,
+// then C(l, r, t) = A(l, t) * B(r, t) for non-zero factors.  C's modes are
+// [A outer, B outer, inter]; its keys run B outer fastest, then A outer, then
+// inter (:1899).
+inline Expr multiply_dense(Tensor &a, const std::vector<Label> &la, const DenseView &b, const std::vector<Label> &lb) {
+  std::vector<int32_t> a_out, a_int, b_out, b_int;
+  for (size_t m = 0; m < la.size(); ++m) {
+    const int r = find_label(lb, la[m].id);
+    if (r < 0) {
+      if (la[m].inter()) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "inter index in one operand only");
+      a_out.push_back((int32_t)m);
+      continue;
+    }
+    if (!la[m].inter() || la[m].bangs != lb[r].bangs)
+      throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT,
+                                  "a sparse x dense product takes only element-wise shared indices");
+    if (a.dims[m] != (*b.dims)[r]) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "matched dims differ");
+    a_int.push_back((int32_t)m);
+    b_int.push_back(r);
+  }
+  for (size_t r = 0; r < lb.size(); ++r)
+    if (find_label(la, lb[r].id) < 0) {
+      if (lb[r].inter()) throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "inter index in one operand only");
+      b_out.push_back((int32_t)r);
+    }
+  std::vector<int32_t> seq_a(a_out);
+  seq_a.insert(seq_a.end(), a_int.begin(), a_int.end());
+  a.sort_to(seq_a);
+  int64_t lsize = 1;
+  for (int32_t m : a_out) lsize *= a.dims[m];
+  auto c = std::make_shared<Tensor>();
+  std::vector<Label> lc;
+  for (int32_t m : a_out) c->dims.push_back(a.dims[m]), lc.push_back(la[m]);
+  for (int32_t r : b_out) c->dims.push_back((*b.dims)[r]), lc.push_back(lb[r]);
+  for (int32_t m : a_int) c->dims.push_back(a.dims[m]), lc.push_back(la[m]);
+  const int nl = (int)a_out.size(), nr = (int)b_out.size(), ni = (int)a_int.size();
+  for (int q = 0; q < nr; ++q) c->seq.push_back(nl + q);
+  for (int q = 0; q < nl; ++q) c->seq.push_back(q);
+  for (int q = 0; q < ni; ++q) c->seq.push_back(nl + nr + q);
+  c->sorted = true;
+  int64_t *ck = nullptr;
+  double *cv = nullptr;
+  int64_t nc = 0;
+  check(lnt_sparse_dense_mult(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), lsize,
+                              (int)b.dims->size(), b.dims->data(), b.data->data(), b_out.data(), nr, b_int.data(), ni,
+                              &ck, &cv, &nc, LNT_MEM_HOST));
+  c->idx.assign(ck, ck + nc);
+  c->data.assign(cv, cv + nc);
+  lnt_free(default_context().get(), ck, LNT_MEM_HOST);
+  lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+  return {c, lc, (size_t)ni};
+}
+
 }  // namespace detail
+
+// B(i,j,...) for a DenseMIA B: the right operand of a sparse x dense product.
+struct DenseAtom {
+  detail::DenseView view;
+  std::vector<Label> labels;
+};
 
 // Expression node: A(i,j,...) or the result of an operation on atoms.
 class Atom {
@@ -357,6 +423,7 @@ class Atom {
   Atom operator+(const Atom &o) const { return detail::merge(*e_.t, e_.labels, *o.e_.t, o.e_.labels, LNT_OP_ADD); }
   Atom operator-(const Atom &o) const { return detail::merge(*e_.t, e_.labels, *o.e_.t, o.e_.labels, LNT_OP_SUB); }
   Atom operator*(const Atom &o) const { return detail::multiply(*e_.t, e_.labels, *o.e_.t, o.e_.labels); }
+  Atom operator*(const DenseAtom &o) const { return detail::multiply_dense(*e_.t, e_.labels, o.view, o.labels); }
   // ~(A*B): the product's inter indices lose one `!` (MIA_Expr.h:257-301,
   // 1376), so `!j` becomes a plain index of the next product and `!!l` an
   // inter index again.  No data moves.
@@ -393,6 +460,7 @@ class AssignTarget {
   Atom operator+(const AssignTarget &o) const { return Atom(*this) + Atom(o); }
   Atom operator-(const AssignTarget &o) const { return Atom(*this) - Atom(o); }
   Atom operator*(const AssignTarget &o) const { return Atom(*this) * Atom(o); }
+  Atom operator*(const DenseAtom &o) const { return Atom(*this) * o; }
   // Assignment matches indices by id, whatever their element-wise counts
   // (MIA_Expr.h:112-135).
   AssignTarget &operator=(const Atom &rhs) {
@@ -564,6 +632,41 @@ class SparseMIA {
   }
 
   std::shared_ptr<detail::Tensor> t_;
+};
+
+// ------------------------------------------------------------- DenseMIA --
+// Dense order-N tensor, elements first mode fastest (the reference's
+// DenseMIA layout, multi_index_array/DenseMIA.h): the dense operand of
+// sparse x dense products A(l,!t) * B(r,!t).
+template <class T, size_t N>
+class DenseMIA {
+  static_assert(std::is_same<T, double>::value, "the GPU path computes in double");
+
+ public:
+  template <class... D, class = typename std::enable_if<sizeof...(D) == N>::type>
+  explicit DenseMIA(D... dims) : dims_{static_cast<int64_t>(dims)...} {
+    data_.assign((size_t)detail::product(dims_), 0.0);
+  }
+  int64_t dim(size_t i) const { return dims_[i]; }
+  int64_t dimensionality() const { return detail::product(dims_); }
+  T *data_begin() { return data_.data(); }
+  T *data_end() { return data_.data() + data_.size(); }
+  const std::vector<T> &data() const { return data_; }
+  // Element at full subscript, first mode fastest.
+  T &at(const std::array<int64_t, N> &sub) {
+    int64_t k = 0, mult = 1;
+    for (size_t t = 0; t < N; ++t) k += sub[t] * mult, mult *= dims_[t];
+    return data_[(size_t)k];
+  }
+  template <class... Ts>
+  DenseAtom operator()(Ts...) const {
+    static_assert(sizeof...(Ts) == N, "one index per mode");
+    return DenseAtom{detail::DenseView{&dims_, &data_}, std::vector<Label>{Label{Ts::id, Ts::elemval}...}};
+  }
+
+ private:
+  std::vector<int64_t> dims_;
+  std::vector<T> data_;
 };
 
 }  // namespace lnt

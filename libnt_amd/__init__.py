@@ -99,6 +99,8 @@ SIGNATURES = {
     "lnt_model_gradient": [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _i64,
                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
     "lnt_context_set_model": [_vp, _vp],
+    "lnt_sparse_dense_mult": [_vp, _vp, _vp, _i64, _i64, _int, _i64p, _vp, _i32p, _int, _i32p, _int,
+                              ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_contract_attract": [_vp, _vp, _vp, _i64, _int, _i64p, _i32p, _i32p, _i32p, _int, ctypes.POINTER(_vp),
                              ctypes.POINTER(_vp), _i64p, _int],
     # include/lnt_mex.h (the alloc argument is a C function pointer; None = calloc)
@@ -369,6 +371,46 @@ class Context:
         n = self.lattice_mult_into(ak, av, a_dims, bk, bv, b_dims, ck, cv, strategy)
         return ck[:n], cv[:n]
 
+    def _take_result(self, ck, cv, m, mem, like):
+        """Copies a library-allocated result of m pairs into arrays of the
+        operand's kind and frees it."""
+        try:
+            if mem == MEM_HOST:
+                ok = np.ctypeslib.as_array(ctypes.cast(ck, _i64p), (max(m, 1),))[:m].copy()
+                ov = np.ctypeslib.as_array(ctypes.cast(cv, ctypes.POINTER(ctypes.c_double)), (max(m, 1),))[:m].copy()
+                return ok, ov
+            import torch
+
+            ok = torch.empty(m, dtype=torch.int64, device=like.device)
+            ov = torch.empty(m, dtype=torch.float64, device=like.device)
+            if m:
+                check(lib().lnt_copy(self._h, ok.data_ptr(), ck, m * 8))
+                check(lib().lnt_copy(self._h, ov.data_ptr(), cv, m * 8))
+            return ok, ov
+        finally:
+            lib().lnt_free(self._h, ck, mem)
+            lib().lnt_free(self._h, cv, mem)
+
+    def sparse_dense_mult(self, ak, av, l_size, b_flat, b_dims, r_modes, i_modes):
+        """lnt_sparse_dense_mult: A sorted with key = l + l_size*t; b_flat the
+        dense B's elements, first mode fastest (a 1-D tensor or numpy array of
+        A's kind), with dims b_dims, modes r_modes outer and i_modes
+        element-wise.  Returns C's (keys, values), key r + l*R + t*l_size*R."""
+        akp, mem, n = _ptr_mem(ak)
+        avp, _, _ = _ptr_mem(av)
+        bp, bmem, nb = _ptr_mem(b_flat)
+        if bmem != mem:
+            raise ValueError("A and B must live in the same memory kind")
+        if nb != int(np.prod(b_dims)):
+            raise ValueError("b_flat must hold prod(b_dims) elements")
+        d, dp = _arr_i64(b_dims)
+        rm, rp = _arr_i32(list(r_modes) or [0])
+        im, ip = _arr_i32(list(i_modes) or [0])
+        ck, cv, nc = _vp(), _vp(), ctypes.c_int64()
+        check(lib().lnt_sparse_dense_mult(self._h, akp, avp, n, int(l_size), len(d), dp, bp, rp, len(r_modes), ip,
+                                          len(i_modes), ctypes.byref(ck), ctypes.byref(cv), ctypes.byref(nc), mem))
+        return self._take_result(ck, cv, nc.value, mem, ak)
+
     def contract_attract(self, keys, vals, dims, seq, group, kind):
         """Unary contraction/attraction (lnt_contract_attract): group[m] 0 keeps
         mode m, g >= 1 puts it in group g; kind[g-1] UNARY_CONTRACT or
@@ -384,23 +426,7 @@ class Context:
         ck, cv, nc = _vp(), _vp(), ctypes.c_int64()
         check(lib().lnt_contract_attract(self._h, kp, vp, n, len(d), dp, sp, gp, kdp, len(kind), ctypes.byref(ck),
                                          ctypes.byref(cv), ctypes.byref(nc), mem))
-        m = nc.value
-        try:
-            if mem == MEM_HOST:
-                ok = np.ctypeslib.as_array(ctypes.cast(ck, _i64p), (max(m, 1),))[:m].copy()
-                ov = np.ctypeslib.as_array(ctypes.cast(cv, ctypes.POINTER(ctypes.c_double)), (max(m, 1),))[:m].copy()
-                return ok, ov
-            import torch
-
-            ok = torch.empty(m, dtype=torch.int64, device=keys.device)
-            ov = torch.empty(m, dtype=torch.float64, device=keys.device)
-            if m:
-                check(lib().lnt_copy(self._h, ok.data_ptr(), ck, m * 8))
-                check(lib().lnt_copy(self._h, ov.data_ptr(), cv, m * 8))
-            return ok, ov
-        finally:
-            lib().lnt_free(self._h, ck, mem)
-            lib().lnt_free(self._h, cv, mem)
+        return self._take_result(ck, cv, nc.value, mem, keys)
 
     # ---------------------------------------------------- synthetic inputs
     def testgen_distinct_keys(self, keys, domain, seed, offset=0):

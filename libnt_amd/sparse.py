@@ -207,6 +207,44 @@ class SparseMIA:
         return c.keys.cpu().numpy(), c.vals.cpu().numpy()
 
 
+class DenseMIA:
+    """Dense order-N tensor as the dense operand of sparse x dense products:
+    dims and the elements, first mode fastest (the reference's DenseMIA
+    layout, multi_index_array/DenseMIA.h), as a 1-D GPU tensor."""
+
+    def __init__(self, dims, data, ctx=None):
+        import torch
+
+        self.dims = tuple(int(d) for d in dims)
+        self.ctx = ctx or default_context()
+        if isinstance(data, np.ndarray):
+            if data.ndim != 1:
+                raise ValueError("data is the flat element array, first mode fastest")
+            data = torch.from_numpy(np.ascontiguousarray(data, dtype=np.float64)).cuda()
+        if data.dim() != 1 or data.numel() != _prod(self.dims):
+            raise ValueError("data must hold prod(dims) elements in one dimension")
+        self.data = data.contiguous()
+
+    @classmethod
+    def from_array(cls, array: np.ndarray, ctx=None) -> "DenseMIA":
+        """From an N-d numpy array indexed [i0, i1, ...]."""
+        return cls(array.shape, np.ravel(array, order="F"), ctx)
+
+    def __call__(self, labels) -> "DenseAtom":
+        tokens = parse_labels(labels)
+        if len(tokens) != len(self.dims):
+            raise ValueError("expected %d index labels, got %r" % (len(self.dims), labels))
+        return DenseAtom(self, tokens)
+
+
+class DenseAtom:
+    """B(i,j,...) for a DenseMIA B: the right operand of Atom * DenseAtom."""
+
+    def __init__(self, dense: DenseMIA, tokens):
+        self.mia = dense
+        self.tokens = parse_labels(tokens)
+
+
 class Atom:
     """A(i,j,...) -- a tensor with index labels (MIA_Atom, MIA_Expr.h:663-689).
 
@@ -302,9 +340,44 @@ class Atom:
 
         return lattice_plan(self.mia.dims, self.labels, other.mia.dims, other.labels)
 
+    def _mul_dense(self, other: DenseAtom) -> "Atom":
+        """Sparse x dense with no contracted index (SparseMIABase::noLatticeMult
+        with a DenseMIA, SparseMIABase.h:1829-1910): A sorted to [outer, inter]
+        order, then C(l, r, t) = A(l, t) * B(r, t) for non-zero factors.  C's
+        modes are [A outer, B outer, inter] and its keys run B outer fastest,
+        then A outer, then inter (:1899)."""
+        a, dense = self.mia, other.mia
+        ta, tb = self.tokens, other.tokens
+        la, lb = [c for c, _ in ta], [c for c, _ in tb]
+        for c, bang in ta:
+            if c in lb:
+                if bang == 0 or dict(tb)[c] != bang:
+                    raise ValueError("index %r: a sparse x dense product takes only element-wise shared indices" % c)
+            elif bang:
+                raise ValueError("inter index %r appears in one operand only" % c)
+        if any(bang and c not in la for c, bang in tb):
+            raise ValueError("inter index appears in one operand only")
+        a_out = [m for m, c in enumerate(la) if c not in lb]
+        a_int = [m for m, c in enumerate(la) if c in lb]
+        b_out = [r for r, c in enumerate(lb) if c not in la]
+        b_int = [lb.index(la[m]) for m in a_int]
+        for m, r in zip(a_int, b_int):
+            if a.dims[m] != dense.dims[r]:
+                raise ValueError("matched indices differ in size")
+        a.sort(a_out + a_int)
+        ck, cv = a.ctx.sparse_dense_mult(a.keys, a.vals, _prod(a.dims[m] for m in a_out), dense.data, dense.dims,
+                                         b_out, b_int)
+        nl, nr, ni = len(a_out), len(b_out), len(a_int)
+        dims = [a.dims[m] for m in a_out] + [dense.dims[r] for r in b_out] + [a.dims[m] for m in a_int]
+        seq = list(range(nl, nl + nr)) + list(range(nl)) + list(range(nl + nr, nl + nr + ni))
+        tokens = [ta[m] for m in a_out] + [tb[r] for r in b_out] + [ta[m] for m in a_int]
+        return Atom(SparseMIA(dims, ck, cv, seq, True, a.ctx), tokens, ni)
+
     def __mul__(self, other: "Atom"):
         """The product; a complete inner product (every index contracted)
         returns its value as a float (sparse_mia_mult_test.cpp:159-165)."""
+        if isinstance(other, DenseAtom):
+            return self._mul_dense(other)
         a, b = self.mia, other.mia
         if b is a:
             b = b.copy()

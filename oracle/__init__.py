@@ -82,6 +82,8 @@ def lib():
             "lnto_contract_plan": (i32, [i32, _i64p, ctypes.c_char_p, i32, _i64p, ctypes.c_char_p, _i32p, _i32p, _i64p, _i64p, _i64p, ctypes.POINTER(ctypes.c_int)]),
             "lnto_is_equal_fuzzy": (i32, [f64, f64, f64]),
             "lnto_compare_sparse": (i32, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, f64]),
+            "lnto_sparse_dense_mult": (i64, [_i64p, _f64p, i64, i64, i32, _i64p, _f64p, _i32p, i32, _i32p, i32,
+                                             _i64p, _f64p]),
             "lnto_contract_attract": (i64, [_i64p, _f64p, i64, i32, _i64p, _i32p, _i32p, _i32p, i32, _i64p, _f64p]),
         }
         for name, (res, args) in sig.items():
@@ -289,6 +291,23 @@ def contract(ak, av, dims_a, seq_a_cur, a_sorted, labels_a, bk, bv, dims_b, seq_
     bk2, bv2 = sort_seq(bk, bv, dims_b, seq_b_cur, seq_b, b_sorted)
     ck, cv = lattice_mult(ak2, av2, tuple(int(x) for x in lat_a), bk2, bv2, tuple(int(x) for x in lat_b))
     return ck, cv, dims_c
+
+
+def sparse_dense_mult(ak, av, l_size, b, b_dims, r_modes, i_modes):
+    """noLatticeMult with a dense B (SparseMIABase.h:1829-1910).  A sorted with
+    key = l + L*t; b dense (first mode fastest); returns (keys, values) in
+    C's key r + l*R + t*L*R, ascending."""
+    ak, av = _i64(ak), _f64(av)
+    bd, b = _i64(b_dims), _f64(np.ravel(b, order="F"))
+    rm, im = _i32(r_modes), _i32(i_modes)
+    R = int(np.prod([b_dims[m] for m in r_modes])) if len(r_modes) else 1
+    cap = max(len(ak) * R, 1)
+    ck = np.empty(cap, dtype=np.int64)
+    cv = np.empty(cap, dtype=np.float64)
+    n = lib().lnto_sparse_dense_mult(_p(ak, _i64p), _p(av, _f64p), len(ak), int(l_size), len(bd), _p(bd, _i64p),
+                                     _p(b, _f64p), _p(rm, _i32p), len(rm), _p(im, _i32p), len(im), _p(ck, _i64p),
+                                     _p(cv, _f64p))
+    return ck[:n].copy(), cv[:n].copy()
 
 
 def contract_attract(keys, vals, dims, seq, group, kind):

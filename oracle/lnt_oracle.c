@@ -753,6 +753,45 @@ int lnto_is_equal_fuzzy(double a, double b, double prec) {
  * (LibMIAUtil.h:137-150). */
 static int below_tolerance(double v) { return fabs(v) <= 1.11e-16; }
 
+/* ======================= sparse x dense, no contraction ==================== */
+/* SparseMIABase.h:1829-1910 noLatticeMult with a dense B.  A: keys sorted with
+ * key = l + L*t (its outer modes, then its inter modes).  B: dense, first mode
+ * fastest; r_modes[] are B's outer modes, i_modes[] its inter modes in the
+ * order of A's.  For every A entry with a non-zero value and every B outer
+ * index r in order whose B value is non-zero (:1881-1896): C entry a*b at key
+ * r + l*R + t*L*R. */
+int64_t lnto_sparse_dense_mult(const int64_t *ak, const double *av, int64_t na, int64_t L, int order_b,
+                               const int64_t *b_dims, const double *b, const int32_t *r_modes, int nr,
+                               const int32_t *i_modes, int ni, int64_t *ck, double *cv) {
+  int64_t place[LNTO_MAX_ORDER], R = 1, T = 1, out = 0;
+  place[0] = 1;
+  for (int m = 1; m < order_b; ++m) place[m] = place[m - 1] * b_dims[m - 1];
+  for (int q = 0; q < nr; ++q) R *= b_dims[r_modes[q]];
+  for (int q = 0; q < ni; ++q) T *= b_dims[i_modes[q]];
+  (void)T;
+  for (int64_t e = 0; e < na; ++e) {
+    const int64_t t = ak[e] / L, l = ak[e] % L;
+    int64_t boff = 0, x = t;  /* B offset of the inter index (ind2sub over the inter dims) */
+    for (int q = 0; q < ni; ++q) {
+      boff += (x % b_dims[i_modes[q]]) * place[i_modes[q]];
+      x /= b_dims[i_modes[q]];
+    }
+    if (!av[e]) continue;
+    for (int64_t r = 0; r < R; ++r) {
+      int64_t off = boff, y = r;
+      for (int q = 0; q < nr; ++q) {
+        off += (y % b_dims[r_modes[q]]) * place[r_modes[q]];
+        y /= b_dims[r_modes[q]];
+      }
+      if (b[off]) {
+        ck[out] = r + l * R + t * L * R;
+        cv[out++] = av[e] * b[off];
+      }
+    }
+  }
+  return out;
+}
+
 /* ==================== unary contraction / attraction ===================== */
 /* SparseMIABase.h:1913-2053 contract_attract.  group[m] = 0: mode m stays;
  * g in 1..ngroups: mode m is in group g; kind[g-1] 0 = contraction, 1 =

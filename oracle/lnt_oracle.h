@@ -137,6 +137,13 @@ void lnto_mlp_forward(const double *params, const double *x, double *y);
 double lnto_mlp_loss_grad(const double *params, const double *X, const double *T, int64_t n,
                           double *grad);
 
+/* -------------------------- sparse x dense ----------------------------- */
+/* SparseMIABase.h:1829-1910 noLatticeMult with a dense B (see lnt_oracle.c).
+ * ck/cv need room for na * prod(B outer dims) entries. */
+int64_t lnto_sparse_dense_mult(const int64_t *ak, const double *av, int64_t na, int64_t L, int order_b,
+                               const int64_t *b_dims, const double *b, const int32_t *r_modes, int nr,
+                               const int32_t *i_modes, int ni, int64_t *ck, double *cv);
+
 /* ------------------------ unary contraction ---------------------------- */
 /* SparseMIABase.h:1913-2053 contract_attract (see lnt_oracle.c).  ck/cv need
  * room for n entries; returns the number written, sorted by key.  Sorts

@@ -372,6 +372,50 @@ int main() {
     }
     CHECK(thrown, "a repeated index in an assignment target not reported");
   }
+  // ---- sparse x dense with no contracted index (SparseMIABase.h:1829-1910)
+  {
+    lnt::SparseMIA<double, 3> a(4, 5, 6);
+    fill_random(a, 60, 51);
+    auto da = dense(a);
+    lnt::DenseMIA<double, 2> bd(6, 3), be(6, 5);
+    std::mt19937_64 rng(52);
+    std::uniform_real_distribution<double> val(-1.0, 1.0);
+    for (double *p = bd.data_begin(); p != bd.data_end(); ++p) *p = (rng() % 4 == 0) ? 0.0 : val(rng);
+    for (double *p = be.data_begin(); p != be.data_end(); ++p) *p = (rng() % 4 == 0) ? 0.0 : val(rng);
+    // C(i,j,k,l) = A(i,j,!k) * B(!k,l): outer x element-wise
+    lnt::SparseMIA<double, 4> c(4, 5, 6, 3);
+    c(i, j, k, l) = a(i, j, !k) * bd(!k, l);
+    auto dc = dense(c);
+    bool ok = true;
+    size_t expect_nnz = 0;
+    for (int64_t ii = 0; ii < 4; ++ii)
+      for (int64_t jj = 0; jj < 5; ++jj)
+        for (int64_t kk = 0; kk < 6; ++kk)
+          for (int64_t ll = 0; ll < 3; ++ll) {
+            const double w = da[ii + 4 * (jj + 5 * kk)] * bd.data()[kk + 6 * ll];
+            ok = ok && dc[ii + 4 * (jj + 5 * (kk + 6 * ll))] == w;
+            expect_nnz += w != 0.0;
+          }
+    CHECK(ok, "sparse x dense (outer x element-wise) mismatch");
+    CHECK(c.size() == expect_nnz, "sparse x dense kept a zero factor");
+    // C(i,k,j) = A(i,!j,!k) * B(!k,!j): pure element-wise over two indices, B's modes swapped
+    lnt::SparseMIA<double, 3> c2(4, 6, 5);
+    c2(i, k, j) = a(i, !j, !k) * be(!k, !j);
+    auto dc2 = dense(c2);
+    ok = true;
+    for (int64_t ii = 0; ii < 4; ++ii)
+      for (int64_t jj = 0; jj < 5; ++jj)
+        for (int64_t kk = 0; kk < 6; ++kk)
+          ok = ok && dc2[ii + 4 * (kk + 6 * jj)] == da[ii + 4 * (jj + 5 * kk)] * be.data()[kk + 6 * jj];
+    CHECK(ok, "sparse x dense (element-wise) mismatch");
+    bool thrown = false;
+    try {
+      c2(i, k, j) = a(i, j, k) * bd(k, l);  // a contracted index
+    } catch (const std::exception &) {
+      thrown = true;
+    }
+    CHECK(thrown, "a contraction with a dense operand not reported");
+  }
   if (failures) return 1;
   std::printf("C++ front end: all checks passed\n");
   return 0;

@@ -473,3 +473,36 @@ def test_contract_attract_summation_order(orc):
     assert group == [1, 2, 1, 2]
     ck, cv = orc.contract_attract(k[o], v[o], dims, (0, 1, 2, 3), group, kind)
     assert cv.tolist() == [1.0]  # (i0,j0) + (i1,j0) + (i0,j1); i-major order would give 0
+
+
+# ---------------------------------------------------------- sparse x dense
+def test_sparse_dense_mult_known_answers(orc):
+    """noLatticeMult with a dense B (SparseMIABase.h:1829-1910): C key
+    r + l*R + t*L*R, one product a*b per A entry and B outer index, skipped
+    when a or b is zero; checked against dense numpy on B modes that
+    interleave outer and inter modes."""
+    rng = np.random.default_rng(2)
+    # B dims (2, 3, 4, 5): outer modes 0 and 2 (R = 8), inter modes 1 and 3 (T = 15)
+    b_dims = (2, 3, 4, 5)
+    b = rng.uniform(-1, 1, b_dims)
+    b[rng.random(b_dims) < 0.3] = 0.0
+    L = 7
+    ak = np.sort(rng.choice(L * 15, 60, replace=False)).astype(np.int64)
+    av = rng.uniform(-1, 1, len(ak))
+    av[3] = 0.0  # a stored zero adds nothing
+    ck, cv = orc.sparse_dense_mult(ak, av, L, b, b_dims, [0, 2], [1, 3])
+    a = np.zeros((L, 15))
+    a[ak % L, ak // L] = av
+    bm = b.transpose(0, 2, 1, 3).reshape(8, 15, order="F")  # (r1 + 2 r2, t1 + 3 t2)
+    want = np.einsum("lt,rt->rlt", a, bm)
+    got = np.zeros(8 * L * 15)
+    got[ck] = cv
+    assert np.all(np.diff(ck) > 0)
+    assert np.array_equal(got, np.ravel(want, order="F"))  # single products: exact
+    assert set(ck.tolist()) == set(np.flatnonzero(np.ravel(want, order="F")).tolist())
+    # no outer modes on B (pure element-wise) and no inter modes (outer product)
+    ck, cv = orc.sparse_dense_mult(ak, av, L, bm[0].copy(), (15,), [], [0])
+    assert np.array_equal(ck, np.sort(ck)) and len(ck) == np.count_nonzero(a * bm[0][None, :])
+    ck, cv = orc.sparse_dense_mult(np.array([0, 2], np.int64), np.array([2.0, 3.0]), 3, np.array([1.0, 0.0, 5.0]), (3,),
+                                   [0], [])
+    assert ck.tolist() == [0, 2, 6, 8] and cv.tolist() == [2.0, 10.0, 3.0, 15.0]
