@@ -102,6 +102,10 @@ int lnt_abi_version(void);
  *   "narrow_keys"  1: keys of at most 32 bits move as 32-bit words between
  *                  radix passes and sit in LDS as 32-bit words during merges
  *                  (default); 0: always 64-bit
+ *   "row_ranks"    1 (default): the lattice product's long output columns sort
+ *                  the ranks of A's rows among the rows present instead of
+ *                  the rows when that saves a radix pass or gives 32-bit keys
+ *                  (one extra pass over A's keys); 0: rows
  * No reference counterpart (the reference hard-codes radix 2048 and the
  * thresholds 3000/50, multi_index_array/LibMIARadix.h:454-457). */
 int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value);

@@ -349,14 +349,57 @@ __device__ inline uint64_t chunk_segment(const ChunkSegs &c, const uint64_t *__r
   return s;
 }
 
-// Products of the large segments with composite key r * h + row, kChunk per
-// workgroup round.
+// Row ranks: the rows of A's entries marked in a bitmap over [0, h), the
+// prefix of its words' popcounts, and the rows in rank order.  A row's rank is
+// its position among the rows present, so ranks order products exactly as
+// rows do while needing fewer key bits (the composite keys of the large
+// segments can then take fewer radix passes, and 32-bit passes).
+__global__ void row_bitmap_kernel(const uint64_t *__restrict__ ak, uint64_t na, Divider ah, uint32_t *__restrict__ bm) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < na; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = ak[e];
+    const uint64_t row = key - divide(key, ah) * ah.d;
+    atomicOr(&bm[row >> 5], 1u << (row & 31));
+  }
+}
+
+__global__ void word_popc_kernel(const uint32_t *__restrict__ bm, uint64_t nw, uint32_t *__restrict__ cnt) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * blockDim.x)
+    cnt[w] = (uint32_t)__popc(bm[w]);
+}
+
+__global__ void rank_rows_kernel(const uint32_t *__restrict__ bm, const uint64_t *__restrict__ pre, uint64_t nw,
+                                 uint32_t *__restrict__ rows) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t bits = bm[w];
+    uint64_t p = pre[w];
+    while (bits) {
+      rows[p++] = (uint32_t)(w * 32 + (uint32_t)(__ffs(bits) - 1));
+      bits &= bits - 1;
+    }
+  }
+}
+
+// arank[e] = rank of the row of A entry e: computed once per A entry, then
+// read by every product of that entry in place of its key.
+__global__ void a_rank_kernel(const uint64_t *__restrict__ ak, uint64_t na, Divider ah,
+                              const uint32_t *__restrict__ bm, const uint64_t *__restrict__ pre,
+                              uint32_t *__restrict__ arank) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < na; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = ak[e];
+    const uint64_t row = key - divide(key, ah) * ah.d;
+    arank[e] = (uint32_t)(pre[row >> 5] + (uint64_t)__popc(bm[row >> 5] & ((1u << (row & 31)) - 1u)));
+  }
+}
+
+// Products of the large segments with composite key r * width + row (or the
+// row's rank when arank is set), kChunk per workgroup round.
 __global__ __launch_bounds__(256) void large_expand_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
     const uint64_t *__restrict__ a_lo, const uint64_t *__restrict__ a_base,
     const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg,
     uint64_t nb, const uint32_t *__restrict__ list3, uint32_t n3, const uint64_t *__restrict__ ex_off,
-    uint64_t total3, uint64_t ah, uint64_t *__restrict__ key3, double *__restrict__ val3) {
+    uint64_t total3, uint64_t width, const uint32_t *__restrict__ arank,
+    uint64_t *__restrict__ key3, double *__restrict__ val3) {
   __shared__ uint64_t s_off[kChunk + 1];
   __shared__ uint64_t s_seg[2];
   for (uint64_t q0 = (uint64_t)blockIdx.x * kChunk; q0 < total3; q0 += (uint64_t)gridDim.x * kChunk) {
@@ -370,7 +413,7 @@ __global__ __launch_bounds__(256) void large_expand_kernel(
       const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
       uint64_t e, ai;
       locate(prod_off, first, last, q - base, a_lo, &e, &ai);
-      key3[q] = r * ah + (ak[ai] - a_base[e]);
+      key3[q] = r * width + (arank ? (uint64_t)arank[ai] : ak[ai] - a_base[e]);
       val3[q] = bv[e] * av[ai];
     }
     __syncthreads();  // s_off is restaged next round
@@ -388,7 +431,8 @@ __global__ void run_heads_kernel(const uint64_t *__restrict__ key3, uint64_t tot
 // region of tmp_row/tmp_val.
 __global__ void large_reduce_kernel(const uint64_t *__restrict__ key3, const double *__restrict__ val3,
                                     const uint32_t *__restrict__ heads, const uint64_t *__restrict__ grank,
-                                    uint64_t total3, Divider ah, const uint32_t *__restrict__ list3,
+                                    uint64_t total3, Divider width, const uint32_t *__restrict__ rows,
+                                    const uint32_t *__restrict__ list3,
                                     const uint64_t *__restrict__ ex_off, const uint64_t *__restrict__ seg_first,
                                     const uint64_t *__restrict__ prod_off, uint64_t *__restrict__ tmp_row,
                                     double *__restrict__ tmp_val) {
@@ -396,8 +440,9 @@ __global__ void large_reduce_kernel(const uint64_t *__restrict__ key3, const dou
   for (; q < total3; q += (uint64_t)gridDim.x * blockDim.x) {
     if (!heads[q]) continue;
     uint64_t key = key3[q];
-    uint64_t r = divide(key, ah);
-    uint64_t row = key - r * ah.d;
+    uint64_t r = divide(key, width);
+    uint64_t row = key - r * width.d;
+    if (rows) row = rows[row];
     double sum = val3[q];
     for (uint64_t x = q + 1; x < total3 && key3[x] == key; ++x) sum += val3[x];
     uint64_t local = grank[q] - grank[ex_off[r]];
@@ -601,10 +646,46 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     LNT_HIP(hipGetLastError());
   }
   if (n3) {
-    // Composite keys r*h + row must stay below 2^63; otherwise the large
+    // Rows replaced by their ranks among A's rows when that takes the
+    // composite keys below a radix-pass boundary or to 32 bits.
+    uint64_t width = ah;
+    const uint32_t *arank = nullptr, *rrows = nullptr;
+    if (ctx->tuning.row_ranks && nprod >= (uint64_t(1) << 20) && ah <= (uint64_t(1) << 31)) {
+      const uint64_t nw = (ah + 31) / 32;
+      LNT_TRY(ensure(ctx->spg_rank, nw * 8 + (nw + 1) * 8 + na * 8 + 1024));
+      Carve c{(char *)ctx->spg_rank.ptr};
+      uint32_t *bm = c.take<uint32_t>(nw);
+      uint32_t *cnt = c.take<uint32_t>(nw);
+      uint64_t *pre = c.take<uint64_t>(nw + 1);
+      uint32_t *rows = c.take<uint32_t>(na);
+      uint32_t *ar = c.take<uint32_t>(na);
+      LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan_workspace_words(nw)) * 8 + 64));
+      LNT_TRY(launch_fill_u32(ctx, bm, 0, nw));
+      row_bitmap_kernel<<<grid_for(na), 256, 0, ctx->stream>>>(ak, na, make_divider(ah), bm);
+      word_popc_kernel<<<grid_for(nw), 256, 0, ctx->stream>>>(bm, nw, cnt);
+      LNT_HIP(hipGetLastError());
+      LNT_TRY(exclusive_scan<uint32_t>(ctx, cnt, pre, nw, pre + nw, ctx->aux, 0));
+      uint64_t nrank;
+      LNT_TRY(read_u64(ctx, pre + nw, &nrank, 1));
+      auto bits = [](unsigned __int128 v) {
+        uint32_t b = 0;
+        while (b < 128 && ((unsigned __int128)1 << b) < v) ++b;
+        return b;
+      };
+      const uint32_t old_bits = bits((unsigned __int128)n3 * ah), new_bits = bits((unsigned __int128)n3 * nrank);
+      if ((new_bits + 7) / 8 < (old_bits + 7) / 8 || (new_bits <= 32 && old_bits > 32)) {
+        rank_rows_kernel<<<grid_for(nw), 256, 0, ctx->stream>>>(bm, pre, nw, rows);
+        a_rank_kernel<<<grid_for(na), 256, 0, ctx->stream>>>(ak, na, make_divider(ah), bm, pre, ar);
+        LNT_HIP(hipGetLastError());
+        width = nrank;
+        arank = ar;
+        rrows = rows;
+      }
+    }
+    // Composite keys r*width + row must stay below 2^63; otherwise the large
     // segments go through in groups that satisfy it.
     uint32_t group = n3;
-    while (group > 1 && (unsigned __int128)group * ah >= ((unsigned __int128)1 << 63)) group = (group + 1) / 2;
+    while (group > 1 && (unsigned __int128)group * width >= ((unsigned __int128)1 << 63)) group = (group + 1) / 2;
     for (uint32_t g0 = 0; g0 < n3; g0 += group) {
       uint32_t gn = std::min(group, n3 - g0);
       const uint32_t *l3 = E.list3 + g0;
@@ -622,16 +703,18 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
       uint64_t *grank = c.take<uint64_t>(total3 + 1);
       LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan3) * 8 + 64));
       large_expand_kernel<<<chunk_grid(total3), 256, 0, ctx->stream>>>(
-          ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first, nseg, nb, l3, gn, E.ex_off, total3, ah, key3, val3);
+          ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first, nseg, nb, l3, gn, E.ex_off, total3, width, arank,
+          key3, val3);
       LNT_HIP(hipGetLastError());
-      LNT_TRY(sort_device(ctx, key3, val3, (uint32_t)total3, (uint64_t)gn * ah, nullptr));
+      LNT_TRY(sort_device(ctx, key3, val3, (uint32_t)total3, (uint64_t)gn * width, nullptr));
       run_heads_kernel<<<grid_for(total3), 256, 0, ctx->stream>>>(key3, total3, heads);
       LNT_HIP(hipGetLastError());
       LNT_TRY(exclusive_scan<uint32_t>(ctx, heads, grank, total3, grank + total3, ctx->aux, 0));
       uint64_t total_heads;
       LNT_TRY(read_u64(ctx, grank + total3, &total_heads, 1));
-      large_reduce_kernel<<<grid_for(total3), 256, 0, ctx->stream>>>(key3, val3, heads, grank, total3, make_divider(ah),
-                                                                      l3, E.ex_off, E.seg_first, E.prod_off, tmp_row, tmp_val);
+      large_reduce_kernel<<<grid_for(total3), 256, 0, ctx->stream>>>(key3, val3, heads, grank, total3,
+                                                                      make_divider(width), rrows, l3, E.ex_off,
+                                                                      E.seg_first, E.prod_off, tmp_row, tmp_val);
       large_counts_kernel<<<(gn + 255) / 256, 256, 0, ctx->stream>>>(l3, gn, E.ex_off, grank, total_heads, total3, E.seg_u);
       LNT_HIP(hipGetLastError());
     }

@@ -89,6 +89,31 @@ def test_lattice_mult_sort_strategy_is_deterministic(ctx, orc):
         assert np.array_equal(host(cv).view(np.int64), host(outs[0][1]).view(np.int64))
 
 
+@pytest.mark.parametrize("shape", [(1 << 20, 64, 1, 4096, 200000, 3000), (4096 * 4096, 512, 1, 4096 * 64, 858993, 13422)])
+def test_lattice_mult_row_ranks(ctx, orc, shape):
+    """Long output columns sorted by row rank instead of row: the same keys
+    and the same value bits as with rows (same sort order, same summation
+    order), for both accumulators, and the oracle's result."""
+    ah, aw, d, bw, n_a, n_b = shape
+    print("row ranks", shape, flush=True)
+    ak, av = _random_lattice(orc, ah, aw, d, n_a, 21)
+    bk, bv = _random_lattice(orc, aw, bw, d, n_b, 22)
+    ek, ev = orc.lattice_mult(ak, av, (ah, aw, d), bk, bv, (aw, bw, d))
+    try:
+        for strategy in (1, 2):
+            outs = []
+            for ranks in (1, 0):
+                ctx.set_param("row_ranks", ranks)
+                outs.append(ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, d), gpu(bk), gpu(bv), (aw, bw, d), strategy))
+            assert np.array_equal(host(outs[0][0]), host(outs[1][0]))
+            if strategy == 1:  # the sort accumulator is deterministic
+                assert np.array_equal(host(outs[0][1]).view(np.int64), host(outs[1][1]).view(np.int64))
+            assert np.array_equal(host(outs[0][0]), ek)
+            assert orc.allclose_fuzzy(host(outs[0][1]), ev, TOL)
+    finally:
+        ctx.set_param("row_ranks", 1)
+
+
 def test_lattice_mult_empty_and_errors(ctx, orc):
     import libnt_amd
     import torch
