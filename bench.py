@@ -69,7 +69,17 @@ def make_c2(ctx, torch, scale=1):
     return ak, av, bk, bv
 
 
-def cpu_baseline(scale=8, repeats=3):
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scale=8, repeats=5):
     """The oracle (CPU restatement of the reference's algorithms) on one core:
     the same step on a 1/scale sample.  Returns the JSON object."""
     import oracle
@@ -102,8 +112,11 @@ def cpu_baseline(scale=8, repeats=3):
             "unit": "keys/s",
             "cores": 1,
             "kind": "port",
-            "sample": "1/%d-scale c2 step (%d+%d keys): 2x change_sort_order (relinearise + in-place MSD radix "
-                      "2048/11-bit, thresholds 3000/50) + scan merge A+B + Hadamard; median of %d, %.2f s each"
+            "cpu_model": _cpu_model(),
+            "nproc": len(old),
+            "sample": "1/%d-scale c2 step (%d+%d keys) on one pinned core: 2x change_sort_order (relinearise + "
+                      "in-place MSD radix 2048/11-bit, thresholds 3000/50) + scan merge A+B + Hadamard (the "
+                      "reference's reserve(nA*nB/2^Inter) replaced by min(nA, nB)); median of %d, %.2f s each"
                       % (scale, len(ak0), len(bk0), repeats, t),
         }
     finally:
