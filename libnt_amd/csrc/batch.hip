@@ -204,17 +204,25 @@ __device__ inline int lds_radix_sort(uint64_t *k0, uint16_t *p0, uint64_t *k1, u
       key[u] = idx < n ? kin[idx] : ~0ull;
       pos[u] = idx < n ? pin[idx] : 0;
       const uint32_t d = (uint32_t)(key[u] >> shift) & 255u;
-      uint64_t match = ~0ull;
+      // Lanes that differ from this one in some digit bit: per bit, m = the
+      // bit sign-extended, and x |= ballot ^ m (v_bitop3 truth table 0xF6 =
+      // src0 | (src1 ^ src2)), one instruction per half of the wave.
+      uint32_t x_lo = 0, x_hi = 0;
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
-        const uint64_t bal = __ballot((d >> b) & 1u);
-        match &= ((d >> b) & 1u) ? bal : ~bal;
+        const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);  // 0 or ~0
+        const uint64_t bal = __ballot(m != 0);
+        x_lo = __builtin_amdgcn_bitop3_b32(x_lo, (uint32_t)bal, m, 0xF6);
+        x_hi = __builtin_amdgcn_bitop3_b32(x_hi, (uint32_t)(bal >> 32), m, 0xF6);
       }
+      const uint64_t match = ~(((uint64_t)x_hi << 32) | x_lo);
       const uint32_t r = (uint32_t)__popcll(match & lt);
+      // Every lane of a digit group reads the count before any lane writes
+      // it, and all write the same new count: no leader, no branch.
       const uint32_t base = s_cnt[wave * 256 + d];
       rank[u] = base + r;
       dig[u] = d;
-      if (r == 0) s_cnt[wave * 256 + d] = base + (uint32_t)__popcll(match);
+      s_cnt[wave * 256 + d] = base + (uint32_t)__popcll(match);
     }
     __syncthreads();
     {
