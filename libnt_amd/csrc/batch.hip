@@ -179,7 +179,8 @@ __device__ inline void bitonic_key_pos(uint64_t *s_key, uint16_t *s_pos, uint32_
 // w*512 + u*64 + l, so ranking slot by slot inside each wave and waves in
 // order keeps equal keys in element order.  Peers with the same digit are
 // found by ballot matching; per-wave digit counters live in s_cnt[4][256].
-__device__ inline int lds_radix_sort(uint64_t *k0, uint16_t *p0, uint64_t *k1, uint16_t *p1, uint32_t n,
+template <typename K>
+__device__ inline int lds_radix_sort(K *k0, uint16_t *p0, K *k1, uint16_t *p1, uint32_t n,
                                      uint32_t lo_bits, uint32_t hi_shift, uint32_t hi_bits, uint32_t *s_cnt,
                                      uint32_t *s_scan) {
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -189,11 +190,11 @@ __device__ inline int lds_radix_sort(uint64_t *k0, uint16_t *p0, uint64_t *k1, u
   const uint32_t lo_passes = (lo_bits + 7) / 8, passes = lo_passes + (hi_bits + 7) / 8;
   for (uint32_t pass = 0; pass < passes; ++pass) {
     const uint32_t shift = pass < lo_passes ? 8 * pass : hi_shift + 8 * (pass - lo_passes);
-    uint64_t *kin = buf ? k1 : k0, *kout = buf ? k0 : k1;
+    K *kin = buf ? k1 : k0, *kout = buf ? k0 : k1;
     uint16_t *pin = buf ? p1 : p0, *pout = buf ? p0 : p1;
     for (uint32_t i = tid; i < 4 * 256; i += kBlock) s_cnt[i] = 0;
     __syncthreads();
-    uint64_t key[8];
+    K key[8];
     uint16_t pos[8];
     uint32_t rank[8], dig[8];
     const uint64_t lt = (1ull << lane) - 1;
@@ -201,7 +202,7 @@ __device__ inline int lds_radix_sort(uint64_t *k0, uint16_t *p0, uint64_t *k1, u
     for (int u = 0; u < 8; ++u) {
       if (u >= slots) break;
       const uint32_t idx = first_idx + u * 64 + lane;
-      key[u] = idx < n ? kin[idx] : ~0ull;
+      key[u] = idx < n ? kin[idx] : (K)~(K)0;
       pos[u] = idx < n ? pin[idx] : 0;
       const uint32_t d = (uint32_t)(key[u] >> shift) & 255u;
       // Lanes that differ from this one in some digit bit: per bit, m = the
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
       continue;
     }
     const uint32_t rowbits = bit_width(block_max_u64(maxrow, s_red));
-    const int which = lds_radix_sort((uint64_t *)s_key, s_pos, s_key2, s_pos2, np, rowbits, kRowBits,
+    const int which = lds_radix_sort<uint64_t>((uint64_t *)s_key, s_pos, s_key2, s_pos2, np, rowbits, kRowBits,
                                      nr > 1 ? bit_width(nr - 1) : 0, s_rcnt, s_scan);
     const uint64_t *kk = which ? s_key2 : (const uint64_t *)s_key;
     const uint16_t *pp = which ? s_pos2 : s_pos;
@@ -612,14 +613,17 @@ __device__ inline uint64_t gallop(const uint64_t *ak, uint64_t c, uint64_t end, 
 // by jumping to the smallest next row.  Symbolic: distinct rows counted in an
 // LDS hash set.  Numeric: each window is sorted by (row, position) and its
 // runs summed, so rows are summed in ascending B-row order.
-template <bool NUMERIC>
+template <bool NUMERIC, typename K>
 __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const uint32_t *__restrict__ list3,
                                                              uint64_t *__restrict__ cur, uint32_t *__restrict__ wcnt,
                                                              uint32_t *__restrict__ woff, uint32_t window,
                                                              uint64_t *__restrict__ ctr) {
-  __shared__ unsigned long long s_key[NUMERIC ? kCap : kHashSlots];
+  // K: key word of the window sort (window-relative rows); 32 bits when
+  // every instance's rows fit, which halves the sort's LDS.
+  __shared__ unsigned long long s_hash[NUMERIC ? 1 : kHashSlots];
+  __shared__ K s_key[NUMERIC ? kCap : 1];
   __shared__ uint16_t s_pos[NUMERIC ? kCap : 1];
-  __shared__ uint64_t s_key2[NUMERIC ? kCap : 1];
+  __shared__ K s_key2[NUMERIC ? kCap : 1];
   __shared__ uint16_t s_pos2[NUMERIC ? kCap : 1];
   __shared__ uint32_t s_rcnt[NUMERIC ? 4 * 256 : 1];
   __shared__ double s_val[NUMERIC ? kCap : 1];
@@ -710,7 +714,7 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
         running += tot;
       }
       if (!NUMERIC)
-        for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_key[i] = kEmptyKey;
+        for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_hash[i] = kEmptyKey;
       __syncthreads();
       const uint32_t n = (uint32_t)cnt;
       uint32_t fresh = 0;
@@ -736,11 +740,11 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
           if (NUMERIC) {
             const uint64_t rel = row - r0;  // rows of the window lie in [r0, r1)
             maxrel = rel > maxrel ? rel : maxrel;
-            s_key[p] = rel;
+            s_key[p] = (K)rel;
             s_pos[p] = (uint16_t)p;
             s_val[p] = bval * A.av[x];
           } else {
-            fresh += hash_insert(s_key, row);
+            fresh += hash_insert(s_hash, row);
           }
         }
       }
@@ -749,8 +753,8 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
         block256_exclusive_sum(fresh, s_scan, &heads);
       } else {
         const uint32_t relbits = bit_width(block_max_u64(maxrel, s_red));
-        const int which = lds_radix_sort((uint64_t *)s_key, s_pos, s_key2, s_pos2, n, relbits, 0, 0, s_rcnt, s_scan);
-        const uint64_t *kk = which ? s_key2 : (const uint64_t *)s_key;
+        const int which = lds_radix_sort<K>(s_key, s_pos, s_key2, s_pos2, n, relbits, 0, 0, s_rcnt, s_scan);
+        const K *kk = which ? s_key2 : s_key;
         const uint16_t *pp = which ? s_pos2 : s_pos;
         uint32_t flags = 0, hc = 0;
 #pragma unroll
@@ -865,6 +869,7 @@ struct lnt_batch_s {
   uint32_t count = 0;
   uint64_t na = 0, nb = 0;
   uint64_t a_bound = 0, b_bound = 0;
+  uint64_t max_h = 0;  // largest A lattice height (row count) of the instances
   int cus = 256;
   lnt::BatchTuning tuning;
   lnt::DeviceBuffer meta;  // InstDev[count], RelinDev[2*count], a_off/b_off[count+1]
@@ -980,7 +985,7 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
   const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
   batch_small_kernel<false><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, P.ctr);
-  batch_large_kernel<false><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
+  batch_large_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
   LNT_HIP(hipGetLastError());
   LNT_TRY(exclusive_scan<uint64_t>(ctx, P.seg_u, P.out_off, nb, P.ctr + kCtrNc, b->scan_tmp, 0));
   return LNT_OK;
@@ -997,7 +1002,10 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
     const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
     batch_single_kernel<<<std::min<uint32_t>(grid_for(nb * 64), 65536), kBlock, 0, st>>>(A, P.kind, P.ctr);
     batch_small_kernel<true><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, P.ctr);
-    batch_large_kernel<true><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
+    if (b->max_h <= (uint64_t(1) << 32))
+      batch_large_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
+    else
+      batch_large_kernel<true, uint64_t><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
     LNT_HIP(hipGetLastError());
   }
   batch_offsets_kernel<<<grid_for(b->count + 1), kBlock, 0, st>>>(P.b_off, b->count, nb, P.seg_idx, P.out_off,
@@ -1070,7 +1078,7 @@ int lnt_batch_create(lnt_context ctx, const lnt_contraction *items, int64_t coun
   std::vector<RelinDev> ra(count), rb(count);
   std::vector<uint64_t> a_off(count + 1), b_off(count + 1);
   unsigned __int128 abound = 0, bbound = 0;
-  uint64_t na = 0, nb = 0;
+  uint64_t na = 0, nb = 0, max_h = 0;
   for (int64_t i = 0; i < count; ++i) {
     const lnt_contraction &c = items[i];
     const std::string at = "instance " + std::to_string(i) + ": ";
@@ -1102,6 +1110,7 @@ int lnt_batch_create(lnt_context ctx, const lnt_contraction *items, int64_t coun
     I.a_dim = da;
     I.b_dim = db;
     I.h = (uint64_t)c.lat_a[0];
+    max_h = std::max(max_h, I.h);
     I.w = (uint64_t)c.lat_a[1];
     I.bw = (uint64_t)c.lat_b[1];
     I.div_w = make_divider(I.w);
@@ -1128,6 +1137,7 @@ int lnt_batch_create(lnt_context ctx, const lnt_contraction *items, int64_t coun
   b->ctx = ctx;
   b->count = (uint32_t)count;
   b->na = na;
+  b->max_h = max_h;
   b->nb = nb;
   b->a_bound = std::max<uint64_t>((uint64_t)abound, 1);
   b->b_bound = std::max<uint64_t>((uint64_t)bbound, 1);

@@ -228,6 +228,27 @@ def test_batch_edge_cases(ctx, orc):
     assert err.value.status == 5
 
 
+def test_batch_tall_lattices(ctx, orc):
+    """An instance whose A lattice has more than 2^32 rows: the large-segment
+    window sort keeps 64-bit keys for the whole batch (32-bit otherwise);
+    results against the oracle either way."""
+    from libnt_amd.batch import Instance
+
+    tall = [Instance([1 << 17, 1 << 17, 4], "ijk", [4, 3], "kl", 20000, 12),  # h = 2^34: long columns
+            Instance([6, 6, 6, 6], "abcd", [6, 6, 6, 6], "cdef", 500, 600)]
+    short = [Instance([1 << 12, 1 << 12, 4], "ijk", [4, 3], "kl", 20000, 12), tall[1]]
+    for insts in (tall, short):
+        b, ops, ck, cv, off = _run(ctx, insts)
+        ck, cv = host(ck), host(cv)
+        for i, x in enumerate(insts):
+            ak, av, bk, bv = _oracle_operands(orc, insts, i)
+            ident_a, ident_b = list(range(len(x.dims_a))), list(range(len(x.dims_b)))
+            ek, ev, _ = orc.contract(ak, av, x.dims_a, ident_a, False, x.labels_a,
+                                     bk, bv, x.dims_b, ident_b, False, x.labels_b)
+            assert np.array_equal(ck[off[i]:off[i + 1]], ek), i
+            assert orc.allclose_fuzzy(cv[off[i]:off[i + 1]], ev, TOL), i
+
+
 def test_batch_invalid_descriptions(ctx):
     import libnt_amd
     from libnt_amd.batch import Batch, Instance
