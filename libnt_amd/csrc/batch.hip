@@ -453,20 +453,27 @@ __device__ inline void load_entry(const ColumnArgs &A, uint64_t e, EntryCursor &
 // then a walk).  Symbolic: distinct (segment, row) keys counted in an LDS
 // hash set.  Numeric: the products are sorted together by (segment, row,
 // position) and equal runs summed in position order.
-template <bool NUMERIC>
+template <bool NUMERIC, typename K>
 __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const uint32_t *__restrict__ small_list,
                                                              const uint64_t *__restrict__ sp_off, uint32_t G,
-                                                             uint64_t *__restrict__ ctr) {
-  __shared__ unsigned long long s_key[NUMERIC ? kCap : kHashSlots];
+                                                             uint32_t rbits, uint64_t *__restrict__ ctr) {
+  // Sort keys (segment of the item << rbits) | row in K words: 32 bits when
+  // rbits plus the item's segment bits fit (the batch's rows are below
+  // 2^rbits).  In
+  // the numeric pass the per-segment rank bases share LDS with the sort's
+  // digit counters (they are written after the sort), and the item's product
+  // offsets are 16-bit (an item holds fewer than 2 * kCap products).
+  __shared__ unsigned long long s_hash[NUMERIC ? 1 : kHashSlots];
+  __shared__ K s_key[NUMERIC ? kCap : 1];
   __shared__ uint16_t s_pos[NUMERIC ? kCap : 1];
-  __shared__ uint64_t s_key2[NUMERIC ? kCap : 1];
+  __shared__ K s_key2[NUMERIC ? kCap : 1];
   __shared__ uint16_t s_pos2[NUMERIC ? kCap : 1];
-  __shared__ uint32_t s_rcnt[NUMERIC ? 4 * 256 : 1];
+  __shared__ uint32_t s_rcnt[NUMERIC ? (4 * 256 > kMaxSegsPerItem ? 4 * 256 : kMaxSegsPerItem) : kMaxSegsPerItem];
   __shared__ uint64_t s_red[kBlock / kWave];
   __shared__ double s_val[NUMERIC ? kCap : 1];
   __shared__ uint32_t s_seg[kMaxSegsPerItem];
-  __shared__ uint32_t s_spo[kMaxSegsPerItem + 1];
-  __shared__ uint32_t s_cnt[kMaxSegsPerItem];  // symbolic: distinct rows; numeric: rank base
+  __shared__ uint16_t s_spo[kMaxSegsPerItem + 1];
+  uint32_t *const s_cnt = s_rcnt;  // symbolic: distinct rows; numeric: rank base
   __shared__ uint32_t s_scan[kBlock / kWave];
   __shared__ uint64_t s_item, s_rb, s_re;
   const uint32_t tid = threadIdx.x;
@@ -494,10 +501,10 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
         s_seg[q] = small_list[rb + q];
         s_cnt[q] = 0;
       }
-      s_spo[q] = (uint32_t)((rb + q < n_small ? sp_off[rb + q] : total) - pbase);
+      s_spo[q] = (uint16_t)((rb + q < n_small ? sp_off[rb + q] : total) - pbase);
     }
     if (!NUMERIC)
-      for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_key[i] = kEmptyKey;
+      for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_hash[i] = kEmptyKey;
     __syncthreads();
     const uint32_t np = s_spo[nr];
     const uint32_t p0 = tid * kPer;
@@ -528,13 +535,13 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
         }
         const uint64_t ai = c.alo + g;
         const uint64_t row = A.ak[ai] - c.abase;
-        const uint64_t key = ((uint64_t)q << kRowBits) | row;
+        const uint64_t key = ((uint64_t)q << rbits) | row;
         maxrow = row > maxrow ? row : maxrow;
         if (NUMERIC) {
-          s_key[p] = key;
+          s_key[p] = (K)key;
           s_pos[p] = (uint16_t)p;
           s_val[p] = c.b * A.av[ai];
-        } else if (hash_insert(s_key, key)) {
+        } else if (hash_insert(s_hash, key)) {
           atomicAdd(&s_cnt[q], 1u);
         }
         ++g;
@@ -547,9 +554,9 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
       continue;
     }
     const uint32_t rowbits = bit_width(block_max_u64(maxrow, s_red));
-    const int which = lds_radix_sort<uint64_t>((uint64_t *)s_key, s_pos, s_key2, s_pos2, np, rowbits, kRowBits,
-                                     nr > 1 ? bit_width(nr - 1) : 0, s_rcnt, s_scan);
-    const uint64_t *kk = which ? s_key2 : (const uint64_t *)s_key;
+    const int which = lds_radix_sort<K>(s_key, s_pos, s_key2, s_pos2, np, rowbits, rbits,
+                                        nr > 1 ? bit_width(nr - 1) : 0, s_rcnt, s_scan);
+    const K *kk = which ? s_key2 : s_key;
     const uint16_t *pp = which ? s_pos2 : s_pos;
     // Heads of equal-key runs, ranked; the first element of every segment
     // records its rank so that ranks can be made segment-relative.
@@ -567,8 +574,8 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
     for (int u = 0; u < kPer; ++u) {
       const uint32_t p = tid * kPer + u;
       if ((flags >> u) & 1u) {
-        if (p == 0 || (kk[p] >> kRowBits) != (kk[p - 1] >> kRowBits))
-          s_cnt[kk[p] >> kRowBits] = rank;
+        if (p == 0 || (kk[p] >> rbits) != (kk[p - 1] >> rbits))
+          s_cnt[kk[p] >> rbits] = rank;
         ++rank;
       }
     }
@@ -578,13 +585,13 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
       if (!((flags >> u) & 1u)) continue;
       const uint32_t p = tid * kPer + u;
       const uint64_t key = kk[p];
-      const uint32_t q = (uint32_t)(key >> kRowBits);
+      const uint32_t q = (uint32_t)(key >> rbits);
       double sum = s_val[pp[p]];
       for (uint32_t r = p + 1; r < np && kk[r] == key; ++r) sum += s_val[pp[r]];
       const uint64_t s = s_seg[q];
       const uint64_t dst = A.out_off[s] + (rank - s_cnt[q]);
       if (dst < A.capacity) {
-        A.ck[dst] = (key & kRowMask) + A.colkey[A.seg_first[s]];
+        A.ck[dst] = (key & ((uint64_t(1) << rbits) - 1)) + A.colkey[A.seg_first[s]];
         A.cv[dst] = sum;
       }
       ++rank;
@@ -984,7 +991,7 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   const uint32_t grid = (uint32_t)b->cus * b->tuning.blocks_per_cu;
   const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
   const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
-  batch_small_kernel<false><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, P.ctr);
+  batch_small_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
   batch_large_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
   LNT_HIP(hipGetLastError());
   LNT_TRY(exclusive_scan<uint64_t>(ctx, P.seg_u, P.out_off, nb, P.ctr + kCtrNc, b->scan_tmp, 0));
@@ -1001,7 +1008,12 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
     const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
     const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
     batch_single_kernel<<<std::min<uint32_t>(grid_for(nb * 64), 65536), kBlock, 0, st>>>(A, P.kind, P.ctr);
-    batch_small_kernel<true><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, P.ctr);
+    // Items hold at most kMaxSegsPerItem segments: 11 bits above the rows.
+    const uint32_t row_bits = b->max_h > 1 ? 64u - (uint32_t)__builtin_clzll(b->max_h - 1) : 0u;
+    if (row_bits + 11 <= 32)
+      batch_small_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, row_bits, P.ctr);
+    else
+      batch_small_kernel<true, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
     if (b->max_h <= (uint64_t(1) << 32))
       batch_large_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
     else
