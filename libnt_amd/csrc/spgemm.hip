@@ -427,14 +427,23 @@ __global__ void run_heads_kernel(const uint64_t *__restrict__ key3, uint64_t tot
     heads[q] = q == 0 || key3[q] != key3[q - 1];
 }
 
+// dst_base[r]: where the runs of large segment r start in tmp_row/tmp_val,
+// less the global rank of its first run (mod 2^64), so that a run's slot is
+// its global rank plus one per-segment word.
+__global__ void large_base_kernel(const uint32_t *__restrict__ list3, uint32_t n3,
+                                  const uint64_t *__restrict__ ex_off, const uint64_t *__restrict__ grank,
+                                  const uint64_t *__restrict__ seg_first, const uint64_t *__restrict__ prod_off,
+                                  uint64_t *__restrict__ dst_base) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n3) dst_base[r] = prod_off[seg_first[list3[r]]] - grank[ex_off[r]];
+}
+
 // Sums every run (in sorted = expansion order) and stores it in its segment's
 // region of tmp_row/tmp_val.
 __global__ void large_reduce_kernel(const uint64_t *__restrict__ key3, const double *__restrict__ val3,
                                     const uint32_t *__restrict__ heads, const uint64_t *__restrict__ grank,
                                     uint64_t total3, Divider width, const uint32_t *__restrict__ rows,
-                                    const uint32_t *__restrict__ list3,
-                                    const uint64_t *__restrict__ ex_off, const uint64_t *__restrict__ seg_first,
-                                    const uint64_t *__restrict__ prod_off, uint64_t *__restrict__ tmp_row,
+                                    const uint64_t *__restrict__ dst_base, uint64_t *__restrict__ tmp_row,
                                     double *__restrict__ tmp_val) {
   uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; q < total3; q += (uint64_t)gridDim.x * blockDim.x) {
@@ -445,8 +454,7 @@ __global__ void large_reduce_kernel(const uint64_t *__restrict__ key3, const dou
     if (rows) row = rows[row];
     double sum = val3[q];
     for (uint64_t x = q + 1; x < total3 && key3[x] == key; ++x) sum += val3[x];
-    uint64_t local = grank[q] - grank[ex_off[r]];
-    uint64_t dst = prod_off[seg_first[list3[r]]] + local;
+    const uint64_t dst = grank[q] + dst_base[r];
     tmp_row[dst] = row;
     tmp_val[dst] = sum;
   }
@@ -712,9 +720,12 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
       LNT_TRY(exclusive_scan<uint32_t>(ctx, heads, grank, total3, grank + total3, ctx->aux, 0));
       uint64_t total_heads;
       LNT_TRY(read_u64(ctx, grank + total3, &total_heads, 1));
+      // np3 is free once ex_off is scanned: it takes the per-segment bases.
+      large_base_kernel<<<(gn + 255) / 256, 256, 0, ctx->stream>>>(l3, gn, E.ex_off, grank, E.seg_first, E.prod_off,
+                                                                    E.np3);
       large_reduce_kernel<<<grid_for(total3), 256, 0, ctx->stream>>>(key3, val3, heads, grank, total3,
-                                                                      make_divider(width), rrows, l3, E.ex_off,
-                                                                      E.seg_first, E.prod_off, tmp_row, tmp_val);
+                                                                      make_divider(width), rrows, E.np3, tmp_row,
+                                                                      tmp_val);
       large_counts_kernel<<<(gn + 255) / 256, 256, 0, ctx->stream>>>(l3, gn, E.ex_off, grank, total_heads, total3, E.seg_u);
       LNT_HIP(hipGetLastError());
     }
