@@ -12,7 +12,8 @@
 // One kernel, one pass over the inputs.  The merge path of A and B (ties: A
 // first) is cut into tiles of BT*ITEMS steps; a tile finds its end points in
 // the splits precomputed by partition_kernel and loads only the KEYS of its two
-// slices into LDS.  Each thread merges ITEMS consecutive steps with the two
+// slices into LDS, as 32-bit words relative to the tile's smallest key (64-bit
+// keys split into two word arrays in the rare tile whose keys span 2^32).  Each thread merges ITEMS consecutive steps with the two
 // current keys held in registers (one LDS load per step) and records, per step,
 // a descriptor: which A entry and/or which B entry produces the output.
 // Because ties put A first, a key present in both inputs appears as A[i]
@@ -29,6 +30,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "lnt_internal.h"
 
@@ -87,84 +89,45 @@ __device__ inline void load_tile_keys(const uint64_t *__restrict__ ak, const uin
   }
 }
 
-// LK: the key type held in LDS -- uint32_t when every key of both inputs is
-// below 2^32 (half the LDS per tile, so more tiles per CU), else uint64_t.
-template <int BT, int ITEMS, typename LK>
-__global__ __launch_bounds__(BT) void merge_kernel(
-    const uint64_t *__restrict__ ak, const double *__restrict__ av, uint32_t na,
-    const uint64_t *__restrict__ bk, const double *__restrict__ bv, uint32_t nb,
-    int op, uint64_t *__restrict__ ck, double *__restrict__ cv,
-    const uint32_t *__restrict__ splits, uint64_t *__restrict__ status,
-    uint32_t epoch, uint32_t *__restrict__ tile_counter,
-    uint32_t *__restrict__ total_out, uint32_t num_tiles, uint64_t capacity) {
-  constexpr int kTile = BT * ITEMS;
-  static_assert(kTile <= (int)kDescIdx, "descriptor indices are 15 bits");
-  static_assert(ITEMS <= 32, "emit mask is 32 bits");
-  __shared__ LK s_key[kTile + 1];
-  __shared__ uint32_t s_desc[kTile];
-  __shared__ uint32_t s_tile, s_i0, s_i1;
-  __shared__ uint64_t s_prev_a;
-  __shared__ uint32_t s_has_prev;
-  __shared__ uint32_t s_scan[BT / kWave];
-  __shared__ uint32_t s_excl;
+// A tile keeps its keys in LDS as 32-bit words.  When the keys of both slices
+// span less than kNoMatch -- the usual case: a tile covers BT*ITEMS
+// consecutive merged keys -- each word is key - base, base being the tile's
+// smallest key, and kNoMatch stands for a key outside the tile that equals
+// none of its keys.  Otherwise (a wide tile) the low words go to s_lo and the
+// high words to s_hi, the descriptor array, which is free until the merge and
+// the count scan are done.  Either way the choice is the tile's own: no host
+// round trip decides it.
+constexpr uint32_t kNoMatch = 0xFFFFFFFFu;
 
-  const uint32_t tid = threadIdx.x;
-  const uint64_t total = (uint64_t)na + nb;
-  // Tiles are numbered in the order workgroups start (a global counter), so a
-  // tile's look-back only waits on tiles that are already running.
-  if (tid == 0) {
-    const uint32_t t = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_tile = t;
-    s_i0 = splits[t];
-    s_i1 = splits[t + 1];
-  }
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t i0 = s_i0, i1 = s_i1;
-  const uint32_t d0 = (uint32_t)min<uint64_t>((uint64_t)tile * kTile, total);
-  const uint32_t d1 = (uint32_t)min<uint64_t>((uint64_t)d0 + kTile, total);
-  const uint32_t j0 = d0 - i0, j1 = d1 - i1;
-  const uint32_t la = i1 - i0, lb = j1 - j0;
-  {
-    uint64_t lk[ITEMS];
-    load_tile_keys<BT, ITEMS>(ak, bk, i0, la, j0, lb, lk);
-#pragma unroll
-    for (int u = 0; u < ITEMS; ++u) {
-      const uint32_t x = (uint32_t)u * BT + tid;
-      if (x < la + lb) s_key[x] = (LK)lk[u];
-    }
-  }
-  const bool has_next_b = j1 < nb;
-  if (tid == 0) {
-    if (has_next_b) s_key[la + lb] = (LK)bk[j1];  // the B key after the slice
-    s_has_prev = i0 > 0;
-    s_prev_a = i0 > 0 ? ak[i0 - 1] : 0;
-  }
-  __syncthreads();
-  const LK *sa = s_key, *sb = s_key + la;
-  const uint32_t nlb = lb + (has_next_b ? 1 : 0);  // B keys visible to peeks
-  const bool hadamard = op == LNT_OP_HADAMARD;
-
-  // Per-thread sub-path.
-  const uint32_t t0 = min<uint32_t>(tid * ITEMS, la + lb);
-  const uint32_t t1 = min<uint32_t>(t0 + ITEMS, la + lb);
+// Sequential merge of the steps [t0, t1) of the tile, one thread.  Slot s of
+// desc gets step t0 + s's descriptor; the returned mask marks the slots that
+// produce an output.  prev is the A key before the tile (as a tile word in a
+// narrow tile), used only if has_prev.
+template <int ITEMS, bool WIDE>
+__device__ inline uint32_t merge_steps(const uint32_t *__restrict__ s_lo, const uint32_t *__restrict__ s_hi,
+                                       uint32_t la, uint32_t lb, uint32_t nlb, uint32_t t0, uint32_t t1,
+                                       bool has_prev, uint64_t prev, bool hadamard, uint32_t (&desc)[ITEMS]) {
+  using K = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+  auto key = [&](uint32_t x) -> K {
+    if constexpr (WIDE) return ((uint64_t)s_hi[x] << 32) | s_lo[x];
+    else return s_lo[x];
+  };
+  // Per-thread sub-path: the B slice starts at word la.
   uint32_t ia, jb;
   {
     uint32_t lo = t0 > lb ? t0 - lb : 0, hi = t0 < la ? t0 : la;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (sa[mid] <= sb[t0 - 1 - mid]) lo = mid + 1;
+      if (key(mid) <= key(la + t0 - 1 - mid)) lo = mid + 1;
       else hi = mid;
     }
     ia = lo;
     jb = t0 - lo;
   }
-  uint64_t ka = ia < la ? sa[ia] : 0;
-  uint64_t kb = jb < nlb ? sb[jb] : 0;
-  bool has_last = ia > 0 || s_has_prev;  // an A key precedes this sub-path
-  uint64_t last_a = ia > 0 ? sa[ia - 1] : s_prev_a;
-  // Step s fills slot s; emit marks the slots that produce an output.
-  uint32_t desc[ITEMS];
+  K ka = ia < la ? key(ia) : 0;
+  K kb = jb < nlb ? key(la + jb) : 0;
+  bool has_last = ia > 0 || has_prev;  // an A key precedes this sub-path
+  K last_a = ia > 0 ? key(ia - 1) : (K)prev;
   uint32_t emit = 0;
 #pragma unroll
   for (int s = 0; s < ITEMS; ++s) {
@@ -184,17 +147,100 @@ __global__ __launch_bounds__(BT) void merge_kernel(
         emit |= (uint32_t)(!absorbed && !hadamard) << s;
         ++jb;
       }
-      // One LDS read per step: the next key of the side just consumed.  The
+      // One key read per step: the next key of the side just consumed.  The
       // index never exceeds la + lb (a B step happens only while jb < lb), so
-      // it stays inside s_key; a key read past its slice is never used (the
-      // ia < la and jb < nlb tests guard every use).
-      const uint64_t nk = s_key[take_a ? ia : la + jb];
+      // it stays inside the arrays; a key read past its slice is never used
+      // (the ia < la and jb < nlb tests guard every use).
+      const K nk = key(take_a ? ia : la + jb);
       if (take_a) ka = nk;
       else kb = nk;
     }
   }
+  return emit;
+}
+
+// force_wide: every tile takes the 64-bit path (a test and tuning switch).
+template <int BT, int ITEMS>
+__global__ __launch_bounds__(BT) void merge_kernel(
+    const uint64_t *__restrict__ ak, const double *__restrict__ av, uint32_t na,
+    const uint64_t *__restrict__ bk, const double *__restrict__ bv, uint32_t nb,
+    int op, uint64_t *__restrict__ ck, double *__restrict__ cv,
+    const uint32_t *__restrict__ splits, uint64_t *__restrict__ status,
+    uint32_t epoch, uint32_t *__restrict__ tile_counter,
+    uint32_t *__restrict__ total_out, uint32_t num_tiles, uint64_t capacity, int force_wide) {
+  constexpr int kTile = BT * ITEMS;
+  static_assert(kTile <= (int)kDescIdx, "descriptor indices are 15 bits");
+  static_assert(ITEMS <= 32, "emit mask is 32 bits");
+  __shared__ uint32_t s_key[kTile + 1];   // key words (low words of a wide tile)
+  __shared__ uint32_t s_desc[kTile + 1];  // high words of a wide tile, then descriptors
+  __shared__ uint32_t s_tile, s_i0, s_i1;
+  __shared__ uint32_t s_scan[BT / kWave];
+  __shared__ uint32_t s_excl;
+
+  const uint32_t tid = threadIdx.x;
+  const uint64_t total = (uint64_t)na + nb;
+  // Tiles are numbered in the order workgroups start (a global counter), so a
+  // tile's look-back only waits on tiles that are already running.
+  if (tid == 0) {
+    const uint32_t t = __hip_atomic_fetch_add(tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_tile = t;
+    s_i0 = splits[t];
+    s_i1 = splits[t + 1];
+  }
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t i0 = s_i0, i1 = s_i1;
+  const uint32_t d0 = (uint32_t)min<uint64_t>((uint64_t)tile * kTile, total);
+  const uint32_t d1 = (uint32_t)min<uint64_t>((uint64_t)d0 + kTile, total);
+  const uint32_t j0 = d0 - i0, j1 = d1 - i1;
+  const uint32_t la = i1 - i0, lb = j1 - j0;  // la + lb >= 1: a tile holds a step
+  const bool has_next_b = j1 < nb;
+  // The tile's smallest and largest keys are the first and last ones of its
+  // slices; the B key after the slice and the A key before it (read by every
+  // thread, alongside the slice loads: the same cache lines).  The next B key
+  // is at least the base (it follows every step of the tile) and the previous
+  // A key at most the base (it precedes every step).
+  const uint64_t lo_key = la ? (lb ? min(ak[i0], bk[j0]) : ak[i0]) : bk[j0];
+  const uint64_t hi_key = la ? (lb ? max(ak[i1 - 1], bk[j1 - 1]) : ak[i1 - 1]) : bk[j1 - 1];
+  const uint64_t next_b = has_next_b ? bk[j1] : 0;
+  const uint64_t prev_a = i0 > 0 ? ak[i0 - 1] : 0;
+  uint64_t lk[ITEMS];
+  load_tile_keys<BT, ITEMS>(ak, bk, i0, la, j0, lb, lk);
+  const bool wide = force_wide || hi_key - lo_key >= kNoMatch;
+  if (wide) {
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      const uint32_t x = (uint32_t)u * BT + tid;
+      if (x < la + lb) {
+        s_key[x] = (uint32_t)lk[u];
+        s_desc[x] = (uint32_t)(lk[u] >> 32);
+      }
+    }
+    if (tid == 0 && has_next_b) {
+      s_key[la + lb] = (uint32_t)next_b;
+      s_desc[la + lb] = (uint32_t)(next_b >> 32);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      const uint32_t x = (uint32_t)u * BT + tid;
+      if (x < la + lb) s_key[x] = (uint32_t)(lk[u] - lo_key);
+    }
+    if (tid == 0 && has_next_b) s_key[la + lb] = (uint32_t)min<uint64_t>(next_b - lo_key, kNoMatch);
+  }
+  __syncthreads();
+  const uint32_t nlb = lb + (has_next_b ? 1 : 0);  // B keys visible to peeks
+  const bool hadamard = op == LNT_OP_HADAMARD;
+  const uint32_t t0 = min<uint32_t>(tid * ITEMS, la + lb);
+  const uint32_t t1 = min<uint32_t>(t0 + ITEMS, la + lb);
+  uint32_t desc[ITEMS];
+  const uint32_t emit =
+      wide ? merge_steps<ITEMS, true>(s_key, s_desc, la, lb, nlb, t0, t1, i0 > 0, prev_a, hadamard, desc)
+           : merge_steps<ITEMS, false>(s_key, s_desc, la, lb, nlb, t0, t1, i0 > 0,
+                                       prev_a == lo_key ? 0u : kNoMatch, hadamard, desc);
   const uint32_t cnt = (uint32_t)__popc(emit);
   uint32_t tile_count;
+  // The scan's barriers also end every thread's reads of s_desc as high words.
   const uint32_t my_off = block_exclusive_sum_bt<BT>(cnt, s_scan, &tile_count);
 
   // The tile's count is published at once; the look-back for its output
@@ -241,7 +287,6 @@ __global__ __launch_bounds__(BT) void merge_kernel(
     if (x < lim) {
       const uint32_t d = dd[u];
       const bool has_a = (d & kDescA) != 0, has_b = (d & kDescB) != 0;
-      const uint64_t key = (uint64_t)s_key[has_a ? (d & kDescIdx) : la + ((d >> 15) & kDescIdx)];
       const double a = va[u], b = vb[u];
       double v;
       if (has_a && has_b)
@@ -250,13 +295,31 @@ __global__ __launch_bounds__(BT) void merge_kernel(
         v = a;
       else
         v = op == LNT_OP_SUB ? 0.0 - b : 0.0 + b;
-      ck[base + x] = key;
+      if (!wide) ck[base + x] = lo_key + s_key[has_a ? (d & kDescIdx) : la + ((d >> 15) & kDescIdx)];
       cv[base + x] = v;
+    }
+  }
+  if (wide) {
+    // A wide tile's high words are gone from LDS: its keys are read again
+    // from the slices (lines the tile has just read), after the value stores
+    // so that the loads take the registers the values held.
+    const uint64_t *tak = ak + i0, *tbk = bk + j0;
+    uint64_t kk[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      const uint32_t x = (uint32_t)u * BT + tid;
+      const uint32_t d = dd[u];
+      kk[u] = x < lim ? ((d & kDescA) ? tak[d & kDescIdx] : tbk[(d >> 15) & kDescIdx]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      const uint32_t x = (uint32_t)u * BT + tid;
+      if (x < lim) ck[base + x] = kk[u];
     }
   }
 }
 
-template <int BT, int ITEMS, typename LK>
+template <int BT, int ITEMS>
 int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
                  uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
                  int op, uint64_t *ck, double *cv, uint64_t capacity,
@@ -275,9 +338,9 @@ int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
   prof_end(ctx, pslot);
   LNT_HIP(hipGetLastError());
   *slot = prof_begin(ctx, kProfMerge, 16.0 * ((double)na + nb));
-  merge_kernel<BT, ITEMS, LK><<<tiles, BT, 0, ctx->stream>>>(
+  merge_kernel<BT, ITEMS><<<tiles, BT, 0, ctx->stream>>>(
       ak, av, na, bk, bv, nb, op, ck, cv, splits, (uint64_t *)ctx->status.ptr,
-      epoch, counter, total, tiles, capacity);
+      epoch, counter, total, tiles, capacity, ctx->tuning.narrow_keys ? 0 : 1);
   prof_end(ctx, *slot);
   LNT_HIP(hipGetLastError());
   return LNT_OK;
@@ -359,41 +422,21 @@ int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
   uint32_t *small = (uint32_t *)ctx->small.ptr;
   uint32_t *counter = small + kSmallCounters + 15;
   uint32_t *total = small + kSmallFlags + 1;
-  // The largest keys are the last ones of the two sorted inputs: when both
-  // are below 2^32 the tile keeps 32-bit keys in LDS.
-  bool narrow = false;
-  if (ctx->tuning.narrow_keys) {
-    uint64_t *last = (uint64_t *)(ctx->pinned + 16);
-    last[0] = last[1] = 0;
-    if (na) LNT_HIP(hipMemcpyAsync(last, ak + na - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-    if (nb) LNT_HIP(hipMemcpyAsync(last + 1, bk + nb - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-    LNT_HIP(hipStreamSynchronize(ctx->stream));
-    narrow = std::max(last[0], last[1]) < (uint64_t(1) << 32);
-  }
   int rc, slot = -1;
-#define LNT_MERGE(BT, IT, LK) \
-  launch_merge<BT, IT, LK>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot)
-  // Measured on 2 x 50M-key merges: 512 x 16 with 32-bit LDS keys (two tiles
-  // per CU), 512 x 12 with 64-bit ones.
-  const int items = ctx->tuning.merge_items ? ctx->tuning.merge_items : (narrow ? 16 : 12);
-  switch (ctx->tuning.merge_block * 100 + items + (narrow ? 1000000 : 0)) {
-    case 25608: rc = LNT_MERGE(256, 8, uint64_t); break;
-    case 25612: rc = LNT_MERGE(256, 12, uint64_t); break;
-    case 25616: rc = LNT_MERGE(256, 16, uint64_t); break;
-    case 51208: rc = LNT_MERGE(512, 8, uint64_t); break;
-    case 51216: rc = LNT_MERGE(512, 16, uint64_t); break;
-    case 102408: rc = LNT_MERGE(1024, 8, uint64_t); break;
-    case 102412: rc = LNT_MERGE(1024, 12, uint64_t); break;
-    case 1025608: rc = LNT_MERGE(256, 8, uint32_t); break;
-    case 1025612: rc = LNT_MERGE(256, 12, uint32_t); break;
-    case 1025616: rc = LNT_MERGE(256, 16, uint32_t); break;
-    case 1051208: rc = LNT_MERGE(512, 8, uint32_t); break;
-    case 1051212: rc = LNT_MERGE(512, 12, uint32_t); break;
-    case 1051216: rc = LNT_MERGE(512, 16, uint32_t); break;
-    case 1102408: rc = LNT_MERGE(1024, 8, uint32_t); break;
-    case 1102412: rc = LNT_MERGE(1024, 12, uint32_t); break;
-    default:  // 512x12; also 1024x16, whose tile does not fit in LDS
-      rc = narrow ? LNT_MERGE(512, 12, uint32_t) : LNT_MERGE(512, 12, uint64_t);
+#define LNT_MERGE(BT, IT) \
+  launch_merge<BT, IT>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot)
+  // Measured on 2 x 50M-key merges: 512 x 16 (two tiles per CU).
+  const int items = ctx->tuning.merge_items ? ctx->tuning.merge_items : 16;
+  switch (ctx->tuning.merge_block * 100 + items) {
+    case 25608: rc = LNT_MERGE(256, 8); break;
+    case 25612: rc = LNT_MERGE(256, 12); break;
+    case 25616: rc = LNT_MERGE(256, 16); break;
+    case 51208: rc = LNT_MERGE(512, 8); break;
+    case 51212: rc = LNT_MERGE(512, 12); break;
+    case 102408: rc = LNT_MERGE(1024, 8); break;
+    case 102412: rc = LNT_MERGE(1024, 12); break;
+    case 102416: rc = LNT_MERGE(1024, 16); break;
+    default: rc = LNT_MERGE(512, 16);
   }
 #undef LNT_MERGE
   if (rc != LNT_OK) return rc;

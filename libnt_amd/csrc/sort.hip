@@ -52,12 +52,16 @@ template <bool RELIN>
 __global__ __launch_bounds__(kBlock) void histogram_kernel(
     const uint64_t *__restrict__ keys, uint32_t n, DigitPlan plan, uint64_t bound,
     const RelinDev *__restrict__ relin, uint32_t *__restrict__ part, uint32_t *__restrict__ counters,
-    uint32_t *__restrict__ err) {
+    uint32_t *__restrict__ hist, uint32_t *__restrict__ err) {
   const int passes = plan.passes;
   __shared__ uint32_t h[kMaxPasses * kRadix];
   const uint32_t tid = threadIdx.x;
   for (int i = tid; i < passes * kRadix; i += kBlock) h[i] = 0;
-  if (blockIdx.x == 0 && tid < 16) counters[tid] = 0;
+  if (blockIdx.x == 0) {
+    if (tid < 16) counters[tid] = 0;
+    // histogram_reduce_kernel adds its row groups' sums into hist.
+    for (int i = tid; i < passes * kRadix; i += kBlock) hist[i] = 0;
+  }
   __syncthreads();
 
   constexpr int kUnroll = 8;
@@ -132,30 +136,37 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
   for (int i = tid; i < passes * kRadix; i += kBlock) mine[i] = h[i];
 }
 
-// hist[i] = sum over blocks of part[b][i].  A block covers 64 consecutive
-// (pass, digit) slots; its four waves each sum every fourth partial row with
-// coalesced, unrolled loads, then add up in LDS.
+// hist[i] = sum over blocks of part[b][i].  Workgroup (x, y) covers 64
+// consecutive (pass, digit) slots of rows [y*rows, (y+1)*rows); its four waves
+// each sum every fourth row of that range with coalesced, unrolled loads, add
+// up in LDS, and wave 0 adds the group's sums into hist (zeroed by
+// histogram_kernel).  Splitting the rows over gridDim.y keeps each wave at one
+// or two rounds of loads instead of a long serial walk over all rows.
 __global__ __launch_bounds__(256) void histogram_reduce_kernel(const uint32_t *__restrict__ part,
                                                                uint32_t blocks, uint32_t width,
+                                                               uint32_t rows,
                                                                uint32_t *__restrict__ hist) {
   __shared__ uint32_t s_sum[4][64];
   const uint32_t lane = threadIdx.x & 63, group = threadIdx.x >> 6;
   const uint32_t slot = blockIdx.x * 64 + lane;
+  const uint32_t r0 = blockIdx.y * rows;
+  const uint32_t r1 = min(blocks, r0 + rows);
   uint32_t acc = 0;
   if (slot < width) {
-    uint32_t b = group;
-    for (; b + 12 < blocks; b += 16) {
+    uint32_t b = r0 + group;
+    for (; b + 12 < r1; b += 16) {
       uint32_t x0 = part[(size_t)b * width + slot];
       uint32_t x1 = part[(size_t)(b + 4) * width + slot];
       uint32_t x2 = part[(size_t)(b + 8) * width + slot];
       uint32_t x3 = part[(size_t)(b + 12) * width + slot];
       acc += x0 + x1 + x2 + x3;
     }
-    for (; b < blocks; b += 4) acc += part[(size_t)b * width + slot];
+    for (; b < r1; b += 4) acc += part[(size_t)b * width + slot];
   }
   s_sum[group][lane] = acc;
   __syncthreads();
-  if (group == 0 && slot < width) hist[slot] = s_sum[0][lane] + s_sum[1][lane] + s_sum[2][lane] + s_sum[3][lane];
+  if (group == 0 && slot < width)
+    atomicAdd(&hist[slot], s_sum[0][lane] + s_sum[1][lane] + s_sum[2][lane] + s_sum[3][lane]);
 }
 
 // -------------------------------------------------------------- onesweep ---
@@ -501,13 +512,15 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   uint32_t *part = (uint32_t *)ctx->hist_part.ptr;
   int slot = prof_begin(ctx, kProfHistogram, 8.0 * n);
   if (do_relin)
-    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, rdev, part, counters, err);
+    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, rdev, part, counters, hist, err);
   else
-    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, rdev, part, counters, err);
+    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, rdev, part, counters, hist, err);
   prof_end(ctx, slot);
   LNT_HIP(hipGetLastError());
   if (passes > 0) {
-    histogram_reduce_kernel<<<(width + 63) / 64, 256, 0, ctx->stream>>>(part, hblocks, width, hist);
+    constexpr uint32_t kRows = 64;  // partial rows per reduce workgroup
+    histogram_reduce_kernel<<<dim3((width + 63) / 64, (hblocks + kRows - 1) / kRows), 256, 0, ctx->stream>>>(
+        part, hblocks, width, kRows, hist);
     LNT_HIP(hipGetLastError());
   }
 

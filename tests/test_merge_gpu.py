@@ -69,13 +69,13 @@ def test_merge_dense_interleaving(ctx, orc):
 
 @pytest.mark.parametrize("narrow", [1, 0])
 @pytest.mark.parametrize("block,items", [(256, 8), (256, 12), (256, 16), (512, 8), (512, 12), (512, 16),
-                                         (1024, 8), (1024, 12)])
+                                         (1024, 8), (1024, 12), (1024, 16)])
 def test_merge_tile_shapes(ctx, orc, block, items, narrow):
-    """Every merge tile shape, with 32-bit LDS keys allowed or not, gives the
-    oracle's result: dense interleaving (equal keys on every tile and thread
-    boundary), long runs of one operand, a tile-sized operand against a single
-    key, and keys on both sides of 2^32 (largest key 2^32 - 1: 32-bit LDS
-    keys; 2^32: 64-bit)."""
+    """Every merge tile shape, with tile-relative 32-bit LDS keys allowed or
+    every tile forced onto the 64-bit path, gives the oracle's result: dense
+    interleaving (equal keys on every tile and thread boundary), long runs of
+    one operand, a tile-sized operand against a single key, and keys on both
+    sides of 2^32."""
     print("merge tile %dx%d narrow %d" % (block, items, narrow), flush=True)
     ctx.set_param("merge_block", block)
     ctx.set_param("merge_items", items)
@@ -103,6 +103,27 @@ def test_merge_tile_shapes(ctx, orc, block, items, narrow):
         ctx.set_param("merge_block", 512)
         ctx.set_param("merge_items", 0)
         ctx.set_param("narrow_keys", 1)
+
+
+def test_merge_mixed_tile_widths(ctx, orc):
+    """One merge whose tiles are partly narrow (keys within 2^32 - 1 of the
+    tile's smallest) and partly wide (keys spread over 2^62), with the
+    switches between them on tile and thread boundaries, equal keys on both
+    sides, and the B key after a tile far above it."""
+    rng = np.random.default_rng(77)
+    parts = [np.arange(0, 30000, dtype=np.int64),                            # dense, narrow
+             rng.integers(1 << 40, 1 << 62, 20000, dtype=np.int64),          # sparse, wide
+             (1 << 62) + np.arange(0, 40000, 3, dtype=np.int64),             # dense again
+             np.array([(1 << 62) + (1 << 33), (1 << 63) - 1], dtype=np.int64)]  # far keys
+    dom = np.unique(np.concatenate(parts))
+    where = rng.integers(0, 3, len(dom))
+    ak, bk = dom[where != 1], dom[where != 0]
+    _check(ctx, orc, ak, rng.uniform(-1, 1, len(ak)), bk, rng.uniform(-1, 1, len(bk)))
+    _check(ctx, orc, bk, rng.uniform(-1, 1, len(bk)), ak[: len(ak) // 2], rng.uniform(-1, 1, len(ak) // 2))
+    # a single far B key after a narrow A run: the peeked key is no match
+    ak = np.arange(0, 9000, dtype=np.int64)
+    bk = np.array([5, 8999, (1 << 32) + 5, (1 << 32) + 8999], dtype=np.int64)
+    _check(ctx, orc, ak, rng.uniform(-1, 1, len(ak)), bk, rng.uniform(-1, 1, len(bk)))
 
 
 def test_merge_signed_zeros(ctx, orc):
