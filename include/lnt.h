@@ -94,14 +94,13 @@ int lnt_abi_version(void);
  *   "digit_bits"   widest radix digit per onesweep pass, 4..8 (default 8)
  *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096
  *   "merge_items"  merge-path steps per thread: 8, 12 or 16 (tile =
- *                  merge_block*items); 0 (default): 16 when both inputs'
- *                  keys are below 2^32, else 12
+ *                  merge_block*items); 0 (default): 16
  *   "merge_block"  merge threads per workgroup: 256, 512 (default) or 1024
- *                  (1024 takes at most 12 items per thread)
  *   "spgemm_small" longest output column (products) kept in LDS, 2..2048
- *   "narrow_keys"  1: keys of at most 32 bits move as 32-bit words between
- *                  radix passes and sit in LDS as 32-bit words during merges
- *                  (default); 0: always 64-bit
+ *   "narrow_keys"  1 (default): keys of at most 32 bits move as 32-bit words
+ *                  between radix passes, and a merge tile whose keys span
+ *                  less than 2^32 - 1 holds them in LDS as 32-bit offsets from
+ *                  its smallest key; 0: always 64-bit
  *   "row_ranks"    1 (default): the lattice product's long output columns sort
  *                  the ranks of A's rows among the rows present instead of
  *                  the rows when that saves a radix pass or gives 32-bit keys

@@ -231,6 +231,7 @@ int lnt_context_destroy(lnt_context ctx) {
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   prof_clear(ctx);
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->check_event) (void)hipEventDestroy(ctx->check_event);
   delete ctx;
   return LNT_OK;
 }

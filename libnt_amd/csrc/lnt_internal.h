@@ -461,6 +461,7 @@ struct lnt_context_s {
   int profiling = 0;
   std::vector<lnt::ProfRecord> prof;   // recorded launches (events owned)
   std::vector<hipEvent_t> event_pool;  // recycled events
+  hipEvent_t check_event = nullptr;    // marks the sort's key-range flag copy
 };
 
 namespace lnt {

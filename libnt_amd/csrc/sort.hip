@@ -179,8 +179,13 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void on
     KOut *__restrict__ kout, double *__restrict__ vout, uint32_t n,
     uint32_t shift, uint32_t dbits, const uint32_t *__restrict__ pass_hist,
     uint64_t *__restrict__ status, uint32_t epoch,
-    uint32_t *__restrict__ tile_counter, const RelinDev *__restrict__ relin) {
+    uint32_t *__restrict__ tile_counter, const RelinDev *__restrict__ relin,
+    const uint32_t *__restrict__ stop) {
   static_assert(BT >= kRadix && BT % kWave == 0, "one thread per digit at least");
+  // stop: the histogram pass's key-range flag when the host checks it.  The
+  // flag is final before this pass starts, so every workgroup of every pass
+  // makes the same choice: a rejected input is left as it was.
+  if (stop != nullptr && *stop != 0) return;
   constexpr int kTile = BT * ITEMS;
   constexpr int kWaves = BT / kWave;
   __shared__ uint32_t s_tile;
@@ -356,16 +361,16 @@ enum KeyWidth { kW64 = 0, kW32 = 1 };
 template <int BT, int ITEMS, bool RELIN, typename KIn, typename KOut>
 void launch_one(lnt_context ctx, uint32_t tiles, const void *kin, const double *vin, void *kout, double *vout,
                 uint32_t n, uint32_t shift, uint32_t dbits, const uint32_t *pass_hist, uint64_t *status,
-                uint32_t epoch, uint32_t *counter, const RelinDev *r) {
+                uint32_t epoch, uint32_t *counter, const RelinDev *r, const uint32_t *stop) {
   onesweep_kernel<BT, ITEMS, RELIN, KIn, KOut><<<tiles, BT, 0, ctx->stream>>>(
-      (const KIn *)kin, vin, (KOut *)kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r);
+      (const KIn *)kin, vin, (KOut *)kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop);
 }
 
 template <int BT, int ITEMS>
 int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
                 const double *vin, void *kout, double *vout, uint32_t n,
                 uint32_t shift, uint32_t dbits, const uint32_t *pass_hist,
-                uint32_t *counter, const RelinDev *r) {
+                uint32_t *counter, const RelinDev *r, const uint32_t *stop) {
   constexpr uint32_t tile = BT * ITEMS;
   uint32_t tiles = (n + tile - 1) / tile;
   LNT_TRY(ensure_status(ctx, (size_t)tiles * kRadix));
@@ -384,12 +389,12 @@ int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
   int slot = prof_begin(ctx, kProfOnesweep, 32.0 * n);
   const int sel = (relin ? 4 : 0) + win * 2 + wout;
   switch (sel) {
-    case 0: launch_one<BT, ITEMS, false, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 1: launch_one<BT, ITEMS, false, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 2: launch_one<BT, ITEMS, false, uint32_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 3: launch_one<BT, ITEMS, false, uint32_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 4: launch_one<BT, ITEMS, true, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
-    case 5: launch_one<BT, ITEMS, true, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r); break;
+    case 0: launch_one<BT, ITEMS, false, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
+    case 1: launch_one<BT, ITEMS, false, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
+    case 2: launch_one<BT, ITEMS, false, uint32_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
+    case 3: launch_one<BT, ITEMS, false, uint32_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
+    case 4: launch_one<BT, ITEMS, true, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
+    case 5: launch_one<BT, ITEMS, true, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
     default: return fail(LNT_ERR_INVALID_ARGUMENT, "internal: unsupported key widths");
   }
   prof_end(ctx, slot);
@@ -524,24 +529,44 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
     LNT_HIP(hipGetLastError());
   }
 
-  if (ctx->validate && !ctx->capturing) {
-    LNT_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    LNT_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->pinned[0])
-      return fail(LNT_ERR_KEY_RANGE, "a key is negative or not below the dimensionality");
+  if (passes > 0) {  // ping-pong buffers, allocated before the flag copy is queued
+    LNT_TRY(ensure(ctx->alt_keys, (size_t)n * 8));
+    LNT_TRY(ensure(ctx->alt_vals, (size_t)n * 8));
+    if (passes & 1) {
+      LNT_TRY(ensure(ctx->alt2_keys, (size_t)n * 8));
+      LNT_TRY(ensure(ctx->alt2_vals, (size_t)n * 8));
+    }
   }
+  // Key-range check: the flag is copied out and, when there are passes, they
+  // are queued before the host waits for the copy (an event, not the stream),
+  // so the GPU goes on with them meanwhile; on a bad key every pass returns at
+  // once (its stop argument) and the arrays are left as they were.
+  const bool check = ctx->validate && !ctx->capturing;
+  if (check) {
+    LNT_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (passes == 0) {
+      LNT_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+      if (!ctx->check_event) LNT_HIP(hipEventCreateWithFlags(&ctx->check_event, hipEventDisableTiming));
+      LNT_HIP(hipEventRecord(ctx->check_event, ctx->stream));
+    }
+  }
+  auto key_range_ok = [&]() -> int {
+    if (check) {
+      if (passes > 0) LNT_HIP(hipEventSynchronize(ctx->check_event));
+      if (ctx->pinned[0]) return fail(LNT_ERR_KEY_RANGE, "a key is negative or not below the dimensionality");
+    }
+    return LNT_OK;
+  };
   if (passes == 0) {
+    LNT_TRY(key_range_ok());
     if (do_relin) LNT_TRY(relin_device(ctx, keys, n, r));
     return LNT_OK;
   }
 
-  LNT_TRY(ensure(ctx->alt_keys, (size_t)n * 8));
-  LNT_TRY(ensure(ctx->alt_vals, (size_t)n * 8));
   uint64_t *bufk[3] = {keys, (uint64_t *)ctx->alt_keys.ptr, nullptr};
   double *bufv[3] = {vals, (double *)ctx->alt_vals.ptr, nullptr};
   if (passes & 1) {
-    LNT_TRY(ensure(ctx->alt2_keys, (size_t)n * 8));
-    LNT_TRY(ensure(ctx->alt2_vals, (size_t)n * 8));
     bufk[2] = (uint64_t *)ctx->alt2_keys.ptr;
     bufv[2] = (double *)ctx->alt2_vals.ptr;
   }
@@ -564,7 +589,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
                                                     : ctx->tuning.items_per_thread;
 #define LNT_PASS(BT, IT)                                                                                     \
   launch_pass<BT, IT>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p], \
-                      plan.bits[p], hist + p * kRadix, counters + p, rdev)
+                      plan.bits[p], hist + p * kRadix, counters + p, rdev, check ? err : nullptr)
     switch (cfg) {
       case 8: rc = LNT_PASS(256, 8); break;
       case 12: rc = LNT_PASS(256, 12); break;
@@ -577,10 +602,13 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
       default: rc = LNT_PASS(256, 16); break;
     }
 #undef LNT_PASS
-    if (rc != LNT_OK) return rc;
+    if (rc != LNT_OK) {
+      (void)key_range_ok();  // the copy must land before pinned[0] is reused
+      return rc;
+    }
     src = dst;
   }
-  return LNT_OK;
+  return key_range_ok();
 }
 
 }  // namespace lnt
