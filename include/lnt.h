@@ -92,7 +92,7 @@ int lnt_abi_version(void);
  *   "sort_block"   onesweep threads per workgroup: 256, 512 or 1024 (default;
  *                  1024 takes at most 12 items per thread)
  *   "digit_bits"   widest radix digit per onesweep pass, 4..8 (default 8)
- *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096
+ *   "hist_blocks"  workgroups of the digit-histogram pass, 1..4096 (default 4096)
  *   "merge_items"  merge-path steps per thread: 8, 12 or 16 (tile =
  *                  merge_block*items); 0 (default): 16
  *   "merge_block"  merge threads per workgroup: 256, 512 (default) or 1024

@@ -418,7 +418,7 @@ inline int check_mem(int mem) {
 
 struct SortTuning {
   int items_per_thread = 12;  // onesweep tile = sort_block * items
-  int hist_blocks = 1024;
+  int hist_blocks = 4096;
   int merge_items = 0;    // merge tile = merge_block * items; 0: 16 with 32-bit LDS keys, else 12
   int merge_block = 512;  // merge threads per workgroup (256 or 512)
   int spgemm_small = 2048;  // largest output column handled in LDS
@@ -452,7 +452,7 @@ struct lnt_context_s {
   lnt::DeviceBuffer small;      // histograms, counters, flags
   lnt::DeviceBuffer aux;        // merge-path splits and other per-call scratch
   lnt::DeviceBuffer hist_part;  // per-block digit histograms
-  lnt::DeviceBuffer relin_dev;  // device copy of the current relinearisation table
+  lnt::DeviceBuffer relin_dev;  // relinearisation table of the current sort's radix passes
   lnt::DeviceBuffer spg_ws, spg_tmp, spg_big, spg_rank;  // lattice-product workspaces
   lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
   lnt::DeviceBuffer exec_buf[6];  // operand copies of a whole contraction (keys, values of both), timing output

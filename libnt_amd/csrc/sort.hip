@@ -19,6 +19,7 @@
 //   Even P ping-pongs caller buffer -> scratch -> caller; odd P goes through two
 //   scratch buffers so the result still lands in the caller's arrays.
 #include <algorithm>
+#include <atomic>
 
 #include "lnt_internal.h"
 
@@ -51,13 +52,17 @@ struct DigitPlan {
 template <bool RELIN>
 __global__ __launch_bounds__(kBlock) void histogram_kernel(
     const uint64_t *__restrict__ keys, uint32_t n, DigitPlan plan, uint64_t bound,
-    const RelinDev *__restrict__ relin, uint32_t *__restrict__ part, uint32_t *__restrict__ counters,
-    uint32_t *__restrict__ hist, uint32_t *__restrict__ err) {
+    uint32_t *__restrict__ part, uint32_t *__restrict__ counters,
+    uint32_t *__restrict__ hist, uint32_t *__restrict__ err, uint32_t tag, RelinDev *__restrict__ relin_out,
+    const RelinDev relin) {
   const int passes = plan.passes;
   __shared__ uint32_t h[kMaxPasses * kRadix];
   const uint32_t tid = threadIdx.x;
   for (int i = tid; i < passes * kRadix; i += kBlock) h[i] = 0;
   if (blockIdx.x == 0) {
+    // The radix passes read the table from device memory: this launch, the
+    // first of every relinearising sort, stores it there.
+    if (RELIN && tid == 0) *relin_out = relin;
     if (tid < 16) counters[tid] = 0;
     // histogram_reduce_kernel adds its row groups' sums into hist.
     for (int i = tid; i < passes * kRadix; i += kBlock) hist[i] = 0;
@@ -83,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
       const uint64_t i = base + (uint64_t)u * kBlock + tid;
       bad |= i < n && k[u] >= bound;
     }
-    if (RELIN) relin_keys<kUnroll>(relin, k);
+    if (RELIN) relin_keys<kUnroll>(&relin, k);
     // Consecutive lanes hold consecutive keys; inputs are often sorted or
     // nearly so.  A wave whose lanes all hold the same digit adds once
     // (sorted inputs make such waves common): no same-address LDS atomics
@@ -128,7 +133,9 @@ __global__ __launch_bounds__(kBlock) void histogram_kernel(
     load(k0, b0);
     count(k1, b1);
   }
-  if (bad) atomicOr(err, 1u);
+  // The flag holds the tag of the last sort that saw a bad key: no clearing
+  // launch is needed, and a stale value never equals a later sort's tag.
+  if (bad) __hip_atomic_store(err, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   // Per-block partial histograms (no contended device-wide atomics: the
   // counters of all blocks would otherwise serialise across the XCDs' L2s).
@@ -179,13 +186,14 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void on
     KOut *__restrict__ kout, double *__restrict__ vout, uint32_t n,
     uint32_t shift, uint32_t dbits, const uint32_t *__restrict__ pass_hist,
     uint64_t *__restrict__ status, uint32_t epoch,
-    uint32_t *__restrict__ tile_counter, const RelinDev *__restrict__ relin,
-    const uint32_t *__restrict__ stop) {
+    uint32_t *__restrict__ tile_counter, const uint32_t *__restrict__ stop, uint32_t stop_tag,
+    const RelinDev *__restrict__ relin) {
   static_assert(BT >= kRadix && BT % kWave == 0, "one thread per digit at least");
-  // stop: the histogram pass's key-range flag when the host checks it.  The
-  // flag is final before this pass starts, so every workgroup of every pass
-  // makes the same choice: a rejected input is left as it was.
-  if (stop != nullptr && *stop != 0) return;
+  // stop: the histogram pass's key-range flag when the host checks it (set
+  // to stop_tag on a bad key).  The flag is final before this pass starts, so
+  // every workgroup of every pass makes the same choice: a rejected input is
+  // left as it was.
+  if (stop != nullptr && *stop == stop_tag) return;
   constexpr int kTile = BT * ITEMS;
   constexpr int kWaves = BT / kWave;
   __shared__ uint32_t s_tile;
@@ -333,11 +341,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void on
   }
 }
 
-__global__ void store_relin_kernel(RelinDev r, RelinDev *out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = r;
-}
-
-__global__ void relin_kernel(uint64_t *keys, uint32_t n, const RelinDev *__restrict__ relin) {
+__global__ void relin_kernel(uint64_t *keys, uint32_t n, const RelinDev relin) {
   constexpr int kPer = 4;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; base < n; base += stride * kPer) {
@@ -347,7 +351,7 @@ __global__ void relin_kernel(uint64_t *keys, uint32_t n, const RelinDev *__restr
       uint64_t i = base + (uint64_t)u * stride;
       k[u] = keys[i < n ? i : n - 1];
     }
-    relin_keys<kPer>(relin, k);
+    relin_keys<kPer>(&relin, k);
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       uint64_t i = base + (uint64_t)u * stride;
@@ -361,16 +365,16 @@ enum KeyWidth { kW64 = 0, kW32 = 1 };
 template <int BT, int ITEMS, bool RELIN, typename KIn, typename KOut>
 void launch_one(lnt_context ctx, uint32_t tiles, const void *kin, const double *vin, void *kout, double *vout,
                 uint32_t n, uint32_t shift, uint32_t dbits, const uint32_t *pass_hist, uint64_t *status,
-                uint32_t epoch, uint32_t *counter, const RelinDev *r, const uint32_t *stop) {
+                uint32_t epoch, uint32_t *counter, const RelinDev *r, const uint32_t *stop, uint32_t tag) {
   onesweep_kernel<BT, ITEMS, RELIN, KIn, KOut><<<tiles, BT, 0, ctx->stream>>>(
-      (const KIn *)kin, vin, (KOut *)kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop);
+      (const KIn *)kin, vin, (KOut *)kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, stop, tag, r);
 }
 
 template <int BT, int ITEMS>
 int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
                 const double *vin, void *kout, double *vout, uint32_t n,
                 uint32_t shift, uint32_t dbits, const uint32_t *pass_hist,
-                uint32_t *counter, const RelinDev *r, const uint32_t *stop) {
+                uint32_t *counter, const RelinDev *r, const uint32_t *stop, uint32_t tag) {
   constexpr uint32_t tile = BT * ITEMS;
   uint32_t tiles = (n + tile - 1) / tile;
   LNT_TRY(ensure_status(ctx, (size_t)tiles * kRadix));
@@ -389,12 +393,12 @@ int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
   int slot = prof_begin(ctx, kProfOnesweep, 32.0 * n);
   const int sel = (relin ? 4 : 0) + win * 2 + wout;
   switch (sel) {
-    case 0: launch_one<BT, ITEMS, false, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
-    case 1: launch_one<BT, ITEMS, false, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
-    case 2: launch_one<BT, ITEMS, false, uint32_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
-    case 3: launch_one<BT, ITEMS, false, uint32_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
-    case 4: launch_one<BT, ITEMS, true, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
-    case 5: launch_one<BT, ITEMS, true, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop); break;
+    case 0: launch_one<BT, ITEMS, false, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop, tag); break;
+    case 1: launch_one<BT, ITEMS, false, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop, tag); break;
+    case 2: launch_one<BT, ITEMS, false, uint32_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop, tag); break;
+    case 3: launch_one<BT, ITEMS, false, uint32_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop, tag); break;
+    case 4: launch_one<BT, ITEMS, true, uint64_t, uint64_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop, tag); break;
+    case 5: launch_one<BT, ITEMS, true, uint64_t, uint32_t>(ctx, tiles, kin, vin, kout, vout, n, shift, dbits, pass_hist, status, epoch, counter, r, stop, tag); break;
     default: return fail(LNT_ERR_INVALID_ARGUMENT, "internal: unsupported key widths");
   }
   prof_end(ctx, slot);
@@ -437,23 +441,22 @@ int next_epoch(lnt_context ctx, uint32_t *epoch) {
   return LNT_OK;
 }
 
-// Stores `r` in the context's device table (stream ordered) and returns it.
-static int upload_relin(lnt_context ctx, const Relin &r, const RelinDev **dev) {
-  LNT_TRY(ensure(ctx->relin_dev, sizeof(RelinDev)));
-  store_relin_kernel<<<1, 64, 0, ctx->stream>>>(relin_for_device(r), (RelinDev *)ctx->relin_dev.ptr);
-  LNT_HIP(hipGetLastError());
-  *dev = (const RelinDev *)ctx->relin_dev.ptr;
-  return LNT_OK;
+// Tags of the sorts' key-range flags: process-wide, so that a flag word left
+// by any earlier sort (of any context, in memory since reused) never matches.
+static uint32_t next_check_tag() {
+  static std::atomic<uint32_t> counter{0};
+  uint32_t t;
+  do t = counter.fetch_add(1, std::memory_order_relaxed) + 1;
+  while (t == 0);
+  return t;
 }
 
 int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin) {
   if (n == 0 || !relin.active) return LNT_OK;
-  const RelinDev *rdev = nullptr;
-  LNT_TRY(upload_relin(ctx, relin, &rdev));
   uint32_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   int slot = prof_begin(ctx, kProfRelin, 16.0 * n);
-  relin_kernel<<<blocks, 256, 0, ctx->stream>>>(keys, n, rdev);
+  relin_kernel<<<blocks, 256, 0, ctx->stream>>>(keys, n, relin_for_device(relin));
   prof_end(ctx, slot);
   LNT_HIP(hipGetLastError());
   return LNT_OK;
@@ -480,8 +483,15 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   const bool do_relin = relin && relin->active;
   Relin r{};
   if (do_relin) r = *relin;
-  const RelinDev *rdev = nullptr;
-  if (do_relin) LNT_TRY(upload_relin(ctx, r, &rdev));
+  // The table travels by value as the histogram kernel's last argument
+  // (captured with the launch; last, so that the other arguments share no
+  // cache line with it), which stores it for the radix passes.
+  const RelinDev rtab = do_relin ? relin_for_device(r) : RelinDev{};
+  RelinDev *rdev = nullptr;
+  if (do_relin) {
+    LNT_TRY(ensure(ctx->relin_dev, sizeof(RelinDev)));
+    rdev = (RelinDev *)ctx->relin_dev.ptr;
+  }
   uint32_t bits = 0;
   while (bits < 64 && (uint64_t(1) << bits) < bound) ++bits;
   // Digit plan over the bits above the already ordered low bits:
@@ -507,7 +517,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   uint32_t *hist = small + kSmallHist;
   uint32_t *counters = small + kSmallCounters;
   uint32_t *err = small + kSmallFlags;
-  LNT_TRY(launch_fill_u32(ctx, err, 0, 1));
+  const uint32_t tag = next_check_tag();
 
   uint32_t hblocks = (uint32_t)ctx->tuning.hist_blocks;
   uint32_t need = (n + kBlock * 8 - 1) / (kBlock * 8);
@@ -517,9 +527,9 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   uint32_t *part = (uint32_t *)ctx->hist_part.ptr;
   int slot = prof_begin(ctx, kProfHistogram, 8.0 * n);
   if (do_relin)
-    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, rdev, part, counters, hist, err);
+    histogram_kernel<true><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, part, counters, hist, err, tag, rdev, rtab);
   else
-    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, rdev, part, counters, hist, err);
+    histogram_kernel<false><<<hblocks, kBlock, 0, ctx->stream>>>(keys, n, plan, bound, part, counters, hist, err, tag, rdev, rtab);
   prof_end(ctx, slot);
   LNT_HIP(hipGetLastError());
   if (passes > 0) {
@@ -554,7 +564,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   auto key_range_ok = [&]() -> int {
     if (check) {
       if (passes > 0) LNT_HIP(hipEventSynchronize(ctx->check_event));
-      if (ctx->pinned[0]) return fail(LNT_ERR_KEY_RANGE, "a key is negative or not below the dimensionality");
+      if (ctx->pinned[0] == tag) return fail(LNT_ERR_KEY_RANGE, "a key is negative or not below the dimensionality");
     }
     return LNT_OK;
   };
@@ -589,7 +599,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
                                                     : ctx->tuning.items_per_thread;
 #define LNT_PASS(BT, IT)                                                                                     \
   launch_pass<BT, IT>(ctx, rl, win, wout, bufk[src], bufv[src], bufk[dst], bufv[dst], n, plan.shift[p], \
-                      plan.bits[p], hist + p * kRadix, counters + p, rdev, check ? err : nullptr)
+                      plan.bits[p], hist + p * kRadix, counters + p, rdev, check ? err : nullptr, tag)
     switch (cfg) {
       case 8: rc = LNT_PASS(256, 8); break;
       case 12: rc = LNT_PASS(256, 12); break;
