@@ -608,7 +608,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
       case 116: rc = LNT_PASS(512, 16); break;
       case 208: rc = LNT_PASS(1024, 8); break;
       case 212: rc = LNT_PASS(1024, 12); break;
-      case 216: rc = LNT_PASS(1024, 12); break;  // 1024x16 exceeds the LDS of one CU
+      case 216: rc = LNT_PASS(1024, 12); break;  // 1024x16: 128 VGPRs with spills, slower
       default: rc = LNT_PASS(256, 16); break;
     }
 #undef LNT_PASS
