@@ -353,3 +353,30 @@ def test_sort_configurations_match_oracle(ctx, orc, block, items):
         ctx.set_param("sort_items", 12)
         ctx.set_param("digit_bits", 8)
         ctx.set_param("narrow_keys", 1)
+
+
+@pytest.mark.parametrize("hist_blocks", [1, 63, 64, 65, 1000, 4096])
+def test_histogram_block_counts(ctx, orc, hist_blocks):
+    """The digit histogram's workgroup count (1..4096; the partial rows are
+    reduced in groups of 64, so counts off a multiple of 64 leave a ragged
+    last group) never changes the result: a full sort and a sorted-input
+    change_sort_order, each against the oracle."""
+    print("histogram blocks %d" % hist_blocks, flush=True)
+    ctx.set_param("hist_blocks", hist_blocks)
+    try:
+        n, bound = 700001, (1 << 40) + 7
+        keys = orc.gen_distinct_keys(n, bound, 5)
+        vals = orc.gen_uniform_vals(n, 6)
+        ek, ev = orc.radix_sort_pairs(keys, vals, bound)
+        k, v = gpu(keys), gpu(vals)
+        ctx.sort_pairs(k, v, bound)
+        assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
+        dims, ident, perm = [64, 64, 64, 64], [0, 1, 2, 3], [2, 0, 3, 1]
+        keys = np.sort(orc.gen_distinct_keys(300000, 1 << 24, 7))
+        vals = orc.gen_uniform_vals(300000, 8)
+        ek, ev = orc.sort_seq(keys, vals, dims, ident, perm, True)
+        k, v = gpu(keys), gpu(vals)
+        ctx.change_sort_order(k, v, dims, ident, perm, True)
+        assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
+    finally:
+        ctx.set_param("hist_blocks", 4096)
