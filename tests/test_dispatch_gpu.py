@@ -69,6 +69,24 @@ def test_every_strategy_matches_oracle(ctx, orc):
         assert torch.equal(F.vals.view(torch.int64), ref.vals.view(torch.int64))
 
 
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_random_contractions_every_strategy(ctx, orc, seed):
+    """Wider sweep: 15 instances per seed from the three families (up to 30k
+    entries per operand, operands in every sort state), every strategy
+    against the oracle."""
+    from libnt_amd import dispatch
+
+    for i, sp in enumerate(dispatch.mixed_specs(15, seed=seed, max_nnz=30000)):
+        A, B = dispatch.materialize(ctx, sp, 1000 * seed + i)
+        ek, ev, edims = _oracle(orc, A, sp.la, B, sp.lb)
+        for s in range(dispatch.NUM_STRATEGIES):
+            print("seed", seed, "spec", i, sp.kind, sp.la, sp.lb, sp.nnz_a, sp.nnz_b, "strategy", s, flush=True)
+            C, labels, used = dispatch.contract(ctx, A, sp.la, B, sp.lb, s)
+            assert list(C.dims) == list(edims)
+            assert np.array_equal(host(C.keys), ek), (seed, i, s)
+            assert orc.allclose_fuzzy(host(C.vals), ev, TOL), (seed, i, s)
+
+
 def test_auto_without_model_is_the_heuristic(ctx):
     from libnt_amd import dispatch
 
