@@ -126,6 +126,21 @@ def test_merge_mixed_tile_widths(ctx, orc):
     _check(ctx, orc, ak, rng.uniform(-1, 1, len(ak)), bk, rng.uniform(-1, 1, len(bk)))
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_merge_random_sweep(ctx, orc, seed):
+    """Random sizes (0..300k), overlaps and key domains from 2^10 to 2^62, so
+    that tiles are narrow, wide or mixed, every operation against the oracle."""
+    rng = np.random.default_rng([seed, 7])
+    for trial in range(6):
+        dom = 1 << int(rng.integers(10, 63))
+        na, nb = (int(rng.integers(0, min(300000, dom // 2) + 1)) for _ in range(2))
+        ov = int(rng.integers(0, min(na, nb) + 1))
+        print("merge sweep seed %d trial %d na %d nb %d ov %d dom 2^%d" % (seed, trial, na, nb, ov, dom.bit_length() - 1),
+              flush=True)
+        ak, av, bk, bv = _pair(orc, na, nb, ov, dom, 100 * seed + trial)
+        _check(ctx, orc, ak, av, bk, bv)
+
+
 def test_merge_signed_zeros(ctx, orc):
     ak = np.array([1, 4], dtype=np.int64)
     av = np.array([-0.0, 0.0])

@@ -380,3 +380,28 @@ def test_histogram_block_counts(ctx, orc, hist_blocks):
         assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
     finally:
         ctx.set_param("hist_blocks", 4096)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_change_sort_order_random_sweep(ctx, orc, seed):
+    """Random orders 1-6, dims 1..300 (powers of two or not), random current
+    and new mode sequences, sorted or unsorted input, up to 200k entries."""
+    rng = np.random.default_rng([seed, 99])
+    for trial in range(5):
+        order = int(rng.integers(1, 7))
+        dims = [int(x) for x in rng.choice([1, 2, 3, 7, 16, 31, 64, 100, 128, 256, 300], order)]
+        dom = int(np.prod(dims))
+        n = int(min(dom, rng.integers(1, 200000)))
+        cur = [int(x) for x in rng.permutation(order)]
+        new = [int(x) for x in rng.permutation(order)]
+        is_sorted = bool(rng.integers(0, 2))
+        print("sweep seed %d trial %d dims %s cur %s new %s n %d sorted %d" % (seed, trial, dims, cur, new, n, is_sorted),
+              flush=True)
+        keys = orc.gen_distinct_keys(n, dom, 1000 * seed + trial)
+        if is_sorted:
+            keys = np.sort(keys)
+        vals = orc.gen_uniform_vals(n, 7 + trial)
+        ek, ev = orc.sort_seq(keys, vals, dims, cur, new, is_sorted)
+        k, v = gpu(keys), gpu(vals)
+        ctx.change_sort_order(k, v, dims, cur, new, is_sorted)
+        assert np.array_equal(host(k), ek) and _bits_equal(host(v), ev)
