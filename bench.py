@@ -26,6 +26,9 @@ Further objects on the same line (skip with --no-extras):
     autotuned parameters, against one lnt_contract call per instance with the
     restated polyalgorithm's strategy and with the learned model's strategy;
   learned: the shipped cost model on the committed held-out timing set.
+With --host-buffers, also "host_buffers": the same step with every call handed
+host arrays (pinned, then pageable), staged through HBM by the library, so
+the rate includes the PCIe copies (never the headline value).
 """
 from __future__ import annotations
 
@@ -301,6 +304,45 @@ def bench_learned(ctx):
     return res
 
 
+def bench_host_buffers(ctx, torch, ak, av, bk, bv, reps=4):
+    """The headline step on host arrays: each call copies its operands to HBM
+    and its results back (LNT_MEM_HOST), as a caller holding the reference's
+    host vectors would use the C-ABI."""
+    perm = _perm()
+    orders = [IDENT, perm]
+    n = ak.shape[0] + bk.shape[0]
+    res = {}
+    for mode in ("pinned", "pageable"):
+        if mode == "pinned":
+            h = [t.cpu().pin_memory() for t in (ak, av, bk, bv)]
+            outs = [torch.empty(n, dtype=dt).pin_memory() for dt in (torch.int64, torch.float64)]
+            m = min(ak.shape[0], bk.shape[0])
+            outs += [torch.empty(m, dtype=dt).pin_memory() for dt in (torch.int64, torch.float64)]
+        else:
+            h = [t.cpu().numpy().copy() for t in (ak, av, bk, bv)]
+            outs = [np.empty(n, dtype=dt) for dt in (np.int64, np.float64)]
+            m = min(ak.shape[0], bk.shape[0])
+            outs += [np.empty(m, dtype=dt) for dt in (np.int64, np.float64)]
+
+        def step(s):
+            cur, new = orders[s % 2], orders[(s + 1) % 2]
+            ctx.change_sort_order(h[0], h[1], DIMS, cur, new, True)
+            ctx.change_sort_order(h[2], h[3], DIMS, cur, new, True)
+            ctx.merge_into(h[0], h[1], h[2], h[3], 0, outs[0], outs[1])
+            ctx.merge_into(h[0], h[1], h[2], h[3], 2, outs[2], outs[3])
+
+        step(0)
+        step(1)
+        t = time.perf_counter()
+        for s in range(reps):
+            step(s)
+        el = (time.perf_counter() - t) / reps
+        res[mode] = {"ms_per_step": round(el * 1e3, 1), "keys_per_s": n / el}
+        _log("host buffers (%s): %s" % (mode, res[mode]))
+        del h, outs
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,6 +354,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, run a plain 1.6 GB write (fill) and read (sum) so that "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE can be scaled by kernels of known traffic")
+    ap.add_argument("--host-buffers", action="store_true",
+                    help="also time the step on host arrays (PCIe copies included; not the headline value)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -412,6 +456,8 @@ def main():
         float(cal.sum())
         torch.cuda.synchronize()
         del cal
+    if args.host_buffers:
+        out["host_buffers"] = bench_host_buffers(ctx, torch, ak, av, bk, bv)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scale=args.cpu_scale)
     if not args.no_extras:
