@@ -83,3 +83,30 @@ def test_sparse_dense_front_end(ctx, orc):
     assert np.array_equal(c2.to_dense(), np.einsum("ijk,kj->ikj", da, be))
     with pytest.raises(ValueError):  # contracted index
         a("ijk") * DenseMIA.from_array(bd)("kl")
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_sparse_dense_random_sweep(ctx, orc, seed):
+    """Random dense operands of order 1-3 (each mode an outer or an inter
+    mode of the product, in random order), random outer sizes of A and
+    densities of both operands, against the oracle bit for bit."""
+    rng = np.random.default_rng([seed, 11])
+    for trial in range(6):
+        nbm = int(rng.integers(1, 4))
+        b_dims = tuple(int(x) for x in rng.integers(1, 12, nbm))
+        modes = [int(m) for m in rng.permutation(nbm)]
+        cut = int(rng.integers(0, nbm + 1))
+        r_modes, i_modes = modes[:cut], modes[cut:]
+        L = int(rng.integers(1, 40))
+        T = int(np.prod([b_dims[m] for m in i_modes])) if i_modes else 1
+        nnz = int(rng.integers(1, L * T + 1))
+        print("dense sweep seed %d L %d b_dims %s r %s i %s nnz %d" % (seed, L, b_dims, r_modes, i_modes, nnz), flush=True)
+        ak = np.sort(orc.gen_distinct_keys(nnz, L * T, 10 * seed + trial))
+        av = orc.gen_uniform_vals(len(ak), 4 + trial)
+        av[rng.random(len(av)) < 0.1] = 0.0
+        b = orc.gen_uniform_vals(int(np.prod(b_dims)), 5 + trial)
+        b[rng.random(len(b)) < 0.3] = 0.0
+        ek, ev = orc.sparse_dense_mult(ak, av, L, b.reshape(b_dims, order="F"), b_dims, r_modes, i_modes)
+        ck, cv = ctx.sparse_dense_mult(gpu(ak), gpu(av), L, gpu(b), b_dims, r_modes, i_modes)
+        assert np.array_equal(ck.cpu().numpy(), ek)
+        assert np.array_equal(bits(cv.cpu().numpy()), bits(ev))

@@ -139,3 +139,28 @@ def test_unary_front_end(ctx, orc):
     got = (a("ijiklm") * bb("klmn")).assign("jn").to_dense()
     want = np.einsum("ijiklm,klmn->jn", dense, bb.to_dense())
     assert orc.allclose_fuzzy(np.ravel(got, order="F"), np.ravel(want, order="F"), 1e-12)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_unary_random_sweep(ctx, orc, seed):
+    """Random index strings of order 2-6 over a four-letter alphabet (every
+    repeated letter contracted or, at random, attracted), random sizes and
+    mode sequences, up to 5000 entries, against the oracle bit for bit."""
+    rng = np.random.default_rng([seed, 5])
+    done = 0
+    while done < 6:
+        order = int(rng.integers(2, 7))
+        labels = "".join(rng.choice(list("abcd"), order))
+        if all(labels.count(c) == 1 for c in labels):
+            continue  # no repeated index: not a unary contraction
+        upper = {c for c in set(labels) if labels.count(c) > 1 and rng.random() < 0.5}
+        labels = "".join(c.upper() if c in upper else c for c in labels)
+        size = {c: int(rng.integers(2, 9)) for c in set(labels.lower())}
+        dims = tuple(size[c.lower()] for c in labels)
+        group, kind = unary_groups(labels)
+        seq = [int(x) for x in rng.permutation(order)]
+        n = int(min(np.prod(dims), rng.integers(1, 5000)))
+        print("unary sweep seed %d labels %s dims %s seq %s n %d" % (seed, labels, dims, seq, n), flush=True)
+        k, v = _operand(orc, dims, n, 100 * seed + done, seq)
+        _check(ctx, orc, k, v, dims, seq, group, kind)
+        done += 1
