@@ -14,17 +14,24 @@ Prints ONE JSON line.  value = sorted keys/s of the whole step
 radix pass) -- algorithmic bytes (32 B per key per pass) over its measured
 average launch duration (HIP events on the library's stream during the timed
 steps).  cpu_baseline: the CPU restatement of the reference algorithms
-(oracle/), one core, on a bounded 1/8-scale sample of the same workload.
+(oracle/), one pinned core, one run of the same step on host copies of the
+same inputs.
 
-Further objects on the same line (skip with --no-extras):
+Further objects on the same line (skip with --no-extras; --section runs one):
+  small_contraction: C(i,j,m,n) = A(i,j,k,l)*B(k,l,m,n), dims 32^4 at 1%
+    (10,486 nnz each), every strategy, plus change_sort_order of both
+    operands to a seeded mode permutation; CPU baseline beside it;
   contraction: the large skewed contraction A(i,j,k)*B(k,l,m), A 4096x4096x512
     (858,993 nnz), B 512x4096x64 (13,422 nnz), i and l Zipf(1.1): whole
     contraction (unfold sorts + lattice product + refold) per accumulator,
-    flops/s = 2 * scalar products / time;
+    flops/s = 2 * scalar products / time, roofline over the compulsory bytes
+    16 * (nnz(A) + nnz(B) + nnz(C)), CPU baseline on the same inputs;
   batched: 10,000 independent contractions (order 3-4, 1-2 contracted modes,
     dims log-U[8,128], nnz log-U[100,50k]) as one recorded hipGraph with
-    autotuned parameters, against one lnt_contract call per instance with the
-    restated polyalgorithm's strategy and with the learned model's strategy;
+    autotuned parameters, against one lnt_contract_into call per instance
+    (output into one reused buffer) with the restated polyalgorithm's
+    strategy, the fixed default strategy and the learned model's strategy;
+    roofline over the compulsory bytes; CPU baseline on a sample of instances;
   learned: the shipped cost model on the committed held-out timing set.
 With --host-buffers, also "host_buffers": the same step with every call handed
 host arrays (pinned, then pageable), staged through HBM by the library, so
@@ -82,48 +89,67 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(scale=8, repeats=5):
-    """The oracle (CPU restatement of the reference's algorithms) on one core:
-    the same step on a 1/scale sample.  Returns the JSON object."""
+class _OneCore:
+    """Pins this process to its first allowed core while alive (the
+    reference's sparse path is single-threaded)."""
+
+    def __enter__(self):
+        self.old = set(os.sched_getaffinity(0))
+        os.sched_setaffinity(0, {min(self.old)})
+        return len(self.old)
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.old)
+
+
+def _timed(fn, repeats):
+    """Median wall time of `repeats` calls of fn (seconds)."""
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
+def cpu_baseline(ak0, av0, bk0, bv0, scale=1):
+    """The oracle (CPU restatement of the reference's algorithms) on one
+    pinned core: one run of the benchmark step on host arrays.  scale 1: host
+    copies of the GPU's own (sorted) inputs; scale s > 1: the generator's
+    first 1/s of the workload instead.  Returns the JSON object."""
     import oracle
 
-    cpus = sorted(os.sched_getaffinity(0))
-    old = set(cpus)
-    os.sched_setaffinity(0, {cpus[0]})
-    try:
+    if scale > 1:
         ns, no = N_SHARED // scale, N_ONLY // scale
         keys = oracle.gen_distinct_keys(ns + 2 * no, 1 << 32, SEED_KEYS)
-        ak0 = keys[: ns + no].copy()
-        bk0 = np.concatenate([keys[:ns], keys[ns + no:]])
-        av0 = oracle.gen_uniform_vals(len(ak0), SEED_A)
-        bv0 = oracle.gen_uniform_vals(len(bk0), SEED_B)
-        ak0, av0 = oracle.radix_sort_pairs(ak0, av0, 1 << 32)
-        bk0, bv0 = oracle.radix_sort_pairs(bk0, bv0, 1 << 32)
-        perm = _perm()
-        times = []
-        for _ in range(repeats):
-            ak, av, bk, bv = ak0.copy(), av0.copy(), bk0.copy(), bv0.copy()
-            t0 = time.perf_counter()
-            oracle.sort_seq_inplace(ak, av, DIMS, IDENT, perm, True)
-            oracle.sort_seq_inplace(bk, bv, DIMS, IDENT, perm, True)
-            oracle.merge(ak, av, bk, bv, oracle.OP_ADD)
-            oracle.merge(ak, av, bk, bv, oracle.OP_HADAMARD)
-            times.append(time.perf_counter() - t0)
-        t = float(np.median(times))
-        return {
-            "value": (len(ak0) + len(bk0)) / t,
-            "unit": "keys/s",
-            "cores": 1,
-            "kind": "port",
-            "cpu_model": _cpu_model(),
-            "nproc": len(old),
-            "sample": "1/%d-scale c2 step (%d+%d keys) on one pinned core: 2x change_sort_order (relinearise + "
-                      "in-place MSD radix 2048/11-bit, thresholds 3000/50) + scan merge A+B + Hadamard (the "
-                      "reference's reserve(nA*nB/2^Inter) replaced by min(nA, nB)); median of %d, %.2f s each"
-                      % (scale, len(ak0), len(bk0), repeats, t),
-        }
-    finally:
-        os.sched_setaffinity(0, old)
+        ak0, av0 = oracle.radix_sort_pairs(keys[: ns + no], oracle.gen_uniform_vals(ns + no, SEED_A), 1 << 32)
+        bk0, bv0 = oracle.radix_sort_pairs(np.concatenate([keys[:ns], keys[ns + no:]]),
+                                           oracle.gen_uniform_vals(ns + no, SEED_B), 1 << 32)
+        del keys
+    perm = _perm()
+    with _OneCore() as nproc:
+        ak, av, bk, bv = ak0.copy(), av0.copy(), bk0.copy(), bv0.copy()
+        t0 = time.perf_counter()
+        oracle.sort_seq_inplace(ak, av, DIMS, IDENT, perm, True)
+        oracle.sort_seq_inplace(bk, bv, DIMS, IDENT, perm, True)
+        nu = len(oracle.merge(ak, av, bk, bv, oracle.OP_ADD)[0])
+        nh = len(oracle.merge(ak, av, bk, bv, oracle.OP_HADAMARD)[0])
+        t = time.perf_counter() - t0
+    assert nu == len(ak0) + len(bk0) - nh
+    what = ("the same inputs as the GPU step (host copies), full size" if scale == 1 else
+            "a 1/%d-scale sample from the same generators" % scale)
+    return {
+        "value": (len(ak0) + len(bk0)) / t,
+        "unit": "keys/s",
+        "cores": 1,
+        "kind": "port",
+        "cpu_model": _cpu_model(),
+        "nproc": nproc,
+        "sample": "one step on %s (%d+%d keys), one pinned core, single run of %.2f s: 2x change_sort_order "
+                  "(relinearise + in-place MSD radix 2048/11-bit, thresholds 3000/50) + scan merge A+B + "
+                  "Hadamard (the reference's reserve(nA*nB/2^Inter) replaced by min(nA, nB))"
+                  % (what, len(ak0), len(bk0), t),
+    }
 
 
 def _profiled_traffic():
@@ -171,36 +197,131 @@ def bench_unsorted_sort(ctx, torch, n=50_000_000, reps=3):
     return {"keys": n, "radix_passes": passes, "ms": round(t * 1e3, 3), "keys_per_s": n / t, "sorted": ok}
 
 
-def bench_contraction(ctx, torch):
-    """Large skewed contraction, both accumulators, timed whole."""
+def _nprod_of(ak, bk, a_cols_div, b_rows_mod):
+    """Exact scalar-product count sum_k nnzA(:,k) * nnzB(k,:) of a lattice
+    pair whose A column is key // a_cols_div and B row key % b_rows_mod."""
+    ca = np.bincount(ak // a_cols_div, minlength=b_rows_mod).astype(np.int64)
+    cb = np.bincount(bk % b_rows_mod, minlength=b_rows_mod).astype(np.int64)
+    return int((ca[:b_rows_mod] * cb).sum())
+
+
+def _roofline(bytes_, seconds, what):
+    gbs = bytes_ / seconds / 1e9
+    return {"bound": "hbm", "kernel": what, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": bytes_, "traffic": None}
+
+
+def _contract_once(ctx, A, la, B, lb, strategy):
+    """nnz(C) of one contraction (device result freed)."""
+    from libnt_amd import dispatch
+
+    C, _, _ = dispatch.contract(ctx, A, la, B, lb, strategy)
+    return C.nnz
+
+
+def bench_small_contraction(ctx, torch, reps=20, cpu_repeats=5):
+    """C(i,j,m,n) = A(i,j,k,l)*B(k,l,m,n) over dims 32^4 at 1% (10,486 nnz each,
+    keys from the oracle's generators, unsorted), every strategy; and
+    change_sort_order of both operands to a seeded permutation.  The CPU
+    baseline runs the oracle's contraction and sort(seq) on the same arrays."""
+    import oracle
+    from libnt_amd import dispatch
+    from libnt_amd.sparse import SparseMIA
+
+    dims, n = [32, 32, 32, 32], 10486
+    ak, av = oracle.gen_distinct_keys(n, 1 << 20, 1), oracle.gen_uniform_vals(n, 1001)
+    bk, bv = oracle.gen_distinct_keys(n, 1 << 20, 2), oracle.gen_uniform_vals(n, 1002)
+    A = SparseMIA(dims, torch.from_numpy(ak).cuda(), torch.from_numpy(av).cuda(), ctx=ctx)
+    B = SparseMIA(dims, torch.from_numpy(bk).cuda(), torch.from_numpy(bv).cuda(), ctx=ctx)
+    nprod = _nprod_of(ak, bk, 1024, 1024)  # A's (k,l) = key >> 10, B's (k,l) = key & 1023
+    ms = dispatch.time_strategies(ctx, A, "ijkl", B, "klmn", (1 << dispatch.NUM_STRATEGIES) - 1, reps)
+    best = int(np.argmin(np.where(ms > 0, ms, np.inf)))
+    nnz_c = _contract_once(ctx, A, "ijkl", B, "klmn", best)
+    perm = _perm(1)
+    t_sort = []
+    for _ in range(reps + 1):
+        k1, v1 = A.keys.clone(), A.vals.clone()
+        k2, v2 = B.keys.clone(), B.vals.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.change_sort_order(k1, v1, dims, IDENT, perm, False)
+        ctx.change_sort_order(k2, v2, dims, IDENT, perm, False)
+        torch.cuda.synchronize()
+        t_sort.append(time.perf_counter() - t0)
+    t_sort = float(np.median(t_sort[1:]))
+    t_gpu = ms[best] * 1e-3
+    res = {"workload": "C(i,j,m,n)=A(i,j,k,l)*B(k,l,m,n), dims 32^4, 10486 nnz each (1%), unsorted; "
+                       "change_sort_order of both to perm %s" % (perm,),
+           "products": nprod, "nnz_c": nnz_c, "unit": "GFLOP/s (2 flops per scalar product)",
+           "strategy_ms": {dispatch.strategy_name(s): round(float(ms[s]), 4) for s in range(len(ms)) if ms[s] > 0},
+           "best": dispatch.strategy_name(best), "best_ms": round(float(ms[best]), 4),
+           "gflops": round(2 * nprod / t_gpu / 1e9, 3),
+           "sort_ms": round(t_sort * 1e3, 4), "sorted_keys_per_s": 2 * n / t_sort,
+           "roofline": _roofline(16.0 * (2 * n + nnz_c), t_gpu, "whole contraction (%s), compulsory bytes"
+                                 % dispatch.strategy_name(best))}
+    with _OneCore():
+        t_cc = _timed(lambda: oracle.contract(ak, av, dims, IDENT, False, "ijkl", bk, bv, dims, IDENT, False,
+                                              "klmn"), cpu_repeats)
+        t_cs = _timed(lambda: (oracle.sort_seq(ak, av, dims, IDENT, perm, False),
+                               oracle.sort_seq(bk, bv, dims, IDENT, perm, False)), cpu_repeats)
+    res["cpu_baseline"] = {"value": 2 * nprod / t_cc / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+                           "ms": round(t_cc * 1e3, 3), "sort_ms": round(t_cs * 1e3, 3),
+                           "sorted_keys_per_s": 2 * n / t_cs,
+                           "sample": "the same arrays, one pinned core, median of %d: csc_no_accum contraction "
+                                     "(unfold sorts, per-column expand/radix sort/collect) and 2x sort(seq)"
+                                     % cpu_repeats}
+    _log("small contraction: %s" % res)
+    return res
+
+
+def bench_contraction(ctx, torch, cpu_repeats=3):
+    """Large skewed contraction, every accumulator, timed whole; roofline over
+    the compulsory bytes; the oracle on one core on the same arrays."""
+    import oracle
     from libnt_amd import dispatch
 
     sp = dispatch.Spec("skewed", [4096, 4096, 512], "ijk", [512, 4096, 64], "klm", 858993, 13422,
                        "default", "default", (0,), (1,))
     A, B = dispatch.materialize(ctx, sp, 31)
-    # scalar products: sum over k of nnzA(:,k) * nnzB(k,:)
-    ka = A.keys.cpu().numpy()
-    kb = B.keys.cpu().numpy()
-    ca = np.bincount(ka // (4096 * 4096), minlength=512)
-    cb = np.bincount(kb % 512, minlength=512)
-    nprod = int((ca.astype(np.int64) * cb).sum())
-    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, 0b0111, 3)
+    ka, va = A.keys.cpu().numpy(), A.vals.cpu().numpy()
+    kb, vb = B.keys.cpu().numpy(), B.vals.cpu().numpy()
+    nprod = _nprod_of(ka, kb, 4096 * 4096, 512)  # A's k = key // (4096*4096), B's k = key % 512
+    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, (1 << dispatch.NUM_STRATEGIES) - 1, 3)
+    timed = [s for s in range(len(ms)) if ms[s] > 0]
+    best = min(timed, key=lambda s: ms[s])
+    nnz_c = _contract_once(ctx, A, sp.la, B, sp.lb, best)
     res = {"workload": "A(i,j,k)*B(k,l,m), A 4096x4096x512 nnz 858993, B 512x4096x64 nnz 13422, "
                        "i and l Zipf(1.1); unfold sorts + lattice product + refold",
-           "products": nprod, "unit": "GFLOP/s (2 flops per scalar product)"}
-    for s in (0, 1, 2):
-        res[dispatch.strategy_name(s)] = {"ms": round(ms[s], 3), "gflops": round(2 * nprod / (ms[s] * 1e-3) / 1e9, 2)}
-    best = min((0, 1, 2), key=lambda s: ms[s])
+           "products": nprod, "nnz_c": nnz_c, "unit": "GFLOP/s (2 flops per scalar product)"}
+    for s in timed:
+        res[dispatch.strategy_name(s)] = {"ms": round(float(ms[s]), 3),
+                                          "gflops": round(2 * nprod / (ms[s] * 1e-3) / 1e9, 2)}
     res["best"] = dispatch.strategy_name(best)
+    res["gflops"] = round(2 * nprod / (ms[best] * 1e-3) / 1e9, 2)
+    res["roofline"] = _roofline(16.0 * (A.nnz + B.nnz + nnz_c), ms[best] * 1e-3,
+                                "whole contraction (%s), compulsory bytes 16*(nnzA+nnzB+nnzC)"
+                                % dispatch.strategy_name(best))
+    if cpu_repeats < 1:
+        _log("contraction: %s" % res)
+        return res
+    with _OneCore():
+        t = _timed(lambda: oracle.contract(ka, va, sp.dims_a, IDENT[:3], True, sp.la, kb, vb, sp.dims_b, IDENT[:3],
+                                           True, sp.lb), cpu_repeats)
+    res["cpu_baseline"] = {"value": 2 * nprod / t / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+                           "ms": round(t * 1e3, 1),
+                           "sample": "the same arrays (host copies), one pinned core, median of %d: csc_no_accum "
+                                     "contraction (per-column expand, in-place MSD radix, collect_duplicates)"
+                                     % cpu_repeats}
     _log("contraction: %s" % res)
     return res
 
 
-def bench_batched(ctx, torch):
+def bench_batched(ctx, torch, cpu_stride=100):
     """The batched mix: autotuned recorded graph vs one call per instance."""
     import ctypes
 
     import libnt_amd
+    import oracle
     from libnt_amd import autotune, dispatch, workloads
     from libnt_amd.batch import Batch, packed_offsets
     from libnt_amd.sparse import SparseMIA
@@ -213,7 +334,7 @@ def bench_batched(ctx, torch):
     autotune.apply_batch(ctx, b, tuned["config"])
     nprod = b.nprod(ops[0], ops[2])
     nnz = b.capture(*ops)
-    b.replay()
+    _, _, c_off = b.replay()
     times = []
     for _ in range(3):
         torch.cuda.synchronize()
@@ -224,7 +345,9 @@ def bench_batched(ctx, torch):
     b.close()
     del b
     torch.cuda.empty_cache()
-    # one lnt_contract call per instance (library-allocated result, freed)
+    nnz_i = np.diff(c_off)
+    compulsory = 16.0 * (sum(x.nnz_a + x.nnz_b for x in insts) + int(nnz))
+    # one lnt_contract_into call per instance, C into one reused buffer
     a_off = packed_offsets([x.nnz_a for x in insts])
     b_off = packed_offsets([x.nnz_b for x in insts])
     ak, av, bk, bv = ops
@@ -233,36 +356,42 @@ def bench_batched(ctx, torch):
         A = SparseMIA(x.dims_a, ak[a_off[i]:a_off[i + 1]], av[a_off[i]:a_off[i + 1]], ctx=ctx)
         B = SparseMIA(x.dims_b, bk[b_off[i]:b_off[i + 1]], bv[b_off[i]:b_off[i + 1]], ctx=ctx)
         pairs.append((dispatch.operand(A, x.labels_a), dispatch.operand(B, x.labels_b)))
+    cap = int(max(nnz_i.max(), 1))
+    out_k = torch.empty(cap, dtype=torch.int64, device="cuda")
+    out_v = torch.empty(cap, dtype=torch.float64, device="cuda")
     L = libnt_amd.lib()
 
     def per_instance(strategy):
-        ck, cv = ctypes.c_void_p(), ctypes.c_void_p()
         n = ctypes.c_int64()
         shape = dispatch.ResultShape()
         used = ctypes.c_int()
         hist = np.zeros(dispatch.NUM_STRATEGIES, dtype=np.int64)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for oa, ob in pairs:
-            libnt_amd.check(L.lnt_contract(ctx.handle, ctypes.byref(oa), ctypes.byref(ob), strategy, ctypes.byref(ck),
-                                           ctypes.byref(cv), ctypes.byref(n), ctypes.byref(shape), ctypes.byref(used)))
-            L.lnt_free(ctx.handle, ck, libnt_amd.MEM_DEVICE)
-            L.lnt_free(ctx.handle, cv, libnt_amd.MEM_DEVICE)
+        for i, (oa, ob) in enumerate(pairs):
+            libnt_amd.check(L.lnt_contract_into(ctx.handle, ctypes.byref(oa), ctypes.byref(ob), strategy,
+                                                out_k.data_ptr(), out_v.data_ptr(), cap, ctypes.byref(n),
+                                                ctypes.byref(shape), ctypes.byref(used)))
             hist[used.value] += 1
         torch.cuda.synchronize()
         return time.perf_counter() - t0, hist
 
+    per_instance(0)  # warm-up: workspaces reach their final sizes
     t_plain, h_plain = per_instance(dispatch.STRAT_AUTO)  # no model: the restated polyalgorithm
+    t_fixed, _ = per_instance(0)
     res = {
         "workload": "10,000 contractions: order 3-4, 1-2 contracted modes, dims log-U[8,128], nnz log-U[100,50k]",
         "products": int(nprod.sum()), "nnz_c": int(nnz),
         "batched_graph_s": round(t_batched, 4),
         "batched_gflops": round(2 * nprod.sum() / t_batched / 1e9, 2),
         "instances_per_s": round(len(insts) / t_batched, 1),
+        "roofline": _roofline(compulsory, t_batched, "whole batched pipeline (graph replay), compulsory bytes "
+                                                     "16*sum(nnzA+nnzB+nnzC)"),
         "autotuned": tuned["config"], "autotune_subsample_ms": {"default": round(tuned["default_ms"], 2),
                                                                  "tuned": round(tuned["ms"], 2)},
         "plain_heuristic_per_instance_s": round(t_plain, 4),
         "plain_strategy_histogram": h_plain.tolist(),
+        "fixed_default_per_instance_s": round(t_fixed, 4),
         "speedup_batched_vs_plain_heuristic": round(t_plain / t_batched, 2),
     }
     model_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnt_amd", "data", "dispatch_model.npy")
@@ -276,6 +405,34 @@ def bench_batched(ctx, torch):
         res["learned_per_instance_s"] = round(t_learned, 4)
         res["learned_strategy_histogram"] = h_learned.tolist()
         res["speedup_learned_vs_plain_heuristic"] = round(t_plain / t_learned, 3)
+    del out_k, out_v
+    if cpu_stride < 1:
+        _log("batched: %s" % res)
+        return res
+    # CPU baseline: the oracle, one instance after another, on every
+    # cpu_stride-th instance (operands rebuilt on the host by the same
+    # counter-based generators).
+    sample = list(range(0, len(insts), cpu_stride))
+    host_ops = []
+    for i in sample:
+        x = insts[i]
+        host_ops.append((oracle.gen_distinct_keys(x.nnz_a, x.dimensionality_a, workloads.KEY_SEED_A + i),
+                         oracle.gen_uniform_vals(x.nnz_a, workloads.VAL_SEED_A + i),
+                         oracle.gen_distinct_keys(x.nnz_b, x.dimensionality_b, workloads.KEY_SEED_B + i),
+                         oracle.gen_uniform_vals(x.nnz_b, workloads.VAL_SEED_B + i)))
+    with _OneCore():
+        t0 = time.perf_counter()
+        for (hak, hav, hbk, hbv), i in zip(host_ops, sample):
+            x = insts[i]
+            oracle.contract(hak, hav, x.dims_a, list(range(len(x.dims_a))), False, x.labels_a, hbk, hbv, x.dims_b,
+                            list(range(len(x.dims_b))), False, x.labels_b)
+        t_cpu = time.perf_counter() - t0
+    sp = int(sum(nprod[i] for i in sample))
+    res["cpu_baseline"] = {"value": len(sample) / t_cpu, "unit": "instances/s", "cores": 1, "kind": "port",
+                           "gflops": round(2 * sp / t_cpu / 1e9, 3),
+                           "sample": "every %dth instance (%d instances, %d products), one after another on one "
+                                     "pinned core, %.2f s: unfold sorts + csc_no_accum + refold"
+                                     % (cpu_stride, len(sample), sp, t_cpu)}
     _log("batched: %s" % res)
     return res
 
@@ -343,33 +500,15 @@ def bench_host_buffers(ctx, torch, ak, av, bk, bv, reps=4):
     return res
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-scale", type=int, default=8)
-    ap.add_argument("--no-extras", action="store_true", help="only the sort/merge workload")
-    ap.add_argument("--calibrate", action="store_true",
-                    help="after the timed steps, run a plain 1.6 GB write (fill) and read (sum) so that "
-                         "rocprofv3 FETCH_SIZE/WRITE_SIZE can be scaled by kernels of known traffic")
-    ap.add_argument("--host-buffers", action="store_true",
-                    help="also time the step on host arrays (PCIe copies included; not the headline value)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
-        sys.exit(2)
+SECTIONS = ("headline", "unsorted_sort", "small_contraction", "contraction", "batched", "learned")
 
-    import ctypes
 
-    import torch
-
+def bench_headline(ctx, torch, args):
+    """The sort/merge-at-scale step (the `value` of the JSON line)."""
     import libnt_amd
 
-    stream = torch.cuda.current_stream()
-    ctx = libnt_amd.Context(0, stream=ctypes.c_void_p(stream.cuda_stream))
     ak, av, bk, bv = make_c2(ctx, torch)
+    host = None if args.no_cpu_baseline or args.cpu_scale > 1 else [t.cpu().numpy() for t in (ak, av, bk, bv)]
     na, nb = ak.shape[0], bk.shape[0]
     cu_k = torch.empty(na + nb, dtype=torch.int64, device="cuda")
     cu_v = torch.empty(na + nb, dtype=torch.float64, device="cuda")
@@ -458,15 +597,57 @@ def main():
         del cal
     if args.host_buffers:
         out["host_buffers"] = bench_host_buffers(ctx, torch, ak, av, bk, bv)
+    del ak, av, bk, bv, cu_k, cu_v, ch_k, ch_v
+    torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(scale=args.cpu_scale)
-    if not args.no_extras:
-        del ak, av, bk, bv, cu_k, cu_v, ch_k, ch_v
-        torch.cuda.empty_cache()
+        out["cpu_baseline"] = cpu_baseline(*(host or (None,) * 4), scale=args.cpu_scale)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-scale", type=int, default=1,
+                    help="1: the CPU baseline runs on host copies of the GPU's inputs; s > 1: on the generator's "
+                         "first 1/s of the workload")
+    ap.add_argument("--no-extras", action="store_true", help="only the sort/merge workload")
+    ap.add_argument("--section", choices=("all",) + SECTIONS, default="all",
+                    help="run one section only (a profiling aid: the line then holds that section alone)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after the timed steps, run a plain 1.6 GB write (fill) and read (sum) so that "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE can be scaled by kernels of known traffic")
+    ap.add_argument("--host-buffers", action="store_true",
+                    help="also time the step on host arrays (PCIe copies included; not the headline value)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import torch
+
+    import libnt_amd
+
+    stream = torch.cuda.current_stream()
+    ctx = libnt_amd.Context(0, stream=ctypes.c_void_p(stream.cuda_stream))
+    want = set(SECTIONS) if args.section == "all" else {args.section}
+    if args.no_extras:
+        want &= {"headline"}
+    out = bench_headline(ctx, torch, args) if "headline" in want else {"section": args.section}
+    if "unsorted_sort" in want:
         out["unsorted_sort"] = bench_unsorted_sort(ctx, torch)
         _log("unsorted sort: %s" % (out["unsorted_sort"],))
-        out["contraction"] = bench_contraction(ctx, torch)
-        out["batched"] = bench_batched(ctx, torch)
+    if "small_contraction" in want:
+        out["small_contraction"] = bench_small_contraction(ctx, torch)
+    if "contraction" in want:
+        out["contraction"] = bench_contraction(ctx, torch, cpu_repeats=0 if args.no_cpu_baseline else 3)
+    if "batched" in want:
+        out["batched"] = bench_batched(ctx, torch, cpu_stride=0 if args.no_cpu_baseline else 100)
+    if "learned" in want:
         out["learned"] = bench_learned(ctx)
     print(json.dumps(out), flush=True)
 

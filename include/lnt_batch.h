@@ -71,7 +71,10 @@ int lnt_batch_numeric(lnt_batch batch, int64_t *c_keys, double *c_vals, int64_t 
  * c_keys/c_vals (capacity entries).  The batch's workspaces are sized by one
  * unrecorded run first, which also returns the exact result size in *nnz_c
  * (may be NULL); a replay on operands with more output than `capacity`
- * reports LNT_ERR_TOO_LARGE and writes nothing past it. */
+ * reports LNT_ERR_TOO_LARGE and writes nothing past it.  The graph's sort
+ * scratch (ping-pong buffers, look-back words, histograms) belongs to the
+ * batch, not to the context: later calls on the context, lnt_context_trim
+ * included, leave a recorded graph valid; the batch frees it on destroy. */
 int lnt_batch_capture(lnt_batch batch, const int64_t *a_keys, const double *a_vals,
                       const int64_t *b_keys, const double *b_vals, int64_t *c_keys,
                       double *c_vals, int64_t capacity, int64_t *nnz_c);
@@ -88,7 +91,7 @@ int lnt_batch_launch(lnt_batch batch);
 /* Tuning of the batch pipeline (results never depend on it):
  *   "item_products"  products per small-segment work item (64..1024)
  *   "small_max"      largest segment grouped into items   (2..1024)
- *   "window"         target products per window of a large segment (64..1024)
+ *   "window"         target products per window of a large segment (64..1536)
  *   "blocks_per_cu"  persistent workgroups per compute unit (1..8)
  * The sort parameters are the context's (lnt_context_set_param). */
 int lnt_batch_set_param(lnt_batch batch, const char *name, int64_t value);

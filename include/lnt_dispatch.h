@@ -70,6 +70,16 @@ int lnt_contract(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
                  int64_t **c_keys, double **c_vals, int64_t *nnz_c, lnt_result_shape *shape,
                  int *used_strategy);
 
+/* As lnt_contract, with C written to caller-allocated device arrays of
+ * `capacity` entries (no allocation per call: for loops over many
+ * contractions).  Fails with LNT_ERR_INVALID_ARGUMENT, writing nothing past
+ * the arrays, when nnz(C) exceeds the capacity; nnz(C) <= the scalar product
+ * count (lnt_lattice_nprod), and a pure Hadamard product needs min(nnz(A),
+ * nnz(B)).  Synchronises. */
+int lnt_contract_into(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, int strategy,
+                      int64_t *c_keys, double *c_vals, int64_t capacity, int64_t *nnz_c,
+                      lnt_result_shape *shape, int *used_strategy);
+
 /* The LNT_NUM_FEATURES features of the cost model (descriptors only, no data
  * is read): log2 of nnz(A), nnz(B), the lattice shape h, w, bw, d; the two
  * orders; contracted and inter mode counts; log2 of the estimated products

@@ -85,6 +85,7 @@ SIGNATURES = {
     "lnt_batch_get_param": [_vp, ctypes.c_char_p, _i64p],
     "lnt_batch_nprod": [_vp, _vp, _vp, _i64p],
     "lnt_contract": [_vp, _vp, _vp, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _vp, ctypes.POINTER(_int)],
+    "lnt_contract_into": [_vp, _vp, _vp, _int, _vp, _vp, _i64, _i64p, _vp, ctypes.POINTER(_int)],
     "lnt_contract_features": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
     "lnt_contract_heuristic": [_vp, _vp, ctypes.POINTER(_int)],
     "lnt_contract_time": [_vp, _vp, _vp, _int, _int, ctypes.POINTER(ctypes.c_double)],

@@ -889,6 +889,13 @@ struct lnt_batch_s {
   hipStream_t own_stream = nullptr;
   int64_t graph_capacity = 0;
   bool symbolic_ready = false;  // tables of the last symbolic phase are live
+  // Sort scratch of the recorded graph (ping-pong buffers, look-back status
+  // words and their epoch, histograms, counters and flags, merge splits): the
+  // batch's own, so that later calls on the context, which grow, trim or
+  // reuse the context's scratch, never touch memory a replay writes.
+  lnt::DeviceBuffer g_alt_keys, g_alt_vals, g_alt2_keys, g_alt2_vals, g_status, g_small, g_hist_part,
+      g_relin_dev, g_aux;
+  uint32_t g_epoch = 0;
 };
 
 namespace lnt {
@@ -1074,6 +1081,28 @@ struct StreamSwap {
   ~StreamSwap() { ctx->stream = saved; }
 };
 
+// While alive, the context's sort scratch is the batch's graph scratch (and
+// the other way round): the sizing run and the capture size and record the
+// batch's buffers, and the context gets its own back unchanged.
+struct ScratchSwap {
+  lnt_context ctx;
+  lnt_batch b;
+  ScratchSwap(lnt_context c, lnt_batch bb) : ctx(c), b(bb) { swap_all(); }
+  ~ScratchSwap() { swap_all(); }
+  void swap_all() {
+    std::swap(ctx->alt_keys, b->g_alt_keys);
+    std::swap(ctx->alt_vals, b->g_alt_vals);
+    std::swap(ctx->alt2_keys, b->g_alt2_keys);
+    std::swap(ctx->alt2_vals, b->g_alt2_vals);
+    std::swap(ctx->status, b->g_status);
+    std::swap(ctx->small, b->g_small);
+    std::swap(ctx->hist_part, b->g_hist_part);
+    std::swap(ctx->relin_dev, b->g_relin_dev);
+    std::swap(ctx->aux, b->g_aux);
+    std::swap(ctx->epoch, b->g_epoch);
+  }
+};
+
 }  // namespace
 }  // namespace lnt
 
@@ -1198,6 +1227,9 @@ int lnt_batch_destroy(lnt_batch b) {
   release(b->ws);
   release(b->scan_tmp);
   release(b->ctrl);
+  for (DeviceBuffer *x : {&b->g_alt_keys, &b->g_alt_vals, &b->g_alt2_keys, &b->g_alt2_vals, &b->g_status,
+                          &b->g_small, &b->g_hist_part, &b->g_relin_dev, &b->g_aux})
+    release(*x);
   if (b->pinned) (void)hipHostFree(b->pinned);
   delete b;
   return LNT_OK;
@@ -1305,16 +1337,22 @@ int lnt_batch_capture(lnt_batch b, const int64_t *a_keys, const double *a_vals, 
   LNT_TRY(prepare(b));
   BatchPtrs P{};
   carve(b, &P);
+  // The old graph goes first: its scratch may be regrown below.
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  // From here to the end of the capture the context sorts in the batch's own
+  // scratch (its stream is drained first: queued work may still use the
+  // context's buffers, which stay where they are).
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  ScratchSwap scratch(ctx, b);
   // One unrecorded run sizes every workspace (sort scratch, status words)
   // and checks the operands.
   LNT_TRY(enqueue_all(b, P, a_keys, a_vals, b_keys, b_vals, (uint64_t *)c_keys, c_vals, (uint64_t)capacity));
   LNT_TRY(read_back(b));
   LNT_TRY(check_key_error(b));
   if (nnz_c) *nnz_c = (int64_t)b->pinned[kCtrNc];
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
   // The legacy null stream cannot be captured: record on a stream of our own
   // (blocking, default priority) when the context has none.
   hipStream_t st = ctx->stream;

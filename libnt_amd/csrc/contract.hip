@@ -420,6 +420,34 @@ int lnt_contract(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
   return LNT_OK;
 }
 
+int lnt_contract_into(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, int strategy, int64_t *c_keys,
+                      double *c_vals, int64_t capacity, int64_t *nnz_c, lnt_result_shape *shape, int *used_strategy) {
+  if (!ctx || !nnz_c || !shape) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  if (capacity < 0 || (capacity > 0 && (!c_keys || !c_vals)))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "negative capacity or null output arrays");
+  *nnz_c = 0;
+  LNT_TRY(check_operand(a, "A"));
+  LNT_TRY(check_operand(b, "B"));
+  if (strategy != LNT_STRAT_AUTO && (strategy < 0 || strategy >= LNT_NUM_STRATEGIES))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "unknown strategy");
+  LNT_CTX(ctx);
+  if (strategy == LNT_STRAT_AUTO) LNT_TRY(choose(ctx, a, b, &strategy));
+  uint64_t *k;
+  double *v;
+  uint64_t n;
+  // A one-element dummy keeps run_strategy's caller-buffer mode for an empty
+  // capacity: any non-empty result then fails the capacity check.
+  uint64_t dummy_k;
+  double dummy_v;
+  uint64_t *ok = capacity > 0 ? (uint64_t *)c_keys : &dummy_k;
+  double *ov = capacity > 0 ? c_vals : &dummy_v;
+  LNT_TRY(run_strategy(ctx, a, b, strategy, ok, ov, (uint64_t)capacity, &k, &v, &n, shape));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  *nnz_c = (int64_t)n;
+  if (used_strategy) *used_strategy = strategy;
+  return LNT_OK;
+}
+
 int lnt_contract_time(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, int mask, int reps,
                       double *ms) {
   if (!ctx || !ms || reps < 1) return fail(LNT_ERR_INVALID_ARGUMENT, "bad arguments");

@@ -753,7 +753,18 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
   }
   write_kernel<<<chunk_grid(nc), 256, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.colid, E.prod_off, E.seg_first,
                                                        nseg, nb, E.out_off, nc, tmp_row, tmp_val, ah, ck, cv);
-  LNT_HIP(hipGetLastError());
+  const hipError_t launch = hipGetLastError();
+  if (launch != hipSuccess) {
+    prof_end(ctx, slot);
+    if (ck_alloc) {  // the result allocated above is not handed back
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipFree(ck);
+      (void)hipFree(cv);
+      *ck_alloc = nullptr;
+      *cv_alloc = nullptr;
+    }
+    return fail(LNT_ERR_DEVICE, std::string("write_kernel launch: ") + hipGetErrorString(launch));
+  }
   prof_end(ctx, slot);
   prof_set_bytes(ctx, slot, 16.0 * ((double)na + nb + nc));
   *nc_host = nc;

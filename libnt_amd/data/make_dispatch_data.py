@@ -1,11 +1,13 @@
 """Measures the dispatch training and held-out sets on the GPU and trains the
-shipped cost model (run on one MI355X from the repository root):
+shipped cost model (run on one MI355X from the repository root).  A
+maintainer's script that writes this data directory; the directory is not a
+Python package and nothing in the library imports this file:
 
-    python -m libnt_amd.data.make_dispatch_data --part train --start 0 --count 7000 --out DIR
-    python -m libnt_amd.data.make_dispatch_data --part train --start 7000 --count 7000 --out DIR
-    python -m libnt_amd.data.make_dispatch_data --part train --start 14000 --count 6000 --out DIR
-    python -m libnt_amd.data.make_dispatch_data --part test --start 0 --count 2000 --out DIR
-    python -m libnt_amd.data.make_dispatch_data --part fit --out DIR
+    python libnt_amd/data/make_dispatch_data.py --part train --start 0 --count 7000 --out DIR
+    python libnt_amd/data/make_dispatch_data.py --part train --start 7000 --count 7000 --out DIR
+    python libnt_amd/data/make_dispatch_data.py --part train --start 14000 --count 6000 --out DIR
+    python libnt_amd/data/make_dispatch_data.py --part test --start 0 --count 2000 --out DIR
+    python libnt_amd/data/make_dispatch_data.py --part fit --out DIR
 
 (chunks keep each run short).  Writes DIR/dispatch_timings.npz (features, log2-ms timings of all 8
 strategies, the restated polyalgorithm's choice) and DIR/dispatch_model.npy
@@ -18,6 +20,10 @@ import sys
 import time
 
 import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
 
 EPOCHS, LR = 300, 1e-3
 

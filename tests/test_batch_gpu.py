@@ -174,6 +174,42 @@ def test_batch_graph_replay_matches_eager(ctx):
     assert torch.equal(gk, ek2) and torch.equal(gv.view(torch.int64), ev2.view(torch.int64))
 
 
+def test_batch_graph_survives_context_scratch_changes(ctx):
+    """A recorded graph keeps its own sort scratch: a larger sort on the same
+    context (which regrows the context's ping-pong buffers and look-back
+    words), a trim (which frees them) and a merge in between leave every
+    replay equal to the eager run."""
+    import torch
+
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch
+
+    insts = W.batched_contractions(150, seed=15)
+    ak, av, bk, bv = W.generate_operands(ctx, insts)
+    b = Batch(ctx, insts)
+    ek, ev, eoff = b.contract(ak, av, bk, bv)
+    b.capture(ak, av, bk, bv)
+    gk, gv, goff = b.replay()
+    assert torch.equal(gk, ek) and torch.equal(gv.view(torch.int64), ev.view(torch.int64))
+    n = max(4 * int(ak.shape[0]), 1 << 22)
+    for step in ("bigger sort", "trim", "sort after trim"):
+        if step == "trim":
+            ctx.trim()
+        else:
+            k = torch.empty(n, dtype=torch.int64, device="cuda")
+            v = torch.empty(n, dtype=torch.float64, device="cuda")
+            ctx.testgen_distinct_keys(k, 1 << 40, 77)
+            ctx.testgen_uniform(v, 78)
+            ctx.sort_pairs(k, v, 1 << 40)
+            torch.cuda.synchronize()
+            assert bool((k[1:] > k[:-1]).all())
+            n *= 2
+        gk, gv, goff = b.replay()
+        assert np.array_equal(goff, eoff), step
+        assert torch.equal(gk, ek) and torch.equal(gv.view(torch.int64), ev.view(torch.int64)), step
+    b.close()
+
+
 def test_batch_graph_capacity_overflow(ctx):
     import libnt_amd
     from libnt_amd import workloads as W

@@ -265,6 +265,39 @@ def test_collect_duplicates(orc):
     assert k.tolist() == [1, 2, 5, 9] and v.tolist() == [3.0, 3.0, 3.0, -4.0]
 
 
+def test_sort_merge_zero_rule_known_answers(orc):
+    """SparseMIA::sortMerge + collect_duplicates (SparseMIA.h:628-665,
+    833-887) by hand.  The loop keeps the first entry of the sorted array
+    unconditionally; any later key opens only on a non-zero entry (a zero
+    first entry is skipped and the key's next entry is tried as new); equal
+    keys add.  Subtraction copies -1*b and then adds."""
+    # A={1:0, 3:1, 5:2, 7:0}, B={1:2, 4:0, 5:-2, 7:-0}: 1 is the smallest key
+    # (kept: 0+2); 4 holds only a zero and 7 only zeros (both vanish); 5 sums
+    # to an explicit zero, which stays (it opened on a non-zero entry).
+    ck, cv = orc.sort_merge(np.array([3, 1, 7, 5]), np.array([1.0, 0.0, 0.0, 2.0]),
+                            np.array([5, 1, 7, 4]), np.array([-2.0, 2.0, -0.0, 0.0]), orc.OP_ADD, 8)
+    assert ck.tolist() == [1, 3, 5] and cv.tolist() == [2.0, 1.0, 0.0]
+    # The smallest key stays even when every entry of it is zero, and keeps
+    # the sum's sign: 0.0 + 0.0 = +0.0 whichever entry comes first.
+    ck, cv = orc.sort_merge(np.array([0, 6]), np.array([0.0, 3.0]), np.array([0]), np.array([0.0]), orc.OP_ADD, 8)
+    assert ck.tolist() == [0, 6] and cv.tolist() == [0.0, 3.0]
+    assert not np.signbit(cv[0])
+    # -0.0 + -0.0 = -0.0 on the smallest key (both zero, kept).
+    ck, cv = orc.sort_merge(np.array([2]), np.array([-0.0]), np.array([2]), np.array([-0.0]), orc.OP_ADD, 8)
+    assert ck.tolist() == [2] and np.signbit(cv[0])
+    # Subtraction negates B first: 3 - 3 = 0 kept, B-only 4 -> -5; a B-only
+    # key whose value is 0 becomes -1*0 = -0.0, a zero, and vanishes.
+    ck, cv = orc.sort_merge(np.array([2, 3]), np.array([1.0, 3.0]), np.array([4, 3, 6]), np.array([5.0, 3.0, 0.0]),
+                            orc.OP_SUB, 8)
+    assert ck.tolist() == [2, 3, 4] and cv.tolist() == [1.0, 0.0, -5.0]
+    # A zero A entry followed by a non-zero B entry of the same (not smallest)
+    # key opens the key with the B value alone: 0 is skipped, not added.
+    ck, cv = orc.sort_merge(np.array([1, 9]), np.array([1.0, 0.0]), np.array([9]), np.array([-0.0]), orc.OP_SUB, 16)
+    assert ck.tolist() == [1] and cv.tolist() == [1.0]
+    ck, cv = orc.sort_merge(np.array([1, 9]), np.array([1.0, 0.0]), np.array([9]), np.array([-7.5]), orc.OP_SUB, 16)
+    assert ck.tolist() == [1, 9] and cv.tolist() == [1.0, 7.5]
+
+
 # ---------------------------------------------------------------- merging
 def _dense(keys, vals, size):
     d = np.zeros(size)
