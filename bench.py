@@ -250,7 +250,7 @@ def bench_small_contraction(ctx, torch, reps=20, cpu_repeats=5):
         t_sort.append(time.perf_counter() - t0)
     t_sort = float(np.median(t_sort[1:]))
     t_gpu = ms[best] * 1e-3
-    res = {"workload": "C(i,j,m,n)=A(i,j,k,l)*B(k,l,m,n), dims 32^4, 10486 nnz each (1%), unsorted; "
+    res = {"workload": "C(i,j,m,n)=A(i,j,k,l)*B(k,l,m,n), dims 32^4, 10486 nnz each (1%%), unsorted; "
                        "change_sort_order of both to perm %s" % (perm,),
            "products": nprod, "nnz_c": nnz_c, "unit": "GFLOP/s (2 flops per scalar product)",
            "strategy_ms": {dispatch.strategy_name(s): round(float(ms[s]), 4) for s in range(len(ms)) if ms[s] > 0},
@@ -316,7 +316,7 @@ def bench_contraction(ctx, torch, cpu_repeats=3):
     return res
 
 
-def bench_batched(ctx, torch, cpu_stride=100):
+def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True):
     """The batched mix: autotuned recorded graph vs one call per instance."""
     import ctypes
 
@@ -347,6 +347,20 @@ def bench_batched(ctx, torch, cpu_stride=100):
     torch.cuda.empty_cache()
     nnz_i = np.diff(c_off)
     compulsory = 16.0 * (sum(x.nnz_a + x.nnz_b for x in insts) + int(nnz))
+    res = {
+        "workload": "10,000 contractions: order 3-4, 1-2 contracted modes, dims log-U[8,128], nnz log-U[100,50k]",
+        "products": int(nprod.sum()), "nnz_c": int(nnz),
+        "batched_graph_s": round(t_batched, 4),
+        "batched_gflops": round(2 * nprod.sum() / t_batched / 1e9, 2),
+        "instances_per_s": round(len(insts) / t_batched, 1),
+        "roofline": _roofline(compulsory, t_batched, "whole batched pipeline (graph replay), compulsory bytes "
+                                                     "16*sum(nnzA+nnzB+nnzC)"),
+        "autotuned": tuned["config"], "autotune_subsample_ms": {"default": round(tuned["default_ms"], 2),
+                                                                 "tuned": round(tuned["ms"], 2)},
+    }
+    if not per_instance_runs:
+        _log("batched: %s" % res)
+        return res
     # one lnt_contract_into call per instance, C into one reused buffer
     a_off = packed_offsets([x.nnz_a for x in insts])
     b_off = packed_offsets([x.nnz_b for x in insts])
@@ -379,21 +393,12 @@ def bench_batched(ctx, torch, cpu_stride=100):
     per_instance(0)  # warm-up: workspaces reach their final sizes
     t_plain, h_plain = per_instance(dispatch.STRAT_AUTO)  # no model: the restated polyalgorithm
     t_fixed, _ = per_instance(0)
-    res = {
-        "workload": "10,000 contractions: order 3-4, 1-2 contracted modes, dims log-U[8,128], nnz log-U[100,50k]",
-        "products": int(nprod.sum()), "nnz_c": int(nnz),
-        "batched_graph_s": round(t_batched, 4),
-        "batched_gflops": round(2 * nprod.sum() / t_batched / 1e9, 2),
-        "instances_per_s": round(len(insts) / t_batched, 1),
-        "roofline": _roofline(compulsory, t_batched, "whole batched pipeline (graph replay), compulsory bytes "
-                                                     "16*sum(nnzA+nnzB+nnzC)"),
-        "autotuned": tuned["config"], "autotune_subsample_ms": {"default": round(tuned["default_ms"], 2),
-                                                                 "tuned": round(tuned["ms"], 2)},
+    res.update({
         "plain_heuristic_per_instance_s": round(t_plain, 4),
         "plain_strategy_histogram": h_plain.tolist(),
         "fixed_default_per_instance_s": round(t_fixed, 4),
         "speedup_batched_vs_plain_heuristic": round(t_plain / t_batched, 2),
-    }
+    })
     model_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnt_amd", "data", "dispatch_model.npy")
     if os.path.exists(model_path):
         m = dispatch.Model(ctx, 1)
@@ -616,6 +621,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="only the sort/merge workload")
     ap.add_argument("--section", choices=("all",) + SECTIONS, default="all",
                     help="run one section only (a profiling aid: the line then holds that section alone)")
+    ap.add_argument("--graph-only", action="store_true",
+                    help="batched section: the recorded graph only, no per-instance loops (a profiling aid)")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, run a plain 1.6 GB write (fill) and read (sum) so that "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE can be scaled by kernels of known traffic")
@@ -646,7 +653,8 @@ def main():
     if "contraction" in want:
         out["contraction"] = bench_contraction(ctx, torch, cpu_repeats=0 if args.no_cpu_baseline else 3)
     if "batched" in want:
-        out["batched"] = bench_batched(ctx, torch, cpu_stride=0 if args.no_cpu_baseline else 100)
+        out["batched"] = bench_batched(ctx, torch, cpu_stride=0 if args.no_cpu_baseline else 100,
+                                       per_instance_runs=not args.graph_only)
     if "learned" in want:
         out["learned"] = bench_learned(ctx)
     print(json.dumps(out), flush=True)

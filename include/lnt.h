@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LNT_ABI_VERSION 1
+#define LNT_ABI_VERSION 2
 #define LNT_MAX_ORDER 16
 
 typedef enum lnt_status {
@@ -205,10 +205,21 @@ int lnt_free(lnt_context ctx, void *ptr, int mem);
 int lnt_copy(lnt_context ctx, void *dst, const void *src, int64_t bytes);
 
 /* ------------------------------------------------------- lattice product -- */
+/* The accumulator of a lattice product, optionally or-ed with
+ * LNT_SPGEMM_CSC.  Every strategy returns the same keys; SORT, DENSE, ESC and
+ * MTT the same values bit for bit (each row summed in ascending B row order),
+ * HASH values within the accumulation-order tolerance. */
 typedef enum lnt_spgemm_strategy {
-  LNT_SPGEMM_AUTO = 0, /* library choice (heuristic or learned dispatch) */
-  LNT_SPGEMM_SORT = 1, /* expand, sort rows, sum runs (deterministic sums) */
-  LNT_SPGEMM_HASH = 2  /* LDS hash accumulation for short columns */
+  LNT_SPGEMM_AUTO = 0,  /* library choice: SORT */
+  LNT_SPGEMM_SORT = 1,  /* expand, sort rows, sum runs: csc_no_accum (SparseLatticeBase.h:1854-1990) */
+  LNT_SPGEMM_HASH = 2,  /* LDS hash accumulation for short columns */
+  LNT_SPGEMM_DENSE = 3, /* dense sparse accumulator over rows in LDS (csc_times, :2224-2351) when
+                           h <= 8192, else SORT */
+  LNT_SPGEMM_ESC = 4,   /* expand every product, one global sort, sum runs (outer_times, :2034-2183) */
+  LNT_SPGEMM_MTT = 5,   /* dense accumulator over the ranks of A's rows present (mtt_times,
+                           :1579-1813) when there are <= 8192 of them, else SORT */
+  LNT_SPGEMM_CSC = 16   /* flag: find A's columns through a column-pointer array (CSC,
+                           MultHelper<.,false>) instead of a search of A's keys (DCSC) */
 } lnt_spgemm_strategy;
 
 /* A lattice is an order-3 view of a tensor: dims {h, w, d} (rows, columns,

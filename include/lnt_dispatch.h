@@ -8,15 +8,25 @@
  * 1574 operator*, csc_no_accum :1854-1990) and refold (SparseMIA.h:362-372;
  * MIA_Expr.h:1114-1165).  The reference chooses an orientation and an
  * algorithm per product from sparsity flags and the operands' current sort
- * orders (MATLAB sparse_times_polyalgorithm.m:1-208); here the choices are
- *   LNT_STRAT_SWAP    compute B*A and re-sort the result into C's order
- *                     (the polyalgorithm's `transposed`);
- *   LNT_STRAT_HASH    LDS hash accumulator instead of sort-accumulate
- *                     (an accumulator-based product, cf. csc_times versus
- *                     csc_no_accum, SparseLatticeBase.h:2224-2351, 1854-1990);
- *   LNT_STRAT_RADIX6  6-bit radix digits in the operand sorts instead of 8.
- * Every strategy returns the same keys bit for bit and values within the
- * accumulation-order tolerance.  LNT_STRAT_AUTO asks the context's model
+ * orders (MATLAB sparse_times_polyalgorithm.m:1-208); here a strategy is
+ * LNT_STRATEGY(accumulator, radix6, swap):
+ *   swap        compute B*A and re-sort the result into C's order (the
+ *               polyalgorithm's `transposed`);
+ *   accumulator LNT_ACC_SORT  expand, sort rows, sum runs (csc_no_accum,
+ *                             SparseLatticeBase.h:1854-1990);
+ *               LNT_ACC_HASH  LDS hash accumulator for short columns;
+ *               LNT_ACC_DENSE dense sparse accumulator (csc_times,
+ *                             :2224-2351) when the lattice height is <= 8192;
+ *               LNT_ACC_ESC   expand every product, one sort, sum runs
+ *                             (outer_times, :2034-2183);
+ *               LNT_ACC_MTT   dense accumulator over the ranks of A's rows
+ *                             present (mtt_times, :1579-1813) when <= 8192;
+ *               (DENSE and MTT use SORT where their accumulator would not
+ *               fit in LDS);
+ *   radix6      6-bit radix digits in the operand sorts instead of 8.
+ * Every strategy returns the same keys bit for bit; SORT, DENSE, ESC and MTT
+ * in one orientation the same values bit for bit, the others values within
+ * the accumulation-order tolerance.  LNT_STRAT_AUTO asks the context's model
  * (lnt_context_set_model), or, without one, the restated polyalgorithm
  * (lnt_contract_heuristic).
  *
@@ -32,9 +42,16 @@ extern "C" {
 #endif
 
 #define LNT_STRAT_SWAP 1
-#define LNT_STRAT_HASH 2
-#define LNT_STRAT_RADIX6 4
-#define LNT_NUM_STRATEGIES 8
+#define LNT_STRAT_RADIX6 2
+#define LNT_ACC_SORT 0
+#define LNT_ACC_HASH 1
+#define LNT_ACC_DENSE 2
+#define LNT_ACC_ESC 3
+#define LNT_ACC_MTT 4
+#define LNT_NUM_ACCUMULATORS 5
+#define LNT_STRATEGY(acc, radix6, swap) (((acc) << 2) | ((radix6) ? LNT_STRAT_RADIX6 : 0) | ((swap) ? LNT_STRAT_SWAP : 0))
+#define LNT_STRATEGY_ACC(s) ((s) >> 2)
+#define LNT_NUM_STRATEGIES (4 * LNT_NUM_ACCUMULATORS)
 #define LNT_STRAT_AUTO (-1)
 #define LNT_NUM_FEATURES 18
 
@@ -63,7 +80,7 @@ typedef struct {
   char labels[LNT_MAX_ORDER + 1];
 } lnt_result_shape;
 
-/* C = A * B with `strategy` (0..7 or LNT_STRAT_AUTO).  The library allocates
+/* C = A * B with `strategy` (0 .. LNT_NUM_STRATEGIES-1 or LNT_STRAT_AUTO).  The library allocates
  * c_keys/c_vals (free with lnt_free, LNT_MEM_DEVICE).  *used_strategy (may
  * be NULL) receives the strategy that ran.  Synchronises. */
 int lnt_contract(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, int strategy,
@@ -89,9 +106,10 @@ int lnt_contract_into(lnt_context ctx, const lnt_operand *a, const lnt_operand *
 int lnt_contract_features(const lnt_operand *a, const lnt_operand *b, float *features);
 
 /* The restated polyalgorithm (sparse_times_polyalgorithm.m): orientation from
- * the operands' current orders and hyper-sparsity flags (dim/nnz > 3).  Its
- * algorithm classes pick CPU accumulators with no GPU counterpart, so the
- * accumulator is always the default sort-accumulate, with 8-bit digits. */
+ * the operands' current orders and hyper-sparsity flags (dim/nnz > 3), and its
+ * algorithm class: 1 and 2 (csc_times CSC / DCSC) -> LNT_ACC_DENSE, 0 and 3
+ * (csc_no_accum) -> LNT_ACC_SORT, 4 (outer product) -> LNT_ACC_ESC; 8-bit
+ * digits. */
 int lnt_contract_heuristic(const lnt_operand *a, const lnt_operand *b, int *strategy);
 
 /* Wall time (ms, host clock around the call and a final synchronisation) of

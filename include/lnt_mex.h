@@ -86,12 +86,15 @@ int lnt_mex_sparse_merge(lnt_context ctx, const double *a_data, const int64_t *a
  * algorithm]) (MATLAB/mex/sparselattice_mul.cpp:15-92, 94-136): C = A*B tab
  * by tab, both lattices (dims {h, w, d}) sorted ascending; C sorted, exact
  * zeros kept (lnt_lattice_mult_into).  `algorithm` is truncated to an
- * integer as the mex does; 0..5 name the reference's strategies (0 operator*
- * = csc_no_accum<true>, 1 csc_times, 2 csc_times<DCSC>, 3 csc_no_accum,
- * 4/5 mtt_times), all of which give the same C up to the summation order of
- * a column's duplicate rows; here 0, 4 and 5 take the library's choice, 1
- * and 2 the LDS-hash accumulation, 3 the sort-and-sum path.  Other values
- * fail as the mex does.  *c_data / *c_idx receive arrays of *nc elements. */
+ * integer as the mex does; 0..5 name the reference's methods, each run by
+ * its GPU counterpart (lnt.h lnt_spgemm_strategy): 0 operator* =
+ * csc_no_accum<true> -> SORT; 1 csc_times<false> (CSC) -> DENSE | CSC; 2
+ * csc_times<true> (DCSC) -> DENSE; 3 csc_no_accum<false> (CSC) -> SORT | CSC;
+ * 5 mtt_times -> MTT; 4 -> MTT as well, because the mex's case 4 falls
+ * through into case 5 (sparselattice_mul.cpp:52-58), whose mtt_times result
+ * on the same arguments is what it returns.  All give the same keys and the
+ * same values bit for bit (each row summed in ascending B row order).  Other
+ * values fail as the mex does.  *c_data / *c_idx receive arrays of *nc elements. */
 int lnt_mex_sparselattice_mult(lnt_context ctx, const double *a_data, const int64_t *a_idx, int64_t na,
                                const int64_t a_dims[3], const double *b_data, const int64_t *b_idx, int64_t nb,
                                const int64_t b_dims[3], double algorithm, lnt_mex_alloc alloc, double **c_data,

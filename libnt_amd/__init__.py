@@ -29,7 +29,8 @@ STATUS_NAMES = {
 MEM_HOST, MEM_DEVICE = 0, 1
 OP_ADD, OP_SUB, OP_HADAMARD = 0, 1, 2
 CMP_EXACT, CMP_FUZZY = 0, 1
-SPGEMM_AUTO, SPGEMM_SORT, SPGEMM_HASH = 0, 1, 2
+SPGEMM_AUTO, SPGEMM_SORT, SPGEMM_HASH, SPGEMM_DENSE, SPGEMM_ESC, SPGEMM_MTT = 0, 1, 2, 3, 4, 5
+SPGEMM_CSC = 16  # flag: A's columns through a column-pointer array
 UNARY_CONTRACT, UNARY_ATTRACT = 0, 1
 
 _lib = None

@@ -224,7 +224,7 @@ int lnt_context_destroy(lnt_context ctx) {
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
                           &ctx->host_tmp_b, &ctx->host_tmp_c, &ctx->spg_ws, &ctx->spg_tmp,
-                          &ctx->spg_big, &ctx->spg_rank, &ctx->hist_part, &ctx->relin_dev, &ctx->exec_buf[0],
+                          &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->hist_part, &ctx->relin_dev, &ctx->exec_buf[0],
                           &ctx->exec_buf[1], &ctx->exec_buf[2], &ctx->exec_buf[3], &ctx->exec_buf[4],
                           &ctx->exec_buf[5]})
     release(*b);
@@ -322,7 +322,7 @@ int lnt_context_trim(lnt_context ctx) {
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->aux, &ctx->host_tmp_a, &ctx->host_tmp_b, &ctx->host_tmp_c,
-                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big, &ctx->spg_rank, &ctx->exec_buf[0], &ctx->exec_buf[1],
+                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->exec_buf[0], &ctx->exec_buf[1],
                           &ctx->exec_buf[2], &ctx->exec_buf[3], &ctx->exec_buf[4], &ctx->exec_buf[5]})
     release(*b);
   return LNT_OK;
@@ -637,7 +637,8 @@ static int lattice_common(lnt_context ctx, const int64_t *a_keys, const double *
   if (!a_dims || !b_dims) return fail(LNT_ERR_INVALID_ARGUMENT, "null lattice dimensions");
   if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
   if (na >= (int64_t(1) << 32) || nb >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "operands must hold < 2^32 entries");
-  if (strategy < LNT_SPGEMM_AUTO || strategy > LNT_SPGEMM_HASH) return fail(LNT_ERR_INVALID_ARGUMENT, "unknown strategy");
+  if ((strategy & ~LNT_SPGEMM_CSC) < LNT_SPGEMM_AUTO || (strategy & ~LNT_SPGEMM_CSC) > LNT_SPGEMM_MTT)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "unknown strategy");
   if ((na && !a_keys) || (nb && !b_keys) || (!nprod_only && ((na && !a_vals) || (nb && !b_vals))))
     return fail(LNT_ERR_INVALID_ARGUMENT, "null array");
   const uint64_t *ak = (const uint64_t *)a_keys, *bk = (const uint64_t *)b_keys;
@@ -669,7 +670,7 @@ static int lattice_common(lnt_context ctx, const int64_t *a_keys, const double *
     *nprod_only = (int64_t)np;
     return LNT_OK;
   }
-  if (strategy == LNT_SPGEMM_AUTO) strategy = LNT_SPGEMM_SORT;
+  if ((strategy & ~LNT_SPGEMM_CSC) == LNT_SPGEMM_AUTO) strategy |= LNT_SPGEMM_SORT;
   uint64_t count = 0;
   if (mem == LNT_MEM_DEVICE) {
     uint64_t *kalloc = nullptr;

@@ -180,6 +180,8 @@ int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
   LNT_TRY(make_plan(a, b, &P0));
   *shape = P0.shape;
   DigitScope digits(ctx, (s & LNT_STRAT_RADIX6) ? 6 : 8);
+  static const int kSpgemm[LNT_NUM_ACCUMULATORS] = {LNT_SPGEMM_SORT, LNT_SPGEMM_HASH, LNT_SPGEMM_DENSE,
+                                                     LNT_SPGEMM_ESC, LNT_SPGEMM_MTT};
   const uint64_t *fk = nullptr, *gk = nullptr;
   const double *fv = nullptr, *gv = nullptr;
   LNT_TRY(unfold(ctx, f, P.seq_a, ctx->exec_buf[0], ctx->exec_buf[1], &fk, &fv));
@@ -214,7 +216,7 @@ int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
     }
     n = c;
   } else {
-    const int acc = (s & LNT_STRAT_HASH) ? LNT_SPGEMM_HASH : LNT_SPGEMM_SORT;
+    const int acc = kSpgemm[LNT_STRATEGY_ACC(s)];
     int rc = fixed ? lattice_mult_device(ctx, fk, fv, (uint64_t)f->nnz, P.lat_a, gk, gv, (uint64_t)g->nnz, P.lat_b,
                                          acc, k, v, capacity, &n, nullptr, nullptr, nullptr)
                    : lattice_mult_device(ctx, fk, fv, (uint64_t)f->nnz, P.lat_a, gk, gv, (uint64_t)g->nnz, P.lat_b,
@@ -293,10 +295,12 @@ int features_of(const lnt_operand *a, const lnt_operand *b, float *f) {
 // sparse_times_polyalgorithm.m restated on the lattice plan: lexorder tests
 // compare the leading (inner + outer) entries of each operand's current
 // sequence with [outer inner] / [inner outer]; hyper-sparsity flags are
-// size/nnz > 3; `transposed` -> LNT_STRAT_SWAP.  Its algorithm choice (CSC,
-// DCSC, CSCNA, outer product) selects CPU accumulators with no GPU
-// counterpart, so every case runs this library's default accumulator
-// (sort-accumulate) with 8-bit digits.
+// size/nnz > 3; `transposed` -> LNT_STRAT_SWAP.  Its algorithm classes map to
+// the accumulators of the methods they name (specialized_mult.m and
+// sparselattice_mul.cpp:24-67): 1 CSC and 2 DCSC (csc_times) -> DENSE, 0
+// operator* and 3 CSCNA (csc_no_accum) -> SORT, 4 outer product -> ESC
+// (orientation A*B, as the .m file's `transposed = 2` lattices are); 8-bit
+// digits.
 int heuristic_of(const lnt_operand *a, const lnt_operand *b, int *strategy) {
   Plan P;
   LNT_TRY(make_plan(a, b, &P));
@@ -324,32 +328,40 @@ int heuristic_of(const lnt_operand *a, const lnt_operand *b, int *strategy) {
   const bool b_io = lead_is(b, P.b_in, P.n_in, P.b_out, P.n_out_b);
   const bool b_oi = lead_is(b, P.b_out, P.n_out_b, P.b_in, P.n_in);
   // The six-way lexorder choice shared by the CSC, DCSC and CSCNA branches.
-  auto orient = [&](int *transposed) {
+  // *fifth: the fifth case was taken (the .m file's sparse x sparse branch
+  // runs algorithm 0 there, :50-52).
+  auto orient = [&](int *transposed, bool *fifth) {
+    *fifth = false;
     if (a_oi && b_io) *transposed = 0;
     else if (a_io && b_oi) *transposed = 1;
     else if (a_oi && !b_oi) *transposed = 0;
     else if (!a_oi && b_oi) *transposed = 1;
-    else if (a_io && !b_io) *transposed = 1;
+    else if (a_io && !b_io) *transposed = 1, *fifth = true;
     else if (!a_io && b_io) *transposed = 0;
     else *transposed = b_outer > a_outer ? 0 : 1;
   };
   int transposed = 0, algorithm = 1;
+  bool fifth = false;
   if (a_sparse) {
-    if (b_sparse) orient(&transposed);
-    else transposed = 0;
     algorithm = 1;
+    if (b_sparse) {
+      orient(&transposed, &fifth);
+      if (fifth) algorithm = 0;
+    } else {
+      transposed = 0;
+    }
   } else if (b_sparse) {
     transposed = 1;
     algorithm = 1;
   } else if (a_col) {
-    if (b_row) orient(&transposed);
+    if (b_row) orient(&transposed, &fifth);
     else transposed = 0;
     algorithm = 2;
   } else if (b_row) {
     transposed = 1;
     algorithm = 2;
   } else if (a_row) {
-    if (b_col) orient(&transposed);
+    if (b_col) orient(&transposed, &fifth);
     else transposed = 0;
     algorithm = 3;
   } else if (b_col) {
@@ -359,8 +371,8 @@ int heuristic_of(const lnt_operand *a, const lnt_operand *b, int *strategy) {
     transposed = 2;
     algorithm = 4;
   }
-  (void)algorithm;
-  *strategy = transposed == 1 ? LNT_STRAT_SWAP : 0;
+  static const int kAcc[5] = {LNT_ACC_SORT, LNT_ACC_DENSE, LNT_ACC_DENSE, LNT_ACC_SORT, LNT_ACC_ESC};
+  *strategy = LNT_STRATEGY(kAcc[algorithm], 0, transposed == 1);
   return LNT_OK;
 }
 

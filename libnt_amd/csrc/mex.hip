@@ -228,8 +228,14 @@ int lnt_mex_sparselattice_mult(lnt_context ctx, const double *a_data, const int6
   // (:24-67).
   if (!(algorithm > -1.0 && algorithm < 6.0))
     return fail(LNT_ERR_INVALID_ARGUMENT, "algorithm must be between 0 and 5");
-  static const int kStrategy[6] = {LNT_SPGEMM_AUTO, LNT_SPGEMM_HASH, LNT_SPGEMM_HASH,
-                                   LNT_SPGEMM_SORT, LNT_SPGEMM_AUTO, LNT_SPGEMM_AUTO};
+  // The method each case runs: 0 operator* = csc_no_accum<true> (DCSC), 1
+  // csc_times<false> (CSC), 2 csc_times<true> (DCSC), 3 csc_no_accum<false>
+  // (CSC), 5 mtt_times.  Case 4 runs outer_times and then falls through into
+  // case 5 (:52-58, no break), which rebuilds both lattices from the same
+  // arguments as column-major lattices and overwrites the result with
+  // mtt_times's: algorithm 4 returns what algorithm 5 returns.
+  static const int kStrategy[6] = {LNT_SPGEMM_SORT, LNT_SPGEMM_DENSE | LNT_SPGEMM_CSC, LNT_SPGEMM_DENSE,
+                                   LNT_SPGEMM_SORT | LNT_SPGEMM_CSC, LNT_SPGEMM_MTT, LNT_SPGEMM_MTT};
   const int strategy = kStrategy[(int)algorithm];
   Staging st;
   LNT_TRY(stage_in(ctx, st, 0, a_idx, a_data, na));

@@ -23,6 +23,13 @@
 //                           onesweep radix sort, run sums;
 //   4. scan of per-segment output counts, then one coalesced write of C with
 //      the lattice keys (row + h * (t*bw + j)).
+// The other accumulators of the reference's strategy set: the dense sparse
+// accumulator of csc_times (:2224-2351) and of mtt_times over compressed rows
+// (:1579-1813) take every multi-entry segment in LDS when the index space
+// fits (dense_kernel); outer_times's expand-sort-collect (:2034-2183) sends
+// every multi-entry segment through the global sort.  A's columns are found by
+// a binary search in its keys (DCSC) or through a column-pointer array (CSC,
+// MultHelper<.,false>, :121-169).
 #include <string.h>
 
 #include <algorithm>
@@ -69,9 +76,19 @@ struct LatticeDims {
   Divider b_h;              // bh (= aw)
 };
 
-// 1. one thread per B entry.
+// A's column pointers (CSC): cp[c] = first A entry of global column c = col +
+// w*tab, for every c in [0, w*d], by one search per column.
+__global__ void col_ptr_kernel(const uint64_t *__restrict__ ak, uint64_t na, uint64_t ah, uint64_t ncols,
+                               uint64_t *__restrict__ cp) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= ncols; c += (uint64_t)gridDim.x * blockDim.x)
+    cp[c] = lower_bound_u64(ak, na, c * ah);
+}
+
+// 1. one thread per B entry.  cp: A's column pointers (CSC), or null to
+// search A's keys for every entry (DCSC).
 __global__ void entry_kernel(const uint64_t *__restrict__ ak, uint64_t na,
                              const uint64_t *__restrict__ bk, uint64_t nb, LatticeDims L,
+                             const uint64_t *__restrict__ cp,
                              uint64_t *__restrict__ a_lo, uint32_t *__restrict__ a_len,
                              uint64_t *__restrict__ a_base, uint64_t *__restrict__ colid,
                              uint32_t *__restrict__ head) {
@@ -83,8 +100,14 @@ __global__ void entry_kernel(const uint64_t *__restrict__ ak, uint64_t na,
     uint64_t col = divide(rem, L.b_h);
     uint64_t k = rem - col * L.b_h.d;
     uint64_t start = t * L.ah * L.aw + k * L.ah;
-    uint64_t lo = lower_bound_u64(ak, na, start);
-    uint64_t hi = lower_bound_u64(ak, na, start + L.ah);
+    uint64_t lo, hi;
+    if (cp) {
+      lo = cp[t * L.aw + k];
+      hi = cp[t * L.aw + k + 1];
+    } else {
+      lo = lower_bound_u64(ak, na, start);
+      hi = lower_bound_u64(ak, na, start + L.ah);
+    }
     a_lo[i] = lo;
     a_len[i] = (uint32_t)(hi - lo);
     a_base[i] = start;
@@ -287,6 +310,76 @@ __global__ __launch_bounds__(kBlock) void small_hash_kernel(
   for (uint32_t i = tid; i < total; i += kBlock) {
     tmp_row[base + i] = s_key[i];
     tmp_val[base + i] = s_val[i];
+  }
+  if (tid == 0) seg_u[s] = total;
+}
+
+// 3a''. dense accumulation (the sparse accumulator of csc_times, or of
+// mtt_times over row ranks): one workgroup per segment with an accumulator of
+// R <= kDenseRows doubles and a presence bitmap in LDS, indexed by the row (or
+// by its rank among A's rows when RANKS).  The segment's B entries are taken
+// in order; the rows of one A column are distinct, so the threads of one entry
+// never meet on an index, and a barrier separates entries.  A row's first
+// product is stored and later ones added: each sum is the same chain of
+// additions, in ascending B row order, as the sort accumulator's, so the
+// values are bit-identical to it.  Reading the bitmap in order gives the rows
+// sorted, with no sort.
+constexpr uint32_t kDenseRows = 8192;
+
+template <bool RANKS>
+__global__ __launch_bounds__(kBlock) void dense_kernel(
+    const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
+    const uint64_t *__restrict__ a_lo, const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
+    const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb,
+    const uint32_t *__restrict__ list2, uint32_t R, const uint32_t *__restrict__ arank,
+    const uint32_t *__restrict__ rank_rows, uint64_t *__restrict__ seg_u, uint64_t *__restrict__ tmp_row,
+    double *__restrict__ tmp_val) {
+  __shared__ double s_acc[kDenseRows];
+  __shared__ uint32_t s_bits[kDenseRows / 32];
+  __shared__ uint32_t s_scan[kBlock / kWave];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t s = list2[blockIdx.x];
+  const uint64_t first = seg_first[s];
+  const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+  const uint64_t base = prod_off[first];
+  const uint32_t words = (R + 31) / 32;
+  for (uint32_t w = tid; w < words; w += kBlock) s_bits[w] = 0;
+  __syncthreads();
+  for (uint64_t e = first; e < last; ++e) {
+    const uint64_t lo = a_lo[e], ab = a_base[e];
+    const uint32_t len = a_len[e];
+    const double beta = bv[e];
+    for (uint32_t i = tid; i < len; i += kBlock) {
+      const uint32_t idx = RANKS ? arank[lo + i] : (uint32_t)(ak[lo + i] - ab);
+      const double v = beta * av[lo + i];
+      const uint32_t bit = 1u << (idx & 31);
+      const uint32_t old = atomicOr(&s_bits[idx >> 5], bit);
+      if (old & bit) s_acc[idx] += v;
+      else s_acc[idx] = v;
+    }
+    __syncthreads();
+  }
+  // Thread t reads bitmap words t*per .. t*per + per - 1 (R <= 8192: per <= 1).
+  constexpr uint32_t kPer = kDenseRows / 32 / kBlock;
+  uint32_t mine[kPer], cnt = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kPer; ++q) {
+    const uint32_t w = tid * kPer + q;
+    mine[q] = w < words ? s_bits[w] : 0u;
+    cnt += (uint32_t)__popc(mine[q]);
+  }
+  uint32_t total;
+  uint32_t rank = block_exclusive_sum_bt<kBlock>(cnt, s_scan, &total);
+#pragma unroll
+  for (uint32_t q = 0; q < kPer; ++q) {
+    uint32_t bits = mine[q];
+    while (bits) {
+      const uint32_t idx = (tid * kPer + q) * 32 + (uint32_t)(__ffs(bits) - 1);
+      bits &= bits - 1;
+      tmp_row[base + rank] = RANKS ? rank_rows[idx] : idx;
+      tmp_val[base + rank] = s_acc[idx];
+      ++rank;
+    }
   }
   if (tid == 0) seg_u[s] = total;
 }
@@ -555,8 +648,12 @@ struct EntryArrays {
   size_t scan_words;
 };
 
+// Largest A column count (w*d) for which the CSC column-pointer array is
+// built; beyond it the lookup searches A's keys (the DCSC form).
+constexpr uint64_t kMaxPtrColumns = uint64_t(1) << 26;
+
 static int entry_phase(lnt_context ctx, const uint64_t *ak, uint64_t na, const int64_t a_dims[3],
-                       const uint64_t *bk, uint64_t nb, const int64_t b_dims[3], EntryArrays *E) {
+                       const uint64_t *bk, uint64_t nb, const int64_t b_dims[3], bool csc, EntryArrays *E) {
   size_t scan_words = scan_workspace_words(nb + 1);
   size_t bytes = (nb + 1) * (8 * 10 + 4 * 4) + 16 * 256 + 64 + scan_words * 8 + 8192;
   LNT_TRY(ensure(ctx->spg_ws, bytes));
@@ -583,8 +680,55 @@ static int entry_phase(lnt_context ctx, const uint64_t *ak, uint64_t na, const i
   L.bw = (uint64_t)b_dims[1];
   L.b_tab = make_divider((uint64_t)b_dims[0] * (uint64_t)b_dims[1]);
   L.b_h = make_divider((uint64_t)b_dims[0]);
-  entry_kernel<<<grid_for(nb), 256, 0, ctx->stream>>>(ak, na, bk, nb, L, E->a_lo, E->a_len, E->a_base,
+  const uint64_t *cp = nullptr;
+  const uint64_t ncols = (uint64_t)a_dims[1] * (uint64_t)a_dims[2];
+  if (csc && ncols <= kMaxPtrColumns) {
+    LNT_TRY(ensure(ctx->spg_cp, (ncols + 1) * 8));
+    col_ptr_kernel<<<grid_for(ncols + 1), 256, 0, ctx->stream>>>(ak, na, L.ah, ncols, (uint64_t *)ctx->spg_cp.ptr);
+    LNT_HIP(hipGetLastError());
+    cp = (const uint64_t *)ctx->spg_cp.ptr;
+  }
+  entry_kernel<<<grid_for(nb), 256, 0, ctx->stream>>>(ak, na, bk, nb, L, cp, E->a_lo, E->a_len, E->a_base,
                                                         E->colid, E->head);
+  LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+// Ranks of A's rows among the rows present (the row compression of
+// mtt_times, SparseLatticeBase.h:1697-1716, over the whole lattice: the map is
+// monotone, so per-tab and global ranks order every column's rows alike).
+struct RankTables {
+  uint32_t *bm = nullptr, *cnt = nullptr, *rows = nullptr, *ar = nullptr;
+  uint64_t *pre = nullptr;
+  uint64_t nw = 0;
+};
+
+// Bitmap of the rows present and the prefix of its word popcounts;
+// *nrank = the number of distinct rows.  Needs h <= 2^31.
+static int rank_count(lnt_context ctx, const uint64_t *ak, uint64_t na, uint64_t ah, size_t scan_words,
+                      RankTables *t, uint64_t *nrank) {
+  const uint64_t nw = (ah + 31) / 32;
+  LNT_TRY(ensure(ctx->spg_rank, nw * 8 + (nw + 1) * 8 + na * 8 + 1024));
+  Carve c{(char *)ctx->spg_rank.ptr};
+  t->nw = nw;
+  t->bm = c.take<uint32_t>(nw);
+  t->cnt = c.take<uint32_t>(nw);
+  t->pre = c.take<uint64_t>(nw + 1);
+  t->rows = c.take<uint32_t>(na);
+  t->ar = c.take<uint32_t>(na);
+  LNT_TRY(ensure(ctx->aux, std::max(scan_words, scan_workspace_words(nw)) * 8 + 64));
+  LNT_TRY(launch_fill_u32(ctx, t->bm, 0, nw));
+  row_bitmap_kernel<<<grid_for(na), 256, 0, ctx->stream>>>(ak, na, make_divider(ah), t->bm);
+  word_popc_kernel<<<grid_for(nw), 256, 0, ctx->stream>>>(t->bm, nw, t->cnt);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, t->cnt, t->pre, nw, t->pre + nw, ctx->aux, 0));
+  return read_u64(ctx, t->pre + nw, nrank, 1);
+}
+
+// rows[rank] = row, ar[e] = rank of A entry e's row (after rank_count).
+static int rank_tables(lnt_context ctx, const uint64_t *ak, uint64_t na, uint64_t ah, const RankTables &t) {
+  rank_rows_kernel<<<grid_for(t.nw), 256, 0, ctx->stream>>>(t.bm, t.pre, t.nw, t.rows);
+  a_rank_kernel<<<grid_for(na), 256, 0, ctx->stream>>>(ak, na, make_divider(ah), t.bm, t.pre, t.ar);
   LNT_HIP(hipGetLastError());
   return LNT_OK;
 }
@@ -595,7 +739,7 @@ int lattice_nprod_device(lnt_context ctx, const uint64_t *ak, uint64_t na, const
   *nprod_host = 0;
   if (na == 0 || nb == 0) return LNT_OK;
   EntryArrays E;
-  LNT_TRY(entry_phase(ctx, ak, na, a_dims, bk, nb, b_dims, &E));
+  LNT_TRY(entry_phase(ctx, ak, na, a_dims, bk, nb, b_dims, false, &E));
   // The scan workspace lives after out_off's region in the same carve: reuse
   // ctx->aux for it.
   LNT_TRY(ensure(ctx->aux, E.scan_words * 8 + 64));
@@ -613,9 +757,10 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
   if (nprod_host) *nprod_host = 0;
   if (na == 0 || nb == 0) return LNT_OK;
   const uint64_t ah = (uint64_t)a_dims[0];
+  const int acc = strategy & ~LNT_SPGEMM_CSC;
   int slot = prof_begin(ctx, kProfLatticeMult, 16.0 * ((double)na + nb));
   EntryArrays E;
-  LNT_TRY(entry_phase(ctx, ak, na, a_dims, bk, nb, b_dims, &E));
+  LNT_TRY(entry_phase(ctx, ak, na, a_dims, bk, nb, b_dims, (strategy & LNT_SPGEMM_CSC) != 0, &E));
   LNT_TRY(ensure(ctx->aux, E.scan_words * 8 + 64));
   LNT_TRY(exclusive_scan<uint32_t>(ctx, E.a_len, E.prod_off, nb, E.prod_off + nb, ctx->aux, 0));
   LNT_TRY(exclusive_scan<uint32_t>(ctx, E.head, E.seg_idx, nb, E.seg_idx + nb, ctx->aux, 0));
@@ -632,10 +777,29 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     prof_end(ctx, slot);
     return LNT_OK;
   }
+  // The dense accumulator takes every multi-entry segment when its index
+  // space fits in LDS: the rows (h <= kDenseRows), or for mtt_times the ranks
+  // of A's rows among the rows present.  Otherwise those strategies use the
+  // sort accumulator.  The expand-sort-compress strategy (outer_times) sends
+  // every multi-entry segment through the global sort.
+  bool dense = false;
+  RankTables rt{};
+  uint64_t nrank = 0;
+  if (acc == LNT_SPGEMM_MTT && ah <= (uint64_t(1) << 31)) {
+    LNT_TRY(rank_count(ctx, ak, na, ah, E.scan_words, &rt, &nrank));
+    if (nrank <= kDenseRows) {
+      LNT_TRY(rank_tables(ctx, ak, na, ah, rt));
+      dense = true;
+    }
+  } else if (acc == LNT_SPGEMM_DENSE && ah <= kDenseRows) {
+    dense = true;
+  }
+  const uint32_t small_max = dense ? 0xFFFFFFFFu
+                             : acc == LNT_SPGEMM_ESC ? 0u
+                                                     : (uint32_t)std::min(ctx->tuning.spgemm_small, kSmall);
   seg_first_kernel<<<grid_for(nb), 256, 0, ctx->stream>>>(E.head, E.seg_idx, nb, E.seg_first);
   LNT_TRY(launch_fill_u32(ctx, E.counts, 0, 4));
-  classify_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.seg_first, nseg, nb, E.prod_off,
-                                                            (uint32_t)std::min(ctx->tuning.spgemm_small, kSmall),
+  classify_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.seg_first, nseg, nb, E.prod_off, small_max,
                                                             E.seg_u, E.list2, E.list3, E.counts);
   LNT_HIP(hipGetLastError());
   LNT_HIP(hipMemcpyAsync(ctx->pinned + 4, E.counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -644,8 +808,18 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
   if (n2 || n3) LNT_TRY(ensure(ctx->spg_tmp, nprod * 16));
   uint64_t *tmp_row = (uint64_t *)ctx->spg_tmp.ptr;
   double *tmp_val = tmp_row ? (double *)(tmp_row + nprod) : nullptr;
-  if (n2) {
-    if (strategy == LNT_SPGEMM_HASH)
+  if (n2 && dense) {
+    if (acc == LNT_SPGEMM_MTT)
+      dense_kernel<true><<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_len, E.a_base, E.prod_off,
+                                                         E.seg_first, nseg, nb, E.list2, (uint32_t)nrank, rt.ar,
+                                                         rt.rows, E.seg_u, tmp_row, tmp_val);
+    else
+      dense_kernel<false><<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_len, E.a_base, E.prod_off,
+                                                          E.seg_first, nseg, nb, E.list2, (uint32_t)ah, nullptr,
+                                                          nullptr, E.seg_u, tmp_row, tmp_val);
+    LNT_HIP(hipGetLastError());
+  } else if (n2) {
+    if (acc == LNT_SPGEMM_HASH)
       small_hash_kernel<<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first,
                                                           nseg, nb, E.list2, E.seg_u, tmp_row, tmp_val);
     else
@@ -659,22 +833,7 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     uint64_t width = ah;
     const uint32_t *arank = nullptr, *rrows = nullptr;
     if (ctx->tuning.row_ranks && nprod >= (uint64_t(1) << 20) && ah <= (uint64_t(1) << 31)) {
-      const uint64_t nw = (ah + 31) / 32;
-      LNT_TRY(ensure(ctx->spg_rank, nw * 8 + (nw + 1) * 8 + na * 8 + 1024));
-      Carve c{(char *)ctx->spg_rank.ptr};
-      uint32_t *bm = c.take<uint32_t>(nw);
-      uint32_t *cnt = c.take<uint32_t>(nw);
-      uint64_t *pre = c.take<uint64_t>(nw + 1);
-      uint32_t *rows = c.take<uint32_t>(na);
-      uint32_t *ar = c.take<uint32_t>(na);
-      LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan_workspace_words(nw)) * 8 + 64));
-      LNT_TRY(launch_fill_u32(ctx, bm, 0, nw));
-      row_bitmap_kernel<<<grid_for(na), 256, 0, ctx->stream>>>(ak, na, make_divider(ah), bm);
-      word_popc_kernel<<<grid_for(nw), 256, 0, ctx->stream>>>(bm, nw, cnt);
-      LNT_HIP(hipGetLastError());
-      LNT_TRY(exclusive_scan<uint32_t>(ctx, cnt, pre, nw, pre + nw, ctx->aux, 0));
-      uint64_t nrank;
-      LNT_TRY(read_u64(ctx, pre + nw, &nrank, 1));
+      if (!rt.bm) LNT_TRY(rank_count(ctx, ak, na, ah, E.scan_words, &rt, &nrank));
       auto bits = [](unsigned __int128 v) {
         uint32_t b = 0;
         while (b < 128 && ((unsigned __int128)1 << b) < v) ++b;
@@ -682,12 +841,10 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
       };
       const uint32_t old_bits = bits((unsigned __int128)n3 * ah), new_bits = bits((unsigned __int128)n3 * nrank);
       if ((new_bits + 7) / 8 < (old_bits + 7) / 8 || (new_bits <= 32 && old_bits > 32)) {
-        rank_rows_kernel<<<grid_for(nw), 256, 0, ctx->stream>>>(bm, pre, nw, rows);
-        a_rank_kernel<<<grid_for(na), 256, 0, ctx->stream>>>(ak, na, make_divider(ah), bm, pre, ar);
-        LNT_HIP(hipGetLastError());
+        LNT_TRY(rank_tables(ctx, ak, na, ah, rt));
         width = nrank;
-        arank = ar;
-        rrows = rows;
+        arank = rt.ar;
+        rrows = rt.rows;
       }
     }
     // Composite keys r*width + row must stay below 2^63; otherwise the large

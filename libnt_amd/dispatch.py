@@ -20,9 +20,12 @@ import numpy as np
 from . import MEM_DEVICE, _i64p, _vp, check, lib
 
 MAX_ORDER = 16
-NUM_STRATEGIES = 8
 NUM_FEATURES = 18
-STRAT_SWAP, STRAT_HASH, STRAT_RADIX6 = 1, 2, 4
+STRAT_SWAP, STRAT_RADIX6 = 1, 2
+ACC_SORT, ACC_HASH, ACC_DENSE, ACC_ESC, ACC_MTT = range(5)
+ACC_NAMES = ("sort", "hash", "dense", "esc", "mtt")
+NUM_STRATEGIES = 4 * len(ACC_NAMES)
+ALL_STRATEGIES = (1 << NUM_STRATEGIES) - 1
 STRAT_AUTO = -1
 HIDDEN = 64
 FEATURE_NAMES = [
@@ -32,9 +35,17 @@ FEATURE_NAMES = [
 ]
 
 
+def strategy(acc: int, radix6: bool = False, swap: bool = False) -> int:
+    """``LNT_STRATEGY(acc, radix6, swap)`` of include/lnt_dispatch.h."""
+    return (acc << 2) | (STRAT_RADIX6 if radix6 else 0) | (STRAT_SWAP if swap else 0)
+
+
+def strategy_acc(s: int) -> int:
+    return s >> 2
+
+
 def strategy_name(s: int) -> str:
-    return "%s/%s/r%d" % ("BA" if s & STRAT_SWAP else "AB", "hash" if s & STRAT_HASH else "sort",
-                          6 if s & STRAT_RADIX6 else 8)
+    return "%s/%s/r%d" % ("BA" if s & STRAT_SWAP else "AB", ACC_NAMES[s >> 2], 6 if s & STRAT_RADIX6 else 8)
 
 
 class Operand(ctypes.Structure):
@@ -120,7 +131,7 @@ def heuristic(a, la: str, b, lb: str) -> int:
     return s.value
 
 
-def time_strategies(ctx, a, la: str, b, lb: str, mask: int = 0xFF, reps: int = 1) -> np.ndarray:
+def time_strategies(ctx, a, la: str, b, lb: str, mask: int = ALL_STRATEGIES, reps: int = 1) -> np.ndarray:
     """Wall ms of each strategy in mask (min over reps after a warm-up); -1
     where not measured."""
     oa, ob = operand(a, la), operand(b, lb)
@@ -329,8 +340,8 @@ def materialize(ctx, spec: Spec, seed: int):
 
 
 def collect_timings(ctx, specs: Sequence[Spec], seed: int, reps: int = 1, log=None):
-    """Features (n x 18, float32) and log2 ms targets (n x 8, float32) of the
-    specs, every strategy measured."""
+    """Features (n x 18, float32) and log2 ms targets (n x NUM_STRATEGIES,
+    float32) of the specs, every strategy measured."""
     X = np.zeros((len(specs), NUM_FEATURES), dtype=np.float32)
     T = np.zeros((len(specs), NUM_STRATEGIES), dtype=np.float32)
     H = np.zeros(len(specs), dtype=np.int32)
@@ -338,7 +349,7 @@ def collect_timings(ctx, specs: Sequence[Spec], seed: int, reps: int = 1, log=No
         A, B = materialize(ctx, sp, seed + i)
         X[i] = features(A, sp.la, B, sp.lb)
         H[i] = heuristic(A, sp.la, B, sp.lb)
-        ms = time_strategies(ctx, A, sp.la, B, sp.lb, 0xFF, reps)
+        ms = time_strategies(ctx, A, sp.la, B, sp.lb, ALL_STRATEGIES, reps)
         T[i] = np.log2(np.maximum(ms, 1e-3))
         if log and (i + 1) % 200 == 0:
             log("%d/%d instances timed" % (i + 1, len(specs)))

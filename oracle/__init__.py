@@ -79,6 +79,10 @@ def lib():
             "lnto_outer_inter_product": (i64, [_i64p, _f64p, i64, i64, _i64p, _f64p, i64, i64, _i64p, _f64p]),
             "lnto_lattice_nprod": (i64, [_i64p, i64, i64, i64, _i64p, i64, i64, i64]),
             "lnto_lattice_mult": (i64, [_i64p, _f64p, i64, i64, i64, i64, _i64p, _f64p, i64, i64, i64, i64, _i64p, _f64p, i64]),
+            "lnto_lattice_mult_spa": (i64, [_i64p, _f64p, i64, i64, i64, _i64p, _f64p, i64, i64, i64, i32, _i64p,
+                                            _f64p, i64]),
+            "lnto_lattice_mult_outer": (i64, [_i64p, _f64p, i64, i64, i64, _i64p, _f64p, i64, i64, i64, _i64p,
+                                              _f64p, i64]),
             "lnto_contract_plan": (i32, [i32, _i64p, ctypes.c_char_p, i32, _i64p, ctypes.c_char_p, _i32p, _i32p, _i64p, _i64p, _i64p, ctypes.POINTER(ctypes.c_int)]),
             "lnto_is_equal_fuzzy": (i32, [f64, f64, f64]),
             "lnto_compare_sparse": (i32, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, f64]),
@@ -267,6 +271,27 @@ def lattice_mult(ak, av, a_dims3, bk, bv, b_dims3):
     return ck[:n].copy(), cv[:n].copy()
 
 
+def lattice_mult_by(method, ak, av, a_dims3, bk, bv, b_dims3):
+    """The lattice product by one of the reference's other methods: "spa"
+    (csc_times), "mtt" (mtt_times, marker zeroed per tab) or "outer"
+    (outer_times).  Both operands sorted (column major lattice keys)."""
+    ak, av, bk, bv = _i64(ak), _f64(av), _i64(bk), _f64(bv)
+    cap = max(lattice_nprod(ak, a_dims3, bk, b_dims3), 1)
+    ck = np.empty(cap, dtype=np.int64)
+    cv = np.empty(cap, dtype=np.float64)
+    args = (_p(ak, _i64p), _p(av, _f64p), len(ak), a_dims3[0], a_dims3[1], _p(bk, _i64p), _p(bv, _f64p), len(bk),
+            b_dims3[0], b_dims3[1])
+    if method in ("spa", "mtt"):
+        n = lib().lnto_lattice_mult_spa(*args, int(method == "mtt"), _p(ck, _i64p), _p(cv, _f64p), cap)
+    elif method == "outer":
+        n = lib().lnto_lattice_mult_outer(*args, _p(ck, _i64p), _p(cv, _f64p), cap)
+    else:
+        raise ValueError("unknown method %r" % method)
+    if n < 0:
+        raise ValueError("lattice_mult_by: dimension mismatch")
+    return ck[:n].copy(), cv[:n].copy()
+
+
 def contract_plan(dims_a, labels_a, dims_b, labels_b):
     """Returns (seq_a, seq_b, lat_a, lat_b, dims_c) for A(labels_a)*B(labels_b)."""
     dims_a, dims_b = _i64(dims_a), _i64(dims_b)
@@ -284,12 +309,19 @@ def contract_plan(dims_a, labels_a, dims_b, labels_b):
     return seq_a, seq_b, lat_a, lat_b, dims_c[: order_c.value].copy()
 
 
-def contract(ak, av, dims_a, seq_a_cur, a_sorted, labels_a, bk, bv, dims_b, seq_b_cur, b_sorted, labels_b):
-    """Full MIA contraction; returns (ck, cv, dims_c).  Inputs are not modified."""
+def contract(ak, av, dims_a, seq_a_cur, a_sorted, labels_a, bk, bv, dims_b, seq_b_cur, b_sorted, labels_b,
+             method="csc_no_accum"):
+    """Full MIA contraction; returns (ck, cv, dims_c).  Inputs are not modified.
+    method: the lattice product -- "csc_no_accum" (operator*), or "spa",
+    "mtt", "outer" (lattice_mult_by)."""
     seq_a, seq_b, lat_a, lat_b, dims_c = contract_plan(dims_a, labels_a, dims_b, labels_b)
     ak2, av2 = sort_seq(ak, av, dims_a, seq_a_cur, seq_a, a_sorted)
     bk2, bv2 = sort_seq(bk, bv, dims_b, seq_b_cur, seq_b, b_sorted)
-    ck, cv = lattice_mult(ak2, av2, tuple(int(x) for x in lat_a), bk2, bv2, tuple(int(x) for x in lat_b))
+    la, lb = tuple(int(x) for x in lat_a), tuple(int(x) for x in lat_b)
+    if method == "csc_no_accum":
+        ck, cv = lattice_mult(ak2, av2, la, bk2, bv2, lb)
+    else:
+        ck, cv = lattice_mult_by(method, ak2, av2, la, bk2, bv2, lb)
     return ck, cv, dims_c
 
 
@@ -345,6 +377,9 @@ def allclose_fuzzy(a, b, prec=1e-12) -> bool:
 
 
 # ------------------------------------------------------------------ cost model
+MLP_OUTPUTS = 20  # LNTO_MLP_S: one predicted log2 time per strategy
+
+
 def mlp_param_count() -> int:
     return int(lib().lnto_mlp_param_count())
 
@@ -354,10 +389,10 @@ def mlp_weight_count() -> int:
 
 
 def mlp_forward(params, X) -> np.ndarray:
-    """Predictions (n x 8) of the dispatch model for features X (n x 18)."""
+    """Predictions (n x MLP_OUTPUTS) of the dispatch model for features X (n x 18)."""
     P = _f64(params)
     X = np.ascontiguousarray(X, dtype=np.float64)
-    out = np.empty((X.shape[0], 8))
+    out = np.empty((X.shape[0], MLP_OUTPUTS))
     for i in range(X.shape[0]):
         lib().lnto_mlp_forward(_p(P, _f64p), _p(X[i], _f64p), _p(out[i], _f64p))
     return out

@@ -655,6 +655,173 @@ int64_t lnto_lattice_mult(const int64_t *ak, const double *av, int64_t na,
   return c;
 }
 
+/* SparseLatticeBase.h:2224-2351 csc_times (and its scatter, mult_scatter
+ * :2186-2220): a sparse accumulator (SPA) per tab -- row_marker[height],
+ * reset per tab (:2283), and data_collector[height].  For each column j of B
+ * (key order) and each B entry (k, j, beta) in key order, every row r of
+ * A(:,k): if row_marker[r] < j+1 the row is new in this column (marker set,
+ * row appended, collector[r] = beta*a), else collector[r] += beta*a.  The
+ * appended rows are sorted (std::sort, :2322) and read out of the collector.
+ * A row's sum therefore runs in ascending B row order, first product
+ * unadded.  compress = 1 is mtt_times (:1579-1813): rows are first replaced
+ * by their rank among the distinct rows of A's tab (a_row_map, :1697-1716) and
+ * mapped back on output (:1800); the sums are the same operations.  The
+ * reference's mtt_times re-purposes inner_index_map as its row marker with a
+ * resize that keeps stale inner-index values (:1770-1771) instead of zeros, so
+ * it can drop products of early columns; this restatement zeroes the marker
+ * per tab, the evident intent.  CSC and DCSC column lookups (MultHelper
+ * <.,false/true>, :121-169) only change how A(:,k) is found: both appear here
+ * as one binary search.  Returns nnz(C), or -1 (mismatch or cap too small). */
+static int cmp_i64(const void *x, const void *y) {
+  const int64_t a = *(const int64_t *)x, b = *(const int64_t *)y;
+  return (a > b) - (a < b);
+}
+
+int64_t lnto_lattice_mult_spa(const int64_t *ak, const double *av, int64_t na, int64_t ah, int64_t aw,
+                              const int64_t *bk, const double *bv, int64_t nb, int64_t bh, int64_t bw,
+                              int compress, int64_t *ck, double *cv, int64_t cap) {
+  if (aw != bh) return -1;
+  const int64_t a_tab_size = ah * aw, b_tab_size = bh * bw;
+  int64_t *marker = NULL, *rowmap = NULL, *rank = NULL;
+  double *collector = NULL;
+  int64_t c = 0, q = 0, rc = 0;
+  int64_t space = ah;
+  if (compress) {
+    rowmap = (int64_t *)malloc((size_t)(na > 0 ? na : 1) * sizeof(int64_t));
+    rank = (int64_t *)malloc((size_t)(na > 0 ? na : 1) * sizeof(int64_t));
+    if (!rowmap || !rank) { rc = -1; goto out; }
+  }
+  while (q < nb) {
+    const int64_t tab = bk[q] / b_tab_size;
+    const int64_t q_end = lower_bound_i64(bk, nb, (tab + 1) * b_tab_size);
+    const int64_t a0 = lower_bound_i64(ak, na, tab * a_tab_size), a1 = lower_bound_i64(ak, na, (tab + 1) * a_tab_size);
+    int64_t nrows = 0;
+    if (compress) {
+      /* a_row_map: the distinct rows of the tab's A entries, ascending; rank[p]
+       * = position of A entry p's row in it. */
+      for (int64_t p = a0; p < a1; ++p) rowmap[p - a0] = ak[p] % ah;
+      qsort(rowmap, (size_t)(a1 - a0), sizeof(int64_t), cmp_i64);
+      for (int64_t p = 0; p < a1 - a0; ++p)
+        if (p == 0 || rowmap[p] != rowmap[nrows - 1]) rowmap[nrows++] = rowmap[p];
+      for (int64_t p = a0; p < a1; ++p) {
+        int64_t lo = 0, hi = nrows, r = ak[p] % ah;
+        while (lo < hi) { int64_t mid = (lo + hi) / 2; if (rowmap[mid] < r) lo = mid + 1; else hi = mid; }
+        rank[p] = lo;
+      }
+      space = nrows > 0 ? nrows : 1;
+    }
+    free(marker);
+    free(collector);
+    marker = (int64_t *)calloc((size_t)space, sizeof(int64_t));
+    collector = (double *)malloc((size_t)space * sizeof(double));
+    if (!marker || !collector) { rc = -1; goto out; }
+    while (q < q_end) {
+      const int64_t col = (bk[q] % b_tab_size) / bh;
+      const int64_t col_end = tab * b_tab_size + (col + 1) * bh;
+      const int64_t start = c;
+      for (; q < q_end && bk[q] < col_end; ++q) {
+        const int64_t krow = bk[q] % bh;
+        const int64_t lo = lower_bound_i64(ak + a0, a1 - a0, tab * a_tab_size + krow * ah) + a0;
+        const int64_t hi = lower_bound_i64(ak + a0, a1 - a0, tab * a_tab_size + (krow + 1) * ah) + a0;
+        for (int64_t p = lo; p < hi; ++p) {
+          const int64_t r = compress ? rank[p] : ak[p] % ah;
+          if (marker[r] < col + 1) {
+            if (c >= cap) { rc = -1; goto out; }
+            marker[r] = col + 1;
+            ck[c++] = r;
+            collector[r] = bv[q] * av[p];
+          } else {
+            collector[r] += bv[q] * av[p];
+          }
+        }
+      }
+      qsort(ck + start, (size_t)(c - start), sizeof(int64_t), cmp_i64);
+      const int64_t off = tab * ah * bw + ah * col;
+      for (int64_t p = start; p < c; ++p) {
+        cv[p] = collector[ck[p]];
+        ck[p] = (compress ? rowmap[ck[p]] : ck[p]) + off;
+      }
+    }
+  }
+  rc = c;
+out:
+  free(marker);
+  free(collector);
+  free(rowmap);
+  free(rank);
+  return rc;
+}
+
+/* SparseLatticeBase.h:2034-2183 outer_times: A sorted column major, B sorted
+ * row major (:2062-2063; given here column major and re-keyed to row major,
+ * key = col + row*bw + tab*bh*bw, then sorted -- the order b.sort(RowMajor)
+ * produces).  Per tab, A's columns and B's rows are walked together; for every
+ * k present in both, every B entry of row k (ascending column) times every A
+ * entry of column k (ascending row) is appended with key row + col*h and value
+ * a*b (:2136-2146); the tab's products are sorted by the in-place MSD radix
+ * with max_size h*bw (:2160), duplicates summed by collect_duplicates
+ * (:2162) and the tab offset added (:2165-2166).  The reference's column /
+ * row skips dereference one entry past the tab before testing the bound
+ * (:2113, :2116); here the bound is tested first, with the same outcome. */
+int64_t lnto_lattice_mult_outer(const int64_t *ak, const double *av, int64_t na, int64_t ah, int64_t aw,
+                                const int64_t *bk, const double *bv, int64_t nb, int64_t bh, int64_t bw,
+                                int64_t *ck, double *cv, int64_t cap) {
+  if (aw != bh) return -1;
+  const int64_t a_tab_size = ah * aw, b_tab_size = bh * bw, c_tab_size = ah * bw;
+  int64_t *rk = (int64_t *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(int64_t));
+  double *rv = (double *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(double));
+  if (!rk || !rv) { free(rk); free(rv); return -1; }
+  for (int64_t q = 0; q < nb; ++q) {
+    const int64_t tab = bk[q] / b_tab_size, row = bk[q] % bh, col = (bk[q] % b_tab_size) / bh;
+    rk[q] = col + row * bw + tab * b_tab_size;
+    rv[q] = bv[q];
+  }
+  int64_t top = 0;
+  for (int64_t q = 0; q < nb; ++q) top = rk[q] > top ? rk[q] : top;
+  if (nb > 1) lnto_radix_sort_pairs(rk, rv, nb, (uint64_t)top + 1);
+  int64_t c = 0, i = 0, j = 0, rc = 0;
+  while (i < na && j < nb) {
+    const int64_t ta = ak[i] / a_tab_size, tb = rk[j] / b_tab_size;
+    if (ta < tb) { i = lower_bound_i64(ak, na, tb * a_tab_size); continue; }
+    if (tb < ta) { j = lower_bound_i64(rk, nb, ta * b_tab_size); continue; }
+    const int64_t tab = ta;
+    const int64_t i_end = lower_bound_i64(ak, na, (tab + 1) * a_tab_size);
+    const int64_t j_end = lower_bound_i64(rk, nb, (tab + 1) * b_tab_size);
+    const int64_t start = c;
+    int64_t ai = i, bj = j;
+    while (ai < i_end && bj < j_end) {
+      const int64_t acol = (ak[ai] % a_tab_size) / ah, brow = (rk[bj] % b_tab_size) / bw;
+      if (acol < brow) { ++ai; continue; }
+      if (brow < acol) { ++bj; continue; }
+      int64_t a_e = ai, b_e = bj;
+      while (a_e < i_end && (ak[a_e] % a_tab_size) / ah == acol) ++a_e;
+      while (b_e < j_end && (rk[b_e] % b_tab_size) / bw == brow) ++b_e;
+      if (c + (a_e - ai) * (b_e - bj) > cap) { rc = -1; goto out; }
+      for (int64_t q = bj; q < b_e; ++q) {
+        const int64_t bcol = rk[q] % bw;
+        for (int64_t p = ai; p < a_e; ++p) {
+          ck[c] = ak[p] % ah + bcol * ah;
+          cv[c++] = av[p] * rv[q];
+        }
+      }
+      ai = a_e;
+      bj = b_e;
+    }
+    if (c - start) {
+      lnto_radix_sort_pairs(ck + start, cv + start, c - start, (uint64_t)c_tab_size);
+      c = start + lnto_collect_duplicates(ck + start, cv + start, c - start);
+      for (int64_t p = start; p < c; ++p) ck[p] += tab * c_tab_size;
+    }
+    i = i_end;
+    j = j_end;
+  }
+  rc = c;
+out:
+  free(rk);
+  free(rv);
+  return rc;
+}
+
 /* Index bookkeeping of the lattice product, MIA_Expr.h:449-561
  * (solve_product_expr_helper + product_lattice_expr_helper): A's rows are its
  * outer modes, columns its inner (contracted) modes, tabs its inter modes, each

@@ -105,6 +105,15 @@ int64_t lnto_lattice_mult(const int64_t *ak, const double *av, int64_t na,
                           const int64_t *bk, const double *bv, int64_t nb,
                           int64_t bh, int64_t bw, int64_t bd, int64_t *ck,
                           double *cv, int64_t cap);
+/* csc_times (compress = 0) / mtt_times (compress = 1) sparse-accumulator
+ * products and the outer_times expand-sort-collect product of the same
+ * lattices (both sorted column major); nnz(C) or -1. */
+int64_t lnto_lattice_mult_spa(const int64_t *ak, const double *av, int64_t na, int64_t ah, int64_t aw,
+                              const int64_t *bk, const double *bv, int64_t nb, int64_t bh, int64_t bw,
+                              int compress, int64_t *ck, double *cv, int64_t cap);
+int64_t lnto_lattice_mult_outer(const int64_t *ak, const double *av, int64_t na, int64_t ah, int64_t aw,
+                                const int64_t *bk, const double *bv, int64_t nb, int64_t bh, int64_t bw,
+                                int64_t *ck, double *cv, int64_t cap);
 /* Full MIA contraction A(labels_a) * B(labels_b) (lattice product path of
  * MIA_Expr.h:864-925).  labels are ASCII mode labels; an upper-case label is an
  * inter ("!") index.  Sorts A and B in place to their lattice orders.  Writes C
@@ -123,13 +132,13 @@ int lnto_contract_plan(int order_a, const int64_t *dims_a, const char *labels_a,
 
 /* ------------------------------ cost model ------------------------------ *
  * FP64 restatement of the dispatch model (no reference counterpart): a
- * perceptron F=18 -> 64 -> 64 -> S=8 with ReLU (derivative 0 at 0); params
- * W1[64][18] b1[64] W2[64][64] b2[64] W3[8][64] b3[8] then mean[18] and
+ * perceptron F=18 -> 64 -> 64 -> S=20 with ReLU (derivative 0 at 0); params
+ * W1[64][18] b1[64] W2[64][64] b2[64] W3[20][64] b3[20] then mean[18] and
  * inv_std[18]; input x is standardised as (x - mean) * inv_std.  Loss =
  * (1/m) sum over measured targets (not NaN) of (y - t)^2, m = their count. */
 #define LNTO_MLP_F 18
 #define LNTO_MLP_H 64
-#define LNTO_MLP_S 8
+#define LNTO_MLP_S 20
 int64_t lnto_mlp_param_count(void);   /* trainable + standardisation */
 int64_t lnto_mlp_weight_count(void);  /* trainable only */
 void lnto_mlp_forward(const double *params, const double *x, double *y);

@@ -32,7 +32,7 @@ def test_python_binding_covers_the_header():
 
 def test_abi_version_and_null_context_errors():
     L = libnt_amd.lib()
-    assert L.lnt_abi_version() == 1
+    assert L.lnt_abi_version() == 2
     assert L.lnt_context_synchronize(None) == 1
     assert b"null context" in L.lnt_last_error()
     assert L.lnt_context_destroy(None) == 0

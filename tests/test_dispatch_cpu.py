@@ -27,8 +27,9 @@ def test_oracle_model_gradient_matches_finite_differences(orc):
     rng = np.random.default_rng(3)
     P = _random_params(orc, rng)
     X = rng.normal(0, 2, (7, 18))
-    T = rng.normal(0, 1, (7, 8))
-    T[rng.random((7, 8)) < 0.3] = np.nan  # unmeasured entries
+    S = orc.MLP_OUTPUTS
+    T = rng.normal(0, 1, (7, S))
+    T[rng.random((7, S)) < 0.3] = np.nan  # unmeasured entries
     loss, g = orc.mlp_loss_grad(P, X, T)
     nw = orc.mlp_weight_count()
     idx = list(rng.choice(nw, 300, replace=False)) + [nw - 1, 0, 18 * 64 + 3]
@@ -48,17 +49,18 @@ def test_oracle_model_loss_closed_form(orc):
     P = np.zeros(orc.mlp_param_count())
     nw = orc.mlp_weight_count()
     P[nw + 18:] = 1.0
-    b3 = rng.normal(0, 1, 8)
-    P[nw - 8:nw] = b3
+    S = orc.MLP_OUTPUTS
+    b3 = rng.normal(0, 1, S)
+    P[nw - S:nw] = b3
     X = rng.normal(0, 1, (5, 18))
-    T = rng.normal(0, 1, (5, 8))
+    T = rng.normal(0, 1, (5, S))
     T[0, 3] = np.nan
     loss, g = orc.mlp_loss_grad(P, X, T)
     m = np.isfinite(T)
     d = np.where(m, b3[None, :] - T, 0.0)
     assert math.isclose(loss, (d ** 2).sum() / m.sum(), rel_tol=1e-14)
-    assert np.allclose(g[nw - 8:], 2 * d.sum(axis=0) / m.sum(), rtol=1e-13, atol=0)
-    assert not np.any(g[: nw - 8])
+    assert np.allclose(g[nw - S:], 2 * d.sum(axis=0) / m.sum(), rtol=1e-13, atol=0)
+    assert not np.any(g[: nw - S])
     assert np.allclose(orc.mlp_forward(P, X), np.tile(b3, (5, 1)))
 
 
@@ -66,6 +68,7 @@ def test_model_parameter_count_matches_oracle(orc):
     from libnt_amd import dispatch
 
     assert dispatch.param_count() == orc.mlp_param_count()
+    assert orc.MLP_OUTPUTS == dispatch.NUM_STRATEGIES
 
 
 def test_struct_layouts_match_c(tmp_path):
@@ -150,40 +153,45 @@ def test_features_closed_form():
 
 
 def test_heuristic_branches():
-    """The orientation (`transposed`) of each branch of the polyalgorithm; the
-    accumulator is always sort-accumulate."""
+    """The orientation (`transposed`) and algorithm class of each branch of
+    the polyalgorithm, mapped to accumulators: CSC/DCSC (1, 2) -> dense,
+    operator*/CSCNA (0, 3) -> sort, outer product (4) -> esc."""
     from libnt_amd import dispatch
 
     SW = dispatch.STRAT_SWAP
+    DENSE, SORT, ESC = (dispatch.strategy(a) for a in (dispatch.ACC_DENSE, dispatch.ACC_SORT, dispatch.ACC_ESC))
     # Both sparse (inner and outer sizes below 3 * nnz): the lexorder rule;
-    # A in [outer inner] and B in [inner outer] -> not transposed.
+    # A in [outer inner] and B in [inner outer] -> not transposed, CSC.
     a = _op([20, 30], "ik", 400)
     b = _op([30, 20], "kj", 400)
-    assert _heuristic(a, b) == 0
+    assert _heuristic(a, b) == DENSE
     # A in [inner outer] and B in [outer inner] -> transposed.
     a_t = _op([20, 30], "ik", 400, seq=[1, 0])
     b_t = _op([30, 20], "kj", 400, seq=[1, 0])
-    assert _heuristic(a_t, b_t) == SW
-    # Both sparse, A in [inner outer], B in neither order's lead -> the
-    # size rule (b_outer > a_outer -> not transposed).
+    assert _heuristic(a_t, b_t) == DENSE | SW
     a3 = _op([20, 30, 5], "ikm", 900, seq=[1, 0, 2])
     b3 = _op([30, 40], "kj", 900, seq=[1, 0])
-    assert _heuristic(a3, b3) == SW  # a_io and b_oi
+    assert _heuristic(a3, b3) == DENSE | SW  # a_io and b_oi
+    # The fifth case (A in [inner outer], B in neither order): transposed
+    # with algorithm 0 (operator*, csc_no_accum).
+    a5 = _op([20, 30], "ik", 400, seq=[1, 0])
+    b5 = _op([30, 4, 5], "kjm", 400, seq=[1, 0, 2])
+    assert _heuristic(a5, b5) == SORT | SW
     # A sparse, B hyper-sparse (inner and outer sizes > 3 * nnz): CSC.
     b_h = _op([1000, 1000], "kj", 50)
     a_s = _op([20, 1000], "ik", 10000)
-    assert _heuristic(a_s, b_h) == 0
+    assert _heuristic(a_s, b_h) == DENSE
     # A hyper-sparse, B sparse: CSR (transposed).
     a_h = _op([1000, 1000], "ik", 50)
     b_s = _op([1000, 20], "kj", 10000)
-    assert _heuristic(a_h, b_s) == SW
+    assert _heuristic(a_h, b_s) == DENSE | SW
     # Both hyper-sparse: the outer-product algorithm, not transposed.
-    assert _heuristic(a_h, b_h) == 0
-    # A row-sparse only, B col-sparse only, default orders: not transposed.
-    assert _heuristic(_op([1000, 10], "ik", 40), _op([10, 1000], "kj", 40)) == 0
+    assert _heuristic(a_h, b_h) == ESC
+    # A row-sparse only, B col-sparse only, default orders: CSCNA, not transposed.
+    assert _heuristic(_op([1000, 10], "ik", 40), _op([10, 1000], "kj", 40)) == SORT
     # A row-sparse only (i: 1000/40 > 3, k: 20/40), B row-sparse only
     # (k: 20/4 > 3, j: 10/4): the DCSR branch, transposed.
-    assert _heuristic(_op([1000, 20], "ik", 40), _op([20, 10], "kj", 4)) == SW
+    assert _heuristic(_op([1000, 20], "ik", 40), _op([20, 10], "kj", 4)) == DENSE | SW
 
 
 def test_dispatch_argument_errors():

@@ -41,6 +41,11 @@ def _oracle(orc, A, la, B, lb):
                         host(B.keys), host(B.vals), list(B.dims), B.seq, B.is_sorted, lb)
 
 
+# Accumulators whose sums are the same chain of additions (each row in
+# ascending B row order): one orientation gives the same bits with any of them.
+_ORDERED = (0, 2, 3, 4)  # sort, dense, esc, mtt
+
+
 def test_every_strategy_matches_oracle(ctx, orc):
     import torch
 
@@ -49,6 +54,11 @@ def test_every_strategy_matches_oracle(ctx, orc):
     for i, sp in enumerate(_specs()):
         A, B = dispatch.materialize(ctx, sp, 100 + i)
         ek, ev, edims = _oracle(orc, A, sp.la, B, sp.lb)
+        # csc_times's sparse accumulator sums in ascending B row order: the
+        # bits of the A*B-oriented sort, dense, esc and mtt strategies.
+        sk, sv, _ = orc.contract(host(A.keys), host(A.vals), list(A.dims), A.seq, A.is_sorted, sp.la,
+                                 host(B.keys), host(B.vals), list(B.dims), B.seq, B.is_sorted, sp.lb, method="spa")
+        assert np.array_equal(sk, ek)
         ref = None
         for s in range(dispatch.NUM_STRATEGIES):
             print("spec", i, sp.kind, sp.la, sp.lb, "strategy", dispatch.strategy_name(s), flush=True)
@@ -57,6 +67,8 @@ def test_every_strategy_matches_oracle(ctx, orc):
             assert list(C.dims) == list(edims)
             assert np.array_equal(host(C.keys), ek), (i, s)
             assert orc.allclose_fuzzy(host(C.vals), ev, TOL), (i, s)
+            if dispatch.strategy_acc(s) in _ORDERED and not s & dispatch.STRAT_SWAP:
+                assert np.array_equal(host(C.vals).view(np.int64), sv.view(np.int64)), (i, s)
             if s == 0:
                 ref = C
         # the front end's product (sort-accumulate, A*B orientation) bit for bit
@@ -101,10 +113,10 @@ def test_time_strategies(ctx):
 
     sp = _specs()[0]
     A, B = dispatch.materialize(ctx, sp, 300)
-    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, 0xFF, 1)
-    assert (ms > 0).all()
-    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, 0b101, 1)
-    assert ms[0] > 0 and ms[2] > 0 and ms[1] == -1 and ms[3] == -1
+    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, dispatch.ALL_STRATEGIES, 1)
+    assert ms.shape == (dispatch.NUM_STRATEGIES,) and (ms > 0).all()
+    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, 0b101 | (1 << 19), 1)
+    assert ms[0] > 0 and ms[2] > 0 and ms[19] > 0 and ms[1] == -1 and ms[3] == -1 and ms[18] == -1
 
 
 def _random_model(ctx, orc, seed):
@@ -134,8 +146,8 @@ def test_model_gradient_matches_oracle(ctx, orc, n):
     m, P = _random_model(ctx, orc, 12 + n)
     rng = np.random.default_rng(n)
     X = rng.normal(0, 2, (n, 18)).astype(np.float32)
-    T = rng.normal(0, 2, (n, 8)).astype(np.float32)
-    T[rng.random((n, 8)) < 0.25] = np.nan
+    T = rng.normal(0, 2, (n, dispatch.NUM_STRATEGIES)).astype(np.float32)
+    T[rng.random((n, dispatch.NUM_STRATEGIES)) < 0.25] = np.nan
     if not np.isfinite(T).any():
         T[0, 0] = 1.0
     loss, g = m.gradient(X, T)
@@ -150,7 +162,7 @@ def test_model_training_lowers_loss(ctx):
 
     rng = np.random.default_rng(5)
     X = rng.normal(0, 1, (2000, 18)).astype(np.float32)
-    W = rng.normal(0, 0.5, (18, 8))
+    W = rng.normal(0, 0.5, (18, dispatch.NUM_STRATEGIES))
     T = (np.tanh(X @ W) + 0.1 * X[:, :8]).astype(np.float32)
     m = dispatch.Model(ctx, 3)
     l0, _ = m.gradient(X, T)

@@ -39,9 +39,18 @@ CASES = [
 ]
 
 
+# sort, hash, dense (csc_times), esc (outer_times), mtt, and the CSC column
+# lookup with sort and dense.
+STRATEGIES = [1, 2, 3, 4, 5, 1 | 16, 3 | 16]
+
+
 @pytest.mark.parametrize("case", CASES)
-@pytest.mark.parametrize("strategy", [1, 2])
+@pytest.mark.parametrize("strategy", STRATEGIES)
 def test_lattice_mult_matches_oracle(ctx, orc, case, strategy):
+    """Every accumulator against the oracle's csc_no_accum (keys exact, values
+    to 1e-12); all but the hash accumulator against the oracle's csc_times
+    sparse accumulator bit for bit (each row summed in ascending B row order),
+    and outer_times/mtt_times restated agree on the keys."""
     ah, aw, d, bw, n_a, n_b = case
     print("lattice", case, "strategy", strategy, flush=True)
     ak, av = _random_lattice(orc, ah, aw, d, n_a, 3)
@@ -52,6 +61,40 @@ def test_lattice_mult_matches_oracle(ctx, orc, case, strategy):
     ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, d), gpu(bk), gpu(bv), (aw, bw, d), strategy)
     assert np.array_equal(host(ck), ek)
     assert orc.allclose_fuzzy(host(cv), ev, TOL)
+    if strategy != 2:
+        sk, sv = orc.lattice_mult_by("spa", ak, av, (ah, aw, d), bk, bv, (aw, bw, d))
+        assert np.array_equal(sk, ek)
+        assert np.array_equal(host(cv).view(np.int64), sv.view(np.int64))
+    if strategy in (4, 5):
+        ok, ov = orc.lattice_mult_by("outer" if strategy == 4 else "mtt", ak, av, (ah, aw, d), bk, bv, (aw, bw, d))
+        assert np.array_equal(ok, ek) and orc.allclose_fuzzy(ov, ev, TOL)
+
+
+@pytest.mark.parametrize("shape", [(8192, 300, 2, 400), (8193, 300, 1, 400), (50000, 64, 1, 2000),
+                                   (1 << 20, 16, 1, 1000)])
+def test_dense_and_mtt_limits(ctx, orc, shape):
+    """The dense accumulators at and past their LDS limit (8192 rows, or 8192
+    distinct rows for mtt_times), with long columns (many B entries each):
+    past the limit they fall back to the sort accumulator; the bits stay the
+    sparse accumulator's either way."""
+    ah, aw, d, bw = shape
+    rng = np.random.default_rng(ah)
+    n_a = min(60000, ah * aw * d // 4)
+    ak, av = _random_lattice(orc, ah, aw, d, n_a, 7)
+    if ah > 20000:  # few distinct rows: mtt's ranks fit where the rows do not
+        rows = rng.choice(ah, 6000, replace=False)
+        sub = np.unique(rows[rng.integers(0, 6000, n_a)] + ah * rng.integers(0, aw * d, n_a))
+        ak, av = sub.astype(np.int64), orc.gen_uniform_vals(len(sub), 8)
+    # B: a few columns with many entries (long segments) and many short ones
+    bk = np.unique(np.concatenate([rng.integers(0, aw, 3000) + aw * rng.integers(0, 4, 3000),
+                                   rng.integers(0, aw * bw, 5000)]) % (aw * bw * d)).astype(np.int64)
+    bv = orc.gen_uniform_vals(len(bk), 9)
+    sk, sv = orc.lattice_mult_by("spa", ak, av, (ah, aw, d), bk, bv, (aw, bw, d))
+    for strategy in (3, 5, 3 | 16):
+        print("dense limits", shape, "strategy", strategy, flush=True)
+        ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, d), gpu(bk), gpu(bv), (aw, bw, d), strategy)
+        assert np.array_equal(host(ck), sk), strategy
+        assert np.array_equal(host(cv).view(np.int64), sv.view(np.int64)), strategy
 
 
 def test_lattice_mult_skewed_columns(ctx, orc):

@@ -215,6 +215,13 @@ def test_sparselattice_mult(ctx, orc, case, algorithm):
     cv, ck = mex.sparselattice_mult(ctx, av, ak, (ah, aw, d), bv, bk, (aw, bw, d), algorithm)
     assert np.array_equal(ck, ek)
     assert orc.allclose_fuzzy(cv, ev, 1e-12)
+    # Every algorithm's GPU counterpart sums each row in ascending B row
+    # order: the bits of csc_times's sparse accumulator (and of mtt_times's,
+    # which performs the same additions; algorithm 4 returns mtt_times's
+    # result through the mex's fall-through into case 5).
+    method = {0: "spa", 1: "spa", 2: "spa", 3: "spa", 4: "mtt", 5: "mtt"}[int(algorithm) if algorithm > 0 else 0]
+    sk, sv = orc.lattice_mult_by(method, ak, av, (ah, aw, d), bk, bv, (aw, bw, d))
+    assert np.array_equal(ck, sk) and np.array_equal(cv.view(np.int64), sv.view(np.int64))
 
 
 def test_sparselattice_mult_checks_and_empty(ctx, orc):

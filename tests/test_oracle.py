@@ -391,6 +391,59 @@ def test_lattice_mult_dense(orc, shape, density):
     assert orc.lattice_nprod(ak, (h, w, d), bk, (w, bw, d)) == nprod
 
 
+@pytest.mark.parametrize("method", ["spa", "mtt", "outer"])
+@pytest.mark.parametrize("shape", [(20, 30, 1), (7, 40, 3), (1, 1, 1), (33, 1, 2), (40, 25, 4), (300, 6, 2)])
+@pytest.mark.parametrize("density", [0.05, 0.3])
+def test_lattice_mult_methods_dense(orc, method, shape, density):
+    """csc_times, mtt_times and outer_times against dense einsum (values) and
+    the boolean product (the structural index set, zero sums kept)."""
+    rng = np.random.default_rng(hash((method, shape, density)) & 0xFFFF)
+    h, w, d = shape
+    bw = 13
+    A = np.where(rng.random((h, w, d)) < density, rng.uniform(-1, 1, (h, w, d)), 0.0)
+    B = np.where(rng.random((w, bw, d)) < density, rng.uniform(-1, 1, (w, bw, d)), 0.0)
+    if h > 100:
+        A[::3] = 0.0  # rows absent from A: the compressed row space of mtt_times is smaller than h
+    ak, av = _lattice_from_dense(A)
+    bk, bv = _lattice_from_dense(B)
+    ck, cv = orc.lattice_mult_by(method, ak, av, (h, w, d), bk, bv, (w, bw, d))
+    C = np.einsum("rkt,kct->rct", A, B)
+    S = np.einsum("rkt,kct->rct", (A != 0).astype(np.int64), (B != 0).astype(np.int64))
+    flatC = C.transpose(2, 1, 0).ravel()
+    flatS = S.transpose(2, 1, 0).ravel()
+    assert np.array_equal(ck, np.flatnonzero(flatS))
+    assert orc.allclose_fuzzy(cv, flatC[ck], 1e-12)
+
+
+def test_lattice_mult_spa_summation_order(orc):
+    """The sparse accumulator adds a row's products in ascending B row order,
+    the first one unadded: A(0,:) = [1, 1e16, -1e16] times B(:,0) = [1, 1, 1]
+    gives (1 + 1e16) - 1e16 = 0 (1e16 + 1 rounds to 1e16), where descending
+    order would give 1.  mtt_times performs the same additions; outer_times
+    sums in the order its radix sort leaves, which on this input (one key
+    three times) keeps the append order."""
+    h, w, bw = 2, 3, 1
+    A = np.zeros((h, w, 1))
+    A[0, :, 0] = [1.0, 1e16, -1e16]
+    A[1, 1, 0] = 2.0
+    B = np.ones((w, bw, 1))
+    ak, av = _lattice_from_dense(A)
+    bk, bv = _lattice_from_dense(B)
+    for method in ("spa", "mtt", "outer"):
+        ck, cv = orc.lattice_mult_by(method, ak, av, (h, w, 1), bk, bv, (w, bw, 1))
+        assert ck.tolist() == [0, 1] and cv.tolist() == [0.0, 2.0], method
+    # csc_times and mtt_times: the same operations, the same bits.
+    rng = np.random.default_rng(5)
+    A = np.where(rng.random((60, 30, 2)) < 0.3, rng.standard_normal((60, 30, 2)) * 10.0 ** rng.integers(-8, 8, (60, 30, 2)), 0.0)
+    B = np.where(rng.random((30, 20, 2)) < 0.3, rng.standard_normal((30, 20, 2)), 0.0)
+    A[::4] = 0.0
+    ak, av = _lattice_from_dense(A)
+    bk, bv = _lattice_from_dense(B)
+    k1, v1 = orc.lattice_mult_by("spa", ak, av, (60, 30, 2), bk, bv, (30, 20, 2))
+    k2, v2 = orc.lattice_mult_by("mtt", ak, av, (60, 30, 2), bk, bv, (30, 20, 2))
+    assert np.array_equal(k1, k2) and np.array_equal(v1.view(np.int64), v2.view(np.int64))
+
+
 def _dense_mia(keys, vals, dims):
     D = np.zeros(int(np.prod(dims)))
     D[keys] = vals
