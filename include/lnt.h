@@ -160,6 +160,18 @@ int lnt_merge_into(lnt_context ctx, const int64_t *a_keys,
                    const double *a_vals, int64_t na, const int64_t *b_keys,
                    const double *b_vals, int64_t nb, int op, int64_t *c_keys,
                    double *c_vals, int64_t capacity, int64_t *nc, int mem);
+/* lnt_merge_into for callers that cannot allocate (code recorded into a HIP
+ * graph, tight serving loops, harnesses that forbid hipMalloc): no reference
+ * counterpart.  Device memory only; the caller passes A, B and C as for
+ * lnt_merge_into.  The call allocates nothing: its scratch (look-back words,
+ * merge-path splits, a counter) is a static 512 KiB array that every GPU the
+ * library runs on holds until the process ends, which limits na + nb to
+ * 43,000 merge tiles (352,256,000 entries at the default 512 x 16 tile;
+ * LNT_ERR_TOO_LARGE beyond).  All calls on one GPU share that array: calls
+ * that overlap on one GPU (other threads, other streams) race. */
+int lnt_merge_into_noalloc(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                           const int64_t *b_keys, const double *b_vals, int64_t nb, int op, int64_t *c_keys,
+                           double *c_vals, int64_t capacity, int64_t *nc);
 /* As lnt_merge_into, but the library allocates C (device memory for
  * LNT_MEM_DEVICE, host memory otherwise); release with lnt_free. */
 int lnt_merge(lnt_context ctx, const int64_t *a_keys, const double *a_vals,

@@ -61,6 +61,7 @@ SIGNATURES = {
     "lnt_sort_pairs": [_vp, _vp, _vp, _i64, _u64, _int],
     "lnt_change_sort_order": [_vp, _vp, _vp, _i64, _int, _i64p, _i32p, _i32p, _int, _int],
     "lnt_merge_into": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _vp, _i64, _i64p, _int],
+    "lnt_merge_into_noalloc": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _vp, _i64, _i64p],
     "lnt_compare": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _int],
     "lnt_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_free": [_vp, _vp, _int],

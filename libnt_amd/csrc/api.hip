@@ -208,7 +208,9 @@ int lnt_context_create(lnt_context *out, int device, void *stream) {
   }
   memset(ctx->pinned, 0, 64 * sizeof(uint32_t));
   int rc = ensure(ctx->small, kSmallWords * sizeof(uint32_t));
+  if (rc == LNT_OK) rc = noalloc_scratch_init(ctx);
   if (rc != LNT_OK) {
+    release(ctx->small);
     (void)hipHostFree(ctx->pinned);
     delete ctx;
     return rc;
@@ -493,6 +495,30 @@ int lnt_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
     LNT_HIP(hipMemcpyAsync(c_vals, st.b[5].ptr, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
     LNT_HIP(hipStreamSynchronize(ctx->stream));
   }
+  *nc = count;
+  return LNT_OK;
+}
+
+int lnt_merge_into_noalloc(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na,
+                           const int64_t *b_keys, const double *b_vals, int64_t nb, int op, int64_t *c_keys,
+                           double *c_vals, int64_t capacity, int64_t *nc) {
+  LNT_CTX(ctx);
+  if (!nc) return fail(LNT_ERR_INVALID_ARGUMENT, "null nc");
+  if (op != LNT_OP_ADD && op != LNT_OP_SUB && op != LNT_OP_HADAMARD)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "op must be LNT_OP_ADD, LNT_OP_SUB or LNT_OP_HADAMARD");
+  if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
+  if (na + nb >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "na + nb must be < 2^32");
+  const int64_t need = op == LNT_OP_HADAMARD ? std::min(na, nb) : na + nb;
+  if (capacity < need)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) +
+                                              " is below the bound " + std::to_string(need));
+  if ((na && (!a_keys || !a_vals)) || (nb && (!b_keys || !b_vals)) || (need && (!c_keys || !c_vals)))
+    return fail(LNT_ERR_INVALID_ARGUMENT, "null array");
+  if (ctx->validate >= 2)
+    LNT_TRY(check_sorted(ctx, (const uint64_t *)a_keys, (uint32_t)na, (const uint64_t *)b_keys, (uint32_t)nb));
+  uint32_t count = 0;
+  LNT_TRY(merge_noalloc_device(ctx, (const uint64_t *)a_keys, a_vals, (uint32_t)na, (const uint64_t *)b_keys, b_vals,
+                               (uint32_t)nb, op, (uint64_t *)c_keys, c_vals, (uint64_t)capacity, &count));
   *nc = count;
   return LNT_OK;
 }

@@ -462,6 +462,7 @@ struct lnt_context_s {
   std::vector<lnt::ProfRecord> prof;   // recorded launches (events owned)
   std::vector<hipEvent_t> event_pool;  // recycled events
   hipEvent_t check_event = nullptr;    // marks the sort's key-range flag copy
+  void *noalloc_scratch = nullptr;     // this device's static merge scratch (lnt_merge_into_noalloc)
 };
 
 namespace lnt {
@@ -516,6 +517,12 @@ int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
                  uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
                  int op, uint64_t *ck, double *cv, uint64_t capacity,
                  uint32_t *nc_host);
+// The static scratch of the no-allocation merge (merge.hip): its address on
+// the context's device, looked up once per device.
+int noalloc_scratch_init(lnt_context ctx);
+int merge_noalloc_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
+                         const double *bv, uint32_t nb, int op, uint64_t *ck, double *cv, uint64_t capacity,
+                         uint32_t *nc_host);
 int sort_merge_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
                       const double *bv, uint32_t nb, int op, uint64_t bound, uint64_t *ck, double *cv,
                       uint64_t capacity, uint64_t *nc_host);

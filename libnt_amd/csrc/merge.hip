@@ -30,6 +30,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "lnt_internal.h"
@@ -319,30 +321,76 @@ __global__ __launch_bounds__(BT) void merge_kernel(
   }
 }
 
+// Device scratch of one merge: look-back words (one per tile), merge-path
+// splits (tiles + 1), the tile counter and the output total.
+struct MergeScratch {
+  uint64_t *status;
+  uint32_t *splits, *counter, *total;
+  uint32_t epoch;
+};
+
 template <int BT, int ITEMS>
 int launch_merge(lnt_context ctx, const uint64_t *ak, const double *av,
                  uint32_t na, const uint64_t *bk, const double *bv, uint32_t nb,
                  int op, uint64_t *ck, double *cv, uint64_t capacity,
-                 uint32_t *counter, uint32_t *total, int *slot) {
+                 const MergeScratch &m, int *slot) {
   constexpr uint64_t tile = BT * ITEMS;
   uint64_t steps = (uint64_t)na + nb;
   uint32_t tiles = (uint32_t)((steps + tile - 1) / tile);
-  LNT_TRY(ensure_status(ctx, tiles));
-  LNT_TRY(ensure(ctx->aux, ((size_t)tiles + 1) * sizeof(uint32_t)));
-  uint32_t *splits = (uint32_t *)ctx->aux.ptr;
-  uint32_t epoch;
-  LNT_TRY(next_epoch(ctx, &epoch));
   int pslot = prof_begin(ctx, kProfPartition, 0.0);
   partition_kernel<<<(tiles + 1 + 255) / 256, 256, 0, ctx->stream>>>(
-      ak, na, bk, nb, (uint32_t)tile, tiles, splits, counter);
+      ak, na, bk, nb, (uint32_t)tile, tiles, m.splits, m.counter);
   prof_end(ctx, pslot);
   LNT_HIP(hipGetLastError());
   *slot = prof_begin(ctx, kProfMerge, 16.0 * ((double)na + nb));
   merge_kernel<BT, ITEMS><<<tiles, BT, 0, ctx->stream>>>(
-      ak, av, na, bk, bv, nb, op, ck, cv, splits, (uint64_t *)ctx->status.ptr,
-      epoch, counter, total, tiles, capacity, ctx->tuning.narrow_keys ? 0 : 1);
+      ak, av, na, bk, bv, nb, op, ck, cv, m.splits, m.status,
+      m.epoch, m.counter, m.total, tiles, capacity, ctx->tuning.narrow_keys ? 0 : 1);
   prof_end(ctx, *slot);
   LNT_HIP(hipGetLastError());
+  return LNT_OK;
+}
+
+// Steps per merge tile under the context's tuning (the tile launch_merge
+// runs: merge_block x merge_items, 16 items when unset).
+uint32_t merge_tile_steps(lnt_context ctx) {
+  const int items = ctx->tuning.merge_items ? ctx->tuning.merge_items : 16;
+  const int block = ctx->tuning.merge_block;
+  switch (block * 100 + items) {
+    case 25608: case 25612: case 25616: case 51208: case 51212: case 102408: case 102412: case 102416:
+      return (uint32_t)(block * items);
+    default:
+      return 512 * 16;
+  }
+}
+
+// Runs the merge with the given scratch; *nc_host = nnz(C).
+int merge_with(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
+               const double *bv, uint32_t nb, int op, uint64_t *ck, double *cv, uint64_t capacity,
+               const MergeScratch &m, uint32_t *nc_host) {
+  int rc, slot = -1;
+#define LNT_MERGE(BT, IT) launch_merge<BT, IT>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, m, &slot)
+  // Measured on 2 x 50M-key merges: 512 x 16 (two tiles per CU).
+  const int items = ctx->tuning.merge_items ? ctx->tuning.merge_items : 16;
+  switch (ctx->tuning.merge_block * 100 + items) {
+    case 25608: rc = LNT_MERGE(256, 8); break;
+    case 25612: rc = LNT_MERGE(256, 12); break;
+    case 25616: rc = LNT_MERGE(256, 16); break;
+    case 51208: rc = LNT_MERGE(512, 8); break;
+    case 51212: rc = LNT_MERGE(512, 12); break;
+    case 102408: rc = LNT_MERGE(1024, 8); break;
+    case 102412: rc = LNT_MERGE(1024, 12); break;
+    case 102416: rc = LNT_MERGE(1024, 16); break;
+    default: rc = LNT_MERGE(512, 16);
+  }
+#undef LNT_MERGE
+  if (rc != LNT_OK) return rc;
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 1, m.total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  *nc_host = ctx->pinned[1];
+  prof_set_bytes(ctx, slot, 16.0 * ((double)na + nb + *nc_host));
+  if (*nc_host > capacity)
+    return fail(LNT_ERR_NOT_SORTED, "merge inputs are not sorted with unique keys (output overflow)");
   return LNT_OK;
 }
 
@@ -420,33 +468,87 @@ int merge_device(lnt_context ctx, const uint64_t *ak, const double *av,
   }
   LNT_TRY(ensure(ctx->small, kSmallWords * sizeof(uint32_t)));
   uint32_t *small = (uint32_t *)ctx->small.ptr;
-  uint32_t *counter = small + kSmallCounters + 15;
-  uint32_t *total = small + kSmallFlags + 1;
-  int rc, slot = -1;
-#define LNT_MERGE(BT, IT) \
-  launch_merge<BT, IT>(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, counter, total, &slot)
-  // Measured on 2 x 50M-key merges: 512 x 16 (two tiles per CU).
-  const int items = ctx->tuning.merge_items ? ctx->tuning.merge_items : 16;
-  switch (ctx->tuning.merge_block * 100 + items) {
-    case 25608: rc = LNT_MERGE(256, 8); break;
-    case 25612: rc = LNT_MERGE(256, 12); break;
-    case 25616: rc = LNT_MERGE(256, 16); break;
-    case 51208: rc = LNT_MERGE(512, 8); break;
-    case 51212: rc = LNT_MERGE(512, 12); break;
-    case 102408: rc = LNT_MERGE(1024, 8); break;
-    case 102412: rc = LNT_MERGE(1024, 12); break;
-    case 102416: rc = LNT_MERGE(1024, 16); break;
-    default: rc = LNT_MERGE(512, 16);
+  const uint32_t tiles = (uint32_t)(((uint64_t)na + nb + merge_tile_steps(ctx) - 1) / merge_tile_steps(ctx));
+  LNT_TRY(ensure_status(ctx, tiles));
+  LNT_TRY(ensure(ctx->aux, ((size_t)tiles + 1) * sizeof(uint32_t)));
+  MergeScratch m;
+  m.status = (uint64_t *)ctx->status.ptr;
+  m.splits = (uint32_t *)ctx->aux.ptr;
+  m.counter = small + kSmallCounters + 15;
+  m.total = small + kSmallFlags + 1;
+  LNT_TRY(next_epoch(ctx, &m.epoch));
+  return merge_with(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, m, nc_host);
+}
+
+// ------------------------------------------- merge with no allocation
+// A file-scope device array holds the scratch of lnt_merge_into_noalloc:
+// kNoAllocTiles look-back words, then the merge-path splits (kNoAllocTiles + 1
+// 32-bit words), the tile counter and the output total.  Every GPU the
+// library runs on holds its own copy until the process ends (512 KiB); calls
+// on one GPU share it.  Host code never passes the array's name to a kernel:
+// it uses the address hipGetSymbolAddress gives for the device, kept per
+// device by noalloc_scratch_init at context creation.
+constexpr uint32_t kNoAllocTiles = 43000;
+constexpr size_t kNoAllocWords = kNoAllocTiles + (kNoAllocTiles + 1 + 1) / 2 + 2;
+static __device__ __align__(256) uint64_t scratch[kNoAllocWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_noalloc_ptr[kMaxDevices];
+std::mutex g_noalloc_mutex;
+
+__global__ void clear_words_kernel(uint64_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+}  // namespace
+
+// Looks up (once per device, with that device current) the address of this
+// device's scratch array and keeps it in the context.
+int noalloc_scratch_init(lnt_context ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return fail(LNT_ERR_INVALID_ARGUMENT, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_ptr[ctx->device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(LNT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[ctx->device] = ptr;
   }
-#undef LNT_MERGE
-  if (rc != LNT_OK) return rc;
-  LNT_HIP(hipMemcpyAsync(ctx->pinned + 1, total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  LNT_HIP(hipStreamSynchronize(ctx->stream));
-  *nc_host = ctx->pinned[1];
-  prof_set_bytes(ctx, slot, 16.0 * ((double)na + nb + *nc_host));
-  if (*nc_host > capacity)
-    return fail(LNT_ERR_NOT_SORTED, "merge inputs are not sorted with unique keys (output overflow)");
+  ctx->noalloc_scratch = g_noalloc_ptr[ctx->device];
   return LNT_OK;
+}
+
+// The merge of lnt_merge_into with its scratch in the static array: no
+// allocation of any kind.  The look-back words of the tiles this call uses
+// are cleared by a kernel first, so a word left by an earlier call never
+// reads as published; epoch 1 then serves every call.
+int merge_noalloc_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
+                         const double *bv, uint32_t nb, int op, uint64_t *ck, double *cv, uint64_t capacity,
+                         uint32_t *nc_host) {
+  if ((uint64_t)na + nb == 0) {
+    *nc_host = 0;
+    return LNT_OK;
+  }
+  if (!ctx->noalloc_scratch) return fail(LNT_ERR_DEVICE, "internal: the no-allocation scratch is not set up");
+  const uint64_t tile = merge_tile_steps(ctx);
+  const uint64_t tiles = ((uint64_t)na + nb + tile - 1) / tile;
+  if (tiles > kNoAllocTiles)
+    return fail(LNT_ERR_TOO_LARGE, "na + nb = " + std::to_string((uint64_t)na + nb) +
+                                       " exceeds the no-allocation merge's limit of " +
+                                       std::to_string(kNoAllocTiles * tile) + " (" + std::to_string(kNoAllocTiles) +
+                                       " tiles of " + std::to_string(tile) + " steps)");
+  uint64_t *base = (uint64_t *)ctx->noalloc_scratch;
+  MergeScratch m;
+  m.status = base;
+  m.splits = (uint32_t *)(base + kNoAllocTiles);
+  m.counter = m.splits + kNoAllocTiles + 1;
+  m.total = m.counter + 1;
+  m.epoch = 1;
+  clear_words_kernel<<<(uint32_t)std::min<uint64_t>((tiles + 255) / 256, 1024), 256, 0, ctx->stream>>>(
+      m.status, (uint32_t)tiles);
+  LNT_HIP(hipGetLastError());
+  return merge_with(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, m, nc_host);
 }
 
 template <typename T>
