@@ -143,6 +143,8 @@ def test_model_forward_matches_oracle(ctx, orc):
 
 @pytest.mark.parametrize("n", [1, 37, 700])
 def test_model_gradient_matches_oracle(ctx, orc, n):
+    from libnt_amd import dispatch
+
     m, P = _random_model(ctx, orc, 12 + n)
     rng = np.random.default_rng(n)
     X = rng.normal(0, 2, (n, 18)).astype(np.float32)
@@ -163,7 +165,7 @@ def test_model_training_lowers_loss(ctx):
     rng = np.random.default_rng(5)
     X = rng.normal(0, 1, (2000, 18)).astype(np.float32)
     W = rng.normal(0, 0.5, (18, dispatch.NUM_STRATEGIES))
-    T = (np.tanh(X @ W) + 0.1 * X[:, :8]).astype(np.float32)
+    T = (np.tanh(X @ W) + 0.1 * X @ np.eye(18, dispatch.NUM_STRATEGIES)).astype(np.float32)
     m = dispatch.Model(ctx, 3)
     l0, _ = m.gradient(X, T)
     l1 = m.train(X, T, epochs=60, batch=128, lr=3e-3, seed=1)
