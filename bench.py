@@ -14,8 +14,8 @@ Prints ONE JSON line.  value = sorted keys/s of the whole step
 radix pass) -- algorithmic bytes (32 B per key per pass) over its measured
 average launch duration (HIP events on the library's stream during the timed
 steps).  cpu_baseline: the CPU restatement of the reference algorithms
-(oracle/), one pinned core, one run of the same step on host copies of the
-same inputs.
+(oracle/; RadixShuffle for the sorted inputs, as the reference sorts them),
+one pinned core, one run of the same step on host copies of the same inputs.
 
 Further objects on the same line (skip with --no-extras; --section runs one):
   small_contraction: C(i,j,m,n) = A(i,j,k,l)*B(k,l,m,n), dims 32^4 at 1%
@@ -130,12 +130,13 @@ def cpu_baseline(ak0, av0, bk0, bv0, scale=1):
     with _OneCore() as nproc:
         ak, av, bk, bv = ak0.copy(), av0.copy(), bk0.copy(), bv0.copy()
         t0 = time.perf_counter()
-        oracle.sort_seq_inplace(ak, av, DIMS, IDENT, perm, True)
-        oracle.sort_seq_inplace(bk, bv, DIMS, IDENT, perm, True)
+        oracle.sort_seq_shuffle_inplace(ak, av, DIMS, IDENT, perm, True)
+        oracle.sort_seq_shuffle_inplace(bk, bv, DIMS, IDENT, perm, True)
         nu = len(oracle.merge(ak, av, bk, bv, oracle.OP_ADD)[0])
         nh = len(oracle.merge(ak, av, bk, bv, oracle.OP_HADAMARD)[0])
         t = time.perf_counter() - t0
     assert nu == len(ak0) + len(bk0) - nh
+    assert bool(np.all(np.diff(ak) > 0)) and bool(np.all(np.diff(bk) > 0))
     what = ("the same inputs as the GPU step (host copies), full size" if scale == 1 else
             "a 1/%d-scale sample from the same generators" % scale)
     return {
@@ -146,8 +147,9 @@ def cpu_baseline(ak0, av0, bk0, bv0, scale=1):
         "cpu_model": _cpu_model(),
         "nproc": nproc,
         "sample": "one step on %s (%d+%d keys), one pinned core, single run of %.2f s: 2x change_sort_order "
-                  "(relinearise + in-place MSD radix 2048/11-bit, thresholds 3000/50) + scan merge A+B + "
-                  "Hadamard (the reference's reserve(nA*nB/2^Inter) replaced by min(nA, nB))"
+                  "of sorted input by the reference's path (relinearise + RadixShuffle::permute, stages from "
+                  "setupPermute) + scan merge A+B + Hadamard (the reference's reserve(nA*nB/2^Inter) replaced by "
+                  "min(nA, nB))"
                   % (what, len(ak0), len(bk0), t),
     }
 

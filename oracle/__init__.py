@@ -72,6 +72,7 @@ def lib():
             "lnto_radix_sort_pairs": (None, [_i64p, _f64p, i64, u64]),
             "lnto_introsort_pairs": (None, [_i64p, _f64p, i64]),
             "lnto_sort_seq": (None, [_i64p, _f64p, i64, i32, _i64p, _i32p, _i32p, i32]),
+            "lnto_sort_seq_shuffle": (None, [_i64p, _f64p, i64, i32, _i64p, _i32p, _i32p, i32]),
             "lnto_collect_duplicates": (i64, [_i64p, _f64p, i64]),
             "lnto_merge": (i64, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, _i64p, _f64p]),
             "lnto_mex_merge": (i64, [_i64p, _f64p, i64, _i64p, _f64p, i64, i32, _i64p, _f64p]),
@@ -204,6 +205,19 @@ def sort_seq(keys, vals, dims, cur_seq, new_seq, is_sorted):
     lib().lnto_sort_seq(_p(k, _i64p), _p(v, _f64p), len(k), len(dims), _p(dims, _i64p),
                         _p(cur, _i32p), _p(new, _i32p), int(bool(is_sorted)))
     return k, v
+
+
+def sort_seq_shuffle(keys, vals, dims, cur_seq, new_seq, is_sorted):
+    """sort(seq) by the reference's own RadixShuffle path (sorted input)."""
+    k, v = _i64(keys).copy(), _f64(vals).copy()
+    sort_seq_shuffle_inplace(k, v, dims, cur_seq, new_seq, is_sorted)
+    return k, v
+
+
+def sort_seq_shuffle_inplace(k, v, dims, cur_seq, new_seq, is_sorted):
+    dims, cur, new = _i64(dims), _i32(cur_seq), _i32(new_seq)
+    lib().lnto_sort_seq_shuffle(_p(k, _i64p), _p(v, _f64p), len(k), len(dims), _p(dims, _i64p),
+                                _p(cur, _i32p), _p(new, _i32p), int(bool(is_sorted)))
 
 
 def sort_seq_inplace(k, v, dims, cur_seq, new_seq, is_sorted):

@@ -61,6 +61,11 @@ void lnto_change_linidx(int64_t *keys, int64_t n, int order,
 void lnto_radix_sort_pairs(int64_t *keys, double *vals, int64_t n,
                            uint64_t max_size);
 void lnto_introsort_pairs(int64_t *keys, double *vals, int64_t n);
+/* sort(seq) by the reference's own path for sorted input of 3000+ keys:
+ * relinearise, then RadixShuffle::permute (LibMIARadix.h:811-900).  Equal to
+ * lnto_sort_seq for power-of-two dims (the CPU baseline's path). */
+void lnto_sort_seq_shuffle(int64_t *keys, double *vals, int64_t n, int order, const int64_t *dims,
+                           const int32_t *cur_seq, const int32_t *new_seq, int is_sorted);
 void lnto_sort_seq(int64_t *keys, double *vals, int64_t n, int order,
                    const int64_t *dims, const int32_t *cur_seq,
                    const int32_t *new_seq, int is_sorted);

@@ -167,6 +167,74 @@ def test_sort_seq_is_relinearise_then_sort(orc, n, is_sorted):
         assert np.array_equal(k, rk[o]) and np.array_equal(v, vals[o])
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_radix_shuffle_sorts_power_of_two_shapes(orc, seed):
+    """The restated RadixShuffle (the reference's sort(seq) on sorted input,
+    SparseMIABase.h:622-634) gives numpy's argsort of the relinearised keys on
+    power-of-two dims, where its divisor approximation is exact: random orders
+    2-6 (sizes 2 to 128), random sequences, 3,000 to 300,000 keys (every branch of its cost
+    model: find-first, sort-first, shuffle and straight runs), plus the
+    sort/merge-at-scale permutation both ways."""
+    rng = np.random.default_rng(100 + seed)
+    cases = []
+    while len(cases) < 12:
+        order = int(rng.integers(2, 7))
+        dims = [int(2 ** rng.integers(1, 8)) for _ in range(order)]
+        tot = int(np.prod(dims))
+        if tot < 4000:
+            continue
+        n = int(rng.integers(3000, min(tot, 300000) + 1))
+        cases.append((dims, [int(x) for x in rng.permutation(order)], [int(x) for x in rng.permutation(order)], n))
+    if seed == 0:
+        cases += [([256] * 4, [0, 1, 2, 3], [0, 2, 1, 3], 400000), ([256] * 4, [0, 2, 1, 3], [0, 1, 2, 3], 400000)]
+    for i, (dims, cur, new, n) in enumerate(cases):
+        keys = orc.gen_distinct_keys(n, int(np.prod(dims)), 7 * seed + i)
+        vals = orc.gen_uniform_vals(n, i)
+        keys, vals = orc.radix_sort_pairs(_np_relin(keys, dims, list(range(len(dims))), cur), vals,
+                                          int(np.prod(dims)))
+        k, v = orc.sort_seq_shuffle(keys, vals, dims, cur, new, True)
+        rk = _np_relin(keys, dims, cur, new)
+        o = np.argsort(rk)
+        assert np.array_equal(k, rk[o]) and np.array_equal(v.view(np.int64), vals[o].view(np.int64)), (dims, cur, new)
+
+
+def test_radix_shuffle_defect_on_other_dims(orc):
+    """RadixShuffle plans a sort stage's passes from the stage's size but takes
+    digits as key >> floor(log2(divisor)); with dims {19,25,35,3} and
+    {2,3,0,1} -> {1,3,0,2} its one stage has divisor 25 and 1,995 values, so
+    key >> 4 reaches 3,117 where one 11-bit pass keeps 2,048: bins mix and the
+    reference leaves the keys unsorted.  The restatement follows the reference
+    there (a version that sized its passes from the approximated range would
+    sort them); the checker's sort_seq sorts fully."""
+    dims, cur, new = [19, 25, 35, 3], [2, 3, 0, 1], [1, 3, 0, 2]
+    keys = np.sort(orc.gen_distinct_keys(13495, int(np.prod(dims)), 5))
+    vals = orc.gen_uniform_vals(13495, 12)
+    k, _ = orc.sort_seq_shuffle(keys, vals, dims, cur, new, True)
+    rk = np.sort(_np_relin(keys, dims, cur, new))
+    assert np.array_equal(np.sort(k), rk)  # a permutation of the right keys ...
+    assert not np.all(np.diff(k) > 0)      # ... not sorted
+    k2, _ = orc.sort_seq(keys, vals, dims, cur, new, True)
+    assert np.array_equal(k2, rk)
+
+
+def test_radix_shuffle_defect_on_unit_dims(orc):
+    """setupPermute (IndexUtil.h:675-728) drops a stage of size 1; when the
+    next index is of the previous stage's kind, the stage is reused but the
+    index is not folded into it (its dims do not multiply the stage size nor
+    lower its divisor), so the stage plan misses it.  dims {4,4,1,32,32,8},
+    {0,4,3,5,1,2} -> {3,4,2,5,0,1}: new position 1 (size 32) is left out and
+    the reference's output is unsorted; the restatement follows it."""
+    dims, cur, new = [4, 4, 1, 32, 32, 8], [0, 4, 3, 5, 1, 2], [3, 4, 2, 5, 0, 1]
+    tot = int(np.prod(dims))
+    keys = np.sort(_np_relin(orc.gen_distinct_keys(47635, tot, 14), dims, list(range(6)), cur))
+    vals = orc.gen_uniform_vals(47635, 0)
+    k, _ = orc.sort_seq_shuffle(keys, vals, dims, cur, new, True)
+    rk = np.sort(_np_relin(keys, dims, cur, new))
+    assert np.array_equal(np.sort(k), rk) and not np.all(np.diff(k) > 0)
+    k2, _ = orc.sort_seq(keys, vals, dims, cur, new, True)
+    assert np.array_equal(k2, rk)
+
+
 def _v2(x):
     b = 0
     while x % 2 == 0 and b < 63:
