@@ -47,9 +47,12 @@ extern "C" {
 typedef void *(*lnt_mex_alloc)(size_t count, size_t size);
 
 /* SparseSortMex(data, idx) (MATLAB/mex/sparse_sort.cpp:15-20, 23-68): sorts
- * idx ascending in place and permutes data alongside.  The reference sorts
- * with Introsort (unstable); for unique indices the result is identical,
- * equal indices keep their input order here.  n < 2^32. */
+ * idx ascending in place and permutes data alongside.  Unique indices have
+ * one sorted order, which the radix sort gives.  The reference sorts with
+ * Introsort (unstable), so equal indices end in the order its swaps leave:
+ * when the input holds equal indices and n <= 65,536, the GPU replays that
+ * Introsort step for step (bit-identical data order); above that, equal
+ * indices keep their input order.  n < 2^32. */
 int lnt_mex_sparse_sort(lnt_context ctx, double *data, int64_t *idx, int64_t n);
 
 /* SparseChangeLinIdxMex(idx, curSeq, newSeq, dims)

@@ -28,6 +28,135 @@ __global__ void key_max_kernel(const uint64_t *__restrict__ keys, uint32_t n, un
   if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicMax(out, (unsigned long long)m);
 }
 
+// Sets *flag when two neighbouring keys of a sorted array are equal.
+__global__ void has_duplicates_kernel(const uint64_t *__restrict__ keys, uint32_t n, uint32_t *flag) {
+  bool dup = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dup |= keys[i] == keys[i + 1];
+  if (__ballot(dup) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, 1u);
+}
+
+// ------------------------------------------------- the reference's Introsort
+// Introsort with a follower, as SparseSortMex runs it (sparse_sort.cpp:14-20 ->
+// LibMIAAlgorithm.h:752-764): IntrosortRec (:570-611; partition :439-511,
+// median of three :547-560, heap sort :522-535, depth 2*floor(log2 n)+2 --
+// the bit length times two, :621-632) and a final stable insertion pass
+// (:641-660).  Equal keys end where the reference's comparisons and swaps put
+// them, which no other sort reproduces; the library runs it only when the
+// input holds equal keys (unique keys have one sorted order).  One thread does
+// every step in the reference's order, on a copy in LDS when the pairs fit.
+struct Pairs {
+  uint64_t *k;
+  double *v;
+  __device__ void swap(int64_t i, int64_t j) const {
+    const double tv = v[i];
+    v[i] = v[j];
+    v[j] = tv;
+    const uint64_t tk = k[i];
+    k[i] = k[j];
+    k[j] = tk;
+  }
+};
+
+__device__ void intro_sift_down(const Pairs &p, int64_t base, int64_t size, int64_t root) {
+  while (root * 2 + 1 < size) {
+    int64_t child = root * 2 + 1;
+    if (child + 1 < size && p.k[base + child] < p.k[base + child + 1]) child += 1;
+    if (p.k[base + root] < p.k[base + child]) {
+      p.swap(base + child, base + root);
+      root = child;
+    } else {
+      return;
+    }
+  }
+}
+
+__device__ void intro_heap_sort(const Pairs &p, int64_t base, int64_t size) {
+  for (int64_t s = (size - 2) / 2; s >= 0; --s) intro_sift_down(p, base, size, s);
+  for (int64_t e = size - 1; e > 0; --e) {
+    p.swap(base + e, base);
+    intro_sift_down(p, base, e, 0);
+  }
+}
+
+__device__ int64_t intro_median(const Pairs &p, int64_t one, int64_t two, int64_t three) {
+  const bool c12 = p.k[one] < p.k[two], c13 = p.k[one] < p.k[three], c23 = p.k[two] < p.k[three];
+  if (c12 && c23) return two;
+  if (c12 && !c23 && c13) return three;
+  if (!c12 && c13) return one;
+  if (!c12 && !c13 && c23) return three;
+  if (c12 && !c13) return one;
+  return two;
+}
+
+__device__ int64_t intro_partition(const Pairs &p, int64_t b, int64_t e) {
+  int64_t lhs = b + 1, rhs = e - 1;
+  for (;;) {
+    while (lhs < rhs && !(p.k[rhs] < p.k[b])) --rhs;
+    while (lhs < rhs && p.k[lhs] < p.k[b]) ++lhs;
+    if (lhs == rhs) break;
+    p.swap(lhs, rhs);
+  }
+  if (!(p.k[lhs] < p.k[b])) return b;
+  p.swap(b, lhs);
+  return lhs;
+}
+
+// The recursion's subarrays are disjoint, so an explicit stack visits them
+// in any order with the same result; each keeps its own depth budget.
+__device__ void intro_sort(const Pairs &p, int64_t n) {
+  struct Range {
+    int64_t b, e;
+    uint32_t depth;
+  } stack[80];
+  int top = 0;
+  uint32_t depth = 0;
+  for (uint64_t m = (uint64_t)n; m; m >>= 1) ++depth;
+  stack[top++] = {0, n, depth * 2};
+  while (top) {
+    Range r = stack[--top];
+    for (;;) {
+      const int64_t len = r.e - r.b;
+      if (len < 50) break;
+      if (r.depth == 0) {
+        intro_heap_sort(p, r.b, len);
+        break;
+      }
+      const int64_t pivot = intro_median(p, r.b, r.b + len / 2, r.e - 1);
+      p.swap(pivot, r.b);
+      const int64_t q = intro_partition(p, r.b, r.e);
+      stack[top++] = {q + 1, r.e, r.depth - 1};
+      r = {r.b, q, r.depth - 1};
+    }
+  }
+  for (int64_t i = 1; i < n; ++i)
+    for (int64_t t = i; t > 0 && p.k[t] < p.k[t - 1]; --t) p.swap(t, t - 1);
+}
+
+constexpr uint32_t kIntroLds = 8192;       // pairs sorted in LDS (128 KiB)
+constexpr uint32_t kIntroMax = 1u << 16;   // largest input replayed exactly
+
+__global__ __launch_bounds__(kBlock) void introsort_kernel(uint64_t *__restrict__ keys, double *__restrict__ vals,
+                                                           uint32_t n) {
+  __shared__ uint64_t s_k[kIntroLds];
+  __shared__ double s_v[kIntroLds];
+  if (n <= kIntroLds) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      s_k[i] = keys[i];
+      s_v[i] = vals[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) intro_sort(Pairs{s_k, s_v}, n);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      keys[i] = s_k[i];
+      vals[i] = s_v[i];
+    }
+  } else if (threadIdx.x == 0) {
+    intro_sort(Pairs{keys, vals}, n);
+  }
+}
+
 // v[i] = -1 * v[i]: the subtraction from an empty left operand
 // (MATLAB/mex/sparse_merge.cpp:52-54).
 __global__ void negate_kernel(double *v, uint32_t n) {
@@ -136,6 +265,22 @@ int lnt_mex_sparse_sort(lnt_context ctx, double *data, int64_t *idx, int64_t n) 
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   if (top >> 63) return fail(LNT_ERR_KEY_RANGE, "an index is negative");
   LNT_TRY(sort_device(ctx, (uint64_t *)st.b[0].ptr, (double *)st.b[1].ptr, (uint32_t)n, top + 1, nullptr));
+  // Equal indices: the radix sort kept their input order, the reference's
+  // Introsort leaves them in the order its swaps produce.  Start again from
+  // the caller's arrays and replay that Introsort (up to kIntroMax entries:
+  // one thread replays it, beyond LDS in global memory).
+  uint32_t *dup = (uint32_t *)ctx->small.ptr + kSmallFlags + 3;
+  LNT_TRY(launch_fill_u32(ctx, dup, 0, 1));
+  has_duplicates_kernel<<<grid_for((uint32_t)n), kBlock, 0, ctx->stream>>>((const uint64_t *)st.b[0].ptr,
+                                                                          (uint32_t)n, dup);
+  LNT_HIP(hipGetLastError());
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 3, dup, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->pinned[3] && (uint64_t)n <= kIntroMax) {
+    LNT_TRY(stage_in(ctx, st, 0, idx, data, n));
+    introsort_kernel<<<1, kBlock, 0, ctx->stream>>>((uint64_t *)st.b[0].ptr, (double *)st.b[1].ptr, (uint32_t)n);
+    LNT_HIP(hipGetLastError());
+  }
   return copy_out(ctx, idx, data, st.b[0].ptr, st.b[1].ptr, n);
 }
 
