@@ -155,7 +155,10 @@ int lnt_change_sort_order(lnt_context ctx, int64_t *keys, double *vals,
  * (multi_index_array/FunctionUtil.h:373-419, reached from
  * SparseMIABase::outside_scanMerge, SparseMIABase.h:1404-1455), the pure
  * Hadamard case of perform_noLatticeMult (SparseMIABase.h:1717-1826) and
- * SparseMergeMex (MATLAB/mex/sparse_merge.cpp:15-162).  na + nb < 2^32. */
+ * SparseMergeMex (MATLAB/mex/sparse_merge.cpp:15-162).  Inputs of 2^32
+ * entries or more together are merged in chunks of at most 2^31 merge-path
+ * steps, each cut so that a key present in both inputs stays in one chunk;
+ * the cut is read back to the host once per chunk. */
 int lnt_merge_into(lnt_context ctx, const int64_t *a_keys,
                    const double *a_vals, int64_t na, const int64_t *b_keys,
                    const double *b_vals, int64_t nb, int op, int64_t *c_keys,

@@ -152,7 +152,7 @@ static void prof_clear(lnt_context ctx) {
 
 namespace {
 
-__global__ void check_sorted_kernel(const uint64_t *keys, uint32_t n, uint32_t *err) {
+__global__ void check_sorted_kernel(const uint64_t *keys, uint64_t n, uint32_t *err) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool bad = false;
   for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -163,12 +163,12 @@ __global__ void check_sorted_kernel(const uint64_t *keys, uint32_t n, uint32_t *
   if (bad) atomicOr(err, 1u);
 }
 
-int check_sorted(lnt_context ctx, const uint64_t *a, uint32_t na, const uint64_t *b, uint32_t nb) {
+int check_sorted(lnt_context ctx, const uint64_t *a, uint64_t na, const uint64_t *b, uint64_t nb) {
   LNT_TRY(ensure(ctx->small, kSmallWords * sizeof(uint32_t)));
   uint32_t *err = (uint32_t *)ctx->small.ptr + kSmallFlags + 2;
   LNT_TRY(launch_fill_u32(ctx, err, 0, 1));
-  if (na) check_sorted_kernel<<<std::min<uint32_t>((na + 255) / 256, 4096), 256, 0, ctx->stream>>>(a, na, err);
-  if (nb) check_sorted_kernel<<<std::min<uint32_t>((nb + 255) / 256, 4096), 256, 0, ctx->stream>>>(b, nb, err);
+  if (na) check_sorted_kernel<<<(uint32_t)std::min<uint64_t>((na + 255) / 256, 4096), 256, 0, ctx->stream>>>(a, na, err);
+  if (nb) check_sorted_kernel<<<(uint32_t)std::min<uint64_t>((nb + 255) / 256, 4096), 256, 0, ctx->stream>>>(b, nb, err);
   LNT_HIP(hipGetLastError());
   LNT_HIP(hipMemcpyAsync(ctx->pinned + 2, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
   LNT_HIP(hipStreamSynchronize(ctx->stream));
@@ -343,17 +343,16 @@ int lnt_change_linidx(lnt_context ctx, int64_t *keys, int64_t n, int order,
   LNT_CTX(ctx);
   LNT_TRY(check_mem(mem));
   if (n < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
-  if (n >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "n must be < 2^32");
   Relin r;
   uint64_t dimensionality;
   LNT_TRY(make_relin(order, dims, cur_seq, new_seq, &r, &dimensionality));
   if (n == 0 || !r.active) return LNT_OK;
   if (!keys) return fail(LNT_ERR_INVALID_ARGUMENT, "null keys");
-  if (mem == LNT_MEM_DEVICE) return relin_device(ctx, (uint64_t *)keys, (uint32_t)n, r);
+  if (mem == LNT_MEM_DEVICE) return relin_device_large(ctx, (uint64_t *)keys, (uint64_t)n, r);
   Staging st;
   LNT_TRY(ensure(st.b[0], (size_t)n * 8));
   LNT_HIP(hipMemcpyAsync(st.b[0].ptr, keys, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
-  LNT_TRY(relin_device(ctx, (uint64_t *)st.b[0].ptr, (uint32_t)n, r));
+  LNT_TRY(relin_device_large(ctx, (uint64_t *)st.b[0].ptr, (uint64_t)n, r));
   LNT_HIP(hipMemcpyAsync(keys, st.b[0].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   return LNT_OK;
@@ -362,17 +361,20 @@ int lnt_change_linidx(lnt_context ctx, int64_t *keys, int64_t n, int order,
 static int sort_common(lnt_context ctx, int64_t *keys, double *vals, int64_t n,
                        uint64_t bound, const Relin *relin, int mem, uint32_t sorted_low = 0) {
   if (n < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
-  if (n >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "n must be < 2^32");
   if (n == 0) return LNT_OK;
   if (!keys || !vals) return fail(LNT_ERR_INVALID_ARGUMENT, "null keys or values");
-  if (mem == LNT_MEM_DEVICE)
-    return sort_device(ctx, (uint64_t *)keys, vals, (uint32_t)n, bound, relin, sorted_low);
+  // 2^32 entries or more: chunked (sort_device_large).
+  auto run = [&](uint64_t *k, double *v) -> int {
+    if (n < (int64_t(1) << 32)) return sort_device(ctx, k, v, (uint32_t)n, bound, relin, sorted_low);
+    return sort_device_large(ctx, k, v, (uint64_t)n, bound, relin);
+  };
+  if (mem == LNT_MEM_DEVICE) return run((uint64_t *)keys, vals);
   Staging st;
   LNT_TRY(ensure(st.b[0], (size_t)n * 8));
   LNT_TRY(ensure(st.b[1], (size_t)n * 8));
   LNT_HIP(hipMemcpyAsync(st.b[0].ptr, keys, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
   LNT_HIP(hipMemcpyAsync(st.b[1].ptr, vals, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
-  int rc = sort_device(ctx, (uint64_t *)st.b[0].ptr, (double *)st.b[1].ptr, (uint32_t)n, bound, relin, sorted_low);
+  int rc = run((uint64_t *)st.b[0].ptr, (double *)st.b[1].ptr);
   if (rc != LNT_OK) return rc;
   LNT_HIP(hipMemcpyAsync(keys, st.b[0].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
   LNT_HIP(hipMemcpyAsync(vals, st.b[1].ptr, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -453,20 +455,31 @@ int lnt_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
   if (op != LNT_OP_ADD && op != LNT_OP_SUB && op != LNT_OP_HADAMARD)
     return fail(LNT_ERR_INVALID_ARGUMENT, "op must be LNT_OP_ADD, LNT_OP_SUB or LNT_OP_HADAMARD");
   if (na < 0 || nb < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "negative length");
-  if (na + nb >= (int64_t(1) << 32)) return fail(LNT_ERR_TOO_LARGE, "na + nb must be < 2^32");
+  if (na + nb >= (int64_t(1) << 62)) return fail(LNT_ERR_TOO_LARGE, "na + nb must be < 2^62");
   int64_t need = op == LNT_OP_HADAMARD ? std::min(na, nb) : na + nb;
   if (capacity < need)
     return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) +
                                               " is below the bound " + std::to_string(need));
   if ((na && (!a_keys || !a_vals)) || (nb && (!b_keys || !b_vals)) || (need && (!c_keys || !c_vals)))
     return fail(LNT_ERR_INVALID_ARGUMENT, "null array");
-  uint32_t count = 0;
+  // Inputs of 2^32 entries or more go through in merge-path chunks.
+  auto run = [&](const uint64_t *ak, const double *av, const uint64_t *bk, const double *bv, uint64_t *ck,
+                 double *cv, uint64_t cap, uint64_t *out) -> int {
+    if (na + nb < (int64_t(1) << 32)) {
+      uint32_t c = 0;
+      LNT_TRY(merge_device(ctx, ak, av, (uint32_t)na, bk, bv, (uint32_t)nb, op, ck, cv, cap, &c));
+      *out = c;
+      return LNT_OK;
+    }
+    return merge_device_large(ctx, ak, av, (uint64_t)na, bk, bv, (uint64_t)nb, op, ck, cv, cap, out);
+  };
+  uint64_t count = 0;
   if (mem == LNT_MEM_DEVICE) {
     if (ctx->validate >= 2)
-      LNT_TRY(check_sorted(ctx, (const uint64_t *)a_keys, (uint32_t)na, (const uint64_t *)b_keys, (uint32_t)nb));
-    LNT_TRY(merge_device(ctx, (const uint64_t *)a_keys, a_vals, (uint32_t)na, (const uint64_t *)b_keys,
-                         b_vals, (uint32_t)nb, op, (uint64_t *)c_keys, c_vals, (uint64_t)capacity, &count));
-    *nc = count;
+      LNT_TRY(check_sorted(ctx, (const uint64_t *)a_keys, (uint64_t)na, (const uint64_t *)b_keys, (uint64_t)nb));
+    LNT_TRY(run((const uint64_t *)a_keys, a_vals, (const uint64_t *)b_keys, b_vals, (uint64_t *)c_keys, c_vals,
+                (uint64_t)capacity, &count));
+    *nc = (int64_t)count;
     return LNT_OK;
   }
   Staging st;
@@ -486,16 +499,15 @@ int lnt_merge_into(lnt_context ctx, const int64_t *a_keys, const double *a_vals,
     LNT_HIP(hipMemcpyAsync(st.b[3].ptr, b_vals, sb, hipMemcpyHostToDevice, ctx->stream));
   }
   if (ctx->validate >= 2)
-    LNT_TRY(check_sorted(ctx, (const uint64_t *)st.b[0].ptr, (uint32_t)na, (const uint64_t *)st.b[2].ptr, (uint32_t)nb));
-  LNT_TRY(merge_device(ctx, (const uint64_t *)st.b[0].ptr, (const double *)st.b[1].ptr, (uint32_t)na,
-                       (const uint64_t *)st.b[2].ptr, (const double *)st.b[3].ptr, (uint32_t)nb, op,
-                       (uint64_t *)st.b[4].ptr, (double *)st.b[5].ptr, (uint64_t)need, &count));
+    LNT_TRY(check_sorted(ctx, (const uint64_t *)st.b[0].ptr, (uint64_t)na, (const uint64_t *)st.b[2].ptr, (uint64_t)nb));
+  LNT_TRY(run((const uint64_t *)st.b[0].ptr, (const double *)st.b[1].ptr, (const uint64_t *)st.b[2].ptr,
+              (const double *)st.b[3].ptr, (uint64_t *)st.b[4].ptr, (double *)st.b[5].ptr, (uint64_t)need, &count));
   if (count) {
     LNT_HIP(hipMemcpyAsync(c_keys, st.b[4].ptr, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
     LNT_HIP(hipMemcpyAsync(c_vals, st.b[5].ptr, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
     LNT_HIP(hipStreamSynchronize(ctx->stream));
   }
-  *nc = count;
+  *nc = (int64_t)count;
   return LNT_OK;
 }
 

@@ -505,6 +505,9 @@ uint32_t sorted_low_bits(int order, const int64_t *dims, const int32_t *cur_seq,
 int compare_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
                    const double *bv, uint32_t nb, int mode, double prec, int *equal);
 int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin);
+int relin_device_large(lnt_context ctx, uint64_t *keys, uint64_t n, const Relin &relin);
+int sort_device_large(lnt_context ctx, uint64_t *keys, double *vals, uint64_t n, uint64_t bound,
+                      const Relin *relin);
 int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na,
                         const int64_t a_dims[3], const uint64_t *bk, const double *bv,
                         uint64_t nb, const int64_t b_dims[3], int strategy, uint64_t *ck,
@@ -523,6 +526,9 @@ int noalloc_scratch_init(lnt_context ctx);
 int merge_noalloc_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
                          const double *bv, uint32_t nb, int op, uint64_t *ck, double *cv, uint64_t capacity,
                          uint32_t *nc_host);
+int merge_device_large(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na, const uint64_t *bk,
+                       const double *bv, uint64_t nb, int op, uint64_t *ck, double *cv, uint64_t capacity,
+                       uint64_t *nc_host);
 int sort_merge_device(lnt_context ctx, const uint64_t *ak, const double *av, uint32_t na, const uint64_t *bk,
                       const double *bv, uint32_t nb, int op, uint64_t bound, uint64_t *ck, double *cv,
                       uint64_t capacity, uint64_t *nc_host);

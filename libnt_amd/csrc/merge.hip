@@ -551,6 +551,61 @@ int merge_noalloc_device(lnt_context ctx, const uint64_t *ak, const double *av, 
   return merge_with(ctx, ak, av, na, bk, bv, nb, op, ck, cv, capacity, m, nc_host);
 }
 
+// ------------------------------------------------- merges of 2^32 and more
+namespace {
+// The merge-path point at diagonal `diag` of the whole arrays (64-bit), moved
+// one step on when A's last key before it equals B's next key, so that the pair
+// stays in one chunk: out[0] = A entries, out[1] = B entries before the point.
+__global__ void split_point_kernel(const uint64_t *__restrict__ a, uint64_t na, const uint64_t *__restrict__ b,
+                                   uint64_t nb, uint64_t diag, uint64_t *out) {
+  uint64_t lo = diag > nb ? diag - nb : 0, hi = diag < na ? diag : na;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= b[diag - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  uint64_t j = diag - lo;
+  if (lo > 0 && j < nb && a[lo - 1] == b[j]) ++j;
+  out[0] = lo;
+  out[1] = j;
+}
+}  // namespace
+
+// A merge whose inputs hold 2^32 entries or more together: the merge path is
+// cut into chunks of at most 2^31 steps (plus the one step that keeps an
+// equal pair together), each merged by merge_device into C at the running
+// output count.
+int merge_device_large(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na, const uint64_t *bk,
+                       const double *bv, uint64_t nb, int op, uint64_t *ck, double *cv, uint64_t capacity,
+                       uint64_t *nc_host) {
+  constexpr uint64_t kChunkSteps = uint64_t(1) << 31;
+  const uint64_t total = na + nb;
+  LNT_TRY(ensure(ctx->small, kSmallWords * sizeof(uint32_t)));
+  uint64_t *pt = (uint64_t *)((uint32_t *)ctx->small.ptr + kSmallFlags + 6);
+  uint64_t i = 0, j = 0, out = 0;
+  while (i + j < total) {
+    uint64_t ie = na, je = nb;
+    if (i + j + kChunkSteps < total) {
+      split_point_kernel<<<1, 1, 0, ctx->stream>>>(ak, na, bk, nb, i + j + kChunkSteps, pt);
+      LNT_HIP(hipGetLastError());
+      LNT_HIP(hipMemcpyAsync(ctx->pinned + 12, pt, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+      LNT_HIP(hipStreamSynchronize(ctx->stream));
+      uint64_t p[2];
+      memcpy(p, ctx->pinned + 12, sizeof(p));
+      ie = p[0];
+      je = p[1];
+    }
+    uint32_t count = 0;
+    LNT_TRY(merge_device(ctx, ak + i, av + i, (uint32_t)(ie - i), bk + j, bv + j, (uint32_t)(je - j), op, ck + out,
+                         cv + out, capacity - out, &count));
+    out += count;
+    i = ie;
+    j = je;
+  }
+  *nc_host = out;
+  return LNT_OK;
+}
+
 template <typename T>
 int exclusive_scan(lnt_context ctx, const T *in, uint64_t *out, uint64_t n, uint64_t *total_dev,
                    DeviceBuffer &tmp, size_t tmp_offset);

@@ -20,6 +20,7 @@
 //   scratch buffers so the result still lands in the caller's arrays.
 #include <algorithm>
 #include <atomic>
+#include <vector>
 
 #include "lnt_internal.h"
 
@@ -376,7 +377,7 @@ int launch_pass(lnt_context ctx, bool relin, int win, int wout, const void *kin,
                 uint32_t shift, uint32_t dbits, const uint32_t *pass_hist,
                 uint32_t *counter, const RelinDev *r, const uint32_t *stop, uint32_t tag) {
   constexpr uint32_t tile = BT * ITEMS;
-  uint32_t tiles = (n + tile - 1) / tile;
+  uint32_t tiles = (uint32_t)(((uint64_t)n + tile - 1) / tile);
   LNT_TRY(ensure_status(ctx, (size_t)tiles * kRadix));
   uint32_t epoch;
   LNT_TRY(next_epoch(ctx, &epoch));
@@ -453,8 +454,7 @@ static uint32_t next_check_tag() {
 
 int relin_device(lnt_context ctx, uint64_t *keys, uint32_t n, const Relin &relin) {
   if (n == 0 || !relin.active) return LNT_OK;
-  uint32_t blocks = (n + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
+  uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192);
   int slot = prof_begin(ctx, kProfRelin, 16.0 * n);
   relin_kernel<<<blocks, 256, 0, ctx->stream>>>(keys, n, relin_for_device(relin));
   prof_end(ctx, slot);
@@ -520,7 +520,7 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
   const uint32_t tag = next_check_tag();
 
   uint32_t hblocks = (uint32_t)ctx->tuning.hist_blocks;
-  uint32_t need = (n + kBlock * 8 - 1) / (kBlock * 8);
+  uint32_t need = (uint32_t)(((uint64_t)n + kBlock * 8 - 1) / (kBlock * 8));
   if (hblocks > need) hblocks = need;
   const uint32_t width = (uint32_t)(passes > 0 ? passes : 1) * kRadix;
   LNT_TRY(ensure(ctx->hist_part, (size_t)kMaxHistBlocks * kMaxPasses * kRadix * sizeof(uint32_t)));
@@ -619,6 +619,154 @@ int sort_device(lnt_context ctx, uint64_t *keys, double *vals, uint32_t n,
     src = dst;
   }
   return key_range_ok();
+}
+
+namespace {
+// pos[b] = first position of a key >= b << shift in a run partitioned by
+// key >> shift (pos[nb] = n).
+__global__ void bucket_bounds_kernel(const uint64_t *__restrict__ keys, uint64_t n, uint32_t shift, uint32_t nb,
+                                     uint64_t *__restrict__ pos) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nb) return;
+  uint64_t lo = 0, hi = n;
+  if (b == nb) lo = n;
+  const uint64_t t = (uint64_t)b << shift;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if ((keys[mid] >> shift) < (t >> shift)) lo = mid + 1;
+    else hi = mid;
+  }
+  pos[b] = lo;
+}
+
+// Moves each entry of one partitioned run to its bucket's place in the
+// output: position + delta[bucket].  Keys beyond the bound (unvalidated input)
+// are clamped to the last bucket and writes past `total` are dropped, so that
+// every access stays in bounds.
+__global__ void bucket_gather_kernel(const uint64_t *__restrict__ k, const double *__restrict__ v, uint64_t n,
+                                     uint32_t shift, uint32_t nb, const int64_t *__restrict__ delta,
+                                     uint64_t total, uint64_t *__restrict__ ok, double *__restrict__ ov) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t key = k[i];
+    const uint64_t b = (key >> shift) < nb ? (key >> shift) : nb - 1;
+    const uint64_t d = (uint64_t)((int64_t)i + delta[b]);
+    if (d < total) {
+      ok[d] = key;
+      ov[d] = v[i];
+    }
+  }
+}
+
+__global__ void key_bound_kernel(const uint64_t *__restrict__ k, uint64_t n, uint64_t bound, uint32_t *err) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) bad |= k[i] >= bound;
+  if (bad) *err = 1;
+}
+}  // namespace
+
+int relin_device_large(lnt_context ctx, uint64_t *keys, uint64_t n, const Relin &relin) {
+  constexpr uint64_t kChunk = uint64_t(1) << 31;
+  for (uint64_t s = 0; s < n; s += kChunk)
+    LNT_TRY(relin_device(ctx, keys + s, (uint32_t)std::min(kChunk, n - s), relin));
+  return LNT_OK;
+}
+
+// A sort of 2^32 entries or more, stable like sort_device:
+//  1. each chunk of 2^31 entries is relinearised and stably ordered by the
+//     top h = min(bits, 16) key bits only (sort_device with the low bits
+//     declared ordered: two 8-bit passes);
+//  2. the 2^h bucket boundaries of every chunk are found by binary search;
+//  3. every chunk's buckets are gathered into bucket order (chunk order
+//     within a bucket, so equal keys keep their input order);
+//  4. consecutive buckets are grouped up to 2^31 entries and each group is
+//     sorted on all bits by sort_device (a bucket of one key value needs no
+//     sort: when h = bits every bucket is one value);
+//  5. the result is copied back in place.
+// Work memory: 16 bytes per entry for the gathered copy, plus sort_device's
+// ping-pong buffers for one chunk or group.
+int sort_device_large(lnt_context ctx, uint64_t *keys, double *vals, uint64_t n, uint64_t bound,
+                      const Relin *relin) {
+  constexpr uint64_t kChunk = uint64_t(1) << 31;
+  uint32_t bits = 0;
+  while (bits < 64 && (uint64_t(1) << bits) < bound) ++bits;
+  const uint32_t h = std::min<uint32_t>(bits, 16), shift = bits - h, nb = 1u << h;
+  const uint64_t chunks = (n + kChunk - 1) / kChunk;
+  if (ctx->validate) {  // before any chunk moves, so that a bad key leaves the arrays as they were
+    LNT_TRY(ensure(ctx->small, kSmallWords * sizeof(uint32_t)));
+    uint32_t *err = (uint32_t *)ctx->small.ptr + kSmallFlags + 2;
+    LNT_TRY(launch_fill_u32(ctx, err, 0, 1));
+    key_bound_kernel<<<16384, 256, 0, ctx->stream>>>(keys, n, bound, err);
+    LNT_HIP(hipGetLastError());
+    LNT_HIP(hipMemcpyAsync(ctx->pinned, err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->pinned[0]) return fail(LNT_ERR_KEY_RANGE, "a key is negative or not below the dimensionality");
+  }
+  for (uint64_t c = 0; c < chunks; ++c) {
+    const uint64_t s = c * kChunk;
+    LNT_TRY(sort_device(ctx, keys + s, vals + s, (uint32_t)std::min(kChunk, n - s), bound, relin, shift));
+  }
+  Staging w;
+  LNT_TRY(ensure(w.b[0], chunks * (nb + 1) * sizeof(uint64_t)));
+  LNT_TRY(ensure(w.b[1], chunks * nb * sizeof(int64_t)));
+  LNT_TRY(ensure(w.b[2], n * sizeof(uint64_t)));
+  LNT_TRY(ensure(w.b[3], n * sizeof(double)));
+  uint64_t *pos = (uint64_t *)w.b[0].ptr;
+  for (uint64_t c = 0; c < chunks; ++c) {
+    const uint64_t s = c * kChunk;
+    bucket_bounds_kernel<<<(nb + 1 + 255) / 256, 256, 0, ctx->stream>>>(keys + s, std::min(kChunk, n - s), shift,
+                                                                        nb, pos + c * (nb + 1));
+    LNT_HIP(hipGetLastError());
+  }
+  std::vector<uint64_t> hp(chunks * (nb + 1));
+  LNT_HIP(hipMemcpyAsync(hp.data(), pos, hp.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  // Bucket sizes, their places in the output, and each chunk's offsets.
+  std::vector<uint64_t> base(nb + 1, 0);
+  for (uint32_t b = 0; b < nb; ++b) {
+    uint64_t t = 0;
+    for (uint64_t c = 0; c < chunks; ++c) {
+      const uint64_t lo = hp[c * (nb + 1) + b], hi = hp[c * (nb + 1) + b + 1];
+      if (hi < lo) return fail(LNT_ERR_KEY_RANGE, "a key is negative or not below the dimensionality");
+      t += hi - lo;
+    }
+    if (t >= (uint64_t(1) << 32) && shift > 0)
+      return fail(LNT_ERR_TOO_LARGE, "2^32 or more keys share their top 16 bits");
+    base[b + 1] = base[b] + t;
+  }
+  std::vector<int64_t> delta(chunks * nb);
+  for (uint32_t b = 0; b < nb; ++b) {
+    uint64_t at = base[b];
+    for (uint64_t c = 0; c < chunks; ++c) {
+      const uint64_t lo = hp[c * (nb + 1) + b], hi = hp[c * (nb + 1) + b + 1];
+      delta[c * nb + b] = (int64_t)at - (int64_t)lo;
+      at += hi - lo;
+    }
+  }
+  int64_t *ddev = (int64_t *)w.b[1].ptr;
+  LNT_HIP(hipMemcpyAsync(ddev, delta.data(), delta.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+  uint64_t *tk = (uint64_t *)w.b[2].ptr;
+  double *tv = (double *)w.b[3].ptr;
+  for (uint64_t c = 0; c < chunks; ++c) {
+    const uint64_t s = c * kChunk;
+    bucket_gather_kernel<<<16384, 256, 0, ctx->stream>>>(keys + s, vals + s, std::min(kChunk, n - s), shift, nb,
+                                                         ddev + c * nb, n, tk, tv);
+    LNT_HIP(hipGetLastError());
+  }
+  if (shift > 0) {
+    for (uint32_t b = 0; b < nb;) {
+      uint32_t e = b + 1;
+      while (e < nb && base[e + 1] - base[b] <= kChunk) ++e;
+      const uint64_t len = base[e] - base[b];
+      if (len > 1) LNT_TRY(sort_device(ctx, tk + base[b], tv + base[b], (uint32_t)len, bound, nullptr, 0));
+      b = e;
+    }
+  }
+  LNT_HIP(hipMemcpyAsync(keys, tk, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+  LNT_HIP(hipMemcpyAsync(vals, tv, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  LNT_HIP(hipStreamSynchronize(ctx->stream));
+  return LNT_OK;
 }
 
 }  // namespace lnt
