@@ -9,7 +9,7 @@ Python package and nothing in the library imports this file:
     python libnt_amd/data/make_dispatch_data.py --part test --start 0 --count 2000 --out DIR
     python libnt_amd/data/make_dispatch_data.py --part fit --out DIR
 
-(chunks keep each run short).  Writes DIR/dispatch_timings.npz (features, log2-ms timings of all 8
+(chunks keep each run short).  Writes DIR/dispatch_timings.npz (features, log2-ms timings of all 20
 strategies, the restated polyalgorithm's choice) and DIR/dispatch_model.npy
 (the model trained on the training split by the library's GPU kernels).
 Instances come from ``dispatch.mixed_specs`` (train seed 1, held-out seed 2).
