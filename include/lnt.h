@@ -105,6 +105,10 @@ int lnt_abi_version(void);
  *                  the ranks of A's rows among the rows present instead of
  *                  the rows when that saves a radix pass or gives 32-bit keys
  *                  (one extra pass over A's keys); 0: rows
+ *   "spgemm_pieces" 1 (default): long output columns are cut by row value into
+ *                  pieces merged in LDS (the global sort takes only columns
+ *                  with more than 64 B entries or rows clustered beyond a
+ *                  piece); 0: every long column through the global sort
  * No reference counterpart (the reference hard-codes radix 2048 and the
  * thresholds 3000/50, multi_index_array/LibMIARadix.h:454-457). */
 int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value);
@@ -129,7 +133,11 @@ int lnt_change_linidx(lnt_context ctx, int64_t *keys, int64_t n, int order,
  * unique and in [0, max_size).  Replaces SparseMIABase::sort()
  * (multi_index_array/SparseMIABase.h:701-734 ->
  * LibMIARadix.h:444-484 RadixSortInPlace_PowerOf2Radix_Unsigned) and
- * SparseSortMex (MATLAB/mex/sparse_sort.cpp:14-68).  n < 2^32. */
+ * SparseSortMex (MATLAB/mex/sparse_sort.cpp:14-68).  Equal keys keep their
+ * input order.  From 2^32 entries on, the sort runs in chunks of 2^31: each
+ * chunk ordered by the top 16 key bits, the chunks gathered by those bits,
+ * then each group of about 2^31 entries sorted (16 bytes per entry of work
+ * memory; LNT_ERR_TOO_LARGE if 2^32 keys share their top 16 bits). */
 int lnt_sort_pairs(lnt_context ctx, int64_t *keys, double *vals, int64_t n,
                    uint64_t max_size, int mem);
 
@@ -140,7 +148,7 @@ int lnt_sort_pairs(lnt_context ctx, int64_t *keys, double *vals, int64_t n,
  * SparseMIABase::sort(seq) (multi_index_array/SparseMIABase.h:590-645, which
  * runs change_linIdx_sequence + RadixShuffle::permute, LibMIARadix.h:811-900)
  * and SparsePermuteMex (MATLAB/mex/sparse_permute.cpp:15-116, which expects the
- * keys already relinearised).  n < 2^32. */
+ * keys already relinearised).  Any n (as lnt_sort_pairs). */
 int lnt_change_sort_order(lnt_context ctx, int64_t *keys, double *vals,
                           int64_t n, int order, const int64_t *dims,
                           const int32_t *cur_seq, const int32_t *new_seq,

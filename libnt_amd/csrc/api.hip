@@ -226,7 +226,7 @@ int lnt_context_destroy(lnt_context ctx) {
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
                           &ctx->host_tmp_b, &ctx->host_tmp_c, &ctx->spg_ws, &ctx->spg_tmp,
-                          &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->hist_part, &ctx->relin_dev, &ctx->exec_buf[0],
+                          &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->spg_piece, &ctx->hist_part, &ctx->relin_dev, &ctx->exec_buf[0],
                           &ctx->exec_buf[1], &ctx->exec_buf[2], &ctx->exec_buf[3], &ctx->exec_buf[4],
                           &ctx->exec_buf[5]})
     release(*b);
@@ -249,6 +249,7 @@ int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
   else if (n == "spgemm_small" && in(2, 2048)) ctx->tuning.spgemm_small = (int)value;
   else if (n == "narrow_keys" && in(0, 1)) ctx->tuning.narrow_keys = (int)value;
   else if (n == "row_ranks" && in(0, 1)) ctx->tuning.row_ranks = (int)value;
+  else if (n == "spgemm_pieces" && in(0, 1)) ctx->tuning.spgemm_pieces = (int)value;
   else if (n == "digit_bits" && in(4, 8)) ctx->tuning.digit_bits = (int)value;
   else if (n == "sort_block" && (value == 256 || value == 512 || value == 1024)) ctx->tuning.sort_block = (int)value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter or value out of range: " + n);
@@ -265,6 +266,7 @@ int lnt_context_get_param(lnt_context ctx, const char *name, int64_t *value) {
   else if (n == "spgemm_small") *value = ctx->tuning.spgemm_small;
   else if (n == "narrow_keys") *value = ctx->tuning.narrow_keys;
   else if (n == "row_ranks") *value = ctx->tuning.row_ranks;
+  else if (n == "spgemm_pieces") *value = ctx->tuning.spgemm_pieces;
   else if (n == "digit_bits") *value = ctx->tuning.digit_bits;
   else if (n == "sort_block") *value = ctx->tuning.sort_block;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter: " + n);
@@ -324,7 +326,7 @@ int lnt_context_trim(lnt_context ctx) {
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->aux, &ctx->host_tmp_a, &ctx->host_tmp_b, &ctx->host_tmp_c,
-                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->exec_buf[0], &ctx->exec_buf[1],
+                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->spg_piece, &ctx->exec_buf[0], &ctx->exec_buf[1],
                           &ctx->exec_buf[2], &ctx->exec_buf[3], &ctx->exec_buf[4], &ctx->exec_buf[5]})
     release(*b);
   return LNT_OK;

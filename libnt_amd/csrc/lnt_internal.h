@@ -424,6 +424,7 @@ struct SortTuning {
   int spgemm_small = 2048;  // largest output column handled in LDS
   int narrow_keys = 1;  // 32-bit keys between passes when they fit
   int row_ranks = 1;    // lattice product: long columns sort row ranks when that saves passes
+  int spgemm_pieces = 1;  // lattice product: long columns in row pieces (0: all through the global sort)
   int digit_bits = 8;  // largest radix digit per onesweep pass (4..8)
   int sort_block = 1024;  // onesweep threads per block (256, 512, 1024): tile = sort_block * items
 };
@@ -453,7 +454,7 @@ struct lnt_context_s {
   lnt::DeviceBuffer aux;        // merge-path splits and other per-call scratch
   lnt::DeviceBuffer hist_part;  // per-block digit histograms
   lnt::DeviceBuffer relin_dev;  // relinearisation table of the current sort's radix passes
-  lnt::DeviceBuffer spg_ws, spg_tmp, spg_big, spg_rank, spg_cp;  // lattice-product workspaces
+  lnt::DeviceBuffer spg_ws, spg_tmp, spg_big, spg_rank, spg_cp, spg_piece;  // lattice-product workspaces
   lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
   lnt::DeviceBuffer exec_buf[6];  // operand copies of a whole contraction (keys, values of both), timing output
   void *model = nullptr;          // lnt_model used by LNT_STRAT_AUTO (not owned)

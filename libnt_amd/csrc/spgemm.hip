@@ -564,6 +564,352 @@ __global__ void large_counts_kernel(const uint32_t *__restrict__ list3, uint32_t
   seg_u[list3[r]] = hi - lo;
 }
 
+// 3c. large segments cut into row pieces.  A segment's rows [rmin, rmax] are
+// split into q equal ranges of about kPieceTarget products; every B entry's A
+// column is sorted by row, so a piece is one contiguous run of each column,
+// found by binary search.  One workgroup per piece loads its runs into LDS,
+// merges them pairwise (ceil(log2 c) rounds, earlier B entries first on equal
+// rows) into (row, B entry) order, and sums equal rows in that order: the
+// same chain of additions, in ascending B row order, as the sort
+// accumulator.  The pieces of a segment chain their
+// unique-row counts (a look-back on workgroup tickets, so a piece waits only
+// for pieces already running), and each writes its rows right after its
+// predecessor's.  A piece of more than kPieceCap products (rows clustered in
+// one range), a segment of more than kPieceLists B entries, or rows spanning
+// more than 2^32 per piece take the global sort instead.
+constexpr uint32_t kPieceBlock = 512;
+constexpr uint32_t kPieceCap = 3584;  // 7 per thread; 48 KB of LDS: three workgroups (24 waves) per CU
+constexpr uint32_t kPieceTarget = 2048;
+constexpr uint32_t kPieceLists = 64;
+constexpr uint32_t kPieceItems = kPieceCap / kPieceBlock;
+// Status word of a piece: [63:62] 1 = its own unique count, 2 = the count up
+// to and including it within its segment, 3 = its segment takes the global
+// sort; [61:0] the count.
+constexpr uint64_t kPieceAgg = uint64_t(1) << 62;
+constexpr uint64_t kPieceIncl = uint64_t(2) << 62;
+constexpr uint64_t kPieceBad = uint64_t(3) << 62;
+constexpr uint64_t kPieceValue = kPieceAgg - 1;
+
+// Exclusive sum over a block of W waves; `scratch` holds W words.
+template <uint32_t W>
+__device__ inline uint64_t block_exclusive_sum_waves(uint64_t v, uint64_t *scratch, uint64_t *total) {
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const uint64_t incl = wave_inclusive_sum64(v);
+  if (lane == kWave - 1) scratch[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < W; ++w) {
+    const uint64_t x = scratch[w];
+    before += w < wave ? x : 0;
+    all += x;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+// Per large segment r: rmin, piece width, and (q << 32 | q * c) for the scan
+// of piece and (piece, B entry) pair offsets.  q = 0 marks a segment for the
+// global sort.
+__global__ void piece_plan_kernel(const uint64_t *__restrict__ ak, const uint64_t *__restrict__ a_lo,
+                                  const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
+                                  const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first,
+                                  uint64_t nseg, uint64_t nb, const uint32_t *__restrict__ list3, uint32_t n3,
+                                  uint64_t *__restrict__ rmin, uint64_t *__restrict__ width,
+                                  uint64_t *__restrict__ qc, uint32_t *__restrict__ ovf,
+                                  uint32_t *__restrict__ ovf_count, uint64_t *__restrict__ seg_pos) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n3) return;
+  const uint64_t s = list3[r];
+  const uint64_t first = seg_first[s];
+  const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+  const uint64_t c = last - first, np = prod_off[last] - prod_off[first];
+  seg_pos[2 * r] = first;
+  seg_pos[2 * r + 1] = prod_off[first];
+  uint64_t q = 0, lo = ~0ull, hi = 0, w = 0;
+  if (c <= kPieceLists) {
+    for (uint64_t e = first; e < last; ++e) {
+      const uint32_t n = a_len[e];
+      if (!n) continue;
+      const uint64_t f = ak[a_lo[e]] - a_base[e], l = ak[a_lo[e] + n - 1] - a_base[e];
+      lo = f < lo ? f : lo;
+      hi = l > hi ? l : hi;
+    }
+    // np > 0, so some column is non-empty and lo <= hi.
+    const uint64_t range = hi - lo + 1;
+    q = (np + kPieceTarget - 1) / kPieceTarget;
+    w = (range + q - 1) / q;
+    if (w > 0xFFFFFFFFull) q = 0;
+    else q = (range + w - 1) / w;
+  }
+  rmin[r] = lo;
+  width[r] = w;
+  ovf[r] = q == 0;
+  if (q == 0) atomicAdd(ovf_count, 1u);
+  qc[r] = (q << 32) | (q * c);
+}
+
+// Last r in [0, n3) with part(off[r]) <= t (part: the high or low 32 bits).
+template <bool HIGH>
+__device__ inline uint32_t piece_segment(const uint64_t *__restrict__ off, uint32_t n3, uint64_t t) {
+  uint32_t lo = 0, hi = n3;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint64_t v = HIGH ? off[mid] >> 32 : off[mid] & 0xFFFFFFFFull;
+    if (v <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// bounds[pair]: first A entry of piece p in the column of B entry i (pair =
+// the segment's pair offset + p * c + i).
+__global__ void piece_bounds_kernel(const uint64_t *__restrict__ ak, const uint64_t *__restrict__ a_lo,
+                                    const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
+                                    const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb,
+                                    const uint32_t *__restrict__ list3, uint32_t n3,
+                                    const uint64_t *__restrict__ off, const uint64_t *__restrict__ rmin,
+                                    const uint64_t *__restrict__ width, uint64_t *__restrict__ bounds,
+                                    uint32_t *__restrict__ piece_seg) {
+  const uint64_t total = off[n3] & 0xFFFFFFFFull;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+    const uint32_t r = piece_segment<false>(off, n3, t);
+    const uint64_t s = list3[r];
+    const uint64_t first = seg_first[s];
+    const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+    const uint64_t c = last - first, local = t - (off[r] & 0xFFFFFFFFull);
+    const uint64_t p = local / c, e = first + (local - p * c);
+    if (e == first) piece_seg[(off[r] >> 32) + p] = r;
+    const uint64_t key = a_base[e] + rmin[r] + p * width[r];
+    uint64_t lo = a_lo[e], hi = lo + a_len[e];
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (ak[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    bounds[t] = lo;
+  }
+}
+
+__global__ __launch_bounds__(kPieceBlock) void piece_kernel(
+    const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
+    const uint64_t *__restrict__ a_lo, const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
+    const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb, const uint32_t *__restrict__ list3,
+    uint32_t n3, const uint64_t *__restrict__ off, const uint64_t *__restrict__ rmin,
+    const uint64_t *__restrict__ width, const uint64_t *__restrict__ bounds, const uint32_t *__restrict__ piece_seg,
+    const uint64_t *__restrict__ seg_pos, uint32_t *__restrict__ ticket, uint64_t *__restrict__ status,
+    uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_count, uint64_t *__restrict__ seg_u,
+    uint64_t *__restrict__ tmp_row, double *__restrict__ tmp_val) {
+  // Rows (offsets from the piece's first row) and load positions, in two
+  // buffers for the merge rounds, and the B entry of each load position.  The
+  // products are formed again from A (in cache) when the rows are summed,
+  // which keeps the LDS at three workgroups per CU.
+  __shared__ uint32_t s_row[2][kPieceCap];
+  __shared__ uint16_t s_idx[2][kPieceCap];
+  __shared__ uint8_t s_run[kPieceCap];
+  __shared__ uint32_t s_off[kPieceLists + 1];
+  __shared__ uint64_t s_src[kPieceLists];
+  __shared__ uint64_t s_key0[kPieceLists];
+  __shared__ double s_b[kPieceLists];
+  __shared__ uint64_t s_scan[kPieceBlock / kWave];
+  __shared__ uint32_t s_t;
+  __shared__ uint64_t s_prefix;
+  __shared__ uint32_t s_bad;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_t = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint64_t t = s_t;
+  // The piece's segment and its places: loads that do not wait on one
+  // another, so the start-up latency is a few round trips.
+  const uint64_t qtot = off[n3] >> 32;
+  const uint32_t r = piece_seg[t];  // piece_seg holds q_ub >= grid entries
+  if (t >= qtot) return;
+  const uint64_t s = list3[r], first = seg_pos[2 * r], seg_base = seg_pos[2 * r + 1];
+  const uint64_t o0 = off[r], o1 = off[r + 1];
+  const uint32_t c = (uint32_t)(((o1 & 0xFFFFFFFFull) - (o0 & 0xFFFFFFFFull)) / ((o1 >> 32) - (o0 >> 32)));
+  const uint64_t q = (o1 >> 32) - (o0 >> 32), p = t - (o0 >> 32);
+  const uint64_t pb = (o0 & 0xFFFFFFFFull) + p * c;
+  const uint64_t v0 = rmin[r] + p * width[r];
+  if (tid < kWave) {  // c <= kPieceLists = one wave: lane i takes B entry i
+    uint64_t len = 0;
+    if (tid < c) {
+      const uint64_t e = first + tid;
+      const uint64_t b0 = bounds[pb + tid];
+      const uint64_t b1 = p + 1 < q ? bounds[pb + c + tid] : a_lo[e] + a_len[e];
+      len = b1 - b0;
+      s_src[tid] = b0;
+      s_key0[tid] = a_base[e] + v0;
+      s_b[tid] = bv[e];
+    }
+    const uint64_t incl = wave_inclusive_sum64(len);
+    if (tid < c) s_off[tid] = (uint32_t)min<uint64_t>(incl - len, kPieceCap + 1);
+    if (tid == kWave - 1) s_off[c] = (uint32_t)min<uint64_t>(incl, kPieceCap + 1);
+  }
+  __syncthreads();
+  const uint32_t total = s_off[c];
+  if (total > kPieceCap) {  // rows clustered: this segment takes the global sort
+    if (tid == 0) {
+      if (atomicCAS(&ovf[r], 0u, 1u) == 0u) atomicAdd(ovf_count, 1u);
+      status_store(&status[t], kPieceBad);
+    }
+    return;
+  }
+  // Run (B entry) of position x: the last run starting at or before x.
+  auto run_of = [&](uint32_t x) {
+    uint32_t lo = 0, hi = c;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_off[mid] <= x) lo = mid;
+      else hi = mid;
+    }
+    return lo;
+  };
+  for (uint32_t x = tid; x < total; x += kPieceBlock) {
+    const uint32_t i = run_of(x);
+    const uint64_t ai = s_src[i] + (x - s_off[i]);
+    s_row[0][x] = (uint32_t)(ak[ai] - s_key0[i]);
+    s_idx[0][x] = (uint16_t)x;
+    s_run[x] = (uint8_t)i;
+  }
+  __syncthreads();
+  // Pairwise merge rounds: after the round of width w, every run holds 2w
+  // consecutive B entries.  Each thread writes one contiguous stretch of
+  // output positions, locating it in its pair by a merge-path search; on equal
+  // rows the left run (earlier B entries) goes first.
+  uint32_t cur = 0;
+  const uint32_t per = (total + kPieceBlock - 1) / kPieceBlock;
+  for (uint32_t w = 1; w < c; w <<= 1) {
+    const uint32_t *kin = s_row[cur];
+    const uint16_t *iin = s_idx[cur];
+    uint32_t *kout = s_row[cur ^ 1];
+    uint16_t *iout = s_idx[cur ^ 1];
+    uint32_t o = tid * per;
+    const uint32_t oe = min(o + per, total);
+    while (o < oe) {
+      const uint32_t g = run_of(o) / (2 * w) * (2 * w);
+      const uint32_t a0 = s_off[g], a1 = s_off[min(c, g + w)], b1 = s_off[min(c, g + 2 * w)];
+      const uint32_t na = a1 - a0, nbr = b1 - a1, d = o - a0;
+      uint32_t lo = d > nbr ? d - nbr : 0, hi = d < na ? d : na;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (kin[a0 + mid] <= kin[a1 + d - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+      }
+      uint32_t i = a0 + lo, j = a1 + (d - lo);
+      const uint32_t stop = min(oe, b1);
+      // The two run heads stay in registers: one LDS read per output.
+      uint32_t ka = i < a1 ? kin[i] : ~0u, kb = j < b1 ? kin[j] : ~0u;
+      for (; o < stop; ++o) {
+        if (j >= b1 || (i < a1 && ka <= kb)) {
+          kout[o] = ka;
+          iout[o] = iin[i];
+          ++i;
+          ka = i < a1 ? kin[i] : ~0u;
+        } else {
+          kout[o] = kb;
+          iout[o] = iin[j];
+          ++j;
+          kb = j < b1 ? kin[j] : ~0u;
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  const uint32_t *krow = s_row[cur];
+  const uint16_t *kidx = s_idx[cur];
+  uint32_t flags = 0, cnt = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kPieceItems; ++k) {
+    const uint32_t x = tid * kPieceItems + k;
+    const bool h = x < total && (x == 0 || krow[x] != krow[x - 1]);
+    flags |= (h ? 1u : 0u) << k;
+    cnt += h;
+  }
+  uint64_t utotal;
+  uint64_t urank = block_exclusive_sum_waves<kPieceBlock / kWave>(cnt, s_scan, &utotal);
+  uint32_t *s_head = s_row[cur ^ 1];  // free after the last merge round
+#pragma unroll
+  for (uint32_t k = 0; k < kPieceItems; ++k)
+    if ((flags >> k) & 1u) s_head[urank++] = tid * kPieceItems + k;
+  // Decoupled look-back within the segment: the piece publishes its own
+  // count at once, then wave 0 sums its predecessors' words, 64 per round
+  // trip, back to the first inclusive one (the segment's first piece
+  // publishes an inclusive count without waiting).  Only counts travel
+  // between pieces (the rows are read by a later kernel), so relaxed
+  // atomics suffice: no cache write-back or invalidate.
+  if (tid < kWave) {
+    uint64_t prefix = 0;
+    bool bad = false;
+    if (p > 0) {
+      if (tid == 0) status_store(&status[t], kPieceAgg | utotal);
+      const int64_t seg0 = (int64_t)(t - p);
+      int64_t base = (int64_t)t - 1;
+      for (;;) {
+        const int64_t idx = base - (int64_t)tid;
+        uint64_t w = 0;
+        bool ready = true, term = true;
+        if (idx >= seg0) {
+          w = status_load(&status[idx]);
+          ready = w != 0;
+          term = ready && (w & kPieceIncl);
+        }
+        const uint64_t term_mask = __ballot(term), wait_mask = __ballot(!ready);
+        const uint32_t stop = term_mask ? (uint32_t)__builtin_ctzll(term_mask) : kWave - 1;
+        const uint64_t needed = stop == kWave - 1 ? ~0ull : ((2ull << stop) - 1ull);
+        if (wait_mask & needed) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        const bool mine = tid <= stop;
+        bad = bad || __ballot(mine && (w >> 62) == 3) != 0;
+        const uint64_t part = mine ? (w & kPieceValue) : 0;
+        prefix += __shfl(wave_inclusive_sum64(part), kWave - 1, kWave);
+        if (term_mask) break;
+        base -= kWave;
+      }
+    }
+    if (tid == 0) {
+      status_store(&status[t], bad ? kPieceBad : (kPieceIncl | (prefix + utotal)));
+      s_prefix = prefix;
+      s_bad = bad;
+    }
+  }
+  __syncthreads();
+  if (s_bad) return;
+  const uint64_t dst0 = seg_base + s_prefix;
+  auto product = [&](uint32_t y) {
+    const uint32_t x = kidx[y], i = s_run[x];
+    return s_b[i] * av[s_src[i] + (x - s_off[i])];
+  };
+  for (uint32_t u = tid; u < utotal; u += kPieceBlock) {
+    // Each product rounded on its own, never fused with the addition that
+    // follows (as in every other accumulator).
+#pragma clang fp contract(off)
+    const uint32_t x = s_head[u], row = krow[x];
+    double sum = product(x);
+    for (uint32_t y = x + 1; y < total && krow[y] == row; ++y) sum += product(y);
+    tmp_row[dst0 + u] = v0 + row;
+    tmp_val[dst0 + u] = sum;
+  }
+  if (tid == 0 && p + 1 == q) seg_u[s] = s_prefix + utotal;
+}
+
+// The segments whose pieces gave up, for the global sort.
+__global__ void piece_fallback_kernel(const uint32_t *__restrict__ list3, uint32_t n3,
+                                      const uint32_t *__restrict__ ovf, uint32_t *__restrict__ out,
+                                      uint32_t *__restrict__ count) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n3 && ovf[r]) out[atomicAdd(count, 1u)] = list3[r];
+}
+
+__global__ void fill_words_kernel(uint64_t *__restrict__ p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
 // 4. output: the elements of C in order, kChunk per workgroup round.
 __global__ __launch_bounds__(256) void write_kernel(const uint64_t *__restrict__ ak, const double *__restrict__ av,
                              const double *__restrict__ bv, const uint64_t *__restrict__ a_lo,
@@ -843,7 +1189,9 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
                                                           nseg, nb, E.list2, E.seg_u, tmp_row, tmp_val);
     LNT_HIP(hipGetLastError());
   }
-  if (n3) {
+  // The global sort over the large segments in `list_esc` (expand with
+  // composite keys, one radix sort, run sums).
+  auto run_esc = [&](const uint32_t *list_esc, uint32_t n_esc) -> int {
     // Rows replaced by their ranks among A's rows when that takes the
     // composite keys below a radix-pass boundary or to 32 bits.
     uint64_t width = ah;
@@ -855,7 +1203,7 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
         while (b < 128 && ((unsigned __int128)1 << b) < v) ++b;
         return b;
       };
-      const uint32_t old_bits = bits((unsigned __int128)n3 * ah), new_bits = bits((unsigned __int128)n3 * nrank);
+      const uint32_t old_bits = bits((unsigned __int128)n_esc * ah), new_bits = bits((unsigned __int128)n_esc * nrank);
       if ((new_bits + 7) / 8 < (old_bits + 7) / 8 || (new_bits <= 32 && old_bits > 32)) {
         LNT_TRY(rank_tables(ctx, ak, na, ah, rt));
         width = nrank;
@@ -865,24 +1213,24 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     }
     // Composite keys r*width + row must stay below 2^63; otherwise the large
     // segments go through in groups that satisfy it.
-    uint32_t group = n3;
+    uint32_t group = n_esc;
     while (group > 1 && (unsigned __int128)group * width >= ((unsigned __int128)1 << 63)) group = (group + 1) / 2;
-    for (uint32_t g0 = 0; g0 < n3; g0 += group) {
-      uint32_t gn = std::min(group, n3 - g0);
-      const uint32_t *l3 = E.list3 + g0;
+    for (uint32_t g0 = 0; g0 < n_esc; g0 += group) {
+      uint32_t gn = std::min(group, n_esc - g0);
+      const uint32_t *l3 = list_esc + g0;
       large_sizes_kernel<<<(gn + 255) / 256, 256, 0, ctx->stream>>>(l3, gn, E.seg_first, nseg, nb, E.prod_off, E.np3);
       LNT_HIP(hipGetLastError());
       LNT_TRY(exclusive_scan<uint64_t>(ctx, E.np3, E.ex_off, gn, E.ex_off + gn, ctx->aux, 0));
       uint64_t total3;
       LNT_TRY(read_u64(ctx, E.ex_off + gn, &total3, 1));
-      size_t scan3 = scan_workspace_words(total3);
+      size_t scan_esc = scan_workspace_words(total3);
       LNT_TRY(ensure(ctx->spg_big, total3 * (8 + 8 + 4 + 8) + 1024));
       Carve c{(char *)ctx->spg_big.ptr};
       uint64_t *key3 = c.take<uint64_t>(total3);
       double *val3 = c.take<double>(total3);
       uint32_t *heads = c.take<uint32_t>(total3);
       uint64_t *grank = c.take<uint64_t>(total3 + 1);
-      LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan3) * 8 + 64));
+      LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan_esc) * 8 + 64));
       large_expand_kernel<<<chunk_grid(total3), 256, 0, ctx->stream>>>(
           ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first, nseg, nb, l3, gn, E.ex_off, total3, width, arank,
           key3, val3);
@@ -905,10 +1253,66 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
       large_counts_kernel<<<(gn + 255) / 256, 256, 0, ctx->stream>>>(l3, gn, E.ex_off, grank, total3, E.seg_u);
       LNT_HIP(hipGetLastError());
     }
+    return LNT_OK;
+  };
+  // The large segments: row pieces (every accumulator but ESC), with the
+  // global sort for the segments the pieces cannot take.
+  uint32_t *piece_ovf = nullptr, *piece_list = nullptr;
+  const bool pieces = n3 && acc != LNT_SPGEMM_ESC && ctx->tuning.spgemm_pieces;
+  if (n3 && !pieces) LNT_TRY(run_esc(E.list3, n3));
+  if (pieces) {
+    const uint64_t q_ub = nprod / kPieceTarget + n3 + 1, pairs_ub = q_ub * kPieceLists;
+    const size_t scan_q = scan_workspace_words(n3 + 1);
+    LNT_TRY(ensure(ctx->spg_piece, (size_t)n3 * (8 * 6 + 4 * 2) + (pairs_ub + q_ub) * 8 + q_ub * 4 + 4096));
+    Carve c{(char *)ctx->spg_piece.ptr};
+    uint64_t *rmin = c.take<uint64_t>(n3);
+    uint64_t *width = c.take<uint64_t>(n3);
+    uint64_t *qc = c.take<uint64_t>(n3);
+    uint64_t *qc_off = c.take<uint64_t>(n3 + 1);
+    piece_ovf = c.take<uint32_t>(n3);
+    piece_list = c.take<uint32_t>(n3);
+    uint64_t *bounds = c.take<uint64_t>(pairs_ub);
+    uint64_t *status = c.take<uint64_t>(q_ub);
+    uint64_t *seg_pos = c.take<uint64_t>(2 * (uint64_t)n3);
+    uint32_t *piece_seg = c.take<uint32_t>(q_ub);
+    LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan_q) * 8 + 64));
+    uint32_t *ticket = E.counts + 2, *ovf_count = E.counts + 3;  // zeroed with the classification counts
+    piece_plan_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(ak, E.a_lo, E.a_len, E.a_base, E.prod_off,
+                                                                  E.seg_first, nseg, nb, E.list3, n3, rmin, width,
+                                                                  qc, piece_ovf, ovf_count, seg_pos);
+    LNT_HIP(hipGetLastError());
+    LNT_TRY(exclusive_scan<uint64_t>(ctx, qc, qc_off, n3, qc_off + n3, ctx->aux, 0));
+    fill_words_kernel<<<grid_for(q_ub, 4096), 256, 0, ctx->stream>>>(status, q_ub);
+    piece_bounds_kernel<<<grid_for(pairs_ub, 8192), 256, 0, ctx->stream>>>(ak, E.a_lo, E.a_len, E.a_base,
+                                                                           E.seg_first, nseg, nb, E.list3, n3,
+                                                                           qc_off, rmin, width, bounds, piece_seg);
+    piece_kernel<<<(uint32_t)q_ub, kPieceBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_len, E.a_base, E.seg_first,
+                                                            nseg, nb, E.list3, n3, qc_off, rmin, width, bounds,
+                                                            piece_seg, seg_pos, ticket, status, piece_ovf,
+                                                            ovf_count, E.seg_u, tmp_row, tmp_val);
+    LNT_HIP(hipGetLastError());
   }
   LNT_TRY(exclusive_scan<uint64_t>(ctx, E.seg_u, E.out_off, nseg, E.out_off + nseg, ctx->aux, 0));
   uint64_t nc;
-  LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
+  if (pieces) {
+    // nnz(C) and the count of segments the pieces gave up, in one wait.
+    LNT_HIP(hipMemcpyAsync(ctx->pinned + 8, E.out_off + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(ctx->pinned + 10, E.counts + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(&nc, ctx->pinned + 8, sizeof(nc));
+    const uint32_t n_back = ctx->pinned[10];
+    if (n_back) {
+      LNT_TRY(launch_fill_u32(ctx, E.counts + 3, 0, 1));
+      piece_fallback_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(E.list3, n3, piece_ovf, piece_list,
+                                                                       E.counts + 3);
+      LNT_HIP(hipGetLastError());
+      LNT_TRY(run_esc(piece_list, n_back));
+      LNT_TRY(exclusive_scan<uint64_t>(ctx, E.seg_u, E.out_off, nseg, E.out_off + nseg, ctx->aux, 0));
+      LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
+    }
+  } else {
+    LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
+  }
   if (ck_alloc) {
     void *k = nullptr, *v = nullptr;
     size_t bytes = (nc ? nc : 1) * 8;

@@ -142,6 +142,7 @@ def test_lattice_mult_row_ranks(ctx, orc, shape):
     ak, av = _random_lattice(orc, ah, aw, d, n_a, 21)
     bk, bv = _random_lattice(orc, aw, bw, d, n_b, 22)
     ek, ev = orc.lattice_mult(ak, av, (ah, aw, d), bk, bv, (aw, bw, d))
+    ctx.set_param("spgemm_pieces", 0)  # every long column through the global sort
     try:
         for strategy in (1, 2):
             outs = []
@@ -155,6 +156,62 @@ def test_lattice_mult_row_ranks(ctx, orc, shape):
             assert orc.allclose_fuzzy(host(outs[0][1]), ev, TOL)
     finally:
         ctx.set_param("row_ranks", 1)
+        ctx.set_param("spgemm_pieces", 1)
+
+
+def _piece_case(kind, rng):
+    """(ah, aw, bw, ak, av, bk, bv) of one tab, built to steer the long
+    output columns through a given branch of the row-piece path."""
+    ah, aw, bw = 1 << 20, 96, 8
+    rows = []
+    for col in range(aw):
+        if kind == "uniform":  # pieces, every column in one piece range
+            r = rng.choice(ah, 3000, replace=False)
+        elif kind == "gap":  # rows in two bands: pieces over the gap are empty
+            r = np.concatenate([rng.choice(30000, 9000, replace=False),
+                                40000 + rng.choice(30000, 9000, replace=False)])
+        elif kind == "cluster":  # one far row: the first piece overflows -> global sort
+            r = np.concatenate([rng.choice(2000, 1500, replace=False), [ah - 1]])
+        else:  # "equal": every column the same rows -> runs of equal rows across all B entries
+            r = np.arange(0, ah, ah // 2500)
+        rows.append(np.unique(r) + ah * col)
+    ak = np.concatenate(rows).astype(np.int64)
+    av = rng.uniform(-1, 1, len(ak))
+    ks, cs = [], []
+    for c, cnt in enumerate([2, 5, 17, 64, 65, 96, 1, 40]):  # B entries per output column
+        ks += list(np.sort(rng.choice(aw, cnt, replace=False)))
+        cs += [c] * cnt
+    bk = (np.array(ks) + aw * np.array(cs)).astype(np.int64)
+    order = np.argsort(bk)
+    bk = bk[order]
+    bv = rng.uniform(-1, 1, len(bk))
+    return ah, aw, bw, ak, av, bk, bv
+
+
+@pytest.mark.parametrize("kind", ["uniform", "gap", "cluster", "equal"])
+def test_lattice_mult_row_pieces(ctx, orc, kind):
+    """Long output columns in row pieces, with 2..64 B entries per column
+    (pieces), 65 and 96 (global sort), uniform rows, two bands with empty
+    pieces between, a cluster that overflows a piece (global sort), and
+    identical rows in every column (runs across all B entries): keys exact
+    and value bits equal to the oracle's sparse accumulator and to the
+    global sort's, for every accumulator that uses the pieces."""
+    rng = np.random.default_rng(["uniform", "gap", "cluster", "equal"].index(kind) + 40)
+    ah, aw, bw, ak, av, bk, bv = _piece_case(kind, rng)
+    print("row pieces", kind, len(ak), len(bk), flush=True)
+    sk, sv = orc.lattice_mult_by("spa", ak, av, (ah, aw, 1), bk, bv, (aw, bw, 1))
+    try:
+        for strategy in (1, 2, 17):
+            for pieces in (1, 0):
+                ctx.set_param("spgemm_pieces", pieces)
+                ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1), strategy)
+                assert np.array_equal(host(ck), sk), (strategy, pieces)
+                if strategy != 2:
+                    assert np.array_equal(host(cv).view(np.int64), sv.view(np.int64)), (strategy, pieces)
+                else:
+                    assert orc.allclose_fuzzy(host(cv), sv, TOL)
+    finally:
+        ctx.set_param("spgemm_pieces", 1)
 
 
 def test_lattice_mult_empty_and_errors(ctx, orc):
