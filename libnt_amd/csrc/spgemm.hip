@@ -820,20 +820,19 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
   }
   const uint32_t *krow = s_row[cur];
   const uint16_t *kidx = s_idx[cur];
+  // Run heads over `per` consecutive positions per thread (per <= kPieceItems).
   uint32_t flags = 0, cnt = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < kPieceItems; ++k) {
-    const uint32_t x = tid * kPieceItems + k;
-    const bool h = x < total && (x == 0 || krow[x] != krow[x - 1]);
-    flags |= (h ? 1u : 0u) << k;
+  const uint32_t h0 = tid * per, h1 = min(h0 + per, total);
+  for (uint32_t x = h0; x < h1; ++x) {
+    const bool h = x == 0 || krow[x] != krow[x - 1];
+    flags |= (h ? 1u : 0u) << (x - h0);
     cnt += h;
   }
   uint64_t utotal;
   uint64_t urank = block_exclusive_sum_waves<kPieceBlock / kWave>(cnt, s_scan, &utotal);
   uint32_t *s_head = s_row[cur ^ 1];  // free after the last merge round
-#pragma unroll
-  for (uint32_t k = 0; k < kPieceItems; ++k)
-    if ((flags >> k) & 1u) s_head[urank++] = tid * kPieceItems + k;
+  for (uint32_t x = h0; x < h1; ++x)
+    if ((flags >> (x - h0)) & 1u) s_head[urank++] = x;
   // Decoupled look-back within the segment: the piece publishes its own
   // count at once, then wave 0 sums its predecessors' words, 64 per round
   // trip, back to the first inclusive one (the segment's first piece
@@ -884,15 +883,33 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
     const uint32_t x = kidx[y], i = s_run[x];
     return s_b[i] * av[s_src[i] + (x - s_off[i])];
   };
-  for (uint32_t u = tid; u < utotal; u += kPieceBlock) {
+  {
     // Each product rounded on its own, never fused with the addition that
-    // follows (as in every other accumulator).
+    // follows (as in every other accumulator).  The first product of every
+    // row is fetched for all of a thread's rows before any is summed, so the
+    // fetches overlap.
 #pragma clang fp contract(off)
-    const uint32_t x = s_head[u], row = krow[x];
-    double sum = product(x);
-    for (uint32_t y = x + 1; y < total && krow[y] == row; ++y) sum += product(y);
-    tmp_row[dst0 + u] = v0 + row;
-    tmp_val[dst0 + u] = sum;
+    uint32_t hx[kPieceItems];
+    double sum[kPieceItems];
+    const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
+#pragma unroll
+    for (uint32_t k = 0; k < kPieceItems; ++k) {
+      const uint32_t u = tid + k * kPieceBlock;
+      if (k < nu && u < utotal) {
+        hx[k] = s_head[u];
+        sum[k] = product(hx[k]);
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPieceItems; ++k) {
+      const uint32_t u = tid + k * kPieceBlock;
+      if (k < nu && u < utotal) {
+        const uint32_t row = krow[hx[k]];
+        for (uint32_t y = hx[k] + 1; y < total && krow[y] == row; ++y) sum[k] += product(y);
+        tmp_row[dst0 + u] = v0 + row;
+        tmp_val[dst0 + u] = sum[k];
+      }
+    }
   }
   if (tid == 0 && p + 1 == q) seg_u[s] = s_prefix + utotal;
 }
@@ -923,24 +940,37 @@ __global__ __launch_bounds__(256) void write_kernel(const uint64_t *__restrict__
   for (uint64_t o0 = (uint64_t)blockIdx.x * kChunk; o0 < nc; o0 += (uint64_t)gridDim.x * kChunk) {
     const uint64_t o1 = min<uint64_t>(o0 + kChunk, nc);
     const ChunkSegs c = stage_chunk(out_off, nseg, o0, o1, s_off, s_seg);
-    for (uint64_t o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
-      uint64_t base;
-      const uint64_t s = chunk_segment(c, out_off, s_off, o, &base);
-      const uint64_t q = o - base;
-      const uint64_t first = seg_first[s];
-      const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
-      uint64_t row;
-      double val;
-      if (last - first == 1) {
-        const uint64_t ai = a_lo[first] + q;
-        row = ak[ai] - a_base[first];
-        val = bv[first] * av[ai];
-      } else {
-        row = tmp_row[prod_off[first] + q];
-        val = tmp_val[prod_off[first] + q];
+    // kPer elements per thread, all loads issued before any store, so that
+    // their chains of dependent loads overlap.
+    constexpr uint32_t kPer = kChunk / 256;
+    uint64_t key[kPer];
+    double val[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint64_t o = o0 + threadIdx.x + k * 256;
+      if (o < o1) {
+        uint64_t base;
+        const uint64_t s = chunk_segment(c, out_off, s_off, o, &base);
+        const uint64_t q = o - base;
+        const uint64_t first = seg_first[s];
+        const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+        if (last - first == 1) {
+          const uint64_t ai = a_lo[first] + q;
+          key[k] = ak[ai] - a_base[first] + ah * colid[first];
+          val[k] = bv[first] * av[ai];
+        } else {
+          key[k] = tmp_row[prod_off[first] + q] + ah * colid[first];
+          val[k] = tmp_val[prod_off[first] + q];
+        }
       }
-      ck[o] = row + ah * colid[first];
-      cv[o] = val;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+      const uint64_t o = o0 + threadIdx.x + k * 256;
+      if (o < o1) {
+        ck[o] = key[k];
+        cv[o] = val[k];
+      }
     }
     __syncthreads();  // s_off is restaged next round
   }
