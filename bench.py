@@ -276,7 +276,7 @@ def bench_small_contraction(ctx, torch, reps=20, cpu_repeats=5):
     return res
 
 
-def bench_contraction(ctx, torch, cpu_repeats=3):
+def bench_contraction(ctx, torch, cpu_repeats=3, mask=None):
     """Large skewed contraction, every accumulator, timed whole; roofline over
     the compulsory bytes; the oracle on one core on the same arrays."""
     import oracle
@@ -288,7 +288,7 @@ def bench_contraction(ctx, torch, cpu_repeats=3):
     ka, va = A.keys.cpu().numpy(), A.vals.cpu().numpy()
     kb, vb = B.keys.cpu().numpy(), B.vals.cpu().numpy()
     nprod = _nprod_of(ka, kb, 4096 * 4096, 512)  # A's k = key // (4096*4096), B's k = key % 512
-    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, (1 << dispatch.NUM_STRATEGIES) - 1, 3)
+    ms = dispatch.time_strategies(ctx, A, sp.la, B, sp.lb, mask or (1 << dispatch.NUM_STRATEGIES) - 1, 3)
     timed = [s for s in range(len(ms)) if ms[s] > 0]
     best = min(timed, key=lambda s: ms[s])
     nnz_c = _contract_once(ctx, A, sp.la, B, sp.lb, best)
@@ -318,7 +318,13 @@ def bench_contraction(ctx, torch, cpu_repeats=3):
     return res
 
 
-def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True):
+# The batch configuration the autotuner chose on MI355X for the batched mix
+# (used instead of tuning with --no-autotune, a profiling aid).
+BATCH_CONFIG = {"digit_bits": 8, "sort_block": 1024, "sort_items": 12, "small_max": 1024, "item_products": 1024,
+                "window": 1536, "blocks_per_cu": 4}
+
+
+def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True, tune=True):
     """The batched mix: autotuned recorded graph vs one call per instance."""
     import ctypes
 
@@ -330,8 +336,11 @@ def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True):
 
     insts = workloads.batched_contractions(10000, seed=4)
     ops = workloads.generate_operands(ctx, insts)
-    tuned = autotune.tune(ctx, insts, ops, stride=10, passes=2, reps=2)
-    _log("autotune: %s" % tuned)
+    if tune:
+        tuned = autotune.tune(ctx, insts, ops, stride=10, passes=2, reps=2)
+        _log("autotune: %s" % tuned)
+    else:
+        tuned = {"config": dict(BATCH_CONFIG), "ms": None, "default_ms": None}
     b = Batch(ctx, insts)
     autotune.apply_batch(ctx, b, tuned["config"])
     nprod = b.nprod(ops[0], ops[2])
@@ -357,8 +366,9 @@ def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True):
         "instances_per_s": round(len(insts) / t_batched, 1),
         "roofline": _roofline(compulsory, t_batched, "whole batched pipeline (graph replay), compulsory bytes "
                                                      "16*sum(nnzA+nnzB+nnzC)"),
-        "autotuned": tuned["config"], "autotune_subsample_ms": {"default": round(tuned["default_ms"], 2),
-                                                                 "tuned": round(tuned["ms"], 2)},
+        "autotuned": tuned["config"],
+        "autotune_subsample_ms": ({"default": round(tuned["default_ms"], 2), "tuned": round(tuned["ms"], 2)}
+                                  if tune else None),
     }
     if not per_instance_runs:
         _log("batched: %s" % res)
@@ -625,6 +635,10 @@ def main():
                     help="run one section only (a profiling aid: the line then holds that section alone)")
     ap.add_argument("--graph-only", action="store_true",
                     help="batched section: the recorded graph only, no per-instance loops (a profiling aid)")
+    ap.add_argument("--strategies", type=lambda x: int(x, 0), default=None,
+                    help="contraction section: time only the strategies of this bit mask (a profiling aid)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched section: the stored MI355X configuration instead of tuning (a profiling aid)")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, run a plain 1.6 GB write (fill) and read (sum) so that "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE can be scaled by kernels of known traffic")
@@ -653,10 +667,11 @@ def main():
     if "small_contraction" in want:
         out["small_contraction"] = bench_small_contraction(ctx, torch)
     if "contraction" in want:
-        out["contraction"] = bench_contraction(ctx, torch, cpu_repeats=0 if args.no_cpu_baseline else 3)
+        out["contraction"] = bench_contraction(ctx, torch, cpu_repeats=0 if args.no_cpu_baseline else 3,
+                                               mask=args.strategies)
     if "batched" in want:
         out["batched"] = bench_batched(ctx, torch, cpu_stride=0 if args.no_cpu_baseline else 100,
-                                       per_instance_runs=not args.graph_only)
+                                       per_instance_runs=not args.graph_only, tune=not args.no_autotune)
     if "learned" in want:
         out["learned"] = bench_learned(ctx)
     print(json.dumps(out), flush=True)
