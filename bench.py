@@ -454,6 +454,36 @@ def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True, tune=True)
     return res
 
 
+def bench_cpp_front_end(steps=10, warmup=2, host_steps=2):
+    """The sort/merge step written against the C++ front end
+    (tests/cpp/front_end_step.cpp), with the tensors resident in HBM and with
+    the reference's host vectors; compiled with g++ here, run as a child
+    process (its own context on GPU 0)."""
+    import subprocess
+    import tempfile
+
+    root = os.path.dirname(os.path.abspath(__file__))
+    src = os.path.join(root, "tests", "cpp", "front_end_step.cpp")
+    libdir = os.path.join(root, "libnt_amd")
+    res = {"workload": "the sort/merge-at-scale step through include/lnt/SparseMIA.hpp: A.sort(perm), B.sort(perm), "
+                       "C(i,j,k,l)=A(i,j,k,l)+B(i,j,k,l), D(i,j,k,l)=A(!i,!j,!k,!l)*B(!i,!j,!k,!l)"}
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "front_end_step")
+        r = subprocess.run(["g++", "-std=c++17", "-O2", src, "-I", os.path.join(root, "include"), "-L", libdir,
+                            "-llnt", "-Wl,-rpath," + libdir, "-o", exe], capture_output=True, text=True)
+        if r.returncode != 0:
+            return {"error": "g++ failed: " + r.stderr[-400:]}
+        perm = [str(x) for x in _perm()]
+        for storage, n, w in (("device", steps, warmup), ("host", host_steps, 1)):
+            out = subprocess.run([exe, storage, str(n), str(w)] + perm, capture_output=True, text=True, timeout=600)
+            if out.returncode != 0:
+                res[storage] = {"error": (out.stdout + out.stderr)[-400:]}
+                continue
+            res[storage] = json.loads(out.stdout.strip().splitlines()[-1])
+    _log("C++ front end: %s" % res)
+    return res
+
+
 def bench_learned(ctx):
     """Held-out regret of the shipped model (committed MI355X timings)."""
     from libnt_amd import dispatch
@@ -517,7 +547,7 @@ def bench_host_buffers(ctx, torch, ak, av, bk, bv, reps=4):
     return res
 
 
-SECTIONS = ("headline", "unsorted_sort", "small_contraction", "contraction", "batched", "learned")
+SECTIONS = ("headline", "unsorted_sort", "small_contraction", "contraction", "batched", "learned", "cpp_front_end")
 
 
 def bench_headline(ctx, torch, args):
@@ -674,6 +704,8 @@ def main():
                                        per_instance_runs=not args.graph_only, tune=not args.no_autotune)
     if "learned" in want:
         out["learned"] = bench_learned(ctx)
+    if "cpp_front_end" in want:
+        out["cpp_front_end"] = bench_cpp_front_end()
     print(json.dumps(out), flush=True)
 
 

@@ -221,11 +221,19 @@ typedef enum lnt_compare_mode { LNT_CMP_EXACT = 0, LNT_CMP_FUZZY = 1 } lnt_compa
 int lnt_compare(lnt_context ctx, const int64_t *a_keys, const double *a_vals, int64_t na, const int64_t *b_keys,
                 const double *b_vals, int64_t nb, int mode, double precision, int *equal, int mem);
 int lnt_free(lnt_context ctx, void *ptr, int mem);
-/* Copies `bytes` bytes from src to dst (device memory, either side may be an
- * array the library allocated) on the context's stream and waits: how a
- * caller moves a library-allocated result into its own buffers before
- * lnt_free.  No counterpart in the reference (mex outputs are mxArrays). */
+/* Copies `bytes` bytes from src to dst on the context's stream and waits.
+ * Either side may be device memory (the library's or the caller's) or host
+ * memory: how a caller moves a library-allocated result into its own
+ * buffers before lnt_free, and how the C++ front end moves a tensor's
+ * entries to and from HBM.  No counterpart in the reference (mex outputs are
+ * mxArrays). */
 int lnt_copy(lnt_context ctx, void *dst, const void *src, int64_t bytes);
+/* *ptr receives `bytes` (>= 1) bytes of device memory (LNT_MEM_DEVICE) or
+ * host memory (LNT_MEM_HOST) on the context's device, released with
+ * lnt_free: storage for tensors kept resident in HBM between calls (the C++
+ * front end's to_device).  No counterpart in the reference (its storage is
+ * std::vector, SparseMIA.h:61-65). */
+int lnt_alloc(lnt_context ctx, int64_t bytes, int mem, void **ptr);
 
 /* ------------------------------------------------------- lattice product -- */
 /* The accumulator of a lattice product, optionally or-ed with

@@ -13,6 +13,13 @@
 // on the GPU through lnt.h (host arrays are staged to HBM and back); only
 // double values are supported.  Errors become exceptions, as in the reference
 // (LibMIAException.h:25-92).
+//
+// A tensor can instead keep its entries in HBM (A.to_device(), or
+// A.assign_device() with library-allocated arrays): every operation on it
+// then runs on the device arrays and its results stay in HBM too, so a chain
+// of expressions moves no data over PCIe; A.to_host() brings the entries
+// back.  An operation that meets one operand in HBM and one on the host moves
+// the host one to HBM first.
 #pragma once
 
 #include <algorithm>
@@ -96,33 +103,133 @@ inline int64_t product(const std::vector<int64_t> &v) {
   return p;
 }
 
-// Type-erased tensor used by expressions (any order).
+// Device memory the library allocated (lnt_alloc, or a result it handed
+// over), released with lnt_free when the last owner goes.  The free needs no
+// context, so tensors that outlive their thread's context stay safe.
+struct DeviceArray {
+  void *p = nullptr;
+  explicit DeviceArray(void *q) : p(q) {}
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  ~DeviceArray() {
+    if (p) lnt_free(nullptr, p, LNT_MEM_DEVICE);
+  }
+  static std::shared_ptr<DeviceArray> make(int64_t bytes) {
+    void *q = nullptr;
+    check(lnt_alloc(default_context().get(), bytes > 0 ? bytes : 1, LNT_MEM_DEVICE, &q));
+    return std::make_shared<DeviceArray>(q);
+  }
+  // A device copy of `bytes` bytes at src (host or device memory).
+  static std::shared_ptr<DeviceArray> copy_of(const void *src, int64_t bytes) {
+    auto d = make(bytes);
+    if (bytes > 0) check(lnt_copy(default_context().get(), d->p, src, bytes));
+    return d;
+  }
+};
+
+// Type-erased tensor used by expressions (any order).  Its entries live in
+// idx/data (host) or, when `device`, in dk/dv (dn entries in HBM).
 struct Tensor {
   std::vector<int64_t> dims;
   std::vector<int32_t> seq;
   std::vector<int64_t> idx;
   std::vector<double> data;
   bool sorted = false;
+  bool device = false;
+  std::shared_ptr<DeviceArray> dk, dv;
+  int64_t dn = 0;
+
+  Tensor() = default;
+  Tensor(Tensor &&) = default;
+  Tensor &operator=(Tensor &&) = default;
+  // A copy owns copies of the entries, in the same memory.
+  Tensor(const Tensor &o)
+      : dims(o.dims), seq(o.seq), idx(o.idx), data(o.data), sorted(o.sorted), device(o.device), dn(o.dn) {
+    if (device) {
+      dk = DeviceArray::copy_of(o.dk->p, dn * 8);
+      dv = DeviceArray::copy_of(o.dv->p, dn * 8);
+    }
+  }
+  Tensor &operator=(const Tensor &o) {
+    if (this != &o) *this = Tensor(o);
+    return *this;
+  }
+
+  int mem() const { return device ? LNT_MEM_DEVICE : LNT_MEM_HOST; }
+  int64_t nnz() const { return device ? dn : (int64_t)idx.size(); }
+  int64_t *keys() { return device ? (int64_t *)dk->p : idx.data(); }
+  double *vals() { return device ? (double *)dv->p : data.data(); }
+
+  // Takes over the entries of a library-allocated result in memory kind m.
+  void adopt(int64_t *ck, double *cv, int64_t nc, int m) {
+    if (m == LNT_MEM_DEVICE) {
+      std::vector<int64_t>().swap(idx);
+      std::vector<double>().swap(data);
+      dk = std::make_shared<DeviceArray>(ck);
+      dv = std::make_shared<DeviceArray>(cv);
+      dn = nc;
+      device = true;
+      return;
+    }
+    idx.assign(ck, ck + nc);
+    data.assign(cv, cv + nc);
+    lnt_free(default_context().get(), ck, LNT_MEM_HOST);
+    lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+    dk.reset();
+    dv.reset();
+    dn = 0;
+    device = false;
+  }
+  // Moves the entries to HBM (no-op when already there).
+  void to_device() {
+    if (device) return;
+    const int64_t n = (int64_t)idx.size();
+    dk = DeviceArray::copy_of(idx.data(), n * 8);
+    dv = DeviceArray::copy_of(data.data(), n * 8);
+    dn = n;
+    device = true;
+    std::vector<int64_t>().swap(idx);
+    std::vector<double>().swap(data);
+  }
+  // Moves the entries back to host vectors.
+  void to_host() {
+    if (!device) return;
+    idx.resize((size_t)dn);
+    data.resize((size_t)dn);
+    if (dn) {
+      check(lnt_copy(default_context().get(), idx.data(), dk->p, dn * 8));
+      check(lnt_copy(default_context().get(), data.data(), dv->p, dn * 8));
+    }
+    dk.reset();
+    dv.reset();
+    dn = 0;
+    device = false;
+  }
 
   // change_linIdx_sequence (SparseMIABase.h:768-814): relinearise, unsorted.
   void relinearise(const std::vector<int32_t> &new_seq) {
     if (new_seq == seq) return;
-    if (!idx.empty())
-      check(lnt_change_linidx(default_context().get(), idx.data(), (int64_t)idx.size(), (int)dims.size(), dims.data(),
-                              seq.data(), new_seq.data(), LNT_MEM_HOST));
+    if (nnz())
+      check(lnt_change_linidx(default_context().get(), keys(), nnz(), (int)dims.size(), dims.data(), seq.data(),
+                              new_seq.data(), mem()));
     seq = new_seq;
     sorted = false;
   }
 
   void sort_to(const std::vector<int32_t> &new_seq) {
-    if (!idx.empty())
-      check(lnt_change_sort_order(default_context().get(), idx.data(), data.data(), (int64_t)idx.size(),
-                                  (int)dims.size(), dims.data(), seq.data(), new_seq.data(), sorted ? 1 : 0,
-                                  LNT_MEM_HOST));
+    if (nnz())
+      check(lnt_change_sort_order(default_context().get(), keys(), vals(), nnz(), (int)dims.size(), dims.data(),
+                                  seq.data(), new_seq.data(), sorted ? 1 : 0, mem()));
     seq = new_seq;
     sorted = true;
   }
 };
+
+// Operands of one call in one memory: a host operand meeting one in HBM moves
+// there.
+inline void same_memory(Tensor &a, Tensor &b) {
+  if (a.device != b.device) (a.device ? b : a).to_device();
+}
 
 struct Expr {
   std::shared_ptr<Tensor> t;  // owning result or a view of a SparseMIA's storage
@@ -155,6 +262,7 @@ inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const s
     bp = &b_copy;
   }
   Tensor &b = *bp;
+  same_memory(a, b);
   const size_t n = la.size();
   std::vector<int32_t> index_order(n), lhs_order(n);
   for (size_t m = 0; m < n; ++m) {
@@ -169,7 +277,7 @@ inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const s
   for (size_t t = 0; t < n; ++t) b_in_a[t] = index_order[a.seq[t]];
   for (size_t t = 0; t < n; ++t) a_in_b[t] = lhs_order[b.seq[t]];
   if (a.sorted && b.sorted) {
-    if (b.idx.size() < a.idx.size()) b.sort_to(b_in_a);
+    if (b.nnz() < a.nnz()) b.sort_to(b_in_a);
     else a.sort_to(a_in_b);
   } else if (b.sorted) {
     a.sort_to(a_in_b);
@@ -182,26 +290,33 @@ inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const s
     auto c = std::make_shared<Tensor>();
     c->dims = a.dims;
     c->sorted = true;
-    std::vector<int64_t> a_idx = a.idx;
-    if (a.idx.size() > b.idx.size()) {
+    // A's keys relinearised into a copy (A itself is left as it is).
+    std::vector<int64_t> a_idx;
+    std::shared_ptr<DeviceArray> a_dk;
+    int64_t *ak = a.keys();
+    if (a.nnz() > b.nnz()) {
       b.relinearise(b_in_a);
       c->seq = a.seq;
     } else {
-      if (!a_idx.empty() && a.seq != a_in_b)
-        check(lnt_change_linidx(default_context().get(), a_idx.data(), (int64_t)a_idx.size(), (int)n, a.dims.data(),
-                                a.seq.data(), a_in_b.data(), LNT_MEM_HOST));
+      if (a.nnz() && a.seq != a_in_b) {
+        if (a.device) {
+          a_dk = DeviceArray::copy_of(a.keys(), a.nnz() * 8);
+          ak = (int64_t *)a_dk->p;
+        } else {
+          a_idx = a.idx;
+          ak = a_idx.data();
+        }
+        check(lnt_change_linidx(default_context().get(), ak, a.nnz(), (int)n, a.dims.data(), a.seq.data(),
+                                a_in_b.data(), a.mem()));
+      }
       c->seq = a_in_b;
     }
     int64_t *ck = nullptr;
     double *cv = nullptr;
     int64_t nc = 0;
-    check(lnt_sort_merge(default_context().get(), a_idx.data(), a.data.data(), (int64_t)a_idx.size(), b.idx.data(),
-                         b.data.data(), (int64_t)b.idx.size(), op, (uint64_t)product(a.dims), &ck, &cv, &nc,
-                         LNT_MEM_HOST));
-    c->idx.assign(ck, ck + nc);
-    c->data.assign(cv, cv + nc);
-    lnt_free(default_context().get(), ck, LNT_MEM_HOST);
-    lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+    check(lnt_sort_merge(default_context().get(), ak, a.vals(), a.nnz(), b.keys(), b.vals(), b.nnz(), op,
+                         (uint64_t)product(a.dims), &ck, &cv, &nc, a.mem()));
+    c->adopt(ck, cv, nc, a.mem());
     return {c, la};
   }
   auto c = std::make_shared<Tensor>();
@@ -211,12 +326,9 @@ inline Expr merge(Tensor &a, const std::vector<Label> &la, Tensor &b_in, const s
   int64_t *ck = nullptr;
   double *cv = nullptr;
   int64_t nc = 0;
-  check(lnt_merge(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), b.idx.data(),
-                  b.data.data(), (int64_t)b.idx.size(), op, &ck, &cv, &nc, LNT_MEM_HOST));
-  c->idx.assign(ck, ck + nc);
-  c->data.assign(cv, cv + nc);
-  lnt_free(default_context().get(), ck, LNT_MEM_HOST);
-  lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+  check(lnt_merge(default_context().get(), a.keys(), a.vals(), a.nnz(), b.keys(), b.vals(), b.nnz(), op, &ck, &cv,
+                  &nc, a.mem()));
+  c->adopt(ck, cv, nc, a.mem());
   return {c, la};
 }
 
@@ -263,13 +375,9 @@ inline Expr unary(const std::shared_ptr<Tensor> &ap, const std::vector<Label> &l
   int64_t *ck = nullptr;
   double *cv = nullptr;
   int64_t nc = 0;
-  check(lnt_contract_attract(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), (int)n,
-                             a.dims.data(), a.seq.data(), group.data(), kind.data(), (int)kind.size(), &ck, &cv, &nc,
-                             LNT_MEM_HOST));
-  c->idx.assign(ck, ck + nc);
-  c->data.assign(cv, cv + nc);
-  lnt_free(default_context().get(), ck, LNT_MEM_HOST);
-  lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+  check(lnt_contract_attract(default_context().get(), a.keys(), a.vals(), a.nnz(), (int)n, a.dims.data(),
+                             a.seq.data(), group.data(), kind.data(), (int)kind.size(), &ck, &cv, &nc, a.mem()));
+  c->adopt(ck, cv, nc, a.mem());
   return {c, lc};
 }
 
@@ -282,6 +390,7 @@ inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, cons
     bp = &b_copy;
   }
   Tensor &b = *bp;
+  same_memory(a, b);
   std::vector<int32_t> a_out, a_in, a_int, b_in_m, b_int, b_out;
   for (size_t m = 0; m < la.size(); ++m) {
     int r = find_label(lb, la[m].id);
@@ -327,22 +436,32 @@ inline Expr multiply(Tensor &a, const std::vector<Label> &la, Tensor &b_in, cons
   // reference's complete inner product (sparse_mia_mult_test.cpp:159-165).
   c->seq = identity(c->dims.size());
   c->sorted = true;
-  a.sort_to(seq_a);
-  b.sort_to(seq_b);
   int64_t *ck = nullptr;
   double *cv = nullptr;
   int64_t nc = 0;
-  if (a_in.empty() && a_out.empty() && b_out.empty())
-    check(lnt_merge(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), b.idx.data(),
-                    b.data.data(), (int64_t)b.idx.size(), LNT_OP_HADAMARD, &ck, &cv, &nc, LNT_MEM_HOST));
-  else
-    check(lnt_lattice_mult(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), lat_a,
-                           b.idx.data(), b.data.data(), (int64_t)b.idx.size(), lat_b, LNT_SPGEMM_AUTO, &ck, &cv,
-                           &nc, LNT_MEM_HOST));
-  c->idx.assign(ck, ck + nc);
-  c->data.assign(cv, cv + nc);
-  lnt_free(default_context().get(), ck, LNT_MEM_HOST);
-  lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+  if (a_in.empty() && a_out.empty() && b_out.empty()) {
+    // Pure Hadamard: one sorted merge in any common order.  Every mode is an
+    // inter index (a_int = A's modes in order, C's modes = A's), so a sorted
+    // A keeps its order, B follows it and C comes out in it.
+    if (a.sorted) {
+      for (size_t t = 0; t < seq_a.size(); ++t) {
+        seq_a[t] = a.seq[t];
+        seq_b[t] = b_int[a.seq[t]];
+        c->seq[t] = a.seq[t];
+      }
+    }
+    a.sort_to(seq_a);
+    b.sort_to(seq_b);
+    check(lnt_merge(default_context().get(), a.keys(), a.vals(), a.nnz(), b.keys(), b.vals(), b.nnz(),
+                    LNT_OP_HADAMARD, &ck, &cv, &nc, a.mem()));
+    c->adopt(ck, cv, nc, a.mem());
+    return {c, lc, a_int.size()};
+  }
+  a.sort_to(seq_a);
+  b.sort_to(seq_b);
+  check(lnt_lattice_mult(default_context().get(), a.keys(), a.vals(), a.nnz(), lat_a, b.keys(), b.vals(), b.nnz(),
+                         lat_b, LNT_SPGEMM_AUTO, &ck, &cv, &nc, a.mem()));
+  c->adopt(ck, cv, nc, a.mem());
   a.sorted = false;  // lattice unfolding leaves operands flagged unsorted
   b.sorted = false;
   return {c, lc, a_int.size()};
@@ -398,13 +517,16 @@ inline Expr multiply_dense(Tensor &a, const std::vector<Label> &la, const DenseV
   int64_t *ck = nullptr;
   double *cv = nullptr;
   int64_t nc = 0;
-  check(lnt_sparse_dense_mult(default_context().get(), a.idx.data(), a.data.data(), (int64_t)a.idx.size(), lsize,
-                              (int)b.dims->size(), b.dims->data(), b.data->data(), b_out.data(), nr, b_int.data(), ni,
-                              &ck, &cv, &nc, LNT_MEM_HOST));
-  c->idx.assign(ck, ck + nc);
-  c->data.assign(cv, cv + nc);
-  lnt_free(default_context().get(), ck, LNT_MEM_HOST);
-  lnt_free(default_context().get(), cv, LNT_MEM_HOST);
+  // A dense operand meeting a sparse one in HBM goes there for the call.
+  std::shared_ptr<DeviceArray> bd;
+  const double *bdata = b.data->data();
+  if (a.device) {
+    bd = DeviceArray::copy_of(bdata, (int64_t)b.data->size() * 8);
+    bdata = (const double *)bd->p;
+  }
+  check(lnt_sparse_dense_mult(default_context().get(), a.keys(), a.vals(), a.nnz(), lsize, (int)b.dims->size(),
+                              b.dims->data(), bdata, b_out.data(), nr, b_int.data(), ni, &ck, &cv, &nc, a.mem()));
+  c->adopt(ck, cv, nc, a.mem());
   return {c, lc, (size_t)ni};
 }
 
@@ -438,7 +560,12 @@ class Atom {
   operator double() const {
     if (!e_.labels.empty())
       throw MIAParameterException(LNT_ERR_INVALID_ARGUMENT, "only a complete inner product is a scalar");
-    return e_.t->data.empty() ? 0.0 : e_.t->data[0];
+    detail::Tensor &t = *e_.t;
+    if (t.nnz() == 0) return 0.0;
+    if (!t.device) return t.data[0];
+    double v = 0.0;
+    check(lnt_copy(default_context().get(), &v, t.vals(), sizeof(double)));
+    return v;
   }
   const detail::Expr &expr() const { return e_; }
 
@@ -479,7 +606,8 @@ class AssignTarget {
       inv[p] = (int32_t)m;
     }
     detail::Tensor &t = *m_.tensor_ptr();
-    detail::Tensor src = *e.t;
+    // A temporary result is moved, a tensor still referenced elsewhere copied.
+    detail::Tensor src = e.t.use_count() == 1 ? std::move(*e.t) : detail::Tensor(*e.t);
     std::vector<int64_t> dims(n);
     for (size_t m = 0; m < n; ++m) dims[m] = src.dims[perm[m]];
     if (dims != t.dims) throw DimensionMismatchException(LNT_ERR_DIMENSION_MISMATCH, "assignment dims differ");
@@ -487,6 +615,10 @@ class AssignTarget {
     for (size_t q = 0; q < n; ++q) seq[q] = inv[src.seq[q]];
     t.idx = std::move(src.idx);
     t.data = std::move(src.data);
+    t.device = src.device;
+    t.dk = std::move(src.dk);
+    t.dv = std::move(src.dv);
+    t.dn = src.dn;
     t.seq = seq;
     t.sorted = src.sorted;
     return *this;
@@ -531,7 +663,7 @@ class SparseMIA {
   }
 
   // ---- storage (SparseMIA.h:61-65, 203-206)
-  size_t size() const { return t_->idx.size(); }
+  size_t size() const { return (size_t)t_->nnz(); }
   std::array<int64_t, N> dims() const {
     std::array<int64_t, N> d;
     std::copy(t_->dims.begin(), t_->dims.end(), d.begin());
@@ -547,24 +679,47 @@ class SparseMIA {
   bool is_sorted() const { return t_->sorted; }
   void setSorted(bool s) { t_->sorted = s; }
   void push_back(T value, int64_t index) {
+    host_only();
     t_->data.push_back(value);
     t_->idx.push_back(index);
     t_->sorted = false;
   }
   void reserve(size_t n) {
+    host_only();
     t_->idx.reserve(n);
     t_->data.reserve(n);
   }
   void resize(size_t n) {
+    host_only();
     t_->idx.resize(n);
     t_->data.resize(n);
   }
-  int64_t *index_begin() { return t_->idx.data(); }
-  int64_t *index_end() { return t_->idx.data() + t_->idx.size(); }
-  T *data_begin() { return t_->data.data(); }
-  T *data_end() { return t_->data.data() + t_->data.size(); }
-  const std::vector<int64_t> &indices() const { return t_->idx; }
-  const std::vector<T> &data() const { return t_->data; }
+  int64_t *index_begin() { return host_only(), t_->idx.data(); }
+  int64_t *index_end() { return host_only(), t_->idx.data() + t_->idx.size(); }
+  T *data_begin() { return host_only(), t_->data.data(); }
+  T *data_end() { return host_only(), t_->data.data() + t_->data.size(); }
+  const std::vector<int64_t> &indices() const { return host_only(), t_->idx; }
+  const std::vector<T> &data() const { return host_only(), t_->data; }
+
+  // ---- residence in HBM (no reference counterpart: its storage is host
+  // memory).  While on the device the entries are reached only through
+  // operations; the host accessors above throw.
+  void to_device() { t_->to_device(); }
+  void to_host() { t_->to_host(); }
+  bool on_device() const { return t_->device; }
+  // Device pointers to the n entries in HBM, valid while the tensor stays
+  // on the device and unchanged.
+  const int64_t *device_indices() const { return t_->device ? (const int64_t *)t_->dk->p : nullptr; }
+  const T *device_data() const { return t_->device ? (const T *)t_->dv->p : nullptr; }
+  // Takes over n entries already in HBM: keys and vals allocated with
+  // lnt_alloc (LNT_MEM_DEVICE), in the mode order seq, sorted or not; the
+  // tensor frees them.
+  template <class S>
+  void assign_device(int64_t *keys, double *vals, int64_t n, const std::array<S, N> &seq, bool sorted) {
+    t_->adopt(keys, vals, n, LNT_MEM_DEVICE);
+    t_->seq.assign(seq.begin(), seq.end());
+    t_->sorted = sorted;
+  }
 
   // ---- linear-index math (IndexUtil.h:318-333, 866-879)
   int64_t sub2ind(const std::array<int64_t, N> &sub) const {
@@ -595,11 +750,7 @@ class SparseMIA {
   void change_linIdx_sequence(const std::array<S, N> &seq) {
     std::vector<int32_t> s(seq.begin(), seq.end());
     if (s == t_->seq || N == 1) return;
-    if (!t_->idx.empty())
-      check(lnt_change_linidx(default_context().get(), t_->idx.data(), (int64_t)t_->idx.size(), (int)N,
-                              t_->dims.data(), t_->seq.data(), s.data(), LNT_MEM_HOST));
-    t_->seq = s;
-    t_->sorted = false;
+    t_->relinearise(s);
   }
 
   // ---- comparison (SparseMIABase.h:1243-1296, 1616-1635): dims must agree;
@@ -622,13 +773,17 @@ class SparseMIA {
  private:
   bool compare(SparseMIA &o, int mode, double precision) {
     if (t_->dims != o.t_->dims) return false;
+    detail::same_memory(*t_, *o.t_);
     sort();
     o.t_->sort_to(t_->seq);
     int eq = 0;
-    check(lnt_compare(default_context().get(), t_->idx.data(), t_->data.data(), (int64_t)t_->idx.size(),
-                      o.t_->idx.data(), o.t_->data.data(), (int64_t)o.t_->idx.size(), mode, precision, &eq,
-                      LNT_MEM_HOST));
+    check(lnt_compare(default_context().get(), t_->keys(), t_->vals(), t_->nnz(), o.t_->keys(), o.t_->vals(),
+                      o.t_->nnz(), mode, precision, &eq, t_->mem()));
     return eq != 0;
+  }
+  void host_only() const {
+    if (t_->device)
+      throw MIAException(LNT_ERR_INVALID_ARGUMENT, "the entries are in HBM: call to_host() to reach them");
   }
 
   std::shared_ptr<detail::Tensor> t_;

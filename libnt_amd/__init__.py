@@ -66,6 +66,7 @@ SIGNATURES = {
     "lnt_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _int],
     "lnt_free": [_vp, _vp, _int],
     "lnt_copy": [_vp, _vp, _vp, _i64],
+    "lnt_alloc": [_vp, _i64, _int, ctypes.POINTER(_vp)],
     "lnt_sort_merge_into": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _u64, _vp, _vp, _i64, _i64p, _int],
     "lnt_sort_merge": [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _int, _u64, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                        _i64p, _int],

@@ -819,8 +819,26 @@ int lnt_copy(lnt_context ctx, void *dst, const void *src, int64_t bytes) {
   LNT_CTX(ctx);
   if (bytes < 0 || (bytes > 0 && (!dst || !src))) return fail(LNT_ERR_INVALID_ARGUMENT, "bad copy arguments");
   if (bytes == 0) return LNT_OK;
-  LNT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  LNT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDefault, ctx->stream));
   LNT_HIP(hipStreamSynchronize(ctx->stream));
+  return LNT_OK;
+}
+
+int lnt_alloc(lnt_context ctx, int64_t bytes, int mem, void **ptr) {
+  LNT_CTX(ctx);
+  LNT_TRY(check_mem(mem));
+  if (!ptr || bytes < 1) return fail(LNT_ERR_INVALID_ARGUMENT, "bytes must be >= 1 and ptr non-null");
+  *ptr = nullptr;
+  if (mem == LNT_MEM_HOST) {
+    *ptr = malloc((size_t)bytes);
+    if (!*ptr) return fail(LNT_ERR_OUT_OF_MEMORY, "malloc failed");
+    return LNT_OK;
+  }
+  if (hipMalloc(ptr, (size_t)bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(LNT_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
   return LNT_OK;
 }
 

@@ -416,6 +416,91 @@ int main() {
     }
     CHECK(thrown, "a contraction with a dense operand not reported");
   }
+  // ---- tensors resident in HBM: the same expressions give the same bits as
+  //      on host storage, their results stay in HBM, and a chain of them
+  //      leaves every tensor on the device
+  {
+    auto same = [](auto &h, auto &d) {  // d is brought back to the host
+      if (!d.on_device()) return false;
+      d.to_host();
+      return h.indices() == d.indices() && h.data() == d.data() && h.linIdxSequence() == d.linIdxSequence() &&
+             h.is_sorted() == d.is_sorted();
+    };
+    lnt::SparseMIA<double, 4> a(6, 5, 4, 3), b(4, 3, 6, 5), ad(6, 5, 4, 3), bd(4, 3, 6, 5);
+    fill_random(a, 150, 51);
+    fill_random(b, 200, 52);
+    ad = a;
+    bd = b;
+    ad.to_device();
+    bd.to_device();
+    CHECK(ad.on_device() && ad.size() == a.size(), "to_device");
+    bool thrown = false;
+    try {
+      (void)ad.indices();
+    } catch (const lnt::MIAException &) {
+      thrown = true;
+    }
+    CHECK(thrown, "host accessor of a device tensor did not throw");
+    lnt::SparseMIA<double, 4> c(6, 5, 4, 3), cd(6, 5, 4, 3), e(6, 5, 4, 3), ed(6, 5, 4, 3);
+    c(i, j, k, l) = a(i, j, k, l) + b(k, l, i, j);  // neither sorted: sortMerge
+    cd(i, j, k, l) = ad(i, j, k, l) + bd(k, l, i, j);
+    CHECK(cd.on_device() && ad.on_device() && bd.on_device(), "device operands or result left HBM");
+    e(i, j, k, l) = c(i, j, k, l) - a(i, j, k, l);  // c sorted: scan merge
+    ed(i, j, k, l) = cd(i, j, k, l) - ad(i, j, k, l);
+    CHECK(same(e, ed), "device merge differs");
+    CHECK(same(c, cd), "device sortMerge differs");
+    std::array<size_t, 4> seq{{2, 0, 3, 1}};
+    lnt::SparseMIA<double, 4> s = a, sd = a;
+    sd.to_device();
+    s.sort(seq);
+    sd.sort(seq);
+    CHECK(same(s, sd), "device sort(seq) differs");
+    lnt::SparseMIA<double, 4> m1(6, 5, 4, 3), m1d(6, 5, 4, 3);
+    m1(i, j, k, l) = a(!i, !j, !k, !l) * b(!k, !l, !i, !j);  // Hadamard
+    m1d(i, j, k, l) = ad(!i, !j, !k, !l) * bd(!k, !l, !i, !j);
+    CHECK(same(m1, m1d), "device Hadamard differs");
+    lnt::SparseMIA<double, 3> x(6, 5, 4), y(4, 7, 3), xd(6, 5, 4), yd(4, 7, 3);
+    lnt::SparseMIA<double, 4> z(6, 5, 7, 3), zd(6, 5, 7, 3);
+    fill_random(x, 80, 53);
+    fill_random(y, 60, 54);
+    xd = x;
+    yd = y;
+    xd.to_device();
+    z(i, j, m, n) = x(i, j, k) * y(k, m, n);  // contraction
+    zd(i, j, m, n) = xd(i, j, k) * yd(k, m, n);  // y still on the host: moved to HBM
+    CHECK(yd.on_device(), "host operand of a device product not moved to HBM");
+    CHECK(same(z, zd), "device contraction differs");
+    lnt::SparseMIA<double, 3> g(5, 6, 5), gd(5, 6, 5);
+    fill_random(g, 90, 56);
+    gd = g;
+    gd.to_device();
+    lnt::SparseMIA<double, 1> tr(6), trd(6);
+    tr(j) = g(i, j, i);  // unary contraction
+    trd(j) = gd(i, j, i);
+    CHECK(same(tr, trd), "device unary contraction differs");
+    lnt::DenseMIA<double, 2> wdense(4, 2);
+    for (int64_t q = 0; q < 8; ++q) wdense.data_begin()[q] = 0.25 * (double)(q + 1);
+    lnt::SparseMIA<double, 4> r(6, 5, 2, 4), rd(6, 5, 2, 4);
+    r(i, j, m, k) = x(i, j, !k) * wdense(!k, m);  // sparse x dense
+    rd(i, j, m, k) = xd(i, j, !k) * wdense(!k, m);
+    CHECK(same(r, rd), "device sparse x dense differs");
+    lnt::SparseMIA<double, 3> w(6, 5, 4), wd(6, 5, 4);
+    fill_random(w, 60, 55);
+    lnt::SparseMIA<double, 3> w0 = w;
+    wd = w;
+    wd.to_device();
+    w(i, j, k) += x(i, j, k);
+    wd(i, j, k) += xd(i, j, k);
+    CHECK(same(w, wd), "device += differs");
+    lnt::SparseMIA<double, 3> q1 = w0, q1d = w0;
+    q1d.to_device();
+    CHECK(q1 == w0 && q1d == w0, "device comparison");
+    double h = x(i, j, k) * x(i, j, k), hd = xd(i, j, k) * xd(i, j, k);
+    CHECK(h == hd, "device complete inner product differs");
+    lnt::SparseMIA<double, 3> back = xd;
+    back.to_host();
+    CHECK(back.indices() == x.indices() && back.data() == x.data(), "to_device/to_host round trip");
+  }
   if (failures) return 1;
   std::printf("C++ front end: all checks passed\n");
   return 0;

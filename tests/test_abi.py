@@ -37,6 +37,8 @@ def test_abi_version_and_null_context_errors():
     assert b"null context" in L.lnt_last_error()
     assert L.lnt_context_destroy(None) == 0
     assert L.lnt_free(None, None, 0) == 0
+    p = ctypes.c_void_p()
+    assert L.lnt_alloc(None, 8, 1, ctypes.byref(p)) == 1 and not p.value
 
 
 def test_header_compiles_as_c_and_cxx(tmp_path):
