@@ -7,12 +7,16 @@ Python package and nothing in the library imports this file:
     python libnt_amd/data/make_dispatch_data.py --part train --start 7000 --count 7000 --out DIR
     python libnt_amd/data/make_dispatch_data.py --part train --start 14000 --count 6000 --out DIR
     python libnt_amd/data/make_dispatch_data.py --part test --start 0 --count 2000 --out DIR
+    python libnt_amd/data/make_dispatch_data.py --part trainbatch --start 0 --count 10000 --out DIR
+    python libnt_amd/data/make_dispatch_data.py --part testbatch --start 0 --count 2000 --out DIR
     python libnt_amd/data/make_dispatch_data.py --part fit --out DIR
 
 (chunks keep each run short).  Writes DIR/dispatch_timings.npz (features, log2-ms timings of all 20
 strategies, the restated polyalgorithm's choice) and DIR/dispatch_model.npy
 (the model trained on the training split by the library's GPU kernels).
-Instances come from ``dispatch.mixed_specs`` (train seed 1, held-out seed 2).
+Instances come from ``dispatch.mixed_specs`` (train seed 1, held-out seed 2)
+and ``dispatch.batch_specs`` (the batched contraction mix with unsorted
+operands: train seed 5, held-out seed 6; the benchmark's own mix is seed 4).
 """
 import argparse
 import os
@@ -30,7 +34,7 @@ EPOCHS, LR = 300, 1e-3
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--part", required=True, choices=["train", "test", "fit"])
+    ap.add_argument("--part", required=True, choices=["train", "test", "trainbatch", "testbatch", "fit"])
     ap.add_argument("--start", type=int, default=0)
     ap.add_argument("--count", type=int, default=0)
     ap.add_argument("--max-nnz", type=int, default=200000)
@@ -52,12 +56,17 @@ def main(argv=None):
         print("[%6.1fs] %s" % (time.time() - t0, msg), flush=True)
 
     os.makedirs(args.out, exist_ok=True)
-    seeds = {"train": 1, "test": 2}
+    seeds = {"train": 1, "test": 2, "trainbatch": 5, "testbatch": 6}
     if args.part in seeds:
         seed = seeds[args.part]
-        specs = dispatch.mixed_specs(args.start + args.count, seed, args.max_nnz)[args.start:]
+        if args.part.endswith("batch"):
+            specs = dispatch.batch_specs(args.start + args.count, seed)[args.start:]
+            name = "dispatch_%s_b%06d.npz" % (args.part[: -len("batch")], args.start)
+        else:
+            specs = dispatch.mixed_specs(args.start + args.count, seed, args.max_nnz)[args.start:]
+            name = "dispatch_%s_%06d.npz" % (args.part, args.start)
         X, T, H = dispatch.collect_timings(ctx, specs, seed * 1000003 + args.start, args.reps, log)
-        path = os.path.join(args.out, "dispatch_%s_%06d.npz" % (args.part, args.start))
+        path = os.path.join(args.out, name)
         np.savez_compressed(path, X=X, T=T, H=H)
         log("%s instances %d..%d -> %s" % (args.part, args.start, args.start + args.count, path))
         return 0

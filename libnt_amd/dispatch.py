@@ -288,6 +288,16 @@ def mixed_specs(count: int, seed: int, max_nnz: int = 200000) -> List[Spec]:
     return specs
 
 
+def batch_specs(count: int, seed: int) -> List[Spec]:
+    """Instances of the batched contraction mix (workloads.batched_contractions:
+    nnz log-U[100, 50k], order 3-4, 1-2 contracted modes, dims log-U[8,128]),
+    both operands unsorted, as the per-instance calls over that mix see them."""
+    from .workloads import batched_contractions
+
+    return [Spec("batch", x.dims_a, x.labels_a, x.dims_b, x.labels_b, x.nnz_a, x.nnz_b, "unsorted", "unsorted")
+            for x in batched_contractions(count, seed=seed)]
+
+
 def _zipf_index(rng, n, size, s=1.1):
     """Zipf(s) over [0, n) by inverse CDF."""
     w = 1.0 / np.arange(1, n + 1) ** s
