@@ -454,6 +454,44 @@ def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True, tune=True)
     return res
 
 
+def bench_permutations(ctx, torch, reps=3):
+    """change_sort_order of the sorted 50M-key operand A to each of the 24
+    mode permutations of its order-4 dims: radix passes and time per sort, so
+    the headline's one seeded permutation can be read against the spread."""
+    import itertools
+
+    ak, av, bk, bv = make_c2(ctx, torch)
+    del bk, bv
+    rows = []
+    for perm in itertools.permutations(range(4)):
+        perm = list(perm)
+        t = []
+        passes = 0
+        for r in range(reps + 1):
+            k, v = ak.clone(), av.clone()
+            ctx.set_profiling(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.change_sort_order(k, v, DIMS, IDENT, perm, True)
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+            passes = ctx.profile("onesweep")[0]
+            ctx.set_profiling(False)
+            del k, v
+        rows.append({"perm": perm, "passes": passes, "ms": round(1e3 * float(np.median(t[1:])), 4)})
+    ms = np.array([r["ms"] for r in rows])
+    ps = np.array([r["passes"] for r in rows])
+    res = {"workload": "change_sort_order of 50M sorted keys (256^4) to every permutation of 4 modes",
+           "per_permutation": rows, "passes_mean": round(float(ps.mean()), 3),
+           "ms_min": float(ms.min()), "ms_median": float(np.median(ms)), "ms_mean": round(float(ms.mean()), 4),
+           "ms_max": float(ms.max()), "headline_perm": _perm(),
+           "headline_perm_ms": next(r["ms"] for r in rows if r["perm"] == _perm())}
+    del ak, av
+    torch.cuda.empty_cache()
+    _log("permutations: %s" % {k: v for k, v in res.items() if k != "per_permutation"})
+    return res
+
+
 def bench_cpp_front_end(steps=10, warmup=2, host_steps=2):
     """The sort/merge step written against the C++ front end
     (tests/cpp/front_end_step.cpp), with the tensors resident in HBM and with
@@ -547,7 +585,8 @@ def bench_host_buffers(ctx, torch, ak, av, bk, bv, reps=4):
     return res
 
 
-SECTIONS = ("headline", "unsorted_sort", "small_contraction", "contraction", "batched", "learned", "cpp_front_end")
+SECTIONS = ("headline", "unsorted_sort", "permutations", "small_contraction", "contraction", "batched", "learned",
+            "cpp_front_end")
 
 
 def bench_headline(ctx, torch, args):
@@ -694,6 +733,8 @@ def main():
     if "unsorted_sort" in want:
         out["unsorted_sort"] = bench_unsorted_sort(ctx, torch)
         _log("unsorted sort: %s" % (out["unsorted_sort"],))
+    if "permutations" in want:
+        out["permutations"] = bench_permutations(ctx, torch)
     if "small_contraction" in want:
         out["small_contraction"] = bench_small_contraction(ctx, torch)
     if "contraction" in want:

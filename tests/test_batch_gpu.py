@@ -4,7 +4,8 @@ values within 1e-12 under isEqualFuzzy, FunctionUtil.h:688-698) and against
 the single-contraction GPU path (bit-identical: both sum equal rows in
 ascending B-row order); the recorded hipGraph against the eager run; and the
 full-size mix through size-independent properties (per-instance sortedness,
-nnz bounds, the linearity checksum sum(C) = sum_k colsum_A(k) * rowsum_B(k))."""
+nnz bounds, the linearity checksum sum(C) = sum_k colsum_A(k) * rowsum_B(k),
+and a sample of its instances against the oracle entry by entry)."""
 import math
 
 import numpy as np
@@ -339,11 +340,12 @@ def _linearity_checksums(insts, ak, av, bk, bv, a_off, b_off):
     return want, bound
 
 
-def test_c4_full_size_batch(ctx):
+def test_c4_full_size_batch(ctx, orc):
     """The full batched mix: 10,000 instances, order 3-4, 1-2 contracted
     modes, dims log-U[8,128], nnz log-U[100,50k].  Checked by the recorded
-    graph against the eager run, per-instance sortedness and bounds, and the
-    linearity checksum of every instance."""
+    graph against the eager run, per-instance sortedness and bounds, the
+    linearity checksum of every instance, and every 100th instance entry by
+    entry against the oracle's full contraction."""
     import torch
 
     from libnt_amd import workloads as W
@@ -378,6 +380,16 @@ def test_c4_full_size_batch(ctx):
     last = ck[torch.from_numpy(off[nz + 1] - 1).cuda()].cpu().numpy()
     dimc = np.array([math.prod(insts[i].dims_c) for i in nz], dtype=object)
     assert (first >= 0).all() and all(int(l) < int(d) for l, d in zip(last, dimc))
+    # every 100th instance of the real mix against the oracle, entry by entry
+    for i in range(0, len(insts), 100):
+        x = insts[i]
+        hak, hav, hbk, hbv = _oracle_operands(orc, insts, i)
+        ek, ev, _ = orc.contract(hak, hav, x.dims_a, list(range(len(x.dims_a))), False, x.labels_a,
+                                 hbk, hbv, x.dims_b, list(range(len(x.dims_b))), False, x.labels_b)
+        got_k = ck[int(off[i]):int(off[i + 1])].cpu().numpy()
+        got_v = cv[int(off[i]):int(off[i + 1])].cpu().numpy()
+        assert np.array_equal(got_k, ek), "instance %d keys" % i
+        assert orc.allclose_fuzzy(got_v, ev, TOL), "instance %d values" % i
     # per-instance sums against the linearity checksum
     fp_eager = _fingerprints(ck, cv, off)
     got = fp_eager[2]
