@@ -21,6 +21,10 @@
 #include "lnt_internal.h"
 
 namespace lnt {
+int model_predict_one_host(lnt_model m, const float *features, float *y);
+}
+
+namespace lnt {
 
 namespace {
 
@@ -401,7 +405,7 @@ static int choose(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, i
   if (!ctx->model) return heuristic_of(a, b, strategy);
   float f[LNT_NUM_FEATURES], y[LNT_NUM_STRATEGIES];
   LNT_TRY(features_of(a, b, f));
-  LNT_TRY(lnt_model_predict((lnt_model)ctx->model, f, 1, y));
+  LNT_TRY(model_predict_one_host((lnt_model)ctx->model, f, y));
   int best = 0;
   for (int s = 1; s < LNT_NUM_STRATEGIES; ++s)
     if (y[s] < y[best]) best = s;

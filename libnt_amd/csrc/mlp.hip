@@ -293,6 +293,36 @@ double full_loss(lnt_model m, const float *Xd, const float *Td, int64_t n, float
 
 using namespace lnt;
 
+namespace lnt {
+// One sample's prediction on the host, from the parameters the model keeps
+// equal to its device copy: the same fused multiply-adds in the same order
+// as mlp_forward_kernel, so the same floats.  A per-call dispatch decision
+// then needs no device round trip (a launch and a synchronisation cost more
+// than the strategy choice saves on small contractions).
+int model_predict_one_host(lnt_model m, const float *features, float *y) {
+  if (!m || m->host.size() != (size_t)NP) return fail(LNT_ERR_INVALID_ARGUMENT, "model has no parameters");
+  const float *w = m->host.data();
+  float x[F], h1[H], h2[H];
+  for (int k = 0; k < F; ++k) x[k] = (features[k] - w[OFF_MEAN + k]) * w[OFF_ISTD + k];
+  for (int j = 0; j < H; ++j) {
+    float z = w[OFF_B1 + j];
+    for (int k = 0; k < F; ++k) z = fmaf(w[OFF_W1 + j * F + k], x[k], z);
+    h1[j] = z > 0.f ? z : 0.f;
+  }
+  for (int j = 0; j < H; ++j) {
+    float z = w[OFF_B2 + j];
+    for (int i = 0; i < H; ++i) z = fmaf(w[OFF_W2 + j * H + i], h1[i], z);
+    h2[j] = z > 0.f ? z : 0.f;
+  }
+  for (int o = 0; o < S; ++o) {
+    float v = w[OFF_B3 + o];
+    for (int j = 0; j < H; ++j) v = fmaf(w[OFF_W3 + o * H + j], h2[j], v);
+    y[o] = v;
+  }
+  return LNT_OK;
+}
+}  // namespace lnt
+
 extern "C" {
 
 int64_t lnt_model_param_count(void) { return NP; }
@@ -357,6 +387,7 @@ int lnt_model_set_params(lnt_model m, const float *params) {
   LNT_HIP(hipStreamSynchronize(m->ctx->stream));
   return LNT_OK;
 }
+
 
 int lnt_model_predict(lnt_model m, const float *features, int64_t n, float *out) {
   if (!m || (n > 0 && (!features || !out)) || n < 0) return fail(LNT_ERR_INVALID_ARGUMENT, "bad arguments");
