@@ -575,19 +575,18 @@ __global__ void large_counts_kernel(const uint32_t *__restrict__ list3, uint32_t
 // unique-row counts (a look-back on workgroup tickets, so a piece waits only
 // for pieces already running), and each writes its rows right after its
 // predecessor's.  A piece of more than kPieceCap products (rows clustered in
-// one range), a segment of more than kPieceLists B entries, or rows spanning
-// more than 2^32 per piece take the global sort instead.
+// one range) goes through in row sub-ranges found by halving; a segment of
+// more than kPieceLists B entries, or rows spanning more than 2^32 per piece,
+// takes the global sort instead.
 constexpr uint32_t kPieceBlock = 512;
 constexpr uint32_t kPieceCap = 3584;  // 7 per thread; 48 KB of LDS: three workgroups (24 waves) per CU
-constexpr uint32_t kPieceTarget = 2048;
+constexpr uint32_t kPieceTarget = 3072;
 constexpr uint32_t kPieceLists = 64;
 constexpr uint32_t kPieceItems = kPieceCap / kPieceBlock;
 // Status word of a piece: [63:62] 1 = its own unique count, 2 = the count up
-// to and including it within its segment, 3 = its segment takes the global
-// sort; [61:0] the count.
+// to and including it within its segment; [61:0] the count.
 constexpr uint64_t kPieceAgg = uint64_t(1) << 62;
 constexpr uint64_t kPieceIncl = uint64_t(2) << 62;
-constexpr uint64_t kPieceBad = uint64_t(3) << 62;
 constexpr uint64_t kPieceValue = kPieceAgg - 1;
 
 // Exclusive sum over a block of W waves; `scratch` holds W words.
@@ -696,12 +695,11 @@ __global__ void piece_bounds_kernel(const uint64_t *__restrict__ ak, const uint6
 __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
     const uint64_t *__restrict__ a_lo, const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
-    const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb, const uint32_t *__restrict__ list3,
-    uint32_t n3, const uint64_t *__restrict__ off, const uint64_t *__restrict__ rmin,
-    const uint64_t *__restrict__ width, const uint64_t *__restrict__ bounds, const uint32_t *__restrict__ piece_seg,
-    const uint64_t *__restrict__ seg_pos, uint32_t *__restrict__ ticket, uint64_t *__restrict__ status,
-    uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_count, uint64_t *__restrict__ seg_u,
-    uint64_t *__restrict__ tmp_row, double *__restrict__ tmp_val) {
+    const uint32_t *__restrict__ list3, uint32_t n3, const uint64_t *__restrict__ off,
+    const uint64_t *__restrict__ rmin, const uint64_t *__restrict__ width, const uint64_t *__restrict__ bounds,
+    const uint32_t *__restrict__ piece_seg, const uint64_t *__restrict__ seg_pos, uint32_t *__restrict__ ticket,
+    uint64_t *__restrict__ status, uint64_t *__restrict__ seg_u, uint64_t *__restrict__ tmp_row,
+    double *__restrict__ tmp_val) {
   // Rows (offsets from the piece's first row) and load positions, in two
   // buffers for the merge rounds, and the B entry of each load position.  The
   // products are formed again from A (in cache) when the rows are summed,
@@ -710,12 +708,14 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
   __shared__ uint16_t s_idx[2][kPieceCap];
   __shared__ uint8_t s_run[kPieceCap];
   __shared__ uint32_t s_off[kPieceLists + 1];
-  __shared__ uint64_t s_src[kPieceLists];
+  __shared__ uint64_t s_src[kPieceLists];   // first A entry of each run of the current range
+  __shared__ uint64_t s_next[kPieceLists];  // first A entry past the current range
+  __shared__ uint64_t s_end[kPieceLists];   // first A entry past the piece
   __shared__ uint64_t s_key0[kPieceLists];
   __shared__ double s_b[kPieceLists];
   __shared__ uint64_t s_scan[kPieceBlock / kWave];
   __shared__ uint32_t s_t;
-  __shared__ uint64_t s_prefix;
+  __shared__ uint64_t s_prefix, s_lo;
   __shared__ uint32_t s_bad;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_t = atomicAdd(ticket, 1u);
@@ -731,7 +731,8 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
   const uint32_t c = (uint32_t)(((o1 & 0xFFFFFFFFull) - (o0 & 0xFFFFFFFFull)) / ((o1 >> 32) - (o0 >> 32)));
   const uint64_t q = (o1 >> 32) - (o0 >> 32), p = t - (o0 >> 32);
   const uint64_t pb = (o0 & 0xFFFFFFFFull) + p * c;
-  const uint64_t v0 = rmin[r] + p * width[r];
+  const uint64_t w = width[r], v0 = rmin[r] + p * w;
+  uint64_t total_all;
   if (tid < kWave) {  // c <= kPieceLists = one wave: lane i takes B entry i
     uint64_t len = 0;
     if (tid < c) {
@@ -740,22 +741,21 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
       const uint64_t b1 = p + 1 < q ? bounds[pb + c + tid] : a_lo[e] + a_len[e];
       len = b1 - b0;
       s_src[tid] = b0;
+      s_next[tid] = b1;
+      s_end[tid] = b1;
       s_key0[tid] = a_base[e] + v0;
       s_b[tid] = bv[e];
     }
     const uint64_t incl = wave_inclusive_sum64(len);
     if (tid < c) s_off[tid] = (uint32_t)min<uint64_t>(incl - len, kPieceCap + 1);
-    if (tid == kWave - 1) s_off[c] = (uint32_t)min<uint64_t>(incl, kPieceCap + 1);
+    if (tid == kWave - 1) {
+      s_off[c] = (uint32_t)min<uint64_t>(incl, kPieceCap + 1);
+      s_scan[0] = incl;
+    }
   }
   __syncthreads();
-  const uint32_t total = s_off[c];
-  if (total > kPieceCap) {  // rows clustered: this segment takes the global sort
-    if (tid == 0) {
-      if (atomicCAS(&ovf[r], 0u, 1u) == 0u) atomicAdd(ovf_count, 1u);
-      status_store(&status[t], kPieceBad);
-    }
-    return;
-  }
+  total_all = s_scan[0];
+  __syncthreads();
   // Run (B entry) of position x: the last run starting at or before x.
   auto run_of = [&](uint32_t x) {
     uint32_t lo = 0, hi = c;
@@ -766,129 +766,95 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
     }
     return lo;
   };
-  for (uint32_t x = tid; x < total; x += kPieceBlock) {
-    const uint32_t i = run_of(x);
-    const uint64_t ai = s_src[i] + (x - s_off[i]);
-    s_row[0][x] = (uint32_t)(ak[ai] - s_key0[i]);
-    s_idx[0][x] = (uint16_t)x;
-    s_run[x] = (uint8_t)i;
-  }
-  __syncthreads();
-  // Pairwise merge rounds: after the round of width w, every run holds 2w
-  // consecutive B entries.  Each thread writes one contiguous stretch of
-  // output positions, locating it in its pair by a merge-path search; on equal
-  // rows the left run (earlier B entries) goes first.
   uint32_t cur = 0;
-  const uint32_t per = (total + kPieceBlock - 1) / kPieceBlock;
-  for (uint32_t w = 1; w < c; w <<= 1) {
-    const uint32_t *kin = s_row[cur];
-    const uint16_t *iin = s_idx[cur];
-    uint32_t *kout = s_row[cur ^ 1];
-    uint16_t *iout = s_idx[cur ^ 1];
-    uint32_t o = tid * per;
-    const uint32_t oe = min(o + per, total);
-    while (o < oe) {
-      const uint32_t g = run_of(o) / (2 * w) * (2 * w);
-      const uint32_t a0 = s_off[g], a1 = s_off[min(c, g + w)], b1 = s_off[min(c, g + 2 * w)];
-      const uint32_t na = a1 - a0, nbr = b1 - a1, d = o - a0;
-      uint32_t lo = d > nbr ? d - nbr : 0, hi = d < na ? d : na;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (kin[a0 + mid] <= kin[a1 + d - 1 - mid]) lo = mid + 1;
-        else hi = mid;
-      }
-      uint32_t i = a0 + lo, j = a1 + (d - lo);
-      const uint32_t stop = min(oe, b1);
-      // The two run heads stay in registers: one LDS read per output.
-      uint32_t ka = i < a1 ? kin[i] : ~0u, kb = j < b1 ? kin[j] : ~0u;
-      for (; o < stop; ++o) {
-        if (j >= b1 || (i < a1 && ka <= kb)) {
-          kout[o] = ka;
-          iout[o] = iin[i];
-          ++i;
-          ka = i < a1 ? kin[i] : ~0u;
-        } else {
-          kout[o] = kb;
-          iout[o] = iin[j];
-          ++j;
-          kb = j < b1 ? kin[j] : ~0u;
-        }
-      }
+  // Loads the runs s_src[i] + [0, s_off[i+1] - s_off[i]) (at most kPieceCap
+  // products), orders them by (row, B entry), marks run heads (s_head) and
+  // returns the number of distinct rows; `cur` names the ordered buffer.
+  auto order_rows = [&]() -> uint64_t {
+    const uint32_t total = s_off[c];
+    for (uint32_t x = tid; x < total; x += kPieceBlock) {
+      const uint32_t i = run_of(x);
+      const uint64_t ai = s_src[i] + (x - s_off[i]);
+      s_row[0][x] = (uint32_t)(ak[ai] - s_key0[i]);
+      s_idx[0][x] = (uint16_t)x;
+      s_run[x] = (uint8_t)i;
     }
     __syncthreads();
-    cur ^= 1;
-  }
-  const uint32_t *krow = s_row[cur];
-  const uint16_t *kidx = s_idx[cur];
-  // Run heads over `per` consecutive positions per thread (per <= kPieceItems).
-  uint32_t flags = 0, cnt = 0;
-  const uint32_t h0 = tid * per, h1 = min(h0 + per, total);
-  for (uint32_t x = h0; x < h1; ++x) {
-    const bool h = x == 0 || krow[x] != krow[x - 1];
-    flags |= (h ? 1u : 0u) << (x - h0);
-    cnt += h;
-  }
-  uint64_t utotal;
-  uint64_t urank = block_exclusive_sum_waves<kPieceBlock / kWave>(cnt, s_scan, &utotal);
-  uint32_t *s_head = s_row[cur ^ 1];  // free after the last merge round
-  for (uint32_t x = h0; x < h1; ++x)
-    if ((flags >> (x - h0)) & 1u) s_head[urank++] = x;
-  // Decoupled look-back within the segment: the piece publishes its own
-  // count at once, then wave 0 sums its predecessors' words, 64 per round
-  // trip, back to the first inclusive one (the segment's first piece
-  // publishes an inclusive count without waiting).  Only counts travel
-  // between pieces (the rows are read by a later kernel), so relaxed
-  // atomics suffice: no cache write-back or invalidate.
-  if (tid < kWave) {
-    uint64_t prefix = 0;
-    bool bad = false;
-    if (p > 0) {
-      if (tid == 0) status_store(&status[t], kPieceAgg | utotal);
-      const int64_t seg0 = (int64_t)(t - p);
-      int64_t base = (int64_t)t - 1;
-      for (;;) {
-        const int64_t idx = base - (int64_t)tid;
-        uint64_t w = 0;
-        bool ready = true, term = true;
-        if (idx >= seg0) {
-          w = status_load(&status[idx]);
-          ready = w != 0;
-          term = ready && (w & kPieceIncl);
+    // Pairwise merge rounds: after the round of width mw, every run holds
+    // 2mw consecutive B entries.  Each thread writes one contiguous stretch
+    // of output positions, locating it in its pair by a merge-path search; on
+    // equal rows the left run (earlier B entries) goes first.
+    cur = 0;
+    const uint32_t per = (total + kPieceBlock - 1) / kPieceBlock;
+    for (uint32_t mw = 1; mw < c; mw <<= 1) {
+      const uint32_t *kin = s_row[cur];
+      const uint16_t *iin = s_idx[cur];
+      uint32_t *kout = s_row[cur ^ 1];
+      uint16_t *iout = s_idx[cur ^ 1];
+      uint32_t o = tid * per;
+      const uint32_t oe = min(o + per, total);
+      while (o < oe) {
+        const uint32_t g = run_of(o) / (2 * mw) * (2 * mw);
+        const uint32_t a0 = s_off[g], a1 = s_off[min(c, g + mw)], b1 = s_off[min(c, g + 2 * mw)];
+        const uint32_t na = a1 - a0, nbr = b1 - a1, d = o - a0;
+        uint32_t lo = d > nbr ? d - nbr : 0, hi = d < na ? d : na;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (kin[a0 + mid] <= kin[a1 + d - 1 - mid]) lo = mid + 1;
+          else hi = mid;
         }
-        const uint64_t term_mask = __ballot(term), wait_mask = __ballot(!ready);
-        const uint32_t stop = term_mask ? (uint32_t)__builtin_ctzll(term_mask) : kWave - 1;
-        const uint64_t needed = stop == kWave - 1 ? ~0ull : ((2ull << stop) - 1ull);
-        if (wait_mask & needed) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
+        uint32_t i = a0 + lo, j = a1 + (d - lo);
+        const uint32_t stop = min(oe, b1);
+        // The two run heads stay in registers: one LDS read per output.
+        uint32_t ka = i < a1 ? kin[i] : ~0u, kb = j < b1 ? kin[j] : ~0u;
+        for (; o < stop; ++o) {
+          if (j >= b1 || (i < a1 && ka <= kb)) {
+            kout[o] = ka;
+            iout[o] = iin[i];
+            ++i;
+            ka = i < a1 ? kin[i] : ~0u;
+          } else {
+            kout[o] = kb;
+            iout[o] = iin[j];
+            ++j;
+            kb = j < b1 ? kin[j] : ~0u;
+          }
         }
-        const bool mine = tid <= stop;
-        bad = bad || __ballot(mine && (w >> 62) == 3) != 0;
-        const uint64_t part = mine ? (w & kPieceValue) : 0;
-        prefix += __shfl(wave_inclusive_sum64(part), kWave - 1, kWave);
-        if (term_mask) break;
-        base -= kWave;
       }
+      __syncthreads();
+      cur ^= 1;
     }
-    if (tid == 0) {
-      status_store(&status[t], bad ? kPieceBad : (kPieceIncl | (prefix + utotal)));
-      s_prefix = prefix;
-      s_bad = bad;
+    // Run heads over `per` consecutive positions per thread (per <= kPieceItems).
+    const uint32_t *krow = s_row[cur];
+    uint32_t flags = 0, cnt = 0;
+    const uint32_t h0 = tid * per, h1 = min(h0 + per, total);
+    for (uint32_t x = h0; x < h1; ++x) {
+      const bool h = x == 0 || krow[x] != krow[x - 1];
+      flags |= (h ? 1u : 0u) << (x - h0);
+      cnt += h;
     }
-  }
-  __syncthreads();
-  if (s_bad) return;
-  const uint64_t dst0 = seg_base + s_prefix;
-  auto product = [&](uint32_t y) {
-    const uint32_t x = kidx[y], i = s_run[x];
-    return s_b[i] * av[s_src[i] + (x - s_off[i])];
+    uint64_t utotal;
+    uint64_t urank = block_exclusive_sum_waves<kPieceBlock / kWave>(cnt, s_scan, &utotal);
+    uint32_t *s_head = s_row[cur ^ 1];  // free after the last merge round
+    for (uint32_t x = h0; x < h1; ++x)
+      if ((flags >> (x - h0)) & 1u) s_head[urank++] = x;
+    __syncthreads();
+    return utotal;
   };
-  {
+  // Writes the ordered rows' sums at dst0 (their positions in tmp_row/tmp_val).
+  auto write_rows = [&](uint64_t dst0, uint64_t utotal) {
     // Each product rounded on its own, never fused with the addition that
     // follows (as in every other accumulator).  The first product of every
     // row is fetched for all of a thread's rows before any is summed, so the
     // fetches overlap.
 #pragma clang fp contract(off)
+    const uint32_t total = s_off[c];
+    const uint32_t *krow = s_row[cur], *s_head = s_row[cur ^ 1];
+    const uint16_t *kidx = s_idx[cur];
+    auto product = [&](uint32_t y) {
+      const uint32_t x = kidx[y], i = s_run[x];
+      return s_b[i] * av[s_src[i] + (x - s_off[i])];
+    };
     uint32_t hx[kPieceItems];
     double sum[kPieceItems];
     const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
@@ -910,8 +876,103 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
         tmp_val[dst0 + u] = sum[k];
       }
     }
+  };
+  // Decoupled look-back within the segment: a piece publishes its own count
+  // as soon as it has it (an oversized piece cannot), then wave 0 sums its
+  // predecessors' words, 64 per round trip, back to the first inclusive one
+  // (the segment's first piece publishes an inclusive count without
+  // waiting).  Only counts travel between pieces (the rows are read by a
+  // later kernel), so relaxed atomics suffice: no cache write-back or
+  // invalidate.  Sets s_prefix.
+  auto look_back = [&](bool publish_own, uint64_t own) {
+    if (tid < kWave) {
+      uint64_t prefix = 0;
+      if (p > 0) {
+        if (publish_own && tid == 0) status_store(&status[t], kPieceAgg | own);
+        const int64_t seg0 = (int64_t)(t - p);
+        int64_t base = (int64_t)t - 1;
+        for (;;) {
+          const int64_t idx = base - (int64_t)tid;
+          uint64_t wd = 0;
+          bool ready = true, term = true;
+          if (idx >= seg0) {
+            wd = status_load(&status[idx]);
+            ready = wd != 0;
+            term = ready && (wd & kPieceIncl);
+          }
+          const uint64_t term_mask = __ballot(term), wait_mask = __ballot(!ready);
+          const uint32_t stop = term_mask ? (uint32_t)__builtin_ctzll(term_mask) : kWave - 1;
+          const uint64_t needed = stop == kWave - 1 ? ~0ull : ((2ull << stop) - 1ull);
+          if (wait_mask & needed) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          const uint64_t part = tid <= stop ? (wd & kPieceValue) : 0;
+          prefix += __shfl(wave_inclusive_sum64(part), kWave - 1, kWave);
+          if (term_mask) break;
+          base -= kWave;
+        }
+      }
+      if (tid == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+  };
+  uint64_t count;
+  if (total_all <= kPieceCap) {
+    count = order_rows();
+    look_back(true, count);
+    if (tid == 0) status_store(&status[t], kPieceIncl | (s_prefix + count));
+    write_rows(seg_base + s_prefix, count);
+  } else {
+    // Rows clustered: the piece goes through in row sub-ranges of at most
+    // kPieceCap products, each found by halving the remaining range (a single
+    // row holds at most c <= kPieceLists products, so the halving ends), one
+    // after another, each written after the last.  Its successors wait for
+    // its inclusive count.
+    look_back(false, 0);
+    count = 0;
+    if (tid == 0) s_lo = 0;
+    __syncthreads();
+    while (s_lo < w) {
+      if (tid < kWave) {
+        uint64_t span = w - s_lo, nxt = 0;
+        for (;;) {
+          uint64_t len = 0;
+          if (tid < c) {  // first A entry of run tid at or past row s_lo + span
+            const uint64_t key = s_key0[tid] + s_lo + span;
+            uint64_t lo = s_src[tid], hi = s_end[tid];
+            while (lo < hi) {
+              const uint64_t mid = (lo + hi) >> 1;
+              if (ak[mid] < key) lo = mid + 1;
+              else hi = mid;
+            }
+            nxt = lo;
+            len = lo - s_src[tid];
+          }
+          const uint64_t tot = __shfl(wave_inclusive_sum64(len), kWave - 1, kWave);
+          if (tot <= kPieceCap || span == 1) break;
+          span = (span + 1) / 2;
+        }
+        uint64_t len = tid < c ? nxt - s_src[tid] : 0;
+        const uint64_t incl = wave_inclusive_sum64(len);
+        if (tid < c) {
+          s_off[tid] = (uint32_t)(incl - len);
+          s_next[tid] = nxt;
+        }
+        if (tid == kWave - 1) s_off[c] = (uint32_t)incl;
+        if (tid == 0) s_lo = s_lo + span;
+      }
+      __syncthreads();
+      const uint64_t u = order_rows();
+      write_rows(seg_base + s_prefix + count, u);
+      count += u;
+      __syncthreads();
+      if (tid < c) s_src[tid] = s_next[tid];
+      __syncthreads();
+    }
+    if (tid == 0) status_store(&status[t], kPieceIncl | (s_prefix + count));
   }
-  if (tid == 0 && p + 1 == q) seg_u[s] = s_prefix + utotal;
+  if (tid == 0 && p + 1 == q) seg_u[s] = s_prefix + count;
 }
 
 // The segments whose pieces gave up, for the global sort.
@@ -1316,10 +1377,9 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     piece_bounds_kernel<<<grid_for(pairs_ub, 8192), 256, 0, ctx->stream>>>(ak, E.a_lo, E.a_len, E.a_base,
                                                                            E.seg_first, nseg, nb, E.list3, n3,
                                                                            qc_off, rmin, width, bounds, piece_seg);
-    piece_kernel<<<(uint32_t)q_ub, kPieceBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_len, E.a_base, E.seg_first,
-                                                            nseg, nb, E.list3, n3, qc_off, rmin, width, bounds,
-                                                            piece_seg, seg_pos, ticket, status, piece_ovf,
-                                                            ovf_count, E.seg_u, tmp_row, tmp_val);
+    piece_kernel<<<(uint32_t)q_ub, kPieceBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_len, E.a_base, E.list3,
+                                                                 n3, qc_off, rmin, width, bounds, piece_seg,
+                                                                 seg_pos, ticket, status, E.seg_u, tmp_row, tmp_val);
     LNT_HIP(hipGetLastError());
   }
   LNT_TRY(exclusive_scan<uint64_t>(ctx, E.seg_u, E.out_off, nseg, E.out_off + nseg, ctx->aux, 0));

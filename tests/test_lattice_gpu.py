@@ -170,7 +170,7 @@ def _piece_case(kind, rng):
         elif kind == "gap":  # rows in two bands: pieces over the gap are empty
             r = np.concatenate([rng.choice(30000, 9000, replace=False),
                                 40000 + rng.choice(30000, 9000, replace=False)])
-        elif kind == "cluster":  # one far row: the first piece overflows -> global sort
+        elif kind == "cluster":  # one far row: the first piece overflows and splits itself
             r = np.concatenate([rng.choice(2000, 1500, replace=False), [ah - 1]])
         else:  # "equal": every column the same rows -> runs of equal rows across all B entries
             r = np.arange(0, ah, ah // 2500)
@@ -192,7 +192,8 @@ def _piece_case(kind, rng):
 def test_lattice_mult_row_pieces(ctx, orc, kind):
     """Long output columns in row pieces, with 2..64 B entries per column
     (pieces), 65 and 96 (global sort), uniform rows, two bands with empty
-    pieces between, a cluster that overflows a piece (global sort), and
+    pieces between, a cluster that overflows a piece (split into row
+    sub-ranges inside its workgroup), and
     identical rows in every column (runs across all B entries): keys exact
     and value bits equal to the oracle's sparse accumulator and to the
     global sort's, for every accumulator that uses the pieces."""
