@@ -226,7 +226,7 @@ int lnt_context_destroy(lnt_context ctx) {
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->status, &ctx->small, &ctx->aux, &ctx->host_tmp_a,
                           &ctx->host_tmp_b, &ctx->host_tmp_c, &ctx->spg_ws, &ctx->spg_tmp,
-                          &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->spg_piece, &ctx->hist_part, &ctx->relin_dev, &ctx->exec_buf[0],
+                          &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->spg_plan, &ctx->spg_piece, &ctx->spg_chunk, &ctx->hist_part, &ctx->relin_dev, &ctx->exec_buf[0],
                           &ctx->exec_buf[1], &ctx->exec_buf[2], &ctx->exec_buf[3], &ctx->exec_buf[4],
                           &ctx->exec_buf[5]})
     release(*b);
@@ -326,7 +326,7 @@ int lnt_context_trim(lnt_context ctx) {
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   for (DeviceBuffer *b : {&ctx->alt_keys, &ctx->alt_vals, &ctx->alt2_keys, &ctx->alt2_vals,
                           &ctx->aux, &ctx->host_tmp_a, &ctx->host_tmp_b, &ctx->host_tmp_c,
-                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->spg_piece, &ctx->exec_buf[0], &ctx->exec_buf[1],
+                          &ctx->spg_ws, &ctx->spg_tmp, &ctx->spg_big, &ctx->spg_rank, &ctx->spg_cp, &ctx->spg_plan, &ctx->spg_piece, &ctx->spg_chunk, &ctx->exec_buf[0], &ctx->exec_buf[1],
                           &ctx->exec_buf[2], &ctx->exec_buf[3], &ctx->exec_buf[4], &ctx->exec_buf[5]})
     release(*b);
   return LNT_OK;

@@ -172,7 +172,7 @@ struct DigitScope {
 // written there; otherwise it is allocated with hipMalloc.
 int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, int s, uint64_t *out_k,
                  double *out_v, uint64_t capacity, uint64_t **ck, double **cv, uint64_t *nc,
-                 lnt_result_shape *shape) {
+                 lnt_result_shape *shape, uint64_t *nprod = nullptr) {
   *ck = nullptr;
   *cv = nullptr;
   *nc = 0;
@@ -222,9 +222,9 @@ int run_strategy(lnt_context ctx, const lnt_operand *a, const lnt_operand *b, in
   } else {
     const int acc = kSpgemm[LNT_STRATEGY_ACC(s)];
     int rc = fixed ? lattice_mult_device(ctx, fk, fv, (uint64_t)f->nnz, P.lat_a, gk, gv, (uint64_t)g->nnz, P.lat_b,
-                                         acc, k, v, capacity, &n, nullptr, nullptr, nullptr)
+                                         acc, k, v, capacity, &n, nprod, nullptr, nullptr)
                    : lattice_mult_device(ctx, fk, fv, (uint64_t)f->nnz, P.lat_a, gk, gv, (uint64_t)g->nnz, P.lat_b,
-                                         acc, nullptr, nullptr, 0, &n, nullptr, &k, &v);
+                                         acc, nullptr, nullptr, 0, &n, nprod, &k, &v);
     if (rc != LNT_OK) return rc;
     if (!fixed && !k) {  // empty product: nothing allocated
       if (hipMalloc((void **)&k, 8) != hipSuccess || hipMalloc((void **)&v, 8) != hipSuccess) {
@@ -472,17 +472,19 @@ int lnt_contract_time(lnt_context ctx, const lnt_operand *a, const lnt_operand *
   LNT_HIP(hipSetDevice(ctx->device));
   for (int s = 0; s < LNT_NUM_STRATEGIES; ++s) ms[s] = -1.0;
   if (!(mask & ((1 << LNT_NUM_STRATEGIES) - 1))) return LNT_OK;
-  // One allocating run gives nnz(C); every timed run then writes into one
-  // reusable buffer, so no allocation falls inside a timed region.
+  // One allocating run gives nnz(C) and the number of scalar products (the
+  // same for every strategy); every timed run then writes into one reusable
+  // buffer of that many entries, so no allocation falls inside a timed
+  // region and the lattice product writes its long columns straight into it.
   uint64_t *k;
   double *v;
-  uint64_t n;
+  uint64_t n, nprod = 0;
   lnt_result_shape shape;
-  LNT_TRY(run_strategy(ctx, a, b, 0, nullptr, nullptr, 0, &k, &v, &n, &shape));
+  LNT_TRY(run_strategy(ctx, a, b, 0, nullptr, nullptr, 0, &k, &v, &n, &shape, &nprod));
   LNT_HIP(hipStreamSynchronize(ctx->stream));
   (void)hipFree(k);
   (void)hipFree(v);
-  const uint64_t cap = std::max<uint64_t>({n, (uint64_t)std::min(a->nnz, b->nnz), 1});
+  const uint64_t cap = std::max<uint64_t>({n, nprod, (uint64_t)std::min(a->nnz, b->nnz), 1});
   LNT_TRY(ensure(ctx->exec_buf[4], cap * 8));
   LNT_TRY(ensure(ctx->exec_buf[5], cap * 8));
   uint64_t *ok = (uint64_t *)ctx->exec_buf[4].ptr;

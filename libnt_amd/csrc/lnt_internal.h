@@ -454,7 +454,7 @@ struct lnt_context_s {
   lnt::DeviceBuffer aux;        // merge-path splits and other per-call scratch
   lnt::DeviceBuffer hist_part;  // per-block digit histograms
   lnt::DeviceBuffer relin_dev;  // relinearisation table of the current sort's radix passes
-  lnt::DeviceBuffer spg_ws, spg_tmp, spg_big, spg_rank, spg_cp, spg_piece;  // lattice-product workspaces
+  lnt::DeviceBuffer spg_ws, spg_tmp, spg_big, spg_rank, spg_cp, spg_plan, spg_piece, spg_chunk;  // lattice-product workspaces
   lnt::DeviceBuffer host_tmp_a, host_tmp_b, host_tmp_c;  // staging for host-memory calls
   lnt::DeviceBuffer exec_buf[6];  // operand copies of a whole contraction (keys, values of both), timing output
   void *model = nullptr;          // lnt_model used by LNT_STRAT_AUTO (not owned)

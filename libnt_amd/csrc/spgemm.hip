@@ -136,12 +136,15 @@ __global__ void seg_first_kernel(const uint32_t *__restrict__ head, const uint64
 __global__ void classify_kernel(const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb,
                                 const uint64_t *__restrict__ prod_off, uint32_t small_max,
                                 uint64_t *__restrict__ seg_u, uint32_t *__restrict__ list2,
-                                uint32_t *__restrict__ list3, uint32_t *__restrict__ counts) {
+                                uint32_t *__restrict__ large, uint8_t *__restrict__ direct,
+                                uint32_t *__restrict__ counts) {
   uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; s < nseg; s += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t first = seg_first[s];
     uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
     uint64_t np = prod_off[last] - prod_off[first];
+    uint32_t is_large = 0;
+    direct[s] = 0;
     if (np == 0 || last - first == 1) {
       seg_u[s] = np;
     } else if (np <= small_max) {
@@ -149,9 +152,18 @@ __global__ void classify_kernel(const uint64_t *__restrict__ seg_first, uint64_t
       list2[atomicAdd(&counts[0], 1u)] = (uint32_t)s;
     } else {
       seg_u[s] = 0;
-      list3[atomicAdd(&counts[1], 1u)] = (uint32_t)s;
+      is_large = 1;
     }
+    large[s] = is_large;
   }
+}
+
+// The large segments in ascending order (list3[pos[s]] = s): their pieces
+// then run in output order.
+__global__ void large_list_kernel(const uint32_t *__restrict__ large, const uint64_t *__restrict__ pos,
+                                  uint64_t nseg, uint32_t *__restrict__ list3) {
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += (uint64_t)gridDim.x * blockDim.x)
+    if (large[s]) list3[pos[s]] = (uint32_t)s;
 }
 
 // Maps product p (0-based inside a segment) to its B entry e and A index.
@@ -424,6 +436,31 @@ __device__ inline ChunkSegs stage_chunk(const uint64_t *__restrict__ off, uint64
   return c;
 }
 
+// The same, with the first covering segment known (s0, from chunk_starts_kernel)
+// and the last bounded by s_hi: one short search instead of two long ones.
+__device__ inline ChunkSegs stage_chunk_from(const uint64_t *__restrict__ off, uint64_t s0, uint64_t s_hi,
+                                             uint64_t q1, uint64_t *s_off, uint64_t *s_seg) {
+  if (threadIdx.x == 0) {
+    s_seg[0] = s0;
+    s_seg[1] = upper_index(off, s0, s_hi, q1 - 1) + 1;
+  }
+  __syncthreads();
+  ChunkSegs c{s_seg[0], s_seg[1], false};
+  c.staged = c.s1 - c.s0 <= kChunk + 1;
+  if (c.staged)
+    for (uint64_t x = threadIdx.x; x < c.s1 - c.s0; x += blockDim.x) s_off[x] = off[c.s0 + x];
+  __syncthreads();
+  return c;
+}
+
+// cs[k] = the segment holding index k * kChunk (k < nchunks), all chunks at
+// once, so that no workgroup walks a long search chain alone.
+__global__ void chunk_starts_kernel(const uint64_t *__restrict__ off, uint64_t nseg, uint64_t nchunks,
+                                    uint64_t *__restrict__ cs) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nchunks; k += (uint64_t)gridDim.x * blockDim.x)
+    cs[k] = upper_index(off, 0, nseg, k * kChunk);
+}
+
 // Segment of index q and that segment's offset.
 __device__ inline uint64_t chunk_segment(const ChunkSegs &c, const uint64_t *__restrict__ off, const uint64_t *s_off,
                                          uint64_t q, uint64_t *seg_off) {
@@ -610,43 +647,57 @@ __device__ inline uint64_t block_exclusive_sum_waves(uint64_t v, uint64_t *scrat
 
 // Per large segment r: rmin, piece width, and (q << 32 | q * c) for the scan
 // of piece and (piece, B entry) pair offsets.  q = 0 marks a segment for the
-// global sort.
-__global__ void piece_plan_kernel(const uint64_t *__restrict__ ak, const uint64_t *__restrict__ a_lo,
-                                  const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
-                                  const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first,
-                                  uint64_t nseg, uint64_t nb, const uint32_t *__restrict__ list3, uint32_t n3,
-                                  uint64_t *__restrict__ rmin, uint64_t *__restrict__ width,
-                                  uint64_t *__restrict__ qc, uint32_t *__restrict__ ovf,
-                                  uint32_t *__restrict__ ovf_count, uint64_t *__restrict__ seg_pos) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n3) return;
-  const uint64_t s = list3[r];
-  const uint64_t first = seg_first[s];
-  const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
-  const uint64_t c = last - first, np = prod_off[last] - prod_off[first];
-  seg_pos[2 * r] = first;
-  seg_pos[2 * r + 1] = prod_off[first];
-  uint64_t q = 0, lo = ~0ull, hi = 0, w = 0;
-  if (c <= kPieceLists) {
-    for (uint64_t e = first; e < last; ++e) {
-      const uint32_t n = a_len[e];
-      if (!n) continue;
-      const uint64_t f = ak[a_lo[e]] - a_base[e], l = ak[a_lo[e] + n - 1] - a_base[e];
-      lo = f < lo ? f : lo;
-      hi = l > hi ? l : hi;
+// global sort.  One wave per segment, lane i on B entry i; the number of
+// large segments is read on the device (the kernel is launched before the
+// host knows it).
+__global__ __launch_bounds__(256) void piece_plan_kernel(
+    const uint64_t *__restrict__ ak, const uint64_t *__restrict__ a_lo, const uint32_t *__restrict__ a_len,
+    const uint64_t *__restrict__ a_base, const uint64_t *__restrict__ prod_off,
+    const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb, const uint32_t *__restrict__ list3,
+    const uint64_t *__restrict__ n3_dev, uint64_t *__restrict__ rmin, uint64_t *__restrict__ width,
+    uint64_t *__restrict__ qc, uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_count,
+    uint64_t *__restrict__ seg_pos) {
+  const uint64_t n3 = *n3_dev;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < n3; r += waves) {
+    const uint64_t s = list3[r];
+    const uint64_t first = seg_first[s];
+    const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+    const uint64_t c = last - first, np = prod_off[last] - prod_off[first];
+    uint64_t q = 0, lo = ~0ull, hi = 0, w = 0;
+    if (c <= kPieceLists) {
+      if (lane < c) {
+        const uint64_t e = first + lane;
+        const uint32_t n = a_len[e];
+        if (n) {
+          lo = ak[a_lo[e]] - a_base[e];
+          hi = ak[a_lo[e] + n - 1] - a_base[e];
+        }
+      }
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint64_t ol = __shfl_xor(lo, off, kWave), oh = __shfl_xor(hi, off, kWave);
+        lo = ol < lo ? ol : lo;
+        hi = oh > hi ? oh : hi;
+      }
+      // np > 0, so some column is non-empty and lo <= hi.
+      const uint64_t range = hi - lo + 1;
+      q = (np + kPieceTarget - 1) / kPieceTarget;
+      w = (range + q - 1) / q;
+      if (w > 0xFFFFFFFFull) q = 0;
+      else q = (range + w - 1) / w;
     }
-    // np > 0, so some column is non-empty and lo <= hi.
-    const uint64_t range = hi - lo + 1;
-    q = (np + kPieceTarget - 1) / kPieceTarget;
-    w = (range + q - 1) / q;
-    if (w > 0xFFFFFFFFull) q = 0;
-    else q = (range + w - 1) / w;
+    if (lane == 0) {
+      seg_pos[2 * r] = first;
+      seg_pos[2 * r + 1] = prod_off[first];
+      rmin[r] = lo;
+      width[r] = w;
+      ovf[r] = q == 0;
+      if (q == 0) atomicAdd(ovf_count, 1u);
+      qc[r] = (q << 32) | (q * c);
+    }
   }
-  rmin[r] = lo;
-  width[r] = w;
-  ovf[r] = q == 0;
-  if (q == 0) atomicAdd(ovf_count, 1u);
-  qc[r] = (q << 32) | (q * c);
 }
 
 // Last r in [0, n3) with part(off[r]) <= t (part: the high or low 32 bits).
@@ -692,14 +743,15 @@ __global__ void piece_bounds_kernel(const uint64_t *__restrict__ ak, const uint6
   }
 }
 
-__global__ __launch_bounds__(kPieceBlock) void piece_kernel(
+__global__ __launch_bounds__(kPieceBlock, 6) void piece_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
     const uint64_t *__restrict__ a_lo, const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
     const uint32_t *__restrict__ list3, uint32_t n3, const uint64_t *__restrict__ off,
     const uint64_t *__restrict__ rmin, const uint64_t *__restrict__ width, const uint64_t *__restrict__ bounds,
     const uint32_t *__restrict__ piece_seg, const uint64_t *__restrict__ seg_pos, uint32_t *__restrict__ ticket,
     uint64_t *__restrict__ status, uint64_t *__restrict__ seg_u, uint64_t *__restrict__ tmp_row,
-    double *__restrict__ tmp_val) {
+    double *__restrict__ tmp_val, int direct, const uint64_t *__restrict__ out_part,
+    const uint64_t *__restrict__ colid, uint64_t ah, uint64_t *__restrict__ ck, double *__restrict__ cv) {
   // Rows (offsets from the piece's first row) and load positions, in two
   // buffers for the merge rounds, and the B entry of each load position.  The
   // products are formed again from A (in cache) when the rows are summed,
@@ -732,6 +784,13 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
   const uint64_t q = (o1 >> 32) - (o0 >> 32), p = t - (o0 >> 32);
   const uint64_t pb = (o0 & 0xFFFFFFFFull) + p * c;
   const uint64_t w = width[r], v0 = rmin[r] + p * w;
+  // Direct mode: rows go straight into C, at the segment's offset among the
+  // other segments plus the count of every earlier piece (the look-back then
+  // runs over all pieces, in output order), keyed by the output column.
+  const uint64_t dst_base = direct ? out_part[s] : seg_base;
+  const uint64_t key_add = direct ? ah * colid[first] : 0;
+  uint64_t *const row_out = direct ? ck : tmp_row;
+  double *const val_out = direct ? cv : tmp_val;
   uint64_t total_all;
   if (tid < kWave) {  // c <= kPieceLists = one wave: lane i takes B entry i
     uint64_t len = 0;
@@ -841,8 +900,11 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
     __syncthreads();
     return utotal;
   };
-  // Writes the ordered rows' sums at dst0 (their positions in tmp_row/tmp_val).
-  auto write_rows = [&](uint64_t dst0, uint64_t utotal) {
+  // The ordered rows' sums, kept in registers (row u = tid + k * kPieceBlock
+  // in slot k) so that they are gathered before the look-back waits.
+  uint32_t hrow[kPieceItems];
+  double hsum[kPieceItems];
+  auto sum_rows = [&](uint64_t utotal) {
     // Each product rounded on its own, never fused with the addition that
     // follows (as in every other accumulator).  The first product of every
     // row is fetched for all of a thread's rows before any is summed, so the
@@ -856,14 +918,13 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
       return s_b[i] * av[s_src[i] + (x - s_off[i])];
     };
     uint32_t hx[kPieceItems];
-    double sum[kPieceItems];
     const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
 #pragma unroll
     for (uint32_t k = 0; k < kPieceItems; ++k) {
       const uint32_t u = tid + k * kPieceBlock;
       if (k < nu && u < utotal) {
         hx[k] = s_head[u];
-        sum[k] = product(hx[k]);
+        hsum[k] = product(hx[k]);
       }
     }
 #pragma unroll
@@ -871,25 +932,36 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
       const uint32_t u = tid + k * kPieceBlock;
       if (k < nu && u < utotal) {
         const uint32_t row = krow[hx[k]];
-        for (uint32_t y = hx[k] + 1; y < total && krow[y] == row; ++y) sum[k] += product(y);
-        tmp_row[dst0 + u] = v0 + row;
-        tmp_val[dst0 + u] = sum[k];
+        for (uint32_t y = hx[k] + 1; y < total && krow[y] == row; ++y) hsum[k] += product(y);
+        hrow[k] = row;
       }
     }
   };
-  // Decoupled look-back within the segment: a piece publishes its own count
-  // as soon as it has it (an oversized piece cannot), then wave 0 sums its
-  // predecessors' words, 64 per round trip, back to the first inclusive one
-  // (the segment's first piece publishes an inclusive count without
-  // waiting).  Only counts travel between pieces (the rows are read by a
+  // Stores the summed rows at dst0 (their positions in row_out/val_out).
+  auto store_rows = [&](uint64_t dst0, uint64_t utotal) {
+    const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
+#pragma unroll
+    for (uint32_t k = 0; k < kPieceItems; ++k) {
+      const uint32_t u = tid + k * kPieceBlock;
+      if (k < nu && u < utotal) {
+        row_out[dst0 + u] = v0 + hrow[k] + key_add;
+        val_out[dst0 + u] = hsum[k];
+      }
+    }
+  };
+  // Decoupled look-back within the segment (over all pieces in direct mode):
+  // a piece publishes its own count as soon as it has it (an oversized piece
+  // cannot; the first piece publishes it as inclusive), gathers its sums,
+  // then wave 0 sums its predecessors' words, 64 per round trip, back to
+  // the first inclusive one.  Only counts travel between pieces (the rows are read by a
   // later kernel), so relaxed atomics suffice: no cache write-back or
   // invalidate.  Sets s_prefix.
-  auto look_back = [&](bool publish_own, uint64_t own) {
+  const bool first_piece = direct ? t == 0 : p == 0;
+  auto look_back = [&]() {
     if (tid < kWave) {
       uint64_t prefix = 0;
-      if (p > 0) {
-        if (publish_own && tid == 0) status_store(&status[t], kPieceAgg | own);
-        const int64_t seg0 = (int64_t)(t - p);
+      if (!first_piece) {
+        const int64_t seg0 = direct ? 0 : (int64_t)(t - p);
         int64_t base = (int64_t)t - 1;
         for (;;) {
           const int64_t idx = base - (int64_t)tid;
@@ -920,16 +992,18 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
   uint64_t count;
   if (total_all <= kPieceCap) {
     count = order_rows();
-    look_back(true, count);
-    if (tid == 0) status_store(&status[t], kPieceIncl | (s_prefix + count));
-    write_rows(seg_base + s_prefix, count);
+    if (tid == 0) status_store(&status[t], (first_piece ? kPieceIncl : kPieceAgg) | count);
+    sum_rows(count);
+    look_back();
+    if (tid == 0 && !first_piece) status_store(&status[t], kPieceIncl | (s_prefix + count));
+    store_rows(dst_base + s_prefix, count);
   } else {
     // Rows clustered: the piece goes through in row sub-ranges of at most
     // kPieceCap products, each found by halving the remaining range (a single
     // row holds at most c <= kPieceLists products, so the halving ends), one
     // after another, each written after the last.  Its successors wait for
     // its inclusive count.
-    look_back(false, 0);
+    look_back();
     count = 0;
     if (tid == 0) s_lo = 0;
     __syncthreads();
@@ -964,7 +1038,8 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
       }
       __syncthreads();
       const uint64_t u = order_rows();
-      write_rows(seg_base + s_prefix + count, u);
+      sum_rows(u);
+      store_rows(dst_base + s_prefix + count, u);
       count += u;
       __syncthreads();
       if (tid < c) s_src[tid] = s_next[tid];
@@ -972,7 +1047,22 @@ __global__ __launch_bounds__(kPieceBlock) void piece_kernel(
     }
     if (tid == 0) status_store(&status[t], kPieceIncl | (s_prefix + count));
   }
-  if (tid == 0 && p + 1 == q) seg_u[s] = s_prefix + count;
+  if (!direct && tid == 0 && p + 1 == q) seg_u[s] = s_prefix + count;
+}
+
+// Direct mode: each pieced segment's row count (the difference of the
+// inclusive counts that close it and its predecessor), and its mark for the
+// output kernel, which skips it.
+__global__ void piece_total_kernel(const uint32_t *__restrict__ list3, uint32_t n3, const uint64_t *__restrict__ off,
+                                   const uint64_t *__restrict__ status, uint64_t *__restrict__ seg_u,
+                                   uint8_t *__restrict__ direct) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n3) return;
+  const uint64_t t0 = off[r] >> 32, t1 = off[r + 1] >> 32;
+  if (t1 == t0) return;  // taken by the global sort
+  const uint64_t hi = status[t1 - 1] & kPieceValue, lo = t0 ? status[t0 - 1] & kPieceValue : 0;
+  seg_u[list3[r]] = hi - lo;
+  direct[list3[r]] = 1;
 }
 
 // The segments whose pieces gave up, for the global sort.
@@ -995,23 +1085,33 @@ __global__ __launch_bounds__(256) void write_kernel(const uint64_t *__restrict__
                              const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first,
                              uint64_t nseg, uint64_t nb, const uint64_t *__restrict__ out_off, uint64_t nc,
                              const uint64_t *__restrict__ tmp_row, const double *__restrict__ tmp_val,
-                             uint64_t ah, uint64_t *__restrict__ ck, double *__restrict__ cv) {
+                             uint64_t ah, const uint64_t *__restrict__ cs,
+                             const uint8_t *__restrict__ direct, uint64_t *__restrict__ ck,
+                             double *__restrict__ cv) {
   __shared__ uint64_t s_off[kChunk + 1];
   __shared__ uint64_t s_seg[2];
-  for (uint64_t o0 = (uint64_t)blockIdx.x * kChunk; o0 < nc; o0 += (uint64_t)gridDim.x * kChunk) {
-    const uint64_t o1 = min<uint64_t>(o0 + kChunk, nc);
-    const ChunkSegs c = stage_chunk(out_off, nseg, o0, o1, s_off, s_seg);
+  const uint64_t nchunks = (nc + kChunk - 1) / kChunk;
+  for (uint64_t k = blockIdx.x; k < nchunks; k += gridDim.x) {
+    const uint64_t o0 = k * kChunk, o1 = min<uint64_t>(o0 + kChunk, nc);
+    const ChunkSegs c = stage_chunk_from(out_off, cs[k], k + 1 < nchunks ? min<uint64_t>(cs[k + 1] + 1, nseg) : nseg,
+                                         o1, s_off, s_seg);
     // kPer elements per thread, all loads issued before any store, so that
     // their chains of dependent loads overlap.
     constexpr uint32_t kPer = kChunk / 256;
+    // Segments whose rows the pieces wrote are skipped.
     uint64_t key[kPer];
     double val[kPer];
+    uint32_t skip = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
       const uint64_t o = o0 + threadIdx.x + k * 256;
       if (o < o1) {
         uint64_t base;
         const uint64_t s = chunk_segment(c, out_off, s_off, o, &base);
+        if (direct[s]) {
+          skip |= 1u << k;
+          continue;
+        }
         const uint64_t q = o - base;
         const uint64_t first = seg_first[s];
         const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
@@ -1028,7 +1128,7 @@ __global__ __launch_bounds__(256) void write_kernel(const uint64_t *__restrict__
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
       const uint64_t o = o0 + threadIdx.x + k * 256;
-      if (o < o1) {
+      if (o < o1 && !((skip >> k) & 1u)) {
         ck[o] = key[k];
         cv[o] = val[k];
       }
@@ -1096,9 +1196,17 @@ static int check_lattice(const int64_t a_dims[3], const int64_t b_dims[3]) {
   return LNT_OK;
 }
 
+// Row-piece arrays: the plan per large segment (sized by the segment count),
+// then per piece and per (piece, B entry) pair.
+struct PieceArrays {
+  uint64_t *rmin, *width, *qc, *qc_off, *seg_pos, *bounds, *status;
+  uint32_t *ovf, *list, *piece_seg;
+};
+
 struct EntryArrays {
   uint64_t *a_lo, *a_base, *colid, *prod_off, *seg_idx, *seg_first, *seg_u, *out_off, *np3, *ex_off;
   uint32_t *a_len, *head, *list2, *list3, *counts;
+  uint8_t *direct;  // segments whose rows the pieces write straight into C
   size_t scan_words;
 };
 
@@ -1109,7 +1217,7 @@ constexpr uint64_t kMaxPtrColumns = uint64_t(1) << 26;
 static int entry_phase(lnt_context ctx, const uint64_t *ak, uint64_t na, const int64_t a_dims[3],
                        const uint64_t *bk, uint64_t nb, const int64_t b_dims[3], bool csc, EntryArrays *E) {
   size_t scan_words = scan_workspace_words(nb + 1);
-  size_t bytes = (nb + 1) * (8 * 10 + 4 * 4) + 16 * 256 + 64 + scan_words * 8 + 8192;
+  size_t bytes = (nb + 1) * (8 * 10 + 4 * 4 + 1) + 17 * 256 + 64 + scan_words * 8 + 8192;
   LNT_TRY(ensure(ctx->spg_ws, bytes));
   Carve c{(char *)ctx->spg_ws.ptr};
   E->a_lo = c.take<uint64_t>(nb + 1);
@@ -1127,6 +1235,7 @@ static int entry_phase(lnt_context ctx, const uint64_t *ak, uint64_t na, const i
   E->list2 = c.take<uint32_t>(nb + 1);
   E->list3 = c.take<uint32_t>(nb + 1);
   E->counts = c.take<uint32_t>(16);
+  E->direct = c.take<uint8_t>(nb + 1);
   E->scan_words = scan_words;
   LatticeDims L;
   L.ah = (uint64_t)a_dims[0];
@@ -1252,12 +1361,42 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
                                                      : (uint32_t)std::min(ctx->tuning.spgemm_small, kSmall);
   seg_first_kernel<<<grid_for(nb), 256, 0, ctx->stream>>>(E.head, E.seg_idx, nb, E.seg_first);
   LNT_TRY(launch_fill_u32(ctx, E.counts, 0, 4));
+  // E.head (free after seg_first_kernel) flags the large segments; their
+  // list is built in ascending order, so that their pieces run in output
+  // order; E.ex_off holds its positions until the global sort reuses it.
   classify_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.seg_first, nseg, nb, E.prod_off, small_max,
-                                                            E.seg_u, E.list2, E.list3, E.counts);
+                                                            E.seg_u, E.list2, E.head, E.direct, E.counts);
   LNT_HIP(hipGetLastError());
-  LNT_HIP(hipMemcpyAsync(ctx->pinned + 4, E.counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, E.head, E.ex_off, nseg, E.ex_off + nseg, ctx->aux, 0));
+  large_list_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.head, E.ex_off, nseg, E.list3);
+  LNT_HIP(hipGetLastError());
+  // Row pieces (every accumulator but ESC): planned before the wait below,
+  // over at most nseg large segments, so that one wait also returns how
+  // many segments the pieces cannot take.
+  const bool try_pieces = acc != LNT_SPGEMM_ESC && ctx->tuning.spgemm_pieces;
+  PieceArrays PA{};
+  if (try_pieces) {
+    LNT_TRY(ensure(ctx->spg_plan, (size_t)nseg * (8 * 6 + 4 * 2) + 8 * 256));
+    Carve c{(char *)ctx->spg_plan.ptr};
+    PA.rmin = c.take<uint64_t>(nseg);
+    PA.width = c.take<uint64_t>(nseg);
+    PA.qc = c.take<uint64_t>(nseg);
+    PA.qc_off = c.take<uint64_t>(nseg + 1);
+    PA.ovf = c.take<uint32_t>(nseg);
+    PA.list = c.take<uint32_t>(nseg);
+    PA.seg_pos = c.take<uint64_t>(2 * nseg);
+    piece_plan_kernel<<<grid_for(nseg * kWave), 256, 0, ctx->stream>>>(ak, E.a_lo, E.a_len, E.a_base, E.prod_off, E.seg_first,
+                                                               nseg, nb, E.list3, E.ex_off + nseg, PA.rmin, PA.width,
+                                                               PA.qc, PA.ovf, E.counts + 3, PA.seg_pos);
+    LNT_HIP(hipGetLastError());
+  }
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 4, E.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 6, E.counts + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  LNT_HIP(hipMemcpyAsync(ctx->pinned + 8, E.ex_off + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
   LNT_HIP(hipStreamSynchronize(ctx->stream));
-  const uint32_t n2 = ctx->pinned[4], n3 = ctx->pinned[5];
+  uint64_t n3_64;
+  memcpy(&n3_64, ctx->pinned + 8, sizeof(n3_64));
+  const uint32_t n2 = ctx->pinned[4], n3 = (uint32_t)n3_64, n_back = ctx->pinned[6];
   if (n2 || n3) LNT_TRY(ensure(ctx->spg_tmp, nprod * 16));
   uint64_t *tmp_row = (uint64_t *)ctx->spg_tmp.ptr;
   double *tmp_val = tmp_row ? (double *)(tmp_row + nprod) : nullptr;
@@ -1346,63 +1485,51 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     }
     return LNT_OK;
   };
-  // The large segments: row pieces (every accumulator but ESC), with the
-  // global sort for the segments the pieces cannot take.
-  uint32_t *piece_ovf = nullptr, *piece_list = nullptr;
-  const bool pieces = n3 && acc != LNT_SPGEMM_ESC && ctx->tuning.spgemm_pieces;
+  // The large segments: row pieces, with the global sort for the segments
+  // the pieces cannot take (run first, so that every count the pieces do not
+  // produce is known before they write).  When the caller's arrays hold
+  // nprod entries (nnz(C) <= nprod), the pieces write their rows straight
+  // into C: a look-back over all pieces in output order adds their counts to
+  // the offsets of the other segments.
+  const bool pieces = n3 && try_pieces;
+  const bool direct = pieces && ck && !ck_alloc && capacity >= nprod;
   if (n3 && !pieces) LNT_TRY(run_esc(E.list3, n3));
   if (pieces) {
     const uint64_t q_ub = nprod / kPieceTarget + n3 + 1, pairs_ub = q_ub * kPieceLists;
-    const size_t scan_q = scan_workspace_words(n3 + 1);
-    LNT_TRY(ensure(ctx->spg_piece, (size_t)n3 * (8 * 6 + 4 * 2) + (pairs_ub + q_ub) * 8 + q_ub * 4 + 4096));
+    LNT_TRY(ensure(ctx->spg_piece, (pairs_ub + q_ub) * 8 + q_ub * 4 + 1024));
     Carve c{(char *)ctx->spg_piece.ptr};
-    uint64_t *rmin = c.take<uint64_t>(n3);
-    uint64_t *width = c.take<uint64_t>(n3);
-    uint64_t *qc = c.take<uint64_t>(n3);
-    uint64_t *qc_off = c.take<uint64_t>(n3 + 1);
-    piece_ovf = c.take<uint32_t>(n3);
-    piece_list = c.take<uint32_t>(n3);
-    uint64_t *bounds = c.take<uint64_t>(pairs_ub);
-    uint64_t *status = c.take<uint64_t>(q_ub);
-    uint64_t *seg_pos = c.take<uint64_t>(2 * (uint64_t)n3);
-    uint32_t *piece_seg = c.take<uint32_t>(q_ub);
-    LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan_q) * 8 + 64));
-    uint32_t *ticket = E.counts + 2, *ovf_count = E.counts + 3;  // zeroed with the classification counts
-    piece_plan_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(ak, E.a_lo, E.a_len, E.a_base, E.prod_off,
-                                                                  E.seg_first, nseg, nb, E.list3, n3, rmin, width,
-                                                                  qc, piece_ovf, ovf_count, seg_pos);
-    LNT_HIP(hipGetLastError());
-    LNT_TRY(exclusive_scan<uint64_t>(ctx, qc, qc_off, n3, qc_off + n3, ctx->aux, 0));
-    fill_words_kernel<<<grid_for(q_ub, 4096), 256, 0, ctx->stream>>>(status, q_ub);
+    PA.bounds = c.take<uint64_t>(pairs_ub);
+    PA.status = c.take<uint64_t>(q_ub);
+    PA.piece_seg = c.take<uint32_t>(q_ub);
+    LNT_TRY(ensure(ctx->aux, std::max(E.scan_words, scan_workspace_words(n3 + 1)) * 8 + 64));
+    if (n_back) {
+      LNT_TRY(launch_fill_u32(ctx, E.counts + 3, 0, 1));
+      piece_fallback_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(E.list3, n3, PA.ovf, PA.list, E.counts + 3);
+      LNT_HIP(hipGetLastError());
+      LNT_TRY(run_esc(PA.list, n_back));
+    }
+    LNT_TRY(exclusive_scan<uint64_t>(ctx, PA.qc, PA.qc_off, n3, PA.qc_off + n3, ctx->aux, 0));
+    uint64_t *out_part = E.np3;  // free once the global sort is done
+    if (direct) LNT_TRY(exclusive_scan<uint64_t>(ctx, E.seg_u, out_part, nseg, out_part + nseg, ctx->aux, 0));
+    fill_words_kernel<<<grid_for(q_ub, 4096), 256, 0, ctx->stream>>>(PA.status, q_ub);
     piece_bounds_kernel<<<grid_for(pairs_ub, 8192), 256, 0, ctx->stream>>>(ak, E.a_lo, E.a_len, E.a_base,
                                                                            E.seg_first, nseg, nb, E.list3, n3,
-                                                                           qc_off, rmin, width, bounds, piece_seg);
-    piece_kernel<<<(uint32_t)q_ub, kPieceBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_len, E.a_base, E.list3,
-                                                                 n3, qc_off, rmin, width, bounds, piece_seg,
-                                                                 seg_pos, ticket, status, E.seg_u, tmp_row, tmp_val);
+                                                                           PA.qc_off, PA.rmin, PA.width, PA.bounds,
+                                                                           PA.piece_seg);
+    piece_kernel<<<(uint32_t)q_ub, kPieceBlock, 0, ctx->stream>>>(
+        ak, av, bv, E.a_lo, E.a_len, E.a_base, E.list3, n3, PA.qc_off, PA.rmin, PA.width, PA.bounds, PA.piece_seg,
+        PA.seg_pos, E.counts + 2, PA.status, E.seg_u, tmp_row, tmp_val, direct ? 1 : 0, out_part, E.colid, ah, ck,
+        cv);
     LNT_HIP(hipGetLastError());
+    if (direct) {
+      piece_total_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(E.list3, n3, PA.qc_off, PA.status, E.seg_u,
+                                                                     E.direct);
+      LNT_HIP(hipGetLastError());
+    }
   }
   LNT_TRY(exclusive_scan<uint64_t>(ctx, E.seg_u, E.out_off, nseg, E.out_off + nseg, ctx->aux, 0));
   uint64_t nc;
-  if (pieces) {
-    // nnz(C) and the count of segments the pieces gave up, in one wait.
-    LNT_HIP(hipMemcpyAsync(ctx->pinned + 8, E.out_off + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-    LNT_HIP(hipMemcpyAsync(ctx->pinned + 10, E.counts + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    LNT_HIP(hipStreamSynchronize(ctx->stream));
-    memcpy(&nc, ctx->pinned + 8, sizeof(nc));
-    const uint32_t n_back = ctx->pinned[10];
-    if (n_back) {
-      LNT_TRY(launch_fill_u32(ctx, E.counts + 3, 0, 1));
-      piece_fallback_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(E.list3, n3, piece_ovf, piece_list,
-                                                                       E.counts + 3);
-      LNT_HIP(hipGetLastError());
-      LNT_TRY(run_esc(piece_list, n_back));
-      LNT_TRY(exclusive_scan<uint64_t>(ctx, E.seg_u, E.out_off, nseg, E.out_off + nseg, ctx->aux, 0));
-      LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
-    }
-  } else {
-    LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
-  }
+  LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
   if (ck_alloc) {
     void *k = nullptr, *v = nullptr;
     size_t bytes = (nc ? nc : 1) * 8;
@@ -1421,8 +1548,13 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) + " is below nnz(C) = " +
                                               std::to_string(nc));
   }
+  const uint64_t nchunks = (nc + kChunk - 1) / kChunk;
+  LNT_TRY(ensure(ctx->spg_chunk, (nchunks + 1) * sizeof(uint64_t)));
+  uint64_t *cs = (uint64_t *)ctx->spg_chunk.ptr;
+  chunk_starts_kernel<<<grid_for(nchunks, 4096), 256, 0, ctx->stream>>>(E.out_off, nseg, nchunks, cs);
   write_kernel<<<chunk_grid(nc), 256, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.colid, E.prod_off, E.seg_first,
-                                                       nseg, nb, E.out_off, nc, tmp_row, tmp_val, ah, ck, cv);
+                                                       nseg, nb, E.out_off, nc, tmp_row, tmp_val, ah, cs, E.direct,
+                                                       ck, cv);
   const hipError_t launch = hipGetLastError();
   if (launch != hipSuccess) {
     prof_end(ctx, slot);

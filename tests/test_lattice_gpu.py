@@ -196,19 +196,30 @@ def test_lattice_mult_row_pieces(ctx, orc, kind):
     sub-ranges inside its workgroup), and
     identical rows in every column (runs across all B entries): keys exact
     and value bits equal to the oracle's sparse accumulator and to the
-    global sort's, for every accumulator that uses the pieces."""
+    global sort's, for every accumulator that uses the pieces.  The pieces
+    run twice: writing straight into C (output arrays of nprod entries) and
+    staged through the workspace (arrays of exactly nnz(C) entries)."""
     rng = np.random.default_rng(["uniform", "gap", "cluster", "equal"].index(kind) + 40)
     ah, aw, bw, ak, av, bk, bv = _piece_case(kind, rng)
     print("row pieces", kind, len(ak), len(bk), flush=True)
     sk, sv = orc.lattice_mult_by("spa", ak, av, (ah, aw, 1), bk, bv, (aw, bw, 1))
     try:
         for strategy in (1, 2, 17):
-            for pieces in (1, 0):
-                ctx.set_param("spgemm_pieces", pieces)
-                ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1), strategy)
-                assert np.array_equal(host(ck), sk), (strategy, pieces)
+            for mode in ("direct", "staged", "sort"):
+                ctx.set_param("spgemm_pieces", int(mode != "sort"))
+                if mode == "staged":
+                    import torch
+
+                    ck = torch.empty(len(sk), dtype=torch.int64, device="cuda")
+                    cv = torch.empty(len(sk), dtype=torch.float64, device="cuda")
+                    n = ctx.lattice_mult_into(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1),
+                                              ck, cv, strategy)
+                    assert n == len(sk), (strategy, mode)
+                else:
+                    ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1), strategy)
+                assert np.array_equal(host(ck), sk), (strategy, mode)
                 if strategy != 2:
-                    assert np.array_equal(host(cv).view(np.int64), sv.view(np.int64)), (strategy, pieces)
+                    assert np.array_equal(host(cv).view(np.int64), sv.view(np.int64)), (strategy, mode)
                 else:
                     assert orc.allclose_fuzzy(host(cv), sv, TOL)
     finally:
