@@ -93,6 +93,12 @@ int lnt_batch_launch(lnt_batch batch);
  *   "small_max"      largest segment grouped into items   (2..1024)
  *   "window"         target products per window of a large segment (64..1536)
  *   "blocks_per_cu"  persistent workgroups per compute unit (1..8)
+ *   "piece_slots"    at most this many row pieces per run (-1: no limit, the
+ *                    default; 0: every large segment in windows).  A large
+ *                    segment of at most 64 B entries is cut by row value into
+ *                    pieces of about 3,072 products merged in LDS; the pool of
+ *                    piece slots grows to what the last run needed, and a
+ *                    segment that finds no room is processed in windows.
  * The sort parameters are the context's (lnt_context_set_param). */
 int lnt_batch_set_param(lnt_batch batch, const char *name, int64_t value);
 int lnt_batch_get_param(lnt_batch batch, const char *name, int64_t *value);

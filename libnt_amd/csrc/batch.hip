@@ -64,6 +64,8 @@ enum {
   kCtrTicketLarge,
   kCtrKeyErr,       // an input key outside its instance's dimensionality
   kCtrOverflow,     // C larger than the output capacity
+  kCtrN3w,          // large segments taken by the windows
+  kCtrPieces,       // piece slots the batch needs
   kCtrWords = 16
 };
 
@@ -624,7 +626,7 @@ template <bool NUMERIC, typename K>
 __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const uint32_t *__restrict__ list3,
                                                              uint64_t *__restrict__ cur, uint32_t *__restrict__ wcnt,
                                                              uint32_t *__restrict__ woff, uint32_t window,
-                                                             uint64_t *__restrict__ ctr) {
+                                                             uint64_t *__restrict__ ctr, uint32_t count_slot) {
   // K: key word of the window sort (window-relative rows); 32 bits when
   // every instance's rows fit, which halves the sort's LDS.
   __shared__ unsigned long long s_hash[NUMERIC ? 1 : kHashSlots];
@@ -640,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
   __shared__ uint64_t s_item;
   const uint32_t tid = threadIdx.x;
   const uint64_t nseg = ctr[kCtrNseg];
-  const uint64_t n3 = ctr[kCtrN3];
+  const uint64_t n3 = ctr[count_slot];  // segments in list3
   for (;;) {
     if (tid == 0) s_item = atomicAdd((unsigned long long *)&ctr[kCtrTicketLarge], 1ull);
     __syncthreads();
@@ -797,6 +799,417 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
   }
 }
 
+// ------------------------------------------ 4/5. large segments, row pieces
+// A large segment of at most kPieceLists B entries is cut by row value into
+// q equal row ranges ("pieces") of about kPieceTarget products: every A
+// column is sorted by row, so a piece is one contiguous run of each column.
+// Pieces are independent work items: one 512-thread
+// workgroup finds its runs (four lanes per bound, in rounds of four probes),
+// loads their rows into LDS, merges them pairwise (ceil(log2 c) merge-path
+// rounds, the earlier B entry first on equal rows) and sums equal rows in
+// that order -- the chain of additions, in ascending B-row order, of the
+// window sort.  The symbolic pass counts the piece's distinct rows (into the
+// piece's slot and its segment's count); the numeric pass takes its output
+// offset from its predecessors' counts and writes its rows.  A piece of more
+// than kPieceCap products (rows clustered in one range) goes through in row
+// sub-ranges found by halving.  Piece slots come from a pool of fixed size;
+// a segment that does not fit in it, has more than kPieceLists entries, or
+// spans more than 2^32 rows per piece takes the windows instead.
+constexpr uint32_t kPieceBlock = 512;
+constexpr uint32_t kPieceCap = 3584;  // 7 per thread; 49 KB of LDS: three workgroups per CU
+constexpr uint32_t kPieceTarget = 3072;
+constexpr uint32_t kPieceLists = 64;
+constexpr uint32_t kPieceItems = kPieceCap / kPieceBlock;
+constexpr uint32_t kNoPiece = 0xFFFFFFFFu;
+
+struct PieceArgs {
+  uint64_t *rmin, *width, *pbase;  // per large segment (position in list3)
+  uint32_t *pq;                    // its pieces; 0: the windows take it
+  uint32_t *piece_seg, *piece_u;   // per piece slot: position in list3 (kNoPiece: unused), distinct rows
+  uint64_t *piece_ent;             // per piece slot: first B entry << 8 | entries
+  uint32_t *list3w;                // the segments the windows take
+  uint64_t cap;                    // piece slots
+};
+
+// First x in [lo, hi) with a[x] >= key (hi if none), searched by the G lanes
+// of a group (G a power of two, 2..32) in rounds of G probes: a[x] < key
+// holds below the answer and fails from it on, so one ballot over G evenly
+// spaced x narrows the range (G+1)-fold.  Every lane of the wave calls it
+// with the same G; a group with `active` false returns lo.
+__device__ inline uint64_t group_lower_bound(const uint64_t *__restrict__ a, uint64_t lo, uint64_t hi, uint64_t key,
+                                             bool active, uint32_t G) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), gl = lane & (G - 1), shift = lane - gl;
+  const uint64_t mask = (1ull << G) - 1;
+  if (!active) hi = lo;
+  while (__ballot(hi > lo)) {
+    const uint64_t w = hi - lo;
+    // Probes inside [lo, hi): lo + gl when the range fits the group, else
+    // strictly increasing points.
+    const uint64_t x = w <= G ? lo + gl : lo + ((uint64_t)(gl + 1) * w) / (G + 1);
+    const bool live = w > G || gl < w;
+    const bool below = live && a[x] < key;
+    const uint32_t trues = (uint32_t)__popcll((__ballot(below) >> shift) & mask);
+    const uint64_t x_last = __shfl(x, shift + (trues ? trues - 1 : 0), kWave);
+    const uint64_t x_fail = __shfl(x, shift + (trues < G ? trues : G - 1), kWave);
+    if (w == 0) continue;
+    if (w <= G) {
+      lo += trues;
+      hi = lo;
+    } else {
+      if (trues) lo = x_last + 1;
+      if (trues < G) hi = x_fail;
+    }
+  }
+  return lo;
+}
+
+// Per large segment (one wave each, lane i on B entry i): the row range of
+// its A columns, the piece width and the number of pieces (0: the windows
+// take it).  pq is written for all nb slots (zero past the last large
+// segment), so that its scan gives every segment its first piece slot and
+// the total the batch needs.
+__global__ __launch_bounds__(kBlock) void batch_piece_plan_kernel(ColumnArgs A, const uint32_t *__restrict__ list3,
+                                                                  PieceArgs Q, const uint64_t *__restrict__ ctr) {
+  const uint64_t n3 = ctr[kCtrN3], nseg = ctr[kCtrNseg];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < A.nb; r += waves) {
+    if (r >= n3) {
+      if (lane == 0) Q.pq[r] = 0;
+      continue;
+    }
+    const uint64_t s = list3[r];
+    const uint64_t first = A.seg_first[s], last = seg_last(A.seg_first, s, nseg, A.nb);
+    const uint64_t c = last - first, np = A.prod_off[last] - A.prod_off[first];
+    uint64_t q = 0, lo = ~0ull, hi = 0, w = 0;
+    if (c <= kPieceLists) {
+      if (lane < c) {
+        const uint64_t e = first + lane;
+        const uint32_t n = A.a_len[e];
+        if (n) {
+          lo = A.ak[A.a_lo[e]] - A.a_base[e];
+          hi = A.ak[A.a_lo[e] + n - 1] - A.a_base[e];
+        }
+      }
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint64_t ol = __shfl_xor(lo, off, kWave), oh = __shfl_xor(hi, off, kWave);
+        lo = ol < lo ? ol : lo;
+        hi = oh > hi ? oh : hi;
+      }
+      // np > 0 (a large segment), so some column is non-empty and lo <= hi.
+      const uint64_t range = hi - lo + 1;
+      q = (np + kPieceTarget - 1) / kPieceTarget;
+      w = (range + q - 1) / q;
+      if (w > 0xFFFFFFFFull) q = 0;
+      else q = (range + w - 1) / w;
+    }
+    if (lane == 0) {
+      Q.rmin[r] = lo;
+      Q.width[r] = w;
+      Q.pq[r] = (uint32_t)q;
+    }
+  }
+}
+
+// After the scan of pq into pbase: the piece slots of every segment that
+// fits in the pool (the first ones, as slots ascend with the segment's
+// position in list3); the others go to the windows.
+__global__ __launch_bounds__(kBlock) void batch_piece_assign_kernel(ColumnArgs A, const uint32_t *__restrict__ list3,
+                                                                    PieceArgs Q, uint64_t *__restrict__ ctr) {
+  const uint64_t n3 = ctr[kCtrN3], nseg = ctr[kCtrNseg];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < n3; r += waves) {
+    const uint64_t q = Q.pq[r], base = Q.pbase[r];
+    const bool fits = q && base + q <= Q.cap;
+    if (fits) {
+      const uint64_t s = list3[r], first = A.seg_first[s], last = seg_last(A.seg_first, s, nseg, A.nb);
+      for (uint64_t t = base + lane; t < base + q; t += kWave) {
+        Q.piece_seg[t] = (uint32_t)r;
+        Q.piece_ent[t] = first << 8 | (last - first);
+      }
+    }
+    else if (q && base < Q.cap)  // the one segment that straddles the pool's end
+      for (uint64_t t = base + lane; t < Q.cap; t += kWave) Q.piece_seg[t] = kNoPiece;
+    if (lane == 0 && !fits) {
+      Q.pq[r] = 0;
+      Q.list3w[atomicAdd((unsigned long long *)&ctr[kCtrN3w], 1ull)] = list3[r];
+    }
+  }
+}
+
+template <bool NUMERIC>
+__global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs A, const uint32_t *__restrict__ list3,
+                                                                     PieceArgs Q, uint64_t *__restrict__ ctr) {
+  // Rows (offsets from the piece's first row) and load positions, in two
+  // buffers for the merge rounds, and the B entry of each load position.
+  __shared__ uint32_t s_row[2][kPieceCap];
+  __shared__ uint16_t s_idx[2][kPieceCap];
+  __shared__ uint8_t s_run[kPieceCap];
+  __shared__ uint32_t s_off[kPieceLists + 1];
+  __shared__ uint64_t s_src[kPieceLists];   // first A entry of each run of the current range
+  __shared__ uint64_t s_next[kPieceLists];  // first A entry past the current range
+  __shared__ uint64_t s_end[kPieceLists];   // first A entry past the piece
+  __shared__ uint64_t s_key0[kPieceLists];  // key of the piece's first row in each column
+  __shared__ double s_b[kPieceLists];
+  __shared__ uint64_t s_scan[kPieceBlock / kWave];
+  __shared__ uint64_t s_all, s_lo, s_base;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t nseg = ctr[kCtrNseg];
+  // Pieces in a fixed stride over the persistent grid (no shared ticket:
+  // pieces are alike in size, and one contended counter would serialise
+  // millions of them).
+  const uint64_t npieces = min<uint64_t>(ctr[kCtrPieces], Q.cap);
+  for (uint64_t t = blockIdx.x; t < npieces; t += gridDim.x) {
+    // Loads that do not wait on one another: the piece's segment position
+    // and entries, then the segment's plan and the entries' columns.
+    const uint32_t r = Q.piece_seg[t];
+    if (r == kNoPiece) continue;
+    const uint64_t ent = Q.piece_ent[t];
+    const uint64_t first = ent >> 8;
+    const uint32_t c = (uint32_t)(ent & 0xFF);
+    const uint64_t s = list3[r];
+    const uint64_t q = Q.pq[r], p = t - Q.pbase[r], w = Q.width[r], v0 = Q.rmin[r] + p * w;
+    const bool last_piece = p + 1 == q;
+    if (NUMERIC && tid < kWave) {
+      // Output offset: the segment's, plus the rows of its earlier pieces
+      // (issued before the searches, which do not need it).
+      uint64_t before = 0;
+      const uint64_t b0 = Q.pbase[r];
+      for (uint64_t j = tid; j < p; j += kWave) before += Q.piece_u[b0 + j];
+      before = __shfl(wave_inclusive_sum64(before), kWave - 1, kWave);
+      if (tid == 0) s_base = A.out_off[s] + before;
+    }
+    {
+      // The piece's runs: 2G lanes per B entry, G per side (0: its first A
+      // entry in the piece, 1: the first past it -- the column's end for the
+      // last piece), G the largest power of two up to 32 that fits all
+      // entries in the block.
+      uint32_t G = 32;
+      while (G > 4 && 2 * G * c > kPieceBlock) G >>= 1;
+      const uint32_t i = tid / (2 * G), side = (tid / G) & 1;
+      uint64_t lo = 0, hi = 0, key = 0;
+      if (i < c) {
+        const uint64_t e = first + i;
+        lo = A.a_lo[e];
+        hi = lo + A.a_len[e];
+        key = A.a_base[e] + v0 + (side ? w : 0);
+      }
+      const bool search = i < c && !(side ? last_piece : p == 0);
+      const uint64_t bnd = group_lower_bound(A.ak, lo, hi, key, search, G);
+      if (i < c && (tid & (G - 1)) == 0) {
+        const uint64_t e = first + i;
+        if (side == 0) {
+          s_src[i] = bnd;
+          s_key0[i] = A.a_base[e] + v0;
+          s_b[i] = A.bv[e];
+        } else {
+          s_end[i] = last_piece ? hi : bnd;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < kWave) {  // c <= kPieceLists = one wave: lane i takes run i
+      uint64_t len = 0;
+      if (tid < c) {
+        len = s_end[tid] - s_src[tid];
+        s_next[tid] = s_end[tid];
+      }
+      const uint64_t incl = wave_inclusive_sum64(len);
+      if (tid < c) s_off[tid] = (uint32_t)min<uint64_t>(incl - len, kPieceCap + 1);
+      if (tid == kWave - 1) {
+        s_off[c] = (uint32_t)min<uint64_t>(incl, kPieceCap + 1);
+        s_all = incl;
+      }
+    }
+    __syncthreads();
+    const uint64_t total_all = s_all;
+    const uint64_t ckey = A.colkey[first];
+    auto run_of = [&](uint32_t x) {
+      uint32_t lo = 0, hi = c;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= x) lo = mid;
+        else hi = mid;
+      }
+      return lo;
+    };
+    uint32_t cur = 0;
+    // Loads the runs s_src[i] + [0, s_off[i+1] - s_off[i]) (at most
+    // kPieceCap products), orders them by (row, B entry), lists the first
+    // position of every distinct row and returns their number; `cur` names
+    // the ordered buffer, s_row[cur ^ 1] holds the list.
+    auto order_rows = [&]() -> uint64_t {
+      const uint32_t total = s_off[c];
+      for (uint32_t x = tid; x < total; x += kPieceBlock) {
+        const uint32_t i = run_of(x);
+        const uint64_t ai = s_src[i] + (x - s_off[i]);
+        s_row[0][x] = (uint32_t)(A.ak[ai] - s_key0[i]);
+        s_idx[0][x] = (uint16_t)x;
+        s_run[x] = (uint8_t)i;
+      }
+      __syncthreads();
+      // Pairwise merge rounds: after the round of width mw, every run holds
+      // 2mw consecutive B entries.  Each thread writes one contiguous
+      // stretch of output positions, located in its pair by a merge-path
+      // search; on equal rows the left run (earlier B entries) goes first.
+      cur = 0;
+      const uint32_t per = (total + kPieceBlock - 1) / kPieceBlock;
+      for (uint32_t mw = 1; mw < c; mw <<= 1) {
+        const uint32_t *kin = s_row[cur];
+        const uint16_t *iin = s_idx[cur];
+        uint32_t *kout = s_row[cur ^ 1];
+        uint16_t *iout = s_idx[cur ^ 1];
+        uint32_t o = tid * per;
+        const uint32_t oe = min(o + per, total);
+        while (o < oe) {
+          const uint32_t g = run_of(o) / (2 * mw) * (2 * mw);
+          const uint32_t a0 = s_off[g], a1 = s_off[min(c, g + mw)], b1 = s_off[min(c, g + 2 * mw)];
+          const uint32_t na = a1 - a0, nbr = b1 - a1, d = o - a0;
+          uint32_t lo = d > nbr ? d - nbr : 0, hi = d < na ? d : na;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (kin[a0 + mid] <= kin[a1 + d - 1 - mid]) lo = mid + 1;
+            else hi = mid;
+          }
+          uint32_t i = a0 + lo, j = a1 + (d - lo);
+          const uint32_t stop = min(oe, b1);
+          uint32_t ka = i < a1 ? kin[i] : ~0u, kb = j < b1 ? kin[j] : ~0u;
+          for (; o < stop; ++o) {
+            if (j >= b1 || (i < a1 && ka <= kb)) {
+              kout[o] = ka;
+              iout[o] = iin[i];
+              ++i;
+              ka = i < a1 ? kin[i] : ~0u;
+            } else {
+              kout[o] = kb;
+              iout[o] = iin[j];
+              ++j;
+              kb = j < b1 ? kin[j] : ~0u;
+            }
+          }
+        }
+        __syncthreads();
+        cur ^= 1;
+      }
+      // Distinct rows over `per` consecutive positions per thread.
+      const uint32_t *krow = s_row[cur];
+      uint32_t flags = 0, cnt = 0;
+      const uint32_t h0 = tid * per, h1 = min(h0 + per, total);
+      for (uint32_t x = h0; x < h1; ++x) {
+        const bool h = x == 0 || krow[x] != krow[x - 1];
+        flags |= (h ? 1u : 0u) << (x - h0);
+        cnt += h;
+      }
+      uint64_t utotal;
+      uint64_t urank = block_exclusive_sum_waves<kPieceBlock / kWave>(cnt, s_scan, &utotal);
+      if (NUMERIC) {
+        uint32_t *s_head = s_row[cur ^ 1];  // free after the last merge round
+        for (uint32_t x = h0; x < h1; ++x)
+          if ((flags >> (x - h0)) & 1u) s_head[urank++] = x;
+      }
+      __syncthreads();
+      return utotal;
+    };
+    // Writes the ordered rows' sums at dst0 of C (numeric pass).
+    auto write_rows = [&](uint64_t dst0, uint64_t utotal) {
+      // Each product rounded on its own, never fused with the addition that
+      // follows (as in the window sort).  The first product of every row is
+      // fetched for all of a thread's rows before any is summed.
+#pragma clang fp contract(off)
+      const uint32_t total = s_off[c];
+      const uint32_t *krow = s_row[cur], *s_head = s_row[cur ^ 1];
+      const uint16_t *kidx = s_idx[cur];
+      auto product = [&](uint32_t y) {
+        const uint32_t x = kidx[y], i = s_run[x];
+        return s_b[i] * A.av[s_src[i] + (x - s_off[i])];
+      };
+      uint32_t hx[kPieceItems];
+      double sum[kPieceItems];
+      const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
+#pragma unroll
+      for (uint32_t k = 0; k < kPieceItems; ++k) {
+        const uint32_t u = tid + k * kPieceBlock;
+        if (k < nu && u < utotal) {
+          hx[k] = s_head[u];
+          sum[k] = product(hx[k]);
+        }
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kPieceItems; ++k) {
+        const uint32_t u = tid + k * kPieceBlock;
+        if (k < nu && u < utotal) {
+          const uint32_t row = krow[hx[k]];
+          for (uint32_t y = hx[k] + 1; y < total && krow[y] == row; ++y) sum[k] += product(y);
+          const uint64_t dst = dst0 + u;
+          if (dst < A.capacity) {
+            A.ck[dst] = v0 + row + ckey;
+            A.cv[dst] = sum[k];
+          }
+        }
+      }
+    };
+    uint64_t count = 0;
+    if (total_all <= kPieceCap) {
+      count = order_rows();
+      if (NUMERIC) write_rows(s_base, count);
+    } else {
+      // Rows clustered: row sub-ranges of at most kPieceCap products, each
+      // found by halving the remaining range (one row holds at most c <=
+      // kPieceLists products, so the halving ends), taken one after another.
+      if (tid == 0) s_lo = 0;
+      __syncthreads();
+      for (;;) {
+        // Every thread reads the rows done before wave 0 moves them on.
+        const uint64_t done_rows = s_lo;
+        if (done_rows >= w) break;
+        __syncthreads();
+        if (tid < kWave) {
+          uint64_t span = w - done_rows, nxt = 0;
+          for (;;) {
+            uint64_t len = 0;
+            if (tid < c) {  // first A entry of run tid at or past row s_lo + span
+              const uint64_t key = s_key0[tid] + done_rows + span;
+              uint64_t lo = s_src[tid], hi = s_end[tid];
+              while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (A.ak[mid] < key) lo = mid + 1;
+                else hi = mid;
+              }
+              nxt = lo;
+              len = lo - s_src[tid];
+            }
+            const uint64_t tot = __shfl(wave_inclusive_sum64(len), kWave - 1, kWave);
+            if (tot <= kPieceCap || span == 1) break;
+            span = (span + 1) / 2;
+          }
+          const uint64_t len = tid < c ? nxt - s_src[tid] : 0;
+          const uint64_t incl = wave_inclusive_sum64(len);
+          if (tid < c) {
+            s_off[tid] = (uint32_t)(incl - len);
+            s_next[tid] = nxt;
+          }
+          if (tid == kWave - 1) s_off[c] = (uint32_t)incl;
+          if (tid == 0) s_lo = done_rows + span;
+        }
+        __syncthreads();
+        const uint64_t u = order_rows();
+        if (NUMERIC) write_rows(s_base + count, u);
+        count += u;
+        __syncthreads();
+        if (tid < c) s_src[tid] = s_next[tid];
+        __syncthreads();
+      }
+    }
+    if (!NUMERIC && tid == 0) {
+      Q.piece_u[t] = (uint32_t)count;
+      atomicAdd((unsigned long long *)&A.seg_u[s], (unsigned long long)count);
+    }
+    __syncthreads();  // LDS is reused by the next piece
+  }
+}
+
 // ------------------------------------------- 5. single-entry segments: copy
 // One wave per segment (grid-stride over segments): C(:,j) = b * A(:,k).
 __global__ __launch_bounds__(kBlock) void batch_single_kernel(ColumnArgs A, const uint32_t *__restrict__ kind,
@@ -867,6 +1280,7 @@ struct BatchTuning {
   uint32_t small_max = 1024;
   uint32_t window = 1280;
   uint32_t blocks_per_cu = 3;
+  int64_t piece_slots = -1;  // at most this many row pieces (-1: the whole pool)
 };
 
 }  // namespace lnt
@@ -883,6 +1297,11 @@ struct lnt_batch_s {
   lnt::DeviceBuffer ws;    // per-entry tables
   lnt::DeviceBuffer scan_tmp;
   lnt::DeviceBuffer ctrl;  // counters, c_off[count+1], nprod[count]
+  // Row-piece slots (16 bytes each): the pool of eager calls, and the
+  // recorded graph's own (grown by the sizing run of a capture only).
+  lnt::DeviceBuffer pieces, g_pieces;
+  uint64_t piece_cap = 0, g_piece_cap = 0;
+  uint64_t piece_need = 0, g_piece_need = 0;  // slots the last run asked for
   uint64_t *pinned = nullptr;  // host mirror of counters + c_off + nprod
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -910,11 +1329,12 @@ struct BatchPtrs {
   uint64_t *a_lo, *a_base, *colkey, *prod_off, *seg_idx, *seg_first, *seg_u, *out_off, *small_rank, *sp_off, *cur;
   uint32_t *a_len, *einst, *head, *kind, *is_small, *small_list, *np_list, *list3, *wcnt, *woff;
   uint64_t *ctr, *c_off, *nprod;
+  PieceArgs pieces;
 };
 
 size_t ws_bytes(uint64_t na, uint64_t nb) {
   const uint64_t n1 = nb + 1;
-  return 256 * 40 + 16 * na + n1 * (8 + 8 + 8 * 11 + 4 * 10);
+  return 256 * 48 + 16 * na + n1 * (8 + 8 + 8 * 14 + 4 * 12);
 }
 
 void carve(lnt_batch b, BatchPtrs *P) {
@@ -951,6 +1371,17 @@ void carve(lnt_batch b, BatchPtrs *P) {
   P->list3 = c.take<uint32_t>(n1);
   P->wcnt = c.take<uint32_t>(n1);
   P->woff = c.take<uint32_t>(n1);
+  P->pieces.rmin = c.take<uint64_t>(n1);
+  P->pieces.width = c.take<uint64_t>(n1);
+  P->pieces.pbase = c.take<uint64_t>(n1);
+  P->pieces.pq = c.take<uint32_t>(n1);
+  P->pieces.list3w = c.take<uint32_t>(n1);
+  Carve q{(char *)b->pieces.ptr};
+  P->pieces.piece_seg = q.take<uint32_t>(b->piece_cap);
+  P->pieces.piece_u = q.take<uint32_t>(b->piece_cap);
+  P->pieces.piece_ent = q.take<uint64_t>(b->piece_cap);
+  P->pieces.cap = b->tuning.piece_slots < 0 ? b->piece_cap
+                                             : std::min<uint64_t>(b->piece_cap, (uint64_t)b->tuning.piece_slots);
   Carve k{(char *)b->ctrl.ptr};
   P->ctr = k.take<uint64_t>(kCtrWords);
   P->c_off = k.take<uint64_t>(b->count + 1);
@@ -998,8 +1429,15 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   const uint32_t grid = (uint32_t)b->cus * b->tuning.blocks_per_cu;
   const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
   const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
+  const uint32_t piece_grid = (uint32_t)b->cus * 3;
+  batch_piece_plan_kernel<<<grid_for(nb * kWave), kBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, P.pieces.pq, P.pieces.pbase, nb, P.ctr + kCtrPieces, b->scan_tmp, 0));
+  batch_piece_assign_kernel<<<grid_for(nb * kWave), kBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_small_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
-  batch_large_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
+  batch_piece_kernel<false><<<piece_grid, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
+  batch_large_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.pieces.list3w, P.cur, P.wcnt, P.woff, window,
+                                                               P.ctr, kCtrN3w);
   LNT_HIP(hipGetLastError());
   LNT_TRY(exclusive_scan<uint64_t>(ctx, P.seg_u, P.out_off, nb, P.ctr + kCtrNc, b->scan_tmp, 0));
   return LNT_OK;
@@ -1021,10 +1459,13 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
       batch_small_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, row_bits, P.ctr);
     else
       batch_small_kernel<true, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
+    batch_piece_kernel<true><<<(uint32_t)b->cus * 3, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
     if (b->max_h <= (uint64_t(1) << 32))
-      batch_large_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
+      batch_large_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.pieces.list3w, P.cur, P.wcnt, P.woff, window,
+                                                                  P.ctr, kCtrN3w);
     else
-      batch_large_kernel<true, uint64_t><<<grid, kBlock, 0, st>>>(A, P.list3, P.cur, P.wcnt, P.woff, window, P.ctr);
+      batch_large_kernel<true, uint64_t><<<grid, kBlock, 0, st>>>(A, P.pieces.list3w, P.cur, P.wcnt, P.woff, window,
+                                                                  P.ctr, kCtrN3w);
     LNT_HIP(hipGetLastError());
   }
   batch_offsets_kernel<<<grid_for(b->count + 1), kBlock, 0, st>>>(P.b_off, b->count, nb, P.seg_idx, P.out_off,
@@ -1068,11 +1509,34 @@ int check_key_error(lnt_batch b) {
   return LNT_OK;
 }
 
+// Piece slots for a first run: one per large segment (there are at most
+// nb / 2) plus room for longer segments; a run that needs more reports its
+// need, and the pool grows for the next run (pieces beyond the pool go to
+// the windows meanwhile).
+int ensure_pieces(lnt_batch b, uint64_t need) {
+  if (b->piece_cap == 0) b->piece_cap = std::max<uint64_t>(65536, b->nb / 4 + 1);
+  if (need > b->piece_cap) b->piece_cap = need + need / 8 + 1024;
+  return ensure(b->pieces, (size_t)b->piece_cap * 16 + 768);
+}
+
 int prepare(lnt_batch b) {
+  LNT_TRY(ensure_pieces(b, b->piece_need));
   LNT_TRY(ensure(b->ws, ws_bytes(b->na, b->nb)));
   LNT_TRY(ensure(b->scan_tmp, (scan_workspace_words(b->nb + 1) + 64) * 8));
   return LNT_OK;
 }
+
+// While alive, the batch's piece pool is the recorded graph's.
+struct PieceSwap {
+  lnt_batch b;
+  explicit PieceSwap(lnt_batch bb) : b(bb) { swap_all(); }
+  ~PieceSwap() { swap_all(); }
+  void swap_all() {
+    std::swap(b->pieces, b->g_pieces);
+    std::swap(b->piece_cap, b->g_piece_cap);
+    std::swap(b->piece_need, b->g_piece_need);
+  }
+};
 
 struct StreamSwap {
   lnt_context ctx;
@@ -1227,6 +1691,8 @@ int lnt_batch_destroy(lnt_batch b) {
   release(b->ws);
   release(b->scan_tmp);
   release(b->ctrl);
+  release(b->pieces);
+  release(b->g_pieces);
   for (DeviceBuffer *x : {&b->g_alt_keys, &b->g_alt_vals, &b->g_alt2_keys, &b->g_alt2_vals, &b->g_status,
                           &b->g_small, &b->g_hist_part, &b->g_relin_dev, &b->g_aux})
     release(*x);
@@ -1250,6 +1716,8 @@ int lnt_batch_symbolic(lnt_batch b, const int64_t *a_keys, const double *a_vals,
   LNT_TRY(check_key_error(b));
   *nnz_c = (int64_t)b->pinned[kCtrNc];
   b->symbolic_ready = true;
+  // The numeric phase uses the pool as it is; a larger need grows it after.
+  b->piece_need = b->pinned[kCtrPieces];
   return LNT_OK;
 }
 
@@ -1334,14 +1802,16 @@ int lnt_batch_capture(lnt_batch b, const int64_t *a_keys, const double *a_vals, 
   lnt_context ctx = b->ctx;
   b->symbolic_ready = false;
   LNT_HIP(hipSetDevice(ctx->device));
-  LNT_TRY(prepare(b));
-  BatchPtrs P{};
-  carve(b, &P);
   // The old graph goes first: its scratch may be regrown below.
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
+  // The graph's piece pool is its own: eager calls may regrow theirs.
+  PieceSwap piece_swap(b);
+  LNT_TRY(prepare(b));
+  BatchPtrs P{};
+  carve(b, &P);
   // From here to the end of the capture the context sorts in the batch's own
   // scratch (its stream is drained first: queued work may still use the
   // context's buffers, which stay where they are).
@@ -1353,6 +1823,11 @@ int lnt_batch_capture(lnt_batch b, const int64_t *a_keys, const double *a_vals, 
   LNT_TRY(read_back(b));
   LNT_TRY(check_key_error(b));
   if (nnz_c) *nnz_c = (int64_t)b->pinned[kCtrNc];
+  // Room for every piece of these operands in the recorded graph.
+  if (b->pinned[kCtrPieces] > b->piece_cap) {
+    LNT_TRY(ensure_pieces(b, b->pinned[kCtrPieces]));
+    carve(b, &P);
+  }
   // The legacy null stream cannot be captured: record on a stream of our own
   // (blocking, default priority) when the context has none.
   hipStream_t st = ctx->stream;
@@ -1417,6 +1892,7 @@ int lnt_batch_set_param(lnt_batch b, const char *name, int64_t value) {
   else if (n == "small_max" && in(2, kCap / 2)) b->tuning.small_max = (uint32_t)value;
   else if (n == "window" && in(64, 3 * kCap / 4)) b->tuning.window = (uint32_t)value;
   else if (n == "blocks_per_cu" && in(1, 8)) b->tuning.blocks_per_cu = (uint32_t)value;
+  else if (n == "piece_slots" && value >= -1) b->tuning.piece_slots = value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown batch parameter or value out of range: " + n);
   return LNT_OK;
 }
@@ -1428,6 +1904,7 @@ int lnt_batch_get_param(lnt_batch b, const char *name, int64_t *value) {
   else if (n == "small_max") *value = b->tuning.small_max;
   else if (n == "window") *value = b->tuning.window;
   else if (n == "blocks_per_cu") *value = b->tuning.blocks_per_cu;
+  else if (n == "piece_slots") *value = b->tuning.piece_slots;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown batch parameter: " + n);
   return LNT_OK;
 }

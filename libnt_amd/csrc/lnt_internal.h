@@ -350,6 +350,25 @@ __device__ inline uint64_t wave_inclusive_sum64(uint64_t v) {
   return v;
 }
 
+// Exclusive sum over a block of W waves; `scratch` holds W words.
+template <uint32_t W>
+__device__ inline uint64_t block_exclusive_sum_waves(uint64_t v, uint64_t *scratch, uint64_t *total) {
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const uint64_t incl = wave_inclusive_sum64(v);
+  if (lane == kWave - 1) scratch[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < W; ++w) {
+    const uint64_t x = scratch[w];
+    before += w < wave ? x : 0;
+    all += x;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
 // 64-bit variant for a 256-thread block; `scratch` holds 4 words.
 __device__ inline uint64_t block_exclusive_sum64_lds(uint64_t v, uint64_t *scratch,
                                                      uint64_t *total) {

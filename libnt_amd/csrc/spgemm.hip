@@ -626,25 +626,6 @@ constexpr uint64_t kPieceAgg = uint64_t(1) << 62;
 constexpr uint64_t kPieceIncl = uint64_t(2) << 62;
 constexpr uint64_t kPieceValue = kPieceAgg - 1;
 
-// Exclusive sum over a block of W waves; `scratch` holds W words.
-template <uint32_t W>
-__device__ inline uint64_t block_exclusive_sum_waves(uint64_t v, uint64_t *scratch, uint64_t *total) {
-  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint64_t incl = wave_inclusive_sum64(v);
-  if (lane == kWave - 1) scratch[wave] = incl;
-  __syncthreads();
-  uint64_t before = 0, all = 0;
-#pragma unroll
-  for (uint32_t w = 0; w < W; ++w) {
-    const uint64_t x = scratch[w];
-    before += w < wave ? x : 0;
-    all += x;
-  }
-  __syncthreads();
-  *total = all;
-  return before + incl - v;
-}
-
 // Per large segment r: rmin, piece width, and (q << 32 | q * c) for the scan
 // of piece and (piece, B entry) pair offsets.  q = 0 marks a segment for the
 // global sort.  One wave per segment, lane i on B entry i; the number of
@@ -1007,13 +988,17 @@ __global__ __launch_bounds__(kPieceBlock, 6) void piece_kernel(
     count = 0;
     if (tid == 0) s_lo = 0;
     __syncthreads();
-    while (s_lo < w) {
+    for (;;) {
+      // Every thread reads the rows done before wave 0 moves them on.
+      const uint64_t done_rows = s_lo;
+      if (done_rows >= w) break;
+      __syncthreads();
       if (tid < kWave) {
-        uint64_t span = w - s_lo, nxt = 0;
+        uint64_t span = w - done_rows, nxt = 0;
         for (;;) {
           uint64_t len = 0;
           if (tid < c) {  // first A entry of run tid at or past row s_lo + span
-            const uint64_t key = s_key0[tid] + s_lo + span;
+            const uint64_t key = s_key0[tid] + done_rows + span;
             uint64_t lo = s_src[tid], hi = s_end[tid];
             while (lo < hi) {
               const uint64_t mid = (lo + hi) >> 1;
@@ -1034,7 +1019,7 @@ __global__ __launch_bounds__(kPieceBlock, 6) void piece_kernel(
           s_next[tid] = nxt;
         }
         if (tid == kWave - 1) s_off[c] = (uint32_t)incl;
-        if (tid == 0) s_lo = s_lo + span;
+        if (tid == 0) s_lo = done_rows + span;
       }
       __syncthreads();
       const uint64_t u = order_rows();

@@ -148,6 +148,73 @@ def test_batch_tuning_does_not_change_results(ctx):
         assert torch.equal(cv.view(torch.int64), ref[1].view(torch.int64)), params
 
 
+def test_batch_row_pieces_match_windows(ctx):
+    """Large segments of at most 64 B entries go through row pieces; with no
+    piece slots they take the windows, and with a few slots some take each:
+    keys, offsets and value bits equal in every case, eager and replayed."""
+    import torch
+
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch
+
+    insts = W.batched_contractions(160, seed=23)
+    ops = W.generate_operands(ctx, insts)
+    b = Batch(ctx, insts)
+    ref = b.contract(*ops)  # row pieces
+    for slots in (0, 5, 300, -1):
+        b.set_param("piece_slots", slots)
+        ck, cv, off = b.contract(*ops)
+        assert np.array_equal(off, ref[2]), slots
+        assert torch.equal(ck, ref[0]), slots
+        assert torch.equal(cv.view(torch.int64), ref[1].view(torch.int64)), slots
+    b.set_param("piece_slots", 7)
+    n = b.capture(*ops)
+    assert n == ref[0].shape[0]
+    for rep in range(2):
+        gk, gv, goff = b.replay()
+        assert np.array_equal(goff, ref[2])
+        assert torch.equal(gk, ref[0]) and torch.equal(gv.view(torch.int64), ref[1].view(torch.int64)), rep
+
+
+@pytest.mark.parametrize("kind", ["uniform", "gap", "cluster", "equal"])
+def test_batch_row_pieces_shapes(ctx, orc, kind):
+    """Long output columns built to steer the batch's row pieces through each
+    branch (2..64 B entries: pieces; 65 and 96: windows; empty pieces over a
+    gap; a cluster that overflows a piece and splits into row sub-ranges;
+    equal rows in every column), as a batch of three instances: keys and
+    value bits equal to the oracle's sparse accumulator."""
+    import importlib.util
+    import os
+
+    import torch
+
+    from libnt_amd.batch import Batch, Instance
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("lattice_cases", os.path.join(here, "test_lattice_gpu.py"))
+    cases = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(cases)
+    rng = np.random.default_rng(["uniform", "gap", "cluster", "equal"].index(kind) + 60)
+    insts, parts, expect = [], [[], [], [], []], []
+    for _ in range(3):
+        ah, aw, bw, ak, av, bk, bv = cases._piece_case(kind, rng)
+        insts.append(Instance([ah, aw], "ca", [aw, bw], "ag", len(ak), len(bk)))
+        for lst, x in zip(parts, (ak, av, bk, bv)):
+            lst.append(x)
+        expect.append(orc.lattice_mult_by("spa", ak, av, (ah, aw, 1), bk, bv, (aw, bw, 1)))
+    print("batch row pieces", kind, [len(x) for x in parts[0]], flush=True)
+    ops = [torch.from_numpy(np.concatenate(lst)).cuda() for lst in parts]
+    b = Batch(ctx, insts)
+    for slots in (-1, 0):
+        b.set_param("piece_slots", slots)
+        ck, cv, off = b.contract(*ops)
+        for i, (ek, ev) in enumerate(expect):
+            k = host(ck[off[i]:off[i + 1]])
+            v = host(cv[off[i]:off[i + 1]])
+            assert np.array_equal(k, ek), (slots, i)
+            assert np.array_equal(v.view(np.int64), ev.view(np.int64)), (slots, i)
+
+
 def test_batch_graph_replay_matches_eager(ctx):
     import torch
 
