@@ -66,6 +66,7 @@ enum {
   kCtrOverflow,     // C larger than the output capacity
   kCtrN3w,          // large segments taken by the windows
   kCtrPieces,       // piece slots the batch needs
+  kCtrTicketPiece,  // pieces written straight into C take tickets (look-back order)
   kCtrWords = 16
 };
 
@@ -354,13 +355,14 @@ __global__ void batch_classify_kernel(const uint64_t *__restrict__ seg_first, co
                                       const uint32_t *__restrict__ einst, const InstDev *__restrict__ inst,
                                       uint64_t nb, uint32_t small_max, uint64_t *__restrict__ ctr,
                                       uint32_t *__restrict__ kind, uint32_t *__restrict__ is_small,
-                                      uint64_t *__restrict__ seg_u, uint32_t *__restrict__ list3) {
+                                      uint64_t *__restrict__ seg_u, uint32_t *__restrict__ is_large) {
   const uint64_t nseg = ctr[kCtrNseg];
   uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; s < nb; s += (uint64_t)gridDim.x * blockDim.x) {
     if (s >= nseg) {
       kind[s] = 0;
       is_small[s] = 0;
+      is_large[s] = 0;
       seg_u[s] = 0;
       continue;
     }
@@ -373,9 +375,17 @@ __global__ void batch_classify_kernel(const uint64_t *__restrict__ seg_first, co
     else k = 3;
     kind[s] = k;
     is_small[s] = k == 2;
+    is_large[s] = k == 3;
     seg_u[s] = k == 1 ? np : 0;
-    if (k == 3) list3[atomicAdd((unsigned long long *)&ctr[kCtrN3], 1ull)] = (uint32_t)s;
   }
+}
+
+// The large segments in ascending order (their pieces then come in output
+// order).
+__global__ void batch_large_list_kernel(const uint32_t *__restrict__ is_large, const uint64_t *__restrict__ rank,
+                                        uint64_t nb, uint32_t *__restrict__ list3) {
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nb; s += (uint64_t)gridDim.x * blockDim.x)
+    if (is_large[s]) list3[rank[s]] = (uint32_t)s;
 }
 
 // Ordered list of small segments and their product counts (np_list must be
@@ -448,6 +458,38 @@ __device__ inline void load_entry(const ColumnArgs &A, uint64_t e, EntryCursor &
   c.b = NUMERIC ? A.bv[e] : 0.0;
 }
 
+// First x in [lo, hi) with a[x] >= key (hi if none), searched by the G lanes
+// of a group (G a power of two, 2..32) in rounds of G probes: a[x] < key
+// holds below the answer and fails from it on, so one ballot over G evenly
+// spaced x narrows the range (G+1)-fold.  Every lane of the wave calls it
+// with the same G; a group with `active` false returns lo.
+__device__ inline uint64_t group_lower_bound(const uint64_t *__restrict__ a, uint64_t lo, uint64_t hi, uint64_t key,
+                                             bool active, uint32_t G) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), gl = lane & (G - 1), shift = lane - gl;
+  const uint64_t mask = (1ull << G) - 1;
+  if (!active) hi = lo;
+  while (__ballot(hi > lo)) {
+    const uint64_t w = hi - lo;
+    // Probes inside [lo, hi): lo + gl when the range fits the group, else
+    // strictly increasing points.
+    const uint64_t x = w <= G ? lo + gl : lo + ((uint64_t)(gl + 1) * w) / (G + 1);
+    const bool live = w > G || gl < w;
+    const bool below = live && a[x] < key;
+    const uint32_t trues = (uint32_t)__popcll((__ballot(below) >> shift) & mask);
+    const uint64_t x_last = __shfl(x, shift + (trues ? trues - 1 : 0), kWave);
+    const uint64_t x_fail = __shfl(x, shift + (trues < G ? trues : G - 1), kWave);
+    if (w == 0) continue;
+    if (w <= G) {
+      lo += trues;
+      hi = lo;
+    } else {
+      if (trues) lo = x_last + 1;
+      if (trues < G) hi = x_fail;
+    }
+  }
+  return lo;
+}
+
 // ------------------------------------------------ 4/5. small segments, items
 // Work item t = the small segments whose compact product offset lies in
 // [t*G, (t+1)*G): at most G segments and G + small_max - 1 products.  Each
@@ -484,13 +526,18 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
   const uint64_t total = ctr[kCtrSmallProd];
   const uint64_t n_items = (total + G - 1) / G;
   for (;;) {
-    if (tid == 0) {
-      uint64_t item = atomicAdd((unsigned long long *)&ctr[kCtrTicketSmall], 1ull);
-      s_item = item;
-      if (item < n_items) {
-        s_rb = lower_bound_u64(sp_off, 0, n_small, item * G);
-        s_re = lower_bound_u64(sp_off, s_rb, n_small, (item + 1) * G);
+    if (tid < kWave) {
+      // The item's first and last small segments: two searches over the
+      // segments' product offsets by 32 lanes each, in rounds of 32 probes.
+      uint64_t item = 0;
+      if (tid == 0) item = atomicAdd((unsigned long long *)&ctr[kCtrTicketSmall], 1ull);
+      item = __shfl(item, 0, kWave);
+      const uint64_t x = group_lower_bound(sp_off, 0, n_small, (item + (tid >= 32 ? 1 : 0)) * G, item < n_items, 32);
+      if (tid == 0) {
+        s_item = item;
+        s_rb = x;
       }
+      if (tid == 32) s_re = x;
     }
     __syncthreads();
     const uint64_t item = s_item;
@@ -827,41 +874,10 @@ struct PieceArgs {
   uint32_t *pq;                    // its pieces; 0: the windows take it
   uint32_t *piece_seg, *piece_u;   // per piece slot: position in list3 (kNoPiece: unused), distinct rows
   uint64_t *piece_ent;             // per piece slot: first B entry << 8 | entries
+  uint64_t *status;                // per piece slot: look-back word of the direct pass
   uint32_t *list3w;                // the segments the windows take
   uint64_t cap;                    // piece slots
 };
-
-// First x in [lo, hi) with a[x] >= key (hi if none), searched by the G lanes
-// of a group (G a power of two, 2..32) in rounds of G probes: a[x] < key
-// holds below the answer and fails from it on, so one ballot over G evenly
-// spaced x narrows the range (G+1)-fold.  Every lane of the wave calls it
-// with the same G; a group with `active` false returns lo.
-__device__ inline uint64_t group_lower_bound(const uint64_t *__restrict__ a, uint64_t lo, uint64_t hi, uint64_t key,
-                                             bool active, uint32_t G) {
-  const uint32_t lane = threadIdx.x & (kWave - 1), gl = lane & (G - 1), shift = lane - gl;
-  const uint64_t mask = (1ull << G) - 1;
-  if (!active) hi = lo;
-  while (__ballot(hi > lo)) {
-    const uint64_t w = hi - lo;
-    // Probes inside [lo, hi): lo + gl when the range fits the group, else
-    // strictly increasing points.
-    const uint64_t x = w <= G ? lo + gl : lo + ((uint64_t)(gl + 1) * w) / (G + 1);
-    const bool live = w > G || gl < w;
-    const bool below = live && a[x] < key;
-    const uint32_t trues = (uint32_t)__popcll((__ballot(below) >> shift) & mask);
-    const uint64_t x_last = __shfl(x, shift + (trues ? trues - 1 : 0), kWave);
-    const uint64_t x_fail = __shfl(x, shift + (trues < G ? trues : G - 1), kWave);
-    if (w == 0) continue;
-    if (w <= G) {
-      lo += trues;
-      hi = lo;
-    } else {
-      if (trues) lo = x_last + 1;
-      if (trues < G) hi = x_fail;
-    }
-  }
-  return lo;
-}
 
 // Per large segment (one wave each, lane i on B entry i): the row range of
 // its A columns, the piece width and the number of pieces (0: the windows
@@ -939,9 +955,23 @@ __global__ __launch_bounds__(kBlock) void batch_piece_assign_kernel(ColumnArgs A
   }
 }
 
-template <bool NUMERIC>
+// Status word of a piece in the direct pass: [63:62] 1 = its own row count,
+// 2 = the count of every piece up to and including it; [61:0] the count.
+constexpr uint64_t kPieceAgg = uint64_t(1) << 62;
+constexpr uint64_t kPieceIncl = uint64_t(2) << 62;
+constexpr uint64_t kPieceValue = kPieceAgg - 1;
+
+// NUMERIC false: the symbolic pass (row counts).  NUMERIC true: the numeric
+// pass, at offsets from the symbolic pass's counts; with DIRECT (a recorded
+// graph, whose output arrays are known before any count), the symbolic pass
+// skips the pieces and this pass chains the pieces' counts by a decoupled
+// look-back over all pieces in output order (pieces take tickets, so a piece
+// waits only on pieces already running), adding them to the offsets of the
+// other segments; piece_total then records each pieced segment's count.
+template <bool NUMERIC, bool DIRECT>
 __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs A, const uint32_t *__restrict__ list3,
                                                                      PieceArgs Q, uint64_t *__restrict__ ctr) {
+  static_assert(NUMERIC || !DIRECT, "the direct pass writes C");
   // Rows (offsets from the piece's first row) and load positions, in two
   // buffers for the merge rounds, and the B entry of each load position.
   __shared__ uint32_t s_row[2][kPieceCap];
@@ -954,25 +984,41 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
   __shared__ uint64_t s_key0[kPieceLists];  // key of the piece's first row in each column
   __shared__ double s_b[kPieceLists];
   __shared__ uint64_t s_scan[kPieceBlock / kWave];
-  __shared__ uint64_t s_all, s_lo, s_base;
+  __shared__ uint64_t s_all, s_lo, s_base, s_t, s_prefix;
   const uint32_t tid = threadIdx.x;
   const uint64_t nseg = ctr[kCtrNseg];
-  // Pieces in a fixed stride over the persistent grid (no shared ticket:
-  // pieces are alike in size, and one contended counter would serialise
-  // millions of them).
+  // Without DIRECT, pieces go in a fixed stride over the persistent grid (no
+  // shared ticket: pieces are alike in size, and one contended counter costs
+  // a little on every piece).
   const uint64_t npieces = min<uint64_t>(ctr[kCtrPieces], Q.cap);
-  for (uint64_t t = blockIdx.x; t < npieces; t += gridDim.x) {
+  uint64_t t_next = blockIdx.x;
+  for (;;) {
+    uint64_t t;
+    if (DIRECT) {
+      if (tid == 0) s_t = atomicAdd((unsigned long long *)&ctr[kCtrTicketPiece], 1ull);
+      __syncthreads();
+      t = s_t;
+      __syncthreads();
+    } else {
+      t = t_next;
+      t_next += gridDim.x;
+    }
+    if (t >= npieces) break;
     // Loads that do not wait on one another: the piece's segment position
     // and entries, then the segment's plan and the entries' columns.
     const uint32_t r = Q.piece_seg[t];
-    if (r == kNoPiece) continue;
+    if (r == kNoPiece) {  // a slot past the pool's last fitting segment: no rows
+      if (DIRECT && tid == 0) status_store(&Q.status[t], t == 0 ? kPieceIncl : kPieceAgg);
+      continue;
+    }
     const uint64_t ent = Q.piece_ent[t];
     const uint64_t first = ent >> 8;
     const uint32_t c = (uint32_t)(ent & 0xFF);
     const uint64_t s = list3[r];
     const uint64_t q = Q.pq[r], p = t - Q.pbase[r], w = Q.width[r], v0 = Q.rmin[r] + p * w;
     const bool last_piece = p + 1 == q;
-    if (NUMERIC && tid < kWave) {
+    if (DIRECT && tid == 0) s_base = A.out_off[s];  // plus the look-back's prefix
+    if (NUMERIC && !DIRECT && tid < kWave) {
       // Output offset: the segment's, plus the rows of its earlier pieces
       // (issued before the searches, which do not need it).
       uint64_t before = 0;
@@ -1026,7 +1072,7 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
     __syncthreads();
     const uint64_t total_all = s_all;
     const uint64_t ckey = A.colkey[first];
-    auto run_of = [&](uint32_t x) {
+    auto run_of = [&](uint32_t x) __attribute__((always_inline)) {
       uint32_t lo = 0, hi = c;
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -1040,7 +1086,7 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
     // kPieceCap products), orders them by (row, B entry), lists the first
     // position of every distinct row and returns their number; `cur` names
     // the ordered buffer, s_row[cur ^ 1] holds the list.
-    auto order_rows = [&]() -> uint64_t {
+    auto order_rows = [&]() __attribute__((always_inline)) -> uint64_t {
       const uint32_t total = s_off[c];
       for (uint32_t x = tid; x < total; x += kPieceBlock) {
         const uint32_t i = run_of(x);
@@ -1112,8 +1158,12 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
       __syncthreads();
       return utotal;
     };
-    // Writes the ordered rows' sums at dst0 of C (numeric pass).
-    auto write_rows = [&](uint64_t dst0, uint64_t utotal) {
+    // The ordered rows' sums, kept in registers (row u = tid + k *
+    // kPieceBlock in slot k), so that a direct piece gathers them before its
+    // look-back waits.
+    uint32_t hrow[kPieceItems];
+    double hsum[kPieceItems];
+    auto sum_rows = [&](uint64_t utotal) __attribute__((always_inline)) {
       // Each product rounded on its own, never fused with the addition that
       // follows (as in the window sort).  The first product of every row is
       // fetched for all of a thread's rows before any is summed.
@@ -1121,19 +1171,18 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
       const uint32_t total = s_off[c];
       const uint32_t *krow = s_row[cur], *s_head = s_row[cur ^ 1];
       const uint16_t *kidx = s_idx[cur];
-      auto product = [&](uint32_t y) {
+      auto product = [&](uint32_t y) __attribute__((always_inline)) {
         const uint32_t x = kidx[y], i = s_run[x];
         return s_b[i] * A.av[s_src[i] + (x - s_off[i])];
       };
       uint32_t hx[kPieceItems];
-      double sum[kPieceItems];
       const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
 #pragma unroll
       for (uint32_t k = 0; k < kPieceItems; ++k) {
         const uint32_t u = tid + k * kPieceBlock;
         if (k < nu && u < utotal) {
           hx[k] = s_head[u];
-          sum[k] = product(hx[k]);
+          hsum[k] = product(hx[k]);
         }
       }
 #pragma unroll
@@ -1141,20 +1190,76 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
         const uint32_t u = tid + k * kPieceBlock;
         if (k < nu && u < utotal) {
           const uint32_t row = krow[hx[k]];
-          for (uint32_t y = hx[k] + 1; y < total && krow[y] == row; ++y) sum[k] += product(y);
-          const uint64_t dst = dst0 + u;
-          if (dst < A.capacity) {
-            A.ck[dst] = v0 + row + ckey;
-            A.cv[dst] = sum[k];
-          }
+          for (uint32_t y = hx[k] + 1; y < total && krow[y] == row; ++y) hsum[k] += product(y);
+          hrow[k] = row;
         }
       }
+    };
+    // Stores the summed rows at dst0 of C.
+    auto store_rows = [&](uint64_t dst0, uint64_t utotal) __attribute__((always_inline)) {
+      const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
+#pragma unroll
+      for (uint32_t k = 0; k < kPieceItems; ++k) {
+        const uint32_t u = tid + k * kPieceBlock;
+        if (k < nu && u < utotal && dst0 + u < A.capacity) {
+          A.ck[dst0 + u] = v0 + hrow[k] + ckey;
+          A.cv[dst0 + u] = hsum[k];
+        }
+      }
+    };
+    // Direct pass: wave 0 sums its predecessors' words, 64 per round trip,
+    // back to the first inclusive one, into s_prefix.  Only counts travel
+    // between pieces (C is read by later launches), so relaxed atomics
+    // suffice.
+    auto look_back = [&]() __attribute__((always_inline)) {
+      if (tid < kWave) {
+        uint64_t prefix = 0;
+        if (t > 0) {
+          int64_t base = (int64_t)t - 1;
+          for (;;) {
+            const int64_t idx = base - (int64_t)tid;
+            uint64_t wd = 0;
+            bool ready = true, term = true;
+            if (idx >= 0) {
+              wd = status_load(&Q.status[idx]);
+              ready = wd != 0;
+              term = ready && (wd & kPieceIncl);
+            }
+            const uint64_t term_mask = __ballot(term), wait_mask = __ballot(!ready);
+            const uint32_t stop = term_mask ? (uint32_t)__builtin_ctzll(term_mask) : kWave - 1;
+            const uint64_t needed = stop == kWave - 1 ? ~0ull : ((2ull << stop) - 1ull);
+            if (wait_mask & needed) {
+              __builtin_amdgcn_s_sleep(1);
+              continue;
+            }
+            const uint64_t part = tid <= stop ? (wd & kPieceValue) : 0;
+            prefix += __shfl(wave_inclusive_sum64(part), kWave - 1, kWave);
+            if (term_mask) break;
+            base -= kWave;
+          }
+        }
+        if (tid == 0) s_prefix = prefix;
+      }
+      __syncthreads();
     };
     uint64_t count = 0;
     if (total_all <= kPieceCap) {
       count = order_rows();
-      if (NUMERIC) write_rows(s_base, count);
+      if (DIRECT) {
+        if (tid == 0) status_store(&Q.status[t], (t == 0 ? kPieceIncl : kPieceAgg) | count);
+        sum_rows(count);
+        look_back();
+        if (tid == 0 && t > 0) status_store(&Q.status[t], kPieceIncl | (s_prefix + count));
+        store_rows(s_base + s_prefix, count);
+      } else if (NUMERIC) {
+        sum_rows(count);
+        store_rows(s_base, count);
+      }
     } else {
+      // An oversized piece cannot publish its count early: it takes its
+      // prefix first, and its successors wait for its inclusive count.
+      if (DIRECT) look_back();
+      const uint64_t dst = DIRECT ? s_base + s_prefix : s_base;
       // Rows clustered: row sub-ranges of at most kPieceCap products, each
       // found by halving the remaining range (one row holds at most c <=
       // kPieceLists products, so the halving ends), taken one after another.
@@ -1195,18 +1300,37 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
         }
         __syncthreads();
         const uint64_t u = order_rows();
-        if (NUMERIC) write_rows(s_base + count, u);
+        if (NUMERIC) {
+          sum_rows(u);
+          store_rows(dst + count, u);
+        }
         count += u;
         __syncthreads();
         if (tid < c) s_src[tid] = s_next[tid];
         __syncthreads();
       }
     }
+    if (DIRECT && total_all > kPieceCap && tid == 0) status_store(&Q.status[t], kPieceIncl | (s_prefix + count));
     if (!NUMERIC && tid == 0) {
       Q.piece_u[t] = (uint32_t)count;
       atomicAdd((unsigned long long *)&A.seg_u[s], (unsigned long long)count);
     }
     __syncthreads();  // LDS is reused by the next piece
+  }
+}
+
+// Direct pass: each pieced segment's row count, the difference of the
+// inclusive counts that close it and its predecessor (every slot before a
+// fitting segment's first piece holds a piece of an earlier one).
+__global__ void batch_piece_total_kernel(const uint32_t *__restrict__ list3, PieceArgs Q, uint64_t *__restrict__ seg_u,
+                                         const uint64_t *__restrict__ ctr) {
+  const uint64_t n3 = ctr[kCtrN3];
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n3; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = Q.pq[r];
+    if (!q) continue;
+    const uint64_t b0 = Q.pbase[r];
+    const uint64_t hi = Q.status[b0 + q - 1] & kPieceValue, lo = b0 ? Q.status[b0 - 1] & kPieceValue : 0;
+    seg_u[list3[r]] = hi - lo;
   }
 }
 
@@ -1297,7 +1421,7 @@ struct lnt_batch_s {
   lnt::DeviceBuffer ws;    // per-entry tables
   lnt::DeviceBuffer scan_tmp;
   lnt::DeviceBuffer ctrl;  // counters, c_off[count+1], nprod[count]
-  // Row-piece slots (16 bytes each): the pool of eager calls, and the
+  // Row-piece slots (24 bytes each): the pool of eager calls, and the
   // recorded graph's own (grown by the sizing run of a capture only).
   lnt::DeviceBuffer pieces, g_pieces;
   uint64_t piece_cap = 0, g_piece_cap = 0;
@@ -1326,15 +1450,16 @@ struct BatchPtrs {
   uint64_t *a_off, *b_off;
   uint64_t *ka, *kb;
   double *va, *vb;
-  uint64_t *a_lo, *a_base, *colkey, *prod_off, *seg_idx, *seg_first, *seg_u, *out_off, *small_rank, *sp_off, *cur;
-  uint32_t *a_len, *einst, *head, *kind, *is_small, *small_list, *np_list, *list3, *wcnt, *woff;
+  uint64_t *a_lo, *a_base, *colkey, *prod_off, *seg_idx, *seg_first, *seg_u, *out_off, *small_rank, *sp_off, *cur,
+      *large_rank;
+  uint32_t *a_len, *einst, *head, *kind, *is_small, *is_large, *small_list, *np_list, *list3, *wcnt, *woff;
   uint64_t *ctr, *c_off, *nprod;
   PieceArgs pieces;
 };
 
 size_t ws_bytes(uint64_t na, uint64_t nb) {
   const uint64_t n1 = nb + 1;
-  return 256 * 48 + 16 * na + n1 * (8 + 8 + 8 * 14 + 4 * 12);
+  return 256 * 52 + 16 * na + n1 * (8 + 8 + 8 * 15 + 4 * 13);
 }
 
 void carve(lnt_batch b, BatchPtrs *P) {
@@ -1371,6 +1496,8 @@ void carve(lnt_batch b, BatchPtrs *P) {
   P->list3 = c.take<uint32_t>(n1);
   P->wcnt = c.take<uint32_t>(n1);
   P->woff = c.take<uint32_t>(n1);
+  P->large_rank = c.take<uint64_t>(n1);
+  P->is_large = c.take<uint32_t>(n1);
   P->pieces.rmin = c.take<uint64_t>(n1);
   P->pieces.width = c.take<uint64_t>(n1);
   P->pieces.pbase = c.take<uint64_t>(n1);
@@ -1380,6 +1507,7 @@ void carve(lnt_batch b, BatchPtrs *P) {
   P->pieces.piece_seg = q.take<uint32_t>(b->piece_cap);
   P->pieces.piece_u = q.take<uint32_t>(b->piece_cap);
   P->pieces.piece_ent = q.take<uint64_t>(b->piece_cap);
+  P->pieces.status = q.take<uint64_t>(b->piece_cap);
   P->pieces.cap = b->tuning.piece_slots < 0 ? b->piece_cap
                                              : std::min<uint64_t>(b->piece_cap, (uint64_t)b->tuning.piece_slots);
   Carve k{(char *)b->ctrl.ptr};
@@ -1391,7 +1519,7 @@ void carve(lnt_batch b, BatchPtrs *P) {
 // Everything up to and including the column tables and the symbolic pass
 // (when `symbolic`); no host synchronisation.
 int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const double *a_vals,
-                  const int64_t *b_keys, const double *b_vals, bool symbolic) {
+                  const int64_t *b_keys, const double *b_vals, bool symbolic, bool direct = false) {
   lnt_context ctx = b->ctx;
   hipStream_t st = ctx->stream;
   const uint64_t nb = b->nb;
@@ -1416,7 +1544,10 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   batch_seg_first_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.head, P.seg_idx, nb, P.seg_first);
   const uint32_t small_max = std::min<uint32_t>(b->tuning.small_max, kCap / 2);
   batch_classify_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.seg_first, P.prod_off, P.einst, P.inst, nb, small_max,
-                                                         P.ctr, P.kind, P.is_small, P.seg_u, P.list3);
+                                                         P.ctr, P.kind, P.is_small, P.seg_u, P.is_large);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, P.is_large, P.large_rank, nb, P.ctr + kCtrN3, b->scan_tmp, 0));
+  batch_large_list_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.is_large, P.large_rank, nb, P.list3);
   LNT_HIP(hipGetLastError());
   LNT_TRY(exclusive_scan<uint32_t>(ctx, P.is_small, P.small_rank, nb, P.ctr + kCtrNsmall, b->scan_tmp, 0));
   batch_zero_u32_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.np_list, nb);
@@ -1435,7 +1566,7 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   LNT_TRY(exclusive_scan<uint32_t>(ctx, P.pieces.pq, P.pieces.pbase, nb, P.ctr + kCtrPieces, b->scan_tmp, 0));
   batch_piece_assign_kernel<<<grid_for(nb * kWave), kBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_small_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
-  batch_piece_kernel<false><<<piece_grid, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
+  if (!direct) batch_piece_kernel<false, false><<<piece_grid, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_large_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.pieces.list3w, P.cur, P.wcnt, P.woff, window,
                                                                P.ctr, kCtrN3w);
   LNT_HIP(hipGetLastError());
@@ -1443,12 +1574,25 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   return LNT_OK;
 }
 
-int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, uint64_t capacity) {
-  hipStream_t st = b->ctx->stream;
+int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, uint64_t capacity,
+                    bool direct = false) {
+  lnt_context ctx = b->ctx;
+  hipStream_t st = ctx->stream;
   const uint64_t nb = b->nb;
   if (nb) {
     ColumnArgs A{P.ka, P.va, P.vb, P.a_lo, P.a_base, P.colkey, P.prod_off, P.seg_first, P.out_off,
                  P.a_len, P.einst, P.inst, nb, P.seg_u, ck, cv, capacity};
+    if (direct) {
+      // The pieces first, at the other segments' offsets (out_off counts
+      // every segment but the pieced ones), then every segment's offset.
+      if (P.pieces.cap)
+        batch_zero_u32_kernel<<<grid_for(2 * P.pieces.cap), kBlock, 0, st>>>((uint32_t *)P.pieces.status,
+                                                                             2 * P.pieces.cap);
+      batch_piece_kernel<true, true><<<(uint32_t)b->cus * 3, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
+      batch_piece_total_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.list3, P.pieces, P.seg_u, P.ctr);
+      LNT_HIP(hipGetLastError());
+      LNT_TRY(exclusive_scan<uint64_t>(ctx, P.seg_u, P.out_off, nb, P.ctr + kCtrNc, b->scan_tmp, 0));
+    }
     const uint32_t grid = (uint32_t)b->cus * b->tuning.blocks_per_cu;
     const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
     const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
@@ -1459,7 +1603,8 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
       batch_small_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, row_bits, P.ctr);
     else
       batch_small_kernel<true, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
-    batch_piece_kernel<true><<<(uint32_t)b->cus * 3, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
+    if (!direct)
+      batch_piece_kernel<true, false><<<(uint32_t)b->cus * 3, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
     if (b->max_h <= (uint64_t(1) << 32))
       batch_large_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.pieces.list3w, P.cur, P.wcnt, P.woff, window,
                                                                   P.ctr, kCtrN3w);
@@ -1479,15 +1624,19 @@ __global__ void batch_reset_tickets_kernel(uint64_t *__restrict__ ctr) {
   if (threadIdx.x == 0) {
     ctr[kCtrTicketSmall] = 0;
     ctr[kCtrTicketLarge] = 0;
+    ctr[kCtrTicketPiece] = 0;
   }
 }
 
+// The whole pipeline into arrays known in advance (a recorded graph): the
+// row pieces write C directly (one pass instead of a counting and a writing
+// one).
 int enqueue_all(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const double *a_vals, const int64_t *b_keys,
                 const double *b_vals, uint64_t *ck, double *cv, uint64_t capacity) {
-  LNT_TRY(enqueue_front(b, P, a_keys, a_vals, b_keys, b_vals, true));
+  LNT_TRY(enqueue_front(b, P, a_keys, a_vals, b_keys, b_vals, true, true));
   batch_reset_tickets_kernel<<<1, 64, 0, b->ctx->stream>>>(P.ctr);
   LNT_HIP(hipGetLastError());
-  return enqueue_numeric(b, P, ck, cv, capacity);
+  return enqueue_numeric(b, P, ck, cv, capacity, true);
 }
 
 // Copies counters, offsets and products to the host mirror and waits.
@@ -1516,7 +1665,7 @@ int check_key_error(lnt_batch b) {
 int ensure_pieces(lnt_batch b, uint64_t need) {
   if (b->piece_cap == 0) b->piece_cap = std::max<uint64_t>(65536, b->nb / 4 + 1);
   if (need > b->piece_cap) b->piece_cap = need + need / 8 + 1024;
-  return ensure(b->pieces, (size_t)b->piece_cap * 16 + 768);
+  return ensure(b->pieces, (size_t)b->piece_cap * 24 + 1024);
 }
 
 int prepare(lnt_batch b) {
