@@ -1658,12 +1658,12 @@ int check_key_error(lnt_batch b) {
   return LNT_OK;
 }
 
-// Piece slots for a first run: one per large segment (there are at most
-// nb / 2) plus room for longer segments; a run that needs more reports its
-// need, and the pool grows for the next run (pieces beyond the pool go to
-// the windows meanwhile).
+// Piece slots for a first run: one per 16 B entries (a large segment has at
+// least two entries and usually more than one piece); a run that needs more
+// reports its need, and the pool grows for the next run (pieces beyond the
+// pool go to the windows meanwhile).
 int ensure_pieces(lnt_batch b, uint64_t need) {
-  if (b->piece_cap == 0) b->piece_cap = std::max<uint64_t>(65536, b->nb / 4 + 1);
+  if (b->piece_cap == 0) b->piece_cap = std::max<uint64_t>(65536, b->nb / 16 + 1);
   if (need > b->piece_cap) b->piece_cap = need + need / 8 + 1024;
   return ensure(b->pieces, (size_t)b->piece_cap * 24 + 1024);
 }

@@ -56,6 +56,43 @@ __global__ __launch_bounds__(kBlock) void chunk_scan_kernel(const T *__restrict_
   if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total_out = run;
 }
 
+// Scans of up to kOneBlockMax elements in one launch: one 1024-thread
+// workgroup walks the array in rounds of 1024 x kItems elements, carrying the
+// running total (three launches become one for the lattice product's
+// per-column tables).
+constexpr int kOneBlock = 1024;
+constexpr uint64_t kOneBlockMax = (uint64_t)kOneBlock * kItems * 4;
+
+template <typename T>
+__global__ __launch_bounds__(kOneBlock) void one_block_scan_kernel(const T *__restrict__ in, uint64_t n,
+                                                                   uint64_t *__restrict__ out,
+                                                                   uint64_t *__restrict__ total_out) {
+  __shared__ uint64_t s[kOneBlock / kWave];
+  uint64_t carry = 0;
+  for (uint64_t base0 = 0; base0 < n; base0 += (uint64_t)kOneBlock * kItems) {
+    const uint64_t base = base0 + (uint64_t)threadIdx.x * kItems;
+    uint64_t v[kItems];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const uint64_t p = base + i;
+      const uint64_t x = (uint64_t)in[p < n ? p : n - 1];  // unconditional load
+      v[i] = p < n ? x : 0;
+      acc += v[i];
+    }
+    uint64_t total;
+    uint64_t run = block_exclusive_sum_waves<kOneBlock / kWave>(acc, s, &total) + carry;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const uint64_t p = base + i;
+      if (p < n) out[p] = run;
+      run += v[i];
+    }
+    carry += total;
+  }
+  if (total_out && threadIdx.x == 0) *total_out = carry;
+}
+
 }  // namespace
 
 // out[i] = sum_{j<i} in[j] for i < n; *total_dev (optional, device) = sum of all.
@@ -70,6 +107,11 @@ int exclusive_scan(lnt_context ctx, const T *in, uint64_t *out, uint64_t n, uint
   uint64_t chunks = (n + kChunk - 1) / kChunk;
   if (chunks == 1) {
     chunk_scan_kernel<T><<<1, kBlock, 0, ctx->stream>>>(in, n, nullptr, out, total_dev);
+    LNT_HIP(hipGetLastError());
+    return LNT_OK;
+  }
+  if (n <= kOneBlockMax) {
+    one_block_scan_kernel<T><<<1, kOneBlock, 0, ctx->stream>>>(in, n, out, total_dev);
     LNT_HIP(hipGetLastError());
     return LNT_OK;
   }
