@@ -346,34 +346,19 @@ def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True, tune=True)
     nprod = b.nprod(ops[0], ops[2])
     nnz = b.capture(*ops)
     _, _, c_off = b.replay()
-    times = []
-    for _ in range(3):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        b.replay()
-        times.append(time.perf_counter() - t0)
-    t_batched = float(np.median(times))
-    b.close()
-    del b
-    torch.cuda.empty_cache()
-    nnz_i = np.diff(c_off)
-    compulsory = 16.0 * (sum(x.nnz_a + x.nnz_b for x in insts) + int(nnz))
-    res = {
-        "workload": "10,000 contractions: order 3-4, 1-2 contracted modes, dims log-U[8,128], nnz log-U[100,50k]",
-        "products": int(nprod.sum()), "nnz_c": int(nnz),
-        "batched_graph_s": round(t_batched, 4),
-        "batched_gflops": round(2 * nprod.sum() / t_batched / 1e9, 2),
-        "instances_per_s": round(len(insts) / t_batched, 1),
-        "roofline": _roofline(compulsory, t_batched, "whole batched pipeline (graph replay), compulsory bytes "
-                                                     "16*sum(nnzA+nnzB+nnzC)"),
-        "autotuned": tuned["config"],
-        "autotune_subsample_ms": ({"default": round(tuned["default_ms"], 2), "tuned": round(tuned["ms"], 2)}
-                                  if tune else None),
-    }
-    if not per_instance_runs:
-        _log("batched: %s" % res)
-        return res
-    # one lnt_contract_into call per instance, C into one reused buffer
+
+    def replay_s():
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.replay()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times))
+
+    t_batched = replay_s()
+    # Operand descriptors of every instance (the per-instance dispatch and the
+    # per-instance accumulator choices inside the batch read them).
     a_off = packed_offsets([x.nnz_a for x in insts])
     b_off = packed_offsets([x.nnz_b for x in insts])
     ak, av, bk, bv = ops
@@ -382,6 +367,54 @@ def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True, tune=True)
         A = SparseMIA(x.dims_a, ak[a_off[i]:a_off[i + 1]], av[a_off[i]:a_off[i + 1]], ctx=ctx)
         B = SparseMIA(x.dims_b, bk[b_off[i]:b_off[i + 1]], bv[b_off[i]:b_off[i + 1]], ctx=ctx)
         pairs.append((dispatch.operand(A, x.labels_a), dispatch.operand(B, x.labels_b)))
+    model_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnt_amd", "data", "dispatch_model.npy")
+    model = None
+    if os.path.exists(model_path):
+        model = dispatch.Model(ctx, 1)
+        model.load(model_path)
+    # The same recorded graph with each policy's per-instance accumulators
+    # (results are bit-identical; only the time differs).
+    policies = {"sort": {"s": round(t_batched, 4), "dense_instances": 0, "dense_segments": 0}}
+    for name in ("dense", "heuristic", "learned"):
+        if name == "learned" and model is None:
+            continue
+        acc = dispatch.batch_accumulators(pairs, name, model)
+        b.set_accumulators(acc)
+        b.replay()
+        policies[name] = {"s": round(replay_s(), 4),
+                          "dense_instances": int(np.isin(acc, (dispatch.ACC_DENSE, dispatch.ACC_MTT)).sum()),
+                          "dense_segments": b.get_param("dense_segments")}
+    b.set_accumulators(dispatch.batch_accumulators(pairs, "sort"))
+    b.close()
+    del b
+    torch.cuda.empty_cache()
+    nnz_i = np.diff(c_off)
+    compulsory = 16.0 * (sum(x.nnz_a + x.nnz_b for x in insts) + int(nnz))
+    best = min(policies, key=lambda k: policies[k]["s"])
+    t_best = policies[best]["s"]
+    res = {
+        "workload": "10,000 contractions: order 3-4, 1-2 contracted modes, dims log-U[8,128], nnz log-U[100,50k]",
+        "products": int(nprod.sum()), "nnz_c": int(nnz),
+        "batched_graph_s": round(t_batched, 4),
+        "batched_gflops": round(2 * nprod.sum() / t_batched / 1e9, 2),
+        "instances_per_s": round(len(insts) / t_batched, 1),
+        "accumulator_policies": policies,
+        "best_policy": best,
+        "roofline": _roofline(compulsory, t_best, "whole batched pipeline (graph replay, %s accumulators), "
+                                                  "compulsory bytes 16*sum(nnzA+nnzB+nnzC)" % best),
+        "autotuned": tuned["config"],
+        "autotune_subsample_ms": ({"default": round(tuned["default_ms"], 2), "tuned": round(tuned["ms"], 2)}
+                                  if tune else None),
+    }
+    if "learned" in policies:
+        res["learned_batched_s"] = policies["learned"]["s"]
+        res["speedup_learned_batched_vs_batched"] = round(t_batched / policies["learned"]["s"], 3)
+    if not per_instance_runs:
+        if model is not None:
+            model.close()
+        _log("batched: %s" % res)
+        return res
+    # one lnt_contract_into call per instance, C into one reused buffer
     cap = int(max(nnz_i.max(), 1))
     out_k = torch.empty(cap, dtype=torch.int64, device="cuda")
     out_v = torch.empty(cap, dtype=torch.float64, device="cuda")
@@ -411,17 +444,15 @@ def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True, tune=True)
         "fixed_default_per_instance_s": round(t_fixed, 4),
         "speedup_batched_vs_plain_heuristic": round(t_plain / t_batched, 2),
     })
-    model_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnt_amd", "data", "dispatch_model.npy")
-    if os.path.exists(model_path):
-        m = dispatch.Model(ctx, 1)
-        m.load(model_path)
-        m.attach()
+    if model is not None:
+        model.attach()
         t_learned, h_learned = per_instance(dispatch.STRAT_AUTO)
         libnt_amd.check(L.lnt_context_set_model(ctx.handle, None))
-        m.close()
+        model.close()
         res["learned_per_instance_s"] = round(t_learned, 4)
         res["learned_strategy_histogram"] = h_learned.tolist()
         res["speedup_learned_vs_plain_heuristic"] = round(t_plain / t_learned, 3)
+        res["speedup_learned_batched_vs_plain_heuristic"] = round(t_plain / policies["learned"]["s"], 2)
     del out_k, out_v
     if cpu_stride < 1:
         _log("batched: %s" % res)

@@ -99,9 +99,28 @@ int lnt_batch_launch(lnt_batch batch);
  *                    pieces of about 3,072 products merged in LDS; the pool of
  *                    piece slots grows to what the last run needed, and a
  *                    segment that finds no room is processed in windows.
+ *   "dense_min"      smallest segment (products) of a dense instance that
+ *                    takes the dense accumulator (16..2^30; default 256)
  * The sort parameters are the context's (lnt_context_set_param). */
 int lnt_batch_set_param(lnt_batch batch, const char *name, int64_t value);
+/* lnt_batch_get_param also reads "dense_segments": the output columns the
+ * dense accumulator took in the last run (see lnt_batch_set_accumulators). */
 int lnt_batch_get_param(lnt_batch batch, const char *name, int64_t *value);
+
+/* The accumulator of each instance (count entries, LNT_ACC_* of
+ * lnt_dispatch.h), the per-contraction choice of the strategy set inside one
+ * batch.  LNT_ACC_DENSE and LNT_ACC_MTT: the instance's output columns of two
+ * or more B entries and at least "dense_min" products (8 or more per entry)
+ * are summed in a dense accumulator over the lattice rows in LDS -- the
+ * sparse accumulator of csc_times (SparseLatticeBase.h:2224-2351) -- when its
+ * lattice height is at most 8192 (otherwise the choice is ignored); every
+ * other value: the batch's merge and sort paths.  Results are bit-identical
+ * whatever the choice (every row summed in ascending B-row order), except a
+ * row met by more than 2,048 products of one output column, which the sort
+ * paths sum by a tree (within the accumulation-order tolerance); only the
+ * time differs.  Applies to later calls and to replays of a recorded graph.
+ * Default after lnt_batch_create: LNT_ACC_SORT everywhere.  Synchronises. */
+int lnt_batch_set_accumulators(lnt_batch batch, const int32_t *acc);
 
 /* Exact scalar-product count of every instance (sum over k of
  * nnzA(:,k) * nnzB(k,:)); products: host array of count entries.  Runs the

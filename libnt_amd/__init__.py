@@ -86,6 +86,7 @@ SIGNATURES = {
     "lnt_batch_launch": [_vp],
     "lnt_batch_set_param": [_vp, ctypes.c_char_p, _i64],
     "lnt_batch_get_param": [_vp, ctypes.c_char_p, _i64p],
+    "lnt_batch_set_accumulators": [_vp, _i32p],
     "lnt_batch_nprod": [_vp, _vp, _vp, _i64p],
     "lnt_contract": [_vp, _vp, _vp, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64p, _vp, ctypes.POINTER(_int)],
     "lnt_contract_into": [_vp, _vp, _vp, _int, _vp, _vp, _i64, _i64p, _vp, ctypes.POINTER(_int)],

@@ -164,6 +164,16 @@ class Batch:
         check(lib().lnt_batch_get_param(self._h, name.encode(), ctypes.byref(v)))
         return v.value
 
+    def set_accumulators(self, acc):
+        """Accumulator of each instance (``dispatch.ACC_*``): ACC_DENSE and
+        ACC_MTT sum the instance's multi-entry output columns in a dense LDS
+        accumulator over its lattice rows (height <= 8192), any other value
+        keeps the batch's merge and sort paths.  Results do not change."""
+        a = np.ascontiguousarray(acc, dtype=np.int32)
+        if a.shape != (self.count,):
+            raise ValueError("one accumulator per instance is required")
+        check(lib().lnt_batch_set_accumulators(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+
     @staticmethod
     def _dev(x):
         p, mem, _ = _ptr_mem(x)

@@ -26,7 +26,7 @@
 // Steps 4 and 5 run on a persistent grid that takes work items from atomic
 // tickets (a work queue).  Equal rows are summed in ascending B-row order --
 // the order of the sort strategy of spgemm.hip -- except for one row met by
-// more than `window` products of one column, which is summed by a tree.
+// more than kCap products of one column, which is summed by a tree.
 #include <string.h>
 
 #include <algorithm>
@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "../../include/lnt_batch.h"
+#include "../../include/lnt_dispatch.h"
 #include "lnt_internal.h"
 
 namespace lnt {
@@ -51,6 +52,13 @@ constexpr int kPer = kCap / kBlock;  // sorted elements per thread
 constexpr uint32_t kMaxSegsPerItem = 1024 + 1;
 constexpr int kRowBits = 40;         // small items sort (segment, row) in one word
 constexpr uint64_t kRowMask = (uint64_t(1) << kRowBits) - 1;
+// Dense segments: rows per accumulator, threads, products per thread and
+// entries per chunk (batch_dense_kernel).
+constexpr uint32_t kDenseH = 8192;
+constexpr int kDenseBlock = 512;
+constexpr int kDenseU = 4;
+constexpr uint32_t kDenseChunk = kDenseBlock * kDenseU;
+constexpr uint32_t kDenseEnt = 64;
 
 // Counters and totals (uint64 words of the batch's control block).
 enum {
@@ -67,6 +75,8 @@ enum {
   kCtrN3w,          // large segments taken by the windows
   kCtrPieces,       // piece slots the batch needs
   kCtrTicketPiece,  // pieces written straight into C take tickets (look-back order)
+  kCtrNdense,       // dense segments (instances with the dense accumulator)
+  kCtrTicketDense,
   kCtrWords = 16
 };
 
@@ -348,14 +358,17 @@ __device__ inline uint64_t seg_last(const uint64_t *seg_first, uint64_t s, uint6
 }
 
 // -------------------------------------------------------------- 3. classify
-// kind: 0 no products, 1 single B entry, 2 small, 3 large.  Arrays are
+// kind: 0 no products, 1 single B entry, 2 small, 3 large, 4 dense (an
+// instance with the dense accumulator: see batch_dense_kernel).  Arrays are
 // written for all nb slots (zero past the last segment) so that scans over
 // nb elements see only real segments.
 __global__ void batch_classify_kernel(const uint64_t *__restrict__ seg_first, const uint64_t *__restrict__ prod_off,
                                       const uint32_t *__restrict__ einst, const InstDev *__restrict__ inst,
-                                      uint64_t nb, uint32_t small_max, uint64_t *__restrict__ ctr,
+                                      const uint32_t *__restrict__ inst_dense, uint64_t nb, uint32_t small_max,
+                                      uint32_t dense_min, uint64_t *__restrict__ ctr,
                                       uint32_t *__restrict__ kind, uint32_t *__restrict__ is_small,
-                                      uint64_t *__restrict__ seg_u, uint32_t *__restrict__ is_large) {
+                                      uint64_t *__restrict__ seg_u, uint32_t *__restrict__ is_large,
+                                      uint32_t *__restrict__ is_dense) {
   const uint64_t nseg = ctr[kCtrNseg];
   uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; s < nb; s += (uint64_t)gridDim.x * blockDim.x) {
@@ -363,21 +376,33 @@ __global__ void batch_classify_kernel(const uint64_t *__restrict__ seg_first, co
       kind[s] = 0;
       is_small[s] = 0;
       is_large[s] = 0;
+      is_dense[s] = 0;
       seg_u[s] = 0;
       continue;
     }
     const uint64_t first = seg_first[s], last = seg_last(seg_first, s, nseg, nb);
     const uint64_t np = prod_off[last] - prod_off[first];
+    const uint32_t i = einst[first];
+    const uint64_t h = inst[i].h;
     uint32_t k;
     if (np == 0) k = 0;
     else if (last - first == 1) k = 1;
-    else if (np <= small_max && inst[einst[first]].h <= kRowMask) k = 2;
+    else if (inst_dense[i] && h <= kDenseH && np >= dense_min && np >= 8 * (last - first)) k = 4;
+    else if (np <= small_max && h <= kRowMask) k = 2;
     else k = 3;
     kind[s] = k;
     is_small[s] = k == 2;
     is_large[s] = k == 3;
+    is_dense[s] = k == 4;
     seg_u[s] = k == 1 ? np : 0;
   }
+}
+
+// The dense segments, in ascending order.
+__global__ void batch_dense_list_kernel(const uint32_t *__restrict__ is_dense, const uint64_t *__restrict__ rank,
+                                        uint64_t nb, uint32_t *__restrict__ list) {
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nb; s += (uint64_t)gridDim.x * blockDim.x)
+    if (is_dense[s]) list[rank[s]] = (uint32_t)s;
 }
 
 // The large segments in ascending order (their pieces then come in output
@@ -1334,6 +1359,150 @@ __global__ void batch_piece_total_kernel(const uint32_t *__restrict__ list3, Pie
   }
 }
 
+// --------------------------------- 4/5. dense segments (chosen per instance)
+// An instance whose accumulator is dense (lnt_batch_set_accumulators) and
+// whose lattice height h is at most kDenseH sends its segments of two or more
+// entries and at least dense_min products (8 or more per entry on average)
+// here: one workgroup per segment, a presence bitmap of h bits and, in the
+// numeric pass, h doubles of accumulator in LDS, indexed by the row -- the
+// sparse accumulator of csc_times (SparseLatticeBase.h:2224-2351).  The
+// segment is taken in chunks of at most kDenseChunk products over at most
+// kDenseEnt entries: every load of a chunk is issued at once, then its entries
+// are applied one after another with a barrier between them (the rows of one
+// entry are distinct, so its threads never meet on a row).  A row's first
+// product is stored and the later ones added in entry order, which is
+// ascending B-row order: the same chain of additions as the sort paths, so
+// the values are bit-identical to them.  Symbolic: the bitmap's population is
+// the segment's output count.  Numeric: output position o (consecutive
+// threads, consecutive positions) finds its bitmap word by a search over the
+// words' ranks and its row by selecting the bit.
+
+// Position of the k-th (0-based) set bit of x; x has more than k set bits.
+__host__ __device__ inline uint32_t select_bit(uint32_t x, uint32_t k) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t width = 16; width >= 1; width >>= 1) {
+    const uint32_t lo = x & ((1u << width) - 1u);
+    const uint32_t c = (uint32_t)__builtin_popcount(lo);
+    if (k >= c) {
+      k -= c;
+      x >>= width;
+      pos += width;
+    } else {
+      x = lo;
+    }
+  }
+  return pos;
+}
+
+template <bool NUMERIC>
+__global__ __launch_bounds__(kDenseBlock) void batch_dense_kernel(ColumnArgs A, const uint32_t *__restrict__ dense_list,
+                                                                  uint64_t *__restrict__ ctr) {
+  __shared__ double s_acc[NUMERIC ? kDenseH : 1];
+  __shared__ uint32_t s_bits[kDenseH / 32];
+  __shared__ uint32_t s_wrank[NUMERIC ? kDenseH / 32 : 1];
+  __shared__ uint64_t s_poff[kDenseEnt + 1];
+  __shared__ uint64_t s_alo[kDenseEnt], s_abase[kDenseEnt];
+  __shared__ double s_b[NUMERIC ? kDenseEnt : 1];
+  __shared__ uint32_t s_scan[kDenseBlock / kWave];
+  __shared__ uint64_t s_item;
+  static_assert(kDenseH / 32 <= (uint32_t)kDenseBlock, "one bitmap word per thread");
+  const uint32_t tid = threadIdx.x;
+  const uint64_t nseg = ctr[kCtrNseg], nd = ctr[kCtrNdense];
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd((unsigned long long *)&ctr[kCtrTicketDense], 1ull);
+    __syncthreads();
+    const uint64_t item = s_item;
+    if (item >= nd) break;
+    const uint64_t s = dense_list[item];
+    const uint64_t first = A.seg_first[s], last = seg_last(A.seg_first, s, nseg, A.nb);
+    const uint32_t words = (uint32_t)((A.inst[A.einst[first]].h + 31) / 32);
+    if (tid < words) s_bits[tid] = 0;
+    const uint64_t pend = A.prod_off[last];
+    uint64_t p0 = A.prod_off[first];
+    uint64_t e0 = first;  // the entry holding product p0, or an empty one before it
+    while (p0 < pend) {
+      // The previous chunk's readers of the entry tables are done (and the
+      // bitmap is clear before the first chunk).
+      __syncthreads();
+      const uint32_t ne = (uint32_t)min<uint64_t>(kDenseEnt, last - e0);
+      for (uint32_t i = tid; i <= ne; i += kDenseBlock) {
+        const uint64_t e = e0 + i;  // <= last: prod_off has nb + 1 entries
+        s_poff[i] = A.prod_off[e];
+        if (i < ne) {
+          s_alo[i] = A.a_lo[e];
+          s_abase[i] = A.a_base[e];
+          if (NUMERIC) s_b[i] = A.bv[e];
+        }
+      }
+      __syncthreads();
+      const uint64_t p1 = min<uint64_t>(p0 + kDenseChunk, s_poff[ne]);
+      uint32_t row[kDenseU], ent[kDenseU];
+      double val[kDenseU];
+#pragma unroll
+      for (int u = 0; u < kDenseU; ++u) {
+        const uint64_t p = p0 + tid + (uint64_t)u * kDenseBlock;
+        ent[u] = ~0u;
+        row[u] = 0;
+        val[u] = 0.0;
+        if (p < p1) {
+          // The entry of p: the last one whose products start at or before p
+          // (empty entries share their successor's start).
+          const uint32_t i = (uint32_t)upper_index(s_poff, 0, ne, p);
+          const uint64_t ai = s_alo[i] + (p - s_poff[i]);
+          row[u] = (uint32_t)(A.ak[ai] - s_abase[i]);
+          if (NUMERIC) val[u] = s_b[i] * A.av[ai];
+          ent[u] = i;
+        }
+      }
+      if (!NUMERIC) {
+#pragma unroll
+        for (int u = 0; u < kDenseU; ++u)
+          if (ent[u] != ~0u) atomicOr(&s_bits[row[u] >> 5], 1u << (row[u] & 31));
+      } else if (p1 > p0) {  // (a run of 64 empty entries holds no product)
+        const uint32_t i_lo = (uint32_t)upper_index(s_poff, 0, ne, p0);
+        const uint32_t i_hi = (uint32_t)upper_index(s_poff, 0, ne, p1 - 1);
+        for (uint32_t r = i_lo; r <= i_hi; ++r) {
+#pragma unroll
+          for (int u = 0; u < kDenseU; ++u) {
+            if (ent[u] == r) {
+              const uint32_t bit = 1u << (row[u] & 31);
+              const uint32_t old = atomicOr(&s_bits[row[u] >> 5], bit);
+              if (old & bit) s_acc[row[u]] += val[u];
+              else s_acc[row[u]] = val[u];
+            }
+          }
+          __syncthreads();
+        }
+      }
+      e0 += p1 < s_poff[ne] ? upper_index(s_poff, 0, ne, p1) : ne;
+      p0 = p1;
+    }
+    __syncthreads();
+    const uint32_t wv = tid < words ? s_bits[tid] : 0u;
+    uint32_t total;
+    const uint32_t rank = block_exclusive_sum_bt<kDenseBlock>((uint32_t)__popc(wv), s_scan, &total);
+    if (!NUMERIC) {
+      if (tid == 0) A.seg_u[s] = total;
+    } else {
+      if (tid < words) s_wrank[tid] = rank;
+      __syncthreads();
+      const uint64_t out0 = A.out_off[s], ck0 = A.colkey[first];
+      for (uint32_t o = tid; o < total; o += kDenseBlock) {
+        const uint32_t w = (uint32_t)upper_index(s_wrank, 0, words, o);
+        const uint32_t r = w * 32 + select_bit(s_bits[w], o - s_wrank[w]);
+        const uint64_t dst = out0 + o;
+        if (dst < A.capacity) {
+          A.ck[dst] = ck0 + r;
+          A.cv[dst] = s_acc[r];
+        }
+      }
+    }
+    // Every thread is done with this segment's LDS before the next ticket.
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------- 5. single-entry segments: copy
 // One wave per segment (grid-stride over segments): C(:,j) = b * A(:,k).
 __global__ __launch_bounds__(kBlock) void batch_single_kernel(ColumnArgs A, const uint32_t *__restrict__ kind,
@@ -1405,6 +1574,7 @@ struct BatchTuning {
   uint32_t window = 1280;
   uint32_t blocks_per_cu = 3;
   int64_t piece_slots = -1;  // at most this many row pieces (-1: the whole pool)
+  uint32_t dense_min = 256;  // smallest segment of a dense instance taken densely
 };
 
 }  // namespace lnt
@@ -1446,25 +1616,29 @@ namespace {
 
 struct BatchPtrs {
   InstDev *inst;
+  uint32_t *inst_dense;  // per instance: 1 = dense accumulator
   RelinDev *relin_a, *relin_b;
   uint64_t *a_off, *b_off;
   uint64_t *ka, *kb;
   double *va, *vb;
   uint64_t *a_lo, *a_base, *colkey, *prod_off, *seg_idx, *seg_first, *seg_u, *out_off, *small_rank, *sp_off, *cur,
       *large_rank;
-  uint32_t *a_len, *einst, *head, *kind, *is_small, *is_large, *small_list, *np_list, *list3, *wcnt, *woff;
+  uint32_t *a_len, *einst, *head, *kind, *is_small, *is_large, *small_list, *np_list, *list3, *wcnt, *woff,
+      *is_dense, *dense_list;
+  uint64_t *dense_rank;
   uint64_t *ctr, *c_off, *nprod;
   PieceArgs pieces;
 };
 
 size_t ws_bytes(uint64_t na, uint64_t nb) {
   const uint64_t n1 = nb + 1;
-  return 256 * 52 + 16 * na + n1 * (8 + 8 + 8 * 15 + 4 * 13);
+  return 256 * 56 + 16 * na + n1 * (8 + 8 + 8 * 16 + 4 * 15);
 }
 
 void carve(lnt_batch b, BatchPtrs *P) {
   Carve m{(char *)b->meta.ptr};
   P->inst = m.take<InstDev>(b->count);
+  P->inst_dense = m.take<uint32_t>(b->count);
   P->relin_a = m.take<RelinDev>(b->count);
   P->relin_b = m.take<RelinDev>(b->count);
   P->a_off = m.take<uint64_t>(b->count + 1);
@@ -1498,6 +1672,9 @@ void carve(lnt_batch b, BatchPtrs *P) {
   P->woff = c.take<uint32_t>(n1);
   P->large_rank = c.take<uint64_t>(n1);
   P->is_large = c.take<uint32_t>(n1);
+  P->is_dense = c.take<uint32_t>(n1);
+  P->dense_list = c.take<uint32_t>(n1);
+  P->dense_rank = c.take<uint64_t>(n1);
   P->pieces.rmin = c.take<uint64_t>(n1);
   P->pieces.width = c.take<uint64_t>(n1);
   P->pieces.pbase = c.take<uint64_t>(n1);
@@ -1543,8 +1720,12 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   if (!symbolic) return LNT_OK;
   batch_seg_first_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.head, P.seg_idx, nb, P.seg_first);
   const uint32_t small_max = std::min<uint32_t>(b->tuning.small_max, kCap / 2);
-  batch_classify_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.seg_first, P.prod_off, P.einst, P.inst, nb, small_max,
-                                                         P.ctr, P.kind, P.is_small, P.seg_u, P.is_large);
+  batch_classify_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.seg_first, P.prod_off, P.einst, P.inst, P.inst_dense,
+                                                         nb, small_max, b->tuning.dense_min, P.ctr, P.kind,
+                                                         P.is_small, P.seg_u, P.is_large, P.is_dense);
+  LNT_HIP(hipGetLastError());
+  LNT_TRY(exclusive_scan<uint32_t>(ctx, P.is_dense, P.dense_rank, nb, P.ctr + kCtrNdense, b->scan_tmp, 0));
+  batch_dense_list_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.is_dense, P.dense_rank, nb, P.dense_list);
   LNT_HIP(hipGetLastError());
   LNT_TRY(exclusive_scan<uint32_t>(ctx, P.is_large, P.large_rank, nb, P.ctr + kCtrN3, b->scan_tmp, 0));
   batch_large_list_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.is_large, P.large_rank, nb, P.list3);
@@ -1566,6 +1747,7 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   LNT_TRY(exclusive_scan<uint32_t>(ctx, P.pieces.pq, P.pieces.pbase, nb, P.ctr + kCtrPieces, b->scan_tmp, 0));
   batch_piece_assign_kernel<<<grid_for(nb * kWave), kBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_small_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
+  batch_dense_kernel<false><<<(uint32_t)b->cus * 4, kDenseBlock, 0, st>>>(A, P.dense_list, P.ctr);
   if (!direct) batch_piece_kernel<false, false><<<piece_grid, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_large_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.pieces.list3w, P.cur, P.wcnt, P.woff, window,
                                                                P.ctr, kCtrN3w);
@@ -1603,6 +1785,7 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
       batch_small_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, row_bits, P.ctr);
     else
       batch_small_kernel<true, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
+    batch_dense_kernel<true><<<(uint32_t)b->cus * 2, kDenseBlock, 0, st>>>(A, P.dense_list, P.ctr);
     if (!direct)
       batch_piece_kernel<true, false><<<(uint32_t)b->cus * 3, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
     if (b->max_h <= (uint64_t(1) << 32))
@@ -1625,6 +1808,7 @@ __global__ void batch_reset_tickets_kernel(uint64_t *__restrict__ ctr) {
     ctr[kCtrTicketSmall] = 0;
     ctr[kCtrTicketLarge] = 0;
     ctr[kCtrTicketPiece] = 0;
+    ctr[kCtrTicketDense] = 0;
   }
 }
 
@@ -1802,8 +1986,9 @@ int lnt_batch_create(lnt_context ctx, const lnt_contraction *items, int64_t coun
     lnt_batch_destroy(b);
     return rc;
   };
-  const size_t meta = (((count * sizeof(InstDev) + 255) / 256) + 2 * ((count * sizeof(RelinDev) + 255) / 256) +
-                       2 * (((count + 1) * 8 + 255) / 256)) * 256 + 256;
+  const size_t meta = (((count * sizeof(InstDev) + 255) / 256) + ((count * 4 + 255) / 256) +
+                       2 * ((count * sizeof(RelinDev) + 255) / 256) + 2 * (((count + 1) * 8 + 255) / 256)) * 256 +
+                       256;
   int rc = ensure(b->meta, meta);
   if (rc != LNT_OK) return bail(rc);
   const size_t ctrl = (((kCtrWords * 8 + 255) / 256) + (((count + 1) * 8 + 255) / 256) + ((count * 8 + 255) / 256)) * 256 + 256;
@@ -1822,6 +2007,10 @@ int lnt_batch_create(lnt_context ctx, const lnt_contraction *items, int64_t coun
     return LNT_OK;
   };
   if ((rc = up(P.inst, inst.data(), count * sizeof(InstDev))) != LNT_OK) return bail(rc);
+  {
+    const std::vector<uint32_t> none(count, 0u);  // every instance on the batch's own paths
+    if ((rc = up(P.inst_dense, none.data(), count * 4)) != LNT_OK) return bail(rc);
+  }
   if ((rc = up(P.relin_a, ra.data(), count * sizeof(RelinDev))) != LNT_OK) return bail(rc);
   if ((rc = up(P.relin_b, rb.data(), count * sizeof(RelinDev))) != LNT_OK) return bail(rc);
   if ((rc = up(P.a_off, a_off.data(), (count + 1) * 8)) != LNT_OK) return bail(rc);
@@ -2042,7 +2231,27 @@ int lnt_batch_set_param(lnt_batch b, const char *name, int64_t value) {
   else if (n == "window" && in(64, 3 * kCap / 4)) b->tuning.window = (uint32_t)value;
   else if (n == "blocks_per_cu" && in(1, 8)) b->tuning.blocks_per_cu = (uint32_t)value;
   else if (n == "piece_slots" && value >= -1) b->tuning.piece_slots = value;
+  else if (n == "dense_min" && in(16, (int64_t)1 << 30)) b->tuning.dense_min = (uint32_t)value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown batch parameter or value out of range: " + n);
+  return LNT_OK;
+}
+
+int lnt_batch_set_accumulators(lnt_batch b, const int32_t *acc) {
+  if (!b || (b->count && !acc)) return fail(LNT_ERR_INVALID_ARGUMENT, "null argument");
+  std::vector<uint32_t> dense(b->count);
+  for (uint32_t i = 0; i < b->count; ++i) {
+    if (acc[i] < 0 || acc[i] >= LNT_NUM_ACCUMULATORS)
+      return fail(LNT_ERR_INVALID_ARGUMENT, "instance " + std::to_string(i) + ": unknown accumulator " +
+                                                std::to_string(acc[i]));
+    dense[i] = acc[i] == LNT_ACC_DENSE || acc[i] == LNT_ACC_MTT;
+  }
+  if (!b->count) return LNT_OK;
+  LNT_HIP(hipSetDevice(b->ctx->device));
+  BatchPtrs P{};
+  carve(b, &P);
+  // Stream ordered: calls already queued keep the choices they were made with.
+  LNT_HIP(hipMemcpyAsync(P.inst_dense, dense.data(), (size_t)b->count * 4, hipMemcpyHostToDevice, b->ctx->stream));
+  LNT_HIP(hipStreamSynchronize(b->ctx->stream));
   return LNT_OK;
 }
 
@@ -2054,6 +2263,8 @@ int lnt_batch_get_param(lnt_batch b, const char *name, int64_t *value) {
   else if (n == "window") *value = b->tuning.window;
   else if (n == "blocks_per_cu") *value = b->tuning.blocks_per_cu;
   else if (n == "piece_slots") *value = b->tuning.piece_slots;
+  else if (n == "dense_min") *value = b->tuning.dense_min;
+  else if (n == "dense_segments") *value = b->pinned ? (int64_t)b->pinned[kCtrNdense] : 0;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown batch parameter: " + n);
   return LNT_OK;
 }

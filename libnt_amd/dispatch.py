@@ -131,6 +131,41 @@ def heuristic(a, la: str, b, lb: str) -> int:
     return s.value
 
 
+BATCH_POLICIES = ("sort", "dense", "heuristic", "learned")
+
+
+def batch_accumulators(pairs, policy: str, model: Optional["Model"] = None) -> np.ndarray:
+    """Per-instance accumulators of a batch (``Batch.set_accumulators``) from
+    the instances' operand descriptors ``pairs`` = [(Operand A, Operand B)]:
+    "sort" the batch's own merge/sort paths everywhere, "dense" the dense
+    accumulator wherever it fits, "heuristic" the accumulator class of the
+    restated polyalgorithm (``lnt_contract_heuristic``), "learned" the
+    accumulator of the model's fastest predicted strategy in the batch's
+    orientation (A*B)."""
+    n = len(pairs)
+    if policy == "sort":
+        return np.full(n, ACC_SORT, dtype=np.int32)
+    if policy == "dense":
+        return np.full(n, ACC_DENSE, dtype=np.int32)
+    if policy == "heuristic":
+        out = np.zeros(n, dtype=np.int32)
+        s = ctypes.c_int()
+        for i, (oa, ob) in enumerate(pairs):
+            check(lib().lnt_contract_heuristic(ctypes.byref(oa), ctypes.byref(ob), ctypes.byref(s)))
+            out[i] = strategy_acc(s.value)
+        return out
+    if policy == "learned":
+        if model is None:
+            raise ValueError("the learned policy needs a model")
+        X = np.zeros((n, NUM_FEATURES), dtype=np.float32)
+        for i, (oa, ob) in enumerate(pairs):
+            check(lib().lnt_contract_features(ctypes.byref(oa), ctypes.byref(ob), _fp(X[i])))
+        pred = model.predict(X) if n else np.zeros((0, NUM_STRATEGIES), dtype=np.float32)
+        ab = [s for s in range(NUM_STRATEGIES) if not s & STRAT_SWAP]
+        return np.array([strategy_acc(ab[int(j)]) for j in np.argmin(pred[:, ab], axis=1)], dtype=np.int32)
+    raise ValueError("unknown policy %r (one of %s)" % (policy, ", ".join(BATCH_POLICIES)))
+
+
 def time_strategies(ctx, a, la: str, b, lb: str, mask: int = ALL_STRATEGIES, reps: int = 1) -> np.ndarray:
     """Wall ms of each strategy in mask (min over reps after a warm-up); -1
     where not measured."""

@@ -215,6 +215,139 @@ def test_batch_row_pieces_shapes(ctx, orc, kind):
             assert np.array_equal(v.view(np.int64), ev.view(np.int64)), (slots, i)
 
 
+@pytest.mark.parametrize("seed", [21, 23])
+def test_batch_accumulator_choices_do_not_change_results(ctx, seed):
+    """The per-instance accumulator choice (lnt_batch_set_accumulators):
+    every instance dense, a seeded random mix of choices and the smallest
+    dense segments allowed give keys, offsets and value bits equal to the
+    batch's own merge/sort paths, eager and in a recorded graph whose
+    replays follow later choices."""
+    import torch
+
+    from libnt_amd import dispatch as D
+    from libnt_amd import workloads as W
+    from libnt_amd.batch import Batch
+
+    insts = W.batched_contractions(160, seed=seed)
+    ops = W.generate_operands(ctx, insts)
+    b = Batch(ctx, insts)
+    ref = b.contract(*ops)
+    assert b.get_param("dense_segments") == 0
+    rng = np.random.default_rng(seed)
+    picks = {
+        "dense": np.full(len(insts), D.ACC_DENSE),
+        "mixed": rng.integers(0, D.NUM_STRATEGIES // 4, len(insts)),
+    }
+    for name, acc in picks.items():
+        for dense_min in (256, 16):
+            print("accumulators", seed, name, dense_min, flush=True)
+            b.set_accumulators(acc)
+            b.set_param("dense_min", dense_min)
+            ck, cv, off = b.contract(*ops)
+            assert b.get_param("dense_segments") > 0, (name, dense_min)
+            assert np.array_equal(off, ref[2]), (name, dense_min)
+            assert torch.equal(ck, ref[0]), (name, dense_min)
+            assert torch.equal(cv.view(torch.int64), ref[1].view(torch.int64)), (name, dense_min)
+    b.set_accumulators(picks["dense"])
+    n = b.capture(*ops)
+    assert n == ref[0].shape[0]
+    for acc in (picks["dense"], picks["mixed"], np.full(len(insts), D.ACC_SORT)):
+        b.set_accumulators(acc)
+        gk, gv, goff = b.replay()
+        assert np.array_equal(goff, ref[2])
+        assert torch.equal(gk, ref[0]) and torch.equal(gv.view(torch.int64), ref[1].view(torch.int64))
+    assert b.get_param("dense_segments") == 0  # the last replay ran with no dense instance
+
+
+def _dense_case(h, rng):
+    """(aw, bw, ak, av, bk, bv) of one tab of height h: A columns empty, short,
+    long (one B entry's products span several chunks of the dense kernel),
+    full and identical; B output columns of 1 .. 299 entries (more than one
+    table of entries per chunk)."""
+    aw, bw = 300, 7
+    rows = []
+    for col in range(aw):
+        kind = col % 6
+        if kind == 0:
+            r = np.zeros(0, dtype=np.int64)
+        elif kind == 1:
+            r = rng.choice(h, min(h, int(rng.integers(1, 5))), replace=False)
+        elif kind == 2:
+            r = rng.choice(h, min(h, 2500), replace=False)
+        elif kind == 3:
+            r = np.arange(h)
+        elif kind == 4:
+            r = np.arange(0, h, 3)
+        else:
+            r = rng.choice(h, max(1, h // 10), replace=False)
+        rows.append(np.sort(r).astype(np.int64) + h * col)
+    ak = np.concatenate(rows).astype(np.int64)
+    av = rng.uniform(-1, 1, len(ak))
+    av[::97] = -0.0  # signed zeros: a row's first product is stored, not added to 0
+    ks, cs = [], []
+    for c, cnt in enumerate([2, 17, 64, 65, 130, 299, 1]):
+        ks += list(np.sort(rng.choice(aw, cnt, replace=False)))
+        cs += [c] * cnt
+    bk = (np.array(ks) + aw * np.array(cs)).astype(np.int64)
+    order = np.argsort(bk)
+    bk = bk[order]
+    bv = rng.uniform(-1, 1, len(bk))
+    bv[::13] = -0.0
+    return aw, bw, ak, av, bk, bv
+
+
+def test_batch_dense_accumulator_shapes(ctx, orc):
+    """The dense accumulator on columns built for its branches (see
+    _dense_case) at heights 33, 8191 and 8192 (the largest it takes) and 8193
+    (too tall: the merge/sort paths), every instance dense and the smallest
+    dense segments allowed: keys and value bits equal to the oracle's sparse
+    accumulator (csc_times, ascending B-row order), eager and replayed."""
+    import torch
+
+    from libnt_amd import dispatch as D
+    from libnt_amd.batch import Batch, Instance
+
+    rng = np.random.default_rng(77)
+    insts, parts, expect = [], [[], [], [], []], []
+    for h in (33, 8191, 8192, 8193):
+        aw, bw, ak, av, bk, bv = _dense_case(h, rng)
+        insts.append(Instance([h, aw], "ca", [aw, bw], "ag", len(ak), len(bk)))
+        for lst, x in zip(parts, (ak, av, bk, bv)):
+            lst.append(x)
+        expect.append(orc.lattice_mult_by("spa", ak, av, (h, aw, 1), bk, bv, (aw, bw, 1)))
+    print("batch dense shapes", [len(x) for x in parts[0]], [len(x) for x in parts[2]], flush=True)
+    ops = [torch.from_numpy(np.concatenate(lst)).cuda() for lst in parts]
+    b = Batch(ctx, insts)
+    b.set_accumulators(np.full(len(insts), D.ACC_DENSE))
+    b.set_param("dense_min", 16)
+
+    def check_all(ck, cv, off):
+        for i, (ek, ev) in enumerate(expect):
+            k = host(ck[off[i]:off[i + 1]])
+            v = host(cv[off[i]:off[i + 1]])
+            assert np.array_equal(k, ek), i
+            assert np.array_equal(v.view(np.int64), ev.view(np.int64)), i
+
+    # Expected dense columns: height <= 8192, two or more entries, at least
+    # dense_min products and 8 per entry.
+    want = 0
+    for x, ak, bk in zip(insts, parts[0], parts[2]):
+        h, aw = x.dims_a
+        if h > 8192:
+            continue
+        alen = np.bincount(ak // h, minlength=aw)
+        for j in np.unique(bk // aw):
+            ks = bk[bk // aw == j] % aw
+            npr = int(alen[ks].sum())
+            want += len(ks) >= 2 and npr >= 16 and npr >= 8 * len(ks)
+    assert want >= 12
+    check_all(*b.contract(*ops))
+    assert b.get_param("dense_segments") == want
+    b.capture(*ops)
+    for _ in range(2):
+        check_all(*b.replay())
+
+
 def test_batch_graph_replay_matches_eager(ctx):
     import torch
 
