@@ -52,12 +52,11 @@ constexpr int kPer = kCap / kBlock;  // sorted elements per thread
 constexpr uint32_t kMaxSegsPerItem = 1024 + 1;
 constexpr int kRowBits = 40;         // small items sort (segment, row) in one word
 constexpr uint64_t kRowMask = (uint64_t(1) << kRowBits) - 1;
-// Dense segments: rows per accumulator, threads, products per thread and
-// entries per chunk (batch_dense_kernel).
+// Dense segments (batch_dense_kernel): the tallest lattice the dense
+// accumulator takes, the height up to which its smaller variant serves, and
+// entries per chunk.
 constexpr uint32_t kDenseH = 8192;
-constexpr int kDenseBlock = 512;
-constexpr int kDenseU = 4;
-constexpr uint32_t kDenseChunk = kDenseBlock * kDenseU;
+constexpr uint32_t kDenseSmallH = 2048;
 constexpr uint32_t kDenseEnt = 64;
 
 // Counters and totals (uint64 words of the batch's control block).
@@ -68,15 +67,16 @@ enum {
   kCtrSmallProd,    // products of small segments
   kCtrNc,           // nnz of all C
   kCtrN3,           // large segments
-  kCtrTicketSmall,
   kCtrTicketLarge,
+  kCtrTicketSmall,
+  kCtrTicketDense,  // symbolic pass, then the numeric pass of the smaller heights
+  kCtrTicketDense2, // numeric pass of the taller ones
   kCtrKeyErr,       // an input key outside its instance's dimensionality
   kCtrOverflow,     // C larger than the output capacity
   kCtrN3w,          // large segments taken by the windows
   kCtrPieces,       // piece slots the batch needs
   kCtrTicketPiece,  // pieces written straight into C take tickets (look-back order)
-  kCtrNdense,       // dense segments (instances with the dense accumulator)
-  kCtrTicketDense,
+  kCtrNdense,       // dense segments: (lattice height <= kDenseSmallH) | (taller) << 32
   kCtrWords = 16
 };
 
@@ -358,17 +358,19 @@ __device__ inline uint64_t seg_last(const uint64_t *seg_first, uint64_t s, uint6
 }
 
 // -------------------------------------------------------------- 3. classify
-// kind: 0 no products, 1 single B entry, 2 small, 3 large, 4 dense (an
-// instance with the dense accumulator: see batch_dense_kernel).  Arrays are
-// written for all nb slots (zero past the last segment) so that scans over
-// nb elements see only real segments.
+// kind: 0 no products, 1 single B entry, 2 small, 3 large, 4 and 5 dense (an
+// instance with the dense accumulator, lattice height up to kDenseSmallH or
+// taller: see batch_dense_kernel; dense_flag packs the two kinds as 1 and
+// 1 << 32, so that one scan ranks both).  Arrays are written for all nb slots
+// (zero past the last segment) so that scans over nb elements see only real
+// segments.
 __global__ void batch_classify_kernel(const uint64_t *__restrict__ seg_first, const uint64_t *__restrict__ prod_off,
                                       const uint32_t *__restrict__ einst, const InstDev *__restrict__ inst,
                                       const uint32_t *__restrict__ inst_dense, uint64_t nb, uint32_t small_max,
                                       uint32_t dense_min, uint64_t *__restrict__ ctr,
                                       uint32_t *__restrict__ kind, uint32_t *__restrict__ is_small,
                                       uint64_t *__restrict__ seg_u, uint32_t *__restrict__ is_large,
-                                      uint32_t *__restrict__ is_dense) {
+                                      uint64_t *__restrict__ dense_flag) {
   const uint64_t nseg = ctr[kCtrNseg];
   uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; s < nb; s += (uint64_t)gridDim.x * blockDim.x) {
@@ -376,7 +378,7 @@ __global__ void batch_classify_kernel(const uint64_t *__restrict__ seg_first, co
       kind[s] = 0;
       is_small[s] = 0;
       is_large[s] = 0;
-      is_dense[s] = 0;
+      dense_flag[s] = 0;
       seg_u[s] = 0;
       continue;
     }
@@ -387,22 +389,51 @@ __global__ void batch_classify_kernel(const uint64_t *__restrict__ seg_first, co
     uint32_t k;
     if (np == 0) k = 0;
     else if (last - first == 1) k = 1;
-    else if (inst_dense[i] && h <= kDenseH && np >= dense_min && np >= 8 * (last - first)) k = 4;
+    else if (inst_dense[i] && h <= kDenseH && np >= dense_min && np >= 8 * (last - first))
+      k = h <= kDenseSmallH ? 4 : 5;
     else if (np <= small_max && h <= kRowMask) k = 2;
     else k = 3;
     kind[s] = k;
     is_small[s] = k == 2;
     is_large[s] = k == 3;
-    is_dense[s] = k == 4;
+    dense_flag[s] = k == 4 ? 1ull : k == 5 ? (1ull << 32) : 0ull;
     seg_u[s] = k == 1 ? np : 0;
   }
 }
 
-// The dense segments, in ascending order.
-__global__ void batch_dense_list_kernel(const uint32_t *__restrict__ is_dense, const uint64_t *__restrict__ rank,
-                                        uint64_t nb, uint32_t *__restrict__ list) {
-  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nb; s += (uint64_t)gridDim.x * blockDim.x)
-    if (is_dense[s]) list[rank[s]] = (uint32_t)s;
+// What a dense segment's workgroup needs first, in one load: its entries
+// [first, first + nent), products [p0, p1), segment id and lattice height.
+struct DenseDesc {
+  uint64_t first, p0, p1;
+  uint32_t seg, nent, h, pad;
+};
+
+// The descriptors of the dense segments: the smaller heights from the front
+// of `desc` (nb + 1 slots), the taller ones from the back, each in ascending
+// segment order.
+__global__ void batch_dense_list_kernel(const uint64_t *__restrict__ dense_flag, const uint64_t *__restrict__ rank,
+                                        const uint64_t *__restrict__ seg_first, const uint64_t *__restrict__ prod_off,
+                                        const uint32_t *__restrict__ einst, const InstDev *__restrict__ inst,
+                                        const uint64_t *__restrict__ ctr, uint64_t nb,
+                                        DenseDesc *__restrict__ desc) {
+  const uint64_t nseg = ctr[kCtrNseg];
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t f = dense_flag[s];
+    if (!f) continue;
+    const uint64_t r = rank[s];
+    const uint64_t idx = (f & 1) ? (r & 0xFFFFFFFFull) : nb - (r >> 32);
+    const uint64_t first = seg_first[s], last = seg_last(seg_first, s, nseg, nb);
+    DenseDesc d;
+    d.first = first;
+    d.p0 = prod_off[first];
+    d.p1 = prod_off[last];
+    d.seg = (uint32_t)s;
+    d.nent = (uint32_t)(last - first);
+    d.h = (uint32_t)inst[einst[first]].h;
+    d.pad = 0;
+    desc[idx] = d;
+  }
 }
 
 // The large segments in ascending order (their pieces then come in output
@@ -438,6 +469,24 @@ __device__ inline void locate(const uint64_t *prod_off, uint64_t first, uint64_t
   *e_out = e;
   *a_idx = a_lo[e] + (base + p - prod_off[e]);
 }
+
+// Work tickets taken K at a time by one thread of a workgroup: every atomic
+// on one address is serialised device-wide (a few ns each), so a counter
+// bumped once per small work item caps the rate at which workgroups can
+// start items; K consecutive items keep the workgroups on neighbouring items
+// (their operands share cache lines) and cut the atomics K-fold.  The
+// workgroup runs its items in order; none waits on another item.
+template <uint32_t K>
+struct TicketBlock {
+  uint64_t cur = 0, end = 0;
+  __device__ uint64_t next(unsigned long long *counter) {
+    if (cur == end) {
+      cur = atomicAdd(counter, (unsigned long long)K);
+      end = cur + K;
+    }
+    return cur++;
+  }
+};
 
 struct ColumnArgs {
   const uint64_t *ak;
@@ -550,12 +599,13 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
   const uint64_t n_small = ctr[kCtrNsmall];
   const uint64_t total = ctr[kCtrSmallProd];
   const uint64_t n_items = (total + G - 1) / G;
+  TicketBlock<4> tickets;
   for (;;) {
     if (tid < kWave) {
       // The item's first and last small segments: two searches over the
       // segments' product offsets by 32 lanes each, in rounds of 32 probes.
       uint64_t item = 0;
-      if (tid == 0) item = atomicAdd((unsigned long long *)&ctr[kCtrTicketSmall], 1ull);
+      if (tid == 0) item = tickets.next((unsigned long long *)&ctr[kCtrTicketSmall]);
       item = __shfl(item, 0, kWave);
       const uint64_t x = group_lower_bound(sp_off, 0, n_small, (item + (tid >= 32 ? 1 : 0)) * G, item < n_items, 32);
       if (tid == 0) {
@@ -1363,19 +1413,32 @@ __global__ void batch_piece_total_kernel(const uint32_t *__restrict__ list3, Pie
 // An instance whose accumulator is dense (lnt_batch_set_accumulators) and
 // whose lattice height h is at most kDenseH sends its segments of two or more
 // entries and at least dense_min products (8 or more per entry on average)
-// here: one workgroup per segment, a presence bitmap of h bits and, in the
+// here: one workgroup per segment, a presence flag byte per row and, in the
 // numeric pass, h doubles of accumulator in LDS, indexed by the row -- the
-// sparse accumulator of csc_times (SparseLatticeBase.h:2224-2351).  The
-// segment is taken in chunks of at most kDenseChunk products over at most
-// kDenseEnt entries: every load of a chunk is issued at once, then its entries
+// sparse accumulator of csc_times (SparseLatticeBase.h:2224-2351).  Flags are
+// bytes, set by plain stores: a bitmap set by LDS atomics serialises the
+// lanes of a wave that hit one word, which dense columns do all the time.  The
+// segment is taken in chunks of at most BT*U products over at most kDenseEnt
+// entries: every load of a chunk is issued at once, then (numeric) its entries
 // are applied one after another with a barrier between them (the rows of one
 // entry are distinct, so its threads never meet on a row).  A row's first
 // product is stored and the later ones added in entry order, which is
 // ascending B-row order: the same chain of additions as the sort paths, so
-// the values are bit-identical to them.  Symbolic: the bitmap's population is
-// the segment's output count.  Numeric: output position o (consecutive
-// threads, consecutive positions) finds its bitmap word by a search over the
-// words' ranks and its row by selecting the bit.
+// the values are bit-identical to them.  Symbolic: the number of flags set is
+// the segment's output count.  Numeric: the flags are packed into a bitmap,
+// and output position o (consecutive threads, consecutive positions) finds
+// its bitmap word by a search over the words' ranks and its row by selecting
+// the bit.
+//
+// The work is latency-bound (a descriptor, the chunk's entry tables and its
+// products are three dependent loads), so the kernels keep many segments in
+// flight: the symbolic pass (flags only) runs 256 threads per segment; the
+// numeric pass has a 16 KB accumulator variant for heights up to
+// kDenseSmallH (eight workgroups per compute unit) and a 64 KB one for the
+// rest (two).  Workgroups take segments from a shared counter a few at a
+// time (TicketBlock).
+//   which = 0: symbolic, every dense segment; 1: numeric, heights up to
+//   kDenseSmallH; 2: numeric, taller (descriptors from the back of `desc`).
 
 // Position of the k-th (0-based) set bit of x; x has more than k set bits.
 __host__ __device__ inline uint32_t select_bit(uint32_t x, uint32_t k) {
@@ -1395,38 +1458,50 @@ __host__ __device__ inline uint32_t select_bit(uint32_t x, uint32_t k) {
   return pos;
 }
 
-template <bool NUMERIC>
-__global__ __launch_bounds__(kDenseBlock) void batch_dense_kernel(ColumnArgs A, const uint32_t *__restrict__ dense_list,
-                                                                  uint64_t *__restrict__ ctr) {
-  __shared__ double s_acc[NUMERIC ? kDenseH : 1];
-  __shared__ uint32_t s_bits[kDenseH / 32];
-  __shared__ uint32_t s_wrank[NUMERIC ? kDenseH / 32 : 1];
+// Four flag bytes (each 0 or 1) of a word -> four bits.
+__device__ inline uint32_t flag_nibble(uint32_t f) { return (f | (f >> 7) | (f >> 14) | (f >> 21)) & 0xFu; }
+
+template <bool NUMERIC, int BT, int U, uint32_t HMAX>
+__global__ __launch_bounds__(BT) void batch_dense_kernel(ColumnArgs A, const DenseDesc *__restrict__ desc, int which,
+                                                         uint64_t *__restrict__ ctr) {
+  constexpr uint32_t kWords = HMAX / 32;
+  constexpr uint32_t kWpt = (kWords + BT - 1) / BT;  // bitmap words per thread
+  constexpr uint32_t kChunk = (uint32_t)BT * U;
+  __shared__ double s_acc[NUMERIC ? HMAX : 1];
+  __shared__ uint32_t s_flag32[HMAX / 4];  // one flag byte per row
+  __shared__ uint32_t s_bits[NUMERIC ? kWords : 1];
+  __shared__ uint32_t s_wrank[NUMERIC ? kWords : 1];
   __shared__ uint64_t s_poff[kDenseEnt + 1];
   __shared__ uint64_t s_alo[kDenseEnt], s_abase[kDenseEnt];
   __shared__ double s_b[NUMERIC ? kDenseEnt : 1];
-  __shared__ uint32_t s_scan[kDenseBlock / kWave];
-  __shared__ uint64_t s_item;
-  static_assert(kDenseH / 32 <= (uint32_t)kDenseBlock, "one bitmap word per thread");
+  __shared__ uint32_t s_scan[BT / kWave];
+  __shared__ uint64_t s_item, s_out0, s_ck0;
   const uint32_t tid = threadIdx.x;
-  const uint64_t nseg = ctr[kCtrNseg], nd = ctr[kCtrNdense];
+  const uint64_t packed = ctr[kCtrNdense];
+  const uint64_t nsm = packed & 0xFFFFFFFFull, nlg = packed >> 32;
+  const uint64_t nitems = which == 0 ? nsm + nlg : which == 1 ? nsm : nlg;
+  TicketBlock<NUMERIC ? 2 : 4> tickets;
+  unsigned long long *const counter =
+      (unsigned long long *)&ctr[which == 2 ? kCtrTicketDense2 : kCtrTicketDense];
   for (;;) {
-    if (tid == 0) s_item = atomicAdd((unsigned long long *)&ctr[kCtrTicketDense], 1ull);
+    if (tid == 0) s_item = tickets.next(counter);
     __syncthreads();
     const uint64_t item = s_item;
-    if (item >= nd) break;
-    const uint64_t s = dense_list[item];
-    const uint64_t first = A.seg_first[s], last = seg_last(A.seg_first, s, nseg, A.nb);
-    const uint32_t words = (uint32_t)((A.inst[A.einst[first]].h + 31) / 32);
-    if (tid < words) s_bits[tid] = 0;
-    const uint64_t pend = A.prod_off[last];
-    uint64_t p0 = A.prod_off[first];
+    if (item >= nitems) break;
+    const uint64_t di = which == 1 || (which == 0 && item < nsm) ? item : A.nb - (which == 0 ? item - nsm : item);
+    const DenseDesc d = desc[di];
+    const uint64_t first = d.first, last = d.first + d.nent;
+    const uint32_t words = (d.h + 31) / 32;
+    uint8_t *const s_flag = reinterpret_cast<uint8_t *>(s_flag32);
+    for (uint32_t w = tid; w < 8 * words; w += BT) s_flag32[w] = 0;
+    uint64_t p0 = d.p0;
     uint64_t e0 = first;  // the entry holding product p0, or an empty one before it
-    while (p0 < pend) {
+    while (p0 < d.p1) {
       // The previous chunk's readers of the entry tables are done (and the
       // bitmap is clear before the first chunk).
       __syncthreads();
       const uint32_t ne = (uint32_t)min<uint64_t>(kDenseEnt, last - e0);
-      for (uint32_t i = tid; i <= ne; i += kDenseBlock) {
+      for (uint32_t i = tid; i <= ne; i += BT) {
         const uint64_t e = e0 + i;  // <= last: prod_off has nb + 1 entries
         s_poff[i] = A.prod_off[e];
         if (i < ne) {
@@ -1435,13 +1510,17 @@ __global__ __launch_bounds__(kDenseBlock) void batch_dense_kernel(ColumnArgs A, 
           if (NUMERIC) s_b[i] = A.bv[e];
         }
       }
+      if (NUMERIC && tid == 0 && e0 == first) {
+        s_out0 = A.out_off[d.seg];
+        s_ck0 = A.colkey[first];
+      }
       __syncthreads();
-      const uint64_t p1 = min<uint64_t>(p0 + kDenseChunk, s_poff[ne]);
-      uint32_t row[kDenseU], ent[kDenseU];
-      double val[kDenseU];
+      const uint64_t p1 = min<uint64_t>(p0 + kChunk, s_poff[ne]);
+      uint32_t row[U], ent[U];
+      double val[U];
 #pragma unroll
-      for (int u = 0; u < kDenseU; ++u) {
-        const uint64_t p = p0 + tid + (uint64_t)u * kDenseBlock;
+      for (int u = 0; u < U; ++u) {
+        const uint64_t p = p0 + tid + (uint64_t)u * BT;
         ent[u] = ~0u;
         row[u] = 0;
         val[u] = 0.0;
@@ -1457,19 +1536,21 @@ __global__ __launch_bounds__(kDenseBlock) void batch_dense_kernel(ColumnArgs A, 
       }
       if (!NUMERIC) {
 #pragma unroll
-        for (int u = 0; u < kDenseU; ++u)
-          if (ent[u] != ~0u) atomicOr(&s_bits[row[u] >> 5], 1u << (row[u] & 31));
-      } else if (p1 > p0) {  // (a run of 64 empty entries holds no product)
+        for (int u = 0; u < U; ++u)
+          if (ent[u] != ~0u) s_flag[row[u]] = 1;
+      } else if (p1 > p0) {  // (a run of kDenseEnt empty entries holds no product)
         const uint32_t i_lo = (uint32_t)upper_index(s_poff, 0, ne, p0);
         const uint32_t i_hi = (uint32_t)upper_index(s_poff, 0, ne, p1 - 1);
         for (uint32_t r = i_lo; r <= i_hi; ++r) {
 #pragma unroll
-          for (int u = 0; u < kDenseU; ++u) {
+          for (int u = 0; u < U; ++u) {
             if (ent[u] == r) {
-              const uint32_t bit = 1u << (row[u] & 31);
-              const uint32_t old = atomicOr(&s_bits[row[u] >> 5], bit);
-              if (old & bit) s_acc[row[u]] += val[u];
-              else s_acc[row[u]] = val[u];
+              if (s_flag[row[u]]) {
+                s_acc[row[u]] += val[u];
+              } else {
+                s_acc[row[u]] = val[u];
+                s_flag[row[u]] = 1;
+              }
             }
           }
           __syncthreads();
@@ -1479,16 +1560,36 @@ __global__ __launch_bounds__(kDenseBlock) void batch_dense_kernel(ColumnArgs A, 
       p0 = p1;
     }
     __syncthreads();
-    const uint32_t wv = tid < words ? s_bits[tid] : 0u;
+    // Bitmap word w = the flags of rows [32w, 32w + 32) (flags past h stay 0).
+    uint32_t cnt = 0, mine[kWpt];
+#pragma unroll
+    for (uint32_t q = 0; q < kWpt; ++q) {
+      const uint32_t w = tid * kWpt + q;
+      uint32_t m = 0;
+      if (w < words) {
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) m |= flag_nibble(s_flag32[8 * w + j]) << (4 * j);
+      }
+      mine[q] = m;
+      cnt += (uint32_t)__popc(m);
+    }
     uint32_t total;
-    const uint32_t rank = block_exclusive_sum_bt<kDenseBlock>((uint32_t)__popc(wv), s_scan, &total);
+    uint32_t rank = block_exclusive_sum_bt<BT>(cnt, s_scan, &total);
     if (!NUMERIC) {
-      if (tid == 0) A.seg_u[s] = total;
+      if (tid == 0) A.seg_u[d.seg] = total;
     } else {
-      if (tid < words) s_wrank[tid] = rank;
+#pragma unroll
+      for (uint32_t q = 0; q < kWpt; ++q) {
+        const uint32_t w = tid * kWpt + q;
+        if (w < words) {
+          s_bits[w] = mine[q];
+          s_wrank[w] = rank;
+          rank += (uint32_t)__popc(mine[q]);
+        }
+      }
       __syncthreads();
-      const uint64_t out0 = A.out_off[s], ck0 = A.colkey[first];
-      for (uint32_t o = tid; o < total; o += kDenseBlock) {
+      const uint64_t out0 = s_out0, ck0 = s_ck0;
+      for (uint32_t o = tid; o < total; o += BT) {
         const uint32_t w = (uint32_t)upper_index(s_wrank, 0, words, o);
         const uint32_t r = w * 32 + select_bit(s_bits[w], o - s_wrank[w]);
         const uint64_t dst = out0 + o;
@@ -1498,7 +1599,7 @@ __global__ __launch_bounds__(kDenseBlock) void batch_dense_kernel(ColumnArgs A, 
         }
       }
     }
-    // Every thread is done with this segment's LDS before the next ticket.
+    // Every thread is done with this segment's LDS before the next one.
     __syncthreads();
   }
 }
@@ -1623,16 +1724,16 @@ struct BatchPtrs {
   double *va, *vb;
   uint64_t *a_lo, *a_base, *colkey, *prod_off, *seg_idx, *seg_first, *seg_u, *out_off, *small_rank, *sp_off, *cur,
       *large_rank;
-  uint32_t *a_len, *einst, *head, *kind, *is_small, *is_large, *small_list, *np_list, *list3, *wcnt, *woff,
-      *is_dense, *dense_list;
-  uint64_t *dense_rank;
+  uint32_t *a_len, *einst, *head, *kind, *is_small, *is_large, *small_list, *np_list, *list3, *wcnt, *woff;
+  uint64_t *dense_flag, *dense_rank;
+  DenseDesc *dense_desc;
   uint64_t *ctr, *c_off, *nprod;
   PieceArgs pieces;
 };
 
 size_t ws_bytes(uint64_t na, uint64_t nb) {
   const uint64_t n1 = nb + 1;
-  return 256 * 56 + 16 * na + n1 * (8 + 8 + 8 * 16 + 4 * 15);
+  return 256 * 56 + 16 * na + n1 * (8 + 8 + 8 * 17 + 4 * 13 + sizeof(DenseDesc));
 }
 
 void carve(lnt_batch b, BatchPtrs *P) {
@@ -1672,9 +1773,9 @@ void carve(lnt_batch b, BatchPtrs *P) {
   P->woff = c.take<uint32_t>(n1);
   P->large_rank = c.take<uint64_t>(n1);
   P->is_large = c.take<uint32_t>(n1);
-  P->is_dense = c.take<uint32_t>(n1);
-  P->dense_list = c.take<uint32_t>(n1);
+  P->dense_flag = c.take<uint64_t>(n1);
   P->dense_rank = c.take<uint64_t>(n1);
+  P->dense_desc = c.take<DenseDesc>(n1);
   P->pieces.rmin = c.take<uint64_t>(n1);
   P->pieces.width = c.take<uint64_t>(n1);
   P->pieces.pbase = c.take<uint64_t>(n1);
@@ -1722,10 +1823,11 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   const uint32_t small_max = std::min<uint32_t>(b->tuning.small_max, kCap / 2);
   batch_classify_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.seg_first, P.prod_off, P.einst, P.inst, P.inst_dense,
                                                          nb, small_max, b->tuning.dense_min, P.ctr, P.kind,
-                                                         P.is_small, P.seg_u, P.is_large, P.is_dense);
+                                                         P.is_small, P.seg_u, P.is_large, P.dense_flag);
   LNT_HIP(hipGetLastError());
-  LNT_TRY(exclusive_scan<uint32_t>(ctx, P.is_dense, P.dense_rank, nb, P.ctr + kCtrNdense, b->scan_tmp, 0));
-  batch_dense_list_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.is_dense, P.dense_rank, nb, P.dense_list);
+  LNT_TRY(exclusive_scan<uint64_t>(ctx, P.dense_flag, P.dense_rank, nb, P.ctr + kCtrNdense, b->scan_tmp, 0));
+  batch_dense_list_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.dense_flag, P.dense_rank, P.seg_first, P.prod_off,
+                                                           P.einst, P.inst, P.ctr, nb, P.dense_desc);
   LNT_HIP(hipGetLastError());
   LNT_TRY(exclusive_scan<uint32_t>(ctx, P.is_large, P.large_rank, nb, P.ctr + kCtrN3, b->scan_tmp, 0));
   batch_large_list_kernel<<<grid_for(nb), kBlock, 0, st>>>(P.is_large, P.large_rank, nb, P.list3);
@@ -1747,7 +1849,7 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   LNT_TRY(exclusive_scan<uint32_t>(ctx, P.pieces.pq, P.pieces.pbase, nb, P.ctr + kCtrPieces, b->scan_tmp, 0));
   batch_piece_assign_kernel<<<grid_for(nb * kWave), kBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_small_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
-  batch_dense_kernel<false><<<(uint32_t)b->cus * 4, kDenseBlock, 0, st>>>(A, P.dense_list, P.ctr);
+  batch_dense_kernel<false, 256, 8, kDenseH><<<(uint32_t)b->cus * 8, 256, 0, st>>>(A, P.dense_desc, 0, P.ctr);
   if (!direct) batch_piece_kernel<false, false><<<piece_grid, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_large_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.pieces.list3w, P.cur, P.wcnt, P.woff, window,
                                                                P.ctr, kCtrN3w);
@@ -1785,7 +1887,8 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
       batch_small_kernel<true, uint32_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, row_bits, P.ctr);
     else
       batch_small_kernel<true, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
-    batch_dense_kernel<true><<<(uint32_t)b->cus * 2, kDenseBlock, 0, st>>>(A, P.dense_list, P.ctr);
+    batch_dense_kernel<true, 256, 8, kDenseSmallH><<<(uint32_t)b->cus * 7, 256, 0, st>>>(A, P.dense_desc, 1, P.ctr);
+    batch_dense_kernel<true, 512, 4, kDenseH><<<(uint32_t)b->cus * 2, 512, 0, st>>>(A, P.dense_desc, 2, P.ctr);
     if (!direct)
       batch_piece_kernel<true, false><<<(uint32_t)b->cus * 3, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
     if (b->max_h <= (uint64_t(1) << 32))
@@ -1805,10 +1908,11 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
 // Tickets of the numeric pass start from zero again.
 __global__ void batch_reset_tickets_kernel(uint64_t *__restrict__ ctr) {
   if (threadIdx.x == 0) {
-    ctr[kCtrTicketSmall] = 0;
     ctr[kCtrTicketLarge] = 0;
     ctr[kCtrTicketPiece] = 0;
+    ctr[kCtrTicketSmall] = 0;
     ctr[kCtrTicketDense] = 0;
+    ctr[kCtrTicketDense2] = 0;
   }
 }
 
@@ -2264,7 +2368,8 @@ int lnt_batch_get_param(lnt_batch b, const char *name, int64_t *value) {
   else if (n == "blocks_per_cu") *value = b->tuning.blocks_per_cu;
   else if (n == "piece_slots") *value = b->tuning.piece_slots;
   else if (n == "dense_min") *value = b->tuning.dense_min;
-  else if (n == "dense_segments") *value = b->pinned ? (int64_t)b->pinned[kCtrNdense] : 0;
+  else if (n == "dense_segments")
+    *value = b->pinned ? (int64_t)((b->pinned[kCtrNdense] & 0xFFFFFFFFull) + (b->pinned[kCtrNdense] >> 32)) : 0;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown batch parameter: " + n);
   return LNT_OK;
 }
