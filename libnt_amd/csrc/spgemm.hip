@@ -882,8 +882,8 @@ __global__ __launch_bounds__(kPieceBlock, 6) void piece_kernel(
     return utotal;
   };
   // The ordered rows' sums, kept in registers (row u = tid + k * kPieceBlock
-  // in slot k) so that they are gathered before the look-back waits.
-  uint32_t hrow[kPieceItems];
+  // in slot k) so that they are gathered before the look-back waits; the rows
+  // themselves stay in LDS (head list, ordered rows) until they are stored.
   double hsum[kPieceItems];
   auto sum_rows = [&](uint64_t utotal) {
     // Each product rounded on its own, never fused with the addition that
@@ -898,34 +898,30 @@ __global__ __launch_bounds__(kPieceBlock, 6) void piece_kernel(
       const uint32_t x = kidx[y], i = s_run[x];
       return s_b[i] * av[s_src[i] + (x - s_off[i])];
     };
-    uint32_t hx[kPieceItems];
     const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
 #pragma unroll
     for (uint32_t k = 0; k < kPieceItems; ++k) {
       const uint32_t u = tid + k * kPieceBlock;
-      if (k < nu && u < utotal) {
-        hx[k] = s_head[u];
-        hsum[k] = product(hx[k]);
-      }
+      if (k < nu && u < utotal) hsum[k] = product(s_head[u]);
     }
 #pragma unroll
     for (uint32_t k = 0; k < kPieceItems; ++k) {
       const uint32_t u = tid + k * kPieceBlock;
       if (k < nu && u < utotal) {
-        const uint32_t row = krow[hx[k]];
-        for (uint32_t y = hx[k] + 1; y < total && krow[y] == row; ++y) hsum[k] += product(y);
-        hrow[k] = row;
+        const uint32_t x = s_head[u], row = krow[x];
+        for (uint32_t y = x + 1; y < total && krow[y] == row; ++y) hsum[k] += product(y);
       }
     }
   };
   // Stores the summed rows at dst0 (their positions in row_out/val_out).
   auto store_rows = [&](uint64_t dst0, uint64_t utotal) {
     const uint32_t nu = (uint32_t)((utotal + kPieceBlock - 1) / kPieceBlock);
+    const uint32_t *krow = s_row[cur], *s_head = s_row[cur ^ 1];
 #pragma unroll
     for (uint32_t k = 0; k < kPieceItems; ++k) {
       const uint32_t u = tid + k * kPieceBlock;
       if (k < nu && u < utotal) {
-        row_out[dst0 + u] = v0 + hrow[k] + key_add;
+        row_out[dst0 + u] = v0 + krow[s_head[u]] + key_add;
         val_out[dst0 + u] = hsum[k];
       }
     }
