@@ -28,32 +28,53 @@ __global__ __launch_bounds__(kBlock) void chunk_sum_kernel(const T *__restrict__
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// Each thread owns kItems consecutive elements of the chunk.
+// Exclusive scan of the BT * kItems elements from `base`: wave w owns the
+// kWave * kItems consecutive elements from base + w * kWave * kItems, as
+// kItems rows of kWave (lane l of row i holds element i * kWave + l), so
+// every load and store is one coalesced row; each row is scanned across the
+// wave with shuffles, carrying the running sum from row to row, and the
+// waves' totals are scanned in LDS.  Returns the block's total; out[p] =
+// `offset` + the sum of the elements before p.
+template <int BT, typename T>
+__device__ inline uint64_t block_rows_scan(const T *__restrict__ in, uint64_t n, uint64_t base, uint64_t offset,
+                                           uint64_t *__restrict__ out, uint64_t *__restrict__ s) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint64_t wbase = base + (uint64_t)wave * kWave * kItems;
+  uint64_t v[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t p = wbase + (uint64_t)i * kWave + lane;
+    const uint64_t x = (uint64_t)in[p < n ? p : n - 1];  // unconditional load
+    v[i] = p < n ? x : 0;
+  }
+  uint64_t run = 0;  // sum of the wave's earlier rows
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t incl = wave_inclusive_sum64(v[i]);
+    const uint64_t row_total = __shfl(incl, kWave - 1, kWave);
+    v[i] = run + incl - v[i];
+    run += row_total;
+  }
+  uint64_t total;
+  const uint64_t before = block_exclusive_sum_waves<BT / kWave>(lane == 0 ? run : 0, s, &total);
+  const uint64_t wave_off = offset + __shfl(before, 0, kWave);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const uint64_t p = wbase + (uint64_t)i * kWave + lane;
+    if (p < n) out[p] = wave_off + v[i];
+  }
+  return total;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void chunk_scan_kernel(const T *__restrict__ in, uint64_t n,
                                                             const uint64_t *__restrict__ offsets,
                                                             uint64_t *__restrict__ out,
                                                             uint64_t *__restrict__ total_out) {
   __shared__ uint64_t s[kBlock / kWave];
-  const uint64_t base = (uint64_t)blockIdx.x * kChunk + (uint64_t)threadIdx.x * kItems;
-  uint64_t v[kItems];
-  uint64_t acc = 0;
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    uint64_t p = base + i;
-    uint64_t x = (uint64_t)in[p < n ? p : n - 1];  // unconditional load
-    v[i] = p < n ? x : 0;
-    acc += v[i];
-  }
-  uint64_t total;
-  uint64_t run = block_exclusive_sum64_lds(acc, s, &total) + (offsets ? offsets[blockIdx.x] : 0);
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    uint64_t p = base + i;
-    if (p < n) out[p] = run;
-    run += v[i];
-  }
-  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total_out = run;
+  const uint64_t offset = offsets ? offsets[blockIdx.x] : 0;
+  const uint64_t total = block_rows_scan<kBlock, T>(in, n, (uint64_t)blockIdx.x * kChunk, offset, out, s);
+  if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = offset + total;
 }
 
 // Scans of up to kOneBlockMax elements in one launch: one 1024-thread
@@ -67,6 +88,8 @@ template <typename T>
 __global__ __launch_bounds__(kOneBlock) void one_block_scan_kernel(const T *__restrict__ in, uint64_t n,
                                                                    uint64_t *__restrict__ out,
                                                                    uint64_t *__restrict__ total_out) {
+  // Each thread owns kItems consecutive elements of a round (one workgroup:
+  // the shuffle-based row scan of chunk_scan_kernel measured slower here).
   __shared__ uint64_t s[kOneBlock / kWave];
   uint64_t carry = 0;
   for (uint64_t base0 = 0; base0 < n; base0 += (uint64_t)kOneBlock * kItems) {
