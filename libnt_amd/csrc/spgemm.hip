@@ -454,9 +454,10 @@ __device__ inline ChunkSegs stage_chunk_from(const uint64_t *__restrict__ off, u
 }
 
 // cs[k] = the segment holding index k * kChunk (k < nchunks), all chunks at
-// once, so that no workgroup walks a long search chain alone.
-__global__ void chunk_starts_kernel(const uint64_t *__restrict__ off, uint64_t nseg, uint64_t nchunks,
-                                    uint64_t *__restrict__ cs) {
+// once, so that no workgroup walks a long search chain alone.  nnz(C) is
+// read on the device (off[nseg]), so the host need not wait for it.
+__global__ void chunk_starts_kernel(const uint64_t *__restrict__ off, uint64_t nseg, uint64_t *__restrict__ cs) {
+  const uint64_t nchunks = (off[nseg] + kChunk - 1) / kChunk;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nchunks; k += (uint64_t)gridDim.x * blockDim.x)
     cs[k] = upper_index(off, 0, nseg, k * kChunk);
 }
@@ -1064,13 +1065,14 @@ __global__ __launch_bounds__(256) void write_kernel(const uint64_t *__restrict__
                              const double *__restrict__ bv, const uint64_t *__restrict__ a_lo,
                              const uint64_t *__restrict__ a_base, const uint64_t *__restrict__ colid,
                              const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first,
-                             uint64_t nseg, uint64_t nb, const uint64_t *__restrict__ out_off, uint64_t nc,
+                             uint64_t nseg, uint64_t nb, const uint64_t *__restrict__ out_off,
                              const uint64_t *__restrict__ tmp_row, const double *__restrict__ tmp_val,
                              uint64_t ah, const uint64_t *__restrict__ cs,
                              const uint8_t *__restrict__ direct, uint64_t *__restrict__ ck,
                              double *__restrict__ cv) {
   __shared__ uint64_t s_off[kChunk + 1];
   __shared__ uint64_t s_seg[2];
+  const uint64_t nc = out_off[nseg];  // nnz(C), read on the device
   const uint64_t nchunks = (nc + kChunk - 1) / kChunk;
   for (uint64_t k = blockIdx.x; k < nchunks; k += gridDim.x) {
     const uint64_t o0 = k * kChunk, o1 = min<uint64_t>(o0 + kChunk, nc);
@@ -1509,8 +1511,13 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     }
   }
   LNT_TRY(exclusive_scan<uint64_t>(ctx, E.seg_u, E.out_off, nseg, E.out_off + nseg, ctx->aux, 0));
-  uint64_t nc;
-  LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
+  // Output arrays of nprod entries hold C whatever its size (nnz(C) <=
+  // nprod): the output launches are queued at once, sized for nprod, and
+  // read nnz(C) on the device; the host reads it after them.  Otherwise
+  // nnz(C) sizes the allocation or is checked against the capacity first.
+  const bool queue_ahead = ck && !ck_alloc && capacity >= nprod;
+  uint64_t nc = nprod;
+  if (!queue_ahead) LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
   if (ck_alloc) {
     void *k = nullptr, *v = nullptr;
     size_t bytes = (nc ? nc : 1) * 8;
@@ -1529,12 +1536,13 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     return fail(LNT_ERR_INVALID_ARGUMENT, "capacity " + std::to_string(capacity) + " is below nnz(C) = " +
                                               std::to_string(nc));
   }
+  // nc: nnz(C), or its bound nprod when the launches are queued ahead.
   const uint64_t nchunks = (nc + kChunk - 1) / kChunk;
   LNT_TRY(ensure(ctx->spg_chunk, (nchunks + 1) * sizeof(uint64_t)));
   uint64_t *cs = (uint64_t *)ctx->spg_chunk.ptr;
-  chunk_starts_kernel<<<grid_for(nchunks, 4096), 256, 0, ctx->stream>>>(E.out_off, nseg, nchunks, cs);
+  chunk_starts_kernel<<<grid_for(nchunks, 4096), 256, 0, ctx->stream>>>(E.out_off, nseg, cs);
   write_kernel<<<chunk_grid(nc), 256, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.colid, E.prod_off, E.seg_first,
-                                                       nseg, nb, E.out_off, nc, tmp_row, tmp_val, ah, cs, E.direct,
+                                                       nseg, nb, E.out_off, tmp_row, tmp_val, ah, cs, E.direct,
                                                        ck, cv);
   const hipError_t launch = hipGetLastError();
   if (launch != hipSuccess) {
@@ -1549,6 +1557,7 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     return fail(LNT_ERR_DEVICE, std::string("write_kernel launch: ") + hipGetErrorString(launch));
   }
   prof_end(ctx, slot);
+  if (queue_ahead) LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
   prof_set_bytes(ctx, slot, 16.0 * ((double)na + nb + nc));
   *nc_host = nc;
   return LNT_OK;
