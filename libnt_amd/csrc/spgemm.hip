@@ -48,6 +48,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kSmall = 2048;  // largest segment handled inside one workgroup's LDS
+constexpr uint32_t kSegGrid = 2048;  // workgroups of the short-segment kernels (they stride)
 
 __device__ inline uint64_t lower_bound_u64(const uint64_t *a, uint64_t n, uint64_t x) {
   uint64_t lo = 0, hi = n;
@@ -181,14 +182,17 @@ __global__ __launch_bounds__(kBlock) void small_sort_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
     const uint64_t *__restrict__ a_lo, const uint64_t *__restrict__ a_base,
     const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg,
-    uint64_t nb, const uint32_t *__restrict__ list2, uint64_t *__restrict__ seg_u,
+    uint64_t nb, const uint32_t *__restrict__ list2, const uint32_t *__restrict__ n2_dev, uint64_t *__restrict__ seg_u,
     uint64_t *__restrict__ tmp_row, double *__restrict__ tmp_val) {
   __shared__ uint64_t s_row[kSmall];
   __shared__ uint16_t s_pos[kSmall];
   __shared__ double s_val[kSmall];
   __shared__ uint64_t s_scan[kBlock / kWave];
   const uint32_t tid = threadIdx.x;
-  const uint64_t s = list2[blockIdx.x];
+  // Segments in a stride over the grid (their count is read on the device).
+  const uint32_t n2 = *n2_dev;
+  for (uint32_t it = blockIdx.x; it < n2; it += gridDim.x) {
+  const uint64_t s = list2[it];
   const uint64_t first = seg_first[s];
   const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
   const uint64_t base = prod_off[first];
@@ -250,6 +254,8 @@ __global__ __launch_bounds__(kBlock) void small_sort_kernel(
     }
   }
   if (tid == 0) seg_u[s] = total;
+  __syncthreads();  // LDS is reused by the next segment
+  }
 }
 
 // 3a'. small segments, "hash" strategy: accumulate products of equal rows in
@@ -263,14 +269,17 @@ __global__ __launch_bounds__(kBlock) void small_hash_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
     const uint64_t *__restrict__ a_lo, const uint64_t *__restrict__ a_base,
     const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg,
-    uint64_t nb, const uint32_t *__restrict__ list2, uint64_t *__restrict__ seg_u,
+    uint64_t nb, const uint32_t *__restrict__ list2, const uint32_t *__restrict__ n2_dev, uint64_t *__restrict__ seg_u,
     uint64_t *__restrict__ tmp_row, double *__restrict__ tmp_val) {
   __shared__ unsigned long long s_key[kHashSlots];
   __shared__ double s_val[kHashSlots];
   __shared__ uint64_t s_scan[kBlock / kWave];
   constexpr unsigned long long kEmpty = ~0ull;
   const uint32_t tid = threadIdx.x;
-  const uint64_t s = list2[blockIdx.x];
+  // Segments in a stride over the grid (their count is read on the device).
+  const uint32_t n2 = *n2_dev;
+  for (uint32_t it = blockIdx.x; it < n2; it += gridDim.x) {
+  const uint64_t s = list2[it];
   const uint64_t first = seg_first[s];
   const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
   const uint64_t base = prod_off[first];
@@ -324,6 +333,8 @@ __global__ __launch_bounds__(kBlock) void small_hash_kernel(
     tmp_val[base + i] = s_val[i];
   }
   if (tid == 0) seg_u[s] = total;
+  __syncthreads();  // LDS is reused by the next segment
+  }
 }
 
 // 3a''. dense accumulation (the sparse accumulator of csc_times, or of
@@ -343,14 +354,17 @@ __global__ __launch_bounds__(kBlock) void dense_kernel(
     const uint64_t *__restrict__ ak, const double *__restrict__ av, const double *__restrict__ bv,
     const uint64_t *__restrict__ a_lo, const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
     const uint64_t *__restrict__ prod_off, const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb,
-    const uint32_t *__restrict__ list2, uint32_t R, const uint32_t *__restrict__ arank,
+    const uint32_t *__restrict__ list2, const uint32_t *__restrict__ n2_dev, uint32_t R, const uint32_t *__restrict__ arank,
     const uint32_t *__restrict__ rank_rows, uint64_t *__restrict__ seg_u, uint64_t *__restrict__ tmp_row,
     double *__restrict__ tmp_val) {
   __shared__ double s_acc[kDenseRows];
   __shared__ uint32_t s_bits[kDenseRows / 32];
   __shared__ uint32_t s_scan[kBlock / kWave];
   const uint32_t tid = threadIdx.x;
-  const uint64_t s = list2[blockIdx.x];
+  // Segments in a stride over the grid (their count is read on the device).
+  const uint32_t n2 = *n2_dev;
+  for (uint32_t it = blockIdx.x; it < n2; it += gridDim.x) {
+  const uint64_t s = list2[it];
   const uint64_t first = seg_first[s];
   const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
   const uint64_t base = prod_off[first];
@@ -394,6 +408,8 @@ __global__ __launch_bounds__(kBlock) void dense_kernel(
     }
   }
   if (tid == 0) seg_u[s] = total;
+  __syncthreads();  // LDS is reused by the next segment
+  }
 }
 
 // 3b. large segments.  np3[r] = products of list3[r].
@@ -636,12 +652,16 @@ __global__ __launch_bounds__(256) void piece_plan_kernel(
     const uint64_t *__restrict__ ak, const uint64_t *__restrict__ a_lo, const uint32_t *__restrict__ a_len,
     const uint64_t *__restrict__ a_base, const uint64_t *__restrict__ prod_off,
     const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb, const uint32_t *__restrict__ list3,
-    const uint64_t *__restrict__ n3_dev, uint64_t *__restrict__ rmin, uint64_t *__restrict__ width,
+    const uint64_t *__restrict__ n3_dev, uint64_t n3_ub, uint64_t *__restrict__ rmin, uint64_t *__restrict__ width,
     uint64_t *__restrict__ qc, uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_count,
     uint64_t *__restrict__ seg_pos) {
   const uint64_t n3 = *n3_dev;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  // qc is zero from n3 up to the host's bound n3_ub: the scans and searches
+  // over qc may run to the bound when the host has not read n3.
+  for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave + n3; r < n3_ub; r += waves)
+    if (lane == 0) qc[r] = 0;
   for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < n3; r += waves) {
     const uint64_t s = list3[r];
     const uint64_t first = seg_first[s];
@@ -750,14 +770,16 @@ __global__ __launch_bounds__(kPieceBlock, 6) void piece_kernel(
   __shared__ uint64_t s_scan[kPieceBlock / kWave];
   __shared__ uint32_t s_t;
   __shared__ uint64_t s_prefix, s_lo;
-  __shared__ uint32_t s_bad;
   const uint32_t tid = threadIdx.x;
+  // The grid may be sized by a bound: workgroups past the number of pieces
+  // leave before taking a ticket (every ticket below it is taken).
+  const uint64_t qtot = off[n3] >> 32;
+  if (blockIdx.x >= qtot) return;
   if (tid == 0) s_t = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint64_t t = s_t;
   // The piece's segment and its places: loads that do not wait on one
   // another, so the start-up latency is a few round trips.
-  const uint64_t qtot = off[n3] >> 32;
   const uint32_t r = piece_seg[t];  // piece_seg holds q_ub >= grid entries
   if (t >= qtot) return;
   const uint64_t s = list3[r], first = seg_pos[2 * r], seg_base = seg_pos[2 * r + 1];
@@ -1035,11 +1057,11 @@ __global__ __launch_bounds__(kPieceBlock, 6) void piece_kernel(
 // Direct mode: each pieced segment's row count (the difference of the
 // inclusive counts that close it and its predecessor), and its mark for the
 // output kernel, which skips it.
-__global__ void piece_total_kernel(const uint32_t *__restrict__ list3, uint32_t n3, const uint64_t *__restrict__ off,
-                                   const uint64_t *__restrict__ status, uint64_t *__restrict__ seg_u,
-                                   uint8_t *__restrict__ direct) {
+__global__ void piece_total_kernel(const uint32_t *__restrict__ list3, const uint64_t *__restrict__ n3_dev,
+                                   const uint64_t *__restrict__ off, const uint64_t *__restrict__ status,
+                                   uint64_t *__restrict__ seg_u, uint8_t *__restrict__ direct) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n3) return;
+  if (r >= *n3_dev) return;
   const uint64_t t0 = off[r] >> 32, t1 = off[r + 1] >> 32;
   if (t1 == t0) return;  // taken by the global sort
   const uint64_t hi = status[t1 - 1] & kPieceValue, lo = t0 ? status[t0 - 1] & kPieceValue : 0;
@@ -1293,11 +1315,21 @@ int lattice_nprod_device(lnt_context ctx, const uint64_t *ak, uint64_t na, const
   return read_u64(ctx, E.prod_off + nb, nprod_host, 1);
 }
 
-int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na,
-                        const int64_t a_dims[3], const uint64_t *bk, const double *bv, uint64_t nb,
-                        const int64_t b_dims[3], int strategy, uint64_t *ck, double *cv,
-                        uint64_t capacity, uint64_t *nc_host, uint64_t *nprod_host,
-                        uint64_t **ck_alloc, double **cv_alloc) {
+namespace {
+
+// lattice_mult_impl's answer when it ran without waiting for the segment
+// lists and found segments the row pieces cannot take: run again, waiting.
+constexpr int kRunAgainWaiting = 1 << 20;
+
+// wait_for_lists: read the number of short, long and fallback segments on
+// the host before launching their kernels (otherwise the launches are sized
+// by bounds and read the numbers on the device, and a fallback segment --
+// more than kPieceLists B entries, or rows too far apart -- found at the end
+// asks for a second run that waits).
+int lattice_mult_impl(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na, const int64_t a_dims[3],
+                      const uint64_t *bk, const double *bv, uint64_t nb, const int64_t b_dims[3], int strategy,
+                      uint64_t *ck, double *cv, uint64_t capacity, uint64_t *nc_host, uint64_t *nprod_host,
+                      uint64_t **ck_alloc, double **cv_alloc, bool wait_for_lists) {
   LNT_TRY(check_lattice(a_dims, b_dims));
   *nc_host = 0;
   if (nprod_host) *nprod_host = 0;
@@ -1342,6 +1374,10 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
   const uint32_t small_max = dense ? 0xFFFFFFFFu
                              : acc == LNT_SPGEMM_ESC ? 0u
                                                      : (uint32_t)std::min(ctx->tuning.spgemm_small, kSmall);
+  // Every long segment has more than small_max products: a bound on their
+  // number the host knows now (none when every multi-entry segment is short).
+  const uint64_t n3_ub =
+      small_max == 0xFFFFFFFFu ? 0 : std::min<uint64_t>(nseg, nprod / ((uint64_t)small_max + 1) + 1);
   seg_first_kernel<<<grid_for(nb), 256, 0, ctx->stream>>>(E.head, E.seg_idx, nb, E.seg_first);
   LNT_TRY(launch_fill_u32(ctx, E.counts, 0, 4));
   // E.head (free after seg_first_kernel) flags the large segments; their
@@ -1369,37 +1405,52 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     PA.list = c.take<uint32_t>(nseg);
     PA.seg_pos = c.take<uint64_t>(2 * nseg);
     piece_plan_kernel<<<grid_for(nseg * kWave), 256, 0, ctx->stream>>>(ak, E.a_lo, E.a_len, E.a_base, E.prod_off, E.seg_first,
-                                                               nseg, nb, E.list3, E.ex_off + nseg, PA.rmin, PA.width,
-                                                               PA.qc, PA.ovf, E.counts + 3, PA.seg_pos);
+                                                               nseg, nb, E.list3, E.ex_off + nseg, n3_ub, PA.rmin,
+                                                               PA.width, PA.qc, PA.ovf, E.counts + 3, PA.seg_pos);
     LNT_HIP(hipGetLastError());
   }
-  LNT_HIP(hipMemcpyAsync(ctx->pinned + 4, E.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  LNT_HIP(hipMemcpyAsync(ctx->pinned + 6, E.counts + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  LNT_HIP(hipMemcpyAsync(ctx->pinned + 8, E.ex_off + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-  LNT_HIP(hipStreamSynchronize(ctx->stream));
-  uint64_t n3_64;
-  memcpy(&n3_64, ctx->pinned + 8, sizeof(n3_64));
-  const uint32_t n2 = ctx->pinned[4], n3 = (uint32_t)n3_64, n_back = ctx->pinned[6];
+  // With row pieces the host goes on without the lists' sizes: the kernels
+  // below are sized by bounds and read them on the device (n2 in counts[0],
+  // n3 in ex_off[nseg]); the fallback count comes back with nnz(C).
+  const bool ahead = try_pieces && !wait_for_lists;
+  uint32_t n2, n3, n_back = 0;
+  if (ahead) {
+    n2 = (uint32_t)std::min<uint64_t>(nseg, kSegGrid);
+    n3 = (uint32_t)n3_ub;
+  } else {
+    LNT_HIP(hipMemcpyAsync(ctx->pinned + 4, E.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(ctx->pinned + 6, E.counts + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipMemcpyAsync(ctx->pinned + 8, E.ex_off + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    LNT_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t n3_64;
+    memcpy(&n3_64, ctx->pinned + 8, sizeof(n3_64));
+    n2 = std::min<uint32_t>(ctx->pinned[4], kSegGrid);
+    n3 = (uint32_t)n3_64;
+    n_back = ctx->pinned[6];
+  }
+  // n2: the grid of the short-segment kernels (they stride over the list);
+  // n3: the exact number of long segments, or its bound (qc is zero past
+  // the exact number, so scans and searches over n3 entries agree).
   if (n2 || n3) LNT_TRY(ensure(ctx->spg_tmp, nprod * 16));
   uint64_t *tmp_row = (uint64_t *)ctx->spg_tmp.ptr;
   double *tmp_val = tmp_row ? (double *)(tmp_row + nprod) : nullptr;
   if (n2 && dense) {
     if (acc == LNT_SPGEMM_MTT)
       dense_kernel<true><<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_len, E.a_base, E.prod_off,
-                                                         E.seg_first, nseg, nb, E.list2, (uint32_t)nrank, rt.ar,
-                                                         rt.rows, E.seg_u, tmp_row, tmp_val);
+                                                         E.seg_first, nseg, nb, E.list2, E.counts, (uint32_t)nrank,
+                                                         rt.ar, rt.rows, E.seg_u, tmp_row, tmp_val);
     else
       dense_kernel<false><<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_len, E.a_base, E.prod_off,
-                                                          E.seg_first, nseg, nb, E.list2, (uint32_t)ah, nullptr,
-                                                          nullptr, E.seg_u, tmp_row, tmp_val);
+                                                          E.seg_first, nseg, nb, E.list2, E.counts, (uint32_t)ah,
+                                                          nullptr, nullptr, E.seg_u, tmp_row, tmp_val);
     LNT_HIP(hipGetLastError());
   } else if (n2) {
     if (acc == LNT_SPGEMM_HASH)
       small_hash_kernel<<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first,
-                                                          nseg, nb, E.list2, E.seg_u, tmp_row, tmp_val);
+                                                          nseg, nb, E.list2, E.counts, E.seg_u, tmp_row, tmp_val);
     else
       small_sort_kernel<<<n2, kBlock, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.prod_off, E.seg_first,
-                                                          nseg, nb, E.list2, E.seg_u, tmp_row, tmp_val);
+                                                          nseg, nb, E.list2, E.counts, E.seg_u, tmp_row, tmp_val);
     LNT_HIP(hipGetLastError());
   }
   // The global sort over the large segments in `list_esc` (expand with
@@ -1505,8 +1556,8 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
         cv);
     LNT_HIP(hipGetLastError());
     if (direct) {
-      piece_total_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(E.list3, n3, PA.qc_off, PA.status, E.seg_u,
-                                                                     E.direct);
+      piece_total_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(E.list3, E.ex_off + nseg, PA.qc_off, PA.status,
+                                                                     E.seg_u, E.direct);
       LNT_HIP(hipGetLastError());
     }
   }
@@ -1517,7 +1568,15 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
   // nnz(C) sizes the allocation or is checked against the capacity first.
   const bool queue_ahead = ck && !ck_alloc && capacity >= nprod;
   uint64_t nc = nprod;
-  if (!queue_ahead) LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
+  if (!queue_ahead) {
+    uint64_t hv2[2];
+    LNT_TRY(read_two_u64(ctx, E.out_off + nseg, (const uint64_t *)(E.counts + 2), hv2));
+    nc = hv2[0];
+    if (ahead && (hv2[1] >> 32)) {  // counts[3]: segments the pieces could not take
+      prof_end(ctx, slot);
+      return kRunAgainWaiting;
+    }
+  }
   if (ck_alloc) {
     void *k = nullptr, *v = nullptr;
     size_t bytes = (nc ? nc : 1) * 8;
@@ -1557,10 +1616,31 @@ int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, u
     return fail(LNT_ERR_DEVICE, std::string("write_kernel launch: ") + hipGetErrorString(launch));
   }
   prof_end(ctx, slot);
-  if (queue_ahead) LNT_TRY(read_u64(ctx, E.out_off + nseg, &nc, 1));
+  if (queue_ahead) {
+    uint64_t hv2[2];
+    LNT_TRY(read_two_u64(ctx, E.out_off + nseg, (const uint64_t *)(E.counts + 2), hv2));
+    nc = hv2[0];
+    // Fallback segments: C is written again by a run that waits for them.
+    if (ahead && (hv2[1] >> 32)) return kRunAgainWaiting;
+  }
   prof_set_bytes(ctx, slot, 16.0 * ((double)na + nb + nc));
   *nc_host = nc;
   return LNT_OK;
+}
+
+}  // namespace
+
+int lattice_mult_device(lnt_context ctx, const uint64_t *ak, const double *av, uint64_t na,
+                        const int64_t a_dims[3], const uint64_t *bk, const double *bv, uint64_t nb,
+                        const int64_t b_dims[3], int strategy, uint64_t *ck, double *cv,
+                        uint64_t capacity, uint64_t *nc_host, uint64_t *nprod_host,
+                        uint64_t **ck_alloc, double **cv_alloc) {
+  int rc = lattice_mult_impl(ctx, ak, av, na, a_dims, bk, bv, nb, b_dims, strategy, ck, cv, capacity, nc_host,
+                             nprod_host, ck_alloc, cv_alloc, false);
+  if (rc == kRunAgainWaiting)
+    rc = lattice_mult_impl(ctx, ak, av, na, a_dims, bk, bv, nb, b_dims, strategy, ck, cv, capacity, nc_host,
+                           nprod_host, ck_alloc, cv_alloc, true);
+  return rc;
 }
 
 }  // namespace lnt
