@@ -1057,11 +1057,16 @@ __global__ __launch_bounds__(kPieceBlock, 6) void piece_kernel(
 // Direct mode: each pieced segment's row count (the difference of the
 // inclusive counts that close it and its predecessor), and its mark for the
 // output kernel, which skips it.
-__global__ void piece_total_kernel(const uint32_t *__restrict__ list3, const uint64_t *__restrict__ n3_dev,
-                                   const uint64_t *__restrict__ off, const uint64_t *__restrict__ status,
-                                   uint64_t *__restrict__ seg_u, uint8_t *__restrict__ direct) {
+// n3_host: the number of long segments or the host's bound on it; n3_dev:
+// the number on the device, read when the host has only the bound (the
+// global sort of fallback segments reuses that word, but it runs only when
+// the host has read the number).
+__global__ void piece_total_kernel(const uint32_t *__restrict__ list3, uint32_t n3_host,
+                                   const uint64_t *__restrict__ n3_dev, const uint64_t *__restrict__ off,
+                                   const uint64_t *__restrict__ status, uint64_t *__restrict__ seg_u,
+                                   uint8_t *__restrict__ direct) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= *n3_dev) return;
+  if (r >= n3_host || (n3_dev && r >= *n3_dev)) return;
   const uint64_t t0 = off[r] >> 32, t1 = off[r + 1] >> 32;
   if (t1 == t0) return;  // taken by the global sort
   const uint64_t hi = status[t1 - 1] & kPieceValue, lo = t0 ? status[t0 - 1] & kPieceValue : 0;
@@ -1556,8 +1561,8 @@ int lattice_mult_impl(lnt_context ctx, const uint64_t *ak, const double *av, uin
         cv);
     LNT_HIP(hipGetLastError());
     if (direct) {
-      piece_total_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(E.list3, E.ex_off + nseg, PA.qc_off, PA.status,
-                                                                     E.seg_u, E.direct);
+      piece_total_kernel<<<(n3 + 255) / 256, 256, 0, ctx->stream>>>(E.list3, n3, ahead ? E.ex_off + nseg : nullptr,
+                                                                     PA.qc_off, PA.status, E.seg_u, E.direct);
       LNT_HIP(hipGetLastError());
     }
   }

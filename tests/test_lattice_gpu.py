@@ -226,6 +226,42 @@ def test_lattice_mult_row_pieces(ctx, orc, kind):
         ctx.set_param("spgemm_pieces", 1)
 
 
+@pytest.mark.parametrize("extra_short", [False, True])
+def test_lattice_mult_all_long_columns_fall_back(ctx, orc, extra_short):
+    """Every long output column has more than 64 B entries, so the row pieces
+    take none of them and all go to the global sort (the contraction finds
+    this at the end and runs again waiting for the lists); with and without
+    a single-entry column beside them.  Keys and value bits equal the
+    oracle's sparse accumulator, into arrays of nprod entries (output
+    launches queued ahead) and of exactly nnz(C) entries."""
+    import torch
+
+    rng = np.random.default_rng(91 + int(extra_short))
+    ah, aw = 300, 200
+    rows = [np.sort(rng.choice(ah, int(rng.integers(20, 120)), replace=False)) + ah * c for c in range(aw)]
+    ak = np.concatenate(rows).astype(np.int64)
+    av = rng.uniform(-1, 1, len(ak))
+    counts = [100, 130, 65] + ([1] if extra_short else [])
+    ks, cs = [], []
+    for c, cnt in enumerate(counts):
+        ks += list(np.sort(rng.choice(aw, cnt, replace=False)))
+        cs += [c] * cnt
+    bk = (np.array(ks) + aw * np.array(cs)).astype(np.int64)
+    bv = rng.uniform(-1, 1, len(bk))
+    bw = len(counts)
+    sk, sv = orc.lattice_mult_by("spa", ak, av, (ah, aw, 1), bk, bv, (aw, bw, 1))
+    print("all-fallback columns", extra_short, len(ak), len(bk), flush=True)
+    for strategy in (1, 17):
+        ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1), strategy)
+        assert np.array_equal(host(ck), sk), strategy
+        assert np.array_equal(host(cv).view(np.int64), sv.view(np.int64)), strategy
+        ck = torch.empty(len(sk), dtype=torch.int64, device="cuda")
+        cv = torch.empty(len(sk), dtype=torch.float64, device="cuda")
+        n = ctx.lattice_mult_into(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1), ck, cv, strategy)
+        assert n == len(sk)
+        assert np.array_equal(host(ck), sk) and np.array_equal(host(cv).view(np.int64), sv.view(np.int64))
+
+
 def test_lattice_mult_empty_and_errors(ctx, orc):
     import libnt_amd
     import torch
