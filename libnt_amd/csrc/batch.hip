@@ -1601,22 +1601,90 @@ __global__ __launch_bounds__(BT) void batch_dense_kernel(ColumnArgs A, const Den
 }
 
 // ------------------------------------------- 5. single-entry segments: copy
-// One wave per segment (grid-stride over segments): C(:,j) = b * A(:,k).
+// C(:,j) = b * A(:,k) for every segment of one B entry.  A wave takes 64
+// consecutive segments (lane j: segment g*64 + j, its A run, output offset,
+// key shift and B value in registers) and spreads their products over its
+// lanes: a wave prefix of the single segments' lengths numbers the products,
+// a chunk of 64*kSingleItems products finds each product's segment by walking
+// the few segments that meet the chunk (a ballot, then one compare per item
+// and segment), takes that segment's place with lane shuffles, and issues
+// every load of the chunk before its stores -- so a wave keeps that many
+// loads in flight whatever the segments' lengths (they are often shorter
+// than a wave).  The groups are cut into eight contiguous ranges, one per XCD
+// (workgroups are dealt to the XCDs in turn): the waves of one XCD then work
+// on few instances at a time, whose A columns -- each read by many segments
+// -- stay in that XCD's L2 (HBM reads fell about fourfold; 18.7 -> 13.1 ms on
+// the 10,000-contraction mix).  Correctness does not depend on the placement.
+constexpr uint32_t kSingleItems = 8;
+
+__device__ inline uint64_t readlane64(uint64_t v, uint32_t j) {
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, (int)j);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// The grid is a multiple of 8.
 __global__ __launch_bounds__(kBlock) void batch_single_kernel(ColumnArgs A, const uint32_t *__restrict__ kind,
                                                               const uint64_t *__restrict__ ctr) {
   const uint64_t nseg = ctr[kCtrNseg];
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
-  for (uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6); s < nseg; s += waves) {
-    if (kind[s] != 1) continue;
-    const uint64_t e = A.seg_first[s];
-    const uint64_t lo = A.a_lo[e], n = A.a_len[e], ab = A.a_base[e], ck0 = A.colkey[e];
-    const double b = A.bv[e];
-    const uint64_t o = A.out_off[s];
-    for (uint64_t q = lane; q < n; q += kWave) {
-      if (o + q >= A.capacity) break;
-      A.ck[o + q] = A.ak[lo + q] - ab + ck0;
-      A.cv[o + q] = b * A.av[lo + q];
+  const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint64_t ngroups = (nseg + kWave - 1) / kWave;
+  const uint32_t x = blockIdx.x & 7, nx = gridDim.x >> 3;
+  const uint64_t g0 = ngroups * x / 8, g1 = ngroups * (x + 1) / 8;
+  const uint64_t waves = (uint64_t)nx * (kBlock / kWave);
+  for (uint64_t g = g0 + (uint64_t)(blockIdx.x >> 3) * (kBlock / kWave) + w; g < g1; g += waves) {
+    const uint64_t s = g * kWave + lane;
+    const bool single = s < nseg && kind[s] == 1;
+    if (!__ballot(single)) continue;
+    uint64_t n = 0, o = 0, lo = 0, kshift = 0;
+    double b = 0.0;
+    if (single) {
+      const uint64_t e = A.seg_first[s];
+      o = A.out_off[s];
+      lo = A.a_lo[e];
+      n = A.a_len[e];
+      kshift = A.colkey[e] - A.a_base[e];
+      b = A.bv[e];
+    }
+    const uint64_t incl = wave_inclusive_sum64(n), excl = incl - n;
+    const uint64_t ashift = lo - excl, oshift = o - excl;  // A index and output index - product number
+    const uint64_t total = __shfl(incl, kWave - 1, kWave);
+    for (uint64_t p0 = 0; p0 < total; p0 += (uint64_t)kWave * kSingleItems) {
+      const uint64_t pend = min<uint64_t>(p0 + (uint64_t)kWave * kSingleItems, total);
+      // The segments that meet [p0, pend), ascending: the first holds p0, so
+      // every item gets a segment; later ones take the items past their start.
+      uint64_t m = __ballot(n > 0 && excl < pend && incl > p0);
+      uint32_t js[kSingleItems];
+#pragma unroll
+      for (uint32_t u = 0; u < kSingleItems; ++u) js[u] = 0;
+      while (m) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        const uint64_t ej = readlane64(excl, j);
+#pragma unroll
+        for (uint32_t u = 0; u < kSingleItems; ++u)
+          if (p0 + u * kWave + lane >= ej) js[u] = j;
+      }
+      uint64_t key[kSingleItems], dst[kSingleItems];
+      double val[kSingleItems];
+#pragma unroll
+      for (uint32_t u = 0; u < kSingleItems; ++u) {
+        const uint64_t p = p0 + u * kWave + lane;
+        const uint64_t pc = p < total ? p : total - 1;  // a place inside the last segment
+        // Every lane shuffles: a lane reading from an inactive lane gets nothing.
+        const uint64_t ai = __shfl(ashift, (int)js[u], kWave) + pc;
+        const uint64_t os = __shfl(oshift, (int)js[u], kWave);
+        key[u] = A.ak[ai] + __shfl(kshift, (int)js[u], kWave);
+        val[u] = __shfl(b, (int)js[u], kWave) * A.av[ai];
+        dst[u] = p < total ? os + p : ~0ull;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kSingleItems; ++u) {
+        if (dst[u] < A.capacity) {
+          A.ck[dst[u]] = key[u];
+          A.cv[dst[u]] = val[u];
+        }
+      }
     }
   }
 }
@@ -1876,7 +1944,7 @@ int enqueue_numeric(lnt_batch b, const BatchPtrs &P, uint64_t *ck, double *cv, u
     const uint32_t grid = (uint32_t)b->cus * b->tuning.blocks_per_cu;
     const uint32_t G = std::min<uint32_t>(b->tuning.item_products, kCap / 2);
     const uint32_t window = std::min<uint32_t>(b->tuning.window, 3 * kCap / 4);
-    batch_single_kernel<<<std::min<uint32_t>(grid_for(nb * 64), 65536), kBlock, 0, st>>>(A, P.kind, P.ctr);
+    batch_single_kernel<<<grid_for(nb) / 8 * 8 + 8, kBlock, 0, st>>>(A, P.kind, P.ctr);
     // Items hold at most kMaxSegsPerItem segments: 11 bits above the rows.
     const uint32_t row_bits = b->max_h > 1 ? 64u - (uint32_t)__builtin_clzll(b->max_h - 1) : 0u;
     if (row_bits + 11 <= 32)

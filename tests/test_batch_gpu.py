@@ -215,6 +215,58 @@ def test_batch_row_pieces_shapes(ctx, orc, kind):
             assert np.array_equal(v.view(np.int64), ev.view(np.int64)), (slots, i)
 
 
+def test_batch_single_entry_columns(ctx, orc):
+    """Output columns of one B entry (a scaled copy of one A column), spread
+    over the lanes of a wave 64 columns at a time: A columns of 0 .. 3,000
+    entries (around the wave width and the 512-product chunk), runs of
+    one-entry columns broken by columns of two and three entries, more
+    instances than XCD ranges and a first instance with none: keys and value
+    bits equal to the oracle's sparse accumulator, eager and replayed."""
+    import torch
+
+    from libnt_amd.batch import Batch, Instance
+
+    rng = np.random.default_rng(31)
+    lens = [0, 1, 2, 5, 63, 64, 65, 127, 128, 129, 511, 512, 513, 600, 1000, 3000]
+    insts, parts, expect = [], [[], [], [], []], []
+    for inst in range(11):
+        h, aw = 4096, len(lens)
+        bw = 1 if inst == 0 else int(rng.integers(50, 400))
+        rows = [np.sort(rng.choice(h, n, replace=False)).astype(np.int64) + h * c for c, n in enumerate(lens)]
+        ak = np.concatenate(rows).astype(np.int64)
+        av = rng.uniform(-1, 1, len(ak))
+        av[::37] = -0.0
+        ks, cs = [], []
+        for c in range(bw):
+            cnt = 2 if inst == 0 else (1 if rng.random() < 0.85 else int(rng.integers(2, 4)))
+            ks += list(np.sort(rng.choice(aw, cnt, replace=False)))
+            cs += [c] * cnt
+        bk = (np.array(ks) + aw * np.array(cs)).astype(np.int64)
+        order = np.argsort(bk)
+        bk = bk[order]
+        bv = rng.uniform(-1, 1, len(bk))
+        bv[::11] = -0.0
+        insts.append(Instance([h, aw], "ca", [aw, bw], "ag", len(ak), len(bk)))
+        for lst, x in zip(parts, (ak, av, bk, bv)):
+            lst.append(x)
+        expect.append(orc.lattice_mult_by("spa", ak, av, (h, aw, 1), bk, bv, (aw, bw, 1)))
+    print("batch single-entry columns", [len(x) for x in parts[2]], flush=True)
+    ops = [torch.from_numpy(np.concatenate(lst)).cuda() for lst in parts]
+    b = Batch(ctx, insts)
+
+    def check_all(ck, cv, off):
+        for i, (ek, ev) in enumerate(expect):
+            k = host(ck[off[i]:off[i + 1]])
+            v = host(cv[off[i]:off[i + 1]])
+            assert np.array_equal(k, ek), i
+            assert np.array_equal(v.view(np.int64), ev.view(np.int64)), i
+
+    check_all(*b.contract(*ops))
+    b.capture(*ops)
+    for _ in range(2):
+        check_all(*b.replay())
+
+
 @pytest.mark.parametrize("seed", [21, 23])
 def test_batch_accumulator_choices_do_not_change_results(ctx, seed):
     """The per-instance accumulator choice (lnt_batch_set_accumulators):
