@@ -109,6 +109,11 @@ int lnt_abi_version(void);
  *                  pieces merged in LDS (the global sort takes only columns
  *                  with more than 64 B entries or rows clustered beyond a
  *                  piece); 0: every long column through the global sort
+ *   "spgemm_single_copy" 1 (default): when output columns hold 64 or more
+ *                  products on average, the columns of one B entry (a scaled
+ *                  copy of one A column) are written by a kernel that deals
+ *                  them to the XCDs by their place in A, so that each XCD's
+ *                  L2 keeps the part of A it reads; 0: by the output kernel
  * No reference counterpart (the reference hard-codes radix 2048 and the
  * thresholds 3000/50, multi_index_array/LibMIARadix.h:454-457). */
 int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value);

@@ -250,6 +250,7 @@ int lnt_context_set_param(lnt_context ctx, const char *name, int64_t value) {
   else if (n == "narrow_keys" && in(0, 1)) ctx->tuning.narrow_keys = (int)value;
   else if (n == "row_ranks" && in(0, 1)) ctx->tuning.row_ranks = (int)value;
   else if (n == "spgemm_pieces" && in(0, 1)) ctx->tuning.spgemm_pieces = (int)value;
+  else if (n == "spgemm_single_copy" && in(0, 1)) ctx->tuning.spgemm_single_copy = (int)value;
   else if (n == "digit_bits" && in(4, 8)) ctx->tuning.digit_bits = (int)value;
   else if (n == "sort_block" && (value == 256 || value == 512 || value == 1024)) ctx->tuning.sort_block = (int)value;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter or value out of range: " + n);
@@ -267,6 +268,7 @@ int lnt_context_get_param(lnt_context ctx, const char *name, int64_t *value) {
   else if (n == "narrow_keys") *value = ctx->tuning.narrow_keys;
   else if (n == "row_ranks") *value = ctx->tuning.row_ranks;
   else if (n == "spgemm_pieces") *value = ctx->tuning.spgemm_pieces;
+  else if (n == "spgemm_single_copy") *value = ctx->tuning.spgemm_single_copy;
   else if (n == "digit_bits") *value = ctx->tuning.digit_bits;
   else if (n == "sort_block") *value = ctx->tuning.sort_block;
   else return fail(LNT_ERR_INVALID_ARGUMENT, "unknown parameter: " + n);

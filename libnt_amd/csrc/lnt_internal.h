@@ -444,6 +444,7 @@ struct SortTuning {
   int narrow_keys = 1;  // 32-bit keys between passes when they fit
   int row_ranks = 1;    // lattice product: long columns sort row ranks when that saves passes
   int spgemm_pieces = 1;  // lattice product: long columns in row pieces (0: all through the global sort)
+  int spgemm_single_copy = 1;  // lattice product: single-entry columns by the XCD-local copy kernel
   int digit_bits = 8;  // largest radix digit per onesweep pass (4..8)
   int sort_block = 1024;  // onesweep threads per block (256, 512, 1024): tile = sort_block * items
 };

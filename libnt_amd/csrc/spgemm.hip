@@ -138,14 +138,16 @@ __global__ void classify_kernel(const uint64_t *__restrict__ seg_first, uint64_t
                                 const uint64_t *__restrict__ prod_off, uint32_t small_max,
                                 uint64_t *__restrict__ seg_u, uint32_t *__restrict__ list2,
                                 uint32_t *__restrict__ large, uint8_t *__restrict__ direct,
-                                uint32_t *__restrict__ counts) {
+                                uint32_t *__restrict__ counts, int mark_single) {
   uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; s < nseg; s += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t first = seg_first[s];
     uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
     uint64_t np = prod_off[last] - prod_off[first];
     uint32_t is_large = 0;
-    direct[s] = 0;
+    // Marked segments are written by another kernel (single_copy_kernel, or
+    // the pieces): the output kernel skips them.
+    direct[s] = mark_single && last - first == 1 && np > 0;
     if (np == 0 || last - first == 1) {
       seg_u[s] = np;
     } else if (np <= small_max) {
@@ -472,7 +474,10 @@ __device__ inline ChunkSegs stage_chunk_from(const uint64_t *__restrict__ off, u
 // cs[k] = the segment holding index k * kChunk (k < nchunks), all chunks at
 // once, so that no workgroup walks a long search chain alone.  nnz(C) is
 // read on the device (off[nseg]), so the host need not wait for it.
-__global__ void chunk_starts_kernel(const uint64_t *__restrict__ off, uint64_t nseg, uint64_t *__restrict__ cs) {
+// n_short: as in write_kernel (nothing to do when it points at zero).
+__global__ void chunk_starts_kernel(const uint64_t *__restrict__ off, uint64_t nseg, uint64_t *__restrict__ cs,
+                                    const uint32_t *__restrict__ n_short) {
+  if (n_short && *n_short == 0) return;
   const uint64_t nchunks = (off[nseg] + kChunk - 1) / kChunk;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nchunks; k += (uint64_t)gridDim.x * blockDim.x)
     cs[k] = upper_index(off, 0, nseg, k * kChunk);
@@ -1087,6 +1092,96 @@ __global__ void fill_words_kernel(uint64_t *__restrict__ p, uint64_t n) {
     p[i] = 0;
 }
 
+// 4a. output of the single-entry segments (C(:,j) = b * A(:,k)), when they
+// are long on average.  Every A column is read by many such segments: the
+// segments are dealt to the XCDs by where their A run starts (the x-th eighth
+// of A to XCD x; workgroups are dealt to the XCDs in turn), so that each
+// XCD's L2 holds the eighth of A it reads instead of every XCD streaming all
+// of A.  A workgroup takes 64 consecutive segments (lane j: segment g*64 + j
+// when it is single and in the XCD's eighth) and its waves take turns over
+// chunks of 64*kCopyItems of their products; each product finds its segment
+// by walking the segments that meet its chunk (a ballot, then one compare
+// per item and segment) and takes that segment's place by lane shuffles.
+// Every load of a chunk is issued before its stores.  Correctness does not
+// depend on the placement.
+constexpr uint32_t kCopyItems = 8;
+
+__device__ inline uint64_t readlane64(uint64_t v, uint32_t j) {
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, (int)j);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// The grid is a multiple of 8.
+__global__ __launch_bounds__(256) void single_copy_kernel(
+    const uint64_t *__restrict__ ak, const double *__restrict__ av, uint64_t na, const double *__restrict__ bv,
+    const uint64_t *__restrict__ a_lo, const uint32_t *__restrict__ a_len, const uint64_t *__restrict__ a_base,
+    const uint64_t *__restrict__ colid, const uint64_t *__restrict__ seg_first, uint64_t nseg, uint64_t nb,
+    const uint64_t *__restrict__ out_off, uint64_t ah, uint64_t *__restrict__ ck, double *__restrict__ cv) {
+  const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint32_t x = blockIdx.x & 7;
+  const uint64_t ngroups = (nseg + kWave - 1) / kWave;
+  for (uint64_t g = blockIdx.x >> 3; g < ngroups; g += gridDim.x >> 3) {
+    const uint64_t s = g * kWave + lane;
+    uint64_t n = 0, o = 0, lo = 0, kshift = 0;
+    double b = 0.0;
+    if (s < nseg) {
+      const uint64_t first = seg_first[s];
+      const uint64_t last = s + 1 < nseg ? seg_first[s + 1] : nb;
+      if (last - first == 1) {
+        lo = a_lo[first];
+        if (lo * 8 / na == x) {
+          n = a_len[first];
+          o = out_off[s];
+          kshift = ah * colid[first] - a_base[first];
+          b = bv[first];
+        }
+      }
+    }
+    const uint64_t incl = wave_inclusive_sum64(n), excl = incl - n;
+    const uint64_t ashift = lo - excl, oshift = o - excl;  // A index and output index - product number
+    const uint64_t total = __shfl(incl, kWave - 1, kWave);
+    constexpr uint64_t kSpan = (uint64_t)kWave * kCopyItems;
+    for (uint64_t p0 = (uint64_t)w * kSpan; p0 < total; p0 += 4 * kSpan) {
+      const uint64_t pend = min<uint64_t>(p0 + kSpan, total);
+      // The segments that meet [p0, pend), ascending: the first holds p0, so
+      // every item gets a segment; later ones take the items past their start.
+      uint64_t m = __ballot(n > 0 && excl < pend && incl > p0);
+      uint32_t js[kCopyItems];
+#pragma unroll
+      for (uint32_t u = 0; u < kCopyItems; ++u) js[u] = 0;
+      while (m) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        const uint64_t ej = readlane64(excl, j);
+#pragma unroll
+        for (uint32_t u = 0; u < kCopyItems; ++u)
+          if (p0 + u * kWave + lane >= ej) js[u] = j;
+      }
+      uint64_t key[kCopyItems], dst[kCopyItems];
+      double val[kCopyItems];
+#pragma unroll
+      for (uint32_t u = 0; u < kCopyItems; ++u) {
+        const uint64_t p = p0 + u * kWave + lane;
+        const uint64_t pc = p < total ? p : total - 1;  // a place inside the last segment
+        // Every lane shuffles: a lane reading from an inactive lane gets nothing.
+        const uint64_t ai = __shfl(ashift, (int)js[u], kWave) + pc;
+        const uint64_t os = __shfl(oshift, (int)js[u], kWave);
+        key[u] = ak[ai] + __shfl(kshift, (int)js[u], kWave);
+        val[u] = __shfl(b, (int)js[u], kWave) * av[ai];
+        dst[u] = p < total ? os + p : ~0ull;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kCopyItems; ++u) {
+        if (dst[u] != ~0ull) {
+          ck[dst[u]] = key[u];
+          cv[dst[u]] = val[u];
+        }
+      }
+    }
+  }
+}
+
 // 4. output: the elements of C in order, kChunk per workgroup round.
 __global__ __launch_bounds__(256) void write_kernel(const uint64_t *__restrict__ ak, const double *__restrict__ av,
                              const double *__restrict__ bv, const uint64_t *__restrict__ a_lo,
@@ -1095,10 +1190,13 @@ __global__ __launch_bounds__(256) void write_kernel(const uint64_t *__restrict__
                              uint64_t nseg, uint64_t nb, const uint64_t *__restrict__ out_off,
                              const uint64_t *__restrict__ tmp_row, const double *__restrict__ tmp_val,
                              uint64_t ah, const uint64_t *__restrict__ cs,
-                             const uint8_t *__restrict__ direct, uint64_t *__restrict__ ck,
-                             double *__restrict__ cv) {
+                             const uint8_t *__restrict__ direct, const uint32_t *__restrict__ n_short,
+                             uint64_t *__restrict__ ck, double *__restrict__ cv) {
   __shared__ uint64_t s_off[kChunk + 1];
   __shared__ uint64_t s_seg[2];
+  // Only short segments left to write (every other one is marked), and none
+  // of them: nothing to do.
+  if (n_short && *n_short == 0) return;
   const uint64_t nc = out_off[nseg];  // nnz(C), read on the device
   const uint64_t nchunks = (nc + kChunk - 1) / kChunk;
   for (uint64_t k = blockIdx.x; k < nchunks; k += gridDim.x) {
@@ -1388,8 +1486,13 @@ int lattice_mult_impl(lnt_context ctx, const uint64_t *ak, const double *av, uin
   // E.head (free after seg_first_kernel) flags the large segments; their
   // list is built in ascending order, so that their pieces run in output
   // order; E.ex_off holds its positions until the global sort reuses it.
+  // Single-entry segments that are long on average are copied by
+  // single_copy_kernel (A shared through each XCD's L2), not by the output
+  // kernel.
+  const bool single_copy = ctx->tuning.spgemm_single_copy && nprod >= 64 * nseg;
   classify_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.seg_first, nseg, nb, E.prod_off, small_max,
-                                                            E.seg_u, E.list2, E.head, E.direct, E.counts);
+                                                            E.seg_u, E.list2, E.head, E.direct, E.counts,
+                                                            single_copy ? 1 : 0);
   LNT_HIP(hipGetLastError());
   LNT_TRY(exclusive_scan<uint32_t>(ctx, E.head, E.ex_off, nseg, E.ex_off + nseg, ctx->aux, 0));
   large_list_kernel<<<grid_for(nseg), 256, 0, ctx->stream>>>(E.head, E.ex_off, nseg, E.list3);
@@ -1604,10 +1707,18 @@ int lattice_mult_impl(lnt_context ctx, const uint64_t *ak, const double *av, uin
   const uint64_t nchunks = (nc + kChunk - 1) / kChunk;
   LNT_TRY(ensure(ctx->spg_chunk, (nchunks + 1) * sizeof(uint64_t)));
   uint64_t *cs = (uint64_t *)ctx->spg_chunk.ptr;
-  chunk_starts_kernel<<<grid_for(nchunks, 4096), 256, 0, ctx->stream>>>(E.out_off, nseg, cs);
+  if (single_copy)
+    single_copy_kernel<<<8 * (uint32_t)std::min<uint64_t>((nseg + kWave - 1) / kWave, 4096), 256, 0, ctx->stream>>>(
+        ak, av, na, bv, E.a_lo, E.a_len, E.a_base, E.colid, E.seg_first, nseg, nb, E.out_off, ah, ck, cv);
+  // With the single-entry segments copied and the long ones written by
+  // their pieces (none staged, none through the global sort), the output
+  // kernel has only the short segments, and leaves when there are none.
+  const bool only_short = single_copy && (!n3 || (pieces && direct)) && n_back == 0;
+  const uint32_t *n_short = only_short ? E.counts : nullptr;
+  chunk_starts_kernel<<<grid_for(nchunks, 4096), 256, 0, ctx->stream>>>(E.out_off, nseg, cs, n_short);
   write_kernel<<<chunk_grid(nc), 256, 0, ctx->stream>>>(ak, av, bv, E.a_lo, E.a_base, E.colid, E.prod_off, E.seg_first,
                                                        nseg, nb, E.out_off, tmp_row, tmp_val, ah, cs, E.direct,
-                                                       ck, cv);
+                                                       n_short, ck, cv);
   const hipError_t launch = hipGetLastError();
   if (launch != hipSuccess) {
     prof_end(ctx, slot);

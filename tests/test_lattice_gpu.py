@@ -226,6 +226,66 @@ def test_lattice_mult_row_pieces(ctx, orc, kind):
         ctx.set_param("spgemm_pieces", 1)
 
 
+@pytest.mark.parametrize("others", ["none", "short", "long", "both"])
+def test_lattice_mult_single_entry_columns(ctx, orc, others):
+    """Output columns of one B entry (a scaled copy of one A column) long
+    enough on average for the XCD-local copy kernel: A columns of 0 .. 3,000
+    entries, alone or beside short columns (LDS accumulators, staged), long
+    ones (row pieces) or both.  Keys and value bits equal the oracle's
+    sparse accumulator with and without the copy kernel, into arrays of
+    nprod entries (launches queued ahead, pieces direct), of exactly nnz(C)
+    entries (pieces staged) and allocated."""
+    import torch
+
+    rng = np.random.default_rng(["none", "short", "long", "both"].index(others) + 120)
+    ah = 8192
+    lens = [0, 1, 2, 5, 63, 64, 65, 127, 128, 129, 511, 512, 513, 1000, 3000, 2500] * 4
+    aw = len(lens)
+    rows = [np.sort(rng.choice(ah, n, replace=False)).astype(np.int64) + ah * c for c, n in enumerate(lens)]
+    ak = np.concatenate(rows).astype(np.int64)
+    av = rng.uniform(-1, 1, len(ak))
+    av[::41] = -0.0
+    counts = [1] * 300
+    if others in ("short", "both"):
+        counts += [2, 3] * 10
+    if others in ("long", "both"):
+        counts += [6, 12, 30] * 4
+    rng.shuffle(counts)
+    ks, cs = [], []
+    for c, cnt in enumerate(counts):
+        ks += list(np.sort(rng.choice(aw, cnt, replace=False)))
+        cs += [c] * cnt
+    bw = len(counts)
+    bk = (np.array(ks) + aw * np.array(cs)).astype(np.int64)
+    order = np.argsort(bk)
+    bk = bk[order]
+    bv = rng.uniform(-1, 1, len(bk))
+    bv[::7] = -0.0
+    sk, sv = orc.lattice_mult_by("spa", ak, av, (ah, aw, 1), bk, bv, (aw, bw, 1))
+    nprod = int(orc.lattice_nprod(ak, (ah, aw, 1), bk, (aw, bw, 1)))
+    print("single-entry columns", others, len(ak), len(bk), nprod, len(sk), flush=True)
+    try:
+        for single_copy in (1, 0):
+            ctx.set_param("spgemm_single_copy", single_copy)
+            for strategy in (1, 17):
+                for cap in (nprod, len(sk), None):
+                    if cap is None:
+                        ck, cv = ctx.lattice_mult(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1),
+                                                  strategy)
+                    else:
+                        ck = torch.empty(cap, dtype=torch.int64, device="cuda")
+                        cv = torch.empty(cap, dtype=torch.float64, device="cuda")
+                        n = ctx.lattice_mult_into(gpu(ak), gpu(av), (ah, aw, 1), gpu(bk), gpu(bv), (aw, bw, 1),
+                                                  ck, cv, strategy)
+                        assert n == len(sk)
+                        ck, cv = ck[:n], cv[:n]
+                    tag = (single_copy, strategy, cap)
+                    assert np.array_equal(host(ck), sk), tag
+                    assert np.array_equal(host(cv).view(np.int64), sv.view(np.int64)), tag
+    finally:
+        ctx.set_param("spgemm_single_copy", 1)
+
+
 @pytest.mark.parametrize("extra_short", [False, True])
 def test_lattice_mult_all_long_columns_fall_back(ctx, orc, extra_short):
     """Every long output column has more than 64 B entries, so the row pieces
