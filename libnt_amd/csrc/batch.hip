@@ -951,8 +951,15 @@ struct PieceArgs {
   uint64_t *piece_ent;             // per piece slot: first B entry << 8 | entries
   uint64_t *status;                // per piece slot: look-back word of the direct pass
   uint32_t *list3w;                // the segments the windows take
+  uint32_t *bnd;                   // per piece slot: kPieceBnd first A entries of its runs (offsets in
+                                   // their columns), for pieces of at most kPieceBnd B entries
   uint64_t cap;                    // piece slots
 };
+
+// Pieces of at most kPieceBnd B entries find their runs in bnd (searched
+// beforehand, all pieces at once) instead of searching the columns first.
+constexpr uint32_t kPieceBnd = 16;
+constexpr size_t kPieceSlotBytes = 24 + 4 * kPieceBnd;
 
 // Per large segment (one wave each, lane i on B entry i): the row range of
 // its A columns, the piece width and the number of pieces (0: the windows
@@ -1030,6 +1037,37 @@ __global__ __launch_bounds__(kBlock) void batch_piece_assign_kernel(ColumnArgs A
   }
 }
 
+// bnd[t * kPieceBnd + i]: the first A entry of piece slot t in the column of
+// its segment's B entry i (an offset in that column), for every piece of at
+// most kPieceBnd entries -- one thread per (piece, entry): the searches run
+// side by side instead of on each piece's path.
+__global__ __launch_bounds__(kBlock) void batch_piece_bounds_kernel(ColumnArgs A, PieceArgs Q,
+                                                                    const uint64_t *__restrict__ ctr) {
+  const uint64_t npieces = min<uint64_t>(ctr[kCtrPieces], Q.cap);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < npieces * kPieceBnd; x += stride) {
+    const uint64_t t = x / kPieceBnd;
+    const uint32_t i = (uint32_t)(x % kPieceBnd);
+    const uint32_t r = Q.piece_seg[t];
+    if (r == kNoPiece) continue;
+    const uint64_t ent = Q.piece_ent[t];
+    const uint32_t c = (uint32_t)(ent & 0xFF);
+    if (c > kPieceBnd || i >= c) continue;
+    const uint64_t e = (ent >> 8) + i, p = t - Q.pbase[r];
+    const uint64_t base = A.a_lo[e];
+    uint64_t lo = base, hi = base + A.a_len[e];
+    if (p > 0) {
+      const uint64_t key = A.a_base[e] + Q.rmin[r] + p * Q.width[r];
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (A.ak[mid] < key) lo = mid + 1;
+        else hi = mid;
+      }
+    }
+    Q.bnd[x] = (uint32_t)(lo - base);
+  }
+}
+
 // Status word of a piece in the direct pass: [63:62] 1 = its own row count,
 // 2 = the count of every piece up to and including it; [61:0] the count.
 constexpr uint64_t kPieceAgg = uint64_t(1) << 62;
@@ -1102,7 +1140,17 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
       before = __shfl(wave_inclusive_sum64(before), kWave - 1, kWave);
       if (tid == 0) s_base = A.out_off[s] + before;
     }
-    {
+    if (c <= kPieceBnd) {
+      // The piece's runs from the bounds searched beforehand: its own start
+      // and the next piece's (the column's end for the last piece).
+      if (tid < c) {
+        const uint64_t e = first + tid, lo = A.a_lo[e];
+        s_src[tid] = lo + Q.bnd[t * kPieceBnd + tid];
+        s_end[tid] = last_piece ? lo + A.a_len[e] : lo + Q.bnd[(t + 1) * kPieceBnd + tid];
+        s_key0[tid] = A.a_base[e] + v0;
+        s_b[tid] = A.bv[e];
+      }
+    } else {
       // The piece's runs: 2G lanes per B entry, G per side (0: its first A
       // entry in the piece, 1: the first past it -- the column's end for the
       // last piece), G the largest power of two up to 32 that fits all
@@ -1756,7 +1804,7 @@ struct lnt_batch_s {
   lnt::DeviceBuffer ws;    // per-entry tables
   lnt::DeviceBuffer scan_tmp;
   lnt::DeviceBuffer ctrl;  // counters, c_off[count+1], nprod[count]
-  // Row-piece slots (24 bytes each): the pool of eager calls, and the
+  // Row-piece slots (kPieceSlotBytes each): the pool of eager calls, and the
   // recorded graph's own (grown by the sizing run of a capture only).
   lnt::DeviceBuffer pieces, g_pieces;
   uint64_t piece_cap = 0, g_piece_cap = 0;
@@ -1850,6 +1898,7 @@ void carve(lnt_batch b, BatchPtrs *P) {
   P->pieces.piece_u = q.take<uint32_t>(b->piece_cap);
   P->pieces.piece_ent = q.take<uint64_t>(b->piece_cap);
   P->pieces.status = q.take<uint64_t>(b->piece_cap);
+  P->pieces.bnd = q.take<uint32_t>(b->piece_cap * kPieceBnd);
   P->pieces.cap = b->tuning.piece_slots < 0 ? b->piece_cap
                                              : std::min<uint64_t>(b->piece_cap, (uint64_t)b->tuning.piece_slots);
   Carve k{(char *)b->ctrl.ptr};
@@ -1912,6 +1961,7 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   LNT_HIP(hipGetLastError());
   LNT_TRY(exclusive_scan<uint32_t>(ctx, P.pieces.pq, P.pieces.pbase, nb, P.ctr + kCtrPieces, b->scan_tmp, 0));
   batch_piece_assign_kernel<<<grid_for(nb * kWave), kBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
+  batch_piece_bounds_kernel<<<grid_for(P.pieces.cap * kPieceBnd), kBlock, 0, st>>>(A, P.pieces, P.ctr);
   batch_small_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
   batch_dense_kernel<false, 256, 8, kDenseH><<<(uint32_t)b->cus * 8, 256, 0, st>>>(A, P.dense_desc, 0, P.ctr);
   if (!direct) batch_piece_kernel<false, false><<<piece_grid, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
@@ -2017,7 +2067,7 @@ int check_key_error(lnt_batch b) {
 int ensure_pieces(lnt_batch b, uint64_t need) {
   if (b->piece_cap == 0) b->piece_cap = std::max<uint64_t>(65536, b->nb / 16 + 1);
   if (need > b->piece_cap) b->piece_cap = need + need / 8 + 1024;
-  return ensure(b->pieces, (size_t)b->piece_cap * 24 + 1024);
+  return ensure(b->pieces, (size_t)b->piece_cap * kPieceSlotBytes + 2048);
 }
 
 int prepare(lnt_batch b) {
