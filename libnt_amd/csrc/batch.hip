@@ -1118,13 +1118,14 @@ __global__ __launch_bounds__(kPieceBlock, 6) void batch_piece_kernel(ColumnArgs 
     }
     if (t >= npieces) break;
     // Loads that do not wait on one another: the piece's segment position
-    // and entries, then the segment's plan and the entries' columns.
+    // and entries (both before the test of the first), then the segment's
+    // plan and the entries' columns.
     const uint32_t r = Q.piece_seg[t];
+    const uint64_t ent = Q.piece_ent[t];
     if (r == kNoPiece) {  // a slot past the pool's last fitting segment: no rows
       if (DIRECT && tid == 0) status_store(&Q.status[t], t == 0 ? kPieceIncl : kPieceAgg);
       continue;
     }
-    const uint64_t ent = Q.piece_ent[t];
     const uint64_t first = ent >> 8;
     const uint32_t c = (uint32_t)(ent & 0xFF);
     const uint64_t s = list3[r];
