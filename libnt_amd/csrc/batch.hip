@@ -506,6 +506,17 @@ struct ColumnArgs {
 constexpr uint32_t kHashSlots = 2 * kCap;
 constexpr unsigned long long kEmptyKey = ~0ull;
 
+// The same set of 32-bit keys (every key below 0xFFFFFFFF), in half the LDS.
+__device__ inline bool hash_insert(unsigned int *table, uint32_t key) {
+  uint32_t slot = (uint32_t)(((uint64_t)key * 0x9E3779B97F4A7C15ull) >> 52) & (kHashSlots - 1);
+  for (;;) {
+    const unsigned int prev = atomicCAS(&table[slot], 0xFFFFFFFFu, key);
+    if (prev == 0xFFFFFFFFu) return true;
+    if (prev == key) return false;
+    slot = (slot + 1) & (kHashSlots - 1);
+  }
+}
+
 __device__ inline bool hash_insert(unsigned long long *table, uint64_t key) {
   uint32_t slot = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 52) & (kHashSlots - 1);
   for (;;) {
@@ -581,7 +592,8 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
   // the numeric pass the per-segment rank bases share LDS with the sort's
   // digit counters (they are written after the sort), and the item's product
   // offsets are 16-bit (an item holds fewer than 2 * kCap products).
-  __shared__ unsigned long long s_hash[NUMERIC ? 1 : kHashSlots];
+  using HK = typename std::conditional<sizeof(K) == 4, unsigned int, unsigned long long>::type;
+  __shared__ HK s_hash[NUMERIC ? 1 : kHashSlots];
   __shared__ K s_key[NUMERIC ? kCap : 1];
   __shared__ uint16_t s_pos[NUMERIC ? kCap : 1];
   __shared__ K s_key2[NUMERIC ? kCap : 1];
@@ -628,7 +640,7 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
       s_spo[q] = (uint16_t)((rb + q < n_small ? sp_off[rb + q] : total) - pbase);
     }
     if (!NUMERIC)
-      for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_hash[i] = kEmptyKey;
+      for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_hash[i] = (HK)kEmptyKey;
     __syncthreads();
     const uint32_t np = s_spo[nr];
     const uint32_t p0 = tid * kPer;
@@ -665,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void batch_small_kernel(ColumnArgs A, const
           s_key[p] = (K)key;
           s_pos[p] = (uint16_t)p;
           s_val[p] = c.b * A.av[ai];
-        } else if (hash_insert(s_hash, key)) {
+        } else if (hash_insert(s_hash, (K)key)) {
           atomicAdd(&s_cnt[q], 1u);
         }
         ++g;
@@ -751,7 +763,8 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
                                                              uint64_t *__restrict__ ctr, uint32_t count_slot) {
   // K: key word of the window sort (window-relative rows); 32 bits when
   // every instance's rows fit, which halves the sort's LDS.
-  __shared__ unsigned long long s_hash[NUMERIC ? 1 : kHashSlots];
+  using HK = typename std::conditional<sizeof(K) == 4, unsigned int, unsigned long long>::type;
+  __shared__ HK s_hash[NUMERIC ? 1 : kHashSlots];
   __shared__ K s_key[NUMERIC ? kCap : 1];
   __shared__ uint16_t s_pos[NUMERIC ? kCap : 1];
   __shared__ K s_key2[NUMERIC ? kCap : 1];
@@ -845,7 +858,7 @@ __global__ __launch_bounds__(kBlock) void batch_large_kernel(ColumnArgs A, const
         running += tot;
       }
       if (!NUMERIC)
-        for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_hash[i] = kEmptyKey;
+        for (uint32_t i = tid; i < kHashSlots; i += kBlock) s_hash[i] = (HK)kEmptyKey;
       __syncthreads();
       const uint32_t n = (uint32_t)cnt;
       uint32_t fresh = 0;
@@ -1963,7 +1976,17 @@ int enqueue_front(lnt_batch b, const BatchPtrs &P, const int64_t *a_keys, const 
   LNT_TRY(exclusive_scan<uint32_t>(ctx, P.pieces.pq, P.pieces.pbase, nb, P.ctr + kCtrPieces, b->scan_tmp, 0));
   batch_piece_assign_kernel<<<grid_for(nb * kWave), kBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_piece_bounds_kernel<<<grid_for(P.pieces.cap * kPieceBnd), kBlock, 0, st>>>(A, P.pieces, P.ctr);
-  batch_small_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
+  {
+    // 32-bit hash keys (half the LDS: twice the workgroups per CU) when the
+    // item's segment (at most kMaxSegsPerItem - 1 = 1024, so the key stays
+    // below 0xFFFFFFFF) and the rows fit in 32 bits.
+    const uint32_t row_bits = b->max_h > 1 ? 64u - (uint32_t)__builtin_clzll(b->max_h - 1) : 0u;
+    if (row_bits + 11 <= 32)
+      batch_small_kernel<false, uint32_t><<<std::max<uint32_t>(grid, (uint32_t)b->cus * 6), kBlock, 0, st>>>(
+          A, P.small_list, P.sp_off, G, row_bits, P.ctr);
+    else
+      batch_small_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.small_list, P.sp_off, G, kRowBits, P.ctr);
+  }
   batch_dense_kernel<false, 256, 8, kDenseH><<<(uint32_t)b->cus * 8, 256, 0, st>>>(A, P.dense_desc, 0, P.ctr);
   if (!direct) batch_piece_kernel<false, false><<<piece_grid, kPieceBlock, 0, st>>>(A, P.list3, P.pieces, P.ctr);
   batch_large_kernel<false, uint64_t><<<grid, kBlock, 0, st>>>(A, P.pieces.list3w, P.cur, P.wcnt, P.woff, window,
