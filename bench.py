@@ -154,14 +154,14 @@ def cpu_baseline(ak0, av0, bk0, bv0, scale=1):
     }
 
 
-def _profiled_traffic():
-    """HBM bytes per onesweep launch measured by rocprofv3 (FETCH_SIZE +
-    WRITE_SIZE, scaled by the calibration kernels), from the committed profile
-    summary; None if absent."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "onesweep_sort_merge_at_scale_pmc.md")
+def _profiled_traffic(name="onesweep_sort_merge_at_scale_pmc.md", key="traffic_bytes_per_launch"):
+    """HBM bytes measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by the
+    calibration kernels), from a committed profile summary (per onesweep
+    launch by default); None if absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         return None
@@ -303,6 +303,9 @@ def bench_contraction(ctx, torch, cpu_repeats=3, mask=None):
     res["roofline"] = _roofline(16.0 * (A.nnz + B.nnz + nnz_c), ms[best] * 1e-3,
                                 "whole contraction (%s), compulsory bytes 16*(nnzA+nnzB+nnzC)"
                                 % dispatch.strategy_name(best))
+    # HBM bytes of one contraction, measured by rocprofv3 on this workload.
+    res["roofline"]["traffic"] = _profiled_traffic("traffic_contraction_skewed_4096x4096x512.md",
+                                                   "traffic_bytes_per_contraction")
     if cpu_repeats < 1:
         _log("contraction: %s" % res)
         return res
@@ -400,8 +403,11 @@ def bench_batched(ctx, torch, cpu_stride=100, per_instance_runs=True, tune=True)
         "instances_per_s": round(len(insts) / t_batched, 1),
         "accumulator_policies": policies,
         "best_policy": best,
-        "roofline": _roofline(compulsory, t_best, "whole batched pipeline (graph replay, %s accumulators), "
-                                                  "compulsory bytes 16*sum(nnzA+nnzB+nnzC)" % best),
+        "roofline": dict(_roofline(compulsory, t_best, "whole batched pipeline (graph replay, %s accumulators), "
+                                                       "compulsory bytes 16*sum(nnzA+nnzB+nnzC)" % best),
+                         # HBM bytes of one dense replay, measured by rocprofv3 on this mix.
+                         traffic=_profiled_traffic("traffic_batched_10000_contractions.md",
+                                                   "traffic_bytes_per_replay") if best == "dense" else None),
         "autotuned": tuned["config"],
         "autotune_subsample_ms": ({"default": round(tuned["default_ms"], 2), "tuned": round(tuned["ms"], 2)}
                                   if tune else None),
